@@ -1,0 +1,209 @@
+/*
+ * mpflow_amd — MI355X-native EIT forward/inverse hot path behind a C ABI.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an int
+ * status code and never throws.  Device pointers are caller-owned unless stated.
+ * Work is stream-ordered on the context's HIP stream; a call returns when its work is
+ * enqueued unless it says it synchronises.
+ *
+ * Reference seams replaced (paths relative to the mpFlow source tree):
+ *   numeric::SparseMatrix::multiply            src/numeric/sparse_matrix.cpp:262-288
+ *   FEM::Equation::update / updateMatrix        src/fem/equation.cpp:261-274, 317-345
+ *   numeric::ConjugateGradient::solve           src/numeric/conjugate_gradient.cpp:61-161
+ *   numeric::BiCGSTAB::solve                    src/numeric/bicgstab.cpp:83-193
+ *   preconditioner::diagonal                    src/numeric/preconditioner.cpp:59-76
+ *   FEM::Equation::calcJacobian                 src/fem/equation.cpp:282-310
+ *   models::EIT::EIT / solve / applyMeasurementPattern   src/models/eit.cpp:29-95, 139-245
+ *   solver::Inverse::calcJacobianSquare / calcExcitation / solve
+ *                                               src/solver/inverse.cpp:136-238
+ *   solver::Solver::preSolve / solveAbsolute    src/solver/solver.cpp:142-225
+ *   FEM::Ports::circularBoundary                src/fem/ports.cpp:32-77
+ */
+#ifndef MPFLOW_AMD_H
+#define MPFLOW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MPF_OK        0
+#define MPF_EINVAL    1   /* bad argument (null pointer, shape mismatch, ...)      */
+#define MPF_ENOMEM    2   /* device or host allocation failed                      */
+#define MPF_EHIP      3   /* HIP runtime error                                     */
+#define MPF_ENOCONV   4   /* an iterative solve hit its iteration cap              */
+#define MPF_EFULL     5   /* ELL row needs more slots than allowed ("sparse format full",
+                             src/numeric/sparse_matrix.cpp:417-419)                   */
+
+/* stop rules of the Krylov solvers */
+#define MPF_STOP_REFERENCE 0  /* sqrt(|r.z|/(b.b)) < FLT_EPSILON on column 0 only
+                                 (src/numeric/conjugate_gradient.cpp:145-151)        */
+#define MPF_STOP_TOLERANCE 1  /* every column ||r|| <= tol * ||b||                     */
+
+/* precision of the forward solve */
+#define MPF_PREC_FP32   0     /* FP32 PCG, FP64 scalars (reference arithmetic)          */
+#define MPF_PREC_MIXED  1     /* FP32 PCG + FP64 iterative refinement of the residual   */
+
+/* Krylov method */
+#define MPF_KRYLOV_CG       0
+#define MPF_KRYLOV_BICGSTAB 1
+
+typedef struct mpf_ctx mpf_ctx;
+typedef struct mpf_model mpf_model;
+
+/* Library version string (replaces version::getVersionString, src/version.cpp:23-25). */
+const char* mpf_version(void);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* mpf_last_error(void);
+
+/* Context: device, HIP stream (NULL = create a default-priority stream), scratch
+ * arena, autotune cache.  One context per host thread or stream. */
+int mpf_ctx_create(int device, void* hip_stream, mpf_ctx** out);
+int mpf_ctx_destroy(mpf_ctx* ctx);
+int mpf_ctx_synchronize(mpf_ctx* ctx);
+/* hipStream_t of the context, as void*. */
+void* mpf_ctx_stream(mpf_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Host helpers (no device work)
+ * ------------------------------------------------------------------------- */
+
+/* Ports::circularBoundary (src/fem/ports.cpp:32-77): for each of `count` electrodes of
+ * angular `width` starting at offset + 2*pi*l/count, the indices (rows of `boundary`)
+ * of the counter-clockwise boundary edges inside the arc.  `out` receives
+ * count x max_edges entries (row-major, -1 = unused); `*max_edges_out` the width used.
+ * Pass out=NULL to query the width first. */
+int mpf_ports_circular(int32_t n_nodes, const double* nodes, int32_t n_elements,
+                       int32_t nodes_per_element, const int32_t* elements,
+                       int32_t n_boundary, int32_t nodes_per_edge, const int32_t* boundary,
+                       int32_t count, double width, double offset, int32_t clockwise,
+                       int32_t* out, int32_t out_capacity, int32_t* max_edges_out);
+
+/* ---------------------------------------------------------------------------
+ * EIT model: mesh + ports + sources + 2.5D set-up (models::EIT::EIT,
+ * src/models/eit.cpp:29-95; FEM::Equation::Equation, src/fem/equation.cpp:29-68).
+ * ------------------------------------------------------------------------- */
+typedef struct mpf_model_desc {
+    /* mesh (host arrays, row-major) */
+    int32_t n_nodes;              /* N                                             */
+    int32_t n_elements;           /* E                                             */
+    int32_t nodes_per_element;    /* 3 = P1, 6 = P2 [v0 v1 v2 m01 m12 m20]          */
+    const double* nodes;          /* N x 2                                          */
+    const int32_t* elements;      /* E x nodes_per_element                          */
+    int32_t n_boundary;           /* boundary edge count                            */
+    int32_t nodes_per_edge;       /* 2 = P1 [a b], 3 = P2 [a mid b]; counter-clockwise */
+    const int32_t* boundary;      /* n_boundary x nodes_per_edge                    */
+    /* ports: rows of `boundary` per port, -1 = unused (Ports::edges, ports.h:26-46) */
+    int32_t n_ports;
+    int32_t port_max_edges;
+    const int32_t* port_edges;    /* n_ports x port_max_edges                       */
+    /* sources (Sources::Sources, src/fem/sources.cpp:27-75), Open/current type     */
+    int32_t n_drive;              /* D                                              */
+    int32_t n_meas;               /* M                                              */
+    const double* drive_pattern;  /* n_ports x D, row-major                         */
+    const double* meas_pattern;   /* n_ports x M, row-major                         */
+    const double* drive_current;  /* D values (Sources::values)                     */
+    /* material and 2.5D (EIT::fromConfig, src/models/eit.cpp:119-126)              */
+    double reference_value;
+    int32_t components;           /* harmonics C >= 1 (at most 8)                   */
+    double height;
+    double port_height;
+    int32_t logarithmic;          /* Equation<..., logarithmic>                     */
+    int32_t quirks;               /* 1: reproduce the reference's P2 boundary integral */
+    int32_t jacobian_exact;       /* 0: reference Jacobian (K_e only, factor height);
+                                     1: exact 2.5D derivative (K_e + alpha_c M_e, 1/beta_c) */
+    int32_t ell_max_width;        /* 0 -> 32 (reference cap); at most 64            */
+    int32_t frames;               /* frames solved together on this mesh (>= 1)     */
+} mpf_model_desc;
+
+typedef struct mpf_solve_opts {
+    int32_t stop_rule;            /* MPF_STOP_*                                     */
+    double tol;                   /* for MPF_STOP_TOLERANCE                         */
+    int32_t max_iter;             /* 0 -> 10 * N (conjugate_gradient.cpp:78)         */
+    int32_t precision;            /* MPF_PREC_*                                     */
+    int32_t refine_steps;         /* FP64 refinement passes (MPF_PREC_MIXED)        */
+    double refine_tol;            /* inner tolerance of each refinement solve       */
+    int32_t krylov;               /* MPF_KRYLOV_*                                   */
+    int32_t check_every;          /* iterations between host convergence polls (0 -> 16) */
+} mpf_solve_opts;
+
+typedef struct mpf_solve_stats {
+    int64_t steps;                /* sum over frames and harmonics of Krylov steps    */
+    int64_t max_steps;            /* largest per (frame, harmonic) step count        */
+    int32_t not_converged;        /* (frame, harmonic) pairs that hit the cap          */
+    double refine_residual;       /* max relative FP64 residual after refinement      */
+    double seconds;               /* wall time of the call (synchronised)             */
+} mpf_solve_stats;
+
+void mpf_solve_opts_default(mpf_solve_opts* o);
+
+int mpf_model_create(mpf_ctx* ctx, const mpf_model_desc* desc, mpf_model** out);
+int mpf_model_destroy(mpf_model* m);
+
+/* dims[0]=N, [1]=E, [2]=padded rows np, [3]=ELL width, [4]=Jacobian rows
+ * m = ceil16(M)*ceil16(D), [5]=ceil16(M), [6]=ceil16(D), [7]=R=D+M, [8]=RHS lanes,
+ * [9]=frames, [10]=C, [11]=nnz */
+int mpf_model_dims(const mpf_model* m, int64_t dims[12]);
+
+/* models::EIT::solve (src/models/eit.cpp:139-213) for every frame.
+ *   d_gamma : frames x E   (device, float)   material parameter gamma
+ *   d_U     : frames x m   (device, float)   voltages, column-major M x D with
+ *             leading dimension ceil16(M), padded rows zero (EIT::result layout)
+ *   d_U64   : frames x m   (device, double)  same in FP64 (nullable)
+ *   d_J     : frames x m x E (device, float) Jacobian, column-major (nullable)
+ * Synchronises.  Returns MPF_ENOCONV if any (frame, harmonic) hit the cap. */
+int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64, float* d_J,
+                  const mpf_solve_opts* opts, mpf_solve_stats* stats);
+
+/* One Solver::solveAbsolute per frame (src/solver/solver.cpp:195-225):
+ * forward solve + Jacobian at gamma, r = U_meas - U(gamma), then
+ * gamma += (J^T J + (ref*lambda)^2 I)^-1 J^T r, solved in the m x m data space
+ * (J J^T on MFMA, BF16 in / FP32 accumulate, FP64 Cholesky, FP64-residual
+ * refinement `gn_refine` times).
+ *   d_gamma  : frames x E (device float, in/out)
+ *   d_Umeas  : frames x m (device double)
+ *   d_Ucalc  : frames x m (device double, out, nullable) voltages at the input gamma */
+int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lambda,
+                int32_t gn_refine, const mpf_solve_opts* opts, double* d_Ucalc,
+                mpf_solve_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * Low-level kernels on caller-owned device buffers
+ * ------------------------------------------------------------------------- */
+
+/* Multi-harmonic ELL SpMV, the S/R form of SparseMatrix::multiply
+ * (src/numeric/sparse_matrix_kernel.cu:157-196) fused with Equation::update's
+ * A_c = S + alpha_c R (src/fem/equation_kernel.cu:137-157):
+ *   Y[c][row][j] = sum_slot (S[slot][row] + alpha[c] R[slot][row]) * X[c][ids[slot][row]][j]
+ * ELL is column-major (slot-major): entry (row, slot) at slot*np + row; padding slots
+ * hold id = row and value 0.  X/Y: C x np x ld, j < R <= ld, ld in {16,32,64}. */
+int mpf_ell_spmv(mpf_ctx* ctx, int32_t np, int32_t width, const int32_t* d_ids,
+                 const float* d_S, const float* d_R, int32_t C, const float* alpha,
+                 const float* d_X, float* d_Y, int32_t R, int32_t ld);
+
+/* Symmetric Gram matrix on MFMA (BF16 inputs, FP32 accumulation):
+ *   trans == 0: G (rows x rows) = A A^T, A rows x k column-major (lda >= rows)
+ *   trans == 1: G (k x k)       = A^T A (J^T J of Inverse::calcJacobianSquare,
+ *                                  src/solver/inverse.cpp:136-156)
+ * G is written full (both triangles), column-major, ldg = its order. */
+int mpf_gram_bf16(mpf_ctx* ctx, int32_t trans, int32_t rows, int32_t k, const float* d_A,
+                  int32_t lda, float* d_G);
+
+/* y = A^T x (FP64 accumulation): Inverse::calcExcitation's J^T (U_meas - U_calc)
+ * (src/solver/inverse.cpp:163-205).  A rows x cols column-major (lda >= rows). */
+int mpf_gemv_t(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_A, int32_t lda,
+               const double* d_x, double* d_y);
+
+/* Solve (G + shift I) y = b for SPD G (n x n, column-major FP32) by FP64 Cholesky on
+ * the device; b, y FP64. */
+int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const double* d_b,
+                  double* d_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPFLOW_AMD_H */

@@ -1,0 +1,258 @@
+// C ABI of mpflow_amd: every entry point catches, records the message and returns a
+// status code (the reference throws std::invalid_argument / runtime_error instead).
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "model.h"
+
+using mpf::Error;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return MPF_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MPF_EINVAL;
+    }
+}
+
+void set_device(mpf_ctx* ctx) { MPF_HIP(hipSetDevice(ctx->device)); }
+}  // namespace
+
+struct mpf_model {
+    mpf::Model* impl;
+};
+
+extern "C" {
+
+const char* mpf_version(void) { return "mpflow_amd 0.1.0 (gfx950)"; }
+
+const char* mpf_last_error(void) { return g_last_error.c_str(); }
+
+int mpf_ctx_create(int device, void* hip_stream, mpf_ctx** out) {
+    return guarded([&] {
+        MPF_REQUIRE(out != nullptr, "mpf_ctx_create: out == nullptr");
+        int n = 0;
+        MPF_HIP(hipGetDeviceCount(&n));
+        MPF_REQUIRE(device >= 0 && device < n, "mpf_ctx_create: no such device");
+        auto* c = new mpf_ctx;
+        c->device = device;
+        MPF_HIP(hipSetDevice(device));
+        if (hip_stream) {
+            c->stream = (hipStream_t)hip_stream;
+            c->own_stream = false;
+        } else {
+            MPF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        *out = c;
+        return MPF_OK;
+    });
+}
+
+int mpf_ctx_destroy(mpf_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return MPF_OK;
+        if (ctx->own_stream && ctx->stream) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipStreamDestroy(ctx->stream);
+        }
+        delete ctx;
+        return MPF_OK;
+    });
+}
+
+int mpf_ctx_synchronize(mpf_ctx* ctx) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx, "mpf_ctx_synchronize: ctx == nullptr");
+        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        return MPF_OK;
+    });
+}
+
+void* mpf_ctx_stream(mpf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+void mpf_solve_opts_default(mpf_solve_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->stop_rule = MPF_STOP_REFERENCE;
+    o->tol = 1e-6;
+    o->max_iter = 0;
+    o->precision = MPF_PREC_FP32;
+    o->refine_steps = 2;
+    o->refine_tol = 1e-3;
+    o->krylov = MPF_KRYLOV_CG;
+    o->check_every = 16;
+}
+
+int mpf_ports_circular(int32_t n_nodes, const double* nodes, int32_t n_elements,
+                       int32_t nodes_per_element, const int32_t* elements, int32_t n_boundary,
+                       int32_t nodes_per_edge, const int32_t* boundary, int32_t count, double width,
+                       double offset, int32_t clockwise, int32_t* out, int32_t out_capacity,
+                       int32_t* max_edges_out) {
+    return guarded([&] {
+        MPF_REQUIRE(nodes && elements && boundary && max_edges_out, "mpf_ports_circular: null argument");
+        auto ports = mpf::circular_ports(n_nodes, nodes, n_elements, nodes_per_element, elements, n_boundary,
+                                         nodes_per_edge, boundary, count, width, offset, clockwise != 0);
+        int w = 0;
+        for (auto& p : ports) w = std::max<int>(w, (int)p.size());
+        *max_edges_out = w;
+        if (out) {
+            MPF_REQUIRE(out_capacity >= count * w, "mpf_ports_circular: out too small");
+            for (int p = 0; p < count; ++p)
+                for (int q = 0; q < w; ++q) out[p * w + q] = q < (int)ports[p].size() ? ports[p][q] : -1;
+        }
+        return MPF_OK;
+    });
+}
+
+int mpf_model_create(mpf_ctx* ctx, const mpf_model_desc* desc, mpf_model** out) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && desc && out, "mpf_model_create: null argument");
+        set_device(ctx);
+        auto* m = new mpf_model;
+        try {
+            m->impl = new mpf::Model(ctx, *desc);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return MPF_OK;
+    });
+}
+
+int mpf_model_destroy(mpf_model* m) {
+    return guarded([&] {
+        if (!m) return MPF_OK;
+        if (m->impl) {
+            (void)hipStreamSynchronize(m->impl->ctx->stream);
+            delete m->impl;
+        }
+        delete m;
+        return MPF_OK;
+    });
+}
+
+int mpf_model_dims(const mpf_model* m, int64_t dims[12]) {
+    return guarded([&] {
+        MPF_REQUIRE(m && dims, "mpf_model_dims: null argument");
+        const auto& S = m->impl->S;
+        dims[0] = S.N; dims[1] = S.E; dims[2] = S.np; dims[3] = S.width; dims[4] = S.mrows;
+        dims[5] = S.Mp; dims[6] = S.Dp; dims[7] = S.R; dims[8] = S.Rp; dims[9] = m->impl->frames;
+        dims[10] = S.C; dims[11] = S.nnz;
+        return MPF_OK;
+    });
+}
+
+int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64, float* d_J,
+                  const mpf_solve_opts* opts, mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_gamma, "mpf_eit_solve: null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        set_device(m->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        auto& M = *m->impl;
+        MPF_HIP(hipMemcpyAsync(M.gamma.p, d_gamma, sizeof(float) * (size_t)M.S.E * M.frames,
+                               hipMemcpyDeviceToDevice, M.ctx->stream));
+        int rc = M.forward(o, d_U, d_U64, d_J, stats);
+        if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc == MPF_ENOCONV) g_last_error = "mpf_eit_solve: a Krylov solve hit its iteration cap";
+        return rc;
+    });
+}
+
+int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lambda, int32_t gn_refine,
+                const mpf_solve_opts* opts, double* d_Ucalc, mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_gamma && d_Umeas, "mpf_gn_step: null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        set_device(m->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        int rc = m->impl->gn_step(d_gamma, d_Umeas, lambda, gn_refine, o, d_Ucalc, stats);
+        if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    });
+}
+
+int mpf_ell_spmv(mpf_ctx* ctx, int32_t np, int32_t width, const int32_t* d_ids, const float* d_S,
+                 const float* d_R, int32_t C, const float* alpha, const float* d_X, float* d_Y, int32_t R,
+                 int32_t ld) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_ids && d_S && d_R && alpha && d_X && d_Y, "mpf_ell_spmv: null argument");
+        MPF_REQUIRE(np > 0 && np % mpf::kTileRows == 0, "mpf_ell_spmv: np must be a positive multiple of 64");
+        MPF_REQUIRE(ld == 16 || ld == 32 || ld == 64, "mpf_ell_spmv: ld must be 16, 32 or 64");
+        MPF_REQUIRE(R >= 1 && R <= ld, "mpf_ell_spmv: R out of range");
+        MPF_REQUIRE(C >= 1 && C <= mpf::kMaxSlices && width >= 1 && width <= 64, "mpf_ell_spmv: bad C or width");
+        set_device(ctx);
+        // alpha is a host array; stage it on the device
+        mpf::DBuf<float> da(C);
+        da.upload(alpha, C, ctx->stream);
+        mpf::launch_spmv(np, width, d_ids, d_S, d_R, C, da.p, d_X, d_Y, ld, ctx->stream);
+        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_gram_bf16(mpf_ctx* ctx, int32_t trans, int32_t rows, int32_t k, const float* d_A, int32_t lda,
+                  float* d_G) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_A && d_G, "mpf_gram_bf16: null argument");
+        MPF_REQUIRE(rows > 0 && k > 0 && lda >= rows, "mpf_gram_bf16: bad shape");
+        set_device(ctx);
+        const int n = trans ? k : rows, nk = trans ? rows : k;
+        mpf::DBuf<char> work(mpf::gram_workspace_bytes(n, nk));
+        mpf::launch_gram(d_A, trans, n, nk, lda, d_G, work.p, ctx->stream);
+        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_gemv_t(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_A, int32_t lda, const double* d_x,
+               double* d_y) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_A && d_x && d_y, "mpf_gemv_t: null argument");
+        MPF_REQUIRE(rows > 0 && cols > 0 && lda >= rows, "mpf_gemv_t: bad shape");
+        set_device(ctx);
+        mpf::launch_gemv_t(rows, cols, d_A, lda, 0, d_x, 0, d_y, 0, 1, ctx->stream);
+        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const double* d_b, double* d_y) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_G && d_b && d_y, "mpf_spd_solve: null argument");
+        MPF_REQUIRE(n > 0, "mpf_spd_solve: n <= 0");
+        set_device(ctx);
+        mpf::DBuf<double> A((size_t)n * n);
+        mpf::DBuf<int2> pairs(mpf::cholesky_pairs(n));
+        mpf::DBuf<int> info(1);
+        MPF_HIP(hipMemsetAsync(info.p, 0, sizeof(int), ctx->stream));
+        mpf::launch_cholesky(n, d_G, 0, shift, A.p, 0, 1, info.p, pairs.p, ctx->stream);
+        mpf::launch_potrs(n, A.p, 0, d_b, 0, d_y, 0, 1, ctx->stream);
+        int h = 0;
+        MPF_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        MPF_REQUIRE(h == 0, "mpf_spd_solve: matrix is not positive definite");
+        return MPF_OK;
+    });
+}
+
+}  // extern "C"

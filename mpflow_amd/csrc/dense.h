@@ -1,0 +1,53 @@
+// Launch interface of the dense kernels (dense.hip).
+#pragma once
+
+#include "jobs.h"
+
+namespace mpf {
+
+// measurement map W (CSR over measurement patterns), shared by all jobs of a mesh
+struct MeasDev {
+    const int32_t* w_off; const int32_t* w_node; const double* w_val;
+    int D, M, Dp, Mp;
+};
+
+// element data for the Jacobian, shared by all jobs of a mesh
+struct JacDev {
+    const int32_t* elements;     // [E][k]
+    const float* ek; const float* em;   // [E][k*k]
+    const float* gamma;          // [jobs][E]
+    int E, k, D, M, Dp, Mp, exact, logarithmic;
+    double ref;
+    float alpha[kMaxSlices];
+    float weight[kMaxSlices];
+};
+
+void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use64, float* U32,
+                     double* U64, hipStream_t st);
+void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use64, float* J,
+                     hipStream_t st);
+
+int gram_kslab(int n, int nk);
+size_t gram_workspace_bytes(int n, int nk);
+void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
+                 hipStream_t st);
+
+void launch_gemv_t(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                   size_t stridex, double* y, size_t stridey, int batch, hipStream_t st);
+size_t gemv_n_workspace(int rows, int cols, int batch);
+void launch_gemv_n(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                   size_t stridex, double shift, const double* z, size_t stridez, const double* rhs,
+                   size_t striderhs, double* out, size_t strideout, int batch, double* work,
+                   hipStream_t st);
+
+size_t cholesky_pairs(int n);
+void launch_cholesky(int n, const float* G, size_t strideG, double shift, double* A, size_t strideA,
+                     int batch, int* d_info, int2* d_pairs, hipStream_t st);
+void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_t strideb, double* y,
+                  size_t stridey, int batch, hipStream_t st);
+
+void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t st);
+void launch_add_to(int n, double* y, const double* dy, hipStream_t st);
+void launch_gamma_update(int n, float* g, const double* d, hipStream_t st);
+
+}  // namespace mpf

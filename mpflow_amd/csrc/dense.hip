@@ -1,0 +1,550 @@
+// Dense kernels of the EIT path on CDNA4 (gfx950):
+//   electrode voltages, Jacobian assembly from drive/measurement fields, the Gram
+//   matrix of the Jacobian on MFMA (BF16 in, FP32 accumulate), FP64 matrix-vector
+//   products and a blocked FP64 Cholesky for the Tikhonov normal equations.
+#include <cmath>
+
+#include "dense.h"
+
+namespace mpf {
+
+// --------------------------------------------------------------------------------
+// voltages U[d*Mp + m] = sum_c sum_n W[m,n] Phi_c[n, d]
+// (EIT::applyMeasurementPattern, src/models/eit.cpp:220-245; W = Mpat^T B^T)
+// --------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_voltages(const JobDev* __restrict__ jobs, MeasDev md,
+                                                  int use64, float* __restrict__ U32,
+                                                  double* __restrict__ U64) {
+    const int j = blockIdx.x;
+    const JobDev& J = jobs[j];
+    const int mrows = md.Mp * md.Dp;
+    for (int idx = threadIdx.x; idx < mrows; idx += blockDim.x) {
+        const int d = idx / md.Mp, m = idx % md.Mp;
+        double v = 0.0;
+        if (d < md.D && m < md.M) {
+            for (int s = 0; s < J.C; ++s)
+                for (int q = md.w_off[m]; q < md.w_off[m + 1]; ++q) {
+                    const size_t i = vec_index(J, s, md.w_node[q], d);
+                    v += md.w_val[q] * (use64 ? J.x64[i] : (double)J.x[i]);
+                }
+        }
+        if (U32) U32[(size_t)j * mrows + idx] = (float)v;
+        if (U64) U64[(size_t)j * mrows + idx] = v;
+    }
+}
+
+// --------------------------------------------------------------------------------
+// Jacobian (calcJacobianKernel, src/fem/equation_kernel.cu:197-245 with the scaling of
+// src/models/eit.cpp:180-182, 202-205):
+//   J[d*Mp + m, e] = -sigma'_e sum_c w_c Phi_c[el, d]^T Kc_e Phi_c[el, D+m]
+//   reference: Kc = K_e, w_c = height;   exact: Kc = K_e + alpha_c M_e, w_c = 1/beta_c
+// One block per (element, job); the column J[:, e] is written once, contiguously.
+// --------------------------------------------------------------------------------
+constexpr int kJacMaxRows = 4096;   // Dp * Mp <= 64 * 64
+
+__global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ jobs, JacDev jd,
+                                                  int use64, float* __restrict__ Jout) {
+    const int e = blockIdx.x;
+    const int j = blockIdx.y;
+    const JobDev& Jb = jobs[j];
+    const int k = jd.k, D = jd.D, M = jd.M, Mp = jd.Mp, Dp = jd.Dp, R = D + M;
+    const int mrows = Mp * Dp;
+    __shared__ float phi[6][64];
+    __shared__ float T[6][64];
+    __shared__ float Kc[36];
+    __shared__ int nodes[6];
+    if (threadIdx.x < k) nodes[threadIdx.x] = jd.elements[(size_t)e * k + threadIdx.x];
+    float acc[kJacMaxRows / 256];
+#pragma unroll
+    for (int q = 0; q < kJacMaxRows / 256; ++q) acc[q] = 0.0f;
+    const int per = (mrows + 255) / 256;
+    __syncthreads();
+    for (int s = 0; s < Jb.C; ++s) {
+        const float al = jd.exact ? jd.alpha[s] : 0.0f;
+        const float w = jd.weight[s];
+        for (int i = threadIdx.x; i < k * k; i += blockDim.x)
+            Kc[i] = jd.ek[(size_t)e * k * k + i] + al * jd.em[(size_t)e * k * k + i];
+        for (int i = threadIdx.x; i < k * R; i += blockDim.x) {
+            const int a = i / R, c = i % R;
+            const size_t vi = vec_index(Jb, s, nodes[a], c);
+            phi[a][c] = use64 ? (float)Jb.x64[vi] : Jb.x[vi];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k * M; i += blockDim.x) {
+            const int a = i / M, m = i % M;
+            float t = 0.0f;
+            for (int b = 0; b < k; ++b) t += Kc[a * k + b] * phi[b][D + m];
+            T[a][m] = w * t;
+        }
+        __syncthreads();
+        for (int q = 0; q < per; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            if (idx >= mrows) break;
+            const int d = idx / Mp, m = idx % Mp;
+            if (d < D && m < M) {
+                float v = 0.0f;
+                for (int a = 0; a < k; ++a) v += phi[a][d] * T[a][m];
+                acc[q] += v;
+            }
+        }
+        __syncthreads();
+    }
+    const float g = jd.gamma[(size_t)j * jd.E + e];
+    const double sd = jd.logarithmic ? (2.302585092994046 / 10.0) * jd.ref * exp(2.302585092994046 * (double)g / 10.0)
+                                     : jd.ref;
+    const float scale = (float)(-sd);
+    float* col = Jout + ((size_t)j * jd.E + e) * mrows;
+    for (int q = 0; q < per; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx >= mrows) break;
+        col[idx] = scale * acc[q];
+    }
+}
+
+// --------------------------------------------------------------------------------
+// Gram matrix on MFMA: G = A A^T (trans 0) or A^T A (trans 1), BF16 operands, FP32
+// accumulation with v_mfma_f32_32x32x16_bf16.  Block = 4 waves = 64x64 tile of G,
+// wave w owns the 32x32 sub-tile (w/2, w%2).  The contraction is split over
+// gridDim.z slabs whose partial tiles are summed in slab order by k_gram_reduce.
+// Only tiles with I <= J are computed; the reduction mirrors them.
+// --------------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kGT = 64;      // G tile
+constexpr int kGK = 32;      // contraction step staged in LDS
+constexpr int kGLds = kGK + 8;
+
+__device__ __forceinline__ float gram_load(const float* A, int trans, int lda, int nrow, int nk,
+                                           int r, int kk) {
+    if (r >= nrow || kk >= nk) return 0.0f;
+    return trans ? A[(size_t)r * lda + kk] : A[r + (size_t)kk * lda];
+}
+
+__global__ void __launch_bounds__(256) k_gram_mfma(const float* __restrict__ A, int trans, int nrow,
+                                                   int nk, int lda, const int2* __restrict__ tiles,
+                                                   int kslab, float* __restrict__ part) {
+    const int2 tile = tiles[blockIdx.x];
+    const int I0 = tile.x * kGT, J0 = tile.y * kGT;
+    const int k0 = blockIdx.z * kslab, k1 = min(nk, k0 + kslab);
+    __shared__ __attribute__((aligned(16))) __bf16 sa[kGT][kGLds];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[kGT][kGLds];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+    f32x16_t acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int kb = k0; kb < k1; kb += kGK) {
+        // stage 64 x 32 of each operand, converted to BF16 (round to nearest even)
+        for (int i = tid; i < kGT * kGK; i += 256) {
+            int r, kk;
+            if (trans) { r = i / kGK; kk = i % kGK; }     // k contiguous in memory
+            else       { r = i % kGT; kk = i / kGT; }     // rows contiguous in memory
+            const int kg = kb + kk;
+            const bool ok = kg < k1;
+            sa[r][kk] = (__bf16)(ok ? gram_load(A, trans, lda, nrow, nk, I0 + r, kg) : 0.0f);
+            sb[r][kk] = (__bf16)(ok ? gram_load(A, trans, lda, nrow, nk, J0 + r, kg) : 0.0f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kGK; ks += 16) {
+            const int kq = ks + (lane >> 5) * 8;
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&sa[wi + (lane & 31)][kq]);
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&sb[wj + (lane & 31)][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // C layout: lane -> column (lane & 31); register v -> row 8*(v/4) + 4*(lane/32) + v%4
+    float* out = part + ((size_t)blockIdx.z * gridDim.x + blockIdx.x) * kGT * kGT;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = wi + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int col = wj + (lane & 31);
+        out[row * kGT + col] = acc[v];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gram_reduce(const float* __restrict__ part, const int2* __restrict__ tiles,
+                                                     int ntiles, int nslab, int n, float* __restrict__ G) {
+    const int b = blockIdx.x;
+    const int2 tile = tiles[b];
+    for (int i = threadIdx.x; i < kGT * kGT; i += 256) {
+        float s = 0.0f;
+        for (int z = 0; z < nslab; ++z) s += part[((size_t)z * ntiles + b) * kGT * kGT + i];
+        const int r = tile.x * kGT + i / kGT, c = tile.y * kGT + i % kGT;
+        if (r < n && c < n) {
+            G[r + (size_t)c * n] = s;
+            G[c + (size_t)r * n] = s;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------
+// FP64-accumulating matrix-vector products with a column-major FP32 matrix
+// --------------------------------------------------------------------------------
+// y[c] = sum_r A[r + c*lda] x[r]: one wavefront per column
+__global__ void __launch_bounds__(256) k_gemv_t(int rows, int cols, const float* __restrict__ A, int lda,
+                                                size_t strideA, const double* __restrict__ x, size_t stridex,
+                                                double* __restrict__ y, size_t stridey, double ybeta) {
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.y;
+    if (col >= cols) return;
+    const float* a = A + b * strideA + (size_t)col * lda;
+    const double* xv = x + b * stridex;
+    double s = 0.0;
+    for (int r = lane; r < rows; r += 64) s += (double)a[r] * xv[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) {
+        double* yp = y + b * stridey + col;
+        *yp = (ybeta != 0.0 ? ybeta * *yp : 0.0) + s;
+    }
+}
+
+// partial[slab][r] = sum_{c in slab} A[r + c*lda] x[c]
+constexpr int kNSlab = 256;
+__global__ void __launch_bounds__(256) k_gemv_n_part(int rows, int cols, const float* __restrict__ A, int lda,
+                                                     size_t strideA, const double* __restrict__ x, size_t stridex,
+                                                     double* __restrict__ part) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int slab = blockIdx.y, b = blockIdx.z;
+    const int c0 = slab * kNSlab, c1 = min(cols, c0 + kNSlab);
+    if (r >= rows) return;
+    const float* a = A + b * strideA;
+    const double* xv = x + b * stridex;
+    double s = 0.0;
+    for (int c = c0; c < c1; ++c) s += (double)a[r + (size_t)c * lda] * xv[c];
+    part[((size_t)b * gridDim.y + slab) * rows + r] = s;
+}
+
+// y = part-sum (+ shift * z) ; out = rhs - y when rhs given
+__global__ void __launch_bounds__(256) k_gemv_n_reduce(int rows, int nslab, const double* __restrict__ part,
+                                                       double shift, const double* __restrict__ z, size_t stridez,
+                                                       const double* __restrict__ rhs, size_t striderhs,
+                                                       double* __restrict__ out, size_t strideout) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (r >= rows) return;
+    double s = 0.0;
+    for (int q = 0; q < nslab; ++q) s += part[((size_t)b * nslab + q) * rows + r];
+    if (z) s += shift * z[b * stridez + r];
+    out[b * strideout + r] = rhs ? rhs[b * striderhs + r] - s : s;
+}
+
+// --------------------------------------------------------------------------------
+// blocked FP64 Cholesky (lower), column-major n x n, batch over blockIdx.z
+// --------------------------------------------------------------------------------
+constexpr int kCB = 32;
+
+// A64 = G + shift I (and zero the strictly upper triangle is not needed)
+__global__ void __launch_bounds__(256) k_to64_shift(int n, const float* __restrict__ G, size_t strideG,
+                                                    double shift, double* __restrict__ A, size_t strideA) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= (size_t)n * n) return;
+    const int r = (int)(i % n), c = (int)(i / n);
+    A[b * strideA + i] = (double)G[b * strideG + i] + (r == c ? shift : 0.0);
+}
+
+// factor the diagonal block kb in place
+__global__ void __launch_bounds__(256) k_potrf_diag(int n, double* __restrict__ A, size_t strideA, int kb,
+                                                    int* __restrict__ info) {
+    double* a = A + blockIdx.z * strideA;
+    __shared__ double L[kCB][kCB + 1];
+    const int o = kb * kCB, nb = min(kCB, n - o);
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (threadIdx.x == 0) {
+            double d = L[j][j];
+            if (!(d > 0.0)) { atomicExch(info, 1); d = 1e-300; }
+            L[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const double djj = L[j][j];
+        for (int r = j + 1 + threadIdx.x; r < nb; r += 256) L[r][j] /= djj;
+        __syncthreads();
+        // trailing update of the block: L[r][c] -= L[r][j] L[c][j], c in (j, r]
+        for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+            const int r = i % kCB, c = i / kCB;
+            if (c > j && r >= c && r < nb) L[r][c] -= L[r][j] * L[c][j];
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        if (r < nb && c < nb && r >= c) a[(o + r) + (size_t)(o + c) * n] = L[r][c];
+    }
+}
+
+// panel: A[ib, kb] <- A[ib, kb] L_kk^-T for every row block ib > kb (blockIdx.x)
+__global__ void __launch_bounds__(256) k_potrf_panel(int n, double* __restrict__ A, size_t strideA, int kb) {
+    double* a = A + blockIdx.z * strideA;
+    const int o = kb * kCB;
+    const int ib = kb + 1 + blockIdx.x;
+    const int ro = ib * kCB;
+    if (ro >= n) return;
+    const int nr = min(kCB, n - ro), nb = min(kCB, n - o);
+    __shared__ double L[kCB][kCB + 1];
+    __shared__ double X[kCB][kCB + 1];
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : 0.0;
+        X[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : 0.0;
+    }
+    __syncthreads();
+    // each row r of X solves x L^T = a  (forward substitution over columns)
+    if (threadIdx.x < nr) {
+        const int r = threadIdx.x;
+        for (int c = 0; c < nb; ++c) {
+            double s = X[r][c];
+            for (int t = 0; t < c; ++t) s -= X[r][t] * L[c][t];
+            X[r][c] = s / L[c][c];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        if (r < nr && c < nb) a[(ro + r) + (size_t)(o + c) * n] = X[r][c];
+    }
+}
+
+// trailing update: A[ib, jb] -= L[ib, kb] L[jb, kb]^T for kb < jb <= ib
+__global__ void __launch_bounds__(256) k_potrf_update(int n, double* __restrict__ A, size_t strideA, int kb,
+                                                      const int2* __restrict__ pairs) {
+    double* a = A + blockIdx.z * strideA;
+    const int2 pr = pairs[blockIdx.x];
+    const int ib = pr.x, jb = pr.y;
+    const int o = kb * kCB, ro = ib * kCB, co = jb * kCB;
+    const int nb = min(kCB, n - o), nr = min(kCB, n - ro), nc = min(kCB, n - co);
+    __shared__ double Li[kCB][kCB + 1];
+    __shared__ double Lj[kCB][kCB + 1];
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        Li[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : 0.0;
+        Lj[r][c] = (r < nc && c < nb) ? a[(co + r) + (size_t)(o + c) * n] : 0.0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
+        const int r = i % kCB, c = i / kCB;
+        if (r < nr && c < nc && (ib != jb || r >= c)) {
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += Li[r][t] * Lj[c][t];
+            a[(ro + r) + (size_t)(co + c) * n] -= s;
+        }
+    }
+}
+
+// solve L L^T y = b (one block per batch entry), blocked substitution
+__global__ void __launch_bounds__(256) k_potrs(int n, const double* __restrict__ A, size_t strideA,
+                                               const double* __restrict__ b, size_t strideb,
+                                               double* __restrict__ y, size_t stridey) {
+    const double* a = A + blockIdx.x * strideA;
+    double* yv = y + blockIdx.x * stridey;
+    const double* bv = b + blockIdx.x * strideb;
+    for (int i = threadIdx.x; i < n; i += 256) yv[i] = bv[i];
+    __syncthreads();
+    __shared__ double z[kCB];
+    const int nblk = (n + kCB - 1) / kCB;
+    // forward: L z = b
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int o = kb * kCB, nb = min(kCB, n - o);
+        if (threadIdx.x == 0) {
+            for (int r = 0; r < nb; ++r) {
+                double s = yv[o + r];
+                for (int t = 0; t < r; ++t) s -= a[(o + r) + (size_t)(o + t) * n] * z[t];
+                z[r] = s / a[(o + r) + (size_t)(o + r) * n];
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nb; r += 256) yv[o + r] = z[r];
+        for (int r = o + nb + threadIdx.x; r < n; r += 256) {
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += a[r + (size_t)(o + t) * n] * z[t];
+            yv[r] -= s;
+        }
+        __syncthreads();
+    }
+    // backward: L^T y = z
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int o = kb * kCB, nb = min(kCB, n - o);
+        if (threadIdx.x == 0) {
+            for (int r = nb - 1; r >= 0; --r) {
+                double s = yv[o + r];
+                for (int t = r + 1; t < nb; ++t) s -= a[(o + t) + (size_t)(o + r) * n] * z[t];
+                z[r] = s / a[(o + r) + (size_t)(o + r) * n];
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nb; r += 256) yv[o + r] = z[r];
+        // y[0:o] -= L[o:o+nb, 0:o]^T z
+        for (int c = threadIdx.x; c < o; c += 256) {
+            double s = 0.0;
+            for (int t = 0; t < nb; ++t) s += a[(o + t) + (size_t)c * n] * z[t];
+            yv[c] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+// residual r = U_meas - U_calc (FP64), gamma += delta
+__global__ void __launch_bounds__(256) k_sub(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                             double* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n) out[i] = a[i] - b[i];
+}
+
+__global__ void __launch_bounds__(256) k_add_to(int n, double* __restrict__ y, const double* __restrict__ dy) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n) y[i] += dy[i];
+}
+
+__global__ void __launch_bounds__(256) k_gamma_update(int n, float* __restrict__ g, const double* __restrict__ d) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n) g[i] = (float)((double)g[i] + d[i]);
+}
+
+// --------------------------------------------------------------------------------
+// launch wrappers
+// --------------------------------------------------------------------------------
+static inline void check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(MPF_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use64, float* U32,
+                     double* U64, hipStream_t st) {
+    k_voltages<<<njobs, 256, 0, st>>>(d_jobs, md, use64, U32, U64);
+    check_launch();
+}
+
+void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use64, float* J,
+                     hipStream_t st) {
+    MPF_REQUIRE(jd.Mp * jd.Dp <= kJacMaxRows, "jacobian: too many rows");
+    MPF_REQUIRE(jd.D + jd.M <= 64 && jd.k <= 6, "jacobian: unsupported shape");
+    dim3 grid(jd.E, njobs);
+    k_jacobian<<<grid, 256, 0, st>>>(d_jobs, jd, use64, J);
+    check_launch();
+}
+
+size_t gram_workspace_bytes(int n, int nk) {
+    const int nt = (n + kGT - 1) / kGT;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int kslab = gram_kslab(n, nk);
+    const int nslab = (nk + kslab - 1) / kslab;
+    return (size_t)ntiles * nslab * kGT * kGT * sizeof(float) + (size_t)ntiles * sizeof(int2);
+}
+
+int gram_kslab(int n, int nk) {
+    const int nt = (n + kGT - 1) / kGT;
+    const int ntiles = nt * (nt + 1) / 2;
+    // aim for >= 1024 blocks, slabs a multiple of kGK and at least 256 deep
+    int nslab = (1024 + ntiles - 1) / ntiles;
+    int kslab = (nk + nslab - 1) / nslab;
+    kslab = ((kslab + kGK - 1) / kGK) * kGK;
+    if (kslab < 256) kslab = 256;
+    return kslab;
+}
+
+void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
+                 hipStream_t st) {
+    const int nt = (n + kGT - 1) / kGT;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int kslab = gram_kslab(n, nk);
+    const int nslab = (nk + kslab - 1) / kslab;
+    float* part = (float*)work;
+    int2* tiles = (int2*)((char*)work + (size_t)ntiles * nslab * kGT * kGT * sizeof(float));
+    std::vector<int2> h;
+    for (int i = 0; i < nt; ++i)
+        for (int j = i; j < nt; ++j) h.push_back(make_int2(i, j));
+    MPF_HIP(hipMemcpyAsync(tiles, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    MPF_HIP(hipStreamSynchronize(st));  // h goes out of scope
+    dim3 grid(ntiles, 1, nslab);
+    k_gram_mfma<<<grid, 256, 0, st>>>(A, trans, n, nk, lda, tiles, kslab, part);
+    k_gram_reduce<<<ntiles, 256, 0, st>>>(part, tiles, ntiles, nslab, n, G);
+    check_launch();
+}
+
+void launch_gemv_t(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                   size_t stridex, double* y, size_t stridey, int batch, hipStream_t st) {
+    dim3 grid((cols + 3) / 4, batch);
+    k_gemv_t<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, y, stridey, 0.0);
+    check_launch();
+}
+
+size_t gemv_n_workspace(int rows, int cols, int batch) {
+    return (size_t)batch * ((cols + kNSlab - 1) / kNSlab) * rows * sizeof(double);
+}
+
+void launch_gemv_n(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                   size_t stridex, double shift, const double* z, size_t stridez, const double* rhs,
+                   size_t striderhs, double* out, size_t strideout, int batch, double* work,
+                   hipStream_t st) {
+    const int nslab = (cols + kNSlab - 1) / kNSlab;
+    dim3 g1((rows + 255) / 256, nslab, batch);
+    k_gemv_n_part<<<g1, 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, work);
+    dim3 g2((rows + 255) / 256, batch);
+    k_gemv_n_reduce<<<g2, 256, 0, st>>>(rows, nslab, work, shift, z, stridez, rhs, striderhs, out, strideout);
+    check_launch();
+}
+
+void launch_cholesky(int n, const float* G, size_t strideG, double shift, double* A, size_t strideA,
+                     int batch, int* d_info, int2* d_pairs, hipStream_t st) {
+    dim3 g0((unsigned)(((size_t)n * n + 255) / 256), batch);
+    k_to64_shift<<<g0, 256, 0, st>>>(n, G, strideG, shift, A, strideA);
+    const int nblk = (n + kCB - 1) / kCB;
+    // all (ib, jb) pairs with jb <= ib, ordered so that pairs for step kb are a suffix
+    std::vector<int2> pairs;
+    std::vector<int> start(nblk + 1, 0);
+    for (int kb = 0; kb < nblk; ++kb) {
+        start[kb] = (int)pairs.size();
+        for (int jb = kb + 1; jb < nblk; ++jb)
+            for (int ib = jb; ib < nblk; ++ib) pairs.push_back(make_int2(ib, jb));
+    }
+    start[nblk] = (int)pairs.size();
+    if (!pairs.empty()) {
+        MPF_HIP(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+        MPF_HIP(hipStreamSynchronize(st));
+    }
+    for (int kb = 0; kb < nblk; ++kb) {
+        k_potrf_diag<<<dim3(1, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_info);
+        if (kb + 1 < nblk) {
+            k_potrf_panel<<<dim3(nblk - kb - 1, 1, batch), 256, 0, st>>>(n, A, strideA, kb);
+            const int np = start[kb + 1] - start[kb];
+            k_potrf_update<<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + start[kb]);
+        }
+    }
+    check_launch();
+}
+
+size_t cholesky_pairs(int n) {
+    const int nblk = (n + kCB - 1) / kCB;
+    return (size_t)nblk * nblk * nblk / 2 + 16;
+}
+
+void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_t strideb, double* y,
+                  size_t stridey, int batch, hipStream_t st) {
+    k_potrs<<<batch, 256, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
+    check_launch();
+}
+
+void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t st) {
+    k_sub<<<(n + 255) / 256, 256, 0, st>>>(n, a, b, out);
+    check_launch();
+}
+
+void launch_add_to(int n, double* y, const double* dy, hipStream_t st) {
+    k_add_to<<<(n + 255) / 256, 256, 0, st>>>(n, y, dy);
+    check_launch();
+}
+
+void launch_gamma_update(int n, float* g, const double* d, hipStream_t st) {
+    k_gamma_update<<<(n + 255) / 256, 256, 0, st>>>(n, g, d);
+    check_launch();
+}
+
+}  // namespace mpf

@@ -1,0 +1,71 @@
+// Device-side job descriptors shared by the sparse and dense kernels.
+//
+// A "job" is one frame on one mesh: its own sigma-weighted S/R values, harmonics
+// ("slices"), Krylov vectors and scalars.  Jobs of one launch may differ in size;
+// the kernels walk a flat table of (job, slice, row tile) work items.
+//
+// Vector layout (all Krylov vectors): [slice][row][lane], lane stride Rp in
+// {16, 32, 64} >= R = D + M.  A wavefront covers 64/Rp rows x Rp right-hand sides,
+// so every gathered neighbour row is one contiguous Rp*4-byte load.
+#pragma once
+
+#include "mpf_internal.h"
+
+namespace mpf {
+
+// scalar fields per (slice, lane), FP64
+enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
+               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_NFIELDS };
+// per-slice integer state: [0] active, [1] steps, [2] hit cap
+constexpr int kStInts = 4;
+// partial-sum channels per tile
+constexpr int kParts = 4;
+
+struct JobDev {
+    int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, pad0;
+    double tol;
+    const int32_t* ids;          // [width][np]
+    float* s;  float* r;         // FP32 values (hi)        [width][np]
+    float* s_lo; float* r_lo;    // FP32 low parts (FP64 = hi + lo)
+    float* dinv;                 // [C][np]
+    float alpha[kMaxSlices];     // alpha_c of A_c = S + alpha_c R
+    float xscale[kMaxSlices];    // beta_c excitation scale
+    const double* F;             // [np][Rp] excitation B*Pi, FP64
+    const float* rhs;            // optional [C][np][Rp]; null -> beta_c * F
+    float* x; float* rv; float* p; float* q;   // [C][np][Rp]
+    float* t1; float* t2;        // BiCGSTAB extra vectors [C][np][Rp]
+    double* x64;                 // [C][np][Rp] FP64 refined solution
+    double* part;                // [kParts][C][ntiles][Rp]
+    double* sc;                  // [SC_NFIELDS][C][Rp]
+    int* st;                     // [C][kStInts]
+    // assembly
+    const int32_t* ctb_off; const int32_t* ctb_elem;
+    const double* ctb_k; const double* ctb_m;
+    const float* gamma;          // [E]
+    double ref; int logarithmic; int E;
+    const int32_t* diag_slot;    // [np]
+};
+
+__host__ __device__ inline size_t vec_index(const JobDev& J, int s, int row, int lane) {
+    return ((size_t)s * J.np + row) * J.Rp + lane;
+}
+
+// host launch wrappers (sparse.hip)
+void launch_assemble(const JobDev* d_jobs, const int4* d_items, int nitems, hipStream_t st);
+void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
+void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                     const int2* d_js, int njs, int* d_active, hipStream_t st);
+void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                     const int2* d_js, int njs, int* d_active, hipStream_t st);
+void launch_bicgstab_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                          const int2* d_js, int njs, int* d_active, hipStream_t st);
+void launch_bicgstab_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                          const int2* d_js, int njs, int* d_active, hipStream_t st);
+void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                       const int2* d_js, int njs, int first, hipStream_t st);
+void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,
+                       hipStream_t st);
+void launch_spmv(int np, int width, const int32_t* ids, const float* S, const float* R,
+                 int C, const float* alpha, const float* X, float* Y, int Rp, hipStream_t st);
+
+}  // namespace mpf

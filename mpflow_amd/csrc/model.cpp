@@ -1,0 +1,253 @@
+// Host orchestration of the EIT forward solve and the Tikhonov Gauss-Newton step.
+//
+// models::EIT::solve (src/models/eit.cpp:139-213) runs the C harmonics one after the
+// other, each with ~13 launches and a device->host copy per CG iteration.  Here every
+// harmonic of every frame iterates in the same launches; the host only polls a device
+// counter of unconverged (frame, harmonic) pairs every `check_every` iterations.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "model.h"
+
+namespace mpf {
+
+Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
+    MPF_REQUIRE(d.frames >= 1, "model: frames < 1");
+    frames = d.frames;
+    quirks = d.quirks;
+    build_setup(d, S);
+    hipStream_t st0 = ctx->stream;
+    const int np = S.np, W = S.width;
+    ids.alloc((size_t)W * np); ids.upload(S.ids.data(), S.ids.size(), st0);
+    diag_slot.alloc(np); diag_slot.upload(S.diag_slot.data(), np, st0);
+    ctb_off.alloc(S.ctb_off.size()); ctb_off.upload(S.ctb_off.data(), S.ctb_off.size(), st0);
+    ctb_elem.alloc(S.ctb_elem.size()); ctb_elem.upload(S.ctb_elem.data(), S.ctb_elem.size(), st0);
+    ctb_k.alloc(S.ctb_k.size()); ctb_k.upload(S.ctb_k.data(), S.ctb_k.size(), st0);
+    ctb_m.alloc(S.ctb_m.size()); ctb_m.upload(S.ctb_m.data(), S.ctb_m.size(), st0);
+    elements.alloc(S.elements.size()); elements.upload(S.elements.data(), S.elements.size(), st0);
+    ek.alloc(S.elem_k.size()); ek.upload(S.elem_k.data(), S.elem_k.size(), st0);
+    em.alloc(S.elem_m.size()); em.upload(S.elem_m.data(), S.elem_m.size(), st0);
+    F.alloc(S.F.size()); F.upload(S.F.data(), S.F.size(), st0);
+    w_off.alloc(S.w_off.size()); w_off.upload(S.w_off.data(), S.w_off.size(), st0);
+    w_node.alloc(std::max<size_t>(1, S.w_node.size())); if (!S.w_node.empty()) w_node.upload(S.w_node.data(), S.w_node.size(), st0);
+    w_val.alloc(std::max<size_t>(1, S.w_val.size())); if (!S.w_val.empty()) w_val.upload(S.w_val.data(), S.w_val.size(), st0);
+
+    const size_t ell = (size_t)W * np;
+    const size_t V = vec_elems();
+    const int ntile_job = np / kTileRows;
+    s.alloc(ell * frames); r.alloc(ell * frames); s_lo.alloc(ell * frames); r_lo.alloc(ell * frames);
+    dinv.alloc((size_t)S.C * np * frames);
+    x.alloc(V * frames); rv.alloc(V * frames); p.alloc(V * frames); q.alloc(V * frames);
+    rhs.alloc(V * frames);
+    x64.alloc(V * frames);
+    part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
+    sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
+    st.alloc((size_t)S.C * kStInts * frames);
+    gamma.alloc((size_t)S.E * frames);
+    MPF_HIP(hipMemsetAsync(st.p, 0, st.bytes(), st0));
+    MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
+    MPF_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st0));
+
+    hjobs.resize(frames);
+    for (int f = 0; f < frames; ++f) {
+        JobDev& J = hjobs[f];
+        std::memset(&J, 0, sizeof(J));
+        J.n = S.N; J.np = np; J.width = W; J.C = S.C; J.R = S.R; J.Rp = S.Rp;
+        J.lg = S.Rp == 16 ? 4 : (S.Rp == 32 ? 5 : 6);
+        J.ntiles = ntile_job;
+        J.dc = 0;  // Open sources: harmonic 0 is solved dcFree (eit.cpp:175-177)
+        J.maxit = 10 * S.N;
+        J.rule = MPF_STOP_REFERENCE;
+        J.tol = 1e-6;
+        J.ids = ids.p;
+        J.s = s.p + ell * f; J.r = r.p + ell * f; J.s_lo = s_lo.p + ell * f; J.r_lo = r_lo.p + ell * f;
+        J.dinv = dinv.p + (size_t)S.C * np * f;
+        for (int c = 0; c < S.C; ++c) { J.alpha[c] = (float)S.alpha[c]; J.xscale[c] = (float)S.beta[c]; }
+        J.F = F.p;
+        J.rhs = nullptr;
+        J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f;
+        J.t1 = nullptr; J.t2 = nullptr;
+        J.x64 = x64.p + V * f;
+        J.part = part.p + (size_t)kParts * S.C * ntile_job * S.Rp * f;
+        J.sc = sc.p + (size_t)SC_NFIELDS * S.C * S.Rp * f;
+        J.st = st.p + (size_t)S.C * kStInts * f;
+        J.ctb_off = ctb_off.p; J.ctb_elem = ctb_elem.p; J.ctb_k = ctb_k.p; J.ctb_m = ctb_m.p;
+        J.gamma = gamma.p + (size_t)S.E * f;
+        J.ref = S.ref; J.logarithmic = S.logarithmic; J.E = S.E;
+        J.diag_slot = diag_slot.p;
+    }
+    jobs.alloc(frames);
+    upload_jobs();
+
+    // work tables
+    std::vector<int4> ht;
+    std::vector<int2> hj;
+    std::vector<int4> ha;
+    const int nasm_job = (int)((ell + kBlock - 1) / kBlock);
+    for (int f = 0; f < frames; ++f) {
+        for (int c = 0; c < S.C; ++c) {
+            hj.push_back(make_int2(f, c));
+            for (int t = 0; t < ntile_job; ++t) ht.push_back(make_int4(f, c, t, 0));
+        }
+        for (int b = 0; b < nasm_job; ++b) ha.push_back(make_int4(f, b, 0, 0));
+    }
+    ntiles = (int)ht.size(); njs = (int)hj.size(); nasm = (int)ha.size();
+    tiles.alloc(ntiles); tiles.upload(ht.data(), ht.size(), st0);
+    js.alloc(njs); js.upload(hj.data(), hj.size(), st0);
+    asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
+    active.alloc(1);
+    MPF_HIP(hipHostMalloc((void**)&h_active, sizeof(int), hipHostMallocDefault));
+    MPF_HIP(hipStreamSynchronize(st0));
+}
+
+Model::~Model() {
+    if (h_active) (void)hipHostFree(h_active);
+}
+
+void Model::upload_jobs() {
+    jobs.upload(hjobs.data(), hjobs.size(), ctx->stream);
+}
+
+void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, mpf_solve_stats* stats,
+                   bool* capped) {
+    hipStream_t st0 = ctx->stream;
+    const int maxit = o.max_iter > 0 ? o.max_iter : 10 * S.N;
+    for (auto& J : hjobs) {
+        J.rule = rule; J.tol = tol; J.maxit = maxit;
+        J.rhs = use_rhs ? rhs.p + vec_elems() * (&J - hjobs.data()) : nullptr;
+    }
+    upload_jobs();
+    MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));
+    launch_pcg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
+    const int every = o.check_every > 0 ? o.check_every : 16;
+    int it = 0;
+    while (true) {
+        for (int i = 0; i < every; ++i) launch_pcg_iter(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
+        it += every;
+        MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
+        MPF_HIP(hipStreamSynchronize(st0));
+        if (*h_active <= 0 || it >= maxit + every) break;
+    }
+    // step counts
+    std::vector<int> hst((size_t)S.C * kStInts * frames);
+    MPF_HIP(hipMemcpyAsync(hst.data(), st.p, hst.size() * sizeof(int), hipMemcpyDeviceToHost, st0));
+    MPF_HIP(hipStreamSynchronize(st0));
+    for (int f = 0; f < frames; ++f)
+        for (int c = 0; c < S.C; ++c) {
+            const int* sp = &hst[((size_t)f * S.C + c) * kStInts];
+            if (stats) {
+                stats->steps += sp[1];
+                stats->max_steps = std::max<int64_t>(stats->max_steps, sp[1]);
+                stats->not_converged += sp[2];
+            }
+            if (sp[2]) *capped = true;
+        }
+}
+
+int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats) {
+    hipStream_t st0 = ctx->stream;
+    MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG, "only the CG Krylov method is available in this build");
+    // Equation::update: S, R once per sigma (not once per harmonic), then Jacobi
+    launch_assemble(jobs.p, asm_items.p, nasm, st0);
+    launch_dinv(jobs.p, tiles.p, ntiles, st0);
+    bool capped = false;
+    const bool mixed = o.precision == MPF_PREC_MIXED;
+    krylov(o, o.stop_rule, o.tol, false, stats, &capped);
+    if (mixed) {
+        launch_x64_update(jobs.p, tiles.p, ntiles, 1, st0);
+        for (int k = 0; k < o.refine_steps; ++k) {
+            launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, 0, st0);
+            krylov(o, MPF_STOP_TOLERANCE, o.refine_tol, true, stats, &capped);
+            launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
+        }
+        // final FP64 residual for the report
+        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, 0, st0);
+        if (stats) {
+            std::vector<double> hsc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
+            MPF_HIP(hipMemcpyAsync(hsc.data(), sc.p, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, st0));
+            MPF_HIP(hipStreamSynchronize(st0));
+            double worst = 0.0;
+            for (int f = 0; f < frames; ++f)
+                for (int c = 0; c < S.C; ++c)
+                    for (int l = 0; l < S.R; ++l) {
+                        const double* b = &hsc[(size_t)SC_NFIELDS * S.C * S.Rp * f];
+                        const double rr = b[((size_t)SC_X1 * S.C + c) * S.Rp + l];
+                        const double bb = b[((size_t)SC_X2 * S.C + c) * S.Rp + l];
+                        if (bb > 0) worst = std::max(worst, std::sqrt(rr / bb));
+                    }
+            stats->refine_residual = worst;
+        }
+    }
+    MeasDev md{w_off.p, w_node.p, w_val.p, S.D, S.M, S.Dp, S.Mp};
+    launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
+    if (d_J) {
+        JacDev jd;
+        std::memset(&jd, 0, sizeof(jd));
+        jd.elements = elements.p; jd.ek = ek.p; jd.em = em.p; jd.gamma = gamma.p;
+        jd.E = S.E; jd.k = S.k; jd.D = S.D; jd.M = S.M; jd.Dp = S.Dp; jd.Mp = S.Mp;
+        jd.exact = S.jacobian_exact; jd.logarithmic = S.logarithmic; jd.ref = S.ref;
+        for (int c = 0; c < S.C; ++c) {
+            jd.alpha[c] = (float)S.alpha[c];
+            jd.weight[c] = S.jacobian_exact ? (float)(1.0 / S.beta[c]) : (float)S.height;
+        }
+        launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
+    }
+    MPF_HIP(hipStreamSynchronize(st0));
+    return capped ? MPF_ENOCONV : MPF_OK;
+}
+
+void Model::ensure_gn() {
+    const int m = S.mrows;
+    if (Jm.p) return;
+    Jm.alloc((size_t)frames * m * S.E);
+    G.alloc((size_t)frames * m * m);
+    A64.alloc((size_t)frames * m * m);
+    U64.alloc((size_t)frames * m);
+    y.alloc((size_t)frames * m); dy.alloc((size_t)frames * m); res.alloc((size_t)frames * m);
+    rgn.alloc((size_t)frames * m);
+    vE.alloc((size_t)frames * S.E);
+    work_n.alloc(gemv_n_workspace(m, S.E, frames) / sizeof(double) + 1);
+    gram_work.alloc(gram_workspace_bytes(m, S.E));
+    pairs.alloc(cholesky_pairs(m));
+    info.alloc(1);
+}
+
+int Model::gn_step(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine,
+                   const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats) {
+    hipStream_t st0 = ctx->stream;
+    ensure_gn();
+    const int m = S.mrows, E = S.E;
+    MPF_HIP(hipMemcpyAsync(gamma.p, d_gamma, sizeof(float) * (size_t)E * frames, hipMemcpyDeviceToDevice, st0));
+    const int rc = forward(o, nullptr, U64.p, Jm.p, stats);
+    if (rc != MPF_OK && rc != MPF_ENOCONV) return rc;
+    if (d_Ucalc)
+        MPF_HIP(hipMemcpyAsync(d_Ucalc, U64.p, sizeof(double) * (size_t)m * frames, hipMemcpyDeviceToDevice, st0));
+    // r = U_meas - U(gamma)  (Inverse::calcExcitation, inverse.cpp:163-193)
+    launch_sub(m * frames, d_Umeas, U64.p, rgn.p, st0);
+    // J J^T on MFMA, then (J J^T + lam I) by FP64 Cholesky
+    for (int f = 0; f < frames; ++f)
+        launch_gram(Jm.p + (size_t)f * m * E, 0, m, E, m, G.p + (size_t)f * m * m, gram_work.p, st0);
+    const double lam2 = (S.ref * lambda) * (S.ref * lambda);
+    MPF_HIP(hipMemsetAsync(info.p, 0, sizeof(int), st0));
+    launch_cholesky(m, G.p, (size_t)m * m, lam2, A64.p, (size_t)m * m, frames, info.p, pairs.p, st0);
+    launch_potrs(m, A64.p, (size_t)m * m, rgn.p, m, y.p, m, frames, st0);
+    // FP64 refinement of (J J^T + lam I) y = r with residuals from the FP32 Jacobian
+    for (int k = 0; k < gn_refine; ++k) {
+        launch_gemv_t(m, E, Jm.p, m, (size_t)m * E, y.p, m, vE.p, E, frames, st0);
+        launch_gemv_n(m, E, Jm.p, m, (size_t)m * E, vE.p, E, lam2, y.p, m, rgn.p, m, res.p, m, frames,
+                      work_n.p, st0);
+        launch_potrs(m, A64.p, (size_t)m * m, res.p, m, dy.p, m, frames, st0);
+        launch_add_to(m * frames, y.p, dy.p, st0);
+    }
+    // delta = J^T y ; gamma += delta   (solver.cpp:222)
+    launch_gemv_t(m, E, Jm.p, m, (size_t)m * E, y.p, m, vE.p, E, frames, st0);
+    launch_gamma_update(E * frames, d_gamma, vE.p, st0);
+    int hinfo = 0;
+    MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
+    MPF_HIP(hipStreamSynchronize(st0));
+    if (hinfo) throw Error(MPF_EINVAL, "gn_step: normal-equation matrix is not positive definite");
+    return rc;
+}
+
+}  // namespace mpf

@@ -1,0 +1,62 @@
+// Host orchestration of the EIT path: models::EIT and solver::Solver re-laid out for
+// multi-frame batched device solves (model.cpp).
+#pragma once
+
+#include <chrono>
+
+#include "dense.h"
+
+struct mpf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+namespace mpf {
+
+struct Model {
+    mpf_ctx* ctx = nullptr;
+    MeshSetup S;
+    int frames = 1;
+    int quirks = 1;
+    // mesh data on the device
+    DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node;
+    DBuf<double> ctb_k, ctb_m, F, w_val;
+    DBuf<float> ek, em;
+    // per-frame state
+    DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;
+    DBuf<double> x64, part, sc;
+    DBuf<int> st;
+    DBuf<float> gamma;
+    // job tables
+    std::vector<JobDev> hjobs;
+    DBuf<JobDev> jobs;
+    DBuf<int4> tiles, asm_items;
+    DBuf<int2> js;
+    int ntiles = 0, nasm = 0, njs = 0;
+    DBuf<int> active;
+    int* h_active = nullptr;   // pinned
+    // Gauss-Newton buffers (allocated on first use)
+    DBuf<float> Jm, G;
+    DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;
+    DBuf<char> gram_work;
+    DBuf<int2> pairs;
+    DBuf<int> info;
+
+    Model(mpf_ctx* c, const mpf_model_desc& d);
+    ~Model();
+
+    size_t vec_elems() const { return (size_t)S.C * S.np * S.Rp; }
+    void upload_jobs();
+    // one forward solve of every frame; gamma already in `gamma`
+    int forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats);
+    int gn_step(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine,
+                const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats);
+
+  private:
+    void krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, mpf_solve_stats* stats,
+                bool* capped);
+    void ensure_gn();
+};
+
+}  // namespace mpf

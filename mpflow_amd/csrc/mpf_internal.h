@@ -1,0 +1,104 @@
+// Internal declarations shared by the host set-up, the HIP kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpflow_amd.h"
+
+namespace mpf {
+
+constexpr int kMaxSlices = 8;      // harmonics per job
+constexpr int kTileRows = 64;      // rows per work tile of the sparse kernels
+constexpr int kBlock = 256;        // threads per block of the sparse kernels
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MPF_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            throw ::mpf::Error(MPF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MPF_REQUIRE(cond, msg)                                                     \
+    do {                                                                           \
+        if (!(cond)) throw ::mpf::Error(MPF_EINVAL, msg);                          \
+    } while (0)
+
+inline int ceil16(int n) { return (n % 16 == 0 || n == 1) ? n : (n / 16 + 1) * 16; }
+inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
+
+// Device buffer with RAII.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) {
+            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            if (e != hipSuccess) { p = nullptr; n = 0; throw Error(MPF_ENOMEM, "hipMalloc failed"); }
+        }
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        MPF_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// Host-side finite-element set-up (setup.cpp)
+// ---------------------------------------------------------------------------
+struct MeshSetup {
+    int N = 0, E = 0, k = 0, np = 0, width = 0, nnz = 0;
+    int P = 0, D = 0, M = 0, R = 0, Rp = 0, Mp = 0, Dp = 0, mrows = 0;
+    // ELL pattern, slot-major: [width][np]; padding slots id = row
+    std::vector<int32_t> ids;
+    std::vector<int32_t> diag_slot;               // [np]
+    // gather plan: contributions of (element, local ij) to ELL entry slot*np+row
+    std::vector<int32_t> ctb_off;                 // [width*np + 1]
+    std::vector<int32_t> ctb_elem;                // [E*k*k]
+    std::vector<double> ctb_k, ctb_m;             // element stiffness / mass entries
+    // element data for the Jacobian: K_e and M_e row-major k x k
+    std::vector<float> elem_k, elem_m;            // [E][k*k]
+    std::vector<int32_t> elements;                // [E][k]
+    // excitation F = B * Pi (FP64), [np][Rp]
+    std::vector<double> F;
+    // measurement W = Mpat^T B^T as CSR over M rows
+    std::vector<int32_t> w_off, w_node;
+    std::vector<double> w_val;
+    // 2.5D constants per harmonic
+    int C = 1;
+    double alpha[kMaxSlices] = {}, beta[kMaxSlices] = {};
+    double ref = 1.0, height = 1.0, port_height = 1.0;
+    int logarithmic = 0, jacobian_exact = 0;
+};
+
+void build_setup(const mpf_model_desc& d, MeshSetup& s);
+
+// element integrals (K_e, M_e) of one straight triangle, P1 or P2, row-major k x k
+void element_integrals(int k, const double* xy, double* K, double* Mm);
+
+// Ports::circularBoundary restatement (host)
+std::vector<std::vector<int32_t>> circular_ports(int N, const double* nodes, int E, int k,
+    const int32_t* elements, int nb, int kb, const int32_t* boundary, int count,
+    double width, double offset, bool clockwise);
+
+}  // namespace mpf

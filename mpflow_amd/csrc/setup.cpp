@@ -1,0 +1,334 @@
+// Host-side finite-element set-up: element integrals, the ELL pattern, the gather
+// plan for sigma-weighted assembly, the gap-model excitation and the measurement map.
+//
+// Follows FEM::Equation::Equation / initElementalMatrices / initExcitationMatrix /
+// initJacobianCalculationMatrix (src/fem/equation.cpp:29-253) and models::EIT::EIT
+// (src/models/eit.cpp:29-95), re-laid out for the device kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "mpf_internal.h"
+
+namespace mpf {
+namespace {
+
+// Polynomial in barycentric coordinates: c[p][q][r] * l0^p l1^q l2^r, degree <= 4.
+struct Poly {
+    double c[5][5][5];
+    Poly() { std::memset(c, 0, sizeof(c)); }
+};
+
+Poly mul(const Poly& a, const Poly& b) {
+    Poly out;
+    for (int p = 0; p < 5; ++p) for (int q = 0; q + p < 5; ++q) for (int r = 0; r + p + q < 5; ++r) {
+        double v = a.c[p][q][r];
+        if (v == 0.0) continue;
+        for (int s = 0; p + s < 5; ++s) for (int t = 0; q + t < 5; ++t) for (int u = 0; r + u < 5; ++u) {
+            if (p + s + q + t + r + u > 4) continue;
+            out.c[p + s][q + t][r + u] += v * b.c[s][t][u];
+        }
+    }
+    return out;
+}
+
+Poly deriv(const Poly& a, int var) {
+    Poly out;
+    for (int p = 0; p < 5; ++p) for (int q = 0; q < 5; ++q) for (int r = 0; r < 5; ++r) {
+        double v = a.c[p][q][r];
+        if (v == 0.0) continue;
+        if (var == 0 && p > 0) out.c[p - 1][q][r] += v * p;
+        if (var == 1 && q > 0) out.c[p][q - 1][r] += v * q;
+        if (var == 2 && r > 0) out.c[p][q][r - 1] += v * r;
+    }
+    return out;
+}
+
+double fact(int n) { double f = 1; for (int i = 2; i <= n; ++i) f *= i; return f; }
+
+// int_T l0^p l1^q l2^r dA = 2 A p! q! r! / (p+q+r+2)!
+double integrate(const Poly& a, double area) {
+    double s = 0.0;
+    for (int p = 0; p < 5; ++p) for (int q = 0; q < 5; ++q) for (int r = 0; r < 5; ++r) {
+        double v = a.c[p][q][r];
+        if (v == 0.0) continue;
+        s += v * fact(p) * fact(q) * fact(r) / fact(p + q + r + 2);
+    }
+    return 2.0 * area * s;
+}
+
+void nodal_basis(int k, Poly* phi) {
+    if (k == 3) {
+        phi[0].c[1][0][0] = 1; phi[1].c[0][1][0] = 1; phi[2].c[0][0][1] = 1;
+        return;
+    }
+    // vertices: l (2 l - 1); midpoints: 4 l_a l_b   (order v0 v1 v2 m01 m12 m20)
+    phi[0].c[2][0][0] = 2; phi[0].c[1][0][0] = -1;
+    phi[1].c[0][2][0] = 2; phi[1].c[0][1][0] = -1;
+    phi[2].c[0][0][2] = 2; phi[2].c[0][0][1] = -1;
+    phi[3].c[1][1][0] = 4;
+    phi[4].c[0][1][1] = 4;
+    phi[5].c[1][0][1] = 4;
+}
+
+}  // namespace
+
+void element_integrals(int k, const double* xy, double* K, double* Mm) {
+    const double x0 = xy[0], y0 = xy[1], x1 = xy[2], y1 = xy[3], x2 = xy[4], y2 = xy[5];
+    const double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+    const double area = 0.5 * std::fabs(det);
+    // grad l_a = (y_b - y_c, x_c - x_b) / det for (a,b,c) cyclic
+    const double gl[3][2] = {{(y1 - y2) / det, (x2 - x1) / det},
+                             {(y2 - y0) / det, (x0 - x2) / det},
+                             {(y0 - y1) / det, (x1 - x0) / det}};
+    Poly phi[6];
+    nodal_basis(k, phi);
+    Poly dphi[6][3];
+    for (int i = 0; i < k; ++i) for (int a = 0; a < 3; ++a) dphi[i][a] = deriv(phi[i], a);
+    for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) {
+        Mm[i * k + j] = integrate(mul(phi[i], phi[j]), area);
+        double s = 0.0;
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
+            double g = gl[a][0] * gl[b][0] + gl[a][1] * gl[b][1];
+            s += g * integrate(mul(dphi[i][a], dphi[j][b]), area);
+        }
+        K[i * k + j] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ports::circularBoundary (src/fem/ports.cpp:32-77) with math::polar /
+// circleParameter / circularPoints (src/mathematics.cpp:26-99)
+// ---------------------------------------------------------------------------
+static double polar_angle(double x, double y) {
+    if (x > 0.0) return std::atan(y / x);
+    if (x < 0.0 && y >= 0.0) return std::atan(y / x) + M_PI;
+    if (x < 0.0 && y < 0.0) return std::atan(y / x) - M_PI;
+    if (x == 0.0 && y > 0.0) return M_PI / 2;
+    if (x == 0.0 && y < 0.0) return -M_PI / 2;
+    return 0.0;
+}
+
+static double circle_parameter(double x, double y, double offset) {
+    double r = std::sqrt(x * x + y * y);
+    double a = polar_angle(x, y) - offset / r;
+    a += (a < -M_PI) ? 2.0 * M_PI : 0.0;
+    a -= (a > M_PI) ? 2.0 * M_PI : 0.0;
+    return a * r;
+}
+
+std::vector<std::vector<int32_t>> circular_ports(int N, const double* nodes, int E, int k,
+    const int32_t* elements, int nb, int kb, const int32_t* boundary, int count,
+    double width, double offset, bool clockwise) {
+    MPF_REQUIRE(count > 0, "ports: count <= 0");
+    double radius = 0.0;
+    for (int e = 0; e < E; ++e)
+        for (int i = 0; i < 3; ++i) {
+            int n = elements[e * k + i];
+            MPF_REQUIRE(n >= 0 && n < N, "ports: element node out of range");
+            radius = std::max(radius, std::sqrt(nodes[2 * n] * nodes[2 * n] + nodes[2 * n + 1] * nodes[2 * n + 1]));
+        }
+    auto corner = [&](double off, int l, double* px, double* py) {
+        double phi = off / radius + l * (2.0 * M_PI * radius / count) / radius;
+        *px = radius * std::cos(phi);
+        *py = (clockwise ? -1.0 : 1.0) * radius * std::sin(phi);
+    };
+    std::vector<std::vector<int32_t>> out(count);
+    for (int port = 0; port < count; ++port) {
+        double sx, sy, ex, ey;
+        corner(offset + (clockwise ? width : 0.0), port, &sx, &sy);
+        corner(offset + (clockwise ? 0.0 : width), port, &ex, &ey);
+        for (int b = 0; b < nb; ++b) {
+            const int32_t* ed = boundary + (size_t)b * kb;
+            double x0 = nodes[2 * ed[0]], y0 = nodes[2 * ed[0] + 1];
+            double xl = nodes[2 * ed[kb - 1]], yl = nodes[2 * ed[kb - 1] + 1];
+            double par_last = std::sqrt((xl - x0) * (xl - x0) + (yl - y0) * (yl - y0));
+            double off0 = circle_parameter(x0, y0, 0.0);
+            double a = circle_parameter(sx, sy, off0);
+            double c = circle_parameter(ex, ey, off0);
+            if (a < c && 0.0 - a >= -1e-9 && par_last - c <= 1e-9) out[port].push_back(b);
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// full set-up
+// ---------------------------------------------------------------------------
+void build_setup(const mpf_model_desc& d, MeshSetup& s) {
+    MPF_REQUIRE(d.nodes && d.elements && d.boundary, "model: null mesh array");
+    MPF_REQUIRE(d.n_nodes > 0 && d.n_elements > 0, "model: empty mesh");
+    MPF_REQUIRE(d.nodes_per_element == 3 || d.nodes_per_element == 6, "model: nodes_per_element must be 3 or 6");
+    MPF_REQUIRE(d.nodes_per_edge == (d.nodes_per_element == 3 ? 2 : 3), "model: nodes_per_edge does not match the element type");
+    MPF_REQUIRE(d.n_ports > 0 && d.port_edges, "model: no ports");
+    MPF_REQUIRE(d.n_drive > 0 && d.n_meas > 0 && d.drive_pattern && d.meas_pattern && d.drive_current, "model: no sources");
+    MPF_REQUIRE(d.components >= 1 && d.components <= kMaxSlices, "model: components must be in [1, 8]");
+    MPF_REQUIRE(d.height > 0.0, "model: height <= 0.0");
+    MPF_REQUIRE(d.port_height > 0.0, "model: portHeight <= 0.0");
+    const int N = d.n_nodes, E = d.n_elements, k = d.nodes_per_element;
+    s.N = N; s.E = E; s.k = k;
+    s.np = round_up(N, kTileRows);
+    s.P = d.n_ports; s.D = d.n_drive; s.M = d.n_meas; s.R = s.D + s.M;
+    MPF_REQUIRE(s.R <= 64, "model: D + M > 64 is not supported");
+    s.Rp = s.R <= 16 ? 16 : (s.R <= 32 ? 32 : 64);
+    s.Mp = ceil16(s.M); s.Dp = ceil16(s.D); s.mrows = s.Mp * s.Dp;
+    s.C = d.components;
+    s.ref = d.reference_value; s.height = d.height; s.port_height = d.port_height;
+    s.logarithmic = d.logarithmic; s.jacobian_exact = d.jacobian_exact;
+    for (int c = 0; c < s.C; ++c) {
+        s.alpha[c] = std::pow(2.0 * c * M_PI / d.height, 2);
+        s.beta[c] = c == 0 ? 1.0 / d.height
+                           : 2.0 * std::sin(c * M_PI * d.port_height / d.height) / (c * M_PI * d.port_height);
+    }
+    const int cap = d.ell_max_width > 0 ? d.ell_max_width : 32;
+    MPF_REQUIRE(cap <= 64, "model: ell_max_width > 64");
+
+    s.elements.assign(d.elements, d.elements + (size_t)E * k);
+    for (auto v : s.elements) MPF_REQUIRE(v >= 0 && v < N, "model: element node index out of range");
+
+    // element integrals
+    std::vector<double> Ke((size_t)E * k * k), Me((size_t)E * k * k);
+    for (int e = 0; e < E; ++e) {
+        double xy[12];
+        const int32_t* el = d.elements + (size_t)e * k;
+        for (int i = 0; i < 3; ++i) { xy[2 * i] = d.nodes[2 * el[i]]; xy[2 * i + 1] = d.nodes[2 * el[i] + 1]; }
+        if (k == 6) {
+            static const int pr[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+            double hx = std::fabs(xy[2] - xy[0]) + std::fabs(xy[3] - xy[1]) + 1e-300;
+            for (int t = 0; t < 3; ++t) {
+                int a = el[pr[t][0]], b = el[pr[t][1]], m = el[3 + t];
+                double mx = 0.5 * (d.nodes[2 * a] + d.nodes[2 * b]), my = 0.5 * (d.nodes[2 * a + 1] + d.nodes[2 * b + 1]);
+                MPF_REQUIRE(std::fabs(mx - d.nodes[2 * m]) + std::fabs(my - d.nodes[2 * m + 1]) < 1e-9 * (1.0 + hx) * 1e3,
+                            "model: P2 element nodes must be ordered [v0 v1 v2 m01 m12 m20] with straight midpoints");
+            }
+        }
+        element_integrals(k, xy, &Ke[(size_t)e * k * k], &Me[(size_t)e * k * k]);
+    }
+    s.elem_k.resize(Ke.size());
+    s.elem_m.resize(Me.size());
+    for (size_t i = 0; i < Ke.size(); ++i) { s.elem_k[i] = (float)Ke[i]; s.elem_m[i] = (float)Me[i]; }
+
+    // ELL pattern: first-insertion order of the element loop (equation.cpp:131-144)
+    std::vector<int32_t> cols((size_t)N * cap, -1);
+    std::vector<int32_t> cnt(N, 0);
+    std::vector<int32_t> slot_of((size_t)E * k * k);
+    int width = 0;
+    for (int e = 0; e < E; ++e) {
+        const int32_t* el = d.elements + (size_t)e * k;
+        for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) {
+            int row = el[i], col = el[j];
+            int32_t* rc = &cols[(size_t)row * cap];
+            int sl = -1;
+            for (int q = 0; q < cnt[row]; ++q) if (rc[q] == col) { sl = q; break; }
+            if (sl < 0) {
+                if (cnt[row] >= cap)
+                    throw Error(MPF_EFULL, "mpFlow::numeric::SparseMatrix::setValue: sparse format full");
+                sl = cnt[row]++;
+                rc[sl] = col;
+                width = std::max(width, cnt[row]);
+            }
+            slot_of[((size_t)e * k + i) * k + j] = sl;
+        }
+    }
+    s.width = width;
+    const int np = s.np;
+    s.ids.assign((size_t)width * np, 0);
+    s.diag_slot.assign(np, 0);
+    s.nnz = 0;
+    for (int row = 0; row < np; ++row) {
+        for (int q = 0; q < width; ++q) {
+            int col = (row < N && q < cnt[row]) ? cols[(size_t)row * cap + q] : row;
+            s.ids[(size_t)q * np + row] = col;
+            if (row < N && q < cnt[row] && col == row) s.diag_slot[row] = q;
+        }
+        if (row < N) s.nnz += cnt[row];
+    }
+    // gather plan, contributions in element order per ELL entry
+    const size_t nent = (size_t)width * np;
+    s.ctb_off.assign(nent + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* el = d.elements + (size_t)e * k;
+        for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j)
+            s.ctb_off[(size_t)slot_of[((size_t)e * k + i) * k + j] * np + el[i] + 1]++;
+    }
+    for (size_t i = 0; i < nent; ++i) s.ctb_off[i + 1] += s.ctb_off[i];
+    const size_t total = (size_t)E * k * k;
+    s.ctb_elem.resize(total); s.ctb_k.resize(total); s.ctb_m.resize(total);
+    std::vector<int32_t> fill(s.ctb_off.begin(), s.ctb_off.end() - 1);
+    for (int e = 0; e < E; ++e) {
+        const int32_t* el = d.elements + (size_t)e * k;
+        for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) {
+            size_t ent = (size_t)slot_of[((size_t)e * k + i) * k + j] * np + el[i];
+            int32_t pos = fill[ent]++;
+            s.ctb_elem[pos] = e;
+            s.ctb_k[pos] = Ke[((size_t)e * k + i) * k + j];
+            s.ctb_m[pos] = Me[((size_t)e * k + i) * k + j];
+        }
+    }
+
+    // gap-model excitation B (equation.cpp:183-215); B stored sparse per node
+    const int kb = d.nodes_per_edge;
+    std::vector<std::map<int, double>> Bn(N);
+    for (int p = 0; p < s.P; ++p) {
+        std::vector<int> edges;
+        for (int q = 0; q < d.port_max_edges; ++q) {
+            int eidx = d.port_edges[(size_t)p * d.port_max_edges + q];
+            if (eidx < 0) continue;
+            MPF_REQUIRE(eidx < d.n_boundary, "model: port edge index out of range");
+            edges.push_back(eidx);
+        }
+        double length = 0.0;
+        for (int eidx : edges) {
+            const int32_t* ed = d.boundary + (size_t)eidx * kb;
+            double dx = d.nodes[2 * ed[kb - 1]] - d.nodes[2 * ed[0]], dy = d.nodes[2 * ed[kb - 1] + 1] - d.nodes[2 * ed[0] + 1];
+            length += std::sqrt(dx * dx + dy * dy);
+        }
+        for (int eidx : edges) {
+            const int32_t* ed = d.boundary + (size_t)eidx * kb;
+            double sp[3];
+            for (int q = 0; q < kb; ++q) {
+                double dx = d.nodes[2 * ed[q]] - d.nodes[2 * ed[0]], dy = d.nodes[2 * ed[q] + 1] - d.nodes[2 * ed[0] + 1];
+                sp[q] = std::sqrt(dx * dx + dy * dy);
+            }
+            double w[3];
+            if (kb == 2) {
+                w[0] = w[1] = 0.5 * sp[1];
+            } else {
+                // 1D quadratic Lagrange basis on nodes (0, s1, s2) integrated over [0, s2]
+                double s1 = sp[1], s2 = sp[2];
+                double i0 = s2 - s2 * s2 * (s1 + s2) / (2.0 * s1 * s2) + s2 * s2 * s2 / (3.0 * s1 * s2);
+                double i1 = (s2 * s2 * s2 / 3.0 - s2 * s2 * s2 / 2.0) / (s1 * (s1 - s2));
+                double i2 = (s2 * s2 * s2 / 3.0 - s1 * s2 * s2 / 2.0) / (s2 * (s2 - s1));
+                w[0] = i0; w[1] = i1;
+                // the generated code evaluates the last node with node 1's coefficients
+                // (src/fem/codegen/templates/basis.cpp.mako:72-84)
+                w[2] = d.quirks ? i1 : i2;
+            }
+            for (int q = 0; q < kb; ++q) Bn[ed[q]][p] += w[q] / length;
+        }
+    }
+    // F = B * Pi, Pi = [I*Dpat | Mpat]  (sources.cpp:57-71)
+    s.F.assign((size_t)np * s.Rp, 0.0);
+    auto Pi = [&](int p, int j) {
+        return j < s.D ? d.drive_current[j] * d.drive_pattern[(size_t)p * s.D + j]
+                       : d.meas_pattern[(size_t)p * s.M + (j - s.D)];
+    };
+    for (int n = 0; n < N; ++n)
+        for (auto& kv : Bn[n])
+            for (int j = 0; j < s.R; ++j) s.F[(size_t)n * s.Rp + j] += kv.second * Pi(kv.first, j);
+    // W = Mpat^T B^T  (eit.cpp:93-94) as CSR
+    s.w_off.assign(s.M + 1, 0);
+    s.w_node.clear(); s.w_val.clear();
+    for (int m = 0; m < s.M; ++m) {
+        for (int n = 0; n < N; ++n) {
+            double v = 0.0;
+            bool any = false;
+            for (auto& kv : Bn[n]) { v += d.meas_pattern[(size_t)kv.first * s.M + m] * kv.second; any = true; }
+            if (any && v != 0.0) { s.w_node.push_back(n); s.w_val.push_back(v); }
+        }
+        s.w_off[m + 1] = (int32_t)s.w_node.size();
+    }
+}
+
+}  // namespace mpf

@@ -1,0 +1,201 @@
+"""Python host mirror of mpFlow's EIT API over the C ABI.
+
+Names and argument meaning follow the reference classes:
+  ``EIT``    -> models::EIT            (src/models/eit.cpp:29-213)
+  ``Solver`` -> solver::Solver         (src/solver/solver.cpp:28-225)
+Device buffers are torch tensors on ``cuda``; torch only provides memory and the
+stream, every computation runs in ``libmpflow_amd.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, lib
+
+__all__ = ["Context", "EIT", "Solver", "circular_ports", "SolveOptions"]
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Context:
+    """mpf_ctx: device + HIP stream.  With ``use_torch_stream`` the context runs on torch's
+    current stream so that torch-allocated buffers are ordered with the library's work."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        import torch
+        self.device = device
+        stream = None
+        if use_torch_stream:
+            stream = C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        h = C.c_void_p()
+        check(lib().mpf_ctx_create(device, stream, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(lib().mpf_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().mpf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwise=False):
+    """Ports::circularBoundary (src/fem/ports.cpp:32-77) via the C ABI; returns a
+    count x maxEdges int32 array of boundary-edge rows, -1 = unused."""
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    elements = np.ascontiguousarray(elements, np.int32)
+    boundary = np.ascontiguousarray(boundary, np.int32)
+    w = C.c_int32()
+    L = lib()
+    args = (nodes.shape[0], nodes.ctypes.data, elements.shape[0], elements.shape[1], elements.ctypes.data,
+            boundary.shape[0], boundary.shape[1], boundary.ctypes.data, int(count), float(width), float(offset),
+            int(bool(clockwise)))
+    check(L.mpf_ports_circular(*args, None, 0, C.byref(w)))
+    out = np.empty((count, max(w.value, 1)), np.int32)
+    check(L.mpf_ports_circular(*args, out.ctypes.data, out.size, C.byref(w)))
+    return out
+
+
+class SolveOptions:
+    def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
+                 refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16):
+        self.s = capi.SolveOpts()
+        lib().mpf_solve_opts_default(C.byref(self.s))
+        self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
+        self.s.precision, self.s.refine_steps, self.s.refine_tol = precision, refine_steps, refine_tol
+        self.s.krylov, self.s.check_every = krylov, check_every
+
+
+class EIT:
+    """models::EIT with Open (current) sources on one mesh, for ``frames`` frames at once."""
+
+    def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
+                 reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
+                 quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None):
+        import torch
+        self.ctx = ctx or Context()
+        self.torch = torch
+        self._keep = dict(
+            nodes=np.ascontiguousarray(nodes, np.float64),
+            elements=np.ascontiguousarray(elements, np.int32),
+            boundary=np.ascontiguousarray(boundary, np.int32),
+            ports=np.ascontiguousarray(ports, np.int32),
+            drive=np.ascontiguousarray(drive_pattern, np.float64),
+            meas=np.ascontiguousarray(meas_pattern, np.float64),
+        )
+        k = self._keep
+        D = k["drive"].shape[1]
+        cur = np.ascontiguousarray(np.broadcast_to(np.asarray(current, np.float64), (D,)))
+        k["current"] = cur
+        d = capi.ModelDesc()
+        d.n_nodes, d.n_elements = k["nodes"].shape[0], k["elements"].shape[0]
+        d.nodes_per_element = k["elements"].shape[1]
+        d.nodes, d.elements = _ptr(k["nodes"], C.c_double), _ptr(k["elements"], C.c_int32)
+        d.n_boundary, d.nodes_per_edge = k["boundary"].shape
+        d.boundary = _ptr(k["boundary"], C.c_int32)
+        d.n_ports, d.port_max_edges = k["ports"].shape
+        d.port_edges = _ptr(k["ports"], C.c_int32)
+        d.n_drive, d.n_meas = D, k["meas"].shape[1]
+        d.drive_pattern, d.meas_pattern = _ptr(k["drive"], C.c_double), _ptr(k["meas"], C.c_double)
+        d.drive_current = _ptr(cur, C.c_double)
+        d.reference_value, d.components = float(reference_value), int(components)
+        d.height, d.port_height = float(height), float(port_height)
+        d.logarithmic, d.quirks, d.jacobian_exact = int(logarithmic), int(quirks), int(jacobian_exact)
+        d.ell_max_width, d.frames = int(ell_max_width), int(frames)
+        h = C.c_void_p()
+        check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        dims = (C.c_int64 * 12)()
+        check(lib().mpf_model_dims(h, dims))
+        (self.N, self.E, self.np, self.width, self.m, self.Mp, self.Dp, self.R, self.Rp, self.frames,
+         self.C, self.nnz) = [int(v) for v in dims]
+        self.D, self.M = D, k["meas"].shape[1]
+        self.reference_value = float(reference_value)
+        dev = torch.device("cuda", self.ctx.device)
+        self.result = torch.zeros(self.frames, self.m, dtype=torch.float32, device=dev)
+        self.result64 = torch.zeros(self.frames, self.m, dtype=torch.float64, device=dev)
+        self.jacobian = None
+        self.last_stats = None
+
+    def _dev(self):
+        return self.torch.device("cuda", self.ctx.device)
+
+    def solve(self, gamma, options: SolveOptions | None = None, jacobian=False):
+        """EIT::solve: voltages (frames x m, column-major M x D with ld ceil16(M)).
+        ``gamma``: (frames, E) or (E,) float32 cuda tensor."""
+        torch = self.torch
+        g = gamma.reshape(self.frames, self.E).to(self._dev(), torch.float32).contiguous()
+        Jp = None
+        if jacobian:
+            if self.jacobian is None:
+                self.jacobian = torch.empty(self.frames, self.E, self.m, dtype=torch.float32, device=self._dev())
+            Jp = C.c_void_p(self.jacobian.data_ptr())
+        st = capi.SolveStats()
+        opts = (options or SolveOptions()).s
+        rc = lib().mpf_eit_solve(self.h, C.c_void_p(g.data_ptr()), C.c_void_p(self.result.data_ptr()),
+                                 C.c_void_p(self.result64.data_ptr()), Jp, C.byref(opts), C.byref(st))
+        check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
+        self.last_stats = st
+        return self.result
+
+    def voltages(self, U=None, dtype=None):
+        """(frames, M, D) view of the padded result."""
+        U = self.result64 if U is None else U
+        u = U.reshape(self.frames, self.Dp, self.Mp)[:, :self.D, :self.M].transpose(1, 2)
+        return u if dtype is None else u.to(dtype)
+
+    def jacobian_matrix(self):
+        """(frames, m, E) Jacobian (stored transposed-contiguous as E x m per frame)."""
+        return self.jacobian.transpose(1, 2)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpf_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """solver::Solver with identity Tikhonov regularisation; ``solve_absolute`` is one
+    Gauss-Newton step per frame (Solver::solveAbsolute, src/solver/solver.cpp:195-225)."""
+
+    def __init__(self, eit: EIT, regularization_factor: float, gn_refine: int = 2,
+                 options: SolveOptions | None = None):
+        torch = eit.torch
+        self.eit = eit
+        self.regularization_factor = float(regularization_factor)
+        self.gn_refine = int(gn_refine)
+        self.options = options or SolveOptions()
+        init = 1.0  # linear parametrisation starts at gamma = 1 (solver.cpp:55-62)
+        self.material_distribution = torch.full((eit.frames, eit.E), init, dtype=torch.float32,
+                                                device=eit._dev())
+        self.measurement = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
+        self.calculation = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
+        self.last_stats = None
+
+    def solve_absolute(self):
+        st = capi.SolveStats()
+        rc = lib().mpf_gn_step(self.eit.h, C.c_void_p(self.material_distribution.data_ptr()),
+                               C.c_void_p(self.measurement.data_ptr()), self.regularization_factor,
+                               self.gn_refine, C.byref(self.options.s),
+                               C.c_void_p(self.calculation.data_ptr()), C.byref(st))
+        check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
+        self.last_stats = st
+        return self.material_distribution

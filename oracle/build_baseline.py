@@ -1,0 +1,22 @@
+"""Build the oracle's C++ CPU baseline (oracle/_build/libcg_host.so) with g++."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libcg_host.so")
+
+
+def build():
+    src = os.path.join(HERE, "cg_host.cpp")
+    if not os.path.exists(src):
+        return None
+    os.makedirs(OUT, exist_ok=True)
+    cmd = ["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c++17", src, "-o", LIB + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build())

@@ -1,0 +1,480 @@
+"""FP64 CPU oracle for the EIT forward/inverse path of mpFlow.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the checker.
+The product path (``mpflow_amd``) never calls it.
+
+This module restates, in float64 numpy/scipy, the algorithm of the reference files
+cited at each function (paths relative to the mpFlow source tree).  The reference
+cannot be compiled here (CUDA 7 + cuBLAS + Eigen + libdistmesh are absent), so this
+restatement is the oracle; it is pinned by the checks listed in
+``tests/test_oracle.py`` (closed forms, invariants, brute force, finite differences,
+the analytic disc solution).
+
+Two modes exist where the reference has quirks:
+  * ``quirks=True`` reproduces the reference exactly (P2 boundary integral of node 2,
+    Jacobian with K_e only and the single factor ``height``);
+  * ``quirks=False`` is the mathematically correct variant.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+FLT_EPSILON = float(np.finfo(np.float32).eps)
+
+# ---------------------------------------------------------------------------
+# element integrals  (src/fem/basis_linear.cpp:27-91, src/fem/basis_quadratic.cpp:27-97)
+# ---------------------------------------------------------------------------
+
+# Strang-Fix 6-point rule, exact to degree 4 on the reference triangle
+# (weights sum to 1, points in barycentric coordinates).
+_Q4_A, _Q4_B = 0.445948490915965, 0.091576213509771
+_Q4_WA, _Q4_WB = 0.223381589678011, 0.109951743655322
+_Q4 = np.array([
+    [_Q4_A, _Q4_A, 1 - 2 * _Q4_A, _Q4_WA],
+    [_Q4_A, 1 - 2 * _Q4_A, _Q4_A, _Q4_WA],
+    [1 - 2 * _Q4_A, _Q4_A, _Q4_A, _Q4_WA],
+    [_Q4_B, _Q4_B, 1 - 2 * _Q4_B, _Q4_WB],
+    [_Q4_B, 1 - 2 * _Q4_B, _Q4_B, _Q4_WB],
+    [1 - 2 * _Q4_B, _Q4_B, _Q4_B, _Q4_WB],
+])
+
+
+def _monomials(x, y, k):
+    if k == 3:
+        return np.array([np.ones_like(x), x, y])
+    return np.array([np.ones_like(x), x, y, x * x, y * y, x * y])
+
+
+def _monomial_grads(x, y, k):
+    one, zero = np.ones_like(x), np.zeros_like(x)
+    if k == 3:
+        gx = np.array([zero, one, zero])
+        gy = np.array([zero, zero, one])
+    else:
+        gx = np.array([zero, one, zero, 2 * x, zero, y])
+        gy = np.array([zero, zero, one, zero, 2 * y, x])
+    return gx, gy
+
+
+def basis_coefficients(xy: np.ndarray) -> np.ndarray:
+    """Coefficients of the nodal basis in global monomials, column i = basis of node i.
+
+    The reference solves the Vandermonde system per basis function by Gaussian
+    elimination (``basis_linear.cpp:27-56``, ``basis_quadratic.cpp:27-59``,
+    ``include/mpflow/mathematics.h:59-110``); monomials are {1,x,y} (P1) and
+    {1,x,y,x^2,y^2,xy} (P2).
+    """
+    k = xy.shape[0]
+    V = _monomials(xy[:, 0], xy[:, 1], k).T  # V[node, monomial]
+    return np.linalg.solve(V, np.eye(k))
+
+
+def element_integrals(xy: np.ndarray):
+    """(K_e, M_e): K[i,j] = int grad(phi_i).grad(phi_j), M[i,j] = int phi_i phi_j,
+    integrated exactly (degree-4 rule) over the straight triangle ``xy[:3]``."""
+    k = xy.shape[0]
+    C = basis_coefficients(xy)
+    p0, p1, p2 = xy[0], xy[1], xy[2]
+    area = 0.5 * abs((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p1[1] - p0[1]) * (p2[0] - p0[0]))
+    lam = _Q4[:, :3]
+    w = _Q4[:, 3] * area
+    X = lam @ np.array([p0[0], p1[0], p2[0]])
+    Y = lam @ np.array([p0[1], p1[1], p2[1]])
+    mon = _monomials(X, Y, k)            # [mono, q]
+    gx, gy = _monomial_grads(X, Y, k)
+    phi = C.T @ mon                      # [node, q]
+    dphx, dphy = C.T @ gx, C.T @ gy
+    K = (dphx * w) @ dphx.T + (dphy * w) @ dphy.T
+    M = (phi * w) @ phi.T
+    return K, M
+
+
+def all_element_integrals(nodes, elements):
+    E, k = elements.shape
+    K = np.empty((E, k, k))
+    M = np.empty((E, k, k))
+    for e in range(E):
+        K[e], M[e] = element_integrals(nodes[elements[e]])
+    return K, M
+
+
+def boundary_integral(s: np.ndarray, node: int, quirks: bool = True) -> float:
+    """int over the edge parameter [s_0, s_last] of the 1D Lagrange basis of ``node``.
+
+    ``s`` holds the distances of the edge nodes from the first edge node.  The
+    reference's generated P2 code evaluates node 2 with node 1's coefficients
+    (``src/fem/codegen/templates/basis.cpp.mako:72-84`` rendered at
+    ``src/fem/basis_quadratic.cpp:135-147``); ``quirks=True`` reproduces that.
+    P1 has two identical branches that both integrate to L/2 (``basis_linear.cpp:95-124``).
+    """
+    n = s.size
+    if n == 2:
+        node = 0  # both branches of the generated code use node 0's coefficients
+    elif quirks and node == n - 1:
+        node = n - 2
+    V = np.vander(s, n, increasing=True)          # V[i, p] = s_i^p
+    c = np.linalg.solve(V, np.eye(n)[:, node])     # coefficients of the basis
+    a, b = s[0], s[-1]
+    return float(sum(c[p] * (b ** (p + 1) - a ** (p + 1)) / (p + 1) for p in range(n)))
+
+
+# ---------------------------------------------------------------------------
+# ports  (src/fem/ports.cpp:32-77, src/mathematics.cpp:26-99)
+# ---------------------------------------------------------------------------
+
+def _polar_angle(x, y):
+    # math::polar (src/mathematics.cpp:26-55)
+    if x > 0.0:
+        return math.atan(y / x)
+    if x < 0.0 and y >= 0.0:
+        return math.atan(y / x) + math.pi
+    if x < 0.0 and y < 0.0:
+        return math.atan(y / x) - math.pi
+    if x == 0.0 and y > 0.0:
+        return math.pi / 2
+    if x == 0.0 and y < 0.0:
+        return -math.pi / 2
+    return 0.0
+
+
+def _circle_parameter(pt, offset):
+    # math::circleParameter (src/mathematics.cpp:64-77)
+    r = math.hypot(pt[0], pt[1])
+    ang = _polar_angle(pt[0], pt[1]) - offset / r
+    ang += 2 * math.pi if ang < -math.pi else 0.0
+    ang -= 2 * math.pi if ang > math.pi else 0.0
+    return ang * r
+
+
+def circular_boundary_ports(nodes, elements, boundary, count, width, offset=0.0,
+                            clockwise=False):
+    """Ports::circularBoundary: edges of ``boundary`` inside each electrode arc.
+
+    Returns a ``count x maxEdges`` int array, -1 for unused slots (the reference's
+    ``invalidIndex``).  ``boundary`` holds counter-clockwise boundary edges as rows of
+    node indices; the returned entries index rows of ``boundary``.
+    """
+    radius = float(np.max(np.hypot(nodes[elements[:, :3], 0], nodes[elements[:, :3], 1])))
+    dist = 2.0 * math.pi * radius / count
+    def circ_points(off):
+        phi0 = off / radius
+        n = int(math.ceil((2 * math.pi) / (dist / radius)))
+        phi = phi0 + np.arange(n) * (dist / radius)
+        sgn = -1.0 if clockwise else 1.0
+        return np.stack([radius * np.cos(phi), sgn * radius * np.sin(phi)], axis=1)
+    ps = circ_points(offset + (width if clockwise else 0.0))
+    pe = circ_points(offset + (0.0 if clockwise else width))
+    out = []
+    for port in range(count):
+        sel = []
+        for ei, edge in enumerate(boundary):
+            pts = nodes[edge]
+            par = np.sqrt(np.sum((pts - pts[0]) ** 2, axis=1))
+            off0 = _circle_parameter(pts[0], 0.0)
+            a = _circle_parameter(ps[port], off0)
+            b = _circle_parameter(pe[port], off0)
+            if a < b and par[0] - a >= -1e-9 and par[-1] - b <= 1e-9:
+                sel.append(ei)
+        out.append(sel)
+    m = max(len(s) for s in out)
+    arr = -np.ones((count, m), dtype=np.int64)
+    for p, s in enumerate(out):
+        arr[p, :len(s)] = s
+    return arr
+
+
+# ---------------------------------------------------------------------------
+# assembly, excitation, pattern  (src/fem/equation.cpp:76-253, equation_kernel.cu:35-157,
+#                                  src/fem/sources.cpp:27-75)
+# ---------------------------------------------------------------------------
+
+def mat_param(gamma, ref, logarithmic=False):
+    """sigma(gamma) (equation_kernel.cu:35-43)."""
+    if logarithmic:
+        return ref * np.exp(np.log(10.0) * gamma / 10.0)
+    return ref * gamma
+
+
+def mat_param_deriv(gamma, ref, logarithmic=False):
+    """d sigma / d gamma (equation_kernel.cu:45-53)."""
+    if logarithmic:
+        return (np.log(10.0) / 10.0) * ref * np.exp(np.log(10.0) * gamma / 10.0)
+    return ref * np.ones_like(gamma)
+
+
+def assemble(elements, Ke, N, weights):
+    """sum_e w_e * Ke[e] scattered to the global N x N matrix (CSR, float64)."""
+    E, k = elements.shape
+    rows = np.repeat(elements, k, axis=1).ravel()
+    cols = np.tile(elements, (1, k)).ravel()
+    vals = (Ke * weights[:, None, None]).reshape(E, k * k).ravel()
+    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+
+
+def excitation_matrix(nodes, boundary, ports, quirks=True):
+    """Gap-model excitation B (N x P): B[n,p] += int_edge phi_n / |port p|
+    (Equation::initExcitationMatrix, src/fem/equation.cpp:183-215)."""
+    N = nodes.shape[0]
+    P = ports.shape[0]
+    B = np.zeros((N, P))
+    for p in range(P):
+        edges = [e for e in ports[p] if e >= 0]
+        length = 0.0
+        for e in edges:
+            pts = nodes[boundary[e]]
+            length += math.sqrt(np.sum((pts[-1] - pts[0]) ** 2))
+        for e in edges:
+            pts = nodes[boundary[e]]
+            s = np.sqrt(np.sum((pts - pts[0]) ** 2, axis=1))
+            for node in range(len(boundary[e])):
+                B[boundary[e][node], p] += boundary_integral(s, node, quirks) / length
+    return B
+
+
+def adjacent_pattern(L):
+    """Column d = e_d - e_{d+1 mod L} (drive and measurement are both adjacent)."""
+    P = np.zeros((L, L))
+    for d in range(L):
+        P[d, d] = 1.0
+        P[(d + 1) % L, d] = -1.0
+    return P
+
+
+def pattern_matrix(drive, meas, current):
+    """Pi = [I * Dpat | Mpat] (Sources::Sources, src/fem/sources.cpp:57-71)."""
+    cur = np.broadcast_to(np.asarray(current, dtype=np.float64), (drive.shape[1],))
+    return np.concatenate([drive * cur[None, :], meas], axis=1)
+
+
+def ceil16(n):
+    """Matrix padding rule (src/numeric/matrix.cpp:49-55, include/mpflow/mathematics.h:42-44)."""
+    return n if (n % 16 == 0 or n == 1) else (n // 16 + 1) * 16
+
+
+# ---------------------------------------------------------------------------
+# PCG restatement (src/numeric/conjugate_gradient.cpp:61-161, conjugate_gradient_kernel.cu)
+# ---------------------------------------------------------------------------
+
+def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
+                  rule="reference", tol=1e-6):
+    """Multi-RHS Jacobi-PCG as ``ConjugateGradient::solve`` runs it, in float64.
+
+    * x starts at 0;  dcFree adds ``1 1^T`` to the operator.
+    * alpha = rho_old / (p.q) and beta = rho / rho_old per column; a zero denominator
+      makes the updated vector 0 (``updateVectorKernel``, conjugate_gradient_kernel.cu:77-89).
+    * ``rule="reference"``: stop after step s when sqrt(|rho[0]| / (b.b)[0]) < eps
+      (column 0 only) and return s+1; otherwise stop after 10*N steps.
+      ``rule="tolerance"``: stop when every column has ||r||/||b|| < tol.
+    Returns (x, steps).
+    """
+    N, R = b.shape
+    iters = 10 * N if max_iter == 0 else max_iter
+    if dinv is None:
+        dinv = np.ones(N)
+    def op(v):
+        y = A @ v
+        if dc_free:
+            y = y + v.sum(axis=0, keepdims=True)
+        return y
+    x = np.zeros((N, R))
+    r = b.copy()
+    z = dinv[:, None] * r
+    p = z.copy()
+    rho_old = np.sum(r * z, axis=0)
+    ref = np.sum(b * b, axis=0)
+    for step in range(iters):
+        q = op(p)
+        pq = np.sum(p * q, axis=0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            a = np.where(pq != 0.0, rho_old / np.where(pq != 0, pq, 1.0), 0.0)
+        r = np.where(pq != 0.0, r - a * q, 0.0)
+        x = np.where(pq != 0.0, x + a * p, 0.0)
+        z = dinv[:, None] * r
+        rho = np.sum(r * z, axis=0)
+        if rule == "reference":
+            with np.errstate(divide="ignore", invalid="ignore"):
+                crit = abs(math.sqrt(abs(rho[0]) / ref[0])) if ref[0] != 0 else float("nan")
+            if not (crit >= eps):
+                return x, step + 1
+        else:
+            rr = np.sum(r * r, axis=0)
+            ok = np.where(ref > 0, rr <= (tol * tol) * ref, True)
+            if np.all(ok):
+                return x, step + 1
+        with np.errstate(divide="ignore", invalid="ignore"):
+            beta = np.where(rho_old != 0.0, rho / np.where(rho_old != 0, rho_old, 1.0), 0.0)
+        p = np.where(rho_old != 0.0, z + beta * p, 0.0)
+        rho_old = rho
+    return x, iters
+
+
+def solve_direct(A, F, dc_free):
+    """Exact FP64 solve of (A [+ 1 1^T]) X = F via a sparse LU.
+
+    With dcFree the bordered system [[A, 1], [1^T, 0]] [x; l] = [F; 1^T F / N] is
+    solved; its x satisfies (A + 1 1^T) x = F because 1^T A = 0 (Neumann stiffness).
+    """
+    N = A.shape[0]
+    if not dc_free:
+        lu = spla.splu(sp.csc_matrix(A))
+        return lu.solve(F)
+    one = sp.csr_matrix(np.ones((N, 1)))
+    Bd = sp.bmat([[A, one], [one.T, None]], format="csc")
+    rhs = np.vstack([F, F.sum(axis=0, keepdims=True) / N])
+    lu = spla.splu(Bd)
+    return lu.solve(rhs)[:N]
+
+
+# ---------------------------------------------------------------------------
+# 2.5D forward model and Jacobian (src/models/eit.cpp:139-245, equation_kernel.cu:197-245)
+# ---------------------------------------------------------------------------
+
+class EITOracle:
+    """FP64 restatement of ``models::EIT`` with Open (current) sources.
+
+    Parameters mirror ``EIT(mesh, sources, referenceValue, ..., components, height,
+    portHeight)`` (src/models/eit.cpp:29-95).
+    """
+
+    def __init__(self, nodes, elements, boundary, ports, drive, meas, current,
+                 reference_value=1.0, components=1, height=1.0, port_height=1.0,
+                 quirks=True, logarithmic=False):
+        self.nodes = np.asarray(nodes, np.float64)
+        self.elements = np.asarray(elements, np.int64)
+        self.boundary = np.asarray(boundary, np.int64)
+        self.ports = np.asarray(ports, np.int64)
+        self.N = self.nodes.shape[0]
+        self.E, self.k = self.elements.shape
+        self.drive = np.asarray(drive, np.float64)
+        self.meas = np.asarray(meas, np.float64)
+        self.D, self.M = self.drive.shape[1], self.meas.shape[1]
+        self.Pi = pattern_matrix(self.drive, self.meas, current)
+        self.ref = float(reference_value)
+        self.C = int(components)
+        self.h = float(height)
+        self.hp = float(port_height)
+        self.quirks = quirks
+        self.log = logarithmic
+        self.Ke, self.Me = all_element_integrals(self.nodes, self.elements)
+        self.B = excitation_matrix(self.nodes, self.boundary, self.ports, quirks)
+        self.W = self.meas.T @ self.B.T          # electrodesAttachmentMatrix (eit.cpp:93-94)
+        self.Mp = ceil16(self.M)
+        self.Dp = ceil16(self.D)
+
+    def alpha_beta(self, c):
+        """2.5D constants (eit.cpp:154-157)."""
+        alpha = (2.0 * c * math.pi / self.h) ** 2
+        beta = (1.0 / self.h) if c == 0 else \
+            2.0 * math.sin(c * math.pi * self.hp / self.h) / (c * math.pi * self.hp)
+        return alpha, beta
+
+    def system(self, gamma, c):
+        sig = mat_param(np.asarray(gamma, np.float64), self.ref, self.log)
+        S = assemble(self.elements, self.Ke, self.N, sig)
+        R = assemble(self.elements, self.Me, self.N, sig)
+        alpha, _ = self.alpha_beta(c)
+        return (S + alpha * R).tocsr()
+
+    def fields(self, gamma, method="direct", **cg):
+        """Phi_c (N x (D+M)) for every harmonic; also returns CG step counts."""
+        out, steps = [], 0
+        for c in range(self.C):
+            A = self.system(gamma, c)
+            _, beta = self.alpha_beta(c)
+            F = beta * (self.B @ self.Pi)
+            dc = (c == 0)
+            if method == "direct":
+                X = solve_direct(A, F, dc)
+            else:
+                dinv = 1.0 / A.diagonal()
+                X, s = pcg_reference(A, F, dinv, dc, **cg)
+                steps += s
+            out.append(X)
+        return out, steps
+
+    def voltages(self, phis):
+        """U = sum_c W Phi_c[:, :D] (M x D) (eit.cpp:192-198, 239-244)."""
+        U = np.zeros((self.M, self.D))
+        for X in phis:
+            U += self.W @ X[:, :self.D]
+        return U
+
+    def solve(self, gamma, method="direct", **cg):
+        phis, steps = self.fields(gamma, method, **cg)
+        return self.voltages(phis), phis, steps
+
+    def jacobian(self, gamma, phis, quirks=None):
+        """J[(d * ceil16(M) + m), e] (rows ceil16(D)*ceil16(M), padding rows 0).
+
+        Reference (quirks): J = -h * sigma'_e * sum_c Phi_c,d^T K_e Phi_c,m
+        (calcJacobianKernel, equation_kernel.cu:197-245; eit.cpp:180-182, 202-205).
+        Exact: J = -sigma'_e * sum_c (1/beta_c) Phi_c,d^T (K_e + alpha_c M_e) Phi_c,m.
+        """
+        quirks = self.quirks if quirks is None else quirks
+        ds = mat_param_deriv(np.asarray(gamma, np.float64), self.ref, self.log)
+        J = np.zeros((self.Dp, self.Mp, self.E))
+        el = self.elements
+        for c, X in enumerate(phis):
+            alpha, beta = self.alpha_beta(c)
+            Kc = self.Ke if quirks else self.Ke + alpha * self.Me
+            w = self.h if quirks else 1.0 / beta
+            Pd = X[el][:, :, :self.D]            # [E, k, D]
+            Pm = X[el][:, :, self.D:]            # [E, k, M]
+            T = np.einsum("eij,ejm->eim", Kc, Pm)
+            J[:self.D, :self.M, :] += w * np.einsum("eid,eim->dme", Pd, T)
+        J *= -ds[None, None, :]
+        return J.reshape(self.Dp * self.Mp, self.E)
+
+    def flatten_voltages(self, U):
+        """U (M x D) column-major with ld ceil16(M), padded to m rows (inverse.cpp:176-193)."""
+        out = np.zeros((self.Dp, self.Mp))
+        out[:self.D, :self.M] = U.T
+        return out.reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# Tikhonov Gauss-Newton (src/solver/inverse.cpp:60-238, src/solver/solver.cpp:142-225)
+# ---------------------------------------------------------------------------
+
+def gn_step_exact(J, r, lam_eff2):
+    """delta = (J^T J + lam I)^-1 J^T r, computed as J^T (J J^T + lam I)^-1 r
+    (Cholesky of the m x m data-space matrix, FP64).  Equal to the E x E normal
+    equations of Inverse::solve by the push-through identity."""
+    G = J @ J.T + lam_eff2 * np.eye(J.shape[0])
+    cf = sla.cho_factor(G)
+    return J.T @ sla.cho_solve(cf, r)
+
+
+def gn_step_normal(J, r, lam_eff2):
+    """delta from the E x E normal equations by dense Cholesky (small E only)."""
+    A = J.T @ J + lam_eff2 * np.eye(J.shape[1])
+    return sla.cho_solve(sla.cho_factor(A), J.T @ r)
+
+
+def gn_step_cg_reference(J, r, lam_eff2, max_iter=0, eps=FLT_EPSILON):
+    """Inverse::solve as written: A = J^T J + (sigma_ref lambda)^2 I, b = J^T r,
+    unpreconditioned CG from 0 with the column-0 stop rule (inverse.cpp:212-238)."""
+    A = J.T @ J + lam_eff2 * np.eye(J.shape[1])
+    b = (J.T @ r)[:, None]
+    x, steps = pcg_reference(A, b, None, False, max_iter, eps)
+    return x[:, 0], steps
+
+
+def reconstruct(oracle: EITOracle, U_meas, gamma0, lam, steps=3, quirks_jacobian=None):
+    """``steps`` x Solver::solveAbsolute from gamma0; lam is regularizationFactor and the
+    diagonal term is (referenceValue * lam)^2 (inverse.cpp:73-75)."""
+    g = np.asarray(gamma0, np.float64).copy()
+    lam2 = (oracle.ref * lam) ** 2
+    um = oracle.flatten_voltages(U_meas)
+    for _ in range(steps):
+        U, phis, _ = oracle.solve(g)
+        J = oracle.jacobian(g, phis, quirks_jacobian)
+        r = um - oracle.flatten_voltages(U)
+        g = g + gn_step_exact(J, r, lam2)
+    return g
