@@ -31,7 +31,9 @@ struct JobDev {
     float alpha[kMaxSlices];     // alpha_c of A_c = S + alpha_c R
     float xscale[kMaxSlices];    // beta_c excitation scale
     const double* F;             // [np][Rp] excitation B*Pi, FP64
-    const float* rhs;            // optional [C][np][Rp]; null -> beta_c * F
+    float* rhs;                  // [C][np][Rp] FP32 right-hand side buffer (always valid)
+    int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
+    int pad1;
     float* x; float* rv; float* p; float* q;   // [C][np][Rp]
     float* t1; float* t2;        // BiCGSTAB extra vectors [C][np][Rp]
     double* x64;                 // [C][np][Rp] FP64 refined solution

@@ -65,7 +65,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.dinv = dinv.p + (size_t)S.C * np * f;
         for (int c = 0; c < S.C; ++c) { J.alpha[c] = (float)S.alpha[c]; J.xscale[c] = (float)S.beta[c]; }
         J.F = F.p;
-        J.rhs = nullptr;
+        J.rhs = rhs.p + V * f;
+        J.use_rhs = 0;
         J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f;
         J.t1 = nullptr; J.t2 = nullptr;
         J.x64 = x64.p + V * f;
@@ -115,7 +116,7 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     const int maxit = o.max_iter > 0 ? o.max_iter : 10 * S.N;
     for (auto& J : hjobs) {
         J.rule = rule; J.tol = tol; J.maxit = maxit;
-        J.rhs = use_rhs ? rhs.p + vec_elems() * (&J - hjobs.data()) : nullptr;
+        J.use_rhs = use_rhs ? 1 : 0;
     }
     upload_jobs();
     MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));

@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(const JobDev* __restrict__ 
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
-        if (L.lane < J.R) b = J.rhs ? J.rhs[i] : (float)(xs * J.F[(size_t)row * J.Rp + L.lane]);
+        if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[(size_t)row * J.Rp + L.lane]);
         const float z = dinv[row] * b;
         J.x[i] = 0.0f;
         J.rv[i] = b;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(kBlock) k_residual64(const JobDev* __restrict_
     const double xs = (double)J.xscale[s];
     const double dcs = sc_ptr(J, SC_X0, s)[L.lane];
     const double* X = J.x64 + (size_t)s * J.np * J.Rp;
-    float* out = const_cast<float*>(J.rhs);
+    float* out = J.rhs;
     double rr = 0.0, bb = 0.0;
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         double acc = 0.0;
