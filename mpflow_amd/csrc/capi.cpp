@@ -196,7 +196,7 @@ int mpf_ell_spmv(mpf_ctx* ctx, int32_t np, int32_t width, const int32_t* d_ids, 
                  int32_t ld) {
     return guarded([&] {
         MPF_REQUIRE(ctx && d_ids && d_S && d_R && alpha && d_X && d_Y, "mpf_ell_spmv: null argument");
-        MPF_REQUIRE(np > 0 && np % mpf::kTileRows == 0, "mpf_ell_spmv: np must be a positive multiple of 64");
+        MPF_REQUIRE(np > 0 && np % mpf::kTileRows == 0, "mpf_ell_spmv: np must be a positive multiple of 256");
         MPF_REQUIRE(ld == 16 || ld == 32 || ld == 64, "mpf_ell_spmv: ld must be 16, 32 or 64");
         MPF_REQUIRE(R >= 1 && R <= ld, "mpf_ell_spmv: R out of range");
         MPF_REQUIRE(C >= 1 && C <= mpf::kMaxSlices && width >= 1 && width <= 64, "mpf_ell_spmv: bad C or width");

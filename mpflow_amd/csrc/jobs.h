@@ -58,13 +58,9 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                      const int2* d_js, int njs, int* d_active, hipStream_t st);
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                     const int2* d_js, int njs, int* d_active, hipStream_t st);
-void launch_bicgstab_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                          const int2* d_js, int njs, int* d_active, hipStream_t st);
-void launch_bicgstab_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                          const int2* d_js, int njs, int* d_active, hipStream_t st);
+                     const int2* d_js, int njs, int* d_active, int lg, int wpad, hipStream_t st);
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                       const int2* d_js, int njs, int first, hipStream_t st);
+                       const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,
                        hipStream_t st);
 void launch_spmv(int np, int width, const int32_t* ids, const float* S, const float* R,

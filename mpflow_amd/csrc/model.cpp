@@ -18,7 +18,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     quirks = d.quirks;
     build_setup(d, S);
     hipStream_t st0 = ctx->stream;
-    const int np = S.np, W = S.width;
+    const int np = S.np, W = S.wpad;
     ids.alloc((size_t)W * np); ids.upload(S.ids.data(), S.ids.size(), st0);
     diag_slot.alloc(np); diag_slot.upload(S.diag_slot.data(), np, st0);
     ctb_off.alloc(S.ctb_off.size()); ctb_off.upload(S.ctb_off.data(), S.ctb_off.size(), st0);
@@ -96,6 +96,10 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     ntiles = (int)ht.size(); njs = (int)hj.size(); nasm = (int)ha.size();
     tiles.alloc(ntiles); tiles.upload(ht.data(), ht.size(), st0);
     js.alloc(njs); js.upload(hj.data(), hj.size(), st0);
+    act_tiles.alloc(ntiles); act_js.alloc(njs);
+    MPF_HIP(hipHostMalloc((void**)&h_tiles, sizeof(int4) * ntiles, hipHostMallocDefault));
+    MPF_HIP(hipHostMalloc((void**)&h_js, sizeof(int2) * njs, hipHostMallocDefault));
+    MPF_HIP(hipHostMalloc((void**)&h_st, sizeof(int) * (size_t)S.C * kStInts * frames, hipHostMallocDefault));
     asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
     active.alloc(1);
     MPF_HIP(hipHostMalloc((void**)&h_active, sizeof(int), hipHostMallocDefault));
@@ -104,6 +108,9 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
 
 Model::~Model() {
     if (h_active) (void)hipHostFree(h_active);
+    if (h_tiles) (void)hipHostFree(h_tiles);
+    if (h_js) (void)hipHostFree(h_js);
+    if (h_st) (void)hipHostFree(h_st);
 }
 
 void Model::upload_jobs() {
@@ -122,18 +129,40 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));
     launch_pcg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
     const int every = o.check_every > 0 ? o.check_every : 16;
+    const int lg = hjobs[0].lg, wpad = S.wpad;
+    const int ntile_job = S.np / kTileRows;
+    const size_t nst = (size_t)S.C * kStInts * frames;
+    // work lists restricted to the (frame, harmonic) pairs still iterating; rebuilt at
+    // every poll so converged harmonics cost no blocks
+    const int4* cur_tiles = tiles.p;
+    const int2* cur_js = js.p;
+    int cur_nt = ntiles, cur_njs = njs;
     int it = 0;
     while (true) {
-        for (int i = 0; i < every; ++i) launch_pcg_iter(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
+        for (int i = 0; i < every; ++i)
+            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, cur_js, cur_njs, active.p, lg, wpad, st0);
         it += every;
         MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
+        MPF_HIP(hipMemcpyAsync(h_st, st.p, nst * sizeof(int), hipMemcpyDeviceToHost, st0));
         MPF_HIP(hipStreamSynchronize(st0));
         if (*h_active <= 0 || it >= maxit + every) break;
+        int nt = 0, nj = 0;
+        for (int f = 0; f < frames; ++f)
+            for (int c = 0; c < S.C; ++c) {
+                if (!h_st[((size_t)f * S.C + c) * kStInts]) continue;
+                h_js[nj++] = make_int2(f, c);
+                for (int t = 0; t < ntile_job; ++t) h_tiles[nt++] = make_int4(f, c, t, 0);
+            }
+        if (nt == 0) break;
+        if (nt != cur_nt) {
+            MPF_HIP(hipMemcpyAsync(act_tiles.p, h_tiles, sizeof(int4) * nt, hipMemcpyHostToDevice, st0));
+            MPF_HIP(hipMemcpyAsync(act_js.p, h_js, sizeof(int2) * nj, hipMemcpyHostToDevice, st0));
+            // h_tiles / h_js are rewritten only after the next poll's synchronisation
+            cur_tiles = act_tiles.p; cur_js = act_js.p; cur_nt = nt; cur_njs = nj;
+        }
     }
     // step counts
-    std::vector<int> hst((size_t)S.C * kStInts * frames);
-    MPF_HIP(hipMemcpyAsync(hst.data(), st.p, hst.size() * sizeof(int), hipMemcpyDeviceToHost, st0));
-    MPF_HIP(hipStreamSynchronize(st0));
+    const int* hst = h_st;
     for (int f = 0; f < frames; ++f)
         for (int c = 0; c < S.C; ++c) {
             const int* sp = &hst[((size_t)f * S.C + c) * kStInts];
@@ -158,12 +187,12 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
     if (mixed) {
         launch_x64_update(jobs.p, tiles.p, ntiles, 1, st0);
         for (int k = 0; k < o.refine_steps; ++k) {
-            launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, 0, st0);
+            launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
             krylov(o, MPF_STOP_TOLERANCE, o.refine_tol, true, stats, &capped);
             launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
         }
         // final FP64 residual for the report
-        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, 0, st0);
+        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
         if (stats) {
             std::vector<double> hsc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
             MPF_HIP(hipMemcpyAsync(hsc.data(), sc.p, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, st0));

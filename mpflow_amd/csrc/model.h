@@ -36,6 +36,11 @@ struct Model {
     int ntiles = 0, nasm = 0, njs = 0;
     DBuf<int> active;
     int* h_active = nullptr;   // pinned
+    DBuf<int4> act_tiles;      // work lists of the (frame, harmonic) pairs still active
+    DBuf<int2> act_js;
+    int4* h_tiles = nullptr;   // pinned staging of the active lists
+    int2* h_js = nullptr;
+    int* h_st = nullptr;       // pinned copy of the per-pair state
     // Gauss-Newton buffers (allocated on first use)
     DBuf<float> Jm, G;
     DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;

@@ -13,8 +13,10 @@
 namespace mpf {
 
 constexpr int kMaxSlices = 8;      // harmonics per job
-constexpr int kTileRows = 64;      // rows per work tile of the sparse kernels
-constexpr int kBlock = 256;        // threads per block of the sparse kernels
+constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
+constexpr int kBlock = 256;        // threads per SpMV block: 4 wavefronts x 64 rows
+constexpr int kEwBlock = 1024;     // threads per streaming / reduction block
+constexpr int kSlotPad = 8;        // ELL width is padded to a multiple of this
 
 struct Error : std::runtime_error {
     int code;
@@ -67,13 +69,13 @@ struct DBuf {
 // Host-side finite-element set-up (setup.cpp)
 // ---------------------------------------------------------------------------
 struct MeshSetup {
-    int N = 0, E = 0, k = 0, np = 0, width = 0, nnz = 0;
+    int N = 0, E = 0, k = 0, np = 0, width = 0, wpad = 0, nnz = 0;
     int P = 0, D = 0, M = 0, R = 0, Rp = 0, Mp = 0, Dp = 0, mrows = 0;
-    // ELL pattern, slot-major: [width][np]; padding slots id = row
+    // ELL pattern, slot-major: [wpad][np]; padding slots id = row, value 0
     std::vector<int32_t> ids;
     std::vector<int32_t> diag_slot;               // [np]
     // gather plan: contributions of (element, local ij) to ELL entry slot*np+row
-    std::vector<int32_t> ctb_off;                 // [width*np + 1]
+    std::vector<int32_t> ctb_off;                 // [wpad*np + 1]
     std::vector<int32_t> ctb_elem;                // [E*k*k]
     std::vector<double> ctb_k, ctb_m;             // element stiffness / mass entries
     // element data for the Jacobian: K_e and M_e row-major k x k

@@ -232,12 +232,14 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
         }
     }
     s.width = width;
+    s.wpad = round_up(width, kSlotPad);
     const int np = s.np;
-    s.ids.assign((size_t)width * np, 0);
+    const int wp = s.wpad;
+    s.ids.assign((size_t)wp * np, 0);
     s.diag_slot.assign(np, 0);
     s.nnz = 0;
     for (int row = 0; row < np; ++row) {
-        for (int q = 0; q < width; ++q) {
+        for (int q = 0; q < wp; ++q) {
             int col = (row < N && q < cnt[row]) ? cols[(size_t)row * cap + q] : row;
             s.ids[(size_t)q * np + row] = col;
             if (row < N && q < cnt[row] && col == row) s.diag_slot[row] = q;
@@ -245,7 +247,7 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
         if (row < N) s.nnz += cnt[row];
     }
     // gather plan, contributions in element order per ELL entry
-    const size_t nent = (size_t)width * np;
+    const size_t nent = (size_t)wp * np;
     s.ctb_off.assign(nent + 1, 0);
     for (int e = 0; e < E; ++e) {
         const int32_t* el = d.elements + (size_t)e * k;

@@ -1,13 +1,18 @@
 // Sparse kernels of the forward solve on CDNA4 (gfx950):
 //   sigma-weighted gather assembly of S and R, the Jacobi diagonal, and the
-//   multi-job / multi-harmonic / multi-RHS Jacobi-PCG and BiCGSTAB building blocks.
+//   multi-job / multi-harmonic / multi-RHS Jacobi-PCG building blocks.
 //
-// Work items are (job, slice, 64-row tile).  A 256-thread block covers one tile of one
-// slice: lane = tid % Rp is the right-hand side, g = tid / Rp a row group, so one
-// wavefront reads 64/Rp rows x Rp contiguous floats of every gathered neighbour.
-// Dot products are accumulated in FP64 per lane, combined across row groups in a fixed
-// order, written as one partial per tile, and summed per (job, slice) by a one-block
-// reduction in tile order: every scalar is deterministic run to run.
+// Work items are (job, slice, 256-row tile).
+//  * SpMV: a 256-thread block = 4 wavefronts, each owning 64 consecutive rows.  The
+//    wavefront loads the column-major (slot-major) ELL of its 64 rows with coalesced
+//    loads, one lane per row, and then walks the rows, broadcasting each row's column
+//    ids and values to all lanes (v_readlane for 64 RHS lanes, ds_bpermute for fewer);
+//    every gathered neighbour is then one contiguous Rp*4-byte vector load.
+//  * Streaming kernels (updates, init, residual): 1024-thread blocks, lane = RHS,
+//    16 (Rp=64) row groups.
+// Dot products are accumulated in FP64 per lane, combined across row groups and
+// wavefronts in a fixed order into one partial per tile, and summed per (job, slice)
+// by a one-block reduction in tile order: every scalar is deterministic run to run.
 #include <cfloat>
 #include <cmath>
 
@@ -48,7 +53,6 @@ __global__ void __launch_bounds__(kBlock) k_dinv(const JobDev* __restrict__ jobs
                                                  const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
-    if (threadIdx.x >= kTileRows) return;
     const int row = t.z * kTileRows + threadIdx.x;
     const int s = t.y;
     float v = 0.0f;
@@ -63,20 +67,6 @@ __global__ void __launch_bounds__(kBlock) k_dinv(const JobDev* __restrict__ jobs
 // --------------------------------------------------------------------------------
 // helpers
 // --------------------------------------------------------------------------------
-struct Lanes {
-    int lane, g, G, Rp, lg;
-};
-
-__device__ __forceinline__ Lanes lanes_of(const JobDev& J) {
-    Lanes L;
-    L.lg = J.lg;
-    L.Rp = 1 << J.lg;
-    L.lane = threadIdx.x & (L.Rp - 1);
-    L.g = threadIdx.x >> J.lg;
-    L.G = kBlock >> J.lg;
-    return L;
-}
-
 __device__ __forceinline__ double* part_ptr(const JobDev& J, int ch, int s, int tile) {
     return J.part + (((size_t)ch * J.C + s) * J.ntiles + tile) * J.Rp;
 }
@@ -85,93 +75,96 @@ __device__ __forceinline__ double* sc_ptr(const JobDev& J, int f, int s) {
     return J.sc + ((size_t)f * J.C + s) * J.Rp;
 }
 
-// Combine per-thread values of NV channels across the row groups of the block in a
-// fixed order and write them as this tile's partials.
-template <int NV>
-__device__ __forceinline__ void block_partials(const JobDev& J, const Lanes& L, int s, int tile,
-                                               const int* ch, double (&v)[NV]) {
-    __shared__ double red[NV][kBlock];
+__device__ __forceinline__ bool slice_active(const JobDev& J, int s) {
+    return J.st[s * kStInts] != 0;
+}
+
+// Combine per-thread values of NV channels across the row groups of an NT-thread block
+// (thread = group * Rp + lane) in a fixed order and write them as this tile's partials.
+template <int NV, int NT>
+__device__ __forceinline__ void block_partials(const JobDev& J, int s, int tile, const int* ch,
+                                               const double (&v)[NV]) {
+    __shared__ double red[NV][NT];
+    const int Rp = J.Rp, lg = J.lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = NT >> lg;
 #pragma unroll
     for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = v[c];
     __syncthreads();
-    if (L.g == 0) {
+    if (g == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
-            double a = red[c][L.lane];
-            for (int gg = 1; gg < L.G; ++gg) a += red[c][gg * L.Rp + L.lane];
-            part_ptr(J, ch[c], s, tile)[L.lane] = a;
+            double a = red[c][lane];
+            for (int gg = 1; gg < G; ++gg) a += red[c][gg * Rp + lane];
+            part_ptr(J, ch[c], s, tile)[lane] = a;
         }
     }
 }
 
-// Sum NV partial channels over the tiles of (job, slice) in tile order; the result is
-// valid in threads with g == 0 (lane = column).
+// Sum NV partial channels over all tiles of (job, slice) in tile order with a
+// kEwBlock-thread block; each row group sums a contiguous range of tiles (loads
+// unrolled so many are in flight), then the groups are combined in order.  The result
+// is valid in threads with g == 0 (lane = column).
 template <int NV>
-__device__ __forceinline__ void reduce_tiles(const JobDev& J, const Lanes& L, int s,
-                                             const int* ch, double (&out)[NV]) {
-    __shared__ double red[NV][kBlock];
+__device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV]) {
+    __shared__ double red[NV][kEwBlock];
+    const int Rp = J.Rp, lg = J.lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
     double a[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) a[c] = 0.0;
-    // each row group sums a contiguous range of tiles
-    const int per = (J.ntiles + L.G - 1) / L.G;
-    const int t0 = L.g * per, t1 = min(J.ntiles, t0 + per);
-    for (int t = t0; t < t1; ++t) {
+    const int per = (J.ntiles + G - 1) / G;
+    const int t0 = g * per, t1 = min(J.ntiles, t0 + per);
+    int t = t0;
+    for (; t + 8 <= t1; t += 8) {
 #pragma unroll
-        for (int c = 0; c < NV; ++c) a[c] += part_ptr(J, ch[c], s, t)[L.lane];
+        for (int c = 0; c < NV; ++c) {
+            const double* pp = part_ptr(J, ch[c], s, t) + lane;
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = pp[(size_t)u * Rp];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[c] += v[u];
+        }
+    }
+    for (; t < t1; ++t) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) a[c] += part_ptr(J, ch[c], s, t)[lane];
     }
 #pragma unroll
     for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = a[c];
     __syncthreads();
-    if (L.g == 0) {
+    if (g == 0) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
-            double r = red[c][L.lane];
-            for (int gg = 1; gg < L.G; ++gg) r += red[c][gg * L.Rp + L.lane];
+            double r = red[c][lane];
+            for (int gg = 1; gg < G; ++gg) r += red[c][gg * Rp + lane];
             out[c] = r;
         }
     }
 }
 
-__device__ __forceinline__ bool slice_active(const JobDev& J, int s) {
-    return J.st[s * kStInts] != 0;
-}
-
-// y = A_s * v for one row and this lane (A_s = S + alpha_s R, FP32)
-__device__ __forceinline__ float row_spmv(const JobDev& J, float al, const float* __restrict__ V,
-                                          int row, int lane) {
-    float acc = 0.0f;
-    const int np = J.np;
-    const int Rp = J.Rp;
-    int q = 0;
-    for (; q + 4 <= J.width; q += 4) {
-        const size_t e = (size_t)q * np + row;
-        const int i0 = J.ids[e], i1 = J.ids[e + np], i2 = J.ids[e + 2 * (size_t)np], i3 = J.ids[e + 3 * (size_t)np];
-        const float v0 = J.s[e] + al * J.r[e];
-        const float v1 = J.s[e + np] + al * J.r[e + np];
-        const float v2 = J.s[e + 2 * (size_t)np] + al * J.r[e + 2 * (size_t)np];
-        const float v3 = J.s[e + 3 * (size_t)np] + al * J.r[e + 3 * (size_t)np];
-        const float x0 = V[(size_t)i0 * Rp + lane], x1 = V[(size_t)i1 * Rp + lane];
-        const float x2 = V[(size_t)i2 * Rp + lane], x3 = V[(size_t)i3 * Rp + lane];
-        acc += v0 * x0; acc += v1 * x1; acc += v2 * x2; acc += v3 * x3;
-    }
-    for (; q < J.width; ++q) {
-        const size_t e = (size_t)q * np + row;
-        acc += (J.s[e] + al * J.r[e]) * V[(size_t)J.ids[e] * Rp + lane];
-    }
-    return acc;
+// streaming-kernel lane mapping (kEwBlock threads per 256-row tile)
+struct Ew {
+    int lane, g, G;
+};
+__device__ __forceinline__ Ew ew_of(const JobDev& J) {
+    Ew e;
+    e.lane = threadIdx.x & (J.Rp - 1);
+    e.g = threadIdx.x >> J.lg;
+    e.G = kEwBlock >> J.lg;
+    return e;
 }
 
 // --------------------------------------------------------------------------------
 // PCG (ConjugateGradient::solve, src/numeric/conjugate_gradient.cpp:61-161)
 // --------------------------------------------------------------------------------
 // init: x = 0, r = b, z = D^-1 r, p = z; partials r.z and b.b
-__global__ void __launch_bounds__(kBlock) k_pcg_init(const JobDev* __restrict__ jobs,
-                                                     const int4* __restrict__ tiles) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict__ jobs,
+                                                       const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     const float* dinv = J.dinv + (size_t)s * J.np;
     const float xs = J.xscale[s];
     double rz = 0.0, bb = 0.0;
@@ -187,24 +180,23 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(const JobDev* __restrict__ 
         bb += (double)b * (double)b;
     }
     const int ch[2] = {0, 1};
-    double v[2] = {rz, bb};
-    block_partials<2>(J, L, s, t.z, ch, v);
+    const double v[2] = {rz, bb};
+    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
-                                                            const int2* __restrict__ js,
-                                                            int* __restrict__ active) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
+                                                              const int2* __restrict__ js,
+                                                              int* __restrict__ active) {
     const int2 t = js[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
     const int ch[2] = {0, 1};
     double out[2];
-    reduce_tiles<2>(J, L, s, ch, out);
-    if (L.g == 0) {
-        sc_ptr(J, SC_RHO, s)[L.lane] = out[0];
-        sc_ptr(J, SC_BB, s)[L.lane] = out[1];
-        sc_ptr(J, SC_RS, s)[L.lane] = out[0];
+    reduce_tiles<2>(J, s, ch, out);
+    if ((int)threadIdx.x < J.Rp) {
+        sc_ptr(J, SC_RHO, s)[threadIdx.x] = out[0];
+        sc_ptr(J, SC_BB, s)[threadIdx.x] = out[1];
+        sc_ptr(J, SC_RS, s)[threadIdx.x] = out[0];
     }
     if (threadIdx.x == 0) {
         J.st[s * kStInts + 0] = 1;
@@ -214,62 +206,97 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init_reduce(const JobDev* __rest
     }
 }
 
-// q = A p (+ 1 1^T p when dcFree); partials p.q and sum(p)
+// broadcast a per-row register value held by lane `src` to the lanes of a row group
+template <int LG>
+__device__ __forceinline__ int bcast_i(int v, int src) {
+    if constexpr (LG == 6) return __builtin_amdgcn_readlane(v, src);
+    else return __builtin_amdgcn_ds_bpermute(src << 2, v);
+}
+template <int LG>
+__device__ __forceinline__ float bcast_f(float v, int src) {
+    return __int_as_float(bcast_i<LG>(__float_as_int(v), src));
+}
+
+// q = A_s p (+ 1 1^T p when dcFree); partials p.q and sum(p).
+// LG = log2(Rp); WMAX = padded ELL width (compile-time, so the slot registers are
+// statically indexed).
+template <int LG, int WMAX>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
                                                      const int4* __restrict__ tiles) {
+    constexpr int Rp = 1 << LG;
+    constexpr int G = 64 / Rp;             // rows per wavefront step
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
-    const Lanes L = lanes_of(J);
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int lane = l64 & (Rp - 1), sub = l64 >> LG;
+    const int np = J.np;
+    const int row0 = t.z * kTileRows + wave * 64;
     const float al = J.alpha[s];
-    const float* P = J.p + (size_t)s * J.np * J.Rp;
-    float* Q = J.q + (size_t)s * J.np * J.Rp;
+    // coalesced ELL load: lane l64 holds row row0 + l64
+    int idr[WMAX];
+    float vr[WMAX];
+#pragma unroll
+    for (int q = 0; q < WMAX; ++q) {
+        const size_t e = (size_t)q * np + row0 + l64;
+        idr[q] = J.ids[e];
+        vr[q] = J.s[e] + al * J.r[e];
+    }
+    const float* __restrict__ P = J.p + (size_t)s * np * Rp;
+    float* __restrict__ Q = J.q + (size_t)s * np * Rp;
     double pq = 0.0, ps = 0.0;
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
-        const float acc = row_spmv(J, al, P, row, L.lane);
-        const size_t i = (size_t)row * J.Rp + L.lane;
-        Q[i] = acc;
-        const float pv = P[i];
+    for (int i = 0; i < 64 / G; ++i) {
+        const int src = i * G + sub;
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) {
+            const int id = bcast_i<LG>(idr[q], src);
+            const float v = bcast_f<LG>(vr[q], src);
+            acc += v * P[(size_t)id * Rp + lane];
+        }
+        const size_t o = (size_t)(row0 + src) * Rp + lane;
+        Q[o] = acc;
+        const float pv = P[o];
         pq += (double)pv * (double)acc;
         ps += (double)pv;
     }
     const int ch[2] = {0, 1};
-    double v[2] = {pq, ps};
-    block_partials<2>(J, L, s, t.z, ch, v);
+    const double v[2] = {pq, ps};
+    block_partials<2, kBlock>(J, s, t.z, ch, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_pcg_reduce_pq(const JobDev* __restrict__ jobs,
-                                                          const int2* __restrict__ js) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_pq(const JobDev* __restrict__ jobs,
+                                                            const int2* __restrict__ js) {
     const int2 t = js[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
-    const Lanes L = lanes_of(J);
     const int ch[2] = {0, 1};
     double out[2];
-    reduce_tiles<2>(J, L, s, ch, out);
-    if (L.g == 0) {
+    reduce_tiles<2>(J, s, ch, out);
+    if ((int)threadIdx.x < J.Rp) {
+        const int lane = threadIdx.x;
         double pq = out[0];
         const double ps = out[1];
         // with dcFree, q = A p + 1 (1^T p), hence p.q = p.Ap + (1^T p)^2
         if (s == J.dc) pq += ps * ps;
-        const double rho = sc_ptr(J, SC_RHO, s)[L.lane];
-        sc_ptr(J, SC_PQ, s)[L.lane] = pq;
-        sc_ptr(J, SC_PSUM, s)[L.lane] = (s == J.dc) ? ps : 0.0;
-        sc_ptr(J, SC_ALPHA, s)[L.lane] = pq != 0.0 ? rho / pq : 0.0;
+        const double rho = sc_ptr(J, SC_RHO, s)[lane];
+        sc_ptr(J, SC_PQ, s)[lane] = pq;
+        sc_ptr(J, SC_PSUM, s)[lane] = (s == J.dc) ? ps : 0.0;
+        sc_ptr(J, SC_ALPHA, s)[lane] = pq != 0.0 ? rho / pq : 0.0;
     }
 }
 
 // x += alpha p, r -= alpha q (a zero p.q zeroes both, updateVectorKernel
 // conjugate_gradient_kernel.cu:77-89); partials r.z and r.r with z = D^-1 r
-__global__ void __launch_bounds__(kBlock) k_pcg_update(const JobDev* __restrict__ jobs,
-                                                       const int4* __restrict__ tiles) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_update(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     const double pq = sc_ptr(J, SC_PQ, s)[L.lane];
     const float a = (float)sc_ptr(J, SC_ALPHA, s)[L.lane];
     const float dcv = (float)sc_ptr(J, SC_PSUM, s)[L.lane];
@@ -289,47 +316,48 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(const JobDev* __restrict_
         rr += (double)rv * (double)rv;
     }
     const int ch[2] = {2, 3};
-    double v[2] = {rz, rr};
-    block_partials<2>(J, L, s, t.z, ch, v);
+    const double v[2] = {rz, rr};
+    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
 
 // rho = r.z, convergence test, beta = rho / rho_old
-__global__ void __launch_bounds__(kBlock) k_pcg_reduce_rz(const JobDev* __restrict__ jobs,
-                                                          const int2* __restrict__ js,
-                                                          int* __restrict__ active) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_rz(const JobDev* __restrict__ jobs,
+                                                            const int2* __restrict__ js,
+                                                            int* __restrict__ active) {
     const int2 t = js[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
-    const Lanes L = lanes_of(J);
     const int ch[2] = {2, 3};
     double out[2];
-    reduce_tiles<2>(J, L, s, ch, out);
+    reduce_tiles<2>(J, s, ch, out);
     __shared__ int not_done;
     if (threadIdx.x == 0) not_done = 0;
     __syncthreads();
+    const int lane = threadIdx.x;
+    const bool owner = lane < J.Rp;
     double rho = 0.0;
-    if (L.g == 0) {
+    if (owner) {
         rho = out[0];
         const double rr = out[1];
-        const double bb = sc_ptr(J, SC_BB, s)[L.lane];
-        sc_ptr(J, SC_RR, s)[L.lane] = rr;
+        const double bb = sc_ptr(J, SC_BB, s)[lane];
+        sc_ptr(J, SC_RR, s)[lane] = rr;
         if (J.rule == MPF_STOP_REFERENCE) {
-            if (L.lane == 0) {
-                // sqrt(|rho| / ref) >= eps keeps iterating; NaN (ref == 0) stops
-                const double crit = sqrt(fabs(rho / bb));
+            if (lane == 0) {
+                // sqrt(rho / ref) >= eps keeps iterating; NaN (ref == 0) stops
+                const double crit = fabs(sqrt(rho / bb));
                 if (crit >= (double)FLT_EPSILON) not_done = 1;
             }
-        } else if (L.lane < J.R) {
+        } else if (lane < J.R) {
             if (bb > 0.0 && rr > J.tol * J.tol * bb) atomicOr(&not_done, 1);
         }
     }
     __syncthreads();
-    if (L.g == 0) {
-        const double rho_old = sc_ptr(J, SC_RHO, s)[L.lane];
-        sc_ptr(J, SC_BETA, s)[L.lane] = rho_old != 0.0 ? rho / rho_old : 0.0;
-        sc_ptr(J, SC_BZERO, s)[L.lane] = rho_old != 0.0 ? 0.0 : 1.0;
-        sc_ptr(J, SC_RHO, s)[L.lane] = rho;
+    if (owner) {
+        const double rho_old = sc_ptr(J, SC_RHO, s)[lane];
+        sc_ptr(J, SC_BETA, s)[lane] = rho_old != 0.0 ? rho / rho_old : 0.0;
+        sc_ptr(J, SC_BZERO, s)[lane] = rho_old != 0.0 ? 0.0 : 1.0;
+        sc_ptr(J, SC_RHO, s)[lane] = rho;
     }
     if (threadIdx.x == 0) {
         const int steps = J.st[s * kStInts + 1] + 1;
@@ -346,13 +374,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_reduce_rz(const JobDev* __restri
 }
 
 // p = z + beta p (rho_old == 0 zeroes p)
-__global__ void __launch_bounds__(kBlock) k_pcg_pupdate(const JobDev* __restrict__ jobs,
-                                                        const int4* __restrict__ tiles) {
+__global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate(const JobDev* __restrict__ jobs,
+                                                          const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     const float b = (float)sc_ptr(J, SC_BETA, s)[L.lane];
     const bool zero = sc_ptr(J, SC_BZERO, s)[L.lane] != 0.0;
     const float* dinv = J.dinv + (size_t)s * J.np;
@@ -365,40 +393,38 @@ __global__ void __launch_bounds__(kBlock) k_pcg_pupdate(const JobDev* __restrict
 // --------------------------------------------------------------------------------
 // FP64 refinement: residual r64 = beta F - (A64 + dc) x64 written as the FP32 RHS
 // --------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_colsum64(const JobDev* __restrict__ jobs,
-                                                     const int4* __restrict__ tiles) {
+__global__ void __launch_bounds__(kEwBlock) k_colsum64(const JobDev* __restrict__ jobs,
+                                                       const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     double a = 0.0;
     if (s == J.dc)
         for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G)
             a += J.x64[vec_index(J, s, row, L.lane)];
     const int ch[1] = {0};
-    double v[1] = {a};
-    block_partials<1>(J, L, s, t.z, ch, v);
+    const double v[1] = {a};
+    block_partials<1, kEwBlock>(J, s, t.z, ch, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_colsum64_reduce(const JobDev* __restrict__ jobs,
-                                                            const int2* __restrict__ js) {
+__global__ void __launch_bounds__(kEwBlock) k_colsum64_reduce(const JobDev* __restrict__ jobs,
+                                                              const int2* __restrict__ js) {
     const int2 t = js[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
     const int ch[1] = {0};
     double out[1];
-    reduce_tiles<1>(J, L, s, ch, out);
-    if (L.g == 0) sc_ptr(J, SC_X0, s)[L.lane] = (s == J.dc) ? out[0] : 0.0;
+    reduce_tiles<1>(J, s, ch, out);
+    if ((int)threadIdx.x < J.Rp) sc_ptr(J, SC_X0, s)[threadIdx.x] = (s == J.dc) ? out[0] : 0.0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_residual64(const JobDev* __restrict__ jobs,
-                                                       const int4* __restrict__ tiles,
-                                                       float* __restrict__ rhs_out_unused) {
+__global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     const double al = (double)J.alpha[s];
     const double xs = (double)J.xscale[s];
     const double dcs = sc_ptr(J, SC_X0, s)[L.lane];
@@ -422,32 +448,31 @@ __global__ void __launch_bounds__(kBlock) k_residual64(const JobDev* __restrict_
         bb += b * b;
     }
     const int ch[2] = {0, 1};
-    double v[2] = {rr, bb};
-    block_partials<2>(J, L, s, t.z, ch, v);
+    const double v[2] = {rr, bb};
+    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_residual64_reduce(const JobDev* __restrict__ jobs,
-                                                              const int2* __restrict__ js) {
+__global__ void __launch_bounds__(kEwBlock) k_residual64_reduce(const JobDev* __restrict__ jobs,
+                                                                const int2* __restrict__ js) {
     const int2 t = js[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
     const int ch[2] = {0, 1};
     double out[2];
-    reduce_tiles<2>(J, L, s, ch, out);
-    if (L.g == 0) {
-        sc_ptr(J, SC_X1, s)[L.lane] = out[0];
-        sc_ptr(J, SC_X2, s)[L.lane] = out[1];
+    reduce_tiles<2>(J, s, ch, out);
+    if ((int)threadIdx.x < J.Rp) {
+        sc_ptr(J, SC_X1, s)[threadIdx.x] = out[0];
+        sc_ptr(J, SC_X2, s)[threadIdx.x] = out[1];
     }
 }
 
 // x64 = x (first) or x64 += x
-__global__ void __launch_bounds__(kBlock) k_x64_update(const JobDev* __restrict__ jobs,
-                                                       const int4* __restrict__ tiles, int first) {
+__global__ void __launch_bounds__(kEwBlock) k_x64_update(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles, int first) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Lanes L = lanes_of(J);
+    const Ew L = ew_of(J);
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         const double v = (double)J.x[i];
@@ -458,13 +483,13 @@ __global__ void __launch_bounds__(kBlock) k_x64_update(const JobDev* __restrict_
 // --------------------------------------------------------------------------------
 // stand-alone multi-harmonic ELL SpMV (no job table)
 // --------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_spmv_plain(int np, int width, const int32_t* __restrict__ ids,
-                                                       const float* __restrict__ S, const float* __restrict__ Rv,
-                                                       const float* __restrict__ alpha,
-                                                       const float* __restrict__ X, float* __restrict__ Y,
-                                                       int lg) {
+__global__ void __launch_bounds__(kEwBlock) k_spmv_plain(int np, int width, const int32_t* __restrict__ ids,
+                                                         const float* __restrict__ S, const float* __restrict__ Rv,
+                                                         const float* __restrict__ alpha,
+                                                         const float* __restrict__ X, float* __restrict__ Y,
+                                                         int lg) {
     const int Rp = 1 << lg;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kBlock >> lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
     const int c = blockIdx.y;
     const float al = alpha[c];
     const float* V = X + (size_t)c * np * Rp;
@@ -500,34 +525,51 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
 
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
                      int njs, int* d_active, hipStream_t st) {
-    k_pcg_init<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles);
-    k_pcg_init_reduce<<<njs, kBlock, 0, st>>>(d_jobs, d_js, d_active);
+    k_pcg_init<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    k_pcg_init_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
     check_launch();
 }
 
+template <int LG>
+static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st) {
+    switch (wpad) {
+        case 8:  k_pcg_spmv<LG, 8><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 16: k_pcg_spmv<LG, 16><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 24: k_pcg_spmv<LG, 24><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 32: k_pcg_spmv<LG, 32><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 40: k_pcg_spmv<LG, 40><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 48: k_pcg_spmv<LG, 48><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 56: k_pcg_spmv<LG, 56><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 64: k_pcg_spmv<LG, 64><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        default: throw Error(MPF_EINVAL, "spmv: unsupported padded ELL width");
+    }
+}
+
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                     int njs, int* d_active, hipStream_t st) {
-    k_pcg_spmv<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles);
-    k_pcg_reduce_pq<<<njs, kBlock, 0, st>>>(d_jobs, d_js);
-    k_pcg_update<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles);
-    k_pcg_reduce_rz<<<njs, kBlock, 0, st>>>(d_jobs, d_js, d_active);
-    k_pcg_pupdate<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles);
+                     int njs, int* d_active, int lg, int wpad, hipStream_t st) {
+    if (lg == 6) spmv_dispatch<6>(wpad, d_jobs, d_tiles, ntiles, st);
+    else if (lg == 5) spmv_dispatch<5>(wpad, d_jobs, d_tiles, ntiles, st);
+    else if (lg == 4) spmv_dispatch<4>(wpad, d_jobs, d_tiles, ntiles, st);
+    else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
+    k_pcg_reduce_pq<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
+    k_pcg_update<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    k_pcg_reduce_rz<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
+    k_pcg_pupdate<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     check_launch();
 }
 
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                       int njs, int first, hipStream_t st) {
-    (void)first;
-    k_colsum64<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles);
-    k_colsum64_reduce<<<njs, kBlock, 0, st>>>(d_jobs, d_js);
-    k_residual64<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles, nullptr);
-    k_residual64_reduce<<<njs, kBlock, 0, st>>>(d_jobs, d_js);
+                       int njs, hipStream_t st) {
+    k_colsum64<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    k_colsum64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
+    k_residual64<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    k_residual64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
     check_launch();
 }
 
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,
                        hipStream_t st) {
-    k_x64_update<<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles, first);
+    k_x64_update<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, first);
     check_launch();
 }
 
@@ -535,7 +577,7 @@ void launch_spmv(int np, int width, const int32_t* ids, const float* S, const fl
                  const float* alpha, const float* X, float* Y, int Rp, hipStream_t st) {
     const int lg = Rp == 16 ? 4 : (Rp == 32 ? 5 : 6);
     dim3 grid(np / kTileRows, C);
-    k_spmv_plain<<<grid, kBlock, 0, st>>>(np, width, ids, S, R, alpha, X, Y, lg);
+    k_spmv_plain<<<grid, kEwBlock, 0, st>>>(np, width, ids, S, R, alpha, X, Y, lg);
     check_launch();
 }
 
