@@ -52,6 +52,16 @@ __host__ __device__ inline size_t vec_index(const JobDev& J, int s, int row, int
     return ((size_t)s * J.np + row) * J.Rp + lane;
 }
 
+// Pointers read from a job descriptor are generic; re-tag them as global so the
+// compiler emits global_load/store (no flat path, no LDS counter waits).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MPF_GLOBAL __attribute__((address_space(1)))
+#else
+#define MPF_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ MPF_GLOBAL T* gp(T* p) { return (MPF_GLOBAL T*)p; }
+
 // host launch wrappers (sparse.hip)
 void launch_assemble(const JobDev* d_jobs, const int4* d_items, int nitems, hipStream_t st);
 void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);

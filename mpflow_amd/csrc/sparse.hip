@@ -67,16 +67,16 @@ __global__ void __launch_bounds__(kBlock) k_dinv(const JobDev* __restrict__ jobs
 // --------------------------------------------------------------------------------
 // helpers
 // --------------------------------------------------------------------------------
-__device__ __forceinline__ double* part_ptr(const JobDev& J, int ch, int s, int tile) {
-    return J.part + (((size_t)ch * J.C + s) * J.ntiles + tile) * J.Rp;
+__device__ __forceinline__ MPF_GLOBAL double* part_ptr(const JobDev& J, int ch, int s, int tile) {
+    return gp(J.part) + (((size_t)ch * J.C + s) * J.ntiles + tile) * J.Rp;
 }
 
-__device__ __forceinline__ double* sc_ptr(const JobDev& J, int f, int s) {
-    return J.sc + ((size_t)f * J.C + s) * J.Rp;
+__device__ __forceinline__ MPF_GLOBAL double* sc_ptr(const JobDev& J, int f, int s) {
+    return gp(J.sc) + ((size_t)f * J.C + s) * J.Rp;
 }
 
 __device__ __forceinline__ bool slice_active(const JobDev& J, int s) {
-    return J.st[s * kStInts] != 0;
+    return gp(J.st)[s * kStInts] != 0;
 }
 
 // Combine per-thread values of NV channels across the row groups of an NT-thread block
@@ -118,7 +118,7 @@ __device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* 
     for (; t + 8 <= t1; t += 8) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
-            const double* pp = part_ptr(J, ch[c], s, t) + lane;
+            const MPF_GLOBAL double* pp = part_ptr(J, ch[c], s, t) + lane;
             double v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) v[u] = pp[(size_t)u * Rp];
@@ -206,6 +206,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
     }
 }
 
+// 4th dword of a raw buffer resource on gfx9-family GPUs (32-bit data, no swizzle)
+constexpr int kBufferWord3 = 0x00020000;
+
 // broadcast a per-row register value held by lane `src` to the lanes of a row group
 template <int LG>
 __device__ __forceinline__ int bcast_i(int v, int src) {
@@ -219,12 +222,19 @@ __device__ __forceinline__ float bcast_f(float v, int src) {
 
 // q = A_s p (+ 1 1^T p when dcFree); partials p.q and sum(p).
 // LG = log2(Rp); WMAX = padded ELL width (compile-time, so the slot registers are
-// statically indexed).
-template <int LG, int WMAX>
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
-                                                     const int4* __restrict__ tiles) {
+// statically indexed); U = rows gathered concurrently per wavefront.
+// Block = 16 wavefronts x 16 rows = one 256-row tile.  Lanes 0..15 of a wavefront load
+// the column-major ELL of its 16 rows (coalesced per slot); each row's ids/values are
+// then broadcast so the RHS lanes gather contiguous Rp-float rows of p.
+constexpr int kSpmvRowsPerWave = 16;
+
+template <int LG, int WMAX, int U>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
+                                                       const int4* __restrict__ tiles) {
     constexpr int Rp = 1 << LG;
-    constexpr int G = 64 / Rp;             // rows per wavefront step
+    constexpr int G = 64 / Rp;                       // rows per wavefront step
+    constexpr int STEPS = kSpmvRowsPerWave / G;      // steps per wavefront
+    static_assert(STEPS % U == 0, "unroll must divide the steps");
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
@@ -232,38 +242,62 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(const JobDev* __restrict__ 
     const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
     const int lane = l64 & (Rp - 1), sub = l64 >> LG;
     const int np = J.np;
-    const int row0 = t.z * kTileRows + wave * 64;
+    const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
     const float al = J.alpha[s];
-    // coalesced ELL load: lane l64 holds row row0 + l64
+    const MPF_GLOBAL int32_t* ids = gp(J.ids);
+    const MPF_GLOBAL float* Sv = gp((const float*)J.s);
+    const MPF_GLOBAL float* Rv = gp((const float*)J.r);
     int idr[WMAX];
     float vr[WMAX];
 #pragma unroll
-    for (int q = 0; q < WMAX; ++q) {
-        const size_t e = (size_t)q * np + row0 + l64;
-        idr[q] = J.ids[e];
-        vr[q] = J.s[e] + al * J.r[e];
-    }
-    const float* __restrict__ P = J.p + (size_t)s * np * Rp;
-    float* __restrict__ Q = J.q + (size_t)s * np * Rp;
-    double pq = 0.0, ps = 0.0;
-    for (int i = 0; i < 64 / G; ++i) {
-        const int src = i * G + sub;
-        float acc = 0.0f;
+    for (int q = 0; q < WMAX; ++q) { idr[q] = 0; vr[q] = 0.0f; }
+    if (l64 < kSpmvRowsPerWave) {
 #pragma unroll
         for (int q = 0; q < WMAX; ++q) {
-            const int id = bcast_i<LG>(idr[q], src);
-            const float v = bcast_f<LG>(vr[q], src);
-            acc += v * P[(size_t)id * Rp + lane];
+            const size_t e = (size_t)q * np + row0 + l64;
+            idr[q] = ids[e];
+            vr[q] = Sv[e] + al * Rv[e];
         }
-        const size_t o = (size_t)(row0 + src) * Rp + lane;
-        Q[o] = acc;
-        const float pv = P[o];
-        pq += (double)pv * (double)acc;
-        ps += (double)pv;
+    }
+    const MPF_GLOBAL float* P = gp((const float*)J.p) + (size_t)s * np * Rp;
+    MPF_GLOBAL float* Q = gp(J.q) + (size_t)s * np * Rp;
+    // gathers go through a buffer resource: 32-bit offsets, reads past the slice return 0
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(J.p + (size_t)s * np * Rp), 0, (int)((size_t)np * Rp * sizeof(float)), kBufferWord3);
+    double pq = 0.0, ps = 0.0;
+    for (int i = 0; i < STEPS; i += U) {
+        int src[U];
+        float own[U], acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            src[u] = (i + u) * G + sub;
+            own[u] = P[(size_t)(row0 + src[u]) * Rp + lane];
+            acc[u] = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int id = bcast_i<LG>(idr[q], src[u]);
+                const float v = bcast_f<LG>(vr[q], src[u]);
+                float x;
+                if constexpr (LG == 6)   // id is wave-uniform: scalar offset, shared lane offset
+                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, id * (Rp * 4), 0));
+                else
+                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (id * Rp + lane) * 4, 0, 0));
+                acc[u] += v * x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            Q[(size_t)(row0 + src[u]) * Rp + lane] = acc[u];
+            pq += (double)own[u] * (double)acc[u];
+            ps += (double)own[u];
+        }
     }
     const int ch[2] = {0, 1};
     const double v[2] = {pq, ps};
-    block_partials<2, kBlock>(J, s, t.z, ch, v);
+    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_pq(const JobDev* __restrict__ jobs,
@@ -301,16 +335,20 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update(const JobDev* __restric
     const float a = (float)sc_ptr(J, SC_ALPHA, s)[L.lane];
     const float dcv = (float)sc_ptr(J, SC_PSUM, s)[L.lane];
     const bool zero = (pq == 0.0);
-    const float* dinv = J.dinv + (size_t)s * J.np;
+    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
+    const MPF_GLOBAL float* P = gp((const float*)J.p);
+    const MPF_GLOBAL float* Q = gp((const float*)J.q);
+    MPF_GLOBAL float* X = gp(J.x);
+    MPF_GLOBAL float* Rr = gp(J.rv);
     double rz = 0.0, rr = 0.0;
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
-        const float pv = J.p[i], qv = J.q[i] + (row < J.n ? dcv : 0.0f);
-        float xv = J.x[i], rv = J.rv[i];
+        const float pv = P[i], qv = Q[i] + (row < J.n ? dcv : 0.0f);
+        float xv = X[i], rv = Rr[i];
         if (zero) { xv = 0.0f; rv = 0.0f; }
         else { xv = xv + a * pv; rv = rv - a * qv; }
-        J.x[i] = xv;
-        J.rv[i] = rv;
+        X[i] = xv;
+        Rr[i] = rv;
         const float z = dinv[row] * rv;
         rz += (double)rv * (double)z;
         rr += (double)rv * (double)rv;
@@ -383,10 +421,12 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate(const JobDev* __restri
     const Ew L = ew_of(J);
     const float b = (float)sc_ptr(J, SC_BETA, s)[L.lane];
     const bool zero = sc_ptr(J, SC_BZERO, s)[L.lane] != 0.0;
-    const float* dinv = J.dinv + (size_t)s * J.np;
+    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
+    const MPF_GLOBAL float* Rr = gp((const float*)J.rv);
+    MPF_GLOBAL float* P = gp(J.p);
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
-        J.p[i] = zero ? 0.0f : dinv[row] * J.rv[i] + b * J.p[i];
+        P[i] = zero ? 0.0f : dinv[row] * Rr[i] + b * P[i];
     }
 }
 
@@ -532,15 +572,17 @@ void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles, cons
 
 template <int LG>
 static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st) {
+    constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
+    constexpr int U2 = STEPS >= 2 ? 2 : 1;
     switch (wpad) {
-        case 8:  k_pcg_spmv<LG, 8><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 16: k_pcg_spmv<LG, 16><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 24: k_pcg_spmv<LG, 24><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 32: k_pcg_spmv<LG, 32><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 40: k_pcg_spmv<LG, 40><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 48: k_pcg_spmv<LG, 48><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 56: k_pcg_spmv<LG, 56><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 64: k_pcg_spmv<LG, 64><<<ntiles, kBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 24: k_pcg_spmv<LG, 24, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 32: k_pcg_spmv<LG, 32, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 40: k_pcg_spmv<LG, 40, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 48: k_pcg_spmv<LG, 48, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 56: k_pcg_spmv<LG, 56, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 64: k_pcg_spmv<LG, 64, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
         default: throw Error(MPF_EINVAL, "spmv: unsupported padded ELL width");
     }
 }
