@@ -209,6 +209,16 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
 // 4th dword of a raw buffer resource on gfx9-family GPUs (32-bit data, no swizzle)
 constexpr int kBufferWord3 = 0x00020000;
 
+// Buffer resource built from wave-uniform values (readfirstlane keeps it in SGPRs, so
+// buffer loads need no per-lane waterfall).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void* base, size_t bytes) {
+    const uint64_t b = (uint64_t)base;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    const int n = __builtin_amdgcn_readfirstlane((int)(bytes > 0x7fffffffu ? 0x7fffffffu : bytes));
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0, n, kBufferWord3);
+}
+
 // broadcast a per-row register value held by lane `src` to the lanes of a row group
 template <int LG>
 __device__ __forceinline__ int bcast_i(int v, int src) {
@@ -262,8 +272,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
     const MPF_GLOBAL float* P = gp((const float*)J.p) + (size_t)s * np * Rp;
     MPF_GLOBAL float* Q = gp(J.q) + (size_t)s * np * Rp;
     // gathers go through a buffer resource: 32-bit offsets, reads past the slice return 0
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(J.p + (size_t)s * np * Rp), 0, (int)((size_t)np * Rp * sizeof(float)), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + (size_t)s * np * Rp, (size_t)np * Rp * sizeof(float));
     double pq = 0.0, ps = 0.0;
     for (int i = 0; i < STEPS; i += U) {
         int src[U];
