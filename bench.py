@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the EIT reconstruction hot path on one MI355X.
+
+Metric: frames/s of 32-electrode, 40k-element (P2, 81k nodes), 5-harmonic 2.5D
+Gauss-Newton reconstruction.  One "step" = one frame: 3 Gauss-Newton steps
+(Solver::solveAbsolute, src/solver/solver.cpp:195-225) from gamma = 1 for one seeded
+ellipse phantom with 60 dB noise, every forward solve in FP32 PCG with FP64 residual
+refinement, the Tikhonov step via the BF16-MFMA Gram of the Jacobian and FP64
+Cholesky.  Measured voltages of each frame are produced by the device forward solver
+before the timed region (inputs resident in HBM when timing starts).
+
+Prints ONE JSON line (rank 0):
+  value        frames/s of the whole job (1 GPU)
+  roofline     dominant kernel (ELL SpMV of the PCG): algorithmic bytes / event-timed
+               launch time vs 8 TB/s HBM3E
+  cpu_baseline the same ELL Jacobi-PCG (FP32 vectors, FP64 scalars, OpenMP over rows)
+               on the host cores, on a bounded sample of harmonic-0 iterations, scaled
+               to frames/s with the device run's iteration counts
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "frames/sec, 32-elec 40k-elem 2.5D GN; err vs FP64; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # rings, electrodes, width, harmonics, height, port height, P2
+    "c3": dict(rings=82, L=32, width=0.05, C=5, height=0.4, port_height=0.1, p2=True,
+               desc="2.5D disc h=0.4, P2 from 82 rings (40344 elements, 81181 nodes), "
+                    "32 electrodes x 0.05 rad, 5 harmonics, adjacent drive, 1-3 ellipses, 60 dB"),
+    "c2": dict(rings=40, L=32, width=0.05, C=5, height=0.4, port_height=0.1, p2=False,
+               desc="2.5D disc h=0.4, P1 40 rings (9600 elements), 32 electrodes, 5 harmonics"),
+    "c1": dict(rings=16, L=16, width=0.1, C=1, height=1.0, port_height=1.0, p2=False,
+               desc="2D disc, P1 16 rings (1536 elements), 16 electrodes"),
+}
+CONFIG_INDEX = {"c1": 1, "c2": 2, "c3": 3}
+
+
+def build_problem(w):
+    import numpy as np
+    from mpflow_amd.mesh import boundary_edges, promote_p2, ring_mesh
+    from mpflow_amd.eit import circular_ports
+    nodes, el = ring_mesh(w["rings"], w["L"], w["width"])
+    if w["p2"]:
+        nodes, el = promote_p2(nodes, el)
+    bd = boundary_edges(el)
+    ports = circular_ports(nodes, el, bd, w["L"], w["width"])
+    L = w["L"]
+    pat = np.zeros((L, L))
+    for d in range(L):
+        pat[d, d] = 1.0
+        pat[(d + 1) % L, d] = -1.0
+    return nodes, el, bd, ports, pat
+
+
+def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=10.0):
+    """Time the oracle's C++ ELL-PCG on harmonic 0 of the workload mesh at gamma = 1."""
+    import numpy as np
+    from oracle import baseline
+    from oracle.eit_oracle import EITOracle
+    o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
+    A = o.system(np.ones(el.shape[0]), 0)
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=5)
+    per = t0 / max(s0, 1)
+    n_it = int(max(10, min(5000, budget_s / max(per, 1e-6))))
+    _, steps, t = baseline.pcg(A, F, 1, tol=0.0, max_iter=n_it)
+    per_iter = t / steps
+    return {
+        "value": 1.0 / (per_iter * slice_iters_per_frame),
+        "unit": "frames/s",
+        "cores": baseline.threads(),
+        "kind": "port",
+        "sample": ("%d Jacobi-PCG iterations of harmonic 0 (N=%d, %d RHS, FP32 vectors, FP64 scalars) "
+                   "on the workload mesh, %.3f s; frames/s = 1 / (s per iteration x %d slice-iterations "
+                   "per frame measured on the device); Jacobian and dense GN work not included"
+                   % (steps, A.shape[0], F.shape[1], t, slice_iters_per_frame)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed frames")
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--gn-steps", type=int, default=3)
+    ap.add_argument("--lam", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from mpflow_amd import capi
+    if not os.path.exists(capi.LIB_PATH):
+        from mpflow_amd import build as b
+        b.build(verbose=False)
+    from mpflow_amd.eit import EIT, SolveOptions, Solver
+    from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
+
+    w = WORKLOADS[args.workload]
+    cfg = CONFIG_INDEX[args.workload]
+    nodes, el, bd, ports, pat = build_problem(w)
+    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+              jacobian_exact=True)
+    dev = torch.device("cuda", 0)
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=1, refine_tol=1e-2)
+
+    # measured voltages (untimed): device forward solve of each phantom + 60 dB noise
+    nframes = args.warmup + args.steps
+    meas = []
+    for f in range(nframes):
+        g = ellipse_phantom(nodes, el, frame_seed(cfg, f))
+        eit.solve(torch.tensor(g, dtype=torch.float32, device=dev), opts)
+        U = eit.result64[0].cpu().numpy()
+        U = add_noise(U, 60.0, frame_seed(cfg, f))
+        meas.append(torch.tensor(U, dtype=torch.float64, device=dev))
+    solver = Solver(eit, args.lam, gn_refine=2, options=opts)
+
+    def frame(f):
+        solver.material_distribution.fill_(1.0)
+        solver.measurement[0].copy_(meas[f])
+        steps = 0
+        for _ in range(args.gn_steps):
+            solver.solve_absolute()
+            steps += solver.last_stats.steps
+        return steps
+
+    for f in range(args.warmup):
+        frame(f)
+    torch.cuda.synchronize()
+    L = capi.lib()
+    capi.check(L.mpf_ctx_profile(eit.ctx.h, 1))
+    slice_iters = 0
+    t0 = time.perf_counter()
+    for f in range(args.warmup, nframes):
+        slice_iters += frame(f)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = (ctypes_array := (__import__("ctypes").c_double * 3)())
+    capi.check(L.mpf_ctx_profile_read(eit.ctx.h, prof))
+    capi.check(L.mpf_ctx_profile(eit.ctx.h, 0))
+    spmv_ms, spmv_launches, spmv_bytes = prof[0], int(prof[1]), prof[2]
+
+    fps = args.steps / elapsed
+    avg_launch_us = 1e3 * spmv_ms / max(spmv_launches, 1)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_pcg_spmv (ELL SpMV, %d RHS, FP32)" % eit.R,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(avg_launch_us, 2),
+        "launches": spmv_launches,
+        "algorithmic_bytes_per_harmonic_launch": 8 * eit.nnz + 8 * eit.N * eit.R,
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 4),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "config %d (%s): %s" % (cfg, args.workload, w["desc"]),
+            "gn_steps_per_frame": args.gn_steps,
+            "lambda": args.lam,
+            "forward": "FP32 Jacobi-PCG to ||r||/||b|| <= %g + 1 FP64 residual refinement" % args.tol,
+            "nodes": eit.N, "elements": eit.E, "nnz": eit.nnz, "harmonics": eit.C,
+            "rhs": eit.R, "jacobian_rows": eit.m,
+            "slice_iterations_per_frame": slice_iters // args.steps,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / args.steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -68,6 +68,15 @@ int mpf_ctx_synchronize(mpf_ctx* ctx);
 /* hipStream_t of the context, as void*. */
 void* mpf_ctx_stream(mpf_ctx* ctx);
 
+/* Live timing of the dominant kernel (the ELL SpMV of the Krylov loop) with HIP events
+ * recorded on the context's stream around every launch.  Enabling resets the counters.
+ * No counterpart in the reference, which has no instrumentation; the timed launch is
+ * the A*p product of the loop in src/numeric/conjugate_gradient.cpp:111-158.
+ * out[0] = summed launch time in ms, out[1] = launches, out[2] = algorithmic bytes
+ * (8 nnz + 8 N R per harmonic per launch). */
+int mpf_ctx_profile(mpf_ctx* ctx, int32_t enable);
+int mpf_ctx_profile_read(mpf_ctx* ctx, double out[3]);
+
 /* ---------------------------------------------------------------------------
  * Host helpers (no device work)
  * ------------------------------------------------------------------------- */

@@ -16,7 +16,7 @@ EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",
     "mpf_ctx_stream", "mpf_solve_opts_default", "mpf_ports_circular", "mpf_model_create",
     "mpf_model_destroy", "mpf_model_dims", "mpf_eit_solve", "mpf_gn_step", "mpf_ell_spmv",
-    "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve",
+    "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read",
 ]
 
 
@@ -83,10 +83,8 @@ def lib() -> C.CDLL:
     L.mpf_gram_bf16.argtypes = [vp, i32, i32, i32, vp, i32, vp]
     L.mpf_gemv_t.argtypes = [vp, i32, i32, vp, i32, vp, vp]
     L.mpf_spd_solve.argtypes = [vp, i32, vp, dbl, vp, vp]
-    for name in EXPORTED:
-        f = getattr(L, name)
-        if f.restype is C.c_int or f.restype is None and name not in ("mpf_solve_opts_default",):
-            pass
+    L.mpf_ctx_profile.argtypes = [vp, i32]
+    L.mpf_ctx_profile_read.argtypes = [vp, C.POINTER(dbl)]
     _lib = L
     return L
 

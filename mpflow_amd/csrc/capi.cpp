@@ -65,10 +65,12 @@ int mpf_ctx_create(int device, void* hip_stream, mpf_ctx** out) {
 int mpf_ctx_destroy(mpf_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return MPF_OK;
-        if (ctx->own_stream && ctx->stream) {
-            (void)hipStreamSynchronize(ctx->stream);
-            (void)hipStreamDestroy(ctx->stream);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (auto& e : ctx->ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
         }
+        if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return MPF_OK;
     });
@@ -83,6 +85,28 @@ int mpf_ctx_synchronize(mpf_ctx* ctx) {
 }
 
 void* mpf_ctx_stream(mpf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mpf_ctx_profile(mpf_ctx* ctx, int32_t enable) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx, "mpf_ctx_profile: ctx == nullptr");
+        ctx->prof = enable != 0;
+        ctx->prof_ms = 0.0;
+        ctx->prof_launches = 0;
+        ctx->prof_bytes = 0.0;
+        ctx->ev_used = 0;
+        return MPF_OK;
+    });
+}
+
+int mpf_ctx_profile_read(mpf_ctx* ctx, double out[3]) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && out, "mpf_ctx_profile_read: null argument");
+        out[0] = ctx->prof_ms;
+        out[1] = (double)ctx->prof_launches;
+        out[2] = ctx->prof_bytes;
+        return MPF_OK;
+    });
+}
 
 void mpf_solve_opts_default(mpf_solve_opts* o) {
     if (!o) return;

@@ -67,8 +67,10 @@ void launch_assemble(const JobDev* d_jobs, const int4* d_items, int nitems, hipS
 void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                      const int2* d_js, int njs, int* d_active, hipStream_t st);
+// ev0/ev1 (nullable) are recorded right before and after the SpMV launch
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                     const int2* d_js, int njs, int* d_active, int lg, int wpad, hipStream_t st);
+                     const int2* d_js, int njs, int* d_active, int lg, int wpad, hipStream_t st,
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                        const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,

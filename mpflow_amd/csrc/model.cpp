@@ -10,6 +10,26 @@
 
 #include "model.h"
 
+std::pair<hipEvent_t, hipEvent_t> mpf_ctx::next_events() {
+    if (ev_used == ev.size()) {
+        hipEvent_t a, b;
+        MPF_HIP(hipEventCreate(&a));
+        MPF_HIP(hipEventCreate(&b));
+        ev.emplace_back(a, b);
+    }
+    return ev[ev_used++];
+}
+
+void mpf_ctx::collect_events() {
+    for (size_t i = 0; i < ev_used; ++i) {
+        float ms = 0.0f;
+        MPF_HIP(hipEventElapsedTime(&ms, ev[i].first, ev[i].second));
+        prof_ms += ms;
+        ++prof_launches;
+    }
+    ev_used = 0;
+}
+
 namespace mpf {
 
 Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
@@ -139,12 +159,17 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     int cur_nt = ntiles, cur_njs = njs;
     int it = 0;
     while (true) {
-        for (int i = 0; i < every; ++i)
-            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, cur_js, cur_njs, active.p, lg, wpad, st0);
+        for (int i = 0; i < every; ++i) {
+            std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+            if (ctx->prof) ev = ctx->next_events();
+            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, cur_js, cur_njs, active.p, lg, wpad, st0,
+                            ev.first, ev.second);
+        }
         it += every;
         MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
         MPF_HIP(hipMemcpyAsync(h_st, st.p, nst * sizeof(int), hipMemcpyDeviceToHost, st0));
         MPF_HIP(hipStreamSynchronize(st0));
+        if (ctx->prof) ctx->collect_events();
         if (*h_active <= 0 || it >= maxit + every) break;
         int nt = 0, nj = 0;
         for (int f = 0; f < frames; ++f)
@@ -163,6 +188,13 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     }
     // step counts
     const int* hst = h_st;
+    if (ctx->prof) {
+        // algorithmic bytes of one harmonic's SpMV with R right-hand sides: A_c values and
+        // column ids once (8 nnz), p read and q written once (8 N R)
+        int64_t slice_iters = 0;
+        for (size_t i = 0; i < (size_t)S.C * frames; ++i) slice_iters += hst[i * kStInts + 1];
+        ctx->prof_bytes += (double)slice_iters * (8.0 * S.nnz + 8.0 * S.N * S.R);
+    }
     for (int f = 0; f < frames; ++f)
         for (int c = 0; c < S.C; ++c) {
             const int* sp = &hst[((size_t)f * S.C + c) * kStInts];

@@ -10,6 +10,15 @@ struct mpf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // live timing of the SpMV launches with HIP events on `stream`
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    double prof_bytes = 0.0;       // algorithmic bytes of the timed launches
+    std::pair<hipEvent_t, hipEvent_t> next_events();
+    void collect_events();
 };
 
 namespace mpf {

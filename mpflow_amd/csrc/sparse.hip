@@ -597,11 +597,14 @@ static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, i
 }
 
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                     int njs, int* d_active, int lg, int wpad, hipStream_t st) {
+                     int njs, int* d_active, int lg, int wpad, hipStream_t st, hipEvent_t ev0,
+                     hipEvent_t ev1) {
+    if (ev0) MPF_HIP(hipEventRecord(ev0, st));
     if (lg == 6) spmv_dispatch<6>(wpad, d_jobs, d_tiles, ntiles, st);
     else if (lg == 5) spmv_dispatch<5>(wpad, d_jobs, d_tiles, ntiles, st);
     else if (lg == 4) spmv_dispatch<4>(wpad, d_jobs, d_tiles, ntiles, st);
     else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
+    if (ev1) MPF_HIP(hipEventRecord(ev1, st));
     k_pcg_reduce_pq<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
     k_pcg_update<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     k_pcg_reduce_rz<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);

@@ -1,0 +1,62 @@
+"""CPU baseline leg: the same ELL Jacobi-PCG (FP32 vectors, FP64 scalars) timed on the host.
+
+TEST INFRASTRUCTURE ONLY (bench.py's cpu_baseline leg and tests/).  The ELL matrix is
+built here from the oracle's own FP64 assembly (eit_oracle.assemble), not from the
+device path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from . import build_baseline
+
+
+def _lib():
+    if not os.path.exists(build_baseline.LIB):
+        build_baseline.build()
+    L = C.CDLL(build_baseline.LIB)
+    L.cgh_pcg.restype = C.c_int
+    L.cgh_pcg.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                          C.c_int, C.c_double, C.c_int]
+    L.cgh_threads.restype = C.c_int
+    L.cgh_set_threads.argtypes = [C.c_int]
+    return L
+
+
+def csr_to_ell(A):
+    """Row-major ELL (ids, vals) of a scipy CSR matrix; padding slots id = row, 0."""
+    A = A.tocsr()
+    n = A.shape[0]
+    nnz_row = np.diff(A.indptr)
+    width = int(nnz_row.max())
+    ids = np.repeat(np.arange(n, dtype=np.int32)[:, None], width, axis=1)
+    vals = np.zeros((n, width), np.float32)
+    for i in range(n):
+        s, e = A.indptr[i], A.indptr[i + 1]
+        ids[i, :e - s] = A.indices[s:e]
+        vals[i, :e - s] = A.data[s:e]
+    return np.ascontiguousarray(ids), np.ascontiguousarray(vals)
+
+
+def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0):
+    """Solve (A + dc 1 1^T) X = B with the C++ baseline; returns (X, steps, seconds)."""
+    L = _lib()
+    if threads:
+        L.cgh_set_threads(int(threads))
+    ids, vals = csr_to_ell(A)
+    n, width = ids.shape
+    B = np.ascontiguousarray(B, np.float32)
+    R = B.shape[1]
+    X = np.zeros_like(B)
+    t0 = time.perf_counter()
+    steps = L.cgh_pcg(n, width, ids.ctypes.data, vals.ctypes.data, R, B.ctypes.data, X.ctypes.data,
+                      int(dc), float(tol), int(max_iter))
+    return X, steps, time.perf_counter() - t0
+
+
+def threads():
+    return _lib().cgh_threads()
