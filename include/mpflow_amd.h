@@ -92,6 +92,12 @@ int mpf_ports_circular(int32_t n_nodes, const double* nodes, int32_t n_elements,
                        int32_t count, double width, double offset, int32_t clockwise,
                        int32_t* out, int32_t out_capacity, int32_t* max_edges_out);
 
+/* basis::Linear / basis::Quadratic integralA and integralB over one straight triangle
+ * (src/fem/basis_linear.cpp:72-91, src/fem/basis_quadratic.cpp:75-97): K[i*k+j] =
+ * int grad(phi_i).grad(phi_j), M[i*k+j] = int phi_i phi_j for k = 3 (P1) or 6 (P2,
+ * nodes [v0 v1 v2 m01 m12 m20]).  xy holds k (x, y) pairs.  Host only. */
+int mpf_element_integrals(int32_t nodes_per_element, const double* xy, double* K, double* M);
+
 /* ---------------------------------------------------------------------------
  * EIT model: mesh + ports + sources + 2.5D set-up (models::EIT::EIT,
  * src/models/eit.cpp:29-95; FEM::Equation::Equation, src/fem/equation.cpp:29-68).

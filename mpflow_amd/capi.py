@@ -17,6 +17,7 @@ EXPORTED = [
     "mpf_ctx_stream", "mpf_solve_opts_default", "mpf_ports_circular", "mpf_model_create",
     "mpf_model_destroy", "mpf_model_dims", "mpf_eit_solve", "mpf_gn_step", "mpf_ell_spmv",
     "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read",
+    "mpf_element_integrals",
 ]
 
 
@@ -85,6 +86,7 @@ def lib() -> C.CDLL:
     L.mpf_spd_solve.argtypes = [vp, i32, vp, dbl, vp, vp]
     L.mpf_ctx_profile.argtypes = [vp, i32]
     L.mpf_ctx_profile_read.argtypes = [vp, C.POINTER(dbl)]
+    L.mpf_element_integrals.argtypes = [i32, vp, vp, vp]
     _lib = L
     return L
 

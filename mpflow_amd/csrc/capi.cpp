@@ -142,6 +142,15 @@ int mpf_ports_circular(int32_t n_nodes, const double* nodes, int32_t n_elements,
     });
 }
 
+int mpf_element_integrals(int32_t nodes_per_element, const double* xy, double* K, double* M) {
+    return guarded([&] {
+        MPF_REQUIRE(xy && K && M, "mpf_element_integrals: null argument");
+        MPF_REQUIRE(nodes_per_element == 3 || nodes_per_element == 6, "mpf_element_integrals: k must be 3 or 6");
+        mpf::element_integrals(nodes_per_element, xy, K, M);
+        return MPF_OK;
+    });
+}
+
 int mpf_model_create(mpf_ctx* ctx, const mpf_model_desc* desc, mpf_model** out) {
     return guarded([&] {
         MPF_REQUIRE(ctx && desc && out, "mpf_model_create: null argument");

@@ -1,0 +1,357 @@
+"""CPU tests that pin the FP64 oracle (oracle/eit_oracle.py) to closed forms,
+invariants, brute force, finite differences and the analytic disc solution, so that a
+plausible restatement mistake (dropped term, wrong sign, transposed operand) fails one
+of them.  The reference ships no golden vectors or tests; these checks are the pin."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from mpflow_amd.mesh import boundary_edges, promote_p2, ring_mesh
+from mpflow_amd.phantom import SplitMix64, ellipse_phantom, frame_seed, inclusion_phantom
+from oracle import eit_oracle as eo
+from tests.problems import make_problem, oracle_of
+
+
+# ----------------------------------------------------------------------------------
+# element integrals
+# ----------------------------------------------------------------------------------
+def _rand_tri(rng):
+    while True:
+        p = rng.uniform(-1, 1, (3, 2))
+        a = 0.5 * ((p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[1, 1] - p[0, 1]) * (p[2, 0] - p[0, 0]))
+        if a > 0.05:
+            return p, a
+
+
+def test_p1_integrals_closed_form():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        p, area = _rand_tri(rng)
+        K, M = eo.element_integrals(p)
+        # grad(lambda_i) = (y_j - y_k, x_k - x_j) / (2 area)
+        g = np.array([[p[(i + 1) % 3, 1] - p[(i + 2) % 3, 1], p[(i + 2) % 3, 0] - p[(i + 1) % 3, 0]]
+                      for i in range(3)]) / (2 * area)
+        np.testing.assert_allclose(K, area * g @ g.T, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(M, area / 12.0 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-12)
+
+
+def _p2_nodes(p):
+    return np.vstack([p, 0.5 * (p[0] + p[1]), 0.5 * (p[1] + p[2]), 0.5 * (p[2] + p[0])])
+
+
+def test_p2_integrals_textbook():
+    """P2 mass matrix A/180 * [[6,-1,-1,0,-4,0], ...] (vertex-midpoint ordering
+    v0 v1 v2 m01 m12 m20); stiffness rows sum to zero and reproduce quadratics."""
+    Mref = np.array([[6, -1, -1, 0, -4, 0], [-1, 6, -1, 0, 0, -4], [-1, -1, 6, -4, 0, 0],
+                     [0, 0, -4, 32, 16, 16], [-4, 0, 0, 16, 32, 16], [0, -4, 0, 16, 16, 32]], float) / 180.0
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        p, area = _rand_tri(rng)
+        xy = _p2_nodes(p)
+        K, M = eo.element_integrals(xy)
+        np.testing.assert_allclose(M, area * Mref, rtol=1e-10, atol=1e-13)
+        np.testing.assert_allclose(K.sum(axis=1), 0.0, atol=1e-11)
+        np.testing.assert_allclose(K, K.T, atol=1e-12)
+        # energy of u = x^2 + xy: int |grad u|^2 = int (2x+y)^2 + x^2 (exact on P2)
+        u = xy[:, 0] ** 2 + xy[:, 0] * xy[:, 1]
+        lam = eo._Q4[:, :3]
+        X = lam @ p[:, 0]
+        Y = lam @ p[:, 1]
+        exact = area * np.sum(eo._Q4[:, 3] * ((2 * X + Y) ** 2 + X ** 2))
+        assert abs(u @ K @ u - exact) < 1e-10 * max(1.0, abs(exact))
+
+
+def test_product_element_integrals_match_oracle():
+    """The product's host set-up (barycentric closed forms) equals the oracle's
+    Vandermonde + quadrature restatement, P1 and P2."""
+    import ctypes as C
+    from mpflow_amd.capi import check, lib
+    rng = np.random.default_rng(2)
+    for k in (3, 6):
+        for _ in range(10):
+            p, _ = _rand_tri(rng)
+            xy = np.ascontiguousarray(p if k == 3 else _p2_nodes(p))
+            K = np.zeros((k, k)); M = np.zeros((k, k))
+            check(lib().mpf_element_integrals(k, xy.ctypes.data, K.ctypes.data, M.ctypes.data))
+            Ko, Mo = eo.element_integrals(xy)
+            np.testing.assert_allclose(K, Ko, rtol=1e-10, atol=1e-12)
+            np.testing.assert_allclose(M, Mo, rtol=1e-10, atol=1e-14)
+
+
+def test_boundary_integrals():
+    L = 0.37
+    s1 = np.array([0.0, L])
+    assert eo.boundary_integral(s1, 0) == pytest.approx(L / 2)
+    assert eo.boundary_integral(s1, 1) == pytest.approx(L / 2)
+    s2 = np.array([0.0, L / 2, L])
+    assert eo.boundary_integral(s2, 0, quirks=False) == pytest.approx(L / 6)
+    assert eo.boundary_integral(s2, 1, quirks=False) == pytest.approx(2 * L / 3)
+    assert eo.boundary_integral(s2, 2, quirks=False) == pytest.approx(L / 6)
+    # reference code evaluates the last P2 edge node with node 1's coefficients
+    assert eo.boundary_integral(s2, 2, quirks=True) == pytest.approx(2 * L / 3)
+
+
+# ----------------------------------------------------------------------------------
+# mesh, ports, phantoms
+# ----------------------------------------------------------------------------------
+@pytest.mark.parametrize("nr", [2, 5, 16, 40])
+def test_ring_mesh_counts(nr):
+    n, e = ring_mesh(nr)
+    assert n.shape[0] == 1 + 3 * nr * (nr + 1)
+    assert e.shape[0] == 6 * nr * nr
+    b = boundary_edges(e)
+    assert b.shape[0] == 6 * nr
+    a, b = n[e[:, 1]] - n[e[:, 0]], n[e[:, 2]] - n[e[:, 0]]
+    area = 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]).sum()
+    # inscribed polygon of 6 nr boundary nodes
+    assert area == pytest.approx(0.5 * 6 * nr * math.sin(2 * math.pi / (6 * nr)), rel=1e-12)
+
+
+def test_p2_promotion():
+    n, e = ring_mesh(4)
+    n2, e2 = promote_p2(n, e)
+    V, E = n.shape[0], e.shape[0]
+    edges = V + E - 1
+    assert n2.shape[0] == V + edges
+    for t, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
+        np.testing.assert_allclose(n2[e2[:, 3 + t]], 0.5 * (n2[e2[:, a]] + n2[e2[:, b]]))
+    bd = boundary_edges(e2)
+    assert bd.shape == (6 * 4, 3)
+    np.testing.assert_allclose(n2[bd[:, 1]], 0.5 * (n2[bd[:, 0]] + n2[bd[:, 2]]))
+
+
+def test_electrode_corners_on_nodes():
+    L, w = 16, 0.1
+    n, e = ring_mesh(16, L, w)
+    ang = np.mod(np.arctan2(n[:, 1], n[:, 0]), 2 * math.pi)
+    r = np.hypot(n[:, 0], n[:, 1])
+    bang = ang[np.abs(r - 1) < 1e-12]
+    for l in range(L):
+        for c in (2 * math.pi * l / L, 2 * math.pi * l / L + w):
+            assert np.min(np.abs(np.angle(np.exp(1j * (bang - c))))) < 1e-12
+
+
+def test_ports_product_matches_oracle():
+    from mpflow_amd.eit import circular_ports
+    for p2 in (False, True):
+        n, e = ring_mesh(16, 16, 0.1)
+        if p2:
+            n, e = promote_p2(n, e)
+        bd = boundary_edges(e)
+        a = eo.circular_boundary_ports(n, e, bd, 16, 0.1)
+        b = circular_ports(n, e, bd, 16, 0.1)
+        np.testing.assert_array_equal(a, b)
+        # every port is covered by edges whose endpoints lie inside its arc
+        assert np.all((a >= 0).sum(axis=1) >= 1)
+
+
+def test_splitmix64_known_answer():
+    # reference outputs of SplitMix64 seeded with 1234567
+    g = SplitMix64(1234567)
+    assert [g.next_u64() for _ in range(5)] == [
+        6457827717110365317, 3203168211198807973, 9817491932198370423,
+        4593380528125082431, 16408922859458223821]
+
+
+def test_phantoms_deterministic():
+    n, e = ring_mesh(16, 16, 0.1)
+    a = ellipse_phantom(n, e, frame_seed(2, 7))
+    b = ellipse_phantom(n, e, frame_seed(2, 7))
+    np.testing.assert_array_equal(a, b)
+    assert a.min() >= 0.2 - 1e-12 and a.max() <= 5.0 + 1e-12
+    g = inclusion_phantom(n, e, frame_seed(1, 0))
+    assert set(np.unique(g)) <= {1.0, 2.0} and np.any(g == 2.0)
+
+
+# ----------------------------------------------------------------------------------
+# assembly, excitation, solver
+# ----------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def small():
+    p = make_problem(8, 8, 0.2)
+    o = oracle_of(p)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 2))
+    return p, o, g
+
+
+def test_assembly_invariants(small):
+    p, o, g = small
+    S = eo.assemble(o.elements, o.Ke, o.N, g)
+    R = eo.assemble(o.elements, o.Me, o.N, g)
+    assert abs(S - S.T).max() < 1e-13
+    np.testing.assert_allclose(S @ np.ones(o.N), 0.0, atol=1e-12)
+    tri = p["nodes"][p["elements"]]
+    a, b = tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]
+    area = 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
+    assert np.ones(o.N) @ R @ np.ones(o.N) == pytest.approx(np.sum(g * area), rel=1e-12)
+
+
+def test_excitation_columns(small):
+    p, o, g = small
+    np.testing.assert_allclose(o.B.sum(axis=0), 1.0, rtol=1e-12)
+    # support of each column = the nodes of the port's edges
+    for q in range(o.B.shape[1]):
+        nodes = set(p["boundary"][[e for e in p["ports"][q] if e >= 0]].ravel())
+        assert set(np.nonzero(o.B[:, q])[0]) == nodes
+
+
+def test_excitation_p2_quirk_factor():
+    p = make_problem(4, 8, 0.3, p2=True)
+    oq = oracle_of(p, quirks=True)
+    oc = oracle_of(p, quirks=False)
+    np.testing.assert_allclose(oc.B.sum(axis=0), 1.0, rtol=1e-12)
+    np.testing.assert_allclose(oq.B.sum(axis=0), 1.5, rtol=1e-12)
+
+
+def test_pcg_restatement_against_direct(small):
+    p, o, g = small
+    A = o.system(g, 0)
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    X = eo.solve_direct(A, F, True)
+    Xc, steps = eo.pcg_reference(A, F, 1.0 / A.diagonal(), True, rule="tolerance", tol=1e-13)
+    assert np.max(np.abs(Xc - X)) < 1e-10 * np.max(np.abs(X))
+    # the dcFree solution has zero nodal sum for zero-sum excitations
+    np.testing.assert_allclose(X.sum(axis=0), 0.0, atol=1e-12)
+    # brute force: dense (A + 1 1^T) solve
+    Ad = A.toarray() + np.ones((o.N, o.N))
+    np.testing.assert_allclose(np.linalg.solve(Ad, F), X, atol=1e-12 * np.max(np.abs(X)))
+    # reference stop rule: column 0 reaches sqrt(r.z / b.b) < FLT_EPSILON
+    Xr, sr = eo.pcg_reference(A, F, 1.0 / A.diagonal(), True)
+    assert sr < 10 * o.N
+    assert np.max(np.abs(Xr - X)) < 1e-5 * np.max(np.abs(X))
+
+
+def test_pcg_zero_rhs_stops_after_one_step(small):
+    p, o, g = small
+    A = o.system(g, 0)
+    b = np.zeros((o.N, 2))
+    x, steps = eo.pcg_reference(A, b, 1.0 / A.diagonal(), True)
+    assert steps == 1 and np.all(x == 0.0)
+
+
+def _disc_voltages(L, width, current, nmax=4000):
+    """Gap-model voltages of the unit disc (sigma = 1): Fourier series of the Neumann
+    problem with current density +-I/w on the driving electrodes, potentials averaged
+    over each measuring electrode; adjacent drive and measurement."""
+    n = np.arange(1, nmax + 1)[:, None]
+    t1 = 2 * np.pi * np.arange(L) / L
+    t2 = t1 + width
+    # Fourier coefficients of the indicator of each electrode, per unit density
+    ca = (np.sin(n * t2) - np.sin(n * t1)) / (n * np.pi)        # [n, L]
+    cb = (np.cos(n * t1) - np.cos(n * t2)) / (n * np.pi)
+    # electrode averages of cos / sin
+    av_c = (np.sin(n * t2) - np.sin(n * t1)) / (n * width)
+    av_s = (np.cos(n * t1) - np.cos(n * t2)) / (n * width)
+    # potential of unit density on electrode e, averaged over electrode m
+    G = (ca.T @ (av_c / n) + cb.T @ (av_s / n))                # [e, m]
+    V = np.zeros((L, L))
+    for d in range(L):
+        j = np.zeros(L)
+        j[d], j[(d + 1) % L] = current / width, -current / width
+        V[:, d] = j @ G
+    P = eo.adjacent_pattern(L)
+    return P.T @ V
+
+
+def test_forward_converges_to_analytic_disc():
+    errs = []
+    for nr in (8, 16, 32):
+        p = make_problem(nr, 8, 0.3)
+        o = oracle_of(p)
+        U, _, _ = o.solve(np.ones(p["elements"].shape[0]))
+        Ua = _disc_voltages(8, 0.3, 1e-3)
+        errs.append(np.max(np.abs(U - Ua)) / np.max(np.abs(Ua)))
+    assert errs[2] < 0.6 * errs[1] < 0.6 * errs[0]
+    assert errs[2] < 2e-2
+
+
+def test_forward_reciprocity_and_2p5d_reduction(small):
+    p, o, g = small
+    U, _, _ = o.solve(g)
+    np.testing.assert_allclose(U, U.T, atol=1e-12 * np.max(np.abs(U)))
+    # C = 1 with h = 1 is the 2D model
+    p2 = dict(p, C=1, height=1.0, port_height=0.3)
+    U2, _, _ = oracle_of(p2).solve(g)
+    np.testing.assert_allclose(U2, U, rtol=1e-12, atol=0)
+
+
+def test_harmonic_constants():
+    p = make_problem(4, 8, 0.3, C=3, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    a1, b1 = o.alpha_beta(1)
+    assert a1 == pytest.approx((2 * math.pi / 0.4) ** 2)
+    assert b1 == pytest.approx(2 * math.sin(math.pi * 0.1 / 0.4) / (math.pi * 0.1))
+    assert o.alpha_beta(0) == (0.0, 1 / 0.4)
+
+
+def _fd_jacobian(o, g, elems, eps=1e-5):
+    cols = []
+    for e in elems:
+        gp, gm = g.copy(), g.copy()
+        gp[e] += eps
+        gm[e] -= eps
+        Up, _, _ = o.solve(gp)
+        Um, _, _ = o.solve(gm)
+        cols.append(o.flatten_voltages((Up - Um) / (2 * eps)))
+    return np.stack(cols, axis=1)
+
+
+def test_jacobian_matches_finite_differences_2d(small):
+    p, o, g = small
+    U, phis, _ = o.solve(g)
+    J = o.jacobian(g, phis)
+    elems = [0, 7, 50, 200, o.E - 1]
+    Jfd = _fd_jacobian(o, g, elems)
+    np.testing.assert_allclose(J[:, elems], Jfd, atol=1e-7 * np.max(np.abs(Jfd)))
+
+
+def test_jacobian_2p5d_exact_vs_reference_formula():
+    p = make_problem(6, 8, 0.3, C=3, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 4))
+    U, phis, _ = o.solve(g)
+    elems = [3, 40, 111]
+    Jfd = _fd_jacobian(o, g, elems)
+    Jx = o.jacobian(g, phis, quirks=False)
+    Jr = o.jacobian(g, phis, quirks=True)
+    np.testing.assert_allclose(Jx[:, elems], Jfd, atol=1e-6 * np.max(np.abs(Jfd)))
+    # the reference formula (K_e only, factor height) is not the derivative for C >= 2
+    assert np.max(np.abs(Jr[:, elems] - Jfd)) > 1e-2 * np.max(np.abs(Jfd))
+
+
+def test_gn_data_space_identity():
+    rng = np.random.default_rng(5)
+    J = rng.standard_normal((40, 120)) * 1e-3
+    r = rng.standard_normal(40) * 1e-4
+    lam2 = 1e-6
+    d1 = eo.gn_step_exact(J, r, lam2)
+    d2 = eo.gn_step_normal(J, r, lam2)
+    np.testing.assert_allclose(d1, d2, rtol=1e-9, atol=1e-15)
+    d3, _ = eo.gn_step_cg_reference(J, r, lam2, eps=1e-30, max_iter=2000)
+    np.testing.assert_allclose(d3, d1, rtol=1e-6, atol=1e-12)
+
+
+def test_reconstruction_improves_fit():
+    p = make_problem(8, 8, 0.2)
+    o = oracle_of(p)
+    gt = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 11))
+    Um, _, _ = o.solve(gt)
+    U0, _, _ = o.solve(np.ones(o.E))
+    res = []
+    for steps in (1, 3):
+        g = eo.reconstruct(o, Um, np.ones(o.E), 1e-4, steps=steps)
+        U1, _, _ = o.solve(g)
+        res.append(np.linalg.norm(U1 - Um) / np.linalg.norm(U0 - Um))
+    assert res[1] < res[0] < 0.8 and res[1] < 0.6
+
+
+def test_cpu_baseline_matches_direct(small):
+    from oracle import baseline
+    p, o, g = small
+    A = o.system(g, 0)
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    X, steps, _ = baseline.pcg(A, F, 1, tol=1e-6)
+    Xd = eo.solve_direct(A, F, True)
+    assert np.max(np.abs(X - Xd)) < 1e-5 * np.max(np.abs(Xd))
