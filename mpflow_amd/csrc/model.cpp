@@ -32,6 +32,19 @@ void mpf_ctx::collect_events() {
 
 namespace mpf {
 
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD b % 8), and
+// each XCD has its own L2.  Reorder a work list so that XCD x receives a contiguous run
+// of consecutive tiles: neighbouring rows gathered by the SpMV then hit the same L2.
+constexpr int kXcds = 8;
+
+static void xcd_order(const int4* in, int n, int4* out) {
+    const int base = n / kXcds, extra = n % kXcds;
+    int start[kXcds + 1];
+    start[0] = 0;
+    for (int x = 0; x < kXcds; ++x) start[x + 1] = start[x] + base + (x < extra ? 1 : 0);
+    for (int b = 0; b < n; ++b) out[b] = in[start[b % kXcds] + b / kXcds];
+}
+
 Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     MPF_REQUIRE(d.frames >= 1, "model: frames < 1");
     frames = d.frames;
@@ -114,10 +127,12 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         for (int b = 0; b < nasm_job; ++b) ha.push_back(make_int4(f, b, 0, 0));
     }
     ntiles = (int)ht.size(); njs = (int)hj.size(); nasm = (int)ha.size();
-    tiles.alloc(ntiles); tiles.upload(ht.data(), ht.size(), st0);
+    std::vector<int4> hto(ht.size());
+    xcd_order(ht.data(), ntiles, hto.data());
+    tiles.alloc(ntiles); tiles.upload(hto.data(), hto.size(), st0);
     js.alloc(njs); js.upload(hj.data(), hj.size(), st0);
     act_tiles.alloc(ntiles); act_js.alloc(njs);
-    MPF_HIP(hipHostMalloc((void**)&h_tiles, sizeof(int4) * ntiles, hipHostMallocDefault));
+    MPF_HIP(hipHostMalloc((void**)&h_tiles, sizeof(int4) * 2 * (size_t)ntiles, hipHostMallocDefault));
     MPF_HIP(hipHostMalloc((void**)&h_js, sizeof(int2) * njs, hipHostMallocDefault));
     MPF_HIP(hipHostMalloc((void**)&h_st, sizeof(int) * (size_t)S.C * kStInts * frames, hipHostMallocDefault));
     asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
@@ -180,7 +195,9 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
             }
         if (nt == 0) break;
         if (nt != cur_nt) {
-            MPF_HIP(hipMemcpyAsync(act_tiles.p, h_tiles, sizeof(int4) * nt, hipMemcpyHostToDevice, st0));
+            int4* ordered = h_tiles + ntiles;   // second half of the pinned staging buffer
+            xcd_order(h_tiles, nt, ordered);
+            MPF_HIP(hipMemcpyAsync(act_tiles.p, ordered, sizeof(int4) * nt, hipMemcpyHostToDevice, st0));
             MPF_HIP(hipMemcpyAsync(act_js.p, h_js, sizeof(int2) * nj, hipMemcpyHostToDevice, st0));
             // h_tiles / h_js are rewritten only after the next poll's synchronisation
             cur_tiles = act_tiles.p; cur_js = act_js.p; cur_nt = nt; cur_njs = nj;
