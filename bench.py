@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--batch", type=int, default=1, help="frames solved together per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -116,41 +117,44 @@ def main():
     w = WORKLOADS[args.workload]
     cfg = CONFIG_INDEX[args.workload]
     nodes, el, bd, ports, pat = build_problem(w)
+    B = max(1, args.batch)
     eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
-              jacobian_exact=True)
+              jacobian_exact=True, frames=B)
     dev = torch.device("cuda", 0)
     opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
                         refine_steps=1, refine_tol=1e-2)
 
-    # measured voltages (untimed): device forward solve of each phantom + 60 dB noise
-    nframes = args.warmup + args.steps
+    # measured voltages (untimed): device forward solve of each phantom + 60 dB noise.
+    # A step is one batch of B frames; the timed frames are steps * B.
+    nbatches = args.warmup + args.steps
     meas = []
-    for f in range(nframes):
-        g = ellipse_phantom(nodes, el, frame_seed(cfg, f))
+    for bi in range(nbatches):
+        fr = [bi * B + j for j in range(B)]
+        g = np.stack([ellipse_phantom(nodes, el, frame_seed(cfg, f)) for f in fr])
         eit.solve(torch.tensor(g, dtype=torch.float32, device=dev), opts)
-        U = eit.result64[0].cpu().numpy()
-        U = add_noise(U, 60.0, frame_seed(cfg, f))
+        U = eit.result64.cpu().numpy()
+        U = np.stack([add_noise(U[j], 60.0, frame_seed(cfg, f)) for j, f in enumerate(fr)])
         meas.append(torch.tensor(U, dtype=torch.float64, device=dev))
     solver = Solver(eit, args.lam, gn_refine=2, options=opts)
 
-    def frame(f):
+    def batch(bi):
         solver.material_distribution.fill_(1.0)
-        solver.measurement[0].copy_(meas[f])
+        solver.measurement.copy_(meas[bi])
         steps = 0
         for _ in range(args.gn_steps):
             solver.solve_absolute()
             steps += solver.last_stats.steps
         return steps
 
-    for f in range(args.warmup):
-        frame(f)
+    for bi in range(args.warmup):
+        batch(bi)
     torch.cuda.synchronize()
     L = capi.lib()
     capi.check(L.mpf_ctx_profile(eit.ctx.h, 1))
     slice_iters = 0
     t0 = time.perf_counter()
-    for f in range(args.warmup, nframes):
-        slice_iters += frame(f)
+    for bi in range(args.warmup, nbatches):
+        slice_iters += batch(bi)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = (ctypes_array := (__import__("ctypes").c_double * 3)())
@@ -158,7 +162,8 @@ def main():
     capi.check(L.mpf_ctx_profile(eit.ctx.h, 0))
     spmv_ms, spmv_launches, spmv_bytes = prof[0], int(prof[1]), prof[2]
 
-    fps = args.steps / elapsed
+    frames_timed = args.steps * B
+    fps = frames_timed / elapsed
     avg_launch_us = 1e3 * spmv_ms / max(spmv_launches, 1)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     roofline = {
@@ -181,6 +186,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "frames_per_step": B,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -193,12 +199,13 @@ def main():
             "forward": "FP32 Jacobi-PCG to ||r||/||b|| <= %g + 1 FP64 residual refinement" % args.tol,
             "nodes": eit.N, "elements": eit.E, "nnz": eit.nnz, "harmonics": eit.C,
             "rhs": eit.R, "jacobian_rows": eit.m,
-            "slice_iterations_per_frame": slice_iters // args.steps,
+            "frames_per_launch": B,
+            "slice_iterations_per_frame": slice_iters // frames_timed,
         },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / args.steps)
+        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / frames_timed)
     print(json.dumps(out))
 
 

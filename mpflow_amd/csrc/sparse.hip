@@ -208,6 +208,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
 
 // 4th dword of a raw buffer resource on gfx9-family GPUs (32-bit data, no swizzle)
 constexpr int kBufferWord3 = 0x00020000;
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // Buffer resource built from wave-uniform values (readfirstlane keeps it in SGPRs, so
 // buffer loads need no per-lane waterfall).
@@ -237,6 +238,94 @@ __device__ __forceinline__ float bcast_f(float v, int src) {
 // the column-major ELL of its 16 rows (coalesced per slot); each row's ids/values are
 // then broadcast so the RHS lanes gather contiguous Rp-float rows of p.
 constexpr int kSpmvRowsPerWave = 16;
+
+// 128-bit gathers: a lane owns 4 consecutive RHS columns, Rp/4 lanes cover one row, so
+// one buffer_load_dwordx4 instruction fetches 64/(Rp/4) neighbour rows at once.  Row
+// ids/values of the wave's 16 rows sit in lanes 0..15 (coalesced column-major ELL load)
+// and are routed to the lanes of each row with ds_bpermute.
+template <int LG, int WMAX>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict__ jobs,
+                                                        const int4* __restrict__ tiles) {
+    constexpr int Rp = 1 << LG;
+    constexpr int LPR = Rp / 4;                       // lanes per row
+    constexpr int RPS = 64 / LPR;                     // rows per wavefront step
+    constexpr int STEPS = kSpmvRowsPerWave / RPS;
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int s = t.y;
+    if (!slice_active(J, s)) return;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+    const int np = J.np;
+    const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
+    const float al = J.alpha[s];
+    const MPF_GLOBAL int32_t* ids = gp(J.ids);
+    const MPF_GLOBAL float* Sv = gp((const float*)J.s);
+    const MPF_GLOBAL float* Rv = gp((const float*)J.r);
+    int idr[WMAX];
+    float vr[WMAX];
+#pragma unroll
+    for (int q = 0; q < WMAX; ++q) { idr[q] = 0; vr[q] = 0.0f; }
+    if (l64 < kSpmvRowsPerWave) {
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) {
+            const size_t e = (size_t)q * np + row0 + l64;
+            idr[q] = ids[e];
+            vr[q] = Sv[e] + al * Rv[e];
+        }
+    }
+    const size_t slice_off = (size_t)s * np * Rp;
+    const MPF_GLOBAL float4* P4 = (const MPF_GLOBAL float4*)(gp((const float*)J.p) + slice_off);
+    MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + slice_off, (size_t)np * Rp * sizeof(float));
+    double pq[4] = {0.0, 0.0, 0.0, 0.0}, ps[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int st = 0; st < STEPS; ++st) {
+        const int src = st * RPS + rsub;              // < 16
+        const int row = row0 + src;
+        const float4 own = P4[(size_t)row * LPR + c4];
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) {
+            const int id = __builtin_amdgcn_ds_bpermute(src << 2, idr[q]);
+            const float v = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(vr[q])));
+            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, (id * Rp + 4 * c4) * 4, 0, 0);
+            acc.x += v * __uint_as_float(raw[0]);
+            acc.y += v * __uint_as_float(raw[1]);
+            acc.z += v * __uint_as_float(raw[2]);
+            acc.w += v * __uint_as_float(raw[3]);
+        }
+        Q4[(size_t)row * LPR + c4] = acc;
+        pq[0] += (double)own.x * (double)acc.x; pq[1] += (double)own.y * (double)acc.y;
+        pq[2] += (double)own.z * (double)acc.z; pq[3] += (double)own.w * (double)acc.w;
+        ps[0] += (double)own.x; ps[1] += (double)own.y; ps[2] += (double)own.z; ps[3] += (double)own.w;
+    }
+    // combine the row sub-groups of the wavefront (same c4), fixed xor tree
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            pq[j] += __shfl_xor(pq[j], off, 64);
+            ps[j] += __shfl_xor(ps[j], off, 64);
+        }
+    }
+    // combine the 16 wavefronts in order
+    __shared__ double red[2][kEwBlock / 64][64];
+    if (rsub == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            red[0][wave][4 * c4 + j] = pq[j];
+            red[1][wave][4 * c4 + j] = ps[j];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < Rp) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kEwBlock / 64; ++w) { a += red[0][w][threadIdx.x]; b += red[1][w][threadIdx.x]; }
+        part_ptr(J, 0, s, t.z)[threadIdx.x] = a;
+        part_ptr(J, 1, s, t.z)[threadIdx.x] = b;
+    }
+}
 
 template <int LG, int WMAX, int U>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
@@ -579,10 +668,22 @@ void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles, cons
     check_launch();
 }
 
+// SpMV variant 0: 128-bit gathers (k_pcg_spmv4); variant 1: 32-bit gathers with
+// v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner).
+int g_spmv_variant = 0;
+
 template <int LG>
 static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st) {
     constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
     constexpr int U2 = STEPS >= 2 ? 2 : 1;
+    if (g_spmv_variant == 0) {
+        switch (wpad) {
+            case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
+            case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
+            case 24: k_pcg_spmv4<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
+            default: break;   // wider rows use the 32-bit variant (no register spills)
+        }
+    }
     switch (wpad) {
         case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
         case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
