@@ -161,8 +161,9 @@ int mpf_model_destroy(mpf_model* m);
 
 /* dims[0]=N, [1]=E, [2]=padded rows np, [3]=ELL width, [4]=Jacobian rows
  * m = ceil16(M)*ceil16(D), [5]=ceil16(M), [6]=ceil16(D), [7]=R=D+M, [8]=RHS lanes,
- * [9]=frames, [10]=C, [11]=nnz */
-int mpf_model_dims(const mpf_model* m, int64_t dims[12]);
+ * [9]=frames, [10]=C, [11]=nnz, [12]=unique RHS columns solved (columns of Pi that are
+ * scalar multiples of an earlier column share one solve), [13]=padded ELL width */
+int mpf_model_dims(const mpf_model* m, int64_t dims[16]);
 
 /* models::EIT::solve (src/models/eit.cpp:139-213) for every frame.
  *   d_gamma : frames x E   (device, float)   material parameter gamma

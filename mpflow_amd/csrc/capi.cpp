@@ -179,13 +179,14 @@ int mpf_model_destroy(mpf_model* m) {
     });
 }
 
-int mpf_model_dims(const mpf_model* m, int64_t dims[12]) {
+int mpf_model_dims(const mpf_model* m, int64_t dims[16]) {
     return guarded([&] {
         MPF_REQUIRE(m && dims, "mpf_model_dims: null argument");
         const auto& S = m->impl->S;
         dims[0] = S.N; dims[1] = S.E; dims[2] = S.np; dims[3] = S.width; dims[4] = S.mrows;
         dims[5] = S.Mp; dims[6] = S.Dp; dims[7] = S.R; dims[8] = S.Rp; dims[9] = m->impl->frames;
-        dims[10] = S.C; dims[11] = S.nnz;
+        dims[10] = S.C; dims[11] = S.nnz; dims[12] = S.Ru; dims[13] = S.wpad;
+        dims[14] = 0; dims[15] = 0;
         return MPF_OK;
     });
 }

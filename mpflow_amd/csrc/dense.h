@@ -8,6 +8,7 @@ namespace mpf {
 // measurement map W (CSR over measurement patterns), shared by all jobs of a mesh
 struct MeasDev {
     const int32_t* w_off; const int32_t* w_node; const double* w_val;
+    const int32_t* colmap; const double* colscale;   // pattern column -> solved lane, scale
     int D, M, Dp, Mp;
 };
 
@@ -16,6 +17,7 @@ struct JacDev {
     const int32_t* elements;     // [E][k]
     const float* ek; const float* em;   // [E][k*k]
     const float* gamma;          // [jobs][E]
+    const int32_t* colmap; const double* colscale;   // pattern column -> solved lane, scale
     int E, k, D, M, Dp, Mp, exact, logarithmic;
     double ref;
     float alpha[kMaxSlices];

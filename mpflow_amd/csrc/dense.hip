@@ -24,8 +24,8 @@ __global__ void __launch_bounds__(256) k_voltages(const JobDev* __restrict__ job
         if (d < md.D && m < md.M) {
             for (int s = 0; s < J.C; ++s)
                 for (int q = md.w_off[m]; q < md.w_off[m + 1]; ++q) {
-                    const size_t i = vec_index(J, s, md.w_node[q], d);
-                    v += md.w_val[q] * (use64 ? J.x64[i] : (double)J.x[i]);
+                    const size_t i = vec_index(J, s, md.w_node[q], md.colmap[d]);
+                    v += md.w_val[q] * md.colscale[d] * (use64 ? J.x64[i] : (double)J.x[i]);
                 }
         }
         if (U32) U32[(size_t)j * mrows + idx] = (float)v;
@@ -66,8 +66,9 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
             Kc[i] = jd.ek[(size_t)e * k * k + i] + al * jd.em[(size_t)e * k * k + i];
         for (int i = threadIdx.x; i < k * R; i += blockDim.x) {
             const int a = i / R, c = i % R;
-            const size_t vi = vec_index(Jb, s, nodes[a], c);
-            phi[a][c] = use64 ? (float)Jb.x64[vi] : Jb.x[vi];
+            const size_t vi = vec_index(Jb, s, nodes[a], jd.colmap[c]);
+            const double sc = jd.colscale[c];
+            phi[a][c] = use64 ? (float)(sc * Jb.x64[vi]) : (float)(sc * (double)Jb.x[vi]);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < k * M; i += blockDim.x) {

@@ -22,7 +22,7 @@ constexpr int kStInts = 4;
 constexpr int kParts = 4;
 
 struct JobDev {
-    int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, pad0;
+    int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, col0;   // R = unique RHS solved
     double tol;
     const int32_t* ids;          // [width][np]
     float* s;  float* r;         // FP32 values (hi)        [width][np]

@@ -65,6 +65,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     w_off.alloc(S.w_off.size()); w_off.upload(S.w_off.data(), S.w_off.size(), st0);
     w_node.alloc(std::max<size_t>(1, S.w_node.size())); if (!S.w_node.empty()) w_node.upload(S.w_node.data(), S.w_node.size(), st0);
     w_val.alloc(std::max<size_t>(1, S.w_val.size())); if (!S.w_val.empty()) w_val.upload(S.w_val.data(), S.w_val.size(), st0);
+    colmap.alloc(S.R); colmap.upload(S.colmap.data(), S.R, st0);
+    colscale.alloc(S.R); colscale.upload(S.colscale.data(), S.R, st0);
 
     const size_t ell = (size_t)W * np;
     const size_t V = vec_elems();
@@ -86,7 +88,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     for (int f = 0; f < frames; ++f) {
         JobDev& J = hjobs[f];
         std::memset(&J, 0, sizeof(J));
-        J.n = S.N; J.np = np; J.width = W; J.C = S.C; J.R = S.R; J.Rp = S.Rp;
+        J.n = S.N; J.np = np; J.width = W; J.C = S.C; J.R = S.Ru; J.Rp = S.Rp; J.col0 = S.colmap[0];
         J.lg = S.Rp == 16 ? 4 : (S.Rp == 32 ? 5 : 6);
         J.ntiles = ntile_job;
         J.dc = 0;  // Open sources: harmonic 0 is solved dcFree (eit.cpp:175-177)
@@ -210,7 +212,7 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
         // column ids once (8 nnz), p read and q written once (8 N R)
         int64_t slice_iters = 0;
         for (size_t i = 0; i < (size_t)S.C * frames; ++i) slice_iters += hst[i * kStInts + 1];
-        ctx->prof_bytes += (double)slice_iters * (8.0 * S.nnz + 8.0 * S.N * S.R);
+        ctx->prof_bytes += (double)slice_iters * (8.0 * S.nnz + 8.0 * S.N * S.Ru);
     }
     for (int f = 0; f < frames; ++f)
         for (int c = 0; c < S.C; ++c) {
@@ -258,12 +260,13 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
             stats->refine_residual = worst;
         }
     }
-    MeasDev md{w_off.p, w_node.p, w_val.p, S.D, S.M, S.Dp, S.Mp};
+    MeasDev md{w_off.p, w_node.p, w_val.p, colmap.p, colscale.p, S.D, S.M, S.Dp, S.Mp};
     launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
     if (d_J) {
         JacDev jd;
         std::memset(&jd, 0, sizeof(jd));
         jd.elements = elements.p; jd.ek = ek.p; jd.em = em.p; jd.gamma = gamma.p;
+        jd.colmap = colmap.p; jd.colscale = colscale.p;
         jd.E = S.E; jd.k = S.k; jd.D = S.D; jd.M = S.M; jd.Dp = S.Dp; jd.Mp = S.Mp;
         jd.exact = S.jacobian_exact; jd.logarithmic = S.logarithmic; jd.ref = S.ref;
         for (int c = 0; c < S.C; ++c) {

@@ -29,8 +29,8 @@ struct Model {
     int frames = 1;
     int quirks = 1;
     // mesh data on the device
-    DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node;
-    DBuf<double> ctb_k, ctb_m, F, w_val;
+    DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node, colmap;
+    DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
     DBuf<float> ek, em;
     // per-frame state
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;

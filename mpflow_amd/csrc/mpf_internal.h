@@ -71,6 +71,11 @@ struct DBuf {
 struct MeshSetup {
     int N = 0, E = 0, k = 0, np = 0, width = 0, wpad = 0, nnz = 0;
     int P = 0, D = 0, M = 0, R = 0, Rp = 0, Mp = 0, Dp = 0, mrows = 0;
+    // right-hand-side deduplication: column j of Pi = colscale[j] * unique column colmap[j];
+    // the device solves Ru unique columns (Rp lanes >= Ru)
+    int Ru = 0;
+    std::vector<int32_t> colmap;
+    std::vector<double> colscale;
     // ELL pattern, slot-major: [wpad][np]; padding slots id = row, value 0
     std::vector<int32_t> ids;
     std::vector<int32_t> diag_slot;               // [np]

@@ -171,7 +171,41 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
     s.np = round_up(N, kTileRows);
     s.P = d.n_ports; s.D = d.n_drive; s.M = d.n_meas; s.R = s.D + s.M;
     MPF_REQUIRE(s.R <= 64, "model: D + M > 64 is not supported");
-    s.Rp = s.R <= 16 ? 16 : (s.R <= 32 ? 32 : 64);
+    // Pi = [I*Dpat | Mpat] (sources.cpp:57-71); columns that are scalar multiples of an
+    // earlier column (adjacent measurement = adjacent drive / I) share one solve
+    {
+        auto Pi = [&](int p, int j) {
+            return j < s.D ? d.drive_current[j] * d.drive_pattern[(size_t)p * s.D + j]
+                           : d.meas_pattern[(size_t)p * s.M + (j - s.D)];
+        };
+        std::vector<std::vector<double>> uniq;   // normalised representatives
+        std::vector<double> uamp;
+        s.colmap.assign(s.R, 0);
+        s.colscale.assign(s.R, 0.0);
+        for (int j = 0; j < s.R; ++j) {
+            double amp = 0.0;
+            for (int p = 0; p < s.P; ++p)
+                if (std::fabs(Pi(p, j)) > std::fabs(amp)) amp = Pi(p, j);
+            std::vector<double> nrm(s.P, 0.0);
+            if (amp != 0.0)
+                for (int p = 0; p < s.P; ++p) nrm[p] = Pi(p, j) / amp;
+            int u = -1;
+            for (size_t k = 0; k < uniq.size() && u < 0; ++k) {
+                bool same = true;
+                for (int p = 0; p < s.P && same; ++p) same = std::fabs(uniq[k][p] - nrm[p]) <= 1e-14;
+                if (same) u = (int)k;
+            }
+            if (u < 0) {
+                u = (int)uniq.size();
+                uniq.push_back(nrm);
+                uamp.push_back(amp);
+            }
+            s.colmap[j] = u;
+            s.colscale[j] = uamp[u] != 0.0 ? amp / uamp[u] : 0.0;
+        }
+        s.Ru = (int)uniq.size();
+    }
+    s.Rp = s.Ru <= 16 ? 16 : (s.Ru <= 32 ? 32 : 64);
     s.Mp = ceil16(s.M); s.Dp = ceil16(s.D); s.mrows = s.Mp * s.Dp;
     s.C = d.components;
     s.ref = d.reference_value; s.height = d.height; s.port_height = d.port_height;
@@ -310,15 +344,20 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
             for (int q = 0; q < kb; ++q) Bn[ed[q]][p] += w[q] / length;
         }
     }
-    // F = B * Pi, Pi = [I*Dpat | Mpat]  (sources.cpp:57-71)
+    // F = B * Pi for the unique columns, Pi = [I*Dpat | Mpat]  (sources.cpp:57-71)
     s.F.assign((size_t)np * s.Rp, 0.0);
     auto Pi = [&](int p, int j) {
         return j < s.D ? d.drive_current[j] * d.drive_pattern[(size_t)p * s.D + j]
                        : d.meas_pattern[(size_t)p * s.M + (j - s.D)];
     };
+    std::vector<int> rep(s.Ru, -1);   // first original column of each unique column
+    for (int j = 0; j < s.R; ++j)
+        if (rep[s.colmap[j]] < 0) rep[s.colmap[j]] = j;
     for (int n = 0; n < N; ++n)
         for (auto& kv : Bn[n])
-            for (int j = 0; j < s.R; ++j) s.F[(size_t)n * s.Rp + j] += kv.second * Pi(kv.first, j);
+            for (int u = 0; u < s.Ru; ++u) s.F[(size_t)n * s.Rp + u] += kv.second * Pi(kv.first, rep[u]);
+    // a representative column has scale 1 by construction
+    for (int u = 0; u < s.Ru; ++u) s.colscale[rep[u]] = 1.0;
     // W = Mpat^T B^T  (eit.cpp:93-94) as CSR
     s.w_off.assign(s.M + 1, 0);
     s.w_node.clear(); s.w_val.clear();

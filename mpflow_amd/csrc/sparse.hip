@@ -479,8 +479,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_rz(const JobDev* __rest
         const double bb = sc_ptr(J, SC_BB, s)[lane];
         sc_ptr(J, SC_RR, s)[lane] = rr;
         if (J.rule == MPF_STOP_REFERENCE) {
-            if (lane == 0) {
-                // sqrt(rho / ref) >= eps keeps iterating; NaN (ref == 0) stops
+            if (lane == J.col0) {
+                // column 0 of the pattern only: sqrt(rho / ref) >= eps keeps iterating;
+                // NaN (ref == 0) stops
                 const double crit = fabs(sqrt(rho / bb));
                 if (crit >= (double)FLT_EPSILON) not_done = 1;
             }

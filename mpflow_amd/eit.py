@@ -117,10 +117,10 @@ class EIT:
         h = C.c_void_p()
         check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h
-        dims = (C.c_int64 * 12)()
+        dims = (C.c_int64 * 16)()
         check(lib().mpf_model_dims(h, dims))
         (self.N, self.E, self.np, self.width, self.m, self.Mp, self.Dp, self.R, self.Rp, self.frames,
-         self.C, self.nnz) = [int(v) for v in dims]
+         self.C, self.nnz, self.R_unique, self.wpad) = [int(v) for v in dims[:14]]
         self.D, self.M = D, k["meas"].shape[1]
         self.reference_value = float(reference_value)
         dev = torch.device("cuda", self.ctx.device)
