@@ -12,7 +12,7 @@ def build():
     if not os.path.exists(src):
         return None
     os.makedirs(OUT, exist_ok=True)
-    cmd = ["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c++17", src, "-o", LIB + ".tmp"]
+    cmd = ["g++", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c++17", src, "-o", LIB + ".tmp"]
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     return LIB
