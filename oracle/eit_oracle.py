@@ -317,18 +317,23 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
 def solve_direct(A, F, dc_free):
     """Exact FP64 solve of (A [+ 1 1^T]) X = F via a sparse LU.
 
-    With dcFree the bordered system [[A, 1], [1^T, 0]] [x; l] = [F; 1^T F / N] is
-    solved; its x satisfies (A + 1 1^T) x = F because 1^T A = 0 (Neumann stiffness).
+    With dcFree, A is the singular Neumann matrix (A 1 = 0).  Then 1^T x = 1^T F / N and
+    A x = F - (1^T F / N) 1, a consistent system: it is solved with the last node
+    grounded (row and column removed) and the result shifted to the required nodal sum.
     """
     N = A.shape[0]
+    F = np.asarray(F, dtype=np.float64)
     if not dc_free:
         lu = spla.splu(sp.csc_matrix(A))
         return lu.solve(F)
-    one = sp.csr_matrix(np.ones((N, 1)))
-    Bd = sp.bmat([[A, one], [one.T, None]], format="csc")
-    rhs = np.vstack([F, F.sum(axis=0, keepdims=True) / N])
-    lu = spla.splu(Bd)
-    return lu.solve(rhs)[:N]
+    mean = F.sum(axis=0, keepdims=True) / N
+    Fc = F - mean
+    A = sp.csr_matrix(A)
+    lu = spla.splu(sp.csc_matrix(A[:N - 1, :N - 1]))
+    X = np.zeros_like(F)
+    X[:N - 1] = lu.solve(Fc[:N - 1])
+    X += (mean - X.sum(axis=0, keepdims=True) / N)
+    return X
 
 
 # ---------------------------------------------------------------------------

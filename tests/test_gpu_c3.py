@@ -1,0 +1,73 @@
+"""Parity at the headline configuration: 32 electrodes, P2 mesh of 40344 elements /
+81181 nodes, 5 harmonics, h = 0.4, h_p = 0.1 (config 3).  Device results against the
+FP64 oracle: forward voltages (FP32 PCG and FP32 + FP64 refinement) and one
+Gauss-Newton step from gamma = 1 (BF16-MFMA Gram + FP64 Cholesky + refinement)."""
+import numpy as np
+import pytest
+
+from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
+from oracle.eit_oracle import EITOracle, gn_step_exact
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+
+@pytest.fixture(scope="module")
+def c3():
+    from bench import WORKLOADS, build_problem
+    w = WORKLOADS["c3"]
+    nodes, el, bd, ports, pat = build_problem(w)
+    print("case: c3 oracle set-up", flush=True)
+    o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, 5, 0.4, 0.1)
+    g = ellipse_phantom(nodes, el, frame_seed(3, 0))
+    U, _, _ = o.solve(g)
+    return dict(w=w, nodes=nodes, el=el, bd=bd, ports=ports, pat=pat, o=o, g=g, U=U)
+
+
+def _eit(c, ctx, exact=True):
+    from mpflow_amd.eit import EIT
+    return EIT(c["nodes"], c["el"], c["bd"], c["ports"], c["pat"], c["pat"], 1e-3, 1.0, 5, 0.4, 0.1,
+               jacobian_exact=exact, ctx=ctx)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_c3_forward_voltages(gpu_ctx, c3, mode):
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import SolveOptions
+    print("case: c3 forward %s" % mode, flush=True)
+    eit = _eit(c3, gpu_ctx)
+    if mode == "fp32":
+        opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    else:
+        opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                            refine_steps=2, refine_tol=1e-2)
+    eit.solve(torch.tensor(c3["g"], dtype=torch.float32), opts)
+    U = eit.voltages().cpu().numpy()[0]
+    err = np.max(np.abs(U - c3["U"])) / np.max(np.abs(c3["U"]))
+    print("c3 %s voltage error %.3e, steps %d, refine residual %.3e"
+          % (mode, err, eit.last_stats.steps, eit.last_stats.refine_residual))
+    assert err < (1e-5 if mode == "fp32" else 1e-7)
+
+
+def test_c3_gn_step(gpu_ctx, c3):
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import SolveOptions, Solver
+    o = c3["o"]
+    Um = add_noise(c3["U"], 60.0, frame_seed(3, 0))
+    g0 = np.ones(c3["el"].shape[0])
+    print("case: c3 oracle GN step", flush=True)
+    U0, phis, _ = o.solve(g0)
+    J = o.jacobian(g0, phis, quirks=False)
+    d_ref = gn_step_exact(J, o.flatten_voltages(Um) - o.flatten_voltages(U0), 1e-6)
+    print("case: c3 device GN step", flush=True)
+    eit = _eit(c3, gpu_ctx)
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=1, refine_tol=1e-2)
+    s = Solver(eit, 1e-3, gn_refine=2, options=opts)
+    s.measurement[0] = torch.tensor(o.flatten_voltages(Um))
+    s.solve_absolute()
+    d = s.material_distribution.cpu().numpy()[0].astype(np.float64) - 1.0
+    err = np.linalg.norm(d - d_ref) / np.linalg.norm(d_ref)
+    print("c3 GN step relative L2 error %.3e" % err)
+    assert err < 1e-3
