@@ -135,7 +135,7 @@ def main():
         U = eit.result64.cpu().numpy()
         U = np.stack([add_noise(U[j], 60.0, frame_seed(cfg, f)) for j, f in enumerate(fr)])
         meas.append(torch.tensor(U, dtype=torch.float64, device=dev))
-    solver = Solver(eit, args.lam, gn_refine=2, options=opts)
+    solver = Solver(eit, args.lam, gn_refine=4, options=opts)
 
     def batch(bi):
         solver.material_distribution.fill_(1.0)

@@ -340,30 +340,43 @@ __global__ void __launch_bounds__(256) k_potrf_update(int n, double* __restrict_
     }
 }
 
-// solve L L^T y = b (one block per batch entry), blocked substitution
-__global__ void __launch_bounds__(256) k_potrs(int n, const double* __restrict__ A, size_t strideA,
-                                               const double* __restrict__ b, size_t strideb,
-                                               double* __restrict__ y, size_t stridey) {
+// solve L L^T y = b (one block per batch entry), blocked substitution.  Each 32x32
+// diagonal block is staged in LDS and solved by wavefront 0 with shuffles; the forward
+// update reads rows of L coalesced across threads, the backward update lets each
+// wavefront read contiguous column segments of L and reduce them with shuffles.
+constexpr int kTrsvThreads = 1024;
+
+__global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __restrict__ A, size_t strideA,
+                                                        const double* __restrict__ b, size_t strideb,
+                                                        double* __restrict__ y, size_t stridey) {
     const double* a = A + blockIdx.x * strideA;
     double* yv = y + blockIdx.x * stridey;
     const double* bv = b + blockIdx.x * strideb;
-    for (int i = threadIdx.x; i < n; i += 256) yv[i] = bv[i];
-    __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = kTrsvThreads / 64;
+    __shared__ double Lb[kCB][kCB + 1];
     __shared__ double z[kCB];
+    for (int i = tid; i < n; i += kTrsvThreads) yv[i] = bv[i];
+    __syncthreads();
     const int nblk = (n + kCB - 1) / kCB;
     // forward: L z = b
     for (int kb = 0; kb < nblk; ++kb) {
         const int o = kb * kCB, nb = min(kCB, n - o);
-        if (threadIdx.x == 0) {
-            for (int r = 0; r < nb; ++r) {
-                double s = yv[o + r];
-                for (int t = 0; t < r; ++t) s -= a[(o + r) + (size_t)(o + t) * n] * z[t];
-                z[r] = s / a[(o + r) + (size_t)(o + r) * n];
-            }
+        for (int i = tid; i < kCB * kCB; i += kTrsvThreads) {
+            const int r = i % kCB, c = i / kCB;
+            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? 1.0 : 0.0);
         }
         __syncthreads();
-        for (int r = threadIdx.x; r < nb; r += 256) yv[o + r] = z[r];
-        for (int r = o + nb + threadIdx.x; r < n; r += 256) {
+        if (wave == 0) {
+            double v = lane < nb ? yv[o + lane] : 0.0;
+            for (int r = 0; r < nb; ++r) {
+                const double zr = __shfl(v, r, 64) / Lb[r][r];
+                if (lane == r) v = zr;
+                else if (lane > r && lane < nb) v -= Lb[lane][r] * zr;
+            }
+            if (lane < nb) { z[lane] = v; yv[o + lane] = v; }
+        }
+        __syncthreads();
+        for (int r = o + nb + tid; r < n; r += kTrsvThreads) {
             double s = 0.0;
             for (int t = 0; t < nb; ++t) s += a[r + (size_t)(o + t) * n] * z[t];
             yv[r] -= s;
@@ -373,20 +386,26 @@ __global__ void __launch_bounds__(256) k_potrs(int n, const double* __restrict__
     // backward: L^T y = z
     for (int kb = nblk - 1; kb >= 0; --kb) {
         const int o = kb * kCB, nb = min(kCB, n - o);
-        if (threadIdx.x == 0) {
-            for (int r = nb - 1; r >= 0; --r) {
-                double s = yv[o + r];
-                for (int t = r + 1; t < nb; ++t) s -= a[(o + t) + (size_t)(o + r) * n] * z[t];
-                z[r] = s / a[(o + r) + (size_t)(o + r) * n];
-            }
+        for (int i = tid; i < kCB * kCB; i += kTrsvThreads) {
+            const int r = i % kCB, c = i / kCB;
+            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? 1.0 : 0.0);
         }
         __syncthreads();
-        for (int r = threadIdx.x; r < nb; r += 256) yv[o + r] = z[r];
-        // y[0:o] -= L[o:o+nb, 0:o]^T z
-        for (int c = threadIdx.x; c < o; c += 256) {
-            double s = 0.0;
-            for (int t = 0; t < nb; ++t) s += a[(o + t) + (size_t)c * n] * z[t];
-            yv[c] -= s;
+        if (wave == 0) {
+            double v = lane < nb ? yv[o + lane] : 0.0;
+            for (int r = nb - 1; r >= 0; --r) {
+                const double zr = __shfl(v, r, 64) / Lb[r][r];
+                if (lane == r) v = zr;
+                else if (lane < r) v -= Lb[r][lane] * zr;
+            }
+            if (lane < nb) { z[lane] = v; yv[o + lane] = v; }
+        }
+        __syncthreads();
+        // y[c] -= sum_t L[o+t][c] z[t] for c < o: column segment a[o..o+nb, c] is contiguous
+        for (int c = wave; c < o; c += nwave) {
+            double p = lane < nb ? a[(o + lane) + (size_t)c * n] * z[lane] : 0.0;
+            for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+            if (lane == 0) yv[c] -= p;
         }
         __syncthreads();
     }
@@ -529,7 +548,7 @@ size_t cholesky_pairs(int n) {
 
 void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_t strideb, double* y,
                   size_t stridey, int batch, hipStream_t st) {
-    k_potrs<<<batch, 256, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
+    k_potrs<<<batch, kTrsvThreads, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
     check_launch();
 }
 

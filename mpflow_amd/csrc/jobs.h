@@ -40,6 +40,8 @@ struct JobDev {
     double* part;                // [kParts][C][ntiles][Rp]
     double* sc;                  // [SC_NFIELDS][C][Rp]
     int* st;                     // [C][kStInts]
+    int* cnt;                    // [C][2] last-block tickets (SpMV, update), zero between launches
+    int* active;                 // unconverged (job, harmonic) pairs of the launch
     // assembly
     const int32_t* ctb_off; const int32_t* ctb_elem;
     const double* ctb_k; const double* ctb_m;

@@ -79,8 +79,11 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
     st.alloc((size_t)S.C * kStInts * frames);
+    cnt.alloc((size_t)S.C * 2 * frames);
+    active.alloc(1);
     gamma.alloc((size_t)S.E * frames);
     MPF_HIP(hipMemsetAsync(st.p, 0, st.bytes(), st0));
+    MPF_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st0));
 
@@ -108,6 +111,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.part = part.p + (size_t)kParts * S.C * ntile_job * S.Rp * f;
         J.sc = sc.p + (size_t)SC_NFIELDS * S.C * S.Rp * f;
         J.st = st.p + (size_t)S.C * kStInts * f;
+        J.cnt = cnt.p + (size_t)S.C * 2 * f;
+        J.active = active.p;
         J.ctb_off = ctb_off.p; J.ctb_elem = ctb_elem.p; J.ctb_k = ctb_k.p; J.ctb_m = ctb_m.p;
         J.gamma = gamma.p + (size_t)S.E * f;
         J.ref = S.ref; J.logarithmic = S.logarithmic; J.E = S.E;
@@ -138,7 +143,6 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     MPF_HIP(hipHostMalloc((void**)&h_js, sizeof(int2) * njs, hipHostMallocDefault));
     MPF_HIP(hipHostMalloc((void**)&h_st, sizeof(int) * (size_t)S.C * kStInts * frames, hipHostMallocDefault));
     asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
-    active.alloc(1);
     MPF_HIP(hipHostMalloc((void**)&h_active, sizeof(int), hipHostMallocDefault));
     MPF_HIP(hipStreamSynchronize(st0));
 }
@@ -251,7 +255,7 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
             double worst = 0.0;
             for (int f = 0; f < frames; ++f)
                 for (int c = 0; c < S.C; ++c)
-                    for (int l = 0; l < S.R; ++l) {
+                    for (int l = 0; l < S.Ru; ++l) {
                         const double* b = &hsc[(size_t)SC_NFIELDS * S.C * S.Rp * f];
                         const double rr = b[((size_t)SC_X1 * S.C + c) * S.Rp + l];
                         const double bb = b[((size_t)SC_X2 * S.C + c) * S.Rp + l];

@@ -35,7 +35,7 @@ struct Model {
     // per-frame state
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;
     DBuf<double> x64, part, sc;
-    DBuf<int> st;
+    DBuf<int> st, cnt;
     DBuf<float> gamma;
     // job tables
     std::vector<JobDev> hjobs;

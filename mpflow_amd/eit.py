@@ -176,7 +176,7 @@ class Solver:
     """solver::Solver with identity Tikhonov regularisation; ``solve_absolute`` is one
     Gauss-Newton step per frame (Solver::solveAbsolute, src/solver/solver.cpp:195-225)."""
 
-    def __init__(self, eit: EIT, regularization_factor: float, gn_refine: int = 2,
+    def __init__(self, eit: EIT, regularization_factor: float, gn_refine: int = 4,
                  options: SolveOptions | None = None):
         torch = eit.torch
         self.eit = eit

@@ -64,7 +64,7 @@ def test_c3_gn_step(gpu_ctx, c3):
     eit = _eit(c3, gpu_ctx)
     opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
                         refine_steps=1, refine_tol=1e-2)
-    s = Solver(eit, 1e-3, gn_refine=2, options=opts)
+    s = Solver(eit, 1e-3, gn_refine=4, options=opts)
     s.measurement[0] = torch.tensor(o.flatten_voltages(Um))
     s.solve_absolute()
     d = s.material_distribution.cpu().numpy()[0].astype(np.float64) - 1.0
