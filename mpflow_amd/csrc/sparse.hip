@@ -143,28 +143,6 @@ __device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* 
     }
 }
 
-// Last-block-done: called by every thread of a block after wavefront 0 wrote this
-// tile's partials.  Wavefront 0 releases them at agent scope (visible to every XCD)
-// and takes a ticket; the block that draws the last ticket acquires and returns true,
-// then reduces all tiles in fixed order and resets the counter for the next launch.
-__device__ __forceinline__ bool last_block(int* counter, int total) {
-    __shared__ int is_last;
-    if (threadIdx.x < 64) {
-        __threadfence();
-        if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1) == total - 1);
-    }
-    __syncthreads();
-    const bool l = is_last != 0;
-    if (l) {
-        if (threadIdx.x == 0) {
-            __threadfence();
-            *counter = 0;
-        }
-        __syncthreads();
-    }
-    return l;
-}
-
 __device__ void finish_pq(const JobDev& J, int s);
 __device__ void finish_rz(const JobDev& J, int s, int* active);
 
@@ -350,7 +328,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict
         part_ptr(J, 0, s, t.z)[threadIdx.x] = a;
         part_ptr(J, 1, s, t.z)[threadIdx.x] = b;
     }
-    if (last_block(J.cnt + 2 * s, J.ntiles)) finish_pq(J, s);
 }
 
 template <int LG, int WMAX, int U>
@@ -422,7 +399,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
     const int ch[2] = {0, 1};
     const double v[2] = {pq, ps};
     block_partials<2, kEwBlock>(J, s, t.z, ch, v);
-    if (last_block(J.cnt + 2 * s, J.ntiles)) finish_pq(J, s);
 }
 
 // p.q and 1^T p over all tiles -> alpha (run by one kEwBlock-thread block per pair)
@@ -485,7 +461,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update(const JobDev* __restric
     const int ch[2] = {2, 3};
     const double v[2] = {rz, rr};
     block_partials<2, kEwBlock>(J, s, t.z, ch, v);
-    if (last_block(J.cnt + 2 * s + 1, J.ntiles)) finish_rz(J, s, J.active);
 }
 
 // rho = r.z, convergence test, beta = rho / rho_old
@@ -742,9 +717,12 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, cons
     else if (lg == 4) spmv_dispatch<4>(wpad, d_jobs, d_tiles, ntiles, st);
     else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
-    // p.q -> alpha and r.z -> convergence / beta are reduced by the last block of the
-    // SpMV and of the update kernel respectively
+    // separate one-block reductions: folding them into the producers with last-block
+    // tickets needs an agent-scope release per block (L2 write-back across XCDs), which
+    // measured slower than these launches
+    k_pcg_reduce_pq<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
     k_pcg_update<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    k_pcg_reduce_rz<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
     k_pcg_pupdate<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     check_launch();
 }
