@@ -30,6 +30,8 @@ struct JobDev {
     float* dinv;                 // [C][np]
     float alpha[kMaxSlices];     // alpha_c of A_c = S + alpha_c R
     float xscale[kMaxSlices];    // beta_c excitation scale
+    double alpha64[kMaxSlices];  // the same two constants in FP64 for the refinement residual
+    double xscale64[kMaxSlices];
     const double* F;             // [np][Rp] excitation B*Pi, FP64
     float* rhs;                  // [C][np][Rp] FP32 right-hand side buffer (always valid)
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F

@@ -101,7 +101,10 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.ids = ids.p;
         J.s = s.p + ell * f; J.r = r.p + ell * f; J.s_lo = s_lo.p + ell * f; J.r_lo = r_lo.p + ell * f;
         J.dinv = dinv.p + (size_t)S.C * np * f;
-        for (int c = 0; c < S.C; ++c) { J.alpha[c] = (float)S.alpha[c]; J.xscale[c] = (float)S.beta[c]; }
+        for (int c = 0; c < S.C; ++c) {
+            J.alpha[c] = (float)S.alpha[c]; J.xscale[c] = (float)S.beta[c];
+            J.alpha64[c] = S.alpha[c]; J.xscale64[c] = S.beta[c];
+        }
         J.F = F.p;
         J.rhs = rhs.p + V * f;
         J.use_rhs = 0;

@@ -574,8 +574,8 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     const Ew L = ew_of(J);
-    const double al = (double)J.alpha[s];
-    const double xs = (double)J.xscale[s];
+    const double al = J.alpha64[s];
+    const double xs = J.xscale64[s];
     const double dcs = sc_ptr(J, SC_X0, s)[L.lane];
     const double* X = J.x64 + (size_t)s * J.np * J.Rp;
     float* out = J.rhs;
