@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--batch", type=int, default=1, help="frames solved together per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -141,8 +142,10 @@ def main():
         solver.material_distribution.fill_(1.0)
         solver.measurement.copy_(meas[bi])
         steps = 0
-        for _ in range(args.gn_steps):
-            solver.solve_absolute()
+        for k in range(args.gn_steps):
+            # GN steps after the first start their forward solves from the previous
+            # step's fields of the same frame; every frame starts cold
+            solver.solve_absolute(warm_start=(k > 0 and not args.cold))
             steps += solver.last_stats.steps
         return steps
 

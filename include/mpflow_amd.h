@@ -144,6 +144,9 @@ typedef struct mpf_solve_opts {
     double refine_tol;            /* inner tolerance of each refinement solve       */
     int32_t krylov;               /* MPF_KRYLOV_*                                   */
     int32_t check_every;          /* iterations between host convergence polls (0 -> 16) */
+    int32_t warm_start;           /* MPF_PREC_MIXED only: start from the model's previous FP64
+                                     fields (a correction solve of the FP64 residual, stopped at
+                                     tol * ||b||) instead of from zero; ignored on the first solve */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {

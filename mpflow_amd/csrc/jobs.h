@@ -35,7 +35,7 @@ struct JobDev {
     const double* F;             // [np][Rp] excitation B*Pi, FP64
     float* rhs;                  // [C][np][Rp] FP32 right-hand side buffer (always valid)
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
-    int pad1;
+    int ref_b;                   // 1: stop relative to ||beta_c F|| (SC_X2) instead of ||rhs||
     float* x; float* rv; float* p; float* q;   // [C][np][Rp]
     float* t1; float* t2;        // BiCGSTAB extra vectors [C][np][Rp]
     double* x64;                 // [C][np][Rp] FP64 refined solution

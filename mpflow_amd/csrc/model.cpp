@@ -161,13 +161,14 @@ void Model::upload_jobs() {
     jobs.upload(hjobs.data(), hjobs.size(), ctx->stream);
 }
 
-void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, mpf_solve_stats* stats,
-                   bool* capped) {
+void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
+                   mpf_solve_stats* stats, bool* capped) {
     hipStream_t st0 = ctx->stream;
     const int maxit = o.max_iter > 0 ? o.max_iter : 10 * S.N;
     for (auto& J : hjobs) {
         J.rule = rule; J.tol = tol; J.maxit = maxit;
         J.use_rhs = use_rhs ? 1 : 0;
+        J.ref_b = ref_b ? 1 : 0;
     }
     upload_jobs();
     MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));
@@ -241,12 +242,22 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
     launch_dinv(jobs.p, tiles.p, ntiles, st0);
     bool capped = false;
     const bool mixed = o.precision == MPF_PREC_MIXED;
-    krylov(o, o.stop_rule, o.tol, false, stats, &capped);
+    const bool warm = mixed && o.warm_start && x64_valid;
+    if (warm) {
+        // start from the previous FP64 fields: solve for the correction of the FP64
+        // residual at the new sigma, to the same absolute accuracy tol * ||beta F||
+        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
+        krylov(o, MPF_STOP_TOLERANCE, o.tol, true, true, stats, &capped);
+        launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
+    } else {
+        krylov(o, o.stop_rule, o.tol, false, false, stats, &capped);
+        if (mixed) launch_x64_update(jobs.p, tiles.p, ntiles, 1, st0);
+    }
+    x64_valid = mixed;
     if (mixed) {
-        launch_x64_update(jobs.p, tiles.p, ntiles, 1, st0);
         for (int k = 0; k < o.refine_steps; ++k) {
             launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
-            krylov(o, MPF_STOP_TOLERANCE, o.refine_tol, true, stats, &capped);
+            krylov(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats, &capped);
             launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
         }
         // final FP64 residual for the report

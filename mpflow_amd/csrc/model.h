@@ -68,8 +68,9 @@ struct Model {
                 const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats);
 
   private:
-    void krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, mpf_solve_stats* stats,
-                bool* capped);
+    void krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
+                mpf_solve_stats* stats, bool* capped);
+    bool x64_valid = false;
     void ensure_gn();
 };
 

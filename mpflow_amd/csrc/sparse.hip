@@ -198,7 +198,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
     reduce_tiles<2>(J, s, ch, out);
     if ((int)threadIdx.x < J.Rp) {
         sc_ptr(J, SC_RHO, s)[threadIdx.x] = out[0];
-        sc_ptr(J, SC_BB, s)[threadIdx.x] = out[1];
+        // a correction solve started from a previous solution stops relative to the
+        // full right-hand side ||beta_c F||^2 left in SC_X2 by the FP64 residual
+        sc_ptr(J, SC_BB, s)[threadIdx.x] = J.ref_b ? sc_ptr(J, SC_X2, s)[threadIdx.x] : out[1];
         sc_ptr(J, SC_RS, s)[threadIdx.x] = out[0];
     }
     if (threadIdx.x == 0) {
@@ -520,6 +522,116 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_rz(const JobDev* __rest
     finish_rz(J, t.y, active);
 }
 
+// ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
+// Combine per-lane FP64 values of 4 columns across the row sub-groups of each
+// wavefront (fixed xor tree) and across the 16 wavefronts (fixed order), then write
+// the tile partials of NV channels.
+template <int LG, int NV>
+__device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile, const int* ch,
+                                                double (&v)[NV][4]) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
+    __shared__ double red[NV][kEwBlock / 64][64];
+    if (rsub == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[c][wave][4 * c4 + j] = v[c][j];
+    __syncthreads();
+    if ((int)threadIdx.x < (1 << LG)) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double a = 0.0;
+            for (int w = 0; w < kEwBlock / 64; ++w) a += red[c][w][threadIdx.x];
+            part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
+        }
+    }
+}
+
+template <int LG>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
+                                                          const int4* __restrict__ tiles) {
+    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int s = t.y;
+    if (!slice_active(J, s)) return;
+    const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
+    float a[4], dcv[4];
+    bool zero[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = 4 * c4 + j;
+        zero[j] = sc_ptr(J, SC_PQ, s)[col] == 0.0;
+        a[j] = (float)sc_ptr(J, SC_ALPHA, s)[col];
+        dcv[j] = (float)sc_ptr(J, SC_PSUM, s)[col];
+    }
+    const size_t base = (size_t)s * J.np * LPR;
+    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
+    const MPF_GLOBAL float4* P = (const MPF_GLOBAL float4*)gp((const float*)J.p) + base;
+    const MPF_GLOBAL float4* Q = (const MPF_GLOBAL float4*)gp((const float*)J.q) + base;
+    MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
+    MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
+    double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
+        const size_t i = (size_t)row * LPR + c4;
+        const float4 p4 = P[i], q4 = Q[i];
+        float4 x4 = X[i], r4 = Rr[i];
+        const float d = dinv[row];
+        const float dc = row < J.n ? 1.0f : 0.0f;
+        float pv[4] = {p4.x, p4.y, p4.z, p4.w}, qv[4] = {q4.x, q4.y, q4.z, q4.w};
+        float xv[4] = {x4.x, x4.y, x4.z, x4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (zero[j]) { xv[j] = 0.0f; rv[j] = 0.0f; }
+            else { xv[j] = xv[j] + a[j] * pv[j]; rv[j] = rv[j] - a[j] * (qv[j] + dc * dcv[j]); }
+            const float z = d * rv[j];
+            v[0][j] += (double)rv[j] * (double)z;
+            v[1][j] += (double)rv[j] * (double)rv[j];
+        }
+        X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+        Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
+    }
+    const int ch[2] = {2, 3};
+    block_partials4<LG, 2>(J, s, t.z, ch, v);
+}
+
+template <int LG>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
+                                                           const int4* __restrict__ tiles) {
+    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int s = t.y;
+    if (!slice_active(J, s)) return;
+    const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
+    float b[4];
+    bool zero[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        b[j] = (float)sc_ptr(J, SC_BETA, s)[4 * c4 + j];
+        zero[j] = sc_ptr(J, SC_BZERO, s)[4 * c4 + j] != 0.0;
+    }
+    const size_t base = (size_t)s * J.np * LPR;
+    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
+    const MPF_GLOBAL float4* Rr = (const MPF_GLOBAL float4*)gp((const float*)J.rv) + base;
+    MPF_GLOBAL float4* P = (MPF_GLOBAL float4*)gp(J.p) + base;
+    for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
+        const size_t i = (size_t)row * LPR + c4;
+        const float4 r4 = Rr[i], p4 = P[i];
+        const float d = dinv[row];
+        P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + b[0] * p4.x, zero[1] ? 0.0f : d * r4.y + b[1] * p4.y,
+                           zero[2] ? 0.0f : d * r4.z + b[2] * p4.z, zero[3] ? 0.0f : d * r4.w + b[3] * p4.w);
+    }
+}
+
 // p = z + beta p (rho_old == 0 zeroes p)
 __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate(const JobDev* __restrict__ jobs,
                                                           const int4* __restrict__ tiles) {
@@ -721,9 +833,13 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, cons
     // tickets needs an agent-scope release per block (L2 write-back across XCDs), which
     // measured slower than these launches
     k_pcg_reduce_pq<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
-    k_pcg_update<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     k_pcg_reduce_rz<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
-    k_pcg_pupdate<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     check_launch();
 }
 

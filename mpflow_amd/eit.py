@@ -70,12 +70,20 @@ def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwis
 
 class SolveOptions:
     def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
-                 refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16):
+                 refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0):
         self.s = capi.SolveOpts()
         lib().mpf_solve_opts_default(C.byref(self.s))
         self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
         self.s.precision, self.s.refine_steps, self.s.refine_tol = precision, refine_steps, refine_tol
         self.s.krylov, self.s.check_every = krylov, check_every
+        self.s.warm_start = int(warm_start)
+
+    def copy(self, **kw):
+        o = SolveOptions()
+        C.memmove(C.byref(o.s), C.byref(self.s), C.sizeof(self.s))
+        for k, v in kw.items():
+            setattr(o.s, k, v)
+        return o
 
 
 class EIT:
@@ -190,11 +198,14 @@ class Solver:
         self.calculation = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
         self.last_stats = None
 
-    def solve_absolute(self):
+    def solve_absolute(self, warm_start: bool = False):
+        """One Gauss-Newton step.  ``warm_start`` starts the forward solves from the fields of
+        the previous step (mixed precision only); the first step of a frame should not."""
         st = capi.SolveStats()
+        opts = self.options.copy(warm_start=int(bool(warm_start)))
         rc = lib().mpf_gn_step(self.eit.h, C.c_void_p(self.material_distribution.data_ptr()),
                                C.c_void_p(self.measurement.data_ptr()), self.regularization_factor,
-                               self.gn_refine, C.byref(self.options.s),
+                               self.gn_refine, C.byref(opts.s),
                                C.c_void_p(self.calculation.data_ptr()), C.byref(st))
         check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
         self.last_stats = st

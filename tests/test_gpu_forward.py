@@ -60,6 +60,31 @@ def test_forward_c1_mixed_refinement(gpu_ctx, c1):
     assert err < 1e-7
 
 
+def test_forward_c1_warm_start(gpu_ctx, c1):
+    """A warm-started mixed solve at a new gamma (correction solve of the FP64 residual of
+    the previous fields) reaches the same voltages as the FP64 oracle at that gamma."""
+    torch = _torch()
+    from mpflow_amd.eit import SolveOptions
+    from mpflow_amd import capi
+    p, o, g, U, _ = c1
+    print("case: c1 warm start", flush=True)
+    g1 = g * (1.0 + 0.05 * np.sin(np.arange(g.size)))
+    U1, _, _ = o.solve(g1)
+    eit = eit_of(p, gpu_ctx)
+    kw = dict(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-5, precision=capi.MPF_PREC_MIXED,
+              refine_steps=2, refine_tol=1e-3)
+    eit.solve(torch.tensor(g1, dtype=torch.float32), SolveOptions(**kw))
+    cold = eit.last_stats.steps
+    eit.solve(torch.tensor(g, dtype=torch.float32), SolveOptions(**kw))
+    eit.solve(torch.tensor(g1, dtype=torch.float32), SolveOptions(warm_start=1, **kw))
+    warm = eit.last_stats.steps
+    Ug = eit.voltages().cpu().numpy()[0]
+    err = _rel(Ug, U1)
+    print("rel err", err, "steps cold", cold, "warm", warm)
+    assert err < 1e-7
+    assert warm <= cold
+
+
 def test_forward_batched_frames(gpu_ctx):
     """4 frames with different phantoms in one batched solve equal 4 oracle solves."""
     torch = _torch()
