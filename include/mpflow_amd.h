@@ -133,6 +133,9 @@ typedef struct mpf_model_desc {
                                      1: exact 2.5D derivative (K_e + alpha_c M_e, 1/beta_c) */
     int32_t ell_max_width;        /* 0 -> 32 (reference cap); at most 64            */
     int32_t frames;               /* frames solved together on this mesh (>= 1)     */
+    int32_t keep_node_order;      /* 1: solve in the caller's node numbering; 0: renumber
+                                     internally for gather locality (results are the same
+                                     up to FP32 rounding; outputs never expose node order) */
 } mpf_model_desc;
 
 typedef struct mpf_solve_opts {

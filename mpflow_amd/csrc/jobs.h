@@ -50,6 +50,7 @@ struct JobDev {
     const float* gamma;          // [E]
     double ref; int logarithmic; int E;
     const int32_t* diag_slot;    // [np]
+    const uint8_t* gw;           // [np / 16] entry count of the widest row of each 16-row group
 };
 
 __host__ __device__ inline size_t vec_index(const JobDev& J, int s, int row, int lane) {

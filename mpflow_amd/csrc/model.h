@@ -30,6 +30,7 @@ struct Model {
     int quirks = 1;
     // mesh data on the device
     DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node, colmap;
+    DBuf<uint8_t> gw;
     DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
     DBuf<float> ek, em;
     // per-frame state

@@ -79,6 +79,10 @@ struct MeshSetup {
     // ELL pattern, slot-major: [wpad][np]; padding slots id = row, value 0
     std::vector<int32_t> ids;
     std::vector<int32_t> diag_slot;               // [np]
+    // per 16-row group: the widest row's entry count (the SpMV skips slots beyond it)
+    std::vector<uint8_t> group_width;             // [np / 16]
+    // device numbering: perm[caller node] = device row
+    std::vector<int32_t> perm;
     // gather plan: contributions of (element, local ij) to ELL entry slot*np+row
     std::vector<int32_t> ctb_off;                 // [wpad*np + 1]
     std::vector<int32_t> ctb_elem;                // [E*k*k]

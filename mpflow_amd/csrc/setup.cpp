@@ -82,16 +82,33 @@ void element_integrals(int k, const double* xy, double* K, double* Mm) {
     const double gl[3][2] = {{(y1 - y2) / det, (x2 - x1) / det},
                              {(y2 - y0) / det, (x0 - x2) / det},
                              {(y0 - y1) / det, (x1 - x0) / det}};
-    Poly phi[6];
-    nodal_basis(k, phi);
-    Poly dphi[6][3];
-    for (int i = 0; i < k; ++i) for (int a = 0; a < 3; ++a) dphi[i][a] = deriv(phi[i], a);
+    // the barycentric integrals are element independent up to the area factor: tabulate
+    // them once per element type (unit area)
+    struct Tables {
+        double m[6][6];
+        double kk[6][6][3][3];
+    };
+    static const auto tabulate = [](int kw) {
+        Tables t;
+        Poly phi[6];
+        nodal_basis(kw, phi);
+        Poly dphi[6][3];
+        for (int i = 0; i < kw; ++i) for (int a = 0; a < 3; ++a) dphi[i][a] = deriv(phi[i], a);
+        for (int i = 0; i < kw; ++i) for (int j = 0; j < kw; ++j) {
+            t.m[i][j] = integrate(mul(phi[i], phi[j]), 1.0);
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b)
+                t.kk[i][j][a][b] = integrate(mul(dphi[i][a], dphi[j][b]), 1.0);
+        }
+        return t;
+    };
+    static const Tables tab[2] = {tabulate(3), tabulate(6)};
+    const Tables& T = tab[k == 3 ? 0 : 1];
     for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) {
-        Mm[i * k + j] = integrate(mul(phi[i], phi[j]), area);
+        Mm[i * k + j] = area * T.m[i][j];
         double s = 0.0;
         for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
             double g = gl[a][0] * gl[b][0] + gl[a][1] * gl[b][1];
-            s += g * integrate(mul(dphi[i][a], dphi[j][b]), area);
+            s += g * (area * T.kk[i][j][a][b]);
         }
         K[i * k + j] = s;
     }
@@ -156,11 +173,108 @@ std::vector<std::vector<int32_t>> circular_ports(int N, const double* nodes, int
 // ---------------------------------------------------------------------------
 // full set-up
 // ---------------------------------------------------------------------------
-void build_setup(const mpf_model_desc& d, MeshSetup& s) {
-    MPF_REQUIRE(d.nodes && d.elements && d.boundary, "model: null mesh array");
-    MPF_REQUIRE(d.n_nodes > 0 && d.n_elements > 0, "model: empty mesh");
-    MPF_REQUIRE(d.nodes_per_element == 3 || d.nodes_per_element == 6, "model: nodes_per_element must be 3 or 6");
-    MPF_REQUIRE(d.nodes_per_edge == (d.nodes_per_element == 3 ? 2 : 3), "model: nodes_per_edge does not match the element type");
+// Device node numbering.  Reverse Cuthill-McKee over the element graph (BFS from a
+// pseudo-peripheral node, neighbours visited by ascending degree, order reversed) keeps the
+// neighbours of every 256-row tile inside a narrow band, so the SpMV's gathers of p stay in
+// the XCD's L2; rows are then sorted by descending degree inside each 256-row window so the
+// 16-row groups of the SpMV have near-uniform ELL widths.  Returns perm[node] = row.
+static std::vector<int32_t> device_order(int N, int E, int k, const int32_t* el) {
+    std::vector<int32_t> off(N + 1, 0), adj;
+    {
+        std::vector<std::vector<int32_t>> nb(N);
+        for (int e = 0; e < E; ++e)
+            for (int i = 0; i < k; ++i)
+                for (int j = 0; j < k; ++j)
+                    if (i != j) nb[el[(size_t)e * k + i]].push_back(el[(size_t)e * k + j]);
+        for (int n = 0; n < N; ++n) {
+            std::sort(nb[n].begin(), nb[n].end());
+            nb[n].erase(std::unique(nb[n].begin(), nb[n].end()), nb[n].end());
+            off[n + 1] = off[n] + (int32_t)nb[n].size();
+        }
+        adj.reserve(off[N]);
+        for (int n = 0; n < N; ++n) adj.insert(adj.end(), nb[n].begin(), nb[n].end());
+    }
+    auto deg = [&](int n) { return off[n + 1] - off[n]; };
+    std::vector<int32_t> level(N, -1), order;
+    order.reserve(N);
+    // BFS from `root` over unvisited nodes; returns the nodes in visiting order
+    auto bfs = [&](int root, std::vector<int32_t>& out, std::vector<int32_t>& lv) {
+        out.clear();
+        lv[root] = 0;
+        out.push_back(root);
+        std::vector<int32_t> nbrs;
+        for (size_t h = 0; h < out.size(); ++h) {
+            const int v = out[h];
+            nbrs.assign(adj.begin() + off[v], adj.begin() + off[v + 1]);
+            std::stable_sort(nbrs.begin(), nbrs.end(), [&](int a, int b) { return deg(a) < deg(b); });
+            for (int w : nbrs)
+                if (lv[w] < 0) { lv[w] = lv[v] + 1; out.push_back(w); }
+        }
+    };
+    std::vector<char> done(N, 0);
+    std::vector<int32_t> comp, scratch(N, -1);
+    for (int seed = 0; seed < N; ++seed) {
+        if (done[seed]) continue;
+        // pseudo-peripheral root: repeat BFS from the last level's lowest-degree node
+        int root = seed;
+        int ecc = -1;
+        for (int pass = 0; pass < 4; ++pass) {
+            bfs(root, comp, scratch);
+            const int last = scratch[comp.back()];
+            int best = comp.back();
+            for (int v : comp)
+                if (scratch[v] == last && deg(v) < deg(best)) best = v;
+            for (int v : comp) scratch[v] = -1;
+            if (last <= ecc) break;
+            ecc = last;
+            root = best;
+        }
+        bfs(root, comp, level);
+        for (int v : comp) done[v] = 1;
+        order.insert(order.end(), comp.begin(), comp.end());
+    }
+    std::reverse(order.begin(), order.end());
+    for (size_t w = 0; w < order.size(); w += kTileRows) {
+        auto b = order.begin() + w, e = order.begin() + std::min(order.size(), w + kTileRows);
+        std::stable_sort(b, e, [&](int a, int c) { return deg(a) > deg(c); });
+    }
+    std::vector<int32_t> perm(N);
+    for (int r = 0; r < N; ++r) perm[order[r]] = r;
+    return perm;
+}
+
+void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
+    const mpf_model_desc& c0 = caller;
+    MPF_REQUIRE(c0.nodes && c0.elements && c0.boundary, "model: null mesh array");
+    MPF_REQUIRE(c0.n_nodes > 0 && c0.n_elements > 0, "model: empty mesh");
+    MPF_REQUIRE(c0.nodes_per_element == 3 || c0.nodes_per_element == 6, "model: nodes_per_element must be 3 or 6");
+    MPF_REQUIRE(c0.nodes_per_edge == (c0.nodes_per_element == 3 ? 2 : 3), "model: nodes_per_edge does not match the element type");
+    for (size_t i = 0; i < (size_t)c0.n_elements * c0.nodes_per_element; ++i)
+        MPF_REQUIRE(c0.elements[i] >= 0 && c0.elements[i] < c0.n_nodes, "model: element node index out of range");
+    for (size_t i = 0; i < (size_t)c0.n_boundary * c0.nodes_per_edge; ++i)
+        MPF_REQUIRE(c0.boundary[i] >= 0 && c0.boundary[i] < c0.n_nodes, "model: boundary node index out of range");
+    // renumber the mesh into the device order; everything below works in that numbering
+    mpf_model_desc d = caller;
+    std::vector<double> nodes_r;
+    std::vector<int32_t> el_r, bd_r;
+    if (caller.keep_node_order) {
+        s.perm.resize(caller.n_nodes);
+        for (int n = 0; n < caller.n_nodes; ++n) s.perm[n] = n;
+    } else {
+        s.perm = device_order(caller.n_nodes, caller.n_elements, caller.nodes_per_element, caller.elements);
+        nodes_r.resize((size_t)2 * caller.n_nodes);
+        for (int n = 0; n < caller.n_nodes; ++n) {
+            nodes_r[2 * (size_t)s.perm[n]] = caller.nodes[2 * (size_t)n];
+            nodes_r[2 * (size_t)s.perm[n] + 1] = caller.nodes[2 * (size_t)n + 1];
+        }
+        el_r.resize((size_t)caller.n_elements * caller.nodes_per_element);
+        for (size_t i = 0; i < el_r.size(); ++i) el_r[i] = s.perm[caller.elements[i]];
+        bd_r.resize((size_t)caller.n_boundary * caller.nodes_per_edge);
+        for (size_t i = 0; i < bd_r.size(); ++i) bd_r[i] = s.perm[caller.boundary[i]];
+        d.nodes = nodes_r.data();
+        d.elements = el_r.data();
+        d.boundary = bd_r.data();
+    }
     MPF_REQUIRE(d.n_ports > 0 && d.port_edges, "model: no ports");
     MPF_REQUIRE(d.n_drive > 0 && d.n_meas > 0 && d.drive_pattern && d.meas_pattern && d.drive_current, "model: no sources");
     MPF_REQUIRE(d.components >= 1 && d.components <= kMaxSlices, "model: components must be in [1, 8]");
@@ -219,7 +333,6 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
     MPF_REQUIRE(cap <= 64, "model: ell_max_width > 64");
 
     s.elements.assign(d.elements, d.elements + (size_t)E * k);
-    for (auto v : s.elements) MPF_REQUIRE(v >= 0 && v < N, "model: element node index out of range");
 
     // element integrals
     std::vector<double> Ke((size_t)E * k * k), Me((size_t)E * k * k);
@@ -280,6 +393,9 @@ void build_setup(const mpf_model_desc& d, MeshSetup& s) {
         }
         if (row < N) s.nnz += cnt[row];
     }
+    s.group_width.assign(np / 16, 0);
+    for (int row = 0; row < N; ++row)
+        s.group_width[row / 16] = (uint8_t)std::max<int>(s.group_width[row / 16], cnt[row]);
     // gather plan, contributions in element order per ELL entry
     const size_t nent = (size_t)wp * np;
     s.ctb_off.assign(nent + 1, 0);

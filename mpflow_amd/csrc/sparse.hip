@@ -248,32 +248,29 @@ constexpr int kSpmvRowsPerWave = 16;
 // one buffer_load_dwordx4 instruction fetches 64/(Rp/4) neighbour rows at once.  Row
 // ids/values of the wave's 16 rows sit in lanes 0..15 (coalesced column-major ELL load)
 // and are routed to the lanes of each row with ds_bpermute.
-template <int LG, int WMAX>
-__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict__ jobs,
-                                                        const int4* __restrict__ tiles) {
+// One wavefront's 16 rows with W ELL slots loaded (W >= the group's widest row).
+// Slots whose value is 0 (padding) gather through an offset past the buffer's range,
+// which the buffer unit answers with 0 without a memory access.
+template <int LG, int W>
+__device__ __forceinline__ void spmv4_rows(const JobDev& J, int s, int row0, int l64,
+                                           double (&pq)[4], double (&ps)[4]) {
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;                       // lanes per row
     constexpr int RPS = 64 / LPR;                     // rows per wavefront step
     constexpr int STEPS = kSpmvRowsPerWave / RPS;
-    const int4 t = tiles[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    const int s = t.y;
-    if (!slice_active(J, s)) return;
-    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
     const int rsub = l64 / LPR, c4 = l64 % LPR;
     const int np = J.np;
-    const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
     const float al = J.alpha[s];
     const MPF_GLOBAL int32_t* ids = gp(J.ids);
     const MPF_GLOBAL float* Sv = gp((const float*)J.s);
     const MPF_GLOBAL float* Rv = gp((const float*)J.r);
-    int idr[WMAX];
-    float vr[WMAX];
+    int idr[W];
+    float vr[W];
 #pragma unroll
-    for (int q = 0; q < WMAX; ++q) { idr[q] = 0; vr[q] = 0.0f; }
+    for (int q = 0; q < W; ++q) { idr[q] = 0; vr[q] = 0.0f; }
     if (l64 < kSpmvRowsPerWave) {
 #pragma unroll
-        for (int q = 0; q < WMAX; ++q) {
+        for (int q = 0; q < W; ++q) {
             const size_t e = (size_t)q * np + row0 + l64;
             idr[q] = ids[e];
             vr[q] = Sv[e] + al * Rv[e];
@@ -282,8 +279,8 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict
     const size_t slice_off = (size_t)s * np * Rp;
     const MPF_GLOBAL float4* P4 = (const MPF_GLOBAL float4*)(gp((const float*)J.p) + slice_off);
     MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
-    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + slice_off, (size_t)np * Rp * sizeof(float));
-    double pq[4] = {0.0, 0.0, 0.0, 0.0}, ps[4] = {0.0, 0.0, 0.0, 0.0};
+    const uint32_t range = (uint32_t)((size_t)np * Rp * sizeof(float));
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + slice_off, (size_t)range);
 #pragma unroll
     for (int st = 0; st < STEPS; ++st) {
         const int src = st * RPS + rsub;              // < 16
@@ -291,10 +288,11 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict
         const float4 own = P4[(size_t)row * LPR + c4];
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-        for (int q = 0; q < WMAX; ++q) {
+        for (int q = 0; q < W; ++q) {
             const int id = __builtin_amdgcn_ds_bpermute(src << 2, idr[q]);
             const float v = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(vr[q])));
-            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, (id * Rp + 4 * c4) * 4, 0, 0);
+            const uint32_t off = v != 0.0f ? (uint32_t)(id * Rp + 4 * c4) * 4u : range;
+            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
             acc.x += v * __uint_as_float(raw[0]);
             acc.y += v * __uint_as_float(raw[1]);
             acc.z += v * __uint_as_float(raw[2]);
@@ -305,6 +303,31 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict
         pq[2] += (double)own.z * (double)acc.z; pq[3] += (double)own.w * (double)acc.w;
         ps[0] += (double)own.x; ps[1] += (double)own.y; ps[2] += (double)own.z; ps[3] += (double)own.w;
     }
+}
+
+// 128-bit gathers: a lane owns 4 consecutive RHS columns, Rp/4 lanes cover one row, so
+// one buffer_load_dwordx4 instruction fetches 64/(Rp/4) neighbour rows at once.  Row
+// ids/values of the wave's 16 rows sit in lanes 0..15 (coalesced column-major ELL load)
+// and are routed to the lanes of each row with ds_bpermute.  A wavefront whose 16 rows
+// are all at most 12 entries wide (P2 midpoint rows: 9) runs the 12-slot body.
+template <int LG, int WMAX>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict__ jobs,
+                                                        const int4* __restrict__ tiles) {
+    constexpr int Rp = 1 << LG;
+    constexpr int LPR = Rp / 4;
+    constexpr int WS = WMAX > 12 ? 12 : WMAX;
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int s = t.y;
+    if (!slice_active(J, s)) return;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+    const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
+    // entry count of the widest of the wavefront's 16 rows
+    const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
+    double pq[4] = {0.0, 0.0, 0.0, 0.0}, ps[4] = {0.0, 0.0, 0.0, 0.0};
+    if (WS < WMAX && wl <= WS) spmv4_rows<LG, WS>(J, s, row0, l64, pq, ps);
+    else spmv4_rows<LG, WMAX>(J, s, row0, l64, pq, ps);
     // combine the row sub-groups of the wavefront (same c4), fixed xor tree
 #pragma unroll
     for (int off = LPR; off < 64; off <<= 1) {
@@ -366,7 +389,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
     const MPF_GLOBAL float* P = gp((const float*)J.p) + (size_t)s * np * Rp;
     MPF_GLOBAL float* Q = gp(J.q) + (size_t)s * np * Rp;
     // gathers go through a buffer resource: 32-bit offsets, reads past the slice return 0
-    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + (size_t)s * np * Rp, (size_t)np * Rp * sizeof(float));
+    // without a memory access (used for the zero-valued padding slots)
+    const uint32_t range = (uint32_t)((size_t)np * Rp * sizeof(float));
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + (size_t)s * np * Rp, (size_t)range);
     double pq = 0.0, ps = 0.0;
     for (int i = 0; i < STEPS; i += U) {
         int src[U];
@@ -385,9 +410,12 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
                 const float v = bcast_f<LG>(vr[q], src[u]);
                 float x;
                 if constexpr (LG == 6)   // id is wave-uniform: scalar offset, shared lane offset
-                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4, id * (Rp * 4), 0));
+                    // (the range check applies to the vector offset, so padding moves that)
+                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        rs, v != 0.0f ? (uint32_t)lane * 4u : range, v != 0.0f ? id * (Rp * 4) : 0, 0));
                 else
-                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (id * Rp + lane) * 4, 0, 0));
+                    x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        rs, v != 0.0f ? (uint32_t)(id * Rp + lane) * 4u : range, 0, 0));
                 acc[u] += v * x;
             }
         }

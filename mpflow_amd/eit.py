@@ -91,7 +91,8 @@ class EIT:
 
     def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
-                 quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None):
+                 quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None,
+                 keep_node_order=False):
         import torch
         self.ctx = ctx or Context()
         self.torch = torch
@@ -122,6 +123,7 @@ class EIT:
         d.height, d.port_height = float(height), float(port_height)
         d.logarithmic, d.quirks, d.jacobian_exact = int(logarithmic), int(quirks), int(jacobian_exact)
         d.ell_max_width, d.frames = int(ell_max_width), int(frames)
+        d.keep_node_order = int(bool(keep_node_order))
         h = C.c_void_p()
         check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h

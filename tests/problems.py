@@ -23,8 +23,8 @@ def oracle_of(p, quirks=True):
                      p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks)
 
 
-def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False):
+def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False, keep_node_order=False):
     from mpflow_amd.eit import EIT
     return EIT(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
                p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks,
-               jacobian_exact=jacobian_exact, frames=frames, ctx=ctx)
+               jacobian_exact=jacobian_exact, frames=frames, ctx=ctx, keep_node_order=keep_node_order)

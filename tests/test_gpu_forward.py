@@ -85,6 +85,27 @@ def test_forward_c1_warm_start(gpu_ctx, c1):
     assert warm <= cold
 
 
+@pytest.mark.parametrize("keep", [True, False])
+def test_forward_node_order(gpu_ctx, keep):
+    """The caller's node numbering and the internal locality numbering give the same
+    voltages and Jacobian (2.5D, P2 mesh with midpoints numbered after the vertices)."""
+    torch = _torch()
+    p = make_problem(12, 16, 0.1, C=3, height=0.4, port_height=0.1, p2=True)
+    o = oracle_of(p)
+    g = ellipse_phantom(p["nodes"], p["elements"], frame_seed(2, 1))
+    U, phis, _ = o.solve(g)
+    Jo = o.jacobian(g, phis, quirks=True)
+    print("case: node order keep=%s" % keep, flush=True)
+    eit = eit_of(p, gpu_ctx, keep_node_order=keep)
+    eit.solve(torch.tensor(g, dtype=torch.float32), jacobian=True)
+    Ug = eit.voltages().cpu().numpy()[0]
+    Jg = eit.jacobian_matrix().cpu().numpy()[0]
+    eu = _rel(Ug, U)
+    ej = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
+    print("voltage rel err", eu, "jacobian rel err", ej)
+    assert eu < 1e-5 and ej < 1e-4
+
+
 def test_forward_batched_frames(gpu_ctx):
     """4 frames with different phantoms in one batched solve equal 4 oracle solves."""
     torch = _torch()
