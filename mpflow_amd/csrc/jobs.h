@@ -15,7 +15,7 @@ namespace mpf {
 
 // scalar fields per (slice, lane), FP64
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
-               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_NFIELDS };
+               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_NFIELDS };
 // per-slice integer state: [0] active, [1] steps, [2] hit cap
 constexpr int kStInts = 4;
 // partial-sum channels per tile
@@ -73,9 +73,10 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                      const int2* d_js, int njs, int* d_active, hipStream_t st);
 // ev0/ev1 (nullable) are recorded right before and after the SpMV launch
-void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                     const int2* d_js, int njs, int* d_active, int lg, int wpad, hipStream_t st,
-                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// one PCG iteration (SpMV, update, pupdate); `iter` counts iterations since the
+// solve's init (its parity selects the rho slot)
+void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
+                     int lg, int wpad, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                        const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,

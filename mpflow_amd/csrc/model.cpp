@@ -143,9 +143,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     xcd_order(ht.data(), ntiles, hto.data());
     tiles.alloc(ntiles); tiles.upload(hto.data(), hto.size(), st0);
     js.alloc(njs); js.upload(hj.data(), hj.size(), st0);
-    act_tiles.alloc(ntiles); act_js.alloc(njs);
+    act_tiles.alloc(ntiles);
     MPF_HIP(hipHostMalloc((void**)&h_tiles, sizeof(int4) * 2 * (size_t)ntiles, hipHostMallocDefault));
-    MPF_HIP(hipHostMalloc((void**)&h_js, sizeof(int2) * njs, hipHostMallocDefault));
     MPF_HIP(hipHostMalloc((void**)&h_st, sizeof(int) * (size_t)S.C * kStInts * frames, hipHostMallocDefault));
     asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
     MPF_HIP(hipHostMalloc((void**)&h_active, sizeof(int), hipHostMallocDefault));
@@ -155,7 +154,6 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
 Model::~Model() {
     if (h_active) (void)hipHostFree(h_active);
     if (h_tiles) (void)hipHostFree(h_tiles);
-    if (h_js) (void)hipHostFree(h_js);
     if (h_st) (void)hipHostFree(h_st);
 }
 
@@ -182,15 +180,13 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     // work lists restricted to the (frame, harmonic) pairs still iterating; rebuilt at
     // every poll so converged harmonics cost no blocks
     const int4* cur_tiles = tiles.p;
-    const int2* cur_js = js.p;
-    int cur_nt = ntiles, cur_njs = njs;
+    int cur_nt = ntiles;
     int it = 0;
     while (true) {
         for (int i = 0; i < every; ++i) {
             std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
             if (ctx->prof) ev = ctx->next_events();
-            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, cur_js, cur_njs, active.p, lg, wpad, st0,
-                            ev.first, ev.second);
+            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
         }
         it += every;
         MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
@@ -198,11 +194,10 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
         MPF_HIP(hipStreamSynchronize(st0));
         if (ctx->prof) ctx->collect_events();
         if (*h_active <= 0 || it >= maxit + every) break;
-        int nt = 0, nj = 0;
+        int nt = 0;
         for (int f = 0; f < frames; ++f)
             for (int c = 0; c < S.C; ++c) {
                 if (!h_st[((size_t)f * S.C + c) * kStInts]) continue;
-                h_js[nj++] = make_int2(f, c);
                 for (int t = 0; t < ntile_job; ++t) h_tiles[nt++] = make_int4(f, c, t, 0);
             }
         if (nt == 0) break;
@@ -210,9 +205,8 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
             int4* ordered = h_tiles + ntiles;   // second half of the pinned staging buffer
             xcd_order(h_tiles, nt, ordered);
             MPF_HIP(hipMemcpyAsync(act_tiles.p, ordered, sizeof(int4) * nt, hipMemcpyHostToDevice, st0));
-            MPF_HIP(hipMemcpyAsync(act_js.p, h_js, sizeof(int2) * nj, hipMemcpyHostToDevice, st0));
-            // h_tiles / h_js are rewritten only after the next poll's synchronisation
-            cur_tiles = act_tiles.p; cur_js = act_js.p; cur_nt = nt; cur_njs = nj;
+            // h_tiles is rewritten only after the next poll's synchronisation
+            cur_tiles = act_tiles.p; cur_nt = nt;
         }
     }
     // step counts

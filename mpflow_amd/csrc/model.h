@@ -47,9 +47,7 @@ struct Model {
     DBuf<int> active;
     int* h_active = nullptr;   // pinned
     DBuf<int4> act_tiles;      // work lists of the (frame, harmonic) pairs still active
-    DBuf<int2> act_js;
     int4* h_tiles = nullptr;   // pinned staging of the active lists
-    int2* h_js = nullptr;
     int* h_st = nullptr;       // pinned copy of the per-pair state
     // Gauss-Newton buffers (allocated on first use)
     DBuf<float> Jm, G;

@@ -143,8 +143,6 @@ __device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* 
     }
 }
 
-__device__ void finish_pq(const JobDev& J, int s);
-__device__ void finish_rz(const JobDev& J, int s, int* active);
 
 // streaming-kernel lane mapping (kEwBlock threads per 256-row tile)
 struct Ew {
@@ -431,125 +429,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
     block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
 
-// p.q and 1^T p over all tiles -> alpha (run by one kEwBlock-thread block per pair)
-__device__ void finish_pq(const JobDev& J, int s) {
-    const int ch[2] = {0, 1};
-    double out[2];
-    reduce_tiles<2>(J, s, ch, out);
-    if ((int)threadIdx.x < J.Rp) {
-        const int lane = threadIdx.x;
-        double pq = out[0];
-        const double ps = out[1];
-        // with dcFree, q = A p + 1 (1^T p), hence p.q = p.Ap + (1^T p)^2
-        if (s == J.dc) pq += ps * ps;
-        const double rho = sc_ptr(J, SC_RHO, s)[lane];
-        sc_ptr(J, SC_PQ, s)[lane] = pq;
-        sc_ptr(J, SC_PSUM, s)[lane] = (s == J.dc) ? ps : 0.0;
-        sc_ptr(J, SC_ALPHA, s)[lane] = pq != 0.0 ? rho / pq : 0.0;
-    }
-}
-
-__global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_pq(const JobDev* __restrict__ jobs,
-                                                            const int2* __restrict__ js) {
-    const int2 t = js[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    if (!slice_active(J, t.y)) return;
-    finish_pq(J, t.y);
-}
-
-// x += alpha p, r -= alpha q (a zero p.q zeroes both, updateVectorKernel
-// conjugate_gradient_kernel.cu:77-89); partials r.z and r.r with z = D^-1 r
-__global__ void __launch_bounds__(kEwBlock) k_pcg_update(const JobDev* __restrict__ jobs,
-                                                         const int4* __restrict__ tiles) {
-    const int4 t = tiles[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    const int s = t.y;
-    if (!slice_active(J, s)) return;
-    const Ew L = ew_of(J);
-    const double pq = sc_ptr(J, SC_PQ, s)[L.lane];
-    const float a = (float)sc_ptr(J, SC_ALPHA, s)[L.lane];
-    const float dcv = (float)sc_ptr(J, SC_PSUM, s)[L.lane];
-    const bool zero = (pq == 0.0);
-    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
-    const MPF_GLOBAL float* P = gp((const float*)J.p);
-    const MPF_GLOBAL float* Q = gp((const float*)J.q);
-    MPF_GLOBAL float* X = gp(J.x);
-    MPF_GLOBAL float* Rr = gp(J.rv);
-    double rz = 0.0, rr = 0.0;
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
-        const size_t i = vec_index(J, s, row, L.lane);
-        const float pv = P[i], qv = Q[i] + (row < J.n ? dcv : 0.0f);
-        float xv = X[i], rv = Rr[i];
-        if (zero) { xv = 0.0f; rv = 0.0f; }
-        else { xv = xv + a * pv; rv = rv - a * qv; }
-        X[i] = xv;
-        Rr[i] = rv;
-        const float z = dinv[row] * rv;
-        rz += (double)rv * (double)z;
-        rr += (double)rv * (double)rv;
-    }
-    const int ch[2] = {2, 3};
-    const double v[2] = {rz, rr};
-    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
-}
-
-// rho = r.z, convergence test, beta = rho / rho_old
-__device__ void finish_rz(const JobDev& J, int s, int* active) {
-    const int ch[2] = {2, 3};
-    double out[2];
-    reduce_tiles<2>(J, s, ch, out);
-    __shared__ int not_done;
-    if (threadIdx.x == 0) not_done = 0;
-    __syncthreads();
-    const int lane = threadIdx.x;
-    const bool owner = lane < J.Rp;
-    double rho = 0.0;
-    if (owner) {
-        rho = out[0];
-        const double rr = out[1];
-        const double bb = sc_ptr(J, SC_BB, s)[lane];
-        sc_ptr(J, SC_RR, s)[lane] = rr;
-        if (J.rule == MPF_STOP_REFERENCE) {
-            if (lane == J.col0) {
-                // column 0 of the pattern only: sqrt(rho / ref) >= eps keeps iterating;
-                // NaN (ref == 0) stops
-                const double crit = fabs(sqrt(rho / bb));
-                if (crit >= (double)FLT_EPSILON) not_done = 1;
-            }
-        } else if (lane < J.R) {
-            if (bb > 0.0 && rr > J.tol * J.tol * bb) atomicOr(&not_done, 1);
-        }
-    }
-    __syncthreads();
-    if (owner) {
-        const double rho_old = sc_ptr(J, SC_RHO, s)[lane];
-        sc_ptr(J, SC_BETA, s)[lane] = rho_old != 0.0 ? rho / rho_old : 0.0;
-        sc_ptr(J, SC_BZERO, s)[lane] = rho_old != 0.0 ? 0.0 : 1.0;
-        sc_ptr(J, SC_RHO, s)[lane] = rho;
-    }
-    if (threadIdx.x == 0) {
-        const int steps = J.st[s * kStInts + 1] + 1;
-        J.st[s * kStInts + 1] = steps;
-        if (!not_done) {
-            J.st[s * kStInts + 0] = 0;
-            atomicSub(active, 1);
-        } else if (steps >= J.maxit) {
-            J.st[s * kStInts + 0] = 0;
-            J.st[s * kStInts + 2] = 1;
-            atomicSub(active, 1);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kEwBlock) k_pcg_reduce_rz(const JobDev* __restrict__ jobs,
-                                                            const int2* __restrict__ js,
-                                                            int* __restrict__ active) {
-    const int2 t = js[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    if (!slice_active(J, t.y)) return;
-    finish_rz(J, t.y, active);
-}
-
 // ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
 // Combine per-lane FP64 values of 4 columns across the row sub-groups of each
 // wavefront (fixed xor tree) and across the 16 wavefronts (fixed order), then write
@@ -583,23 +462,64 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
     }
 }
 
+// rho of iteration `par` parity: r.z of the current residual lives in one of two slots so
+// that the kernel computing the next rho never overwrites the one its other blocks read
+__device__ __forceinline__ int rho_slot(int par) { return par ? SC_RHO1 : SC_RHO; }
+
+// Iteration i runs three launches: SpMV (tile partials of p.q and 1^T p), update and
+// pupdate.  Each update / pupdate block sums the partials it needs over all tiles of
+// its (job, slice) itself, in the same order as every other block (bit-identical
+// scalars, no cross-block synchronisation); tile 0's block also keeps the solver state.
+//
+// update: alpha = rho / p.q (with dcFree q = A p + 1 1^T p, so p.q += (1^T p)^2);
+// x += alpha p, r -= alpha q (a zero p.q zeroes both, updateVectorKernel
+// conjugate_gradient_kernel.cu:77-89); tile partials of r.z (z = D^-1 r) and r.r.
 template <int LG>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
-                                                          const int4* __restrict__ tiles) {
+                                                          const int4* __restrict__ tiles, int par) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
+    __shared__ double sh_al[Rp], sh_dc[Rp];
+    __shared__ int sh_zero[Rp];
+    {
+        double out[2] = {0.0, 0.0};
+        if (s == J.dc) {
+            const int ch[2] = {0, 1};
+            reduce_tiles<2>(J, s, ch, out);
+        } else {
+            const int ch[1] = {0};
+            double o[1];
+            reduce_tiles<1>(J, s, ch, o);
+            out[0] = o[0];
+        }
+        if ((int)threadIdx.x < Rp) {
+            const int lane = threadIdx.x;
+            const double ps = s == J.dc ? out[1] : 0.0;
+            const double pq = out[0] + ps * ps;
+            const double al = pq != 0.0 ? sc_ptr(J, rho_slot(par), s)[lane] / pq : 0.0;
+            sh_al[lane] = al;
+            sh_dc[lane] = ps;
+            sh_zero[lane] = pq == 0.0;
+            if (t.z == 0) {
+                sc_ptr(J, SC_PQ, s)[lane] = pq;
+                sc_ptr(J, SC_PSUM, s)[lane] = ps;
+                sc_ptr(J, SC_ALPHA, s)[lane] = al;
+            }
+        }
+        __syncthreads();
+    }
     const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
     float a[4], dcv[4];
     bool zero[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = 4 * c4 + j;
-        zero[j] = sc_ptr(J, SC_PQ, s)[col] == 0.0;
-        a[j] = (float)sc_ptr(J, SC_ALPHA, s)[col];
-        dcv[j] = (float)sc_ptr(J, SC_PSUM, s)[col];
+        zero[j] = sh_zero[col] != 0;
+        a[j] = (float)sh_al[col];
+        dcv[j] = (float)sh_dc[col];
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
@@ -631,21 +551,81 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     block_partials4<LG, 2>(J, s, t.z, ch, v);
 }
 
+// pupdate: rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p.  Tile 0's
+// block also sums r.r, applies the stop rule (ConjugateGradient::solve,
+// src/numeric/conjugate_gradient.cpp:118-150) and counts the step.  Other blocks of the
+// same launch may see the slice leave mid-launch and skip it: p is not read again.
 template <int LG>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
-                                                           const int4* __restrict__ tiles) {
+                                                           const int4* __restrict__ tiles, int par,
+                                                           int* __restrict__ active) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
+    const bool book = t.z == 0;
+    __shared__ double sh_b[Rp];
+    __shared__ int sh_bz[Rp];
+    __shared__ int not_done;
+    {
+        double out[2] = {0.0, 0.0};
+        if (book) {
+            const int ch[2] = {2, 3};
+            reduce_tiles<2>(J, s, ch, out);
+        } else {
+            const int ch[1] = {2};
+            double o[1];
+            reduce_tiles<1>(J, s, ch, o);
+            out[0] = o[0];
+        }
+        if (threadIdx.x == 0) not_done = 0;
+        __syncthreads();
+        if ((int)threadIdx.x < Rp) {
+            const int lane = threadIdx.x;
+            const double rho = out[0];
+            const double rho_old = sc_ptr(J, rho_slot(par), s)[lane];
+            sh_b[lane] = rho_old != 0.0 ? rho / rho_old : 0.0;
+            sh_bz[lane] = rho_old == 0.0;
+            if (book) {
+                const double rr = out[1];
+                const double bb = sc_ptr(J, SC_BB, s)[lane];
+                sc_ptr(J, SC_RR, s)[lane] = rr;
+                sc_ptr(J, rho_slot(par ^ 1), s)[lane] = rho;
+                sc_ptr(J, SC_BETA, s)[lane] = sh_b[lane];
+                if (J.rule == MPF_STOP_REFERENCE) {
+                    if (lane == J.col0) {
+                        // column 0 of the pattern only: sqrt(rho / ref) >= eps keeps
+                        // iterating; NaN (ref == 0) stops
+                        const double crit = fabs(sqrt(rho / bb));
+                        if (crit >= (double)FLT_EPSILON) not_done = 1;
+                    }
+                } else if (lane < J.R) {
+                    if (bb > 0.0 && rr > J.tol * J.tol * bb) atomicOr(&not_done, 1);
+                }
+            }
+        }
+        __syncthreads();
+        if (book && threadIdx.x == 0) {
+            const int steps = J.st[s * kStInts + 1] + 1;
+            J.st[s * kStInts + 1] = steps;
+            if (!not_done) {
+                J.st[s * kStInts + 0] = 0;
+                atomicSub(active, 1);
+            } else if (steps >= J.maxit) {
+                J.st[s * kStInts + 0] = 0;
+                J.st[s * kStInts + 2] = 1;
+                atomicSub(active, 1);
+            }
+        }
+    }
     const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
     float b[4];
     bool zero[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        b[j] = (float)sc_ptr(J, SC_BETA, s)[4 * c4 + j];
-        zero[j] = sc_ptr(J, SC_BZERO, s)[4 * c4 + j] != 0.0;
+        b[j] = (float)sh_b[4 * c4 + j];
+        zero[j] = sh_bz[4 * c4 + j] != 0;
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
@@ -657,25 +637,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
         const float d = dinv[row];
         P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + b[0] * p4.x, zero[1] ? 0.0f : d * r4.y + b[1] * p4.y,
                            zero[2] ? 0.0f : d * r4.z + b[2] * p4.z, zero[3] ? 0.0f : d * r4.w + b[3] * p4.w);
-    }
-}
-
-// p = z + beta p (rho_old == 0 zeroes p)
-__global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate(const JobDev* __restrict__ jobs,
-                                                          const int4* __restrict__ tiles) {
-    const int4 t = tiles[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    const int s = t.y;
-    if (!slice_active(J, s)) return;
-    const Ew L = ew_of(J);
-    const float b = (float)sc_ptr(J, SC_BETA, s)[L.lane];
-    const bool zero = sc_ptr(J, SC_BZERO, s)[L.lane] != 0.0;
-    const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
-    const MPF_GLOBAL float* Rr = gp((const float*)J.rv);
-    MPF_GLOBAL float* P = gp(J.p);
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
-        const size_t i = vec_index(J, s, row, L.lane);
-        P[i] = zero ? 0.0f : dinv[row] * Rr[i] + b * P[i];
     }
 }
 
@@ -848,26 +809,21 @@ static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, i
     }
 }
 
-void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                     int njs, int* d_active, int lg, int wpad, hipStream_t st, hipEvent_t ev0,
-                     hipEvent_t ev1) {
+void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
+                     int lg, int wpad, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    const int par = iter & 1;
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
     if (lg == 6) spmv_dispatch<6>(wpad, d_jobs, d_tiles, ntiles, st);
     else if (lg == 5) spmv_dispatch<5>(wpad, d_jobs, d_tiles, ntiles, st);
     else if (lg == 4) spmv_dispatch<4>(wpad, d_jobs, d_tiles, ntiles, st);
     else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
-    // separate one-block reductions: folding them into the producers with last-block
-    // tickets needs an agent-scope release per block (L2 write-back across XCDs), which
-    // measured slower than these launches
-    k_pcg_reduce_pq<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
-    if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    k_pcg_reduce_rz<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
-    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
+    else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
+    else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
+    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     check_launch();
 }
 
