@@ -1,0 +1,242 @@
+// Device helpers shared by the Krylov kernels (PCG in sparse.hip, BiCGSTAB in
+// bicgstab.hip): tile partials and their fixed-order reductions, buffer resources and
+// the 128-bit multi-RHS ELL gather core of the SpMV.
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+
+#include "jobs.h"
+
+namespace mpf {
+
+__device__ __forceinline__ MPF_GLOBAL double* part_ptr(const JobDev& J, int ch, int s, int tile) {
+    return gp(J.part) + (((size_t)ch * J.C + s) * J.ntiles + tile) * J.Rp;
+}
+
+__device__ __forceinline__ MPF_GLOBAL double* sc_ptr(const JobDev& J, int f, int s) {
+    return gp(J.sc) + ((size_t)f * J.C + s) * J.Rp;
+}
+
+__device__ __forceinline__ bool slice_active(const JobDev& J, int s) {
+    return gp(J.st)[s * kStInts] != 0;
+}
+
+// Combine per-thread values of NV channels across the row groups of an NT-thread block
+// (thread = group * Rp + lane) in a fixed order and write them as this tile's partials.
+template <int NV, int NT>
+__device__ __forceinline__ void block_partials(const JobDev& J, int s, int tile, const int* ch,
+                                               const double (&v)[NV]) {
+    __shared__ double red[NV][NT];
+    const int Rp = J.Rp, lg = J.lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = NT >> lg;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double a = red[c][lane];
+            for (int gg = 1; gg < G; ++gg) a += red[c][gg * Rp + lane];
+            part_ptr(J, ch[c], s, tile)[lane] = a;
+        }
+    }
+}
+
+// Sum NV partial channels over all tiles of (job, slice) in tile order with a
+// kEwBlock-thread block; each row group sums a contiguous range of tiles (loads
+// unrolled so many are in flight), then the groups are combined in order.  The result
+// is valid in threads with g == 0 (lane = column).
+template <int NV>
+__device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV]) {
+    __shared__ double red[NV][kEwBlock];
+    const int Rp = J.Rp, lg = J.lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
+    double a[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = 0.0;
+    const int per = (J.ntiles + G - 1) / G;
+    const int t0 = g * per, t1 = min(J.ntiles, t0 + per);
+    int t = t0;
+    for (; t + 8 <= t1; t += 8) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            const MPF_GLOBAL double* pp = part_ptr(J, ch[c], s, t) + lane;
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = pp[(size_t)u * Rp];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[c] += v[u];
+        }
+    }
+    for (; t < t1; ++t) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) a[c] += part_ptr(J, ch[c], s, t)[lane];
+    }
+#pragma unroll
+    for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = a[c];
+    __syncthreads();
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double r = red[c][lane];
+            for (int gg = 1; gg < G; ++gg) r += red[c][gg * Rp + lane];
+            out[c] = r;
+        }
+    }
+}
+
+
+// streaming-kernel lane mapping (kEwBlock threads per 256-row tile)
+struct Ew {
+    int lane, g, G;
+};
+__device__ __forceinline__ Ew ew_of(const JobDev& J) {
+    Ew e;
+    e.lane = threadIdx.x & (J.Rp - 1);
+    e.g = threadIdx.x >> J.lg;
+    e.G = kEwBlock >> J.lg;
+    return e;
+}
+
+// 4th dword of a raw buffer resource on gfx9-family GPUs (32-bit data, no swizzle)
+constexpr int kBufferWord3 = 0x00020000;
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// Buffer resource built from wave-uniform values (readfirstlane keeps it in SGPRs, so
+// buffer loads need no per-lane waterfall).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void* base, size_t bytes) {
+    const uint64_t b = (uint64_t)base;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    const int n = __builtin_amdgcn_readfirstlane((int)(bytes > 0x7fffffffu ? 0x7fffffffu : bytes));
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0, n, kBufferWord3);
+}
+
+// broadcast a per-row register value held by lane `src` to the lanes of a row group
+template <int LG>
+__device__ __forceinline__ int bcast_i(int v, int src) {
+    if constexpr (LG == 6) return __builtin_amdgcn_readlane(v, src);
+    else return __builtin_amdgcn_ds_bpermute(src << 2, v);
+}
+template <int LG>
+__device__ __forceinline__ float bcast_f(float v, int src) {
+    return __int_as_float(bcast_i<LG>(__float_as_int(v), src));
+}
+
+// rows per wavefront in the SpMV: a 1024-thread block = 16 wavefronts x 16 rows = one
+// 256-row tile
+constexpr int kSpmvRowsPerWave = 16;
+
+// Multi-RHS ELL gather of one wavefront's 16 rows: y_row = sum_q (S + alpha_s R)[row, q]
+// X[id(row, q)] for the slice's Rp columns.  A lane owns 4 consecutive columns and Rp/4
+// lanes cover one row, so one buffer_load_dwordx4 fetches 64/(Rp/4) neighbour rows.  The
+// wavefront's ELL entries (id, value) sit in LDS as [slot][row]; every lane of a row reads
+// the same entry (broadcast).  W (compile time, >= the group's widest row) fixes the
+// unrolled slot loop.  Zero-valued slots (padding) gather through an offset past the
+// buffer's range, which the buffer unit answers with 0 without a memory access.
+// epi(row, c4, acc) consumes each row's 4-column result.
+template <int LG, int W, class Epi>
+__device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l64, const float* X,
+                                                const int2 (*ell)[kSpmvRowsPerWave], Epi&& epi) {
+    constexpr int Rp = 1 << LG;
+    constexpr int LPR = Rp / 4;                       // lanes per row
+    constexpr int RPS = 64 / LPR;                     // rows per wavefront step
+    constexpr int STEPS = kSpmvRowsPerWave / RPS;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+    const uint32_t range = (uint32_t)((size_t)J.np * Rp * sizeof(float));
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(X, (size_t)range);
+    // slots in groups of 8 so that at most 8 gathers (32 VGPRs) are in flight per lane,
+    // which keeps the kernel at 8 wavefronts per SIMD
+#pragma unroll 1
+    for (int st = 0; st < STEPS; ++st) {
+        const int src = st * RPS + rsub;              // < 16
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 8
+        for (int q = 0; q < W; ++q) {
+            const int2 e = ell[q][src];
+            const float v = __int_as_float(e.y);
+            const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
+            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            acc.x += v * __uint_as_float(raw[0]);
+            acc.y += v * __uint_as_float(raw[1]);
+            acc.z += v * __uint_as_float(raw[2]);
+            acc.w += v * __uint_as_float(raw[3]);
+        }
+        epi(row0 + src, c4, acc);
+    }
+}
+
+// LDS for ell_gather_wave: one [WMAX][16] (id, value) table per wavefront of the block
+template <int WMAX>
+using EllStage = int2[kEwBlock / 64][WMAX][kSpmvRowsPerWave];
+
+// One wavefront's 16 rows of y = A_s X.  The slot count is binned by the wavefront's group
+// width: a group whose rows are all at most 12 entries wide (P2 midpoint rows: 9) runs the
+// 12-slot body.  The wavefront first stages its (id, S + alpha_s R) entries in LDS with
+// coalesced loads spread over all 64 lanes; slots between the group width and the bin are
+// zero.  Contains a block barrier: every wavefront of the block must call it.
+template <int LG, int WMAX, class Epi>
+__device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0, int l64, const float* X,
+                                                EllStage<WMAX>& stage, Epi&& epi) {
+    constexpr int WS = WMAX > 12 ? 12 : WMAX;
+    const int wave = threadIdx.x >> 6;
+    const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
+    const int wb = (WS < WMAX && wl <= WS) ? WS : WMAX;
+    const int np = J.np;
+    const float al = J.alpha[s];
+    const MPF_GLOBAL int32_t* ids = gp(J.ids);
+    const MPF_GLOBAL float* Sv = gp((const float*)J.s);
+    const MPF_GLOBAL float* Rv = gp((const float*)J.r);
+    for (int e = l64; e < wb * kSpmvRowsPerWave; e += 64) {
+        const int q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
+        int2 v = make_int2(0, 0);
+        if (q < wl) {
+            const size_t g = (size_t)q * np + row0 + r;
+            v = make_int2(ids[g], __float_as_int(Sv[g] + al * Rv[g]));
+        }
+        stage[wave][q][r] = v;
+    }
+    __syncthreads();
+    if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS>(J, row0, l64, X, stage[wave], epi);
+    else ell_gather_rows<LG, WMAX>(J, row0, l64, X, stage[wave], epi);
+}
+
+// ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
+// Combine per-lane FP64 values of 4 columns across the row sub-groups of each
+// wavefront (fixed xor tree) and across the 16 wavefronts (fixed order), then write
+// the tile partials of NV channels.
+template <int LG, int NV>
+__device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile, const int* ch,
+                                                double (&v)[NV][4]) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
+    __shared__ double red[NV][kEwBlock / 64][64];
+    if (rsub == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[c][wave][4 * c4 + j] = v[c][j];
+    __syncthreads();
+    if ((int)threadIdx.x < (1 << LG)) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double a = 0.0;
+            for (int w = 0; w < kEwBlock / 64; ++w) a += red[c][w][threadIdx.x];
+            part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
+        }
+    }
+}
+
+// rho of iteration `par` parity: r.z of the current residual lives in one of two slots so
+// that the kernel computing the next rho never overwrites the one its other blocks read
+__device__ __forceinline__ int rho_slot(int par) { return par ? SC_RHO1 : SC_RHO; }
+
+}  // namespace mpf

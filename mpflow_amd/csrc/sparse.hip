@@ -13,10 +13,7 @@
 // Dot products are accumulated in FP64 per lane, combined across row groups and
 // wavefronts in a fixed order into one partial per tile, and summed per (job, slice)
 // by a one-block reduction in tile order: every scalar is deterministic run to run.
-#include <cfloat>
-#include <cmath>
-
-#include "jobs.h"
+#include "krylov_dev.h"
 
 namespace mpf {
 
@@ -62,98 +59,6 @@ __global__ void __launch_bounds__(kBlock) k_dinv(const JobDev* __restrict__ jobs
         v = a != 0.0f ? 1.0f / a : 0.0f;
     }
     J.dinv[(size_t)s * J.np + row] = v;
-}
-
-// --------------------------------------------------------------------------------
-// helpers
-// --------------------------------------------------------------------------------
-__device__ __forceinline__ MPF_GLOBAL double* part_ptr(const JobDev& J, int ch, int s, int tile) {
-    return gp(J.part) + (((size_t)ch * J.C + s) * J.ntiles + tile) * J.Rp;
-}
-
-__device__ __forceinline__ MPF_GLOBAL double* sc_ptr(const JobDev& J, int f, int s) {
-    return gp(J.sc) + ((size_t)f * J.C + s) * J.Rp;
-}
-
-__device__ __forceinline__ bool slice_active(const JobDev& J, int s) {
-    return gp(J.st)[s * kStInts] != 0;
-}
-
-// Combine per-thread values of NV channels across the row groups of an NT-thread block
-// (thread = group * Rp + lane) in a fixed order and write them as this tile's partials.
-template <int NV, int NT>
-__device__ __forceinline__ void block_partials(const JobDev& J, int s, int tile, const int* ch,
-                                               const double (&v)[NV]) {
-    __shared__ double red[NV][NT];
-    const int Rp = J.Rp, lg = J.lg;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = NT >> lg;
-#pragma unroll
-    for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = v[c];
-    __syncthreads();
-    if (g == 0) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) {
-            double a = red[c][lane];
-            for (int gg = 1; gg < G; ++gg) a += red[c][gg * Rp + lane];
-            part_ptr(J, ch[c], s, tile)[lane] = a;
-        }
-    }
-}
-
-// Sum NV partial channels over all tiles of (job, slice) in tile order with a
-// kEwBlock-thread block; each row group sums a contiguous range of tiles (loads
-// unrolled so many are in flight), then the groups are combined in order.  The result
-// is valid in threads with g == 0 (lane = column).
-template <int NV>
-__device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV]) {
-    __shared__ double red[NV][kEwBlock];
-    const int Rp = J.Rp, lg = J.lg;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
-    double a[NV];
-#pragma unroll
-    for (int c = 0; c < NV; ++c) a[c] = 0.0;
-    const int per = (J.ntiles + G - 1) / G;
-    const int t0 = g * per, t1 = min(J.ntiles, t0 + per);
-    int t = t0;
-    for (; t + 8 <= t1; t += 8) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) {
-            const MPF_GLOBAL double* pp = part_ptr(J, ch[c], s, t) + lane;
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = pp[(size_t)u * Rp];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) a[c] += v[u];
-        }
-    }
-    for (; t < t1; ++t) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) a[c] += part_ptr(J, ch[c], s, t)[lane];
-    }
-#pragma unroll
-    for (int c = 0; c < NV; ++c) red[c][threadIdx.x] = a[c];
-    __syncthreads();
-    if (g == 0) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) {
-            double r = red[c][lane];
-            for (int gg = 1; gg < G; ++gg) r += red[c][gg * Rp + lane];
-            out[c] = r;
-        }
-    }
-}
-
-
-// streaming-kernel lane mapping (kEwBlock threads per 256-row tile)
-struct Ew {
-    int lane, g, G;
-};
-__device__ __forceinline__ Ew ew_of(const JobDev& J) {
-    Ew e;
-    e.lane = threadIdx.x & (J.Rp - 1);
-    e.g = threadIdx.x >> J.lg;
-    e.G = kEwBlock >> J.lg;
-    return e;
 }
 
 // --------------------------------------------------------------------------------
@@ -209,148 +114,34 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
     }
 }
 
-// 4th dword of a raw buffer resource on gfx9-family GPUs (32-bit data, no swizzle)
-constexpr int kBufferWord3 = 0x00020000;
-typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-
-// Buffer resource built from wave-uniform values (readfirstlane keeps it in SGPRs, so
-// buffer loads need no per-lane waterfall).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void* base, size_t bytes) {
-    const uint64_t b = (uint64_t)base;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
-    const int n = __builtin_amdgcn_readfirstlane((int)(bytes > 0x7fffffffu ? 0x7fffffffu : bytes));
-    return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0, n, kBufferWord3);
-}
-
-// broadcast a per-row register value held by lane `src` to the lanes of a row group
-template <int LG>
-__device__ __forceinline__ int bcast_i(int v, int src) {
-    if constexpr (LG == 6) return __builtin_amdgcn_readlane(v, src);
-    else return __builtin_amdgcn_ds_bpermute(src << 2, v);
-}
-template <int LG>
-__device__ __forceinline__ float bcast_f(float v, int src) {
-    return __int_as_float(bcast_i<LG>(__float_as_int(v), src));
-}
-
-// q = A_s p (+ 1 1^T p when dcFree); partials p.q and sum(p).
-// LG = log2(Rp); WMAX = padded ELL width (compile-time, so the slot registers are
-// statically indexed); U = rows gathered concurrently per wavefront.
-// Block = 16 wavefronts x 16 rows = one 256-row tile.  Lanes 0..15 of a wavefront load
-// the column-major ELL of its 16 rows (coalesced per slot); each row's ids/values are
-// then broadcast so the RHS lanes gather contiguous Rp-float rows of p.
-constexpr int kSpmvRowsPerWave = 16;
-
-// 128-bit gathers: a lane owns 4 consecutive RHS columns, Rp/4 lanes cover one row, so
-// one buffer_load_dwordx4 instruction fetches 64/(Rp/4) neighbour rows at once.  Row
-// ids/values of the wave's 16 rows sit in lanes 0..15 (coalesced column-major ELL load)
-// and are routed to the lanes of each row with ds_bpermute.
-// One wavefront's 16 rows with W ELL slots loaded (W >= the group's widest row).
-// Slots whose value is 0 (padding) gather through an offset past the buffer's range,
-// which the buffer unit answers with 0 without a memory access.
-template <int LG, int W>
-__device__ __forceinline__ void spmv4_rows(const JobDev& J, int s, int row0, int l64,
-                                           double (&pq)[4], double (&ps)[4]) {
-    constexpr int Rp = 1 << LG;
-    constexpr int LPR = Rp / 4;                       // lanes per row
-    constexpr int RPS = 64 / LPR;                     // rows per wavefront step
-    constexpr int STEPS = kSpmvRowsPerWave / RPS;
-    const int rsub = l64 / LPR, c4 = l64 % LPR;
-    const int np = J.np;
-    const float al = J.alpha[s];
-    const MPF_GLOBAL int32_t* ids = gp(J.ids);
-    const MPF_GLOBAL float* Sv = gp((const float*)J.s);
-    const MPF_GLOBAL float* Rv = gp((const float*)J.r);
-    int idr[W];
-    float vr[W];
-#pragma unroll
-    for (int q = 0; q < W; ++q) { idr[q] = 0; vr[q] = 0.0f; }
-    if (l64 < kSpmvRowsPerWave) {
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            const size_t e = (size_t)q * np + row0 + l64;
-            idr[q] = ids[e];
-            vr[q] = Sv[e] + al * Rv[e];
-        }
-    }
-    const size_t slice_off = (size_t)s * np * Rp;
-    const MPF_GLOBAL float4* P4 = (const MPF_GLOBAL float4*)(gp((const float*)J.p) + slice_off);
-    MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
-    const uint32_t range = (uint32_t)((size_t)np * Rp * sizeof(float));
-    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + slice_off, (size_t)range);
-#pragma unroll
-    for (int st = 0; st < STEPS; ++st) {
-        const int src = st * RPS + rsub;              // < 16
-        const int row = row0 + src;
-        const float4 own = P4[(size_t)row * LPR + c4];
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            const int id = __builtin_amdgcn_ds_bpermute(src << 2, idr[q]);
-            const float v = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(vr[q])));
-            const uint32_t off = v != 0.0f ? (uint32_t)(id * Rp + 4 * c4) * 4u : range;
-            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-            acc.x += v * __uint_as_float(raw[0]);
-            acc.y += v * __uint_as_float(raw[1]);
-            acc.z += v * __uint_as_float(raw[2]);
-            acc.w += v * __uint_as_float(raw[3]);
-        }
-        Q4[(size_t)row * LPR + c4] = acc;
-        pq[0] += (double)own.x * (double)acc.x; pq[1] += (double)own.y * (double)acc.y;
-        pq[2] += (double)own.z * (double)acc.z; pq[3] += (double)own.w * (double)acc.w;
-        ps[0] += (double)own.x; ps[1] += (double)own.y; ps[2] += (double)own.z; ps[3] += (double)own.w;
-    }
-}
-
-// 128-bit gathers: a lane owns 4 consecutive RHS columns, Rp/4 lanes cover one row, so
-// one buffer_load_dwordx4 instruction fetches 64/(Rp/4) neighbour rows at once.  Row
-// ids/values of the wave's 16 rows sit in lanes 0..15 (coalesced column-major ELL load)
-// and are routed to the lanes of each row with ds_bpermute.  A wavefront whose 16 rows
-// are all at most 12 entries wide (P2 midpoint rows: 9) runs the 12-slot body.
+// q = A_s p (+ 1 1^T p when dcFree, applied in the update); tile partials p.q and 1^T p.
+// WMAX = padded ELL width (compile time).
 template <int LG, int WMAX>
-__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_pcg_spmv4(const JobDev* __restrict__ jobs,
                                                         const int4* __restrict__ tiles) {
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;
-    constexpr int WS = WMAX > 12 ? 12 : WMAX;
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
     if (!slice_active(J, s)) return;
     const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-    const int rsub = l64 / LPR, c4 = l64 % LPR;
     const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
-    // entry count of the widest of the wavefront's 16 rows
-    const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
-    double pq[4] = {0.0, 0.0, 0.0, 0.0}, ps[4] = {0.0, 0.0, 0.0, 0.0};
-    if (WS < WMAX && wl <= WS) spmv4_rows<LG, WS>(J, s, row0, l64, pq, ps);
-    else spmv4_rows<LG, WMAX>(J, s, row0, l64, pq, ps);
-    // combine the row sub-groups of the wavefront (same c4), fixed xor tree
-#pragma unroll
-    for (int off = LPR; off < 64; off <<= 1) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            pq[j] += __shfl_xor(pq[j], off, 64);
-            ps[j] += __shfl_xor(ps[j], off, 64);
-        }
-    }
-    // combine the 16 wavefronts in order
-    __shared__ double red[2][kEwBlock / 64][64];
-    if (rsub == 0) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            red[0][wave][4 * c4 + j] = pq[j];
-            red[1][wave][4 * c4 + j] = ps[j];
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < Rp) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < kEwBlock / 64; ++w) { a += red[0][w][threadIdx.x]; b += red[1][w][threadIdx.x]; }
-        part_ptr(J, 0, s, t.z)[threadIdx.x] = a;
-        part_ptr(J, 1, s, t.z)[threadIdx.x] = b;
-    }
+    const size_t slice_off = (size_t)s * J.np * Rp;
+    const MPF_GLOBAL float4* P4 = (const MPF_GLOBAL float4*)(gp((const float*)J.p) + slice_off);
+    MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
+    double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    __shared__ EllStage<WMAX> stage;
+    ell_gather_wave<LG, WMAX>(J, s, row0, l64, J.p + slice_off, stage, [&](int row, int c4, float4 acc) {
+        const float4 own = P4[(size_t)row * LPR + c4];
+        Q4[(size_t)row * LPR + c4] = acc;
+        v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
+        v[0][2] += (double)own.z * (double)acc.z; v[0][3] += (double)own.w * (double)acc.w;
+        v[1][0] += (double)own.x; v[1][1] += (double)own.y; v[1][2] += (double)own.z; v[1][3] += (double)own.w;
+    });
+    const int ch[2] = {0, 1};
+    block_partials4<LG, 2>(J, s, t.z, ch, v);
 }
 
 template <int LG, int WMAX, int U>
@@ -428,43 +219,6 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
     const double v[2] = {pq, ps};
     block_partials<2, kEwBlock>(J, s, t.z, ch, v);
 }
-
-// ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
-// Combine per-lane FP64 values of 4 columns across the row sub-groups of each
-// wavefront (fixed xor tree) and across the 16 wavefronts (fixed order), then write
-// the tile partials of NV channels.
-template <int LG, int NV>
-__device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile, const int* ch,
-                                                double (&v)[NV][4]) {
-    constexpr int LPR = (1 << LG) / 4;
-    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
-    const int rsub = l64 / LPR, c4 = l64 % LPR;
-#pragma unroll
-    for (int off = LPR; off < 64; off <<= 1)
-#pragma unroll
-        for (int c = 0; c < NV; ++c)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
-    __shared__ double red[NV][kEwBlock / 64][64];
-    if (rsub == 0)
-#pragma unroll
-        for (int c = 0; c < NV; ++c)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) red[c][wave][4 * c4 + j] = v[c][j];
-    __syncthreads();
-    if ((int)threadIdx.x < (1 << LG)) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) {
-            double a = 0.0;
-            for (int w = 0; w < kEwBlock / 64; ++w) a += red[c][w][threadIdx.x];
-            part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
-        }
-    }
-}
-
-// rho of iteration `par` parity: r.z of the current residual lives in one of two slots so
-// that the kernel computing the next rho never overwrites the one its other blocks read
-__device__ __forceinline__ int rho_slot(int par) { return par ? SC_RHO1 : SC_RHO; }
 
 // Iteration i runs three launches: SpMV (tile partials of p.q and 1^T p), update and
 // pupdate.  Each update / pupdate block sums the partials it needs over all tiles of
