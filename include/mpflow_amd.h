@@ -150,6 +150,11 @@ typedef struct mpf_solve_opts {
     int32_t warm_start;           /* MPF_PREC_MIXED only: start from the model's previous FP64
                                      fields (a correction solve of the FP64 residual, stopped at
                                      tol * ||b||) instead of from zero; ignored on the first solve */
+    int32_t bicgstab_ignore_dc;   /* BiCGSTAB only.  0: harmonic 0 uses the dcFree operator
+                                     A + 1 1^T like CG (FP32 BiCGSTAB on the singular matrix
+                                     stagnates on large meshes); 1: ignore dcFree as the
+                                     reference does (bicgstab.cpp:87).  Voltages and Jacobian
+                                     are the same either way. */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {

@@ -42,6 +42,7 @@ class SolveOpts(C.Structure):
         ("stop_rule", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
         ("precision", C.c_int32), ("refine_steps", C.c_int32), ("refine_tol", C.c_double),
         ("krylov", C.c_int32), ("check_every", C.c_int32), ("warm_start", C.c_int32),
+        ("bicgstab_ignore_dc", C.c_int32),
     ]
 
 

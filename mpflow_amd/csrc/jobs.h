@@ -15,14 +15,16 @@ namespace mpf {
 
 // scalar fields per (slice, lane), FP64
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
-               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_NFIELDS };
+               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB, SC_NFIELDS };
 // per-slice integer state: [0] active, [1] steps, [2] hit cap
 constexpr int kStInts = 4;
 // partial-sum channels per tile
-constexpr int kParts = 4;
+constexpr int kParts = 7;
 
 struct JobDev {
     int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, col0;   // R = unique RHS solved
+    int dc_rank1;                // 1: the dc slice's operator is A + 1 1^T (CG dcFree);
+                                 // 0: A itself (BiCGSTAB, constants projected out)
     double tol;
     const int32_t* ids;          // [width][np]
     float* s;  float* r;         // FP32 values (hi)        [width][np]
@@ -36,13 +38,13 @@ struct JobDev {
     float* rhs;                  // [C][np][Rp] FP32 right-hand side buffer (always valid)
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
     int ref_b;                   // 1: stop relative to ||beta_c F|| (SC_X2) instead of ||rhs||
-    float* x; float* rv; float* p; float* q;   // [C][np][Rp]
-    float* t1; float* t2;        // BiCGSTAB extra vectors [C][np][Rp]
+    float* x; float* rv; float* p; float* q;   // [C][np][Rp] (q = nu in BiCGSTAB)
+    // BiCGSTAB only (null otherwise): r-hat, y = K^-1 p, s, z = K^-1 s, t   [C][np][Rp]
+    float* rh; float* yv; float* sv; float* zv; float* tv;
     double* x64;                 // [C][np][Rp] FP64 refined solution
     double* part;                // [kParts][C][ntiles][Rp]
     double* sc;                  // [SC_NFIELDS][C][Rp]
     int* st;                     // [C][kStInts]
-    int* cnt;                    // [C][2] last-block tickets (SpMV, update), zero between launches
     int* active;                 // unconverged (job, harmonic) pairs of the launch
     // assembly
     const int32_t* ctb_off; const int32_t* ctb_elem;
@@ -72,6 +74,12 @@ void launch_assemble(const JobDev* d_jobs, const int4* d_items, int nitems, hipS
 void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                      const int2* d_js, int njs, int* d_active, hipStream_t st);
+// BiCGSTAB (bicgstab.hip): init over all pairs, then one iteration per call (five
+// launches: p-update, SpMV, s-update, SpMV, x/r-update); ev0/ev1 bracket the first SpMV
+void launch_bicg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                      const int2* d_js, int njs, int* d_active, int lg, hipStream_t st);
+void launch_bicg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
+                      int lg, int wpad, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // ev0/ev1 (nullable) are recorded right before and after the SpMV launch
 // one PCG iteration (SpMV, update, pupdate); `iter` counts iterations since the
 // solve's init (its parity selects the rho slot)

@@ -80,11 +80,9 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
     st.alloc((size_t)S.C * kStInts * frames);
-    cnt.alloc((size_t)S.C * 2 * frames);
     active.alloc(1);
     gamma.alloc((size_t)S.E * frames);
     MPF_HIP(hipMemsetAsync(st.p, 0, st.bytes(), st0));
-    MPF_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st0));
 
@@ -110,12 +108,10 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.rhs = rhs.p + V * f;
         J.use_rhs = 0;
         J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f;
-        J.t1 = nullptr; J.t2 = nullptr;
         J.x64 = x64.p + V * f;
         J.part = part.p + (size_t)kParts * S.C * ntile_job * S.Rp * f;
         J.sc = sc.p + (size_t)SC_NFIELDS * S.C * S.Rp * f;
         J.st = st.p + (size_t)S.C * kStInts * f;
-        J.cnt = cnt.p + (size_t)S.C * 2 * f;
         J.active = active.p;
         J.ctb_off = ctb_off.p; J.ctb_elem = ctb_elem.p; J.ctb_k = ctb_k.p; J.ctb_m = ctb_m.p;
         J.gamma = gamma.p + (size_t)S.E * f;
@@ -172,7 +168,9 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
     }
     upload_jobs();
     MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));
-    launch_pcg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
+    const bool bicg = o.krylov == MPF_KRYLOV_BICGSTAB;
+    if (bicg) launch_bicg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, hjobs[0].lg, st0);
+    else launch_pcg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
     const int every = o.check_every > 0 ? o.check_every : 16;
     const int lg = hjobs[0].lg, wpad = S.wpad;
     const int ntile_job = S.np / kTileRows;
@@ -186,7 +184,10 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
         for (int i = 0; i < every; ++i) {
             std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
             if (ctx->prof) ev = ctx->next_events();
-            launch_pcg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
+            if (bicg)
+                launch_bicg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
+            else
+                launch_pcg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
         }
         it += every;
         MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
@@ -232,7 +233,17 @@ void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, 
 
 int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats) {
     hipStream_t st0 = ctx->stream;
-    MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG, "only the CG Krylov method is available in this build");
+    MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG || o.krylov == MPF_KRYLOV_BICGSTAB, "solve: unknown Krylov method");
+    if (o.krylov == MPF_KRYLOV_BICGSTAB) {
+        MPF_REQUIRE(S.wpad <= 24, "BiCGSTAB: padded ELL width above 24 is not supported");
+        ensure_bicg();
+    }
+    // Open sources: CG solves harmonic 0 dcFree (eit.cpp:175-177); the reference's BiCGSTAB
+    // ignores the flag (bicgstab.cpp:87), which bicgstab_ignore_dc reproduces
+    for (auto& J : hjobs) {
+        J.dc = (o.krylov == MPF_KRYLOV_BICGSTAB && o.bicgstab_ignore_dc) ? -1 : 0;
+        J.dc_rank1 = o.krylov == MPF_KRYLOV_CG ? 1 : 0;
+    }
     // Equation::update: S, R once per sigma (not once per harmonic), then Jacobi
     launch_assemble(jobs.p, asm_items.p, nasm, st0);
     launch_dinv(jobs.p, tiles.p, ntiles, st0);
@@ -291,6 +302,18 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
     }
     MPF_HIP(hipStreamSynchronize(st0));
     return capped ? MPF_ENOCONV : MPF_OK;
+}
+
+void Model::ensure_bicg() {
+    if (rh.p) return;
+    const size_t V = vec_elems() * frames;
+    rh.alloc(V); yv.alloc(V); sv.alloc(V); zv.alloc(V); tv.alloc(V);
+    const size_t Vf = vec_elems();
+    for (int f = 0; f < frames; ++f) {
+        JobDev& J = hjobs[f];
+        J.rh = rh.p + Vf * f; J.yv = yv.p + Vf * f; J.sv = sv.p + Vf * f; J.zv = zv.p + Vf * f;
+        J.tv = tv.p + Vf * f;
+    }
 }
 
 void Model::ensure_gn() {

@@ -35,8 +35,9 @@ struct Model {
     DBuf<float> ek, em;
     // per-frame state
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;
+    DBuf<float> rh, yv, sv, zv, tv;   // BiCGSTAB vectors (allocated on first use)
     DBuf<double> x64, part, sc;
-    DBuf<int> st, cnt;
+    DBuf<int> st;
     DBuf<float> gamma;
     // job tables
     std::vector<JobDev> hjobs;
@@ -67,6 +68,7 @@ struct Model {
                 const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats);
 
   private:
+    void ensure_bicg();
     void krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
                 mpf_solve_stats* stats, bool* capped);
     bool x64_valid = false;

@@ -357,13 +357,19 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
                 } else if (lane < J.R) {
                     if (bb > 0.0 && rr > J.tol * J.tol * bb) atomicOr(&not_done, 1);
                 }
+                // a NaN residual with a nonzero right-hand side is a breakdown
+                if (lane < J.R && bb > 0.0 && rr != rr) atomicOr(&not_done, 2);
             }
         }
         __syncthreads();
         if (book && threadIdx.x == 0) {
             const int steps = J.st[s * kStInts + 1] + 1;
             J.st[s * kStInts + 1] = steps;
-            if (!not_done) {
+            if (not_done & 2) {
+                J.st[s * kStInts + 0] = 0;
+                J.st[s * kStInts + 2] = 1;
+                atomicSub(active, 1);
+            } else if (!not_done) {
                 J.st[s * kStInts + 0] = 0;
                 atomicSub(active, 1);
             } else if (steps >= J.maxit) {
@@ -420,7 +426,8 @@ __global__ void __launch_bounds__(kEwBlock) k_colsum64_reduce(const JobDev* __re
     const int ch[1] = {0};
     double out[1];
     reduce_tiles<1>(J, s, ch, out);
-    if ((int)threadIdx.x < J.Rp) sc_ptr(J, SC_X0, s)[threadIdx.x] = (s == J.dc) ? out[0] : 0.0;
+    if ((int)threadIdx.x < J.Rp)
+        sc_ptr(J, SC_X0, s)[threadIdx.x] = (s == J.dc && J.dc_rank1) ? out[0] : 0.0;
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restrict__ jobs,

@@ -70,13 +70,15 @@ def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwis
 
 class SolveOptions:
     def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
-                 refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0):
+                 refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0,
+                 bicgstab_ignore_dc=0):
         self.s = capi.SolveOpts()
         lib().mpf_solve_opts_default(C.byref(self.s))
         self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
         self.s.precision, self.s.refine_steps, self.s.refine_tol = precision, refine_steps, refine_tol
         self.s.krylov, self.s.check_every = krylov, check_every
         self.s.warm_start = int(warm_start)
+        self.s.bicgstab_ignore_dc = int(bicgstab_ignore_dc)
 
     def copy(self, **kw):
         o = SolveOptions()

@@ -314,6 +314,77 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     return x, iters
 
 
+def bicgstab_reference(A, b, dinv=None, max_iter=0, eps=FLT_EPSILON, rule="reference", tol=1e-6,
+                       project=False):
+    """Multi-RHS right-preconditioned BiCGSTAB as ``BiCGSTAB::solve`` runs it
+    (src/numeric/bicgstab.cpp:83-193), in float64, x starting at 0.  The reference ignores
+    dcFree (:87).  ``project=True`` starts from r = r^ = b - mean(b) and subtracts the mean
+    from nu = A y and t = A z (the device's default on the singular harmonic; a no-op in
+    exact arithmetic when 1^T A = 0 and 1^T b = 0).
+    Per column, with K^-1 = diag(dinv):
+
+        rho = (r, r^);  beta = (rho / rho_old) (alpha / omega)
+        p = r + beta (p - omega nu);  y = K^-1 p;  nu = A y;  alpha = rho / (nu, r^)
+        s = r - alpha nu;  z = K^-1 s;  t = A z;  w = K^-1 t;  omega = (z, w) / (w, w)
+        x = x + alpha y + omega z;  r = s - omega t
+
+    with r^ = r0 = b, rho_old = alpha = omega = 1, p = nu = 0 at the start and every
+    division by zero giving 0 (elementwiseDivision, matrix_kernel.cu:344-352).
+    ``rule="reference"``: after step s, stop when sqrt(r.r / b.b) < eps for column 0 and
+    return s+1 (:179-187).  ``rule="tolerance"``: stop when every column has
+    ||r|| <= tol ||b||.  Returns (x, steps).
+    """
+    N, R = b.shape
+    iters = 10 * N if max_iter == 0 else max_iter
+    if dinv is None:
+        dinv = np.ones(N)
+    k = dinv[:, None]
+
+    def op(v):
+        y = A @ v
+        return y - y.mean(axis=0, keepdims=True) if project else y
+
+    def div(a, c):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(c != 0.0, a / np.where(c != 0.0, c, 1.0), 0.0)
+
+    x = np.zeros((N, R))
+    r = b - b.mean(axis=0, keepdims=True) if project else b.copy()
+    rh = r.copy()
+    rho = np.ones(R)
+    alpha = np.ones(R)
+    omega = np.ones(R)
+    nu = np.zeros((N, R))
+    p = np.zeros((N, R))
+    ref = np.sum(b * b, axis=0)
+    for step in range(iters):
+        rho_old = rho
+        rho = np.sum(r * rh, axis=0)
+        beta = div(rho, rho_old) * div(alpha, omega)
+        p = r + beta * (p - omega * nu)
+        y = k * p
+        nu = op(y)
+        alpha = div(rho, np.sum(nu * rh, axis=0))
+        s_ = r - alpha * nu
+        z = k * s_
+        t = op(z)
+        w = k * t
+        omega = div(np.sum(z * w, axis=0), np.sum(w * w, axis=0))
+        x = x + alpha * y + omega * z
+        r = s_ - omega * t
+        rr = np.sum(r * r, axis=0)
+        if rule == "reference":
+            with np.errstate(divide="ignore", invalid="ignore"):
+                crit = abs(math.sqrt(rr[0] / ref[0])) if ref[0] != 0 else float("nan")
+            if not (crit >= eps):
+                return x, step + 1
+        else:
+            ok = np.where(ref > 0, rr <= (tol * tol) * ref, True)
+            if np.all(ok):
+                return x, step + 1
+    return x, iters
+
+
 def solve_direct(A, F, dc_free):
     """Exact FP64 solve of (A [+ 1 1^T]) X = F via a sparse LU.
 

@@ -85,6 +85,60 @@ def test_forward_c1_warm_start(gpu_ctx, c1):
     assert warm <= cold
 
 
+@pytest.mark.parametrize("rule,ignore_dc", [("reference", 1), ("tolerance", 1), ("reference", 0)])
+def test_forward_c1_bicgstab(gpu_ctx, c1, rule, ignore_dc):
+    """BiCGSTAB (FP32; the reference's iteration, or with the harmonic-0 Krylov vectors kept
+    orthogonal to constants) reaches the FP64 direct solution's voltages and Jacobian: the
+    constant mode reaches neither.  The step count follows the FP64 restatement's."""
+    torch = _torch()
+    from mpflow_amd.eit import SolveOptions
+    from mpflow_amd import capi
+    import oracle.eit_oracle as eo
+    p, o, g, U, phis = c1
+    print("case: c1 bicgstab %s ignore_dc %d" % (rule, ignore_dc), flush=True)
+    eit = eit_of(p, gpu_ctx)
+    stop = capi.MPF_STOP_REFERENCE if rule == "reference" else capi.MPF_STOP_TOLERANCE
+    eit.solve(torch.tensor(g, dtype=torch.float32),
+              SolveOptions(stop_rule=stop, tol=1e-6, krylov=capi.MPF_KRYLOV_BICGSTAB,
+                           bicgstab_ignore_dc=ignore_dc), jacobian=True)
+    Ug = eit.voltages().cpu().numpy()[0]
+    Jg = eit.jacobian_matrix().cpu().numpy()[0]
+    Jo = o.jacobian(g, phis, quirks=True)
+    eu = _rel(Ug, U)
+    ej = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
+    # the restatement's step count for the same system (FP64)
+    A = o.system(g, 0)
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    _, so = eo.bicgstab_reference(A, F, 1.0 / A.diagonal(), rule=rule, tol=1e-6, project=not ignore_dc)
+    print("rel err", eu, "jacobian", ej, "steps", eit.last_stats.steps, "oracle steps", so)
+    assert eit.last_stats.not_converged == 0
+    assert eu < 1e-5 and ej < 1e-4
+    assert eit.last_stats.steps <= 2 * so + 16
+
+
+def test_forward_2p5d_bicgstab_mixed(gpu_ctx):
+    """BiCGSTAB with FP64 residual refinement on the 2.5D 32-electrode mesh.  FP32
+    BiCGSTAB on the singular harmonic bottoms out near a 1e-6 relative residual, so the
+    FP32 solves stop at 1e-5 and the refinement supplies the rest."""
+    torch = _torch()
+    from mpflow_amd.eit import SolveOptions
+    from mpflow_amd import capi
+    p = make_problem(40, 32, 0.05, C=5, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    g = ellipse_phantom(p["nodes"], p["elements"], frame_seed(2, 2))
+    U, _, _ = o.solve(g)
+    print("case: c2 bicgstab mixed", flush=True)
+    eit = eit_of(p, gpu_ctx)
+    eit.solve(torch.tensor(g, dtype=torch.float32),
+              SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-5, precision=capi.MPF_PREC_MIXED,
+                           refine_steps=3, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_BICGSTAB))
+    Ug = eit.voltages().cpu().numpy()[0]
+    err = _rel(Ug, U)
+    print("rel err", err, "steps", eit.last_stats.steps, "refine residual", eit.last_stats.refine_residual)
+    assert eit.last_stats.not_converged == 0
+    assert err < 1e-7
+
+
 @pytest.mark.parametrize("keep", [True, False])
 def test_forward_node_order(gpu_ctx, keep):
     """The caller's node numbering and the internal locality numbering give the same

@@ -223,6 +223,49 @@ def test_pcg_restatement_against_direct(small):
     assert np.max(np.abs(Xr - X)) < 1e-5 * np.max(np.abs(X))
 
 
+def test_bicgstab_restatement_against_direct(small):
+    """BiCGSTAB (no dcFree) on the 2.5D harmonic-1 matrix (SPD, alpha_1 > 0) reaches the
+    direct solve; on the singular harmonic-0 matrix with zero-sum excitations it reaches
+    a solution that differs from the dcFree one by a constant per column, so electrode
+    difference voltages agree."""
+    p, o, g = small
+    F = o.B @ o.Pi
+    for c in (0, 1):
+        A = o.system(g, c)   # alpha_c = (2 pi c / h)^2
+        X = eo.solve_direct(A, F, c == 0)
+        Xb, steps = eo.bicgstab_reference(A, F, 1.0 / A.diagonal(), rule="tolerance", tol=1e-13)
+        assert steps < 10 * o.N
+        res = np.max(np.abs(A @ Xb - F)) / np.max(np.abs(F))
+        assert res < 1e-11
+        D = Xb - X
+        if c == 0:
+            D = D - D.mean(axis=0, keepdims=True)
+        assert np.max(np.abs(D)) < 1e-9 * np.max(np.abs(X))
+    # the projected variant reaches the same solution up to a constant
+    A = o.system(g, 0)
+    X = eo.solve_direct(A, F, True)
+    Xp, _ = eo.bicgstab_reference(A, F, 1.0 / A.diagonal(), rule="tolerance", tol=1e-13, project=True)
+    D = Xp - X
+    assert np.max(np.abs(D - D.mean(axis=0, keepdims=True))) < 1e-9 * np.max(np.abs(X))
+    # reference stop rule (column 0, sqrt(r.r / b.b) < FLT_EPSILON)
+    Xr, sr = eo.bicgstab_reference(A, F, 1.0 / A.diagonal())
+    assert sr < 10 * o.N
+    r0 = F[:, 0] - A @ Xr[:, 0]
+    assert np.sqrt(r0 @ r0 / (F[:, 0] @ F[:, 0])) < eo.FLT_EPSILON
+
+
+def test_bicgstab_tiny_spd_exact_in_few_steps():
+    """On a 6 x 6 SPD matrix, BiCGSTAB reaches the exact solution within 2N steps (a
+    Krylov method on an N-dimensional space), for several right-hand sides at once."""
+    rng = np.random.default_rng(7)
+    M = rng.standard_normal((6, 6))
+    A = M @ M.T + 6 * np.eye(6)
+    b = rng.standard_normal((6, 3))
+    x, steps = eo.bicgstab_reference(A, b, 1.0 / np.diag(A), rule="tolerance", tol=1e-14)
+    assert steps <= 12
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
 def test_pcg_zero_rhs_stops_after_one_step(small):
     p, o, g = small
     A = o.system(g, 0)
