@@ -43,7 +43,22 @@ WORKLOADS = {
     "c1": dict(rings=16, L=16, width=0.1, C=1, height=1.0, port_height=1.0, p2=False,
                desc="2D disc, P1 16 rings (1536 elements), 16 electrodes"),
 }
-CONFIG_INDEX = {"c1": 1, "c2": 2, "c3": 3}
+CONFIG_INDEX = {"c1": 1, "c2": 2, "c3": 3, "c4": 4}
+
+
+def c4_job(j):
+    """Job j of the uneven multi-job workload (config 4): E_j = 2000 + 38000 u elements
+    (ring mesh with n_r = round(sqrt(E_j / 6)) rings), L_j in {16, 32} electrodes,
+    C_j in {1..5} harmonics (2D h = 1 for C_j = 1, else h = 0.4, h_p = 0.1), all drawn from
+    SplitMix64 with seed 1000003 * 4 + j."""
+    from mpflow_amd.phantom import SplitMix64, frame_seed
+    g = SplitMix64(frame_seed(4, j))
+    E = 2000 + 38000 * g.uniform()
+    rings = int(round(math.sqrt(E / 6.0)))
+    L = 16 if g.uniform() < 0.5 else 32
+    C = 1 + min(4, int(5 * g.uniform()))
+    return dict(rings=rings, L=L, width=0.1 if L == 16 else 0.05, C=C,
+                height=1.0 if C == 1 else 0.4, port_height=1.0 if C == 1 else 0.1, p2=False)
 
 
 def build_problem(w):
@@ -88,12 +103,93 @@ def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=1
     }
 
 
+def run_c4(args):
+    """Config 4: ``args.jobs`` uneven jobs (different meshes, electrode and harmonic
+    counts) reconstructed together: each timed step is 3 batched Gauss-Newton steps of
+    every job (forward solves of all jobs in one set of launches per Krylov iteration,
+    iteration groups replayed as HIP graphs; dense steps per job).  Not the headline
+    metric: printed as its own JSON line with metric "jobs/s"."""
+    import numpy as np
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import EIT, Batch, BatchSolver, Context, SolveOptions, Solver
+    from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
+    t_setup = time.perf_counter()
+    ctx = Context()
+    models, probs = [], []
+    for j in range(args.jobs):
+        w = c4_job(j)
+        nodes, el, bd, ports, pat = build_problem(w)
+        models.append(EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                          jacobian_exact=True, ctx=ctx))
+        probs.append((j, w, nodes, el))
+    batch = Batch(models)
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=1, refine_tol=1e-2, graphs=1)
+    gs = [torch.tensor(ellipse_phantom(n, e, frame_seed(4, j)), dtype=torch.float32) for j, _, n, e in probs]
+    batch.solve(gs, opts)
+    solvers = []
+    for (j, _, _, _), m in zip(probs, models):
+        s = Solver(m, args.lam, gn_refine=4, options=opts)
+        U = m.result64.cpu().numpy()[0]
+        s.measurement[0] = torch.tensor(add_noise(U, 60.0, frame_seed(4, j)), dtype=torch.float64)
+        solvers.append(s)
+    bs = BatchSolver(batch, solvers)
+    t_setup = time.perf_counter() - t_setup
+
+    def step():
+        for s in solvers:
+            s.material_distribution.fill_(1.0)
+        it = 0
+        for k in range(args.gn_steps):
+            bs.solve_absolute(warm_start=(k > 0 and not args.cold))
+            it += bs.last_stats.steps
+        return it
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(args.steps):
+        iters += step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    jobs_done = args.steps * args.jobs
+    out = {
+        "metric": "jobs/s, config 4: uneven multi-job batched 2.5D GN (3 steps per job)",
+        "value": round(jobs_done / elapsed, 3),
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "config 4 (c4): %d jobs, E_j = 2000 + 38000 u P1 elements, L_j in {16, 32}, "
+                        "C_j in 1..5, own ellipse phantom + 60 dB noise each" % args.jobs,
+            "elements_total": int(sum(m.E for m in models)),
+            "nodes_total": int(sum(m.N for m in models)),
+            "gn_steps_per_job": args.gn_steps,
+            "slice_iterations_per_step": iters // max(args.steps, 1),
+            "setup_s": round(t_setup, 1),
+        },
+    }
+    print(json.dumps(out))
+    batch.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed frames")
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--jobs", type=int, default=256, help="config 4: number of jobs")
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -115,6 +211,9 @@ def main():
     from mpflow_amd.eit import EIT, SolveOptions, Solver
     from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
 
+    if args.workload == "c4":
+        run_c4(args)
+        return
     w = WORKLOADS[args.workload]
     cfg = CONFIG_INDEX[args.workload]
     nodes, el, bd, ports, pat = build_problem(w)

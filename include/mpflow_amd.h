@@ -53,6 +53,7 @@ extern "C" {
 
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
+typedef struct mpf_batch mpf_batch;
 
 /* Library version string (replaces version::getVersionString, src/version.cpp:23-25). */
 const char* mpf_version(void);
@@ -155,6 +156,9 @@ typedef struct mpf_solve_opts {
                                      stagnates on large meshes); 1: ignore dcFree as the
                                      reference does (bicgstab.cpp:87).  Voltages and Jacobian
                                      are the same either way. */
+    int32_t graphs;               /* 1: capture the Krylov iterations between two convergence
+                                     polls as a HIP graph and replay it (fewer launch
+                                     overheads for small meshes); not with profiling */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {
@@ -197,6 +201,24 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
 int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lambda,
                 int32_t gn_refine, const mpf_solve_opts* opts, double* d_Ucalc,
                 mpf_solve_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * Batches: several models (meshes of any size, any electrode count and harmonic count)
+ * solved together.  No counterpart in the reference, whose models solve one at a time;
+ * each Krylov iteration is one set of launches per group of models that share the lane
+ * count (16/32/64 right-hand sides) and padded ELL width.  Models keep their own
+ * buffers; a model belongs to at most one batch; destroy the batch before its models.
+ * ------------------------------------------------------------------------- */
+int mpf_batch_create(mpf_ctx* ctx, mpf_model* const* models, int32_t n_models, mpf_batch** out);
+int mpf_batch_destroy(mpf_batch* b);
+/* mpf_eit_solve of every model; the arrays hold one device pointer per model (entries
+ * of d_U64 / d_J and the arrays themselves may be null). */
+int mpf_batch_solve(mpf_batch* b, const float* const* d_gamma, float* const* d_U, double* const* d_U64,
+                    float* const* d_J, const mpf_solve_opts* opts, mpf_solve_stats* stats);
+/* mpf_gn_step of every model, forward solves batched, dense steps per model. */
+int mpf_batch_gn_step(mpf_batch* b, float* const* d_gamma, const double* const* d_Umeas, double lambda,
+                      int32_t gn_refine, const mpf_solve_opts* opts, double* const* d_Ucalc,
+                      mpf_solve_stats* stats);
 
 /* ---------------------------------------------------------------------------
  * Low-level kernels on caller-owned device buffers

@@ -17,7 +17,8 @@ EXPORTED = [
     "mpf_ctx_stream", "mpf_solve_opts_default", "mpf_ports_circular", "mpf_model_create",
     "mpf_model_destroy", "mpf_model_dims", "mpf_eit_solve", "mpf_gn_step", "mpf_ell_spmv",
     "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read",
-    "mpf_element_integrals",
+    "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
+    "mpf_batch_gn_step",
 ]
 
 
@@ -42,7 +43,7 @@ class SolveOpts(C.Structure):
         ("stop_rule", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
         ("precision", C.c_int32), ("refine_steps", C.c_int32), ("refine_tol", C.c_double),
         ("krylov", C.c_int32), ("check_every", C.c_int32), ("warm_start", C.c_int32),
-        ("bicgstab_ignore_dc", C.c_int32),
+        ("bicgstab_ignore_dc", C.c_int32), ("graphs", C.c_int32),
     ]
 
 
@@ -89,6 +90,10 @@ def lib() -> C.CDLL:
     L.mpf_ctx_profile.argtypes = [vp, i32]
     L.mpf_ctx_profile_read.argtypes = [vp, C.POINTER(dbl)]
     L.mpf_element_integrals.argtypes = [i32, vp, vp, vp]
+    L.mpf_batch_create.argtypes = [vp, vp, i32, C.POINTER(vp)]
+    L.mpf_batch_destroy.argtypes = [vp]
+    L.mpf_batch_solve.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.mpf_batch_gn_step.argtypes = [vp, vp, vp, dbl, i32, C.POINTER(SolveOpts), vp, C.POINTER(SolveStats)]
     _lib = L
     return L
 

@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(kEwBlock) k_bicg_init_reduce(const JobDev* __r
         J.st[s * kStInts + 0] = 1;
         J.st[s * kStInts + 1] = 0;
         J.st[s * kStInts + 2] = 0;
+        J.st[s * kStInts + 3] = 1;   // the next B1 is the first: no residual to test yet
         atomicAdd(active, 1);
     }
 }
@@ -161,6 +162,8 @@ __global__ void __launch_bounds__(kEwBlock) k_bicg_p(const JobDev* __restrict__ 
     if (!slice_active(J, s)) return;
     const int par = iter & 1;
     const bool book = t.z == 0;
+    // only tile 0's block reads or writes the first-iteration flag
+    const bool first = book && J.st[s * kStInts + 3] != 0;
     __shared__ float sh_b[Ly::Rp], sh_om[Ly::Rp], sh_m[Ly::Rp];
     __shared__ int not_done;
     {
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(kEwBlock) k_bicg_p(const JobDev* __restrict__ 
             sh_m[lane] = s == J.dc ? (float)sc_ptr(J, SC_V, s)[lane] : 0.0f;
             if (book) {
                 sc_ptr(J, rho_slot(par), s)[lane] = rho;
-                if (iter > 0) {
+                if (!first) {
                     const double rr = out[1];
                     const double bb = sc_ptr(J, SC_BB, s)[lane];
                     sc_ptr(J, SC_RR, s)[lane] = rr;
@@ -205,7 +208,8 @@ __global__ void __launch_bounds__(kEwBlock) k_bicg_p(const JobDev* __restrict__ 
             }
         }
         __syncthreads();
-        if (book && iter > 0 && threadIdx.x == 0) {
+        if (book && threadIdx.x == 0 && first) J.st[s * kStInts + 3] = 0;
+        if (book && !first && threadIdx.x == 0) {
             const int steps = J.st[s * kStInts + 1] + 1;
             J.st[s * kStInts + 1] = steps;
             if (not_done & 2) {

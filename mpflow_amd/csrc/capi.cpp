@@ -35,6 +35,10 @@ struct mpf_model {
     mpf::Model* impl;
 };
 
+struct mpf_batch {
+    mpf::Batch* impl;
+};
+
 extern "C" {
 
 const char* mpf_version(void) { return "mpflow_amd 0.1.0 (gfx950)"; }
@@ -220,6 +224,71 @@ int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lamb
         if (opts) o = *opts; else mpf_solve_opts_default(&o);
         if (stats) std::memset(stats, 0, sizeof(*stats));
         int rc = m->impl->gn_step(d_gamma, d_Umeas, lambda, gn_refine, o, d_Ucalc, stats);
+        if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    });
+}
+
+int mpf_batch_create(mpf_ctx* ctx, mpf_model* const* models, int32_t n_models, mpf_batch** out) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && models && out && n_models > 0, "mpf_batch_create: null argument or no models");
+        set_device(ctx);
+        std::vector<mpf::Model*> ms;
+        for (int i = 0; i < n_models; ++i) {
+            MPF_REQUIRE(models[i] && models[i]->impl, "mpf_batch_create: null model");
+            ms.push_back(models[i]->impl);
+        }
+        auto* b = new mpf_batch;
+        try {
+            b->impl = new mpf::Batch(ctx, ms);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+        return MPF_OK;
+    });
+}
+
+int mpf_batch_destroy(mpf_batch* b) {
+    return guarded([&] {
+        if (!b) return MPF_OK;
+        if (b->impl) {
+            (void)hipStreamSynchronize(b->impl->ctx->stream);
+            delete b->impl;
+        }
+        delete b;
+        return MPF_OK;
+    });
+}
+
+int mpf_batch_solve(mpf_batch* b, const float* const* d_gamma, float* const* d_U, double* const* d_U64,
+                    float* const* d_J, const mpf_solve_opts* opts, mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(b && d_gamma, "mpf_batch_solve: null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        set_device(b->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        int rc = b->impl->solve(d_gamma, d_U, d_U64, d_J, o, stats);
+        if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc == MPF_ENOCONV) g_last_error = "mpf_batch_solve: a Krylov solve hit its iteration cap";
+        return rc;
+    });
+}
+
+int mpf_batch_gn_step(mpf_batch* b, float* const* d_gamma, const double* const* d_Umeas, double lambda,
+                      int32_t gn_refine, const mpf_solve_opts* opts, double* const* d_Ucalc,
+                      mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(b && d_gamma && d_Umeas, "mpf_batch_gn_step: null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        set_device(b->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        int rc = b->impl->gn_step(d_gamma, d_Umeas, lambda, gn_refine, o, d_Ucalc, stats);
         if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return rc;
     });

@@ -16,7 +16,8 @@ namespace mpf {
 // scalar fields per (slice, lane), FP64
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
                SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB, SC_NFIELDS };
-// per-slice integer state: [0] active, [1] steps, [2] hit cap
+// per-slice integer state: [0] active, [1] steps, [2] hit cap or breakdown,
+// [3] BiCGSTAB: the next iteration is the first
 constexpr int kStInts = 4;
 // partial-sum channels per tile
 constexpr int kParts = 7;
@@ -25,6 +26,7 @@ struct JobDev {
     int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, col0;   // R = unique RHS solved
     int dc_rank1;                // 1: the dc slice's operator is A + 1 1^T (CG dcFree);
                                  // 0: A itself (BiCGSTAB, constants projected out)
+    int nnz;                     // true nonzeros of the pattern (bandwidth accounting)
     double tol;
     const int32_t* ids;          // [width][np]
     float* s;  float* r;         // FP32 values (hi)        [width][np]
@@ -81,10 +83,16 @@ void launch_bicg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
 void launch_bicg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                       int lg, int wpad, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // ev0/ev1 (nullable) are recorded right before and after the SpMV launch
-// one PCG iteration (SpMV, update, pupdate); `iter` counts iterations since the
-// solve's init (its parity selects the rho slot)
+// one PCG iteration (SpMV, update, pupdate); only the parity of `iter` (iterations since
+// the solve's init) is used: it selects the rho slot.  `variant` picks the SpMV kernel
+// (0: 128-bit gathers, width-binned, for padded widths <= 24; 1: 32-bit gathers).
+constexpr int kSpmvVariants = 2;
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
-                     int lg, int wpad, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                     int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0 = nullptr,
+                     hipEvent_t ev1 = nullptr);
+// the SpMV of launch_pcg_iter alone (autotuning)
+void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
+                     hipStream_t st);
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                        const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,

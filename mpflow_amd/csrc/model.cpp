@@ -32,19 +32,6 @@ void mpf_ctx::collect_events() {
 
 namespace mpf {
 
-// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD b % 8), and
-// each XCD has its own L2.  Reorder a work list so that XCD x receives a contiguous run
-// of consecutive tiles: neighbouring rows gathered by the SpMV then hit the same L2.
-constexpr int kXcds = 8;
-
-static void xcd_order(const int4* in, int n, int4* out) {
-    const int base = n / kXcds, extra = n % kXcds;
-    int start[kXcds + 1];
-    start[0] = 0;
-    for (int x = 0; x < kXcds; ++x) start[x + 1] = start[x] + base + (x < extra ? 1 : 0);
-    for (int b = 0; b < n; ++b) out[b] = in[start[b % kXcds] + b / kXcds];
-}
-
 Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     MPF_REQUIRE(d.frames >= 1, "model: frames < 1");
     frames = d.frames;
@@ -79,10 +66,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     x64.alloc(V * frames);
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
-    st.alloc((size_t)S.C * kStInts * frames);
-    active.alloc(1);
     gamma.alloc((size_t)S.E * frames);
-    MPF_HIP(hipMemsetAsync(st.p, 0, st.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st0));
 
@@ -111,8 +95,9 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.x64 = x64.p + V * f;
         J.part = part.p + (size_t)kParts * S.C * ntile_job * S.Rp * f;
         J.sc = sc.p + (size_t)SC_NFIELDS * S.C * S.Rp * f;
-        J.st = st.p + (size_t)S.C * kStInts * f;
-        J.active = active.p;
+        J.st = nullptr;          // the Krylov driver points this at its own state
+        J.active = nullptr;
+        J.nnz = S.nnz;
         J.ctb_off = ctb_off.p; J.ctb_elem = ctb_elem.p; J.ctb_k = ctb_k.p; J.ctb_m = ctb_m.p;
         J.gamma = gamma.p + (size_t)S.E * f;
         J.ref = S.ref; J.logarithmic = S.logarithmic; J.E = S.E;
@@ -139,99 +124,21 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     xcd_order(ht.data(), ntiles, hto.data());
     tiles.alloc(ntiles); tiles.upload(hto.data(), hto.size(), st0);
     js.alloc(njs); js.upload(hj.data(), hj.size(), st0);
-    act_tiles.alloc(ntiles);
-    MPF_HIP(hipHostMalloc((void**)&h_tiles, sizeof(int4) * 2 * (size_t)ntiles, hipHostMallocDefault));
-    MPF_HIP(hipHostMalloc((void**)&h_st, sizeof(int) * (size_t)S.C * kStInts * frames, hipHostMallocDefault));
     asm_items.alloc(nasm); asm_items.upload(ha.data(), ha.size(), st0);
-    MPF_HIP(hipHostMalloc((void**)&h_active, sizeof(int), hipHostMallocDefault));
     MPF_HIP(hipStreamSynchronize(st0));
 }
 
 Model::~Model() {
-    if (h_active) (void)hipHostFree(h_active);
-    if (h_tiles) (void)hipHostFree(h_tiles);
-    if (h_st) (void)hipHostFree(h_st);
 }
 
 void Model::upload_jobs() {
     jobs.upload(hjobs.data(), hjobs.size(), ctx->stream);
 }
 
-void Model::krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
-                   mpf_solve_stats* stats, bool* capped) {
-    hipStream_t st0 = ctx->stream;
-    const int maxit = o.max_iter > 0 ? o.max_iter : 10 * S.N;
-    for (auto& J : hjobs) {
-        J.rule = rule; J.tol = tol; J.maxit = maxit;
-        J.use_rhs = use_rhs ? 1 : 0;
-        J.ref_b = ref_b ? 1 : 0;
-    }
-    upload_jobs();
-    MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), st0));
-    const bool bicg = o.krylov == MPF_KRYLOV_BICGSTAB;
-    if (bicg) launch_bicg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, hjobs[0].lg, st0);
-    else launch_pcg_init(jobs.p, tiles.p, ntiles, js.p, njs, active.p, st0);
-    const int every = o.check_every > 0 ? o.check_every : 16;
-    const int lg = hjobs[0].lg, wpad = S.wpad;
-    const int ntile_job = S.np / kTileRows;
-    const size_t nst = (size_t)S.C * kStInts * frames;
-    // work lists restricted to the (frame, harmonic) pairs still iterating; rebuilt at
-    // every poll so converged harmonics cost no blocks
-    const int4* cur_tiles = tiles.p;
-    int cur_nt = ntiles;
-    int it = 0;
-    while (true) {
-        for (int i = 0; i < every; ++i) {
-            std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-            if (ctx->prof) ev = ctx->next_events();
-            if (bicg)
-                launch_bicg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
-            else
-                launch_pcg_iter(jobs.p, cur_tiles, cur_nt, active.p, it + i, lg, wpad, st0, ev.first, ev.second);
-        }
-        it += every;
-        MPF_HIP(hipMemcpyAsync(h_active, active.p, sizeof(int), hipMemcpyDeviceToHost, st0));
-        MPF_HIP(hipMemcpyAsync(h_st, st.p, nst * sizeof(int), hipMemcpyDeviceToHost, st0));
-        MPF_HIP(hipStreamSynchronize(st0));
-        if (ctx->prof) ctx->collect_events();
-        if (*h_active <= 0 || it >= maxit + every) break;
-        int nt = 0;
-        for (int f = 0; f < frames; ++f)
-            for (int c = 0; c < S.C; ++c) {
-                if (!h_st[((size_t)f * S.C + c) * kStInts]) continue;
-                for (int t = 0; t < ntile_job; ++t) h_tiles[nt++] = make_int4(f, c, t, 0);
-            }
-        if (nt == 0) break;
-        if (nt != cur_nt) {
-            int4* ordered = h_tiles + ntiles;   // second half of the pinned staging buffer
-            xcd_order(h_tiles, nt, ordered);
-            MPF_HIP(hipMemcpyAsync(act_tiles.p, ordered, sizeof(int4) * nt, hipMemcpyHostToDevice, st0));
-            // h_tiles is rewritten only after the next poll's synchronisation
-            cur_tiles = act_tiles.p; cur_nt = nt;
-        }
-    }
-    // step counts
-    const int* hst = h_st;
-    if (ctx->prof) {
-        // algorithmic bytes of one harmonic's SpMV with R right-hand sides: A_c values and
-        // column ids once (8 nnz), p read and q written once (8 N R)
-        int64_t slice_iters = 0;
-        for (size_t i = 0; i < (size_t)S.C * frames; ++i) slice_iters += hst[i * kStInts + 1];
-        ctx->prof_bytes += (double)slice_iters * (8.0 * S.nnz + 8.0 * S.N * S.Ru);
-    }
-    for (int f = 0; f < frames; ++f)
-        for (int c = 0; c < S.C; ++c) {
-            const int* sp = &hst[((size_t)f * S.C + c) * kStInts];
-            if (stats) {
-                stats->steps += sp[1];
-                stats->max_steps = std::max<int64_t>(stats->max_steps, sp[1]);
-                stats->not_converged += sp[2];
-            }
-            if (sp[2]) *capped = true;
-        }
-}
-
-int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats) {
+// Per-model set-up of a forward solve: Krylov method checks, the dc handling of the
+// job table, then Equation::update (S and R once per sigma, not once per harmonic) and
+// the Jacobi diagonal.
+void Model::prepare(const mpf_solve_opts& o) {
     hipStream_t st0 = ctx->stream;
     MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG || o.krylov == MPF_KRYLOV_BICGSTAB, "solve: unknown Krylov method");
     if (o.krylov == MPF_KRYLOV_BICGSTAB) {
@@ -244,46 +151,33 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
         J.dc = (o.krylov == MPF_KRYLOV_BICGSTAB && o.bicgstab_ignore_dc) ? -1 : 0;
         J.dc_rank1 = o.krylov == MPF_KRYLOV_CG ? 1 : 0;
     }
-    // Equation::update: S, R once per sigma (not once per harmonic), then Jacobi
+    upload_jobs();
     launch_assemble(jobs.p, asm_items.p, nasm, st0);
     launch_dinv(jobs.p, tiles.p, ntiles, st0);
-    bool capped = false;
+}
+
+// FP64 refinement report, voltages and (optionally) the Jacobian after the solves.
+void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats) {
+    hipStream_t st0 = ctx->stream;
     const bool mixed = o.precision == MPF_PREC_MIXED;
-    const bool warm = mixed && o.warm_start && x64_valid;
-    if (warm) {
-        // start from the previous FP64 fields: solve for the correction of the FP64
-        // residual at the new sigma, to the same absolute accuracy tol * ||beta F||
-        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
-        krylov(o, MPF_STOP_TOLERANCE, o.tol, true, true, stats, &capped);
-        launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
-    } else {
-        krylov(o, o.stop_rule, o.tol, false, false, stats, &capped);
-        if (mixed) launch_x64_update(jobs.p, tiles.p, ntiles, 1, st0);
-    }
-    x64_valid = mixed;
-    if (mixed) {
-        for (int k = 0; k < o.refine_steps; ++k) {
-            launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
-            krylov(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats, &capped);
-            launch_x64_update(jobs.p, tiles.p, ntiles, 0, st0);
-        }
-        // final FP64 residual for the report
-        launch_residual64(jobs.p, tiles.p, ntiles, js.p, njs, st0);
-        if (stats) {
-            std::vector<double> hsc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
-            MPF_HIP(hipMemcpyAsync(hsc.data(), sc.p, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, st0));
-            MPF_HIP(hipStreamSynchronize(st0));
-            double worst = 0.0;
-            for (int f = 0; f < frames; ++f)
-                for (int c = 0; c < S.C; ++c)
-                    for (int l = 0; l < S.Ru; ++l) {
-                        const double* b = &hsc[(size_t)SC_NFIELDS * S.C * S.Rp * f];
-                        const double rr = b[((size_t)SC_X1 * S.C + c) * S.Rp + l];
-                        const double bb = b[((size_t)SC_X2 * S.C + c) * S.Rp + l];
-                        if (bb > 0) worst = std::max(worst, std::sqrt(rr / bb));
+    if (mixed && stats) {
+        std::vector<double> hsc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
+        MPF_HIP(hipMemcpyAsync(hsc.data(), sc.p, hsc.size() * sizeof(double), hipMemcpyDeviceToHost, st0));
+        MPF_HIP(hipStreamSynchronize(st0));
+        double worst = stats->refine_residual;
+        for (int f = 0; f < frames; ++f)
+            for (int c = 0; c < S.C; ++c)
+                for (int l = 0; l < S.Ru; ++l) {
+                    const double* b = &hsc[(size_t)SC_NFIELDS * S.C * S.Rp * f];
+                    const double rr = b[((size_t)SC_X1 * S.C + c) * S.Rp + l];
+                    const double bb = b[((size_t)SC_X2 * S.C + c) * S.Rp + l];
+                    // NaN compares false: report it rather than skip it
+                    if (bb > 0) {
+                        const double v = std::sqrt(rr / bb);
+                        if (!(v <= worst)) worst = v;
                     }
-            stats->refine_residual = worst;
-        }
+                }
+        stats->refine_residual = worst;
     }
     MeasDev md{w_off.p, w_node.p, w_val.p, colmap.p, colscale.p, S.D, S.M, S.Dp, S.Mp};
     launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
@@ -300,6 +194,55 @@ int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
         }
         launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
     }
+}
+
+KrylovDriver& Model::driver() {
+    if (!drv) drv.reset(new KrylovDriver(ctx, {&hjobs}));
+    return *drv;
+}
+
+int Model::forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats) {
+    std::vector<Model*> ms{this};
+    return forward_models(ms, driver(), o, &d_U, &d_U64, &d_J, stats);
+}
+
+// One forward solve of every frame of every model (models::EIT::solve,
+// src/models/eit.cpp:139-213): the Krylov iterations of all models run in the same
+// launches (drv covers exactly these models' jobs); mixed precision adds FP64 residual
+// refinement passes.
+int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_opts& o, float* const* d_U,
+                   double* const* d_U64, float* const* d_J, mpf_solve_stats* stats) {
+    MPF_REQUIRE(!ms.empty(), "solve: no models");
+    hipStream_t st0 = ms[0]->ctx->stream;
+    for (Model* m : ms) m->prepare(o);
+    bool capped = false;
+    const bool mixed = o.precision == MPF_PREC_MIXED;
+    bool warm = mixed && o.warm_start;
+    for (Model* m : ms) warm = warm && m->x64_valid;
+    auto jobs_of = [](Model* m) { return m->jobs.p; };
+    if (warm) {
+        // start from the previous FP64 fields: solve for the correction of the FP64
+        // residual at the new sigma, to the same absolute accuracy tol * ||beta F||
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+        capped |= drv.run(o, MPF_STOP_TOLERANCE, o.tol, true, true, stats);
+        for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
+    } else {
+        capped |= drv.run(o, o.stop_rule, o.tol, false, false, stats);
+        if (mixed)
+            for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 1, st0);
+    }
+    for (Model* m : ms) m->x64_valid = mixed;
+    if (mixed) {
+        for (int k = 0; k < o.refine_steps; ++k) {
+            for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+            capped |= drv.run(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats);
+            for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
+        }
+        // final FP64 residual for the report
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+    }
+    for (size_t i = 0; i < ms.size(); ++i)
+        ms[i]->finish(o, d_U ? d_U[i] : nullptr, d_U64 ? d_U64[i] : nullptr, d_J ? d_J[i] : nullptr, stats);
     MPF_HIP(hipStreamSynchronize(st0));
     return capped ? MPF_ENOCONV : MPF_OK;
 }
@@ -334,12 +277,23 @@ void Model::ensure_gn() {
 
 int Model::gn_step(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine,
                    const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats) {
-    hipStream_t st0 = ctx->stream;
-    ensure_gn();
-    const int m = S.mrows, E = S.E;
-    MPF_HIP(hipMemcpyAsync(gamma.p, d_gamma, sizeof(float) * (size_t)E * frames, hipMemcpyDeviceToDevice, st0));
+    gn_begin(d_gamma);
     const int rc = forward(o, nullptr, U64.p, Jm.p, stats);
     if (rc != MPF_OK && rc != MPF_ENOCONV) return rc;
+    gn_dense(d_gamma, d_Umeas, lambda, gn_refine, d_Ucalc);
+    return rc;
+}
+
+void Model::gn_begin(const float* d_gamma) {
+    ensure_gn();
+    MPF_HIP(hipMemcpyAsync(gamma.p, d_gamma, sizeof(float) * (size_t)S.E * frames, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+}
+
+// The dense part of Solver::solveAbsolute after the forward solve has filled U64 and Jm.
+void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine, double* d_Ucalc) {
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
     if (d_Ucalc)
         MPF_HIP(hipMemcpyAsync(d_Ucalc, U64.p, sizeof(double) * (size_t)m * frames, hipMemcpyDeviceToDevice, st0));
     // r = U_meas - U(gamma)  (Inverse::calcExcitation, inverse.cpp:163-193)
@@ -366,6 +320,43 @@ int Model::gn_step(float* d_gamma, const double* d_Umeas, double lambda, int gn_
     MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
     MPF_HIP(hipStreamSynchronize(st0));
     if (hinfo) throw Error(MPF_EINVAL, "gn_step: normal-equation matrix is not positive definite");
+}
+
+Batch::Batch(mpf_ctx* c, std::vector<Model*> ms) : ctx(c), models(std::move(ms)) {
+    MPF_REQUIRE(!models.empty(), "batch: no models");
+    std::vector<std::vector<JobDev>*> src;
+    for (Model* m : models) {
+        MPF_REQUIRE(m->ctx == ctx, "batch: every model must belong to the batch's context");
+        src.push_back(&m->hjobs);
+    }
+    drv.reset(new KrylovDriver(ctx, src));
+}
+
+int Batch::solve(const float* const* d_gamma, float* const* d_U, double* const* d_U64, float* const* d_J,
+                 const mpf_solve_opts& o, mpf_solve_stats* stats) {
+    for (size_t i = 0; i < models.size(); ++i) {
+        Model& M = *models[i];
+        MPF_REQUIRE(d_gamma[i], "batch solve: null gamma");
+        MPF_HIP(hipMemcpyAsync(M.gamma.p, d_gamma[i], sizeof(float) * (size_t)M.S.E * M.frames,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return forward_models(models, *drv, o, d_U, d_U64, d_J, stats);
+}
+
+int Batch::gn_step(float* const* d_gamma, const double* const* d_Umeas, double lambda, int gn_refine,
+                   const mpf_solve_opts& o, double* const* d_Ucalc, mpf_solve_stats* stats) {
+    std::vector<double*> U64s;
+    std::vector<float*> Jms;
+    for (size_t i = 0; i < models.size(); ++i) {
+        MPF_REQUIRE(d_gamma[i] && d_Umeas[i], "batch gn_step: null argument");
+        models[i]->gn_begin(d_gamma[i]);
+        U64s.push_back(models[i]->U64.p);
+        Jms.push_back(models[i]->Jm.p);
+    }
+    const int rc = forward_models(models, *drv, o, nullptr, U64s.data(), Jms.data(), stats);
+    if (rc != MPF_OK && rc != MPF_ENOCONV) return rc;
+    for (size_t i = 0; i < models.size(); ++i)
+        models[i]->gn_dense(d_gamma[i], d_Umeas[i], lambda, gn_refine, d_Ucalc ? d_Ucalc[i] : nullptr);
     return rc;
 }
 

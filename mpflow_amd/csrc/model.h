@@ -3,8 +3,10 @@
 #pragma once
 
 #include <chrono>
+#include <memory>
 
 #include "dense.h"
+#include "krylov.h"
 
 struct mpf_ctx {
     int device = 0;
@@ -19,6 +21,7 @@ struct mpf_ctx {
     double prof_bytes = 0.0;       // algorithmic bytes of the timed launches
     std::pair<hipEvent_t, hipEvent_t> next_events();
     void collect_events();
+    mpf::AutotuneCache tune;
 };
 
 namespace mpf {
@@ -37,7 +40,6 @@ struct Model {
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;
     DBuf<float> rh, yv, sv, zv, tv;   // BiCGSTAB vectors (allocated on first use)
     DBuf<double> x64, part, sc;
-    DBuf<int> st;
     DBuf<float> gamma;
     // job tables
     std::vector<JobDev> hjobs;
@@ -45,11 +47,6 @@ struct Model {
     DBuf<int4> tiles, asm_items;
     DBuf<int2> js;
     int ntiles = 0, nasm = 0, njs = 0;
-    DBuf<int> active;
-    int* h_active = nullptr;   // pinned
-    DBuf<int4> act_tiles;      // work lists of the (frame, harmonic) pairs still active
-    int4* h_tiles = nullptr;   // pinned staging of the active lists
-    int* h_st = nullptr;       // pinned copy of the per-pair state
     // Gauss-Newton buffers (allocated on first use)
     DBuf<float> Jm, G;
     DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;
@@ -66,13 +63,34 @@ struct Model {
     int forward(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats);
     int gn_step(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine,
                 const mpf_solve_opts& o, double* d_Ucalc, mpf_solve_stats* stats);
+    // phases of a forward solve (shared with batched solves of several models)
+    void prepare(const mpf_solve_opts& o);
+    void finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_J, mpf_solve_stats* stats);
+    bool x64_valid = false;
+    // Gauss-Newton step in two phases around the forward solve (batched GN)
+    void gn_begin(const float* d_gamma);
+    void gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine, double* d_Ucalc);
 
   private:
     void ensure_bicg();
-    void krylov(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
-                mpf_solve_stats* stats, bool* capped);
-    bool x64_valid = false;
     void ensure_gn();
+    KrylovDriver& driver();
+    std::unique_ptr<KrylovDriver> drv;
+};
+
+int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_opts& o, float* const* d_U,
+                   double* const* d_U64, float* const* d_J, mpf_solve_stats* stats);
+
+// several models solved together (one set of Krylov launches per iteration)
+struct Batch {
+    mpf_ctx* ctx = nullptr;
+    std::vector<Model*> models;
+    std::unique_ptr<KrylovDriver> drv;
+    Batch(mpf_ctx* c, std::vector<Model*> ms);
+    int solve(const float* const* d_gamma, float* const* d_U, double* const* d_U64, float* const* d_J,
+              const mpf_solve_opts& o, mpf_solve_stats* stats);
+    int gn_step(float* const* d_gamma, const double* const* d_Umeas, double lambda, int gn_refine,
+                const mpf_solve_opts& o, double* const* d_Ucalc, mpf_solve_stats* stats);
 };
 
 }  // namespace mpf

@@ -543,13 +543,12 @@ void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles, cons
 
 // SpMV variant 0: 128-bit gathers (k_pcg_spmv4); variant 1: 32-bit gathers with
 // v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner).
-int g_spmv_variant = 0;
-
 template <int LG>
-static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st) {
+static void spmv_dispatch(int wpad, int variant, const JobDev* d_jobs, const int4* d_tiles, int ntiles,
+                          hipStream_t st) {
     constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
     constexpr int U2 = STEPS >= 2 ? 2 : 1;
-    if (g_spmv_variant == 0) {
+    if (variant == 0) {
         switch (wpad) {
             case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
             case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
@@ -570,14 +569,19 @@ static void spmv_dispatch(int wpad, const JobDev* d_jobs, const int4* d_tiles, i
     }
 }
 
+void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
+                     hipStream_t st) {
+    if (lg == 6) spmv_dispatch<6>(wpad, variant, d_jobs, d_tiles, ntiles, st);
+    else if (lg == 5) spmv_dispatch<5>(wpad, variant, d_jobs, d_tiles, ntiles, st);
+    else if (lg == 4) spmv_dispatch<4>(wpad, variant, d_jobs, d_tiles, ntiles, st);
+    else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
+}
+
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
-                     int lg, int wpad, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+                     int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const int par = iter & 1;
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
-    if (lg == 6) spmv_dispatch<6>(wpad, d_jobs, d_tiles, ntiles, st);
-    else if (lg == 5) spmv_dispatch<5>(wpad, d_jobs, d_tiles, ntiles, st);
-    else if (lg == 4) spmv_dispatch<4>(wpad, d_jobs, d_tiles, ntiles, st);
-    else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
+    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st);
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
     if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
     else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);

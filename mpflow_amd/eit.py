@@ -15,7 +15,7 @@ import numpy as np
 from . import capi
 from .capi import check, lib
 
-__all__ = ["Context", "EIT", "Solver", "circular_ports", "SolveOptions"]
+__all__ = ["Context", "EIT", "Solver", "Batch", "BatchSolver", "circular_ports", "SolveOptions"]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -71,7 +71,7 @@ def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwis
 class SolveOptions:
     def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
                  refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0,
-                 bicgstab_ignore_dc=0):
+                 bicgstab_ignore_dc=0, graphs=0):
         self.s = capi.SolveOpts()
         lib().mpf_solve_opts_default(C.byref(self.s))
         self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
@@ -79,6 +79,7 @@ class SolveOptions:
         self.s.krylov, self.s.check_every = krylov, check_every
         self.s.warm_start = int(warm_start)
         self.s.bicgstab_ignore_dc = int(bicgstab_ignore_dc)
+        self.s.graphs = int(graphs)
 
     def copy(self, **kw):
         o = SolveOptions()
@@ -214,3 +215,78 @@ class Solver:
         check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
         self.last_stats = st
         return self.material_distribution
+
+
+def _ptrs(tensors):
+    arr = (C.c_void_p * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = None if t is None else t.data_ptr()
+    return arr
+
+
+class Batch:
+    """Several ``EIT`` models (any mesh, electrode and harmonic counts) solved together:
+    one set of Krylov launches per iteration for all of them (mpf_batch_*).  The models
+    must share one ``Context``; close the batch before its models."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        ctx = self.models[0].ctx
+        arr = (C.c_void_p * len(self.models))(*[m.h.value for m in self.models])
+        h = C.c_void_p()
+        check(lib().mpf_batch_create(ctx.h, arr, len(self.models), C.byref(h)))
+        self.h = h
+        self.last_stats = None
+
+    def solve(self, gammas, options: SolveOptions | None = None, jacobian=False):
+        """EIT::solve of every model; ``gammas[i]`` as for ``EIT.solve``.  Results land in
+        each model's ``result`` / ``result64`` / ``jacobian``."""
+        torch = self.models[0].torch
+        gs, Js = [], []
+        for m, g in zip(self.models, gammas):
+            gs.append(g.reshape(m.frames, m.E).to(m._dev(), torch.float32).contiguous())
+            if jacobian and m.jacobian is None:
+                m.jacobian = torch.empty(m.frames, m.E, m.m, dtype=torch.float32, device=m._dev())
+            Js.append(m.jacobian if jacobian else None)
+        st = capi.SolveStats()
+        opts = (options or SolveOptions()).s
+        rc = lib().mpf_batch_solve(self.h, _ptrs(gs), _ptrs([m.result for m in self.models]),
+                                   _ptrs([m.result64 for m in self.models]), _ptrs(Js), C.byref(opts),
+                                   C.byref(st))
+        check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
+        self.last_stats = st
+        return [m.result for m in self.models]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchSolver:
+    """``Solver.solve_absolute`` of several solvers (one per model of a ``Batch``) with the
+    forward solves batched."""
+
+    def __init__(self, batch: Batch, solvers):
+        self.batch = batch
+        self.solvers = list(solvers)
+        assert [s.eit for s in self.solvers] == batch.models
+        self.last_stats = None
+
+    def solve_absolute(self, warm_start: bool = False):
+        s0 = self.solvers[0]
+        st = capi.SolveStats()
+        opts = s0.options.copy(warm_start=int(bool(warm_start)))
+        rc = lib().mpf_batch_gn_step(self.batch.h, _ptrs([s.material_distribution for s in self.solvers]),
+                                     _ptrs([s.measurement for s in self.solvers]), s0.regularization_factor,
+                                     s0.gn_refine, C.byref(opts.s),
+                                     _ptrs([s.calculation for s in self.solvers]), C.byref(st))
+        check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
+        self.last_stats = st
+        return [s.material_distribution for s in self.solvers]
