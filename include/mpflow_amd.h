@@ -137,6 +137,12 @@ typedef struct mpf_model_desc {
     int32_t keep_node_order;      /* 1: solve in the caller's node numbering; 0: renumber
                                      internally for gather locality (results are the same
                                      up to FP32 rounding; outputs never expose node order) */
+    double contact_impedance;     /* > 0: complete electrode model with this contact
+                                     impedance on every port (no counterpart in the reference,
+                                     whose ports use the gap model); electrode potentials
+                                     become extra unknowns, current enters through them and
+                                     the voltages are the electrode potentials; each harmonic
+                                     carries its own CEM system.  0: gap model (reference) */
 } mpf_model_desc;
 
 typedef struct mpf_solve_opts {

@@ -34,7 +34,7 @@ class ModelDesc(C.Structure):
         ("reference_value", C.c_double), ("components", C.c_int32), ("height", C.c_double),
         ("port_height", C.c_double), ("logarithmic", C.c_int32), ("quirks", C.c_int32),
         ("jacobian_exact", C.c_int32), ("ell_max_width", C.c_int32), ("frames", C.c_int32),
-        ("keep_node_order", C.c_int32),
+        ("keep_node_order", C.c_int32), ("contact_impedance", C.c_double),
     ]
 
 

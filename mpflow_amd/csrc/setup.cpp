@@ -173,6 +173,32 @@ std::vector<std::vector<int32_t>> circular_ports(int N, const double* nodes, int
 // ---------------------------------------------------------------------------
 // full set-up
 // ---------------------------------------------------------------------------
+// Lagrange basis of one straight boundary edge with nodes at distances sp[0] = 0 <
+// ... < sp[kb-1] from its first node: m[a][b] = int phi_a phi_b ds and v[a] = int phi_a ds,
+// by 3-point Gauss-Legendre (exact up to degree 5).
+static void edge_integrals(int kb, const double* sp, double (*m)[3], double* v) {
+    static const double xg[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+    static const double wg[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    const double L = sp[kb - 1];
+    for (int a = 0; a < kb; ++a) {
+        v[a] = 0.0;
+        for (int b = 0; b < kb; ++b) m[a][b] = 0.0;
+    }
+    for (int g = 0; g < 3; ++g) {
+        const double t = 0.5 * L * (xg[g] + 1.0), w = 0.5 * L * wg[g];
+        double phi[3];
+        for (int a = 0; a < kb; ++a) {
+            phi[a] = 1.0;
+            for (int c = 0; c < kb; ++c)
+                if (c != a) phi[a] *= (t - sp[c]) / (sp[a] - sp[c]);
+        }
+        for (int a = 0; a < kb; ++a) {
+            v[a] += w * phi[a];
+            for (int b = 0; b < kb; ++b) m[a][b] += w * phi[a] * phi[b];
+        }
+    }
+}
+
 // Device node numbering.  Reverse Cuthill-McKee over the element graph (BFS from a
 // pseudo-peripheral node, neighbours visited by ascending degree, order reversed) keeps the
 // neighbours of every 256-row tile inside a narrow band, so the SpMV's gathers of p stay in
@@ -281,8 +307,12 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     MPF_REQUIRE(d.height > 0.0, "model: height <= 0.0");
     MPF_REQUIRE(d.port_height > 0.0, "model: portHeight <= 0.0");
     const int N = d.n_nodes, E = d.n_elements, k = d.nodes_per_element;
-    s.N = N; s.E = E; s.k = k;
-    s.np = round_up(N, kTileRows);
+    MPF_REQUIRE(d.contact_impedance >= 0.0, "model: contact_impedance < 0");
+    const bool cem = d.contact_impedance > 0.0;
+    // unknowns: the mesh nodes, then (complete electrode model) one potential per port
+    const int Nu = N + (cem ? d.n_ports : 0);
+    s.N = Nu; s.E = E; s.k = k;
+    s.np = round_up(Nu, kTileRows);
     s.P = d.n_ports; s.D = d.n_drive; s.M = d.n_meas; s.R = s.D + s.M;
     MPF_REQUIRE(s.R <= 64, "model: D + M > 64 is not supported");
     // Pi = [I*Dpat | Mpat] (sources.cpp:57-71); columns that are scalar multiples of an
@@ -357,25 +387,53 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     for (size_t i = 0; i < Ke.size(); ++i) { s.elem_k[i] = (float)Ke[i]; s.elem_m[i] = (float)Me[i]; }
 
     // ELL pattern: first-insertion order of the element loop (equation.cpp:131-144)
-    std::vector<int32_t> cols((size_t)N * cap, -1);
-    std::vector<int32_t> cnt(N, 0);
+    std::vector<int32_t> cols((size_t)Nu * cap, -1);
+    std::vector<int32_t> cnt(Nu, 0);
     std::vector<int32_t> slot_of((size_t)E * k * k);
     int width = 0;
+    auto slot = [&](int row, int col) {
+        int32_t* rc = &cols[(size_t)row * cap];
+        for (int q = 0; q < cnt[row]; ++q) if (rc[q] == col) return q;
+        if (cnt[row] >= cap)
+            throw Error(MPF_EFULL, "mpFlow::numeric::SparseMatrix::setValue: sparse format full");
+        const int sl = cnt[row]++;
+        rc[sl] = col;
+        width = std::max(width, cnt[row]);
+        return sl;
+    };
     for (int e = 0; e < E; ++e) {
         const int32_t* el = d.elements + (size_t)e * k;
-        for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j) {
-            int row = el[i], col = el[j];
-            int32_t* rc = &cols[(size_t)row * cap];
-            int sl = -1;
-            for (int q = 0; q < cnt[row]; ++q) if (rc[q] == col) { sl = q; break; }
-            if (sl < 0) {
-                if (cnt[row] >= cap)
-                    throw Error(MPF_EFULL, "mpFlow::numeric::SparseMatrix::setValue: sparse format full");
-                sl = cnt[row]++;
-                rc[sl] = col;
-                width = std::max(width, cnt[row]);
+        for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j)
+            slot_of[((size_t)e * k + i) * k + j] = slot(el[i], el[j]);
+    }
+    // complete electrode model: sigma-independent blocks (1/z) int phi_i phi_j, -(1/z)
+    // int phi_i and |e|/z on the port edges, kept as (row, slot, value) contributions
+    struct Const { int row, sl; double v; };
+    std::vector<Const> consts;
+    if (cem) {
+        const double iz = 1.0 / d.contact_impedance;
+        const int kb = d.nodes_per_edge;
+        for (int p = 0; p < s.P; ++p) {
+            const int ep = N + p;
+            for (int q = 0; q < d.port_max_edges; ++q) {
+                const int eidx = d.port_edges[(size_t)p * d.port_max_edges + q];
+                if (eidx < 0) continue;
+                MPF_REQUIRE(eidx < d.n_boundary, "model: port edge index out of range");
+                const int32_t* ed = d.boundary + (size_t)eidx * kb;
+                double sp[3], m[3][3], v[3];
+                for (int a = 0; a < kb; ++a) {
+                    const double dx = d.nodes[2 * ed[a]] - d.nodes[2 * ed[0]];
+                    const double dy = d.nodes[2 * ed[a] + 1] - d.nodes[2 * ed[0] + 1];
+                    sp[a] = std::sqrt(dx * dx + dy * dy);
+                }
+                edge_integrals(kb, sp, m, v);
+                for (int a = 0; a < kb; ++a) {
+                    for (int b = 0; b < kb; ++b) consts.push_back({ed[a], slot(ed[a], ed[b]), iz * m[a][b]});
+                    consts.push_back({ed[a], slot(ed[a], ep), -iz * v[a]});
+                    consts.push_back({ep, slot(ep, ed[a]), -iz * v[a]});
+                }
+                consts.push_back({ep, slot(ep, ep), iz * sp[kb - 1]});
             }
-            slot_of[((size_t)e * k + i) * k + j] = sl;
         }
     }
     s.width = width;
@@ -387,14 +445,14 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     s.nnz = 0;
     for (int row = 0; row < np; ++row) {
         for (int q = 0; q < wp; ++q) {
-            int col = (row < N && q < cnt[row]) ? cols[(size_t)row * cap + q] : row;
+            int col = (row < Nu && q < cnt[row]) ? cols[(size_t)row * cap + q] : row;
             s.ids[(size_t)q * np + row] = col;
-            if (row < N && q < cnt[row] && col == row) s.diag_slot[row] = q;
+            if (row < Nu && q < cnt[row] && col == row) s.diag_slot[row] = q;
         }
-        if (row < N) s.nnz += cnt[row];
+        if (row < Nu) s.nnz += cnt[row];
     }
     s.group_width.assign(np / 16, 0);
-    for (int row = 0; row < N; ++row)
+    for (int row = 0; row < Nu; ++row)
         s.group_width[row / 16] = (uint8_t)std::max<int>(s.group_width[row / 16], cnt[row]);
     // gather plan, contributions in element order per ELL entry
     const size_t nent = (size_t)wp * np;
@@ -404,8 +462,9 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         for (int i = 0; i < k; ++i) for (int j = 0; j < k; ++j)
             s.ctb_off[(size_t)slot_of[((size_t)e * k + i) * k + j] * np + el[i] + 1]++;
     }
+    for (const auto& c : consts) s.ctb_off[(size_t)c.sl * np + c.row + 1]++;
     for (size_t i = 0; i < nent; ++i) s.ctb_off[i + 1] += s.ctb_off[i];
-    const size_t total = (size_t)E * k * k;
+    const size_t total = (size_t)E * k * k + consts.size();
     s.ctb_elem.resize(total); s.ctb_k.resize(total); s.ctb_m.resize(total);
     std::vector<int32_t> fill(s.ctb_off.begin(), s.ctb_off.end() - 1);
     for (int e = 0; e < E; ++e) {
@@ -418,11 +477,20 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
             s.ctb_m[pos] = Me[((size_t)e * k + i) * k + j];
         }
     }
+    for (const auto& c : consts) {
+        const int32_t pos = fill[(size_t)c.sl * np + c.row]++;
+        s.ctb_elem[pos] = -1;   // sigma-independent
+        s.ctb_k[pos] = c.v;
+        s.ctb_m[pos] = 0.0;
+    }
 
-    // gap-model excitation B (equation.cpp:183-215); B stored sparse per node
+    // gap-model excitation B (equation.cpp:183-215); B stored sparse per node.  With the
+    // complete electrode model the current enters at the electrode rows instead: B = [0; I].
     const int kb = d.nodes_per_edge;
-    std::vector<std::map<int, double>> Bn(N);
-    for (int p = 0; p < s.P; ++p) {
+    std::vector<std::map<int, double>> Bn(Nu);
+    if (cem)
+        for (int p = 0; p < s.P; ++p) Bn[N + p][p] = 1.0;
+    for (int p = 0; p < (cem ? 0 : s.P); ++p) {
         std::vector<int> edges;
         for (int q = 0; q < d.port_max_edges; ++q) {
             int eidx = d.port_edges[(size_t)p * d.port_max_edges + q];
@@ -469,16 +537,17 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     std::vector<int> rep(s.Ru, -1);   // first original column of each unique column
     for (int j = 0; j < s.R; ++j)
         if (rep[s.colmap[j]] < 0) rep[s.colmap[j]] = j;
-    for (int n = 0; n < N; ++n)
+    for (int n = 0; n < Nu; ++n)
         for (auto& kv : Bn[n])
             for (int u = 0; u < s.Ru; ++u) s.F[(size_t)n * s.Rp + u] += kv.second * Pi(kv.first, rep[u]);
     // a representative column has scale 1 by construction
     for (int u = 0; u < s.Ru; ++u) s.colscale[rep[u]] = 1.0;
-    // W = Mpat^T B^T  (eit.cpp:93-94) as CSR
+    // W = Mpat^T B^T  (eit.cpp:93-94) as CSR (with the electrode model: Mpat^T on the
+    // electrode potentials)
     s.w_off.assign(s.M + 1, 0);
     s.w_node.clear(); s.w_val.clear();
     for (int m = 0; m < s.M; ++m) {
-        for (int n = 0; n < N; ++n) {
+        for (int n = 0; n < Nu; ++n) {
             double v = 0.0;
             bool any = false;
             for (auto& kv : Bn[n]) { v += d.meas_pattern[(size_t)kv.first * s.M + m] * kv.second; any = true; }

@@ -35,7 +35,9 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const JobDev* __restrict__ 
     const int b = J.ctb_off[ent], e = J.ctb_off[ent + 1];
     double S = 0.0, R = 0.0;
     for (int i = b; i < e; ++i) {
-        const double sg = sigma_of(J, J.gamma[J.ctb_elem[i]]);
+        // element -1: a sigma-independent contribution (complete-electrode-model blocks)
+        const int el = J.ctb_elem[i];
+        const double sg = el < 0 ? 1.0 : sigma_of(J, J.gamma[el]);
         S += sg * J.ctb_k[i];
         R += sg * J.ctb_m[i];
     }

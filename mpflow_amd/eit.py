@@ -95,7 +95,7 @@ class EIT:
     def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
                  quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None,
-                 keep_node_order=False):
+                 keep_node_order=False, contact_impedance=0.0):
         import torch
         self.ctx = ctx or Context()
         self.torch = torch
@@ -127,6 +127,7 @@ class EIT:
         d.logarithmic, d.quirks, d.jacobian_exact = int(logarithmic), int(quirks), int(jacobian_exact)
         d.ell_max_width, d.frames = int(ell_max_width), int(frames)
         d.keep_node_order = int(bool(keep_node_order))
+        d.contact_impedance = float(contact_impedance)
         h = C.c_void_p()
         check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h

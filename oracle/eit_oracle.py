@@ -237,6 +237,54 @@ def excitation_matrix(nodes, boundary, ports, quirks=True):
     return B
 
 
+def edge_integrals(s: np.ndarray):
+    """Exact 1D Lagrange-basis integrals on one straight boundary edge whose nodes sit
+    at distances ``s`` (s[0] = 0) from its first node: (m, v) with m[a, b] = int phi_a
+    phi_b ds and v[a] = int phi_a ds over [0, s_last] (3-point Gauss-Legendre, exact for
+    the degree <= 4 products of P2)."""
+    n = s.size
+    V = np.vander(s, n, increasing=True)
+    coef = np.linalg.solve(V, np.eye(n))            # column a: coefficients of phi_a
+    xg, wg = np.polynomial.legendre.leggauss(3)
+    L = s[-1]
+    t = 0.5 * L * (xg + 1.0)
+    w = 0.5 * L * wg
+    phi = np.vander(t, n, increasing=True) @ coef   # phi[q, a]
+    return (phi * w[:, None]).T @ phi, (phi * w[:, None]).sum(axis=0)
+
+
+def cem_blocks(nodes, boundary, ports, z):
+    """Complete-electrode-model blocks for contact impedance z on every port (an
+    extension: the reference has the gap model only).  With electrode potentials V_l as
+    extra unknowns the system is
+
+        [ A + Zc   -Cc ] [phi]   [ 0 ]
+        [ -Cc^T     Dc ] [ V ] = [ I ]
+
+    Zc[i, j] = sum_l (1/z) int_{e_l} phi_i phi_j, Cc[i, l] = (1/z) int_{e_l} phi_i,
+    Dc[l, l] = |e_l| / z.  Returns (Zc as CSR N x N, Cc dense N x L, Dc diagonal (L,))."""
+    N = nodes.shape[0]
+    L = ports.shape[0]
+    rows, cols, vals = [], [], []
+    Cc = np.zeros((N, L))
+    Dc = np.zeros(L)
+    for l in range(L):
+        for e in ports[l]:
+            if e < 0:
+                continue
+            ed = boundary[e]
+            pts = nodes[ed]
+            sd = np.sqrt(np.sum((pts - pts[0]) ** 2, axis=1))
+            m, v = edge_integrals(sd)
+            for a in range(len(ed)):
+                Cc[ed[a], l] += v[a] / z
+                for b_ in range(len(ed)):
+                    rows.append(ed[a]); cols.append(ed[b_]); vals.append(m[a, b_] / z)
+            Dc[l] += sd[-1] / z
+    Zc = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    return Zc, Cc, Dc
+
+
 def adjacent_pattern(L):
     """Column d = e_d - e_{d+1 mod L} (drive and measurement are both adjacent)."""
     P = np.zeros((L, L))
@@ -420,7 +468,7 @@ class EITOracle:
 
     def __init__(self, nodes, elements, boundary, ports, drive, meas, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0,
-                 quirks=True, logarithmic=False):
+                 quirks=True, logarithmic=False, contact_impedance=0.0):
         self.nodes = np.asarray(nodes, np.float64)
         self.elements = np.asarray(elements, np.int64)
         self.boundary = np.asarray(boundary, np.int64)
@@ -438,10 +486,20 @@ class EITOracle:
         self.quirks = quirks
         self.log = logarithmic
         self.Ke, self.Me = all_element_integrals(self.nodes, self.elements)
-        self.B = excitation_matrix(self.nodes, self.boundary, self.ports, quirks)
-        self.W = self.meas.T @ self.B.T          # electrodesAttachmentMatrix (eit.cpp:93-94)
         self.Mp = ceil16(self.M)
         self.Dp = ceil16(self.D)
+        self.z = float(contact_impedance)
+        self.L = self.ports.shape[0]
+        if self.z > 0.0:
+            # complete electrode model: unknowns [phi (N); V (L)]; current enters at the
+            # electrode rows and the voltages are the electrode potentials.  Every harmonic
+            # carries its own CEM system with the same electrode blocks (harmonic-wise CEM).
+            self.Zc, self.Cc, self.Dc = cem_blocks(self.nodes, self.boundary, self.ports, self.z)
+            self.B = np.vstack([np.zeros((self.N, self.L)), np.eye(self.L)])
+            self.W = np.hstack([np.zeros((self.M, self.N)), self.meas.T])
+        else:
+            self.B = excitation_matrix(self.nodes, self.boundary, self.ports, quirks)
+            self.W = self.meas.T @ self.B.T          # electrodesAttachmentMatrix (eit.cpp:93-94)
 
     def alpha_beta(self, c):
         """2.5D constants (eit.cpp:154-157)."""
@@ -455,7 +513,11 @@ class EITOracle:
         S = assemble(self.elements, self.Ke, self.N, sig)
         R = assemble(self.elements, self.Me, self.N, sig)
         alpha, _ = self.alpha_beta(c)
-        return (S + alpha * R).tocsr()
+        A = S + alpha * R
+        if self.z > 0.0:
+            A = sp.bmat([[A + self.Zc, sp.csr_matrix(-self.Cc)],
+                         [sp.csr_matrix(-self.Cc.T), sp.diags(self.Dc)]])
+        return A.tocsr()
 
     def fields(self, gamma, method="direct", **cg):
         """Phi_c (N x (D+M)) for every harmonic; also returns CG step counts."""

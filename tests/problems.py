@@ -18,13 +18,16 @@ def make_problem(n_rings=16, L=16, width=0.1, C=1, height=1.0, port_height=1.0, 
                 meas=pat, current=current, C=C, height=height, port_height=port_height)
 
 
-def oracle_of(p, quirks=True):
+def oracle_of(p, quirks=True, contact_impedance=0.0):
     return EITOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
-                     p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks)
+                     p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks,
+                     contact_impedance=contact_impedance)
 
 
-def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False, keep_node_order=False):
+def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False, keep_node_order=False,
+           contact_impedance=0.0):
     from mpflow_amd.eit import EIT
     return EIT(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
                p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks,
-               jacobian_exact=jacobian_exact, frames=frames, ctx=ctx, keep_node_order=keep_node_order)
+               jacobian_exact=jacobian_exact, frames=frames, ctx=ctx, keep_node_order=keep_node_order,
+               contact_impedance=contact_impedance)

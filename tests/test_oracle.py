@@ -398,3 +398,82 @@ def test_cpu_baseline_matches_direct(small):
     X, steps, _ = baseline.pcg(A, F, 1, tol=1e-6)
     Xd = eo.solve_direct(A, F, True)
     assert np.max(np.abs(X - Xd)) < 1e-5 * np.max(np.abs(Xd))
+
+
+# ----------------------------------------------------------------------------------
+# complete electrode model (an extension without a reference counterpart)
+# ----------------------------------------------------------------------------------
+def _cem_oracle(p, z, **kw):
+    return eo.EITOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
+                        p["current"], 1.0, p["C"], p["height"], p["port_height"], contact_impedance=z, **kw)
+
+
+@pytest.mark.parametrize("p2,C", [(False, 1), (True, 1), (False, 3)])
+def test_cem_currents_and_power_balance(p2, C):
+    """Per drive pattern and harmonic: the electrode currents (1/z) int_e (V_l - phi) ds,
+    integrated by 5-point Gauss-Legendre along each electrode edge, equal the injected
+    pattern currents; and the input power V^T I equals phi^T (S + alpha R) phi plus the
+    contact losses (1/z) int_e (phi - V_l)^2 ds, integrated the same way."""
+    p = make_problem(8, 8, 0.3, C=C, height=0.4 if C > 1 else 1.0, port_height=0.1 if C > 1 else 1.0, p2=p2)
+    z = 0.01
+    o = _cem_oracle(p, z, quirks=False)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 6))
+    sig = eo.mat_param(g, 1.0, False)
+    S = eo.assemble(o.elements, o.Ke, o.N, sig)
+    R = eo.assemble(o.elements, o.Me, o.N, sig)
+    xg, wg = np.polynomial.legendre.leggauss(5)
+    for c in range(C):
+        alpha, beta = o.alpha_beta(c)
+        X = eo.solve_direct(o.system(g, c), beta * (o.B @ o.Pi), c == 0)
+        for d in range(o.D):
+            phi, V = X[:o.N, d], X[o.N:, d]
+            I = beta * o.Pi[:, d]
+            cur = np.zeros(o.L)
+            loss = 0.0
+            for l in range(o.L):
+                for e in p["ports"][l]:
+                    if e < 0:
+                        continue
+                    ed = p["boundary"][e]
+                    pts = p["nodes"][ed]
+                    s = np.sqrt(np.sum((pts - pts[0]) ** 2, axis=1))
+                    n = len(ed)
+                    coef = np.linalg.solve(np.vander(s, n, increasing=True), np.eye(n))
+                    t = 0.5 * s[-1] * (xg + 1.0)
+                    u = np.vander(t, n, increasing=True) @ coef @ phi[ed]
+                    cur[l] += np.sum(0.5 * s[-1] * wg * (V[l] - u)) / z
+                    loss += np.sum(0.5 * s[-1] * wg * (u - V[l]) ** 2) / z
+            np.testing.assert_allclose(cur, I, atol=1e-9 * np.max(np.abs(I)))
+            power = V @ I
+            assert abs(power - (phi @ (S @ phi) + alpha * phi @ (R @ phi) + loss)) < 1e-9 * abs(power)
+
+
+def test_cem_reciprocity_and_jacobian_fd():
+    """2D CEM: U = U^T for identical drive and measurement patterns, and the Jacobian
+    (node part of the fields) equals central finite differences of U."""
+    p = make_problem(8, 8, 0.3)
+    o = _cem_oracle(p, 0.01)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 7))
+    U, phis, _ = o.solve(g)
+    np.testing.assert_allclose(U, U.T, atol=1e-12 * np.max(np.abs(U)))
+    J = o.jacobian(g, phis)
+    elems = [0, 9, 60, o.E - 1]
+    Jfd = _fd_jacobian(o, g, elems)
+    np.testing.assert_allclose(J[:, elems], Jfd, atol=1e-7 * np.max(np.abs(Jfd)))
+
+
+def test_cem_approaches_shunt_limit():
+    """As z -> 0 the electrode potentials become the nodal potentials on the electrode
+    (shunt model): the spread of phi over each electrode shrinks in proportion to z."""
+    p = make_problem(8, 8, 0.3)
+    g = np.ones(p["elements"].shape[0])
+    spreads = []
+    for z in (1e-2, 1e-4):
+        o = _cem_oracle(p, z)
+        X = eo.solve_direct(o.system(g, 0), o.B @ o.Pi, True)
+        s = 0.0
+        for l in range(o.L):
+            nodes_l = np.unique(p["boundary"][[e for e in p["ports"][l] if e >= 0]])
+            s = max(s, np.max(np.abs(X[nodes_l, 0] - X[o.N + l, 0])))
+        spreads.append(s)
+    assert spreads[1] < 0.05 * spreads[0]
