@@ -54,6 +54,7 @@ extern "C" {
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
 typedef struct mpf_batch mpf_batch;
+typedef struct mpf_mlp mpf_mlp;
 
 /* Library version string (replaces version::getVersionString, src/version.cpp:23-25). */
 const char* mpf_version(void);
@@ -257,6 +258,26 @@ int mpf_gemv_t(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_A, int32
  * the device; b, y FP64. */
 int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const double* d_b,
                   double* d_y);
+
+/* ---------------------------------------------------------------------------
+ * Learned start: a multilayer perceptron (dense layers, GELU between them, linear
+ * output) trained with MSE and Adam.  No counterpart in the reference.  GEMMs run on
+ * MFMA with BF16 operands and FP32 accumulation; parameters, gradients and Adam moments
+ * are FP32.  Matrices are row-major: X batch x dims[0], Y batch x dims[n_layers],
+ * layer l's weights dims[l] x dims[l+1].
+ * ------------------------------------------------------------------------- */
+int mpf_mlp_create(mpf_ctx* ctx, int32_t n_layers, const int32_t* dims, uint64_t seed, mpf_mlp** out);
+int mpf_mlp_destroy(mpf_mlp* m);
+/* Y (nullable) = net(X); keeps the activations for mpf_mlp_bwd (X must stay valid). */
+int mpf_mlp_fwd(mpf_mlp* m, int32_t batch, const float* d_X, float* d_Y);
+/* gradients of mean((Y - T)^2) for the last forward; *loss (nullable) receives the MSE */
+int mpf_mlp_bwd(mpf_mlp* m, const float* d_T, double* loss);
+/* one Adam step (bias-corrected) on the current gradients */
+int mpf_mlp_adam(mpf_mlp* m, float lr, float beta1, float beta2, float eps);
+/* number of parameters; flat order W_0, b_0, W_1, b_1, ... */
+int64_t mpf_mlp_param_count(const mpf_mlp* m);
+/* which 0: parameters (get or set), 1: gradients (get); host buffer of param_count floats */
+int mpf_mlp_params(mpf_mlp* m, int32_t which, int32_t set, float* host);
 
 #ifdef __cplusplus
 }

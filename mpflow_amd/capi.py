@@ -18,7 +18,8 @@ EXPORTED = [
     "mpf_model_destroy", "mpf_model_dims", "mpf_eit_solve", "mpf_gn_step", "mpf_ell_spmv",
     "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read",
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
-    "mpf_batch_gn_step",
+    "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
+    "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params",
 ]
 
 
@@ -94,6 +95,14 @@ def lib() -> C.CDLL:
     L.mpf_batch_destroy.argtypes = [vp]
     L.mpf_batch_solve.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.mpf_batch_gn_step.argtypes = [vp, vp, vp, dbl, i32, C.POINTER(SolveOpts), vp, C.POINTER(SolveStats)]
+    L.mpf_mlp_create.argtypes = [vp, i32, vp, C.c_uint64, C.POINTER(vp)]
+    L.mpf_mlp_destroy.argtypes = [vp]
+    L.mpf_mlp_fwd.argtypes = [vp, i32, vp, vp]
+    L.mpf_mlp_bwd.argtypes = [vp, vp, C.POINTER(dbl)]
+    L.mpf_mlp_adam.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.mpf_mlp_param_count.argtypes = [vp]
+    L.mpf_mlp_param_count.restype = i64
+    L.mpf_mlp_params.argtypes = [vp, i32, i32, vp]
     _lib = L
     return L
 

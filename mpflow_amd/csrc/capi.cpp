@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "mlp.h"
 #include "model.h"
 
 using mpf::Error;
@@ -37,6 +38,11 @@ struct mpf_model {
 
 struct mpf_batch {
     mpf::Batch* impl;
+};
+
+struct mpf_mlp {
+    mpf::Mlp* impl;
+    mpf_ctx* ctx;
 };
 
 extern "C" {
@@ -359,3 +365,75 @@ int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const
 }
 
 }  // extern "C"
+
+int mpf_mlp_create(mpf_ctx* ctx, int32_t n_layers, const int32_t* dims, uint64_t seed, mpf_mlp** out) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && dims && out && n_layers >= 1, "mpf_mlp_create: null argument or no layers");
+        set_device(ctx);
+        std::vector<int> d(dims, dims + n_layers + 1);
+        auto* m = new mpf_mlp;
+        m->ctx = ctx;
+        try {
+            m->impl = new mpf::Mlp(ctx->stream, d, seed);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return MPF_OK;
+    });
+}
+
+int mpf_mlp_destroy(mpf_mlp* m) {
+    return guarded([&] {
+        if (!m) return MPF_OK;
+        if (m->impl) {
+            (void)hipStreamSynchronize(m->ctx->stream);
+            delete m->impl;
+        }
+        delete m;
+        return MPF_OK;
+    });
+}
+
+int mpf_mlp_fwd(mpf_mlp* m, int32_t batch, const float* d_X, float* d_Y) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_X, "mpf_mlp_fwd: null argument");
+        set_device(m->ctx);
+        m->impl->forward(batch, d_X, d_Y);
+        MPF_HIP(hipStreamSynchronize(m->ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_mlp_bwd(mpf_mlp* m, const float* d_T, double* loss) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_T, "mpf_mlp_bwd: null argument");
+        set_device(m->ctx);
+        const double l = m->impl->backward(d_T);
+        if (loss) *loss = l;
+        return MPF_OK;
+    });
+}
+
+int mpf_mlp_adam(mpf_mlp* m, float lr, float beta1, float beta2, float eps) {
+    return guarded([&] {
+        MPF_REQUIRE(m, "mpf_mlp_adam: null argument");
+        MPF_REQUIRE(lr > 0.0f && beta1 >= 0.0f && beta1 < 1.0f && beta2 >= 0.0f && beta2 < 1.0f && eps > 0.0f,
+                    "mpf_mlp_adam: bad hyper-parameters");
+        set_device(m->ctx);
+        m->impl->adam(lr, beta1, beta2, eps);
+        return MPF_OK;
+    });
+}
+
+int64_t mpf_mlp_param_count(const mpf_mlp* m) { return m && m->impl ? m->impl->count() : -1; }
+
+int mpf_mlp_params(mpf_mlp* m, int32_t which, int32_t set, float* host) {
+    return guarded([&] {
+        MPF_REQUIRE(m && host, "mpf_mlp_params: null argument");
+        set_device(m->ctx);
+        m->impl->params(which, set != 0, host);
+        return MPF_OK;
+    });
+}

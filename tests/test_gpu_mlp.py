@@ -1,0 +1,115 @@
+"""Learned-start network on the device (mpf_mlp_*) against the FP64 oracle
+(oracle/mlp_oracle.py, pinned by finite differences and torch's bfloat16 rounding):
+forward, loss, gradients and the Adam step; training reduces the loss; and the learned
+start gains over plain Gauss-Newton at equal Gauss-Newton cost."""
+import numpy as np
+import pytest
+
+from oracle.mlp_oracle import MlpOracle, adam_step
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(gpu_ctx, dims, seed=3):
+    from mpflow_amd.learned import Mlp
+    return Mlp(dims, seed=seed, ctx=gpu_ctx)
+
+
+@pytest.mark.parametrize("dims,batch", [([64, 96, 80, 40], 37), ([256, 512, 512, 300], 128)])
+def test_mlp_forward_backward_match_oracle(gpu_ctx, dims, batch):
+    import torch
+    print("case: mlp grads dims=%s batch=%d" % (dims, batch), flush=True)
+    net = _net(gpu_ctx, dims)
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((batch, dims[0])).astype(np.float32)
+    T = rng.standard_normal((batch, dims[-1])).astype(np.float32)
+    flat = net.params()
+    ora = MlpOracle(dims, flat)
+    Xd = torch.tensor(X, device="cuda")
+    Y = net.forward(Xd).cpu().numpy()
+    Yo, _ = ora.forward(X, bf16=True)
+    ey = float(np.linalg.norm(Y - Yo) / np.linalg.norm(Yo))
+    loss = net.backward(torch.tensor(T, device="cuda"))
+    g = net.params(grads=True)
+    lo, go = ora.grads_flat(X, T, bf16=True)
+    lx, gx = ora.grads_flat(X, T, bf16=False)
+    eg = float(np.linalg.norm(g - go) / np.linalg.norm(go))
+    ex = float(np.linalg.norm(g - gx) / np.linalg.norm(gx))
+    print("forward rel err %.2e, loss %.6f vs %.6f, grads vs bf16 oracle %.2e, vs fp64 oracle %.2e"
+          % (ey, loss, lo, eg, ex))
+    # FP32 accumulation order decides a BF16 rounding now and then (one BF16 ulp, 2^-8,
+    # on rare elements); the FP64 oracle without BF16 rounding sits 20x further away
+    assert ey < 3e-4
+    assert abs(loss - lo) < 1e-4 * lo
+    assert eg < 1e-3
+    assert ex < 3e-2
+
+
+def test_mlp_adam_matches_oracle(gpu_ctx):
+    import torch
+    dims = [32, 48, 16]
+    net = _net(gpu_ctx, dims, seed=5)
+    rng = np.random.default_rng(2)
+    w = net.params().astype(np.float64)
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    for t in range(1, 4):
+        X = torch.tensor(rng.standard_normal((20, 32)).astype(np.float32), device="cuda")
+        T = torch.tensor(rng.standard_normal((20, 16)).astype(np.float32), device="cuda")
+        net.forward(X)
+        net.backward(T)
+        g = net.params(grads=True).astype(np.float64)
+        net.adam(1e-2)
+        w, m, v = adam_step(w, g, m, v, t, 1e-2)
+        np.testing.assert_allclose(net.params(), w, rtol=1e-5, atol=1e-6)
+
+
+def test_mlp_training_reduces_loss(gpu_ctx):
+    import torch
+    from mpflow_amd.learned import train
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((64, 24)) / 8.0
+    X = rng.standard_normal((2048, 64)).astype(np.float32)
+    T = np.tanh(X @ A).astype(np.float32)
+    net = _net(gpu_ctx, [64, 128, 128, 24], seed=9)
+    hist = train(net, X, T, epochs=20, batch=128, lr=2e-3)
+    print("loss first %.4f last %.4f" % (hist[0], hist[-1]))
+    assert hist[-1] < 0.1 * hist[0]
+
+
+def test_learned_start_gains_over_plain_gn(gpu_ctx):
+    """16-electrode 2D disc (config-1 mesh), 1-3 ellipse phantoms, 60 dB: a network
+    trained on 4096 device-generated pairs, followed by one Gauss-Newton step, reconstructs
+    held-out frames better than one plain Gauss-Newton step from gamma = 1 (the same
+    Gauss-Newton cost) and accepts its start on most frames."""
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import EIT, SolveOptions
+    from mpflow_amd.learned import LearnedReconstruction, Mlp, VoltageFeatures, generate, train
+    from mpflow_amd.phantom import ellipse_phantom, frame_seed
+    from bench import WORKLOADS, build_problem
+    w = WORKLOADS["c1"]
+    nodes, el, bd, ports, pat = build_problem(w)
+    F = 64
+    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, 1, 1.0, 1.0, jacobian_exact=True, frames=F, ctx=gpu_ctx)
+    phantom = lambda s: ellipse_phantom(nodes, el, s)
+    print("case: learned start data", flush=True)
+    U, G = generate(eit, 4096, frame_seed(5, 0), phantom)
+    Uh, Gh = generate(eit, F, frame_seed(5, 1000000), phantom)
+    feats = VoltageFeatures().fit(U)
+    net = Mlp([eit.m, 512, 512, eit.E], seed=7, ctx=gpu_ctx)
+    print("case: learned start training", flush=True)
+    hist = train(net, feats(U), G, epochs=30, batch=128, lr=1e-3, lr_decay=0.93)
+    print("train loss %.4f -> %.4f" % (hist[0], hist[-1]))
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=1, refine_tol=1e-2)
+    rec = LearnedReconstruction(eit, net, feats, 1e-3, opts, gn_steps=1, plain_steps=1)
+    gl, info = rec.reconstruct(Uh, compare_plain=True)
+    gp = rec._gn(torch.tensor(Uh, device="cuda"), 1)
+    gtrue = np.exp(Gh)
+    el_ = np.linalg.norm(gl.cpu().numpy() - gtrue, axis=1) / np.linalg.norm(gtrue, axis=1)
+    ep_ = np.linalg.norm(gp.cpu().numpy() - gtrue, axis=1) / np.linalg.norm(gtrue, axis=1)
+    print("learned+1GN error %.4f, plain 1GN error %.4f, accepted %d/%d"
+          % (el_.mean(), ep_.mean(), info["accepted"].sum(), F))
+    assert el_.mean() < ep_.mean()
+    assert info["accepted"].mean() >= 0.5
