@@ -183,12 +183,140 @@ def run_c4(args):
     batch.close()
 
 
+def run_c5(args):
+    """Config 5: learned start on the config-2 mesh (32 electrodes, 5 harmonics, 9600 P1
+    elements).  Trains the 1024 -> 2048 -> 2048 -> 2048 -> E network on device-generated
+    (voltage, ln gamma) pairs (1-4 ellipses, 60 dB), then times on held-out frames:
+      plain    FP32 Gauss-Newton from gamma = 1, 3 steps, one frame at a time
+      learned  network start + 1 mixed-precision Gauss-Newton step, B frames per launch,
+               with the residual check and the plain fallback included in the time
+    and reports both frame rates, the reconstruction errors against the true phantoms and
+    the learned path's gamma against the FP64 oracle's Gauss-Newton step from the same
+    start.  Not the headline metric: its own JSON line."""
+    import numpy as np
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import EIT, Context, SolveOptions, Solver
+    from mpflow_amd.learned import LearnedReconstruction, Mlp, VoltageFeatures, generate, train
+    from mpflow_amd.phantom import ellipse_phantom, frame_seed
+    from oracle.eit_oracle import EITOracle, reconstruct
+    w = WORKLOADS["c2"]
+    nodes, el, bd, ports, pat = build_problem(w)
+    ctx = Context()
+    B = max(1, args.batch)
+    mk = lambda frames: EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                            jacobian_exact=True, frames=frames, ctx=ctx)
+    phantom = lambda sd: ellipse_phantom(nodes, el, sd, n_max=4)
+    t0 = time.perf_counter()
+    gen = mk(64)
+    U, G = generate(gen, args.train, frame_seed(5, 0), phantom)
+    t_data = time.perf_counter() - t0
+    feats = VoltageFeatures().fit(U)
+    net = Mlp([gen.m, 2048, 2048, 2048, gen.E], seed=5, ctx=ctx)
+    t0 = time.perf_counter()
+    hist = train(net, feats(U), G, epochs=args.epochs, batch=256, lr=1e-3, lr_decay=0.85)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    nh = args.steps * B
+    Uh, Gh = generate(gen, nh, frame_seed(5, 1000000), phantom)
+    gtrue = np.exp(Gh)
+    del gen
+    fp32 = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol)
+    mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
+                         refine_steps=1, refine_tol=1e-2)
+    # plain FP32 Gauss-Newton, one frame at a time
+    e1 = mk(1)
+    sp = Solver(e1, args.lam, gn_refine=4, options=fp32)
+    n_plain = min(nh, 8)
+    gplain = []
+
+    def plain(i):
+        sp.material_distribution.fill_(1.0)
+        sp.measurement[0] = torch.tensor(Uh[i], dtype=torch.float64)
+        for _ in range(3):
+            sp.solve_absolute()
+        return sp.material_distribution[0].cpu().numpy().copy()
+
+    plain(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        gplain.append(plain(i))
+    torch.cuda.synchronize()
+    fps_plain = n_plain / (time.perf_counter() - t0)
+    # learned start, B frames per launch
+    eB = mk(B)
+    rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3)
+    noise = np.sqrt(np.mean(Uh ** 2, axis=1)) * 10.0 ** (-60.0 / 20.0) * math.sqrt(Uh.shape[1])
+    rec.reconstruct(Uh[:B], noise_norm=float(noise[:B].max()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    glearn, acc = [], 0
+    for k in range(args.steps):
+        sl = slice(k * B, (k + 1) * B)
+        g, info = rec.reconstruct(Uh[sl], noise_norm=float(noise[sl].max()))
+        glearn.append(g.cpu().numpy())
+        acc += int(info["accepted"].sum())
+    torch.cuda.synchronize()
+    fps_learn = nh / (time.perf_counter() - t0)
+    glearn = np.concatenate(glearn)
+    err_l = np.linalg.norm(glearn - gtrue, axis=1) / np.linalg.norm(gtrue, axis=1)
+    err_p = np.linalg.norm(np.stack(gplain) - gtrue[:n_plain], axis=1) / np.linalg.norm(gtrue[:n_plain], axis=1)
+    # FP64 oracle: one Gauss-Newton step from the same network start, first frames
+    o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
+    X = torch.tensor(feats(Uh[:2]), dtype=torch.float32, device="cuda")
+    start = torch.exp(net.forward(X)).cpu().numpy().astype(np.float64)
+    starts32 = torch.exp(net.forward(X)).to(torch.float32)
+    e2 = mk(2)
+    s2 = Solver(e2, args.lam, gn_refine=4, options=mixed)
+    s2.material_distribution.copy_(starts32)
+    s2.measurement.copy_(torch.tensor(Uh[:2], dtype=torch.float64))
+    s2.solve_absolute()
+    gdev = s2.material_distribution.cpu().numpy().astype(np.float64)
+    err64 = []
+    for f in range(2):
+        Um = Uh[f].reshape(e2.Dp, e2.Mp)[:e2.D, :e2.M].T
+        g64 = reconstruct(o, Um, start[f], args.lam, steps=1, quirks_jacobian=False)
+        err64.append(float(np.linalg.norm(gdev[f] - g64) / np.linalg.norm(g64)))
+    out = {
+        "metric": "frames/s, config 5: learned-start mixed-precision batched GN vs plain FP32 GN",
+        "value": round(fps_learn, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * B / fps_learn, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "config 5 (c5): config-2 mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
+                        "network %s; %d training pairs, %d epochs; held-out frames 1-4 ellipses, 60 dB"
+                        % (e1.E, e1.N, net.dims, args.train, args.epochs),
+            "frames_per_launch": B,
+            "plain_fp32_gn_fps": round(fps_plain, 3),
+            "speedup_vs_plain": round(fps_learn / fps_plain, 2),
+            "learned_error_vs_phantom": round(float(err_l.mean()), 4),
+            "plain_error_vs_phantom": round(float(err_p.mean()), 4),
+            "learned_vs_fp64_oracle_same_start": [float("%.3g" % v) for v in err64],
+            "accepted": "%d/%d" % (acc, nh),
+            "train_loss": [round(hist[0], 5), round(hist[-1], 5)],
+            "data_s": round(t_data, 1), "train_s": round(t_train, 1),
+        },
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed frames")
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
+    ap.add_argument("--train", type=int, default=16384, help="config 5: training pairs")
+    ap.add_argument("--epochs", type=int, default=12, help="config 5: training epochs")
     ap.add_argument("--jobs", type=int, default=256, help="config 4: number of jobs")
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)
@@ -213,6 +341,9 @@ def main():
 
     if args.workload == "c4":
         run_c4(args)
+        return
+    if args.workload == "c5":
+        run_c5(args)
         return
     w = WORKLOADS[args.workload]
     cfg = CONFIG_INDEX[args.workload]
