@@ -78,7 +78,7 @@ def build_problem(w):
     return nodes, el, bd, ports, pat
 
 
-def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=10.0):
+def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=12.0):
     """Time the oracle's C++ ELL-PCG on harmonic 0 of the workload mesh at gamma = 1."""
     import numpy as np
     from oracle import baseline
@@ -86,9 +86,18 @@ def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=1
     o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
     A = o.system(np.ones(el.shape[0]), 0)
     F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
-    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=5)
+    # the right-hand sides the device solves: columns that are scalar multiples of an
+    # earlier column share one solve (here the measurement patterns repeat the drives)
+    nrm = F / np.where(np.abs(F).max(axis=0) > 0, F[np.abs(F).argmax(axis=0), np.arange(F.shape[1])], 1.0)
+    keep = []
+    for j in range(F.shape[1]):
+        if not any(np.max(np.abs(nrm[:, j] - nrm[:, i])) <= 1e-12 for i in keep):
+            keep.append(j)
+    F = np.ascontiguousarray(F[:, keep])
+    baseline.pcg(A, F, 1, tol=0.0, max_iter=5)                 # first touch
+    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=40)
     per = t0 / max(s0, 1)
-    n_it = int(max(10, min(5000, budget_s / max(per, 1e-6))))
+    n_it = int(max(10, min(20000, budget_s / max(per, 1e-6))))
     _, steps, t = baseline.pcg(A, F, 1, tol=0.0, max_iter=n_it)
     per_iter = t / steps
     return {
@@ -200,7 +209,7 @@ def run_c5(args):
     from mpflow_amd.learned import LearnedReconstruction, Mlp, VoltageFeatures, generate, train
     from mpflow_amd.phantom import ellipse_phantom, frame_seed
     from oracle.eit_oracle import EITOracle, reconstruct
-    w = WORKLOADS["c2"]
+    w = WORKLOADS[args.mesh]
     nodes, el, bd, ports, pat = build_problem(w)
     ctx = Context()
     B = max(1, args.batch)
@@ -208,7 +217,7 @@ def run_c5(args):
                             jacobian_exact=True, frames=frames, ctx=ctx)
     phantom = lambda sd: ellipse_phantom(nodes, el, sd, n_max=4)
     t0 = time.perf_counter()
-    gen = mk(64)
+    gen = mk(64 if not w["p2"] else 32)
     U, G = generate(gen, args.train, frame_seed(5, 0), phantom)
     t_data = time.perf_counter() - t0
     feats = VoltageFeatures().fit(U)
@@ -225,6 +234,7 @@ def run_c5(args):
     mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
                          refine_steps=1, refine_tol=1e-2)
     # plain FP32 Gauss-Newton, one frame at a time
+    del net._x
     e1 = mk(1)
     sp = Solver(e1, args.lam, gn_refine=4, options=fp32)
     n_plain = min(nh, 8)
@@ -292,9 +302,9 @@ def run_c5(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "config 5 (c5): config-2 mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
+            "workload": "config 5 (c5) on the %s mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
                         "network %s; %d training pairs, %d epochs; held-out frames 1-4 ellipses, 60 dB"
-                        % (e1.E, e1.N, net.dims, args.train, args.epochs),
+                        % (args.mesh, e1.E, e1.N, net.dims, args.train, args.epochs),
             "frames_per_launch": B,
             "plain_fp32_gn_fps": round(fps_plain, 3),
             "speedup_vs_plain": round(fps_learn / fps_plain, 2),
@@ -309,20 +319,45 @@ def run_c5(args):
     print(json.dumps(out))
 
 
+def run_probe():
+    """Counter probe for rocprofv3 --pmc passes: 32 PCG iterations of config 3 at frame 0's
+    phantom (all 5 harmonics active, so every launch covers 5 harmonic-SpMVs), then a plain
+    256 MiB device-to-device copy that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import EIT, SolveOptions
+    from mpflow_amd.phantom import ellipse_phantom, frame_seed
+    w = WORKLOADS["c3"]
+    nodes, el, bd, ports, pat = build_problem(w)
+    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"], jacobian_exact=True)
+    g = torch.tensor(ellipse_phantom(nodes, el, frame_seed(3, 0)), dtype=torch.float32)
+    eit.solve(g, SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, max_iter=32, check_every=32))
+    torch.cuda.synchronize()
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 256 MiB
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed frames")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed frames")
+    ap.add_argument("--steps", type=int, default=2, help="timed batches of --batch frames")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed batches")
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
     ap.add_argument("--train", type=int, default=16384, help="config 5: training pairs")
     ap.add_argument("--epochs", type=int, default=12, help="config 5: training epochs")
+    ap.add_argument("--mesh", default="c2", choices=["c2", "c3"], help="config 5: mesh of the learned start")
     ap.add_argument("--jobs", type=int, default=256, help="config 4: number of jobs")
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--batch", type=int, default=1, help="frames solved together per launch")
+    ap.add_argument("--batch", type=int, default=8, help="frames solved together per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="short counter probe for rocprofv3 --pmc (no JSON)")
     ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -339,6 +374,9 @@ def main():
     from mpflow_amd.eit import EIT, SolveOptions, Solver
     from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
 
+    if args.probe:
+        run_probe()
+        return
     if args.workload == "c4":
         run_c4(args)
         return
@@ -398,6 +436,15 @@ def main():
     frames_timed = args.steps * B
     fps = frames_timed / elapsed
     avg_launch_us = 1e3 * spmv_ms / max(spmv_launches, 1)
+    # HBM bytes per launch from the committed rocprofv3 --pmc summary (FETCH_SIZE and
+    # WRITE_SIZE per harmonic-SpMV, calibrated against a copy kernel), scaled by the live
+    # run's harmonic-SpMVs per launch
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "spmv_traffic_%s.json" % args.workload)
+    if os.path.exists(tfile) and spmv_launches:
+        per_h = json.load(open(tfile))["traffic_bytes_per_harmonic_launch"]
+        per_launch_h = spmv_bytes / (8.0 * eit.nnz + 8.0 * eit.N * eit.R_unique) / spmv_launches
+        traffic = round(per_h * per_launch_h)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     roofline = {
         "bound": "hbm",
@@ -406,10 +453,12 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
         "avg_launch_us": round(avg_launch_us, 2),
         "launches": spmv_launches,
         "algorithmic_bytes_per_harmonic_launch": 8 * eit.nnz + 8 * eit.N * eit.R_unique,
+        "algorithmic_bytes_per_launch": round(spmv_bytes / max(spmv_launches, 1)),
+        "traffic_source": "profiles/spmv_traffic_%s.json (x harmonic-SpMVs per launch)" % args.workload,
     }
     out = {
         "metric": METRIC,
