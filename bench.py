@@ -216,44 +216,52 @@ def run_c5(args):
     mk = lambda frames: EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
                             jacobian_exact=True, frames=frames, ctx=ctx)
     phantom = lambda sd: ellipse_phantom(nodes, el, sd, n_max=4)
+    log = lambda msg: print("c5: " + msg, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     gen = mk(64 if not w["p2"] else 32)
-    U, G = generate(gen, args.train, frame_seed(5, 0), phantom)
+    U, G = generate(gen, args.train, frame_seed(5, 0), phantom, log=log)
     t_data = time.perf_counter() - t0
     feats = VoltageFeatures().fit(U)
     net = Mlp([gen.m, 2048, 2048, 2048, gen.E], seed=5, ctx=ctx)
     t0 = time.perf_counter()
-    hist = train(net, feats(U), G, epochs=args.epochs, batch=256, lr=1e-3, lr_decay=0.85)
+    hist = train(net, feats(U), G, epochs=args.epochs, batch=256, lr=1e-3, lr_decay=0.85, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     nh = args.steps * B
     Uh, Gh = generate(gen, nh, frame_seed(5, 1000000), phantom)
+    log("timing plain and learned reconstructions")
     gtrue = np.exp(Gh)
     del gen
     fp32 = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol)
     mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
                          refine_steps=1, refine_tol=1e-2)
-    # plain FP32 Gauss-Newton, one frame at a time
-    del net._x
+    # plain FP32 Gauss-Newton as the reference runs it: 3 steps from gamma = 1, one frame
+    # at a time, FP32 PCG with the reference stop rule (column 0, sqrt(rho / b.b) below
+    # FLT_EPSILON); also timed with the ||r|| <= tol ||b|| rule for comparison
     e1 = mk(1)
-    sp = Solver(e1, args.lam, gn_refine=4, options=fp32)
-    n_plain = min(nh, 8)
-    gplain = []
+    n_plain = min(nh, 6)
 
-    def plain(i):
-        sp.material_distribution.fill_(1.0)
-        sp.measurement[0] = torch.tensor(Uh[i], dtype=torch.float64)
-        for _ in range(3):
-            sp.solve_absolute()
-        return sp.material_distribution[0].cpu().numpy().copy()
+    def plain_fps(opts):
+        sp = Solver(e1, args.lam, gn_refine=4, options=opts)
+        out = []
 
-    plain(0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(n_plain):
-        gplain.append(plain(i))
-    torch.cuda.synchronize()
-    fps_plain = n_plain / (time.perf_counter() - t0)
+        def one(i):
+            sp.material_distribution.fill_(1.0)
+            sp.measurement[0] = torch.tensor(Uh[i], dtype=torch.float64)
+            for _ in range(3):
+                sp.solve_absolute()
+            return sp.material_distribution[0].cpu().numpy().copy()
+
+        one(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(n_plain):
+            out.append(one(i))
+        torch.cuda.synchronize()
+        return n_plain / (time.perf_counter() - t0), out
+
+    fps_plain, gplain = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_REFERENCE))
+    fps_plain_tol, _ = plain_fps(fp32)
     # learned start, B frames per launch
     eB = mk(B)
     rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3)
@@ -307,7 +315,9 @@ def run_c5(args):
                         % (args.mesh, e1.E, e1.N, net.dims, args.train, args.epochs),
             "frames_per_launch": B,
             "plain_fp32_gn_fps": round(fps_plain, 3),
+            "plain_fp32_gn_fps_tolerance_rule": round(fps_plain_tol, 3),
             "speedup_vs_plain": round(fps_learn / fps_plain, 2),
+            "speedup_vs_plain_tolerance_rule": round(fps_learn / fps_plain_tol, 2),
             "learned_error_vs_phantom": round(float(err_l.mean()), 4),
             "plain_error_vs_phantom": round(float(err_p.mean()), 4),
             "learned_vs_fp64_oracle_same_start": [float("%.3g" % v) for v in err64],
