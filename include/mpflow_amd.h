@@ -231,6 +231,24 @@ int mpf_batch_gn_step(mpf_batch* b, float* const* d_gamma, const double* const* 
  * Low-level kernels on caller-owned device buffers
  * ------------------------------------------------------------------------- */
 
+/* ConjugateGradient::solve / BiCGSTAB::solve with a SparseMatrix operator and the
+ * diagonal preconditioner (src/numeric/conjugate_gradient.cpp:61-161,
+ * src/numeric/bicgstab.cpp:83-193, src/numeric/preconditioner.cpp:59-76), all R <= 64
+ * right-hand sides at once, starting from X = 0.
+ *   d_column_ids, d_values : SparseMatrix device arrays (sparse_matrix.cpp:48-114): 32
+ *                            slots per row, entry (row, slot) at row*32 + slot, empty
+ *                            slots with column id 0xFFFFFFFF; `density` used slots
+ *   d_B, d_X               : Matrix column-major n x R with leading dimensions ldb, ldx
+ *   dc_free                : solve (A + 1 1^T) X = B (CG; BiCGSTAB as opts.bicgstab_ignore_dc)
+ * opts: stop rule, tol, max_iter, precision (MPF_PREC_MIXED adds FP64 refinement),
+ * check_every, graphs; `krylov` is set by the entry point.  Synchronises. */
+int mpf_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
+                  const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
+                  int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
+int mpf_bicgstab_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
+                       const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
+                       int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
+
 /* Multi-harmonic ELL SpMV, the S/R form of SparseMatrix::multiply
  * (src/numeric/sparse_matrix_kernel.cu:157-196) fused with Equation::update's
  * A_c = S + alpha_c R (src/fem/equation_kernel.cu:137-157):

@@ -19,7 +19,7 @@ EXPORTED = [
     "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read",
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
-    "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params",
+    "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
 ]
 
 
@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
     L.mpf_mlp_param_count.argtypes = [vp]
     L.mpf_mlp_param_count.restype = i64
     L.mpf_mlp_params.argtypes = [vp, i32, i32, vp]
+    for f in (L.mpf_pcg_solve, L.mpf_bicgstab_solve):
+        f.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, i32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     _lib = L
     return L
 

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "mlp.h"
+#include "numeric.h"
 #include "model.h"
 
 using mpf::Error;
@@ -436,4 +437,37 @@ int mpf_mlp_params(mpf_mlp* m, int32_t which, int32_t set, float* host) {
         m->impl->params(which, set != 0, host);
         return MPF_OK;
     });
+}
+
+static int krylov_entry(const char* name, int krylov, mpf_ctx* ctx, int32_t n, int32_t density,
+                        const uint32_t* d_cid, const float* d_val, int32_t R, const float* d_B, int32_t ldb,
+                        float* d_X, int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts,
+                        mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_cid && d_val && d_B && d_X, std::string(name) + ": null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        set_device(ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        o.krylov = krylov;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        const int rc = mpf::krylov_solve(ctx, n, density, d_cid, d_val, R, d_B, ldb, d_X, ldx, dc_free, o, stats);
+        if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc == MPF_ENOCONV) g_last_error = std::string(name) + ": the iteration cap was hit or the solve broke down";
+        return rc;
+    });
+}
+
+int mpf_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids, const float* d_values,
+                  int32_t R, const float* d_B, int32_t ldb, float* d_X, int32_t ldx, int32_t dc_free,
+                  const mpf_solve_opts* opts, mpf_solve_stats* stats) {
+    return krylov_entry("mpf_pcg_solve", MPF_KRYLOV_CG, ctx, n, density, d_column_ids, d_values, R, d_B, ldb,
+                        d_X, ldx, dc_free, opts, stats);
+}
+
+int mpf_bicgstab_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
+                       const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X, int32_t ldx,
+                       int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats) {
+    return krylov_entry("mpf_bicgstab_solve", MPF_KRYLOV_BICGSTAB, ctx, n, density, d_column_ids, d_values, R,
+                        d_B, ldb, d_X, ldx, dc_free, opts, stats);
 }
