@@ -4,7 +4,8 @@
 Metric: frames/s of 32-electrode, 40k-element (P2, 81k nodes), 5-harmonic 2.5D
 Gauss-Newton reconstruction.  One "step" = one frame: 3 Gauss-Newton steps
 (Solver::solveAbsolute, src/solver/solver.cpp:195-225) from gamma = 1 for one seeded
-ellipse phantom with 60 dB noise, every forward solve in FP32 PCG with FP64 residual
+ellipse phantom with 60 dB noise, every forward solve in FP32 PCG (two-level
+preconditioner by default, --precond jacobi for the reference's) with FP64 residual
 refinement, the Tikhonov step via the BF16-MFMA Gram of the Jacobian and FP64
 Cholesky.  Measured voltages of each frame are produced by the device forward solver
 before the timed region (inputs resident in HBM when timing starts).
@@ -13,7 +14,7 @@ Prints ONE JSON line (rank 0):
   value        frames/s of the whole job (1 GPU)
   roofline     dominant kernel (ELL SpMV of the PCG): algorithmic bytes / event-timed
                launch time vs 8 TB/s HBM3E
-  cpu_baseline the same ELL Jacobi-PCG (FP32 vectors, FP64 scalars, OpenMP over rows)
+  cpu_baseline the same ELL PCG (same preconditioner; FP32 vectors, FP64 scalars, OpenMP)
                on the host cores, on a bounded sample of harmonic-0 iterations, scaled
                to frames/s with the device run's iteration counts
 """
@@ -78,14 +79,22 @@ def build_problem(w):
     return nodes, el, bd, ports, pat
 
 
-def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=12.0):
-    """Time the oracle's C++ ELL-PCG on harmonic 0 of the workload mesh at gamma = 1."""
+def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=12.0, two_level=False):
+    """Time the oracle's C++ ELL-PCG on harmonic 0 of the workload mesh at gamma = 1, with
+    the preconditioner the device run used (two-level: nodes in RCB order, aggregates of
+    256 consecutive rows, as on the device)."""
     import numpy as np
     from oracle import baseline
-    from oracle.eit_oracle import EITOracle
+    from oracle.eit_oracle import EITOracle, rcb_order, two_level_preconditioner
     o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
     A = o.system(np.ones(el.shape[0]), 0)
     F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    einv = None
+    if two_level:
+        order = rcb_order(nodes, 256)
+        A = A[order][:, order].tocsr()
+        F = F[order]
+        einv = two_level_preconditioner(A, A, A.diagonal(), A.diagonal(), True, tile=256)[0].einv
     # the right-hand sides the device solves: columns that are scalar multiples of an
     # earlier column share one solve (here the measurement patterns repeat the drives)
     nrm = F / np.where(np.abs(F).max(axis=0) > 0, F[np.abs(F).argmax(axis=0), np.arange(F.shape[1])], 1.0)
@@ -94,21 +103,22 @@ def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=1
         if not any(np.max(np.abs(nrm[:, j] - nrm[:, i])) <= 1e-12 for i in keep):
             keep.append(j)
     F = np.ascontiguousarray(F[:, keep])
-    baseline.pcg(A, F, 1, tol=0.0, max_iter=5)                 # first touch
-    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=40)
+    baseline.pcg(A, F, 1, tol=0.0, max_iter=5, einv=einv)                 # first touch
+    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=40, einv=einv)
     per = t0 / max(s0, 1)
     n_it = int(max(10, min(20000, budget_s / max(per, 1e-6))))
-    _, steps, t = baseline.pcg(A, F, 1, tol=0.0, max_iter=n_it)
+    _, steps, t = baseline.pcg(A, F, 1, tol=0.0, max_iter=n_it, einv=einv)
     per_iter = t / steps
     return {
         "value": 1.0 / (per_iter * slice_iters_per_frame),
         "unit": "frames/s",
         "cores": baseline.threads(),
         "kind": "port",
-        "sample": ("%d Jacobi-PCG iterations of harmonic 0 (N=%d, %d RHS, FP32 vectors, FP64 scalars) "
+        "sample": ("%d %s-PCG iterations of harmonic 0 (N=%d, %d RHS, FP32 vectors, FP64 scalars) "
                    "on the workload mesh, %.3f s; frames/s = 1 / (s per iteration x %d slice-iterations "
                    "per frame measured on the device); Jacobian and dense GN work not included"
-                   % (steps, A.shape[0], F.shape[1], t, slice_iters_per_frame)),
+                   % (steps, "two-level" if two_level else "Jacobi", A.shape[0], F.shape[1], t,
+                      slice_iters_per_frame)),
     }
 
 
@@ -193,12 +203,16 @@ def run_c4(args):
 
 
 def run_c5(args):
-    """Config 5: learned start on the config-2 mesh (32 electrodes, 5 harmonics, 9600 P1
-    elements).  Trains the 1024 -> 2048 -> 2048 -> 2048 -> E network on device-generated
-    (voltage, ln gamma) pairs (1-4 ellipses, 60 dB), then times on held-out frames:
-      plain    FP32 Gauss-Newton from gamma = 1, 3 steps, one frame at a time
-      learned  network start + 1 mixed-precision Gauss-Newton step, B frames per launch,
-               with the residual check and the plain fallback included in the time
+    """Config 5: learned start on the config-3 mesh (--mesh c3, default: 40344 P2
+    elements, 32 electrodes, 5 harmonics) or the config-2 mesh (9600 P1 elements).  Trains
+    the 1024 -> 2048 -> 2048 -> 2048 -> E network on device-generated (voltage, ln gamma)
+    pairs (1-4 ellipses, 60 dB), then times on held-out frames:
+      plain    FP32 Gauss-Newton from gamma = 1, 3 steps, one frame at a time: the
+               reference's algorithm (Jacobi-PCG, reference stop rule), and the same with
+               this build's two-level PCG
+      learned  network start + 1 mixed-precision Gauss-Newton step (two-level PCG), B
+               frames per launch, with the residual check and the plain fallback included
+               in the time
     and reports both frame rates, the reconstruction errors against the true phantoms and
     the learned path's gamma against the FP64 oracle's Gauss-Newton step from the same
     start.  Not the headline metric: its own JSON line."""
@@ -219,7 +233,10 @@ def run_c5(args):
     log = lambda msg: print("c5: " + msg, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     gen = mk(64 if not w["p2"] else 32)
-    U, G = generate(gen, args.train, frame_seed(5, 0), phantom, log=log)
+    two = capi.MPF_PRECOND_TWO_LEVEL
+    ntrain = args.train if args.train else (4096 if w["p2"] else 16384)
+    U, G = generate(gen, ntrain, frame_seed(5, 0), phantom, log=log,
+                    options=SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-5, preconditioner=two))
     t_data = time.perf_counter() - t0
     feats = VoltageFeatures().fit(U)
     net = Mlp([gen.m, 2048, 2048, 2048, gen.E], seed=5, ctx=ctx)
@@ -232,9 +249,10 @@ def run_c5(args):
     log("timing plain and learned reconstructions")
     gtrue = np.exp(Gh)
     del gen
-    fp32 = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol)
-    mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
-                         refine_steps=1, refine_tol=1e-2)
+    # the learned path's forward solves: FP32 two-level PCG to 1e-4 plus one FP64 residual
+    # refinement to 1e-2 of that (about 1e-6 overall)
+    mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-4, precision=capi.MPF_PREC_MIXED,
+                         refine_steps=1, refine_tol=1e-2, preconditioner=two)
     # plain FP32 Gauss-Newton as the reference runs it: 3 steps from gamma = 1, one frame
     # at a time, FP32 PCG with the reference stop rule (column 0, sqrt(rho / b.b) below
     # FLT_EPSILON); also timed with the ||r|| <= tol ||b|| rule for comparison
@@ -261,7 +279,8 @@ def run_c5(args):
         return n_plain / (time.perf_counter() - t0), out
 
     fps_plain, gplain = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_REFERENCE))
-    fps_plain_tol, _ = plain_fps(fp32)
+    # plain FP32 GN with this build's solver: two-level PCG to ||r|| <= tol ||b||
+    fps_plain_tl, _ = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, preconditioner=two))
     # learned start, B frames per launch
     eB = mk(B)
     rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3)
@@ -312,12 +331,16 @@ def run_c5(args):
         "config": {
             "workload": "config 5 (c5) on the %s mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
                         "network %s; %d training pairs, %d epochs; held-out frames 1-4 ellipses, 60 dB"
-                        % (args.mesh, e1.E, e1.N, net.dims, args.train, args.epochs),
+                        % (args.mesh, e1.E, e1.N, net.dims, ntrain, args.epochs),
             "frames_per_launch": B,
             "plain_fp32_gn_fps": round(fps_plain, 3),
-            "plain_fp32_gn_fps_tolerance_rule": round(fps_plain_tol, 3),
+            "plain_fp32_gn": "3 GN steps from gamma = 1, one frame at a time, FP32 Jacobi-PCG with the "
+                             "reference stop rule (the reference's algorithm)",
+            "plain_fp32_gn_two_level_fps": round(fps_plain_tl, 3),
             "speedup_vs_plain": round(fps_learn / fps_plain, 2),
-            "speedup_vs_plain_tolerance_rule": round(fps_learn / fps_plain_tol, 2),
+            "speedup_vs_plain_two_level": round(fps_learn / fps_plain_tl, 2),
+            "learned_forward": "FP32 two-level PCG to 1e-4 + 1 FP64 refinement to 1e-2; residual check "
+                               "warm-started to 1e-4",
             "learned_error_vs_phantom": round(float(err_l.mean()), 4),
             "plain_error_vs_phantom": round(float(err_p.mean()), 4),
             "learned_vs_fp64_oracle_same_start": [float("%.3g" % v) for v in err64],
@@ -358,9 +381,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2, help="timed batches of --batch frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches")
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
-    ap.add_argument("--train", type=int, default=16384, help="config 5: training pairs")
+    ap.add_argument("--train", type=int, default=0, help="config 5: training pairs (0: 4096 on c3, 16384 on c2)")
     ap.add_argument("--epochs", type=int, default=12, help="config 5: training epochs")
-    ap.add_argument("--mesh", default="c2", choices=["c2", "c3"], help="config 5: mesh of the learned start")
+    ap.add_argument("--mesh", default="c3", choices=["c2", "c3"], help="config 5: mesh of the learned start")
     ap.add_argument("--jobs", type=int, default=256, help="config 4: number of jobs")
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)
@@ -369,6 +392,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", action="store_true", help="short counter probe for rocprofv3 --pmc (no JSON)")
     ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
+    ap.add_argument("--precond", default="two-level", choices=["two-level", "jacobi"],
+                    help="forward PCG preconditioner (jacobi: the reference's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -400,8 +425,10 @@ def main():
     eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
               jacobian_exact=True, frames=B)
     dev = torch.device("cuda", 0)
+    two_level = args.precond == "two-level"
     opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
-                        refine_steps=1, refine_tol=1e-2)
+                        refine_steps=1, refine_tol=1e-2,
+                        preconditioner=capi.MPF_PRECOND_TWO_LEVEL if two_level else capi.MPF_PRECOND_JACOBI)
 
     # measured voltages (untimed): device forward solve of each phantom + 60 dB noise.
     # A step is one batch of B frames; the timed frames are steps * B.
@@ -488,7 +515,8 @@ def main():
             "workload": "config %d (%s): %s" % (cfg, args.workload, w["desc"]),
             "gn_steps_per_frame": args.gn_steps,
             "lambda": args.lam,
-            "forward": "FP32 Jacobi-PCG to ||r||/||b|| <= %g + 1 FP64 residual refinement" % args.tol,
+            "forward": "FP32 %s-PCG to ||r||/||b|| <= %g + 1 FP64 residual refinement"
+                       % ("two-level" if two_level else "Jacobi", args.tol),
             "nodes": eit.N, "elements": eit.E, "nnz": eit.nnz, "harmonics": eit.C,
             "rhs": eit.R, "rhs_solved": eit.R_unique, "jacobian_rows": eit.m,
             "frames_per_launch": B,
@@ -497,7 +525,8 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / frames_timed)
+        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / frames_timed,
+                                           two_level=two_level)
     print(json.dumps(out))
 
 

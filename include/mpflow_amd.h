@@ -51,6 +51,9 @@ extern "C" {
 #define MPF_KRYLOV_CG       0
 #define MPF_KRYLOV_BICGSTAB 1
 
+#define MPF_PRECOND_JACOBI    0
+#define MPF_PRECOND_TWO_LEVEL 1
+
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
 typedef struct mpf_batch mpf_batch;
@@ -166,6 +169,15 @@ typedef struct mpf_solve_opts {
     int32_t graphs;               /* 1: capture the Krylov iterations between two convergence
                                      polls as a HIP graph and replay it (fewer launch
                                      overheads for small meshes); not with profiling */
+    int32_t preconditioner;       /* MPF_PRECOND_*.  JACOBI: the reference's diagonal
+                                     preconditioner (preconditioner.cpp:59-76).  TWO_LEVEL
+                                     (CG on model solves; no counterpart in the reference):
+                                     Jacobi plus an additive coarse correction over compact
+                                     256-node aggregates, M^-1 = D^-1 + W Ec^-1 W^T with Ec the
+                                     aggregates' sigma = 1 coarse matrix scaled per aggregate
+                                     to the frame's conductivity; several times fewer
+                                     iterations on large meshes.  Models above 512 aggregates
+                                     (131072 unknowns) fall back to Jacobi. */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {
@@ -208,6 +220,38 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
 int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lambda,
                 int32_t gn_refine, const mpf_solve_opts* opts, double* d_Ucalc,
                 mpf_solve_stats* stats);
+
+/* Equation::update of one frame (src/fem/equation.cpp:261-274, updateMatrixKernel and
+ * updateSystemMatrixKernel in src/fem/equation_kernel.cu:60-157): the system matrix
+ * A = S(gamma) + alpha R(gamma), S = sum_e sigma(gamma_e) K_e, R = sum_e sigma(gamma_e) M_e,
+ * summed in FP64 and rounded once, written as a SparseMatrix (sparse_matrix.cpp:48-114):
+ * rows x 32 slots, entry (row, slot) at row*32 + slot, empty slots with column id
+ * 0xFFFFFFFF and value 0, slots in the reference's first-insertion order
+ * (equation.cpp:131-144), rows and column ids in the caller's node numbering.
+ *   d_gamma         : E floats (device)
+ *   alpha           : the harmonic's (2 pi c / height)^2 (eit.cpp:154-155), 0 for 2D
+ *   d_column_ids    : rows*32 (device, nullable: the pattern does not depend on gamma)
+ *   d_values        : rows*32 (device); rows = dims[0] (N, plus the electrode potentials
+ *                     of a complete-electrode model)
+ * MPF_EINVAL if the model was built with rows wider than 32 entries.  Synchronises. */
+int mpf_fem_assemble(mpf_model* m, const float* d_gamma, double alpha, uint32_t* d_column_ids,
+                     float* d_values);
+
+/* EIT::applyMeasurementPattern (src/models/eit.cpp:220-245):
+ *   U (+)= W Phi[:, 0:D],  W = measurementPattern^T excitationMatrix^T
+ *   d_phi : fields, column-major rows x (D + M) with leading dimension ldphi >= rows
+ *   d_U   : ceil16(M) x D column-major (ld ceil16(M)); rows >= M get 0 (or + 0)
+ *   additive : 0 overwrite, 1 accumulate (the reference's harmonic > 0 calls).  Synchronises. */
+int mpf_eit_measure(mpf_model* m, const float* d_phi, int32_t ldphi, int32_t additive, float* d_U);
+
+/* Equation::calcJacobian (src/fem/equation.cpp:282-310, calcJacobianKernel in
+ * src/fem/equation_kernel.cu:197-245) for one harmonic's fields:
+ *   J[d*ceil16(M) + m, e] (+)= sigma'(gamma_e) sum_ij Phi[n_i, d] K_e[i][j] Phi[n_j, D + m]
+ * with K_e the element stiffness integrals; rows with d >= D or m >= M are 0.  J is
+ * column-major ceil16(M) ceil16(D) x E.  The reference's scaling by -height
+ * (eit.cpp:202-205) is the caller's.  Synchronises. */
+int mpf_jacobian(mpf_model* m, const float* d_phi, int32_t ldphi, const float* d_gamma, int32_t additive,
+                 float* d_J);
 
 /* ---------------------------------------------------------------------------
  * Batches: several models (meshes of any size, any electrode count and harmonic count)

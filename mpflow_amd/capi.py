@@ -11,6 +11,7 @@ MPF_OK, MPF_EINVAL, MPF_ENOMEM, MPF_EHIP, MPF_ENOCONV, MPF_EFULL = range(6)
 MPF_STOP_REFERENCE, MPF_STOP_TOLERANCE = 0, 1
 MPF_PREC_FP32, MPF_PREC_MIXED = 0, 1
 MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB = 0, 1
+MPF_PRECOND_JACOBI, MPF_PRECOND_TWO_LEVEL = 0, 1
 
 EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",
@@ -20,6 +21,7 @@ EXPORTED = [
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
+    "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian",
 ]
 
 
@@ -44,7 +46,7 @@ class SolveOpts(C.Structure):
         ("stop_rule", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
         ("precision", C.c_int32), ("refine_steps", C.c_int32), ("refine_tol", C.c_double),
         ("krylov", C.c_int32), ("check_every", C.c_int32), ("warm_start", C.c_int32),
-        ("bicgstab_ignore_dc", C.c_int32), ("graphs", C.c_int32),
+        ("bicgstab_ignore_dc", C.c_int32), ("graphs", C.c_int32), ("preconditioner", C.c_int32),
     ]
 
 
@@ -105,6 +107,9 @@ def lib() -> C.CDLL:
     L.mpf_mlp_params.argtypes = [vp, i32, i32, vp]
     for f in (L.mpf_pcg_solve, L.mpf_bicgstab_solve):
         f.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, i32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.mpf_fem_assemble.argtypes = [vp, vp, dbl, vp, vp]
+    L.mpf_eit_measure.argtypes = [vp, vp, i32, i32, vp]
+    L.mpf_jacobian.argtypes = [vp, vp, i32, vp, i32, vp]
     _lib = L
     return L
 

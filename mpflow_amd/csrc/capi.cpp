@@ -5,6 +5,7 @@
 #include <string>
 
 #include "mlp.h"
+#include "fem.h"
 #include "numeric.h"
 #include "model.h"
 
@@ -218,6 +219,37 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
         if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rc == MPF_ENOCONV) g_last_error = "mpf_eit_solve: a Krylov solve hit its iteration cap";
         return rc;
+    });
+}
+
+int mpf_fem_assemble(mpf_model* m, const float* d_gamma, double alpha, uint32_t* d_column_ids, float* d_values) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_gamma && d_values, "mpf_fem_assemble: null argument");
+        set_device(m->impl->ctx);
+        mpf::fem_assemble(*m->impl, d_gamma, alpha, d_column_ids, d_values);
+        MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_eit_measure(mpf_model* m, const float* d_phi, int32_t ldphi, int32_t additive, float* d_U) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_phi && d_U, "mpf_eit_measure: null argument");
+        set_device(m->impl->ctx);
+        mpf::fem_measure(*m->impl, d_phi, ldphi, additive, d_U);
+        MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
+        return MPF_OK;
+    });
+}
+
+int mpf_jacobian(mpf_model* m, const float* d_phi, int32_t ldphi, const float* d_gamma, int32_t additive,
+                 float* d_J) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_phi && d_gamma && d_J, "mpf_jacobian: null argument");
+        set_device(m->impl->ctx);
+        mpf::fem_jacobian(*m->impl, d_phi, ldphi, d_gamma, additive, d_J);
+        MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
+        return MPF_OK;
     });
 }
 

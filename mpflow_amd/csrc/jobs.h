@@ -55,6 +55,13 @@ struct JobDev {
     double ref; int logarithmic; int E;
     const int32_t* diag_slot;    // [np]
     const uint8_t* gw;           // [np / 16] entry count of the widest row of each 16-row group
+    // two-level preconditioner (CG only; null: Jacobi alone).  The aggregates are the
+    // 256-row tiles: M^-1 r = D^-1 r + W Einv W^T r with W the tile indicator,
+    // Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W, sigma = 1, plus
+    // n n^T on the dcFree slice; Sc: per-tile conductivity scale of this frame)
+    const float* cinv;           // [C][ntiles][ntiles]
+    float* csc;                  // [ntiles] Sc^-1/2 (written after each assembly)
+    float* cy;                   // [C][ntiles][Rp] coarse correction Einv W^T r
 };
 
 __host__ __device__ inline size_t vec_index(const JobDev& J, int s, int row, int lane) {
@@ -74,8 +81,9 @@ __device__ __forceinline__ MPF_GLOBAL T* gp(T* p) { return (MPF_GLOBAL T*)p; }
 // host launch wrappers (sparse.hip)
 void launch_assemble(const JobDev* d_jobs, const int4* d_items, int nitems, hipStream_t st);
 void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
+// nchunk > 0: jobs with a coarse level start from the two-level preconditioned residual
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                     const int2* d_js, int njs, int* d_active, hipStream_t st);
+                     const int2* d_js, int njs, int* d_active, hipStream_t st, int nchunk = 0);
 // BiCGSTAB (bicgstab.hip): init over all pairs, then one iteration per call (five
 // launches: p-update, SpMV, s-update, SpMV, x/r-update); ev0/ev1 bracket the first SpMV
 void launch_bicg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
@@ -87,9 +95,17 @@ void launch_bicg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int
 // the solve's init) is used: it selects the rho slot.  `variant` picks the SpMV kernel
 // (0: 128-bit gathers, width-binned, for padded widths <= 24; 1: 32-bit gathers).
 constexpr int kSpmvVariants = 2;
+// With coarse work (ncoarse > 0: d_js lists the group's (job, slice) pairs and
+// `nchunk` the largest job's coarse row chunks) a fourth launch between update and
+// pupdate applies the two-level preconditioner's coarse correction.
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                      int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0 = nullptr,
-                     hipEvent_t ev1 = nullptr);
+                     hipEvent_t ev1 = nullptr, const int2* d_js = nullptr, int njs = 0, int nchunk = 0);
+// per-tile conductivity scale of the two-level preconditioner (after assembly); d1: the
+// model's per-tile sigma = 1 diagonal sums; work items with slice != 0 are skipped
+void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const double* d1,
+                         hipStream_t st);
+
 // the SpMV of launch_pcg_iter alone (autotuning)
 void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
                      hipStream_t st);

@@ -67,12 +67,22 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
     size_t total = 0;
     for (auto& g : groups) {
         std::vector<int4> lst;
+        std::vector<int2> gjs;
         long long rows = 0;
         for (int j : g.jobs) {
-            for (int c = 0; c < h[j].C; ++c)
+            for (int c = 0; c < h[j].C; ++c) {
+                gjs.push_back(make_int2(j, c));
                 for (int t = 0; t < h[j].ntiles; ++t) lst.push_back(make_int4(j, c, t, 0));
+            }
             rows += (long long)h[j].C * h[j].np;
+            if (h[j].cinv) {
+                g.nchunk = std::max(g.nchunk, (h[j].ntiles + kCoarseRows - 1) / kCoarseRows);
+                init_nchunk = std::max(init_nchunk, g.nchunk);
+            }
         }
+        g.njs = (int)gjs.size();
+        g.js.alloc(g.njs);
+        g.js.upload(gjs.data(), gjs.size(), s0);
         g.nall = (int)lst.size();
         g.all.resize(lst.size());
         xcd_order(lst.data(), g.nall, g.all.data());
@@ -122,7 +132,8 @@ void KrylovDriver::launch_group_iter(Group& g, int iter, bool bicg, hipStream_t 
     if (bicg)
         launch_bicg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, s, e0, e1);
     else
-        launch_pcg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, g.variant, s, e0, e1);
+        launch_pcg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, g.variant, s, e0, e1, g.js.p, g.njs,
+                        coarse ? g.nchunk : 0);
 }
 
 // Time each SpMV variant that supports the group's width on the live job data (the SpMV
@@ -163,9 +174,14 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
     hipStream_t s0 = ctx->stream;
     const bool bicg = o.krylov == MPF_KRYLOV_BICGSTAB;
     const int nj = (int)h.size();
+    // the two-level preconditioner (jobs with a coarse level; CG only)
+    const bool want_coarse = !bicg && o.preconditioner == MPF_PRECOND_TWO_LEVEL;
+    if (want_coarse != coarse) drop_graph();
+    coarse = want_coarse && init_nchunk > 0;
     for (int j = 0; j < nj; ++j) {
         JobDev& J = h[j];
         J = (*src[origin[j].first])[origin[j].second];
+        if (!coarse) J.cinv = nullptr;
         J.st = st.p + (size_t)j * kMaxSlices * kStInts;
         J.rule = rule;
         J.tol = tol;
@@ -176,7 +192,8 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
     jobs.upload(h.data(), h.size(), s0);
     MPF_HIP(hipMemsetAsync(active.p, 0, sizeof(int), s0));
     if (bicg) launch_bicg_init(jobs.p, init_tiles.p, n_init_tiles, init_js.p, n_init_js, active.p, 0, s0);
-    else launch_pcg_init(jobs.p, init_tiles.p, n_init_tiles, init_js.p, n_init_js, active.p, s0);
+    else launch_pcg_init(jobs.p, init_tiles.p, n_init_tiles, init_js.p, n_init_js, active.p, s0,
+                         coarse ? init_nchunk : 0);
     for (auto& g : groups) {
         g.cur = g.tiles.p;
         g.ncur = g.nall;

@@ -50,11 +50,15 @@ class KrylovDriver {
         std::vector<int> jobs;        // indices into h
         std::vector<int4> all;        // every tile of the group's jobs, XCD-ordered
         DBuf<int4> tiles, act;        // all tiles / the active ones
+        DBuf<int2> js;                // the group's (job, slice) pairs (coarse correction)
+        int njs = 0, nchunk = 0;      // coarse row chunks of the group's largest job
         int nall = 0, ncur = 0;
         const int4* cur = nullptr;
         size_t hoff = 0;              // the group's region of the pinned staging buffer
     };
     void launch_group_iter(Group& g, int iter, bool bicg, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+    bool coarse = false;              // this run uses the two-level preconditioner
+    int init_nchunk = 0;
     void tune(Group& g);
     void drop_graph();
 

@@ -47,16 +47,19 @@ __device__ __forceinline__ void block_partials(const JobDev& J, int s, int tile,
 // kEwBlock-thread block; each row group sums a contiguous range of tiles (loads
 // unrolled so many are in flight), then the groups are combined in order.  The result
 // is valid in threads with g == 0 (lane = column).
+// `count` (default: the job's tile count) partials are summed.
 template <int NV>
-__device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV]) {
+__device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV],
+                                             int count = -1) {
     __shared__ double red[NV][kEwBlock];
     const int Rp = J.Rp, lg = J.lg;
     const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
+    const int nt = count < 0 ? J.ntiles : count;
     double a[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) a[c] = 0.0;
-    const int per = (J.ntiles + G - 1) / G;
-    const int t0 = g * per, t1 = min(J.ntiles, t0 + per);
+    const int per = (nt + G - 1) / G;
+    const int t0 = g * per, t1 = min(nt, t0 + per);
     int t = t0;
     for (; t + 8 <= t1; t += 8) {
 #pragma unroll
@@ -233,6 +236,11 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
             part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
         }
     }
+}
+
+// coarse-correction row chunks of a job (one partial each in channels 5 and 6)
+__device__ __forceinline__ int coarse_chunks(const JobDev& J) {
+    return (J.ntiles + kCoarseRows - 1) / kCoarseRows;
 }
 
 // rho of iteration `par` parity: r.z of the current residual lives in one of two slots so

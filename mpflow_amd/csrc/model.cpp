@@ -55,6 +55,12 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     w_val.alloc(std::max<size_t>(1, S.w_val.size())); if (!S.w_val.empty()) w_val.upload(S.w_val.data(), S.w_val.size(), st0);
     colmap.alloc(S.R); colmap.upload(S.colmap.data(), S.R, st0);
     colscale.alloc(S.R); colscale.upload(S.colscale.data(), S.R, st0);
+    {
+        std::vector<int32_t> inv(np);
+        for (int r = 0; r < np; ++r) inv[r] = r;
+        for (size_t n = 0; n < S.perm.size(); ++n) inv[S.perm[n]] = (int32_t)n;
+        node_of_row.alloc(np); node_of_row.upload(inv.data(), np, st0);
+    }
 
     const size_t ell = (size_t)W * np;
     const size_t V = vec_elems();
@@ -67,6 +73,14 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
     gamma.alloc((size_t)S.E * frames);
+    const int nct = S.coarse_nt;
+    if (nct) {
+        cinv.alloc(S.coarse_inv.size()); cinv.upload(S.coarse_inv.data(), S.coarse_inv.size(), st0);
+        cd1.alloc(nct); cd1.upload(S.coarse_d1.data(), nct, st0);
+        csc.alloc((size_t)nct * frames);
+        cy.alloc((size_t)S.C * nct * S.Rp * frames);
+        MPF_HIP(hipMemsetAsync(cy.p, 0, cy.bytes(), st0));
+    }
     MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
     MPF_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st0));
 
@@ -103,6 +117,11 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.ref = S.ref; J.logarithmic = S.logarithmic; J.E = S.E;
         J.diag_slot = diag_slot.p;
         J.gw = gw.p;
+        if (nct) {
+            J.cinv = cinv.p;
+            J.csc = csc.p + (size_t)nct * f;
+            J.cy = cy.p + (size_t)S.C * nct * S.Rp * f;
+        }
     }
     jobs.alloc(frames);
     upload_jobs();
@@ -141,6 +160,8 @@ void Model::upload_jobs() {
 void Model::prepare(const mpf_solve_opts& o) {
     hipStream_t st0 = ctx->stream;
     MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG || o.krylov == MPF_KRYLOV_BICGSTAB, "solve: unknown Krylov method");
+    MPF_REQUIRE(o.preconditioner == MPF_PRECOND_JACOBI || o.preconditioner == MPF_PRECOND_TWO_LEVEL,
+                "solve: unknown preconditioner");
     if (o.krylov == MPF_KRYLOV_BICGSTAB) {
         MPF_REQUIRE(S.wpad <= 24, "BiCGSTAB: padded ELL width above 24 is not supported");
         ensure_bicg();
@@ -154,6 +175,8 @@ void Model::prepare(const mpf_solve_opts& o) {
     upload_jobs();
     launch_assemble(jobs.p, asm_items.p, nasm, st0);
     launch_dinv(jobs.p, tiles.p, ntiles, st0);
+    if (S.coarse_nt && o.preconditioner == MPF_PRECOND_TWO_LEVEL && o.krylov == MPF_KRYLOV_CG)
+        launch_coarse_scale(jobs.p, tiles.p, ntiles, cd1.p, st0);
 }
 
 // FP64 refinement report, voltages and (optionally) the Jacobian after the solves.

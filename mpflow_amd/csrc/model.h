@@ -33,8 +33,11 @@ struct Model {
     int quirks = 1;
     // mesh data on the device
     DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node, colmap;
+    DBuf<int32_t> node_of_row;   // [np] device row -> caller's node (identity past the mesh nodes)
     DBuf<uint8_t> gw;
     DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
+    DBuf<float> cinv, csc, cy;          // two-level preconditioner (allocated if the mesh has one)
+    DBuf<double> cd1;
     DBuf<float> ek, em;
     // per-frame state
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, rhs;

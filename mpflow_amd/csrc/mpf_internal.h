@@ -14,6 +14,8 @@ namespace mpf {
 
 constexpr int kMaxSlices = 8;      // harmonics per job
 constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
+constexpr int kCoarseRows = 32;    // coarse rows per block of the coarse-correction kernel
+constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level
 constexpr int kBlock = 256;        // threads per SpMV block: 4 wavefronts x 64 rows
 constexpr int kEwBlock = 1024;     // threads per streaming / reduction block
 constexpr int kSlotPad = 8;        // ELL width is padded to a multiple of this
@@ -100,6 +102,11 @@ struct MeshSetup {
     double alpha[kMaxSlices] = {}, beta[kMaxSlices] = {};
     double ref = 1.0, height = 1.0, port_height = 1.0;
     int logarithmic = 0, jacobian_exact = 0;
+    // two-level preconditioner: tile count (0: none), [C][nt][nt] inverses of the coarse
+    // matrices at sigma = 1, per-tile sigma = 1 stiffness-diagonal sums
+    int coarse_nt = 0;
+    std::vector<float> coarse_inv;
+    std::vector<double> coarse_d1;
 };
 
 void build_setup(const mpf_model_desc& d, MeshSetup& s);

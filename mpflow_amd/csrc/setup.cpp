@@ -199,13 +199,16 @@ static void edge_integrals(int kb, const double* sp, double (*m)[3], double* v) 
     }
 }
 
-// Device node numbering.  Reverse Cuthill-McKee over the element graph (BFS from a
-// pseudo-peripheral node, neighbours visited by ascending degree, order reversed) keeps the
-// neighbours of every 256-row tile inside a narrow band, so the SpMV's gathers of p stay in
-// the XCD's L2; rows are then sorted by descending degree inside each 256-row window so the
-// 16-row groups of the SpMV have near-uniform ELL widths.  Returns perm[node] = row.
-static std::vector<int32_t> device_order(int N, int E, int k, const int32_t* el) {
-    std::vector<int32_t> off(N + 1, 0), adj;
+// Device node numbering.  Recursive coordinate bisection: a node set is split at the
+// median of its longer coordinate extent, the first part holding a whole number of
+// 256-node leaves, until every leaf fits one 256-row tile.  Each tile is then a compact
+// patch of the mesh: most gathers of its SpMV rows stay inside it, consecutive tiles are
+// neighbouring patches (the XCD-striped work lists keep them in one L2), and the tile
+// is one aggregate of the two-level preconditioner.  Rows are sorted by descending
+// degree inside each tile so the 16-row groups of the SpMV have near-uniform ELL widths.
+// Returns perm[node] = row.
+static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, const int32_t* el) {
+    std::vector<int32_t> deg(N, 0);
     {
         std::vector<std::vector<int32_t>> nb(N);
         for (int e = 0; e < E; ++e)
@@ -214,59 +217,120 @@ static std::vector<int32_t> device_order(int N, int E, int k, const int32_t* el)
                     if (i != j) nb[el[(size_t)e * k + i]].push_back(el[(size_t)e * k + j]);
         for (int n = 0; n < N; ++n) {
             std::sort(nb[n].begin(), nb[n].end());
-            nb[n].erase(std::unique(nb[n].begin(), nb[n].end()), nb[n].end());
-            off[n + 1] = off[n] + (int32_t)nb[n].size();
+            deg[n] = (int32_t)(std::unique(nb[n].begin(), nb[n].end()) - nb[n].begin());
         }
-        adj.reserve(off[N]);
-        for (int n = 0; n < N; ++n) adj.insert(adj.end(), nb[n].begin(), nb[n].end());
     }
-    auto deg = [&](int n) { return off[n + 1] - off[n]; };
-    std::vector<int32_t> level(N, -1), order;
-    order.reserve(N);
-    // BFS from `root` over unvisited nodes; returns the nodes in visiting order
-    auto bfs = [&](int root, std::vector<int32_t>& out, std::vector<int32_t>& lv) {
-        out.clear();
-        lv[root] = 0;
-        out.push_back(root);
-        std::vector<int32_t> nbrs;
-        for (size_t h = 0; h < out.size(); ++h) {
-            const int v = out[h];
-            nbrs.assign(adj.begin() + off[v], adj.begin() + off[v + 1]);
-            std::stable_sort(nbrs.begin(), nbrs.end(), [&](int a, int b) { return deg(a) < deg(b); });
-            for (int w : nbrs)
-                if (lv[w] < 0) { lv[w] = lv[v] + 1; out.push_back(w); }
-        }
-    };
-    std::vector<char> done(N, 0);
-    std::vector<int32_t> comp, scratch(N, -1);
-    for (int seed = 0; seed < N; ++seed) {
-        if (done[seed]) continue;
-        // pseudo-peripheral root: repeat BFS from the last level's lowest-degree node
-        int root = seed;
-        int ecc = -1;
-        for (int pass = 0; pass < 4; ++pass) {
-            bfs(root, comp, scratch);
-            const int last = scratch[comp.back()];
-            int best = comp.back();
-            for (int v : comp)
-                if (scratch[v] == last && deg(v) < deg(best)) best = v;
-            for (int v : comp) scratch[v] = -1;
-            if (last <= ecc) break;
-            ecc = last;
-            root = best;
-        }
-        bfs(root, comp, level);
-        for (int v : comp) done[v] = 1;
-        order.insert(order.end(), comp.begin(), comp.end());
+    std::vector<int32_t> order(N);
+    for (int n = 0; n < N; ++n) order[n] = n;
+    std::vector<std::pair<int, int>> todo{{0, N}};
+    while (!todo.empty()) {
+        const auto [b, e] = todo.back();
+        todo.pop_back();
+        const int n = e - b;
+        if (n <= kTileRows) continue;
+        double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+        for (int i = b; i < e; ++i)
+            for (int a = 0; a < 2; ++a) {
+                lo[a] = std::min(lo[a], xy[2 * (size_t)order[i] + a]);
+                hi[a] = std::max(hi[a], xy[2 * (size_t)order[i] + a]);
+            }
+        const int ax = (hi[0] - lo[0]) >= (hi[1] - lo[1]) ? 0 : 1;
+        const int leaves = (n + kTileRows - 1) / kTileRows;
+        const int left = (leaves / 2) * kTileRows;
+        // ties broken by node id: the split does not depend on the library's algorithm
+        std::nth_element(order.begin() + b, order.begin() + b + left, order.begin() + e, [&](int a, int c) {
+            const double va = xy[2 * (size_t)a + ax], vc = xy[2 * (size_t)c + ax];
+            return va < vc || (va == vc && a < c);
+        });
+        todo.emplace_back(b + left, e);
+        todo.emplace_back(b, b + left);
     }
-    std::reverse(order.begin(), order.end());
-    for (size_t w = 0; w < order.size(); w += kTileRows) {
-        auto b = order.begin() + w, e = order.begin() + std::min(order.size(), w + kTileRows);
-        std::stable_sort(b, e, [&](int a, int c) { return deg(a) > deg(c); });
-    }
+    for (int w = 0; w < N; w += kTileRows)
+        std::sort(order.begin() + w, order.begin() + std::min(N, w + kTileRows),
+                  [&](int a, int c) { return deg[a] != deg[c] ? deg[a] > deg[c] : a < c; });
     std::vector<int32_t> perm(N);
     for (int r = 0; r < N; ++r) perm[order[r]] = r;
     return perm;
+}
+
+// In-place inverse of an SPD matrix (n x n, row-major, FP64) by Cholesky: A = L L^T,
+// A^-1 = L^-T L^-1.  Returns false if A is not numerically positive definite.
+static bool spd_inverse(int n, double* A) {
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int q = 0; q < j; ++q) d -= L[(size_t)j * n + q] * L[(size_t)j * n + q];
+        if (!(d > 0.0)) return false;
+        const double ljj = std::sqrt(d);
+        L[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[(size_t)i * n + j];
+            const double* li = &L[(size_t)i * n];
+            const double* lj = &L[(size_t)j * n];
+            for (int q = 0; q < j; ++q) v -= li[q] * lj[q];
+            L[(size_t)i * n + j] = v / ljj;
+        }
+    }
+    // M = L^-1 (lower triangular), row by row of forward substitution
+    std::vector<double> M((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        M[(size_t)c * n + c] = 1.0 / L[(size_t)c * n + c];
+        for (int i = c + 1; i < n; ++i) {
+            double v = 0.0;
+            for (int q = c; q < i; ++q) v -= L[(size_t)i * n + q] * M[(size_t)q * n + c];
+            M[(size_t)i * n + c] = v / L[(size_t)i * n + i];
+        }
+    }
+    // A^-1 = M^T M: (i, j) = sum_{q >= max(i, j)} M[q][i] M[q][j]
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+            for (int q = i; q < n; ++q) v += M[(size_t)q * n + i] * M[(size_t)q * n + j];
+            A[(size_t)i * n + j] = v;
+            A[(size_t)j * n + i] = v;
+        }
+    return true;
+}
+
+// Coarse level of the two-level preconditioner (aggregates = 256-row tiles, W = the
+// indicator of each tile's real rows): per harmonic, the inverse of W^T A_c(1) W at
+// sigma = 1 (plus n n^T, n = real rows per tile, on the dcFree harmonic 0, matching the
+// A + 1 1^T operator), and per tile the sum of the sigma = 1 stiffness diagonal that
+// scales it to a frame's conductivity.  Skipped (coarse_nt = 0) above kCoarseMaxTiles
+// tiles.
+static void coarse_setup(MeshSetup& s) {
+    const int np = s.np, nt = np / kTileRows;
+    s.coarse_nt = 0;
+    if (nt > kCoarseMaxTiles) return;
+    const size_t nent = (size_t)s.wpad * np;
+    std::vector<double> sk(nent, 0.0), sm(nent, 0.0);
+    for (size_t ent = 0; ent < nent; ++ent)
+        for (int i = s.ctb_off[ent]; i < s.ctb_off[ent + 1]; ++i) { sk[ent] += s.ctb_k[i]; sm[ent] += s.ctb_m[i]; }
+    std::vector<double> cnt(nt, 0.0);
+    for (int row = 0; row < s.N; ++row) cnt[row / kTileRows] += 1.0;
+    s.coarse_d1.assign(nt, 0.0);
+    for (int row = 0; row < s.N; ++row)
+        s.coarse_d1[row / kTileRows] += sk[(size_t)s.diag_slot[row] * np + row];
+    s.coarse_inv.assign((size_t)s.C * nt * nt, 0.0f);
+    std::vector<double> Ec((size_t)nt * nt);
+    for (int c = 0; c < s.C; ++c) {
+        std::fill(Ec.begin(), Ec.end(), 0.0);
+        for (int q = 0; q < s.wpad; ++q)
+            for (int row = 0; row < s.N; ++row) {
+                const size_t ent = (size_t)q * np + row;
+                if (s.ctb_off[ent] == s.ctb_off[ent + 1]) continue;   // padding slot
+                Ec[(size_t)(row / kTileRows) * nt + s.ids[ent] / kTileRows] += sk[ent] + s.alpha[c] * sm[ent];
+            }
+        if (c == 0)
+            for (int t = 0; t < nt; ++t)
+                for (int u = 0; u < nt; ++u) Ec[(size_t)t * nt + u] += cnt[t] * cnt[u];
+        // tiles without real rows (none with the padded tile count) keep an identity block
+        for (int t = 0; t < nt; ++t)
+            if (cnt[t] == 0.0) Ec[(size_t)t * nt + t] = 1.0;
+        if (!spd_inverse(nt, Ec.data())) return;
+        for (size_t i = 0; i < Ec.size(); ++i) s.coarse_inv[(size_t)c * nt * nt + i] = (float)Ec[i];
+    }
+    s.coarse_nt = nt;
 }
 
 void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
@@ -287,7 +351,8 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         s.perm.resize(caller.n_nodes);
         for (int n = 0; n < caller.n_nodes; ++n) s.perm[n] = n;
     } else {
-        s.perm = device_order(caller.n_nodes, caller.n_elements, caller.nodes_per_element, caller.elements);
+        s.perm = device_order(caller.n_nodes, caller.nodes, caller.n_elements, caller.nodes_per_element,
+                              caller.elements);
         nodes_r.resize((size_t)2 * caller.n_nodes);
         for (int n = 0; n < caller.n_nodes; ++n) {
             nodes_r[2 * (size_t)s.perm[n]] = caller.nodes[2 * (size_t)n];
@@ -555,6 +620,7 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         }
         s.w_off[m + 1] = (int32_t)s.w_node.size();
     }
+    coarse_setup(s);
 }
 
 }  // namespace mpf

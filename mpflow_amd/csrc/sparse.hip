@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
     const Ew L = ew_of(J);
     const float* dinv = J.dinv + (size_t)s * J.np;
     const float xs = J.xscale[s];
-    double rz = 0.0, bb = 0.0;
+    double rz = 0.0, bb = 0.0, bs = 0.0;
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
@@ -86,10 +86,25 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
         J.p[i] = z;
         rz += (double)b * (double)z;
         bb += (double)b * (double)b;
+        bs += (double)b;
     }
-    const int ch[2] = {0, 1};
-    const double v[2] = {rz, bb};
-    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
+    // channel 4: the tile's W^T r for the two-level preconditioner
+    const int ch[3] = {0, 1, 4};
+    const double v[3] = {rz, bb, bs};
+    block_partials<3, kEwBlock>(J, s, t.z, ch, v);
+}
+
+// two-level start: p = z = D^-1 b + W y (y from the coarse kernel, real rows only)
+__global__ void __launch_bounds__(kEwBlock) k_pcg_init_p(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles) {
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    if (!J.cinv) return;
+    const int s = t.y;
+    const Ew L = ew_of(J);
+    const float y = J.cy[((size_t)s * J.ntiles + t.z) * J.Rp + L.lane];
+    for (int row = t.z * kTileRows + L.g; row < min((t.z + 1) * kTileRows, J.n); row += L.G)
+        J.p[vec_index(J, s, row, L.lane)] += y;
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
@@ -101,6 +116,13 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
     const int ch[2] = {0, 1};
     double out[2];
     reduce_tiles<2>(J, s, ch, out);
+    if (J.cinv) {
+        // rho = b.D^-1 b + (W^T b).y, summed per coarse chunk by the coarse kernel
+        const int chc[1] = {5};
+        double oc[1];
+        reduce_tiles<1>(J, s, chc, oc, coarse_chunks(J));
+        out[0] = oc[0];
+    }
     if ((int)threadIdx.x < J.Rp) {
         sc_ptr(J, SC_RHO, s)[threadIdx.x] = out[0];
         // a correction solve started from a previous solution stops relative to the
@@ -283,7 +305,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     const MPF_GLOBAL float4* Q = (const MPF_GLOBAL float4*)gp((const float*)J.q) + base;
     MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
     MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
-    double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    double v[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
         const float4 p4 = P[i], q4 = Q[i];
@@ -299,12 +321,111 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
             const float z = d * rv[j];
             v[0][j] += (double)rv[j] * (double)z;
             v[1][j] += (double)rv[j] * (double)rv[j];
+            v[2][j] += (double)rv[j];
         }
         X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
         Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
     }
-    const int ch[2] = {2, 3};
-    block_partials4<LG, 2>(J, s, t.z, ch, v);
+    // channel 4: the tile's W^T r for the two-level preconditioner
+    const int ch[3] = {2, 3, 4};
+    block_partials4<LG, 3>(J, s, t.z, ch, v);
+}
+
+// Coarse correction of the two-level preconditioner, one block per (row chunk, job,
+// slice): y_t = c_t sum_u cinv_s[t][u] c_u g_u with g = W^T r (channel 4 tile sums) and
+// c = Sc^-1/2, written to cy; then per chunk the partial sums of
+//   rho = sum_t (r.D^-1 r)_t + g_t y_t   (from channel ch_rz)  -> channel 5
+//   r.r = sum_t (r.r)_t                  (from channel ch_rr)  -> channel 6
+// so the next kernel reduces one partial per chunk instead of one per tile.
+template <int LG>
+__global__ void __launch_bounds__(256) k_pcg_coarse(const JobDev* __restrict__ jobs, const int2* __restrict__ js,
+                                                    int ch_rz, int ch_rr, int in_iteration) {
+    constexpr int Rp = 1 << LG, G = 256 / Rp, PER = kCoarseRows / G, KU = 64;
+    static_assert(kCoarseRows % G == 0, "coarse rows per block must split over the row groups");
+    const int2 pr = js[blockIdx.y];
+    const JobDev& J = jobs[pr.x];
+    const int s = pr.y;
+    // (at the start the slice's state is not set yet: every listed slice runs)
+    if (!J.cinv || J.Rp != Rp || (in_iteration && !slice_active(J, s))) return;
+    const int nt = J.ntiles;
+    const int t0 = blockIdx.x * kCoarseRows;
+    if (t0 >= nt) return;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
+    __shared__ float gs[KU][Rp];
+    __shared__ float cs[kCoarseRows][KU + 1];
+    const MPF_GLOBAL float* cinv = gp(J.cinv) + (size_t)s * nt * nt;
+    const MPF_GLOBAL float* csc = gp((const float*)J.csc);
+    float acc[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
+    for (int u0 = 0; u0 < nt; u0 += KU) {
+        const int un = min(KU, nt - u0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < KU * Rp; i += 256) {
+            const int u = i / Rp, l = i % Rp;
+            gs[u][l] = u < un ? (float)((double)csc[u0 + u] * part_ptr(J, 4, s, u0 + u)[l]) : 0.0f;
+        }
+        for (int i = threadIdx.x; i < kCoarseRows * KU; i += 256) {
+            const int r = i / KU, u = i % KU;
+            cs[r][u] = (t0 + r < nt && u < un) ? cinv[(size_t)(t0 + r) * nt + u0 + u] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int r = g + k * G;
+            float a = 0.0f;
+            for (int u = 0; u < un; ++u) a += cs[r][u] * gs[u][lane];
+            acc[k] += a;
+        }
+    }
+    double rz = 0.0, rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int t = t0 + g + k * G;
+        if (t < nt) {
+            const float y = csc[t] * acc[k];
+            gp(J.cy)[((size_t)s * nt + t) * Rp + lane] = y;
+            rz += part_ptr(J, ch_rz, s, t)[lane] + part_ptr(J, 4, s, t)[lane] * (double)y;
+            rr += part_ptr(J, ch_rr, s, t)[lane];
+        }
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = rz;
+    red[1][threadIdx.x] = rr;
+    __syncthreads();
+    if (g == 0) {
+        double a = 0.0, b = 0.0;
+        for (int gg = 0; gg < G; ++gg) { a += red[0][gg * Rp + lane]; b += red[1][gg * Rp + lane]; }
+        part_ptr(J, 5, s, blockIdx.x)[lane] = a;
+        part_ptr(J, 6, s, blockIdx.x)[lane] = b;
+    }
+}
+
+// Sc^-1/2 per tile: Sc_t = sum of S_ii(gamma) over the tile's real rows / the same sum
+// at sigma = 1 (harmonic independent; one block per (job, tile) of slice 0)
+__global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __restrict__ jobs,
+                                                           const int4* __restrict__ tiles,
+                                                           const double* __restrict__ d1) {
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    if (t.y != 0 || !J.csc) return;
+    const int row = t.z * kTileRows + threadIdx.x;
+    double v = 0.0;
+    if (row < J.n) {
+        const size_t e = (size_t)J.diag_slot[row] * J.np + row;
+        v = (double)J.s[e] + (double)J.s_lo[e];
+    }
+    __shared__ double red[kTileRows];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kTileRows / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double sc = d1[t.z] > 0.0 ? red[0] / d1[t.z] : 0.0;
+        J.csc[t.z] = sc > 0.0 ? (float)(1.0 / sqrt(sc)) : 0.0f;
+    }
 }
 
 // pupdate: rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p.  Tile 0's
@@ -326,13 +447,16 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
     __shared__ int not_done;
     {
         double out[2] = {0.0, 0.0};
+        // two-level: rho and r.r come as one partial per coarse chunk (channels 5, 6)
+        const int cnt = J.cinv ? coarse_chunks(J) : -1;
+        const int c_rz = J.cinv ? 5 : 2, c_rr = J.cinv ? 6 : 3;
         if (book) {
-            const int ch[2] = {2, 3};
-            reduce_tiles<2>(J, s, ch, out);
+            const int ch[2] = {c_rz, c_rr};
+            reduce_tiles<2>(J, s, ch, out, cnt);
         } else {
-            const int ch[1] = {2};
+            const int ch[1] = {c_rz};
             double o[1];
-            reduce_tiles<1>(J, s, ch, o);
+            reduce_tiles<1>(J, s, ch, o, cnt);
             out[0] = o[0];
         }
         if (threadIdx.x == 0) not_done = 0;
@@ -393,12 +517,18 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Rr = (const MPF_GLOBAL float4*)gp((const float*)J.rv) + base;
     MPF_GLOBAL float4* P = (MPF_GLOBAL float4*)gp(J.p) + base;
+    // two-level: z = D^-1 r + y_tile on the tile's real rows
+    float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (J.cinv) y4 = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))[((size_t)s * J.ntiles + t.z) * LPR + c4];
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
         const float4 r4 = Rr[i], p4 = P[i];
         const float d = dinv[row];
-        P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + b[0] * p4.x, zero[1] ? 0.0f : d * r4.y + b[1] * p4.y,
-                           zero[2] ? 0.0f : d * r4.z + b[2] * p4.z, zero[3] ? 0.0f : d * r4.w + b[3] * p4.w);
+        const float w = row < J.n ? 1.0f : 0.0f;
+        P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
+                           zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
+                           zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
+                           zero[3] ? 0.0f : d * r4.w + w * y4.w + b[3] * p4.w);
     }
 }
 
@@ -473,6 +603,13 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64_reduce(const JobDev* __
     const int ch[2] = {0, 1};
     double out[2];
     reduce_tiles<2>(J, s, ch, out);
+    if (J.cinv) {
+        // rho = b.D^-1 b + (W^T b).y, summed per coarse chunk by the coarse kernel
+        const int chc[1] = {5};
+        double oc[1];
+        reduce_tiles<1>(J, s, chc, oc, coarse_chunks(J));
+        out[0] = oc[0];
+    }
     if ((int)threadIdx.x < J.Rp) {
         sc_ptr(J, SC_X1, s)[threadIdx.x] = out[0];
         sc_ptr(J, SC_X2, s)[threadIdx.x] = out[1];
@@ -536,10 +673,36 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
     check_launch();
 }
 
+template <int LG>
+static void coarse_launch(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, int ch_rz, int ch_rr,
+                          hipStream_t st) {
+    k_pcg_coarse<LG><<<dim3(nchunk, njs), 256, 0, st>>>(d_jobs, d_js, ch_rz, ch_rr, ch_rz != 0);
+}
+
+// every lane count present in the list runs its own instantiation; blocks whose job has
+// another lane count exit at once
+static void launch_coarse_all(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, int lg, int ch_rz,
+                              int ch_rr, hipStream_t st) {
+    if (lg == 6 || lg < 0) coarse_launch<6>(d_jobs, d_js, njs, nchunk, ch_rz, ch_rr, st);
+    if (lg == 5 || lg < 0) coarse_launch<5>(d_jobs, d_js, njs, nchunk, ch_rz, ch_rr, st);
+    if (lg == 4 || lg < 0) coarse_launch<4>(d_jobs, d_js, njs, nchunk, ch_rz, ch_rr, st);
+}
+
 void launch_pcg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                     int njs, int* d_active, hipStream_t st) {
+                     int njs, int* d_active, hipStream_t st, int nchunk) {
     k_pcg_init<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    if (nchunk > 0) {
+        // the init list mixes lane counts: all three instantiations run
+        launch_coarse_all(d_jobs, d_js, njs, nchunk, -1, 0, 1, st);
+        k_pcg_init_p<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    }
     k_pcg_init_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js, d_active);
+    check_launch();
+}
+
+void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const double* d1,
+                         hipStream_t st) {
+    k_coarse_scale<<<ntiles, kTileRows, 0, st>>>(d_jobs, d_tiles, d1);
     check_launch();
 }
 
@@ -580,7 +743,8 @@ void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int 
 }
 
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
-                     int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+                     int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
+                     const int2* d_js, int njs, int nchunk) {
     const int par = iter & 1;
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
     launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st);
@@ -588,6 +752,7 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
     if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
     else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
     else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
+    if (nchunk > 0 && njs > 0) launch_coarse_all(d_jobs, d_js, njs, nchunk, lg, 2, 3, st);
     if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);

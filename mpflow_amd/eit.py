@@ -71,7 +71,7 @@ def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwis
 class SolveOptions:
     def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
                  refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0,
-                 bicgstab_ignore_dc=0, graphs=0):
+                 bicgstab_ignore_dc=0, graphs=0, preconditioner=capi.MPF_PRECOND_JACOBI):
         self.s = capi.SolveOpts()
         lib().mpf_solve_opts_default(C.byref(self.s))
         self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
@@ -80,6 +80,7 @@ class SolveOptions:
         self.s.warm_start = int(warm_start)
         self.s.bicgstab_ignore_dc = int(bicgstab_ignore_dc)
         self.s.graphs = int(graphs)
+        self.s.preconditioner = int(preconditioner)
 
     def copy(self, **kw):
         o = SolveOptions()
@@ -163,6 +164,37 @@ class EIT:
         check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
         self.last_stats = st
         return self.result
+
+    def update(self, gamma, alpha=0.0, column_ids=None, values=None):
+        """Equation::update of one frame (mpf_fem_assemble): the system matrix
+        S(gamma) + alpha R(gamma) as the SparseMatrix arrays (rows x 32 column ids, values),
+        caller's node numbering.  Returns (column_ids uint32-as-int32, values) cuda tensors."""
+        torch = self.torch
+        g = gamma.reshape(self.E).to(self._dev(), torch.float32).contiguous()
+        if values is None:
+            values = torch.empty(self.N, 32, dtype=torch.float32, device=self._dev())
+        if column_ids is None:
+            column_ids = torch.empty(self.N, 32, dtype=torch.int32, device=self._dev())
+        check(lib().mpf_fem_assemble(self.h, C.c_void_p(g.data_ptr()), float(alpha),
+                                     C.c_void_p(column_ids.data_ptr()), C.c_void_p(values.data_ptr())))
+        return column_ids, values
+
+    def apply_measurement_pattern(self, phi, result, additive=False):
+        """EIT::applyMeasurementPattern (mpf_eit_measure): result (ceil16(M) x D,
+        column-major) (+)= W phi[:, :D]; phi is a (D + M, ld) tensor (column-major rows x
+        (D + M) fields, ld >= rows)."""
+        check(lib().mpf_eit_measure(self.h, C.c_void_p(phi.data_ptr()), int(phi.shape[-1]), int(bool(additive)),
+                                    C.c_void_p(result.data_ptr())))
+        return result
+
+    def calc_jacobian(self, phi, gamma, jacobian, additive=False):
+        """Equation::calcJacobian (mpf_jacobian) for one harmonic's fields ``phi`` (as in
+        ``apply_measurement_pattern``); ``jacobian`` is E x m (column-major m x E)."""
+        torch = self.torch
+        g = gamma.reshape(self.E).to(self._dev(), torch.float32).contiguous()
+        check(lib().mpf_jacobian(self.h, C.c_void_p(phi.data_ptr()), int(phi.shape[-1]), C.c_void_p(g.data_ptr()),
+                                 int(bool(additive)), C.c_void_p(jacobian.data_ptr())))
+        return jacobian
 
     def voltages(self, U=None, dtype=None):
         """(frames, M, D) view of the padded result."""

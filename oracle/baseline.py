@@ -1,4 +1,5 @@
-"""CPU baseline leg: the same ELL Jacobi-PCG (FP32 vectors, FP64 scalars) timed on the host.
+"""CPU baseline leg: the same ELL PCG (Jacobi or two-level; FP32 vectors, FP64 scalars)
+timed on the host.
 
 TEST INFRASTRUCTURE ONLY (bench.py's cpu_baseline leg and tests/).  The ELL matrix is
 built here from the oracle's own FP64 assembly (eit_oracle.assemble), not from the
@@ -21,7 +22,7 @@ def _lib():
     L = C.CDLL(build_baseline.LIB)
     L.cgh_pcg.restype = C.c_int
     L.cgh_pcg.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                          C.c_int, C.c_double, C.c_int]
+                          C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int]
     L.cgh_threads.restype = C.c_int
     L.cgh_set_threads.argtypes = [C.c_int]
     return L
@@ -42,8 +43,10 @@ def csr_to_ell(A):
     return np.ascontiguousarray(ids), np.ascontiguousarray(vals)
 
 
-def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0):
-    """Solve (A + dc 1 1^T) X = B with the C++ baseline; returns (X, steps, seconds)."""
+def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0, einv=None, tile=256):
+    """Solve (A + dc 1 1^T) X = B with the C++ baseline; returns (X, steps, seconds).
+    ``einv`` (nt x nt, nt = ceil(N / tile)): the two-level preconditioner's scaled coarse
+    inverse (``eit_oracle.two_level_preconditioner``); None: Jacobi alone."""
     L = _lib()
     if threads:
         L.cgh_set_threads(int(threads))
@@ -52,9 +55,10 @@ def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0):
     B = np.ascontiguousarray(B, np.float32)
     R = B.shape[1]
     X = np.zeros_like(B)
+    E = None if einv is None else np.ascontiguousarray(einv, np.float32)
     t0 = time.perf_counter()
     steps = L.cgh_pcg(n, width, ids.ctypes.data, vals.ctypes.data, R, B.ctypes.data, X.ctypes.data,
-                      int(dc), float(tol), int(max_iter))
+                      int(dc), float(tol), int(max_iter), None if E is None else E.ctypes.data, int(tile))
     return X, steps, time.perf_counter() - t0
 
 

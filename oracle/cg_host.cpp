@@ -1,11 +1,12 @@
-// CPU baseline of the forward hot loop: multi-RHS Jacobi-PCG on an ELL matrix in FP32
-// storage with FP64 scalars, the arithmetic of ConjugateGradient::solve
+// CPU baseline of the forward hot loop: multi-RHS Jacobi-PCG (optionally with the
+// two-level coarse correction) on an ELL matrix in FP32 storage with FP64 scalars, the arithmetic of ConjugateGradient::solve
 // (src/numeric/conjugate_gradient.cpp:61-161) with the dcFree 1 1^T term
 // (conjugate_gradient_kernel.cu:34-44).  TEST INFRASTRUCTURE ONLY: used as the timed
 // CPU baseline of bench.py and cross-checked against the FP64 oracle in tests/.
 //
 // Layout: ELL ids/vals row-major [row][width]; vectors [row][R] (RHS contiguous).
 // OpenMP parallelises over rows.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -19,10 +20,39 @@ int cgh_threads(void) { return omp_get_max_threads(); }
 
 void cgh_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
+// Two-level coarse term: g = W^T r over `tile`-row aggregates, y = einv g (nt x nt,
+// row-major, already scaled), returns sum_t g_t . y_t per column into gy.
+static void coarse(int n, int R, int tile, int nt, const float* einv, const std::vector<float>& r,
+                   std::vector<double>& g, std::vector<float>& y, std::vector<double>& gy) {
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        for (int c = 0; c < R; ++c) g[(size_t)t * R + c] = 0.0;
+        for (int i = t * tile; i < std::min(n, (t + 1) * tile); ++i)
+            for (int c = 0; c < R; ++c) g[(size_t)t * R + c] += r[(size_t)i * R + c];
+    }
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        std::vector<float> a(R, 0.0f);
+        for (int u = 0; u < nt; ++u) {
+            const float e = einv[(size_t)t * nt + u];
+            for (int c = 0; c < R; ++c) a[c] += e * (float)g[(size_t)u * R + c];
+        }
+        for (int c = 0; c < R; ++c) y[(size_t)t * R + c] = a[c];
+    }
+    for (int c = 0; c < R; ++c) gy[c] = 0.0;
+    for (int t = 0; t < nt; ++t)
+        for (int c = 0; c < R; ++c) gy[c] += g[(size_t)t * R + c] * (double)y[(size_t)t * R + c];
+}
+
 // Solves (A + dc 1 1^T) X = B for R columns.  Stops when every column has
 // ||r|| <= tol ||b|| (tol > 0), or after max_iter steps.  Returns steps taken.
+// einv (nullable): the two-level preconditioner's scaled coarse inverse over
+// aggregates of `tile` consecutive rows, M^-1 r = D^-1 r + W einv W^T r.
 int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, const float* B,
-            float* X, int dc, double tol, int max_iter) {
+            float* X, int dc, double tol, int max_iter, const float* einv, int tile) {
+    const int nt = einv ? (n + tile - 1) / tile : 0;
+    std::vector<double> cg((size_t)nt * R), gy(R, 0.0);
+    std::vector<float> cy((size_t)nt * R, 0.0f);
     std::vector<float> r((size_t)n * R), p((size_t)n * R), q((size_t)n * R), dinv(n);
     for (int i = 0; i < n; ++i) {
         float d = 0.0f;
@@ -49,6 +79,13 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
             }
 #pragma omp critical
         for (int c = 0; c < R; ++c) { rho[c] += lrho[c]; bb[c] += lbb[c]; }
+    }
+    if (nt) {
+        coarse(n, R, tile, nt, einv, r, cg, cy, gy);
+        for (int c = 0; c < R; ++c) rho[c] += gy[c];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < R; ++c) p[(size_t)i * R + c] += cy[(size_t)(i / tile) * R + c];
     }
     int step = 0;
     for (; step < max_iter; ++step) {
@@ -98,6 +135,10 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
 #pragma omp critical
             for (int c = 0; c < R; ++c) { nrho[c] += lrz[c]; rr[c] += lrr[c]; }
         }
+        if (nt) {
+            coarse(n, R, tile, nt, einv, r, cg, cy, gy);
+            for (int c = 0; c < R; ++c) nrho[c] += gy[c];
+        }
         bool done = true;
         for (int c = 0; c < R; ++c) {
             if (bb[c] > 0.0 && rr[c] > tol * tol * bb[c]) done = false;
@@ -109,7 +150,7 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < R; ++c) {
                 const size_t o = (size_t)i * R + c;
-                p[o] = dinv[i] * r[o] + (float)beta[c] * p[o];
+                p[o] = dinv[i] * r[o] + (nt ? cy[(size_t)(i / tile) * R + c] : 0.0f) + (float)beta[c] * p[o];
             }
     }
     return step;

@@ -310,7 +310,7 @@ def ceil16(n):
 # ---------------------------------------------------------------------------
 
 def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
-                  rule="reference", tol=1e-6):
+                  rule="reference", tol=1e-6, prec=None):
     """Multi-RHS Jacobi-PCG as ``ConjugateGradient::solve`` runs it, in float64.
 
     * x starts at 0;  dcFree adds ``1 1^T`` to the operator.
@@ -319,6 +319,8 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     * ``rule="reference"``: stop after step s when sqrt(|rho[0]| / (b.b)[0]) < eps
       (column 0 only) and return s+1; otherwise stop after 10*N steps.
       ``rule="tolerance"``: stop when every column has ||r||/||b|| < tol.
+    * ``prec`` (optional callable r -> z) replaces the diagonal preconditioner (the
+      product's two-level preconditioner, ``two_level_preconditioner``).
     Returns (x, steps).
     """
     N, R = b.shape
@@ -330,9 +332,10 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         if dc_free:
             y = y + v.sum(axis=0, keepdims=True)
         return y
+    precond = prec if prec is not None else (lambda v: dinv[:, None] * v)
     x = np.zeros((N, R))
     r = b.copy()
-    z = dinv[:, None] * r
+    z = precond(r)
     p = z.copy()
     rho_old = np.sum(r * z, axis=0)
     ref = np.sum(b * b, axis=0)
@@ -343,7 +346,7 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
             a = np.where(pq != 0.0, rho_old / np.where(pq != 0, pq, 1.0), 0.0)
         r = np.where(pq != 0.0, r - a * q, 0.0)
         x = np.where(pq != 0.0, x + a * p, 0.0)
-        z = dinv[:, None] * r
+        z = precond(r)
         rho = np.sum(r * z, axis=0)
         if rule == "reference":
             with np.errstate(divide="ignore", invalid="ignore"):
@@ -360,6 +363,60 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         p = np.where(rho_old != 0.0, z + beta * p, 0.0)
         rho_old = rho
     return x, iters
+
+
+def rcb_order(xy, leaf=256):
+    """Node order of recursive coordinate bisection: a set is split at the median of its
+    longer coordinate extent (ties by node id), the first part holding a whole number of
+    ``leaf``-node leaves, until every leaf fits; consecutive ``leaf`` nodes of the order
+    are then compact aggregates (the product's device tiles are built the same way)."""
+    xy = np.asarray(xy, np.float64)
+
+    def split(idx):
+        if len(idx) <= leaf:
+            return [idx]
+        pts = xy[idx]
+        ax = 0 if np.ptp(pts[:, 0]) >= np.ptp(pts[:, 1]) else 1
+        o = idx[np.lexsort((idx, pts[:, ax]))]
+        left = (-(-len(idx) // leaf) // 2) * leaf
+        return split(o[:left]) + split(o[left:])
+
+    return np.concatenate(split(np.arange(xy.shape[0])))
+
+
+def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256):
+    """The product's two-level preconditioner (an extension; the reference has only the
+    diagonal one, preconditioner.cpp:59-76), on aggregates of ``tile`` consecutive rows:
+
+        M^-1 r = D^-1 r + W C E1^-1 C W^T r
+        D  = diag(A),  W = aggregate indicator (N x nt)
+        E1 = W^T A1 W  (+ n n^T with n = rows per aggregate when dc_free: the A + 1 1^T
+             operator of CG's dcFree solve), A1 = the system matrix at sigma = 1
+        C  = diag(c),  c_t = (sum_{i in t} S_ii / sum_{i in t} S1_ii)^(-1/2)
+
+    (S: the stiffness part, harmonic 0).  Returns (prec, Minv) with prec(r) -> z and Minv
+    the dense N x N M^-1 when N <= 2000 (else None); prec.einv holds C E1^-1 C."""
+    N = A.shape[0]
+    nt = -(-N // tile)
+    W = sp.csr_matrix((np.ones(N), (np.arange(N), np.arange(N) // tile)), shape=(N, nt))
+    E1 = (W.T @ A1 @ W).toarray()
+    n = np.asarray(W.sum(axis=0)).ravel()
+    if dc_free:
+        E1 = E1 + np.outer(n, n)
+    c = 1.0 / np.sqrt(np.asarray(W.T @ S_diag).ravel() / np.asarray(W.T @ S1_diag).ravel())
+    Einv = c[:, None] * np.linalg.inv(E1) * c[None, :]
+    dinv = 1.0 / A.diagonal()
+
+    def prec(r):
+        return dinv[:, None] * r + W @ (Einv @ (W.T @ r))
+
+    prec.einv = Einv
+
+    Minv = None
+    if N <= 2000:
+        Wd = W.toarray()
+        Minv = np.diag(dinv) + Wd @ Einv @ Wd.T
+    return prec, Minv
 
 
 def bicgstab_reference(A, b, dinv=None, max_iter=0, eps=FLT_EPSILON, rule="reference", tol=1e-6,

@@ -31,3 +31,18 @@ def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False, keep_node_o
                p["current"], 1.0, p["C"], p["height"], p["port_height"], quirks=quirks,
                jacobian_exact=jacobian_exact, frames=frames, ctx=ctx, keep_node_order=keep_node_order,
                contact_impedance=contact_impedance)
+
+
+def rcb_renumbered(p, leaf=256):
+    """The problem with its nodes renumbered by recursive coordinate bisection into
+    leaves of ``leaf`` consecutive nodes (compact aggregates for the two-level
+    preconditioner when the device keeps the caller's node order)."""
+    from oracle.eit_oracle import rcb_order
+    order = rcb_order(p["nodes"], leaf)
+    perm = np.empty_like(order)
+    perm[order] = np.arange(order.size)
+    q = dict(p)
+    q["nodes"] = p["nodes"][order]
+    q["elements"] = perm[p["elements"]].astype(p["elements"].dtype)
+    q["boundary"] = perm[p["boundary"]].astype(p["boundary"].dtype)
+    return q
