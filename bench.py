@@ -353,9 +353,9 @@ def run_c5(args):
 
 
 def run_probe():
-    """Counter probe for rocprofv3 --pmc passes: 32 PCG iterations of config 3 at frame 0's
-    phantom (all 5 harmonics active, so every launch covers 5 harmonic-SpMVs), then a plain
-    256 MiB device-to-device copy that calibrates FETCH_SIZE / WRITE_SIZE."""
+    """Counter probe for rocprofv3 --pmc passes: 32 two-level PCG iterations of config 3 at
+    frame 0's phantom (all 5 harmonics active, so every launch covers 5 harmonic-SpMVs),
+    then a plain 256 MiB device-to-device copy that calibrates FETCH_SIZE / WRITE_SIZE."""
     import torch
     from mpflow_amd import capi
     from mpflow_amd.eit import EIT, SolveOptions
@@ -364,7 +364,8 @@ def run_probe():
     nodes, el, bd, ports, pat = build_problem(w)
     eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"], jacobian_exact=True)
     g = torch.tensor(ellipse_phantom(nodes, el, frame_seed(3, 0)), dtype=torch.float32)
-    eit.solve(g, SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, max_iter=32, check_every=32))
+    eit.solve(g, SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, max_iter=32, check_every=32,
+                              preconditioner=capi.MPF_PRECOND_TWO_LEVEL))
     torch.cuda.synchronize()
     a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 256 MiB
     b = torch.empty_like(a)

@@ -59,7 +59,8 @@ struct JobDev {
     // 256-row tiles: M^-1 r = D^-1 r + W Einv W^T r with W the tile indicator,
     // Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W, sigma = 1, plus
     // n n^T on the dcFree slice; Sc: per-tile conductivity scale of this frame)
-    const float* cinv;           // [C][ntiles][ntiles]
+    const float* cinv;           // [C][ntiles][cld], rows zero-padded to cld = ceil4(ntiles)
+    int cld;
     float* csc;                  // [ntiles] Sc^-1/2 (written after each assembly)
     float* cy;                   // [C][ntiles][Rp] coarse correction Einv W^T r
 };
@@ -95,9 +96,10 @@ void launch_bicg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int
 // the solve's init) is used: it selects the rho slot.  `variant` picks the SpMV kernel
 // (0: 128-bit gathers, width-binned, for padded widths <= 24; 1: 32-bit gathers).
 constexpr int kSpmvVariants = 2;
-// With coarse work (ncoarse > 0: d_js lists the group's (job, slice) pairs and
-// `nchunk` the largest job's coarse row chunks) a fourth launch between update and
-// pupdate applies the two-level preconditioner's coarse correction.
+// d_js lists the group's (job, slice) pairs and `nchunk` the largest job's 32-tile
+// chunks: after the SpMV a small launch sums p.q and 1^T p per chunk, and between update
+// and pupdate one launch applies the two-level coarse correction (jobs that have one) and
+// sums r.z and r.r per chunk, so update and pupdate reduce chunk partials only.
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                      int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0 = nullptr,
                      hipEvent_t ev1 = nullptr, const int2* d_js = nullptr, int njs = 0, int nchunk = 0);

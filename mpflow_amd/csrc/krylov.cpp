@@ -75,10 +75,8 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
                 for (int t = 0; t < h[j].ntiles; ++t) lst.push_back(make_int4(j, c, t, 0));
             }
             rows += (long long)h[j].C * h[j].np;
-            if (h[j].cinv) {
-                g.nchunk = std::max(g.nchunk, (h[j].ntiles + kCoarseRows - 1) / kCoarseRows);
-                init_nchunk = std::max(init_nchunk, g.nchunk);
-            }
+            g.nchunk = std::max(g.nchunk, (h[j].ntiles + kCoarseRows - 1) / kCoarseRows);
+            if (h[j].cinv) init_nchunk = std::max(init_nchunk, g.nchunk);
         }
         g.njs = (int)gjs.size();
         g.js.alloc(g.njs);
@@ -133,7 +131,7 @@ void KrylovDriver::launch_group_iter(Group& g, int iter, bool bicg, hipStream_t 
         launch_bicg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, s, e0, e1);
     else
         launch_pcg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, g.variant, s, e0, e1, g.js.p, g.njs,
-                        coarse ? g.nchunk : 0);
+                        g.nchunk);
 }
 
 // Time each SpMV variant that supports the group's width on the live job data (the SpMV

@@ -119,6 +119,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.gw = gw.p;
         if (nct) {
             J.cinv = cinv.p;
+            J.cld = S.coarse_ld;
             J.csc = csc.p + (size_t)nct * f;
             J.cy = cy.p + (size_t)S.C * nct * S.Rp * f;
         }

@@ -102,9 +102,9 @@ struct MeshSetup {
     double alpha[kMaxSlices] = {}, beta[kMaxSlices] = {};
     double ref = 1.0, height = 1.0, port_height = 1.0;
     int logarithmic = 0, jacobian_exact = 0;
-    // two-level preconditioner: tile count (0: none), [C][nt][nt] inverses of the coarse
+    // two-level preconditioner: tile count (0: none), [C][nt][ld] inverses of the coarse
     // matrices at sigma = 1, per-tile sigma = 1 stiffness-diagonal sums
-    int coarse_nt = 0;
+    int coarse_nt = 0, coarse_ld = 0;   // tiles, padded row length of coarse_inv
     std::vector<float> coarse_inv;
     std::vector<double> coarse_d1;
 };

@@ -311,7 +311,8 @@ static void coarse_setup(MeshSetup& s) {
     s.coarse_d1.assign(nt, 0.0);
     for (int row = 0; row < s.N; ++row)
         s.coarse_d1[row / kTileRows] += sk[(size_t)s.diag_slot[row] * np + row];
-    s.coarse_inv.assign((size_t)s.C * nt * nt, 0.0f);
+    const int ld = (nt + 3) / 4 * 4;
+    s.coarse_inv.assign((size_t)s.C * nt * ld, 0.0f);
     std::vector<double> Ec((size_t)nt * nt);
     for (int c = 0; c < s.C; ++c) {
         std::fill(Ec.begin(), Ec.end(), 0.0);
@@ -328,9 +329,12 @@ static void coarse_setup(MeshSetup& s) {
         for (int t = 0; t < nt; ++t)
             if (cnt[t] == 0.0) Ec[(size_t)t * nt + t] = 1.0;
         if (!spd_inverse(nt, Ec.data())) return;
-        for (size_t i = 0; i < Ec.size(); ++i) s.coarse_inv[(size_t)c * nt * nt + i] = (float)Ec[i];
+        for (int t = 0; t < nt; ++t)
+            for (int u = 0; u < nt; ++u)
+                s.coarse_inv[((size_t)c * nt + t) * ld + u] = (float)Ec[(size_t)t * nt + u];
     }
     s.coarse_nt = nt;
+    s.coarse_ld = ld;
 }
 
 void build_setup(const mpf_model_desc& caller, MeshSetup& s) {

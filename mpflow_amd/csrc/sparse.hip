@@ -263,14 +263,16 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     __shared__ double sh_al[Rp], sh_dc[Rp];
     __shared__ int sh_zero[Rp];
     {
+        // p.q and 1^T p arrive as one partial per 32-tile chunk (k_pcg_chunks)
         double out[2] = {0.0, 0.0};
+        const int cnt = coarse_chunks(J);
         if (s == J.dc) {
-            const int ch[2] = {0, 1};
-            reduce_tiles<2>(J, s, ch, out);
+            const int ch[2] = {5, 6};
+            reduce_tiles<2>(J, s, ch, out, cnt);
         } else {
-            const int ch[1] = {0};
+            const int ch[1] = {5};
             double o[1];
-            reduce_tiles<1>(J, s, ch, o);
+            reduce_tiles<1>(J, s, ch, o, cnt);
             out[0] = o[0];
         }
         if ((int)threadIdx.x < Rp) {
@@ -331,50 +333,98 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     block_partials4<LG, 3>(J, s, t.z, ch, v);
 }
 
+template <int LG>
+constexpr int coarse_threads() { return (1 << LG) * kCoarseRows > 1024 ? 1024 : (1 << LG) * kCoarseRows; }
+
+// After the SpMV: the tile partials p.q (channel 0) and 1^T p (channel 1) summed per
+// 32-tile chunk in tile order into channels 5 and 6, so every update block reduces a
+// handful of chunk partials instead of every tile's.  One block per (chunk, job, slice).
+template <int LG>
+__global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_chunks(const JobDev* __restrict__ jobs,
+                                                                     const int2* __restrict__ js) {
+    constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp, PER = kCoarseRows / G;
+    const int2 pr = js[blockIdx.y];
+    const JobDev& J = jobs[pr.x];
+    const int s = pr.y;
+    if (J.Rp != Rp || !slice_active(J, s)) return;
+    const int nt = J.ntiles, t0 = blockIdx.x * kCoarseRows;
+    if (t0 >= nt) return;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int t = t0 + g * PER + k;
+        if (t < nt) {
+            a += part_ptr(J, 0, s, t)[lane];
+            b += part_ptr(J, 1, s, t)[lane];
+        }
+    }
+    __shared__ double red[2][NT];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    if (g == 0) {
+        double x = 0.0, y = 0.0;
+        for (int gg = 0; gg < G; ++gg) { x += red[0][gg * Rp + lane]; y += red[1][gg * Rp + lane]; }
+        part_ptr(J, 5, s, blockIdx.x)[lane] = x;
+        part_ptr(J, 6, s, blockIdx.x)[lane] = y;
+    }
+}
+
 // Coarse correction of the two-level preconditioner, one block per (row chunk, job,
 // slice): y_t = c_t sum_u cinv_s[t][u] c_u g_u with g = W^T r (channel 4 tile sums) and
 // c = Sc^-1/2, written to cy; then per chunk the partial sums of
 //   rho = sum_t (r.D^-1 r)_t + g_t y_t   (from channel ch_rz)  -> channel 5
 //   r.r = sum_t (r.r)_t                  (from channel ch_rr)  -> channel 6
-// so the next kernel reduces one partial per chunk instead of one per tile.
+// so the next kernel reduces one partial per chunk instead of one per tile.  A thread
+// owns (coarse row, lane): the scaled g of KU tiles at a time sits in LDS and each thread
+// streams its cinv row with 128-bit loads (rows padded to a multiple of 4 with zeros; the
+// lanes of a row read the same addresses).
 template <int LG>
-__global__ void __launch_bounds__(256) k_pcg_coarse(const JobDev* __restrict__ jobs, const int2* __restrict__ js,
-                                                    int ch_rz, int ch_rr, int in_iteration) {
-    constexpr int Rp = 1 << LG, G = 256 / Rp, PER = kCoarseRows / G, KU = 64;
+__global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDev* __restrict__ jobs,
+                                                                     const int2* __restrict__ js, int ch_rz,
+                                                                     int ch_rr, int in_iteration) {
+    constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp, PER = kCoarseRows / G, KU = 128;
     static_assert(kCoarseRows % G == 0, "coarse rows per block must split over the row groups");
     const int2 pr = js[blockIdx.y];
     const JobDev& J = jobs[pr.x];
     const int s = pr.y;
-    // (at the start the slice's state is not set yet: every listed slice runs)
-    if (!J.cinv || J.Rp != Rp || (in_iteration && !slice_active(J, s))) return;
-    const int nt = J.ntiles;
+    // (at the start the slice's state is not set yet: every listed slice runs; jobs
+    // without a coarse level only form the chunk sums)
+    if (J.Rp != Rp || (in_iteration && !slice_active(J, s)) || (!in_iteration && !J.cinv)) return;
+    const int nt = J.ntiles, ld = J.cld;
     const int t0 = blockIdx.x * kCoarseRows;
     if (t0 >= nt) return;
     const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
     __shared__ float gs[KU][Rp];
-    __shared__ float cs[kCoarseRows][KU + 1];
-    const MPF_GLOBAL float* cinv = gp(J.cinv) + (size_t)s * nt * nt;
     const MPF_GLOBAL float* csc = gp((const float*)J.csc);
+    const MPF_GLOBAL float4* crow[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int t = min(t0 + g + k * G, nt - 1);   // rows past nt compute a copy, discarded
+        crow[k] = J.cinv ? (const MPF_GLOBAL float4*)(gp(J.cinv) + ((size_t)s * nt + t) * ld) : nullptr;
+    }
     float acc[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
-    for (int u0 = 0; u0 < nt; u0 += KU) {
+    for (int u0 = 0; J.cinv && u0 < nt; u0 += KU) {
         const int un = min(KU, nt - u0);
         __syncthreads();
-        for (int i = threadIdx.x; i < KU * Rp; i += 256) {
+        for (int i = threadIdx.x; i < KU * Rp; i += NT) {
             const int u = i / Rp, l = i % Rp;
             gs[u][l] = u < un ? (float)((double)csc[u0 + u] * part_ptr(J, 4, s, u0 + u)[l]) : 0.0f;
         }
-        for (int i = threadIdx.x; i < kCoarseRows * KU; i += 256) {
-            const int r = i / KU, u = i % KU;
-            cs[r][u] = (t0 + r < nt && u < un) ? cinv[(size_t)(t0 + r) * nt + u0 + u] : 0.0f;
-        }
         __syncthreads();
+        const int un4 = (un + 3) / 4;                // cinv rows are zero-padded to ld
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int r = g + k * G;
             float a = 0.0f;
-            for (int u = 0; u < un; ++u) a += cs[r][u] * gs[u][lane];
+#pragma unroll 8
+            for (int u = 0; u < un4; ++u) {
+                const float4 c = crow[k][u0 / 4 + u];
+                a += c.x * gs[4 * u][lane] + c.y * gs[4 * u + 1][lane] + c.z * gs[4 * u + 2][lane] +
+                     c.w * gs[4 * u + 3][lane];
+            }
             acc[k] += a;
         }
     }
@@ -383,13 +433,16 @@ __global__ void __launch_bounds__(256) k_pcg_coarse(const JobDev* __restrict__ j
     for (int k = 0; k < PER; ++k) {
         const int t = t0 + g + k * G;
         if (t < nt) {
-            const float y = csc[t] * acc[k];
-            gp(J.cy)[((size_t)s * nt + t) * Rp + lane] = y;
-            rz += part_ptr(J, ch_rz, s, t)[lane] + part_ptr(J, 4, s, t)[lane] * (double)y;
+            rz += part_ptr(J, ch_rz, s, t)[lane];
             rr += part_ptr(J, ch_rr, s, t)[lane];
+            if (J.cinv) {
+                const float y = csc[t] * acc[k];
+                gp(J.cy)[((size_t)s * nt + t) * Rp + lane] = y;
+                rz += part_ptr(J, 4, s, t)[lane] * (double)y;
+            }
         }
     }
-    __shared__ double red[2][256];
+    __shared__ double red[2][NT];
     red[0][threadIdx.x] = rz;
     red[1][threadIdx.x] = rr;
     __syncthreads();
@@ -447,9 +500,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
     __shared__ int not_done;
     {
         double out[2] = {0.0, 0.0};
-        // two-level: rho and r.r come as one partial per coarse chunk (channels 5, 6)
-        const int cnt = J.cinv ? coarse_chunks(J) : -1;
-        const int c_rz = J.cinv ? 5 : 2, c_rr = J.cinv ? 6 : 3;
+        // rho and r.r come as one partial per 32-tile chunk (k_pcg_coarse, channels 5, 6)
+        const int cnt = coarse_chunks(J);
+        const int c_rz = 5, c_rr = 6;
         if (book) {
             const int ch[2] = {c_rz, c_rr};
             reduce_tiles<2>(J, s, ch, out, cnt);
@@ -676,11 +729,16 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
 template <int LG>
 static void coarse_launch(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, int ch_rz, int ch_rr,
                           hipStream_t st) {
-    k_pcg_coarse<LG><<<dim3(nchunk, njs), 256, 0, st>>>(d_jobs, d_js, ch_rz, ch_rr, ch_rz != 0);
+    k_pcg_coarse<LG><<<dim3(nchunk, njs), coarse_threads<LG>(), 0, st>>>(d_jobs, d_js, ch_rz, ch_rr, ch_rz != 0);
 }
 
 // every lane count present in the list runs its own instantiation; blocks whose job has
 // another lane count exit at once
+template <int LG>
+static void chunks_launch(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, hipStream_t st) {
+    k_pcg_chunks<LG><<<dim3(nchunk, njs), coarse_threads<LG>(), 0, st>>>(d_jobs, d_js);
+}
+
 static void launch_coarse_all(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, int lg, int ch_rz,
                               int ch_rr, hipStream_t st) {
     if (lg == 6 || lg < 0) coarse_launch<6>(d_jobs, d_js, njs, nchunk, ch_rz, ch_rr, st);
@@ -749,10 +807,15 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
     launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st);
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
+    MPF_REQUIRE(d_js && njs > 0 && nchunk > 0, "pcg iteration: no (job, slice) list");
+    if (lg == 6) chunks_launch<6>(d_jobs, d_js, njs, nchunk, st);
+    else if (lg == 5) chunks_launch<5>(d_jobs, d_js, njs, nchunk, st);
+    else chunks_launch<4>(d_jobs, d_js, njs, nchunk, st);
     if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
     else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
     else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
-    if (nchunk > 0 && njs > 0) launch_coarse_all(d_jobs, d_js, njs, nchunk, lg, 2, 3, st);
+    // coarse correction where the job has one; chunk sums of r.z and r.r for every job
+    launch_coarse_all(d_jobs, d_js, njs, nchunk, lg, 2, 3, st);
     if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
