@@ -629,7 +629,10 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
     double rr = 0.0, bb = 0.0;
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         double acc = 0.0;
-        for (int q = 0; q < J.width; ++q) {
+        // slots past the 16-row group's widest row are padding (value 0)
+        const int w = J.gw ? J.gw[row >> 4] : J.width;
+#pragma unroll 4
+        for (int q = 0; q < w; ++q) {
             const size_t e = (size_t)q * J.np + row;
             const double v = ((double)J.s[e] + (double)J.s_lo[e]) + al * ((double)J.r[e] + (double)J.r_lo[e]);
             acc += v * X[(size_t)J.ids[e] * J.Rp + L.lane];
@@ -656,13 +659,6 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64_reduce(const JobDev* __
     const int ch[2] = {0, 1};
     double out[2];
     reduce_tiles<2>(J, s, ch, out);
-    if (J.cinv) {
-        // rho = b.D^-1 b + (W^T b).y, summed per coarse chunk by the coarse kernel
-        const int chc[1] = {5};
-        double oc[1];
-        reduce_tiles<1>(J, s, chc, oc, coarse_chunks(J));
-        out[0] = oc[0];
-    }
     if ((int)threadIdx.x < J.Rp) {
         sc_ptr(J, SC_X1, s)[threadIdx.x] = out[0];
         sc_ptr(J, SC_X2, s)[threadIdx.x] = out[1];
