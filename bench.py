@@ -144,7 +144,9 @@ def run_c4(args):
         probs.append((j, w, nodes, el))
     batch = Batch(models)
     opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
-                        refine_steps=1, refine_tol=1e-2, graphs=1)
+                        refine_steps=1, refine_tol=1e-2, graphs=1,
+                        preconditioner=capi.MPF_PRECOND_TWO_LEVEL if args.precond == "two-level"
+                        else capi.MPF_PRECOND_JACOBI)
     gs = [torch.tensor(ellipse_phantom(n, e, frame_seed(4, j)), dtype=torch.float32) for j, _, n, e in probs]
     batch.solve(gs, opts)
     solvers = []
@@ -194,6 +196,7 @@ def run_c4(args):
             "elements_total": int(sum(m.E for m in models)),
             "nodes_total": int(sum(m.N for m in models)),
             "gn_steps_per_job": args.gn_steps,
+            "forward": "FP32 %s-PCG, HIP-graph replay, + 1 FP64 residual refinement" % args.precond,
             "slice_iterations_per_step": iters // max(args.steps, 1),
             "setup_s": round(t_setup, 1),
         },
