@@ -54,6 +54,9 @@ extern "C" {
 #define MPF_PRECOND_JACOBI    0
 #define MPF_PRECOND_TWO_LEVEL 1
 
+#define MPF_REG_IDENTITY 0
+#define MPF_REG_NOSER    1
+
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
 typedef struct mpf_batch mpf_batch;
@@ -252,6 +255,30 @@ int mpf_eit_measure(mpf_model* m, const float* d_phi, int32_t ldphi, int32_t add
  * (eit.cpp:202-205) is the caller's.  Synchronises. */
 int mpf_jacobian(mpf_model* m, const float* d_phi, int32_t ldphi, const float* d_gamma, int32_t additive,
                  float* d_J);
+
+/* Inverse::setRegularizationParameter's type (src/solver/inverse.cpp:83-135,
+ * src/solver/solver.cpp:116-132) for the model's later GN steps and differential
+ * preparations; the normal matrix is J^T J + (ref lambda)^2 R.
+ *   MPF_REG_IDENTITY : R = I (the default)
+ *   MPF_REG_NOSER    : R = diag(J0^T J0), J0 the Jacobian at d_gamma_ref (E floats;
+ *                      null: the reference distribution, 1, or 0 when logarithmic),
+ *                      computed here with one forward solve (opts) and kept, as the
+ *                      reference keeps the matrix it computed when the type was set.
+ *                      Entries below 1e-12 of the largest are raised to that floor.
+ * The reference's laplacian type is not supported (MPF_EINVAL).  Synchronises. */
+int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_ref,
+                              const mpf_solve_opts* opts, mpf_solve_stats* stats);
+
+/* Solver::preSolve + Solver::solveDifferential (src/solver/solver.cpp:136-188).
+ * mpf_diff_prepare: forward solve, Jacobian and the factored data-space normal matrix at
+ * d_gamma_ref (E floats), once.  mpf_diff_solve: for every frame's measurement
+ *   d_Umeas : frames x m (FP64, EIT::result layout)
+ *   d_gamma : frames x E (out) = gamma_ref + (J^T J + (ref lambda)^2 R)^-1 J^T (U_meas - U(gamma_ref))
+ * with gn_refine FP64-residual passes: no forward solve, one triangular solve pair and
+ * three GEMVs per frame and pass.  Both synchronise. */
+int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, const mpf_solve_opts* opts,
+                     mpf_solve_stats* stats);
+int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float* d_gamma);
 
 /* ---------------------------------------------------------------------------
  * Batches: several models (meshes of any size, any electrode count and harmonic count)

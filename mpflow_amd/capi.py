@@ -12,6 +12,7 @@ MPF_STOP_REFERENCE, MPF_STOP_TOLERANCE = 0, 1
 MPF_PREC_FP32, MPF_PREC_MIXED = 0, 1
 MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB = 0, 1
 MPF_PRECOND_JACOBI, MPF_PRECOND_TWO_LEVEL = 0, 1
+MPF_REG_IDENTITY, MPF_REG_NOSER = 0, 1
 
 EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",
@@ -21,7 +22,8 @@ EXPORTED = [
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
-    "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian",
+    "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian", "mpf_gn_set_regularization", "mpf_diff_prepare",
+    "mpf_diff_solve",
 ]
 
 
@@ -110,6 +112,9 @@ def lib() -> C.CDLL:
     L.mpf_fem_assemble.argtypes = [vp, vp, dbl, vp, vp]
     L.mpf_eit_measure.argtypes = [vp, vp, i32, i32, vp]
     L.mpf_jacobian.argtypes = [vp, vp, i32, vp, i32, vp]
+    L.mpf_gn_set_regularization.argtypes = [vp, i32, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.mpf_diff_prepare.argtypes = [vp, vp, dbl, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.mpf_diff_solve.argtypes = [vp, vp, i32, vp]
     _lib = L
     return L
 

@@ -222,6 +222,44 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
     });
 }
 
+int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_ref, const mpf_solve_opts* opts,
+                              mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(m, "mpf_gn_set_regularization: null model");
+        MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER,
+                    "mpf_gn_set_regularization: only identity and noser are supported");
+        set_device(m->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        m->impl->set_regularization(type, d_gamma_ref, o, stats);
+        return MPF_OK;
+    });
+}
+
+int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, const mpf_solve_opts* opts,
+                     mpf_solve_stats* stats) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_gamma_ref, "mpf_diff_prepare: null argument");
+        set_device(m->impl->ctx);
+        mpf_solve_opts o;
+        if (opts) o = *opts; else mpf_solve_opts_default(&o);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        m->impl->diff_prepare(d_gamma_ref, lambda, o, stats);
+        return MPF_OK;
+    });
+}
+
+int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float* d_gamma) {
+    return guarded([&] {
+        MPF_REQUIRE(m && d_Umeas && d_gamma, "mpf_diff_solve: null argument");
+        MPF_REQUIRE(gn_refine >= 0, "mpf_diff_solve: gn_refine < 0");
+        set_device(m->impl->ctx);
+        m->impl->diff_solve(d_Umeas, gn_refine, d_gamma);
+        return MPF_OK;
+    });
+}
+
 int mpf_fem_assemble(mpf_model* m, const float* d_gamma, double alpha, uint32_t* d_column_ids, float* d_values) {
     return guarded([&] {
         MPF_REQUIRE(m && d_gamma && d_values, "mpf_fem_assemble: null argument");
@@ -359,8 +397,14 @@ int mpf_gram_bf16(mpf_ctx* ctx, int32_t trans, int32_t rows, int32_t k, const fl
         MPF_REQUIRE(rows > 0 && k > 0 && lda >= rows, "mpf_gram_bf16: bad shape");
         set_device(ctx);
         const int n = trans ? k : rows, nk = trans ? rows : k;
-        mpf::DBuf<char> work(mpf::gram_workspace_bytes(n, nk));
-        mpf::launch_gram(d_A, trans, n, nk, lda, d_G, work.p, ctx->stream);
+        if (trans == 0 && mpf::gram_nt_applies(n, lda)) {
+            mpf::DBuf<char> work(mpf::gram_nt_workspace_bytes(n, nk, 1));
+            mpf::launch_gram_nt(d_A, 0, n, nk, lda, d_G, 0, 1, work.p, ctx->stream);
+            MPF_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+            mpf::DBuf<char> work(mpf::gram_workspace_bytes(n, nk));
+            mpf::launch_gram(d_A, trans, n, nk, lda, d_G, work.p, ctx->stream);
+        }
         MPF_HIP(hipStreamSynchronize(ctx->stream));
         return MPF_OK;
     });

@@ -29,6 +29,12 @@ void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use
 void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use64, float* J,
                      hipStream_t st);
 
+// G_b = A_b A_b^T for `batch` column-major A_b (n x nk, lda; n, lda multiples of 4) on
+// MFMA with 128 x 128 tiles; G_b full (both triangles), column-major n x n
+bool gram_nt_applies(int n, int lda);
+size_t gram_nt_workspace_bytes(int n, int nk, int batch);
+void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG, int batch,
+                    void* work, hipStream_t st);
 int gram_kslab(int n, int nk);
 size_t gram_workspace_bytes(int n, int nk);
 void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
@@ -51,5 +57,16 @@ void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_
 void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t st);
 void launch_add_to(int n, double* y, const double* dy, hipStream_t st);
 void launch_gamma_update(int n, float* g, const double* d, hipStream_t st);
+// out[b][i] = a[b][i] - u[i]
+void launch_sub_bcast(int n, int batch, const double* a, const double* u, double* out, hipStream_t st);
+// d[e] = sum_i A[i + e lda]^2
+void launch_col_sq(int rows, int cols, const float* A, int lda, double* d, hipStream_t st);
+// A_b[:, e] *= w[e]
+void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, int batch, const float* w,
+                       hipStream_t st);
+// v[b][e] *= w[e]
+void launch_scale_vec(int n, int batch, double* v, const float* w, hipStream_t st);
+// g[b][e] = g0[e] + d[b][e]
+void launch_gamma_from(int n, int batch, const float* g0, const double* d, float* g, hipStream_t st);
 
 }  // namespace mpf

@@ -2,6 +2,7 @@
 //   electrode voltages, Jacobian assembly from drive/measurement fields, the Gram
 //   matrix of the Jacobian on MFMA (BF16 in, FP32 accumulate), FP64 matrix-vector
 //   products and a blocked FP64 Cholesky for the Tikhonov normal equations.
+#include <algorithm>
 #include <cmath>
 
 #include "dense.h"
@@ -166,6 +167,136 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const float* __restrict__ A, 
     }
 }
 
+// G = A A^T for a column-major A (nrow x nk, lda; nrow and lda multiples of 4): the
+// Gauss-Newton data-space Gram J J^T.  Block = 4 waves = one 128 x 128 tile (I <= J) of
+// G over one contraction slab; wave w owns the 64 x 64 quarter (w/2, w%2) as 2 x 2
+// v_mfma_f32_32x32x16_bf16 tiles.  Each 32-deep stage of both operand panels is read
+// with 128-bit loads (a column of A holds consecutive rows), converted to BF16 and stored
+// [row][k] in LDS; the next stage's loads are issued before the current stage's MFMAs
+// (two LDS buffers).  Partial tiles go to part[batch][slab][tile]; k_gram_reduce128 sums
+// them in slab order.  The (I, J) pair of a block is derived from its index.
+constexpr int kGT2 = 128;
+constexpr int kGK2 = 32;
+constexpr int kGL2 = kGK2 + 8;   // LDS row stride (BF16): 16-byte aligned 8-element reads
+
+__device__ __forceinline__ int2 upper_tile(int b, int nt) {
+    int i = 0;
+    while (b >= nt - i) { b -= nt - i; ++i; }
+    return make_int2(i, i + b);
+}
+
+__global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, size_t strideA, int nrow, int nk,
+                                                 int lda, int nt, int kslab, float* __restrict__ part) {
+    const int ntiles = nt * (nt + 1) / 2;
+    const int2 tile = upper_tile(blockIdx.x, nt);
+    const int I0 = tile.x * kGT2, J0 = tile.y * kGT2;
+    const int k0 = blockIdx.z * kslab, k1 = min(nk, k0 + kslab);
+    const float* Ab = A + blockIdx.y * strideA;
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][kGT2][kGL2];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][kGT2][kGL2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    // stage loads: each thread owns a 4-row x 4-k block of each operand panel (row quad
+    // tid & 31, k quad tid >> 5): four 128-bit loads (one per k, consecutive rows), then
+    // per row the 4 k values packed into one 64-bit LDS write
+    float4 ra[4], rb[4];
+    const int rq = (tid & 31) * 4, kq4 = (tid >> 5) * 4;
+    auto load = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kg = kb + kq4 + u;
+            const bool kin = kg < k1;
+            ra[u] = (kin && I0 + rq < nrow) ? *reinterpret_cast<const float4*>(Ab + (size_t)kg * lda + I0 + rq)
+                                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            rb[u] = (kin && J0 + rq < nrow) ? *reinterpret_cast<const float4*>(Ab + (size_t)kg * lda + J0 + rq)
+                                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+    auto store = [&](int buf) {
+        bf16x4_t a0 = {(__bf16)ra[0].x, (__bf16)ra[1].x, (__bf16)ra[2].x, (__bf16)ra[3].x};
+        bf16x4_t a1 = {(__bf16)ra[0].y, (__bf16)ra[1].y, (__bf16)ra[2].y, (__bf16)ra[3].y};
+        bf16x4_t a2 = {(__bf16)ra[0].z, (__bf16)ra[1].z, (__bf16)ra[2].z, (__bf16)ra[3].z};
+        bf16x4_t a3 = {(__bf16)ra[0].w, (__bf16)ra[1].w, (__bf16)ra[2].w, (__bf16)ra[3].w};
+        bf16x4_t b0 = {(__bf16)rb[0].x, (__bf16)rb[1].x, (__bf16)rb[2].x, (__bf16)rb[3].x};
+        bf16x4_t b1 = {(__bf16)rb[0].y, (__bf16)rb[1].y, (__bf16)rb[2].y, (__bf16)rb[3].y};
+        bf16x4_t b2 = {(__bf16)rb[0].z, (__bf16)rb[1].z, (__bf16)rb[2].z, (__bf16)rb[3].z};
+        bf16x4_t b3 = {(__bf16)rb[0].w, (__bf16)rb[1].w, (__bf16)rb[2].w, (__bf16)rb[3].w};
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq][kq4]) = a0;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 1][kq4]) = a1;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 2][kq4]) = a2;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 3][kq4]) = a3;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq][kq4]) = b0;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 1][kq4]) = b1;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 2][kq4]) = b2;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 3][kq4]) = b3;
+    };
+    const int nst = (k1 - k0 + kGK2 - 1) / kGK2;
+    if (nst > 0) {
+        load(k0);
+        store(0);
+    }
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) load(k0 + (st + 1) * kGK2);
+#pragma unroll
+        for (int ks = 0; ks < kGK2; ks += 16) {
+            const int kq = ks + (lane >> 5) * 8;
+            bf16x8_t av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][wi + 32 * a + (lane & 31)][kq]);
+                bv[a] = *reinterpret_cast<const bf16x8_t*>(&sb[buf][wj + 32 * a + (lane & 31)][kq]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        if (st + 1 < nst) store(buf ^ 1);
+        __syncthreads();
+    }
+    // C layout: lane -> column (lane & 31); register v -> row 8*(v/4) + 4*(lane/32) + v%4
+    float* out = part + (((size_t)blockIdx.y * gridDim.z + blockIdx.z) * ntiles + blockIdx.x) * kGT2 * kGT2;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = wi + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int col = wj + 32 * b + (lane & 31);
+                out[row * kGT2 + col] = acc[a][b][v];
+            }
+}
+
+__global__ void __launch_bounds__(256) k_gram_reduce128(const float* __restrict__ part, int nt, int nslab, int n,
+                                                        float* __restrict__ G, size_t strideG) {
+    const int ntiles = nt * (nt + 1) / 2;
+    const int2 tile = upper_tile(blockIdx.x, nt);
+    const float* pb = part + (size_t)blockIdx.z * nslab * ntiles * kGT2 * kGT2;
+    float* Gb = G + blockIdx.z * strideG;
+    for (int i = blockIdx.y * 256 + threadIdx.x; i < kGT2 * kGT2; i += 256 * gridDim.y) {
+        float s = 0.0f;
+        for (int z = 0; z < nslab; ++z) s += pb[((size_t)z * ntiles + blockIdx.x) * kGT2 * kGT2 + i];
+        const int r = tile.x * kGT2 + i / kGT2, c = tile.y * kGT2 + i % kGT2;
+        // on diagonal tiles the upper entry of each pair is written to both places
+        if (r < n && c < n && (tile.x != tile.y || r <= c)) {
+            Gb[r + (size_t)c * n] = s;
+            Gb[c + (size_t)r * n] = s;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gram_reduce(const float* __restrict__ part, const int2* __restrict__ tiles,
                                                      int ntiles, int nslab, int n, float* __restrict__ G) {
     const int b = blockIdx.x;
@@ -174,7 +305,7 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const float* __restrict__ p
         float s = 0.0f;
         for (int z = 0; z < nslab; ++z) s += part[((size_t)z * ntiles + b) * kGT * kGT + i];
         const int r = tile.x * kGT + i / kGT, c = tile.y * kGT + i % kGT;
-        if (r < n && c < n) {
+        if (r < n && c < n && (tile.x != tile.y || r <= c)) {
             G[r + (size_t)c * n] = s;
             G[c + (size_t)r * n] = s;
         }
@@ -428,6 +559,49 @@ __global__ void __launch_bounds__(256) k_gamma_update(int n, float* __restrict__
     if (i < (size_t)n) g[i] = (float)((double)g[i] + d[i]);
 }
 
+// out[b][i] = a[b][i] - u[i]  (every frame against one reference voltage vector)
+__global__ void __launch_bounds__(256) k_sub_bcast(int n, int batch, const double* __restrict__ a,
+                                                   const double* __restrict__ u, double* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n * batch) out[i] = a[i] - u[i % n];
+}
+
+// d_e = sum_i A[i + e*lda]^2 (column squares, FP64), one wavefront per column
+__global__ void __launch_bounds__(256) k_col_sq(int rows, int cols, const float* __restrict__ A, int lda,
+                                                double* __restrict__ d) {
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (col >= cols) return;
+    const float* a = A + (size_t)col * lda;
+    double s = 0.0;
+    for (int r = lane; r < rows; r += 64) s += (double)a[r] * (double)a[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) d[col] = s;
+}
+
+// A_b[i + e*lda] *= w[e] for every batch entry
+__global__ void __launch_bounds__(256) k_scale_cols(int rows, int cols, float* __restrict__ A, int lda,
+                                                    size_t strideA, const float* __restrict__ w) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    const size_t e = i / rows, r = i % rows;
+    A[blockIdx.y * strideA + r + e * lda] *= w[e];
+}
+
+// v[b][e] *= w[e]
+__global__ void __launch_bounds__(256) k_scale_vec(int n, int batch, double* __restrict__ v,
+                                                   const float* __restrict__ w) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n * batch) v[i] *= (double)w[i % n];
+}
+
+// g[b][e] = g0[e] + d[b][e]
+__global__ void __launch_bounds__(256) k_gamma_from(int n, int batch, const float* __restrict__ g0,
+                                                    const double* __restrict__ d, float* __restrict__ g) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (size_t)n * batch) g[i] = (float)((double)g0[i % n] + d[i]);
+}
+
 // --------------------------------------------------------------------------------
 // launch wrappers
 // --------------------------------------------------------------------------------
@@ -468,6 +642,38 @@ int gram_kslab(int n, int nk) {
     kslab = ((kslab + kGK - 1) / kGK) * kGK;
     if (kslab < 256) kslab = 256;
     return kslab;
+}
+
+// slabs of the 128-tile kernel: about 512 blocks per launch over all batch entries
+static int gram_nt_slabs(int n, int nk, int batch) {
+    const int nt = (n + kGT2 - 1) / kGT2;
+    const int ntiles = nt * (nt + 1) / 2;
+    int nslab = std::max(1, (512 + ntiles * batch - 1) / (ntiles * batch));
+    nslab = std::min(nslab, std::max(1, (nk + 255) / 256));
+    return nslab;
+}
+
+bool gram_nt_applies(int n, int lda) { return n % 4 == 0 && lda % 4 == 0; }
+
+size_t gram_nt_workspace_bytes(int n, int nk, int batch) {
+    const int nt = (n + kGT2 - 1) / kGT2;
+    const int ntiles = nt * (nt + 1) / 2;
+    return (size_t)batch * gram_nt_slabs(n, nk, batch) * ntiles * kGT2 * kGT2 * sizeof(float);
+}
+
+void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG, int batch,
+                    void* work, hipStream_t st) {
+    MPF_REQUIRE(gram_nt_applies(n, lda), "gram: rows and leading dimension must be multiples of 4");
+    const int nt = (n + kGT2 - 1) / kGT2;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int nslab = gram_nt_slabs(n, nk, batch);
+    int kslab = (nk + nslab - 1) / nslab;
+    kslab = (kslab + kGK2 - 1) / kGK2 * kGK2;
+    const int used = (nk + kslab - 1) / kslab;
+    float* part = (float*)work;
+    k_gram_nt<<<dim3(ntiles, batch, used), 256, 0, st>>>(A, strideA, n, nk, lda, nt, kslab, part);
+    k_gram_reduce128<<<dim3(ntiles, 16, batch), 256, 0, st>>>(part, nt, used, n, G, strideG);
+    check_launch();
 }
 
 void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
@@ -564,6 +770,33 @@ void launch_add_to(int n, double* y, const double* dy, hipStream_t st) {
 
 void launch_gamma_update(int n, float* g, const double* d, hipStream_t st) {
     k_gamma_update<<<(n + 255) / 256, 256, 0, st>>>(n, g, d);
+    check_launch();
+}
+
+void launch_sub_bcast(int n, int batch, const double* a, const double* u, double* out, hipStream_t st) {
+    k_sub_bcast<<<(unsigned)(((size_t)n * batch + 255) / 256), 256, 0, st>>>(n, batch, a, u, out);
+    check_launch();
+}
+
+void launch_col_sq(int rows, int cols, const float* A, int lda, double* d, hipStream_t st) {
+    k_col_sq<<<(cols + 3) / 4, 256, 0, st>>>(rows, cols, A, lda, d);
+    check_launch();
+}
+
+void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, int batch, const float* w,
+                       hipStream_t st) {
+    dim3 grid((unsigned)(((size_t)rows * cols + 255) / 256), batch);
+    k_scale_cols<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, w);
+    check_launch();
+}
+
+void launch_scale_vec(int n, int batch, double* v, const float* w, hipStream_t st) {
+    k_scale_vec<<<(unsigned)(((size_t)n * batch + 255) / 256), 256, 0, st>>>(n, batch, v, w);
+    check_launch();
+}
+
+void launch_gamma_from(int n, int batch, const float* g0, const double* d, float* g, hipStream_t st) {
+    k_gamma_from<<<(unsigned)(((size_t)n * batch + 255) / 256), 256, 0, st>>>(n, batch, g0, d, g);
     check_launch();
 }
 

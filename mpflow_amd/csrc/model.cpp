@@ -294,7 +294,7 @@ void Model::ensure_gn() {
     rgn.alloc((size_t)frames * m);
     vE.alloc((size_t)frames * S.E);
     work_n.alloc(gemv_n_workspace(m, S.E, frames) / sizeof(double) + 1);
-    gram_work.alloc(gram_workspace_bytes(m, S.E));
+    gram_work.alloc(std::max(gram_workspace_bytes(m, S.E), gram_nt_workspace_bytes(m, S.E, frames)));
     pairs.alloc(cholesky_pairs(m));
     info.alloc(1);
 }
@@ -322,28 +322,128 @@ void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int g
         MPF_HIP(hipMemcpyAsync(d_Ucalc, U64.p, sizeof(double) * (size_t)m * frames, hipMemcpyDeviceToDevice, st0));
     // r = U_meas - U(gamma)  (Inverse::calcExcitation, inverse.cpp:163-193)
     launch_sub(m * frames, d_Umeas, U64.p, rgn.p, st0);
+    // noser: (J^T J + lam^2 W^-2) delta = J^T r is solved as J W = Js, delta = W Js^T y
+    if (reg_type == MPF_REG_NOSER) launch_scale_cols(m, E, Jm.p, m, (size_t)m * E, frames, reg_w.p, st0);
     // J J^T on MFMA, then (J J^T + lam I) by FP64 Cholesky
-    for (int f = 0; f < frames; ++f)
-        launch_gram(Jm.p + (size_t)f * m * E, 0, m, E, m, G.p + (size_t)f * m * m, gram_work.p, st0);
+    if (gram_nt_applies(m, m))
+        launch_gram_nt(Jm.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
+    else
+        for (int f = 0; f < frames; ++f)
+            launch_gram(Jm.p + (size_t)f * m * E, 0, m, E, m, G.p + (size_t)f * m * m, gram_work.p, st0);
     const double lam2 = (S.ref * lambda) * (S.ref * lambda);
     MPF_HIP(hipMemsetAsync(info.p, 0, sizeof(int), st0));
     launch_cholesky(m, G.p, (size_t)m * m, lam2, A64.p, (size_t)m * m, frames, info.p, pairs.p, st0);
-    launch_potrs(m, A64.p, (size_t)m * m, rgn.p, m, y.p, m, frames, st0);
-    // FP64 refinement of (J J^T + lam I) y = r with residuals from the FP32 Jacobian
-    for (int k = 0; k < gn_refine; ++k) {
-        launch_gemv_t(m, E, Jm.p, m, (size_t)m * E, y.p, m, vE.p, E, frames, st0);
-        launch_gemv_n(m, E, Jm.p, m, (size_t)m * E, vE.p, E, lam2, y.p, m, rgn.p, m, res.p, m, frames,
-                      work_n.p, st0);
-        launch_potrs(m, A64.p, (size_t)m * m, res.p, m, dy.p, m, frames, st0);
-        launch_add_to(m * frames, y.p, dy.p, st0);
-    }
+    normal_solve(Jm.p, (size_t)m * E, A64.p, (size_t)m * m, lam2, frames, gn_refine);
     // delta = J^T y ; gamma += delta   (solver.cpp:222)
     launch_gemv_t(m, E, Jm.p, m, (size_t)m * E, y.p, m, vE.p, E, frames, st0);
+    if (reg_type == MPF_REG_NOSER) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
     launch_gamma_update(E * frames, d_gamma, vE.p, st0);
     int hinfo = 0;
     MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
     MPF_HIP(hipStreamSynchronize(st0));
     if (hinfo) throw Error(MPF_EINVAL, "gn_step: normal-equation matrix is not positive definite");
+}
+
+void Model::normal_solve(const float* Jp, size_t strideJ, const double* A, size_t strideA, double lam2, int nb,
+                         int gn_refine) {
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
+    launch_potrs(m, A, strideA, rgn.p, m, y.p, m, nb, st0);
+    // FP64 refinement of (J J^T + lam I) y = r with residuals from the FP32 Jacobian
+    for (int k = 0; k < gn_refine; ++k) {
+        launch_gemv_t(m, E, Jp, m, strideJ, y.p, m, vE.p, E, nb, st0);
+        launch_gemv_n(m, E, Jp, m, strideJ, vE.p, E, lam2, y.p, m, rgn.p, m, res.p, m, nb, work_n.p, st0);
+        launch_potrs(m, A, strideA, res.p, m, dy.p, m, nb, st0);
+        launch_add_to(m * nb, y.p, dy.p, st0);
+    }
+}
+
+int Model::forward_at(const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats) {
+    ensure_gn();
+    for (int f = 0; f < frames; ++f)
+        MPF_HIP(hipMemcpyAsync(gamma.p + (size_t)f * S.E, d_gamma_ref, sizeof(float) * S.E, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    return forward(o, nullptr, U64.p, Jm.p, stats);
+}
+
+void Model::set_regularization(int type, const float* d_gamma_ref, const mpf_solve_opts& o,
+                               mpf_solve_stats* stats) {
+    MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER, "regularization: unknown type");
+    reg_type = MPF_REG_IDENTITY;
+    diff_ready = false;
+    if (type == MPF_REG_IDENTITY) return;
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
+    std::vector<float> g1;
+    DBuf<float> gref;
+    if (!d_gamma_ref) {
+        // the Solver's reference distribution: 1 (linear) or 0 (logarithmic), solver.cpp:55-62
+        g1.assign(E, S.logarithmic ? 0.0f : 1.0f);
+        gref.alloc(E);
+        gref.upload(g1.data(), E, st0);
+        d_gamma_ref = gref.p;
+    }
+    const int rc = forward_at(d_gamma_ref, o, stats);
+    MPF_REQUIRE(rc == MPF_OK, "regularization: the reference forward solve did not converge");
+    DBuf<double> d(E);
+    launch_col_sq(m, E, Jm.p, m, d.p, st0);
+    std::vector<double> hd(E);
+    MPF_HIP(hipMemcpyAsync(hd.data(), d.p, sizeof(double) * E, hipMemcpyDeviceToHost, st0));
+    MPF_HIP(hipStreamSynchronize(st0));
+    double dmax = 0.0;
+    for (double v : hd) dmax = std::max(dmax, v);
+    MPF_REQUIRE(dmax > 0.0, "regularization: zero Jacobian at the reference gamma");
+    // an element without sensitivity would make R singular; its weight is capped
+    const double floor = 1e-12 * dmax;
+    std::vector<float> w(E);
+    for (int e = 0; e < E; ++e) w[e] = (float)(1.0 / std::sqrt(std::max(hd[e], floor)));
+    if (!reg_w.p) reg_w.alloc(E);
+    reg_w.upload(w.data(), E, st0);
+    MPF_HIP(hipStreamSynchronize(st0));
+    reg_type = type;
+}
+
+void Model::diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solve_opts& o, mpf_solve_stats* stats) {
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
+    diff_ready = false;
+    const int rc = forward_at(d_gamma_ref, o, stats);
+    MPF_REQUIRE(rc == MPF_OK, "differential: the reference forward solve did not converge");
+    if (!diff_J.p) {
+        diff_J.alloc((size_t)m * E);
+        diff_A.alloc((size_t)m * m);
+        diff_U.alloc(m);
+        diff_gamma.alloc(E);
+    }
+    // frame 0's Jacobian (every frame was solved at the same gamma)
+    MPF_HIP(hipMemcpyAsync(diff_J.p, Jm.p, sizeof(float) * (size_t)m * E, hipMemcpyDeviceToDevice, st0));
+    MPF_HIP(hipMemcpyAsync(diff_U.p, U64.p, sizeof(double) * m, hipMemcpyDeviceToDevice, st0));
+    MPF_HIP(hipMemcpyAsync(diff_gamma.p, d_gamma_ref, sizeof(float) * E, hipMemcpyDeviceToDevice, st0));
+    if (reg_type == MPF_REG_NOSER) launch_scale_cols(m, E, diff_J.p, m, 0, 1, reg_w.p, st0);
+    if (gram_nt_applies(m, m))
+        launch_gram_nt(diff_J.p, 0, m, E, m, G.p, 0, 1, gram_work.p, st0);
+    else
+        launch_gram(diff_J.p, 0, m, E, m, G.p, gram_work.p, st0);
+    diff_lam2 = (S.ref * lambda) * (S.ref * lambda);
+    MPF_HIP(hipMemsetAsync(info.p, 0, sizeof(int), st0));
+    launch_cholesky(m, G.p, 0, diff_lam2, diff_A.p, 0, 1, info.p, pairs.p, st0);
+    int hinfo = 0;
+    MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
+    MPF_HIP(hipStreamSynchronize(st0));
+    if (hinfo) throw Error(MPF_EINVAL, "differential: normal-equation matrix is not positive definite");
+    diff_ready = true;
+}
+
+void Model::diff_solve(const double* d_Umeas, int gn_refine, float* d_gamma_out) {
+    MPF_REQUIRE(diff_ready, "differential: call the preparation first");
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
+    launch_sub_bcast(m, frames, d_Umeas, diff_U.p, rgn.p, st0);
+    normal_solve(diff_J.p, 0, diff_A.p, 0, diff_lam2, frames, gn_refine);
+    launch_gemv_t(m, E, diff_J.p, m, 0, y.p, m, vE.p, E, frames, st0);
+    if (reg_type == MPF_REG_NOSER) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
+    launch_gamma_from(E, frames, diff_gamma.p, vE.p, d_gamma_out, st0);
+    MPF_HIP(hipStreamSynchronize(st0));
 }
 
 Batch::Batch(mpf_ctx* c, std::vector<Model*> ms) : ctx(c), models(std::move(ms)) {

@@ -73,10 +73,30 @@ struct Model {
     // Gauss-Newton step in two phases around the forward solve (batched GN)
     void gn_begin(const float* d_gamma);
     void gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine, double* d_Ucalc);
+    // regularisation of the GN normal equations (Inverse::calcRegularizationMatrix,
+    // inverse.cpp:83-135): identity, or noser R = diag(J0^T J0) with J0 the Jacobian at a
+    // reference gamma, kept as per-element weights w = R^-1/2
+    int reg_type = MPF_REG_IDENTITY;
+    DBuf<float> reg_w;
+    void set_regularization(int type, const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats);
+    // differential solves (Solver::preSolve + solveDifferential, solver.cpp:136-188): U,
+    // J and the factored normal matrix at a reference gamma, reused by every later call
+    bool diff_ready = false;
+    double diff_lam2 = 0.0;
+    DBuf<float> diff_gamma, diff_J;
+    DBuf<double> diff_U, diff_A;
+    void diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solve_opts& o, mpf_solve_stats* stats);
+    void diff_solve(const double* d_Umeas, int gn_refine, float* d_gamma_out);
 
   private:
     void ensure_bicg();
     void ensure_gn();
+    // frames' gamma <- one reference gamma; forward solve with Jacobian into U64 / Jm
+    int forward_at(const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats);
+    // (J J^T + lam2 I) y = r for `nb` right-hand sides against one factored matrix A
+    // (stride 0 when shared), with gn_refine FP64-residual passes using J
+    void normal_solve(const float* J, size_t strideJ, const double* A, size_t strideA, double lam2, int nb,
+                      int gn_refine);
     KrylovDriver& driver();
     std::unique_ptr<KrylovDriver> drv;
 };

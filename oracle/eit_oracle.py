@@ -652,6 +652,18 @@ def gn_step_normal(J, r, lam_eff2):
     return sla.cho_solve(sla.cho_factor(A), J.T @ r)
 
 
+def noser_diag(J0):
+    """RegularizationType::noser: R = diag(J0^T J0) (inverse.cpp:93-95)."""
+    return np.einsum("ie,ie->e", J0, J0)
+
+
+def gn_step_regularized(J, r, lam_eff2, R_diag):
+    """delta from the E x E system (J^T J + lam R) delta = J^T r with a diagonal R by dense
+    Cholesky (Inverse::updateJacobian's system matrix, inverse.cpp:60-76; small E only)."""
+    A = J.T @ J + lam_eff2 * np.diag(R_diag)
+    return sla.cho_solve(sla.cho_factor(A), J.T @ r)
+
+
 def gn_step_cg_reference(J, r, lam_eff2, max_iter=0, eps=FLT_EPSILON):
     """Inverse::solve as written: A = J^T J + (sigma_ref lambda)^2 I, b = J^T r,
     unpreconditioned CG from 0 with the column-0 stop rule (inverse.cpp:212-238)."""

@@ -19,7 +19,8 @@ def _bf16(a):
     return torch.tensor(a, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
 
 
-@pytest.mark.parametrize("trans,rows,k", [(0, 32, 16), (0, 256, 1000), (0, 100, 333), (1, 200, 130)])
+@pytest.mark.parametrize("trans,rows,k", [(0, 32, 16), (0, 256, 1000), (0, 100, 333), (0, 102, 77),
+                                          (0, 1024, 4003), (1, 200, 130)])
 def test_gram_bf16_mfma(gpu_ctx, trans, rows, k):
     """MFMA lane layout and tiling: G equals the FP64 Gram of the BF16-rounded operand."""
     import torch
@@ -36,6 +37,7 @@ def test_gram_bf16_mfma(gpu_ctx, trans, rows, k):
     Gh = G.cpu().numpy().astype(np.float64)        # column-major n x n, symmetric
     err = np.max(np.abs(Gh - ref)) / np.max(np.abs(ref))
     assert err < 1e-5, err
+    assert np.array_equal(Gh, Gh.T)
 
 
 def test_spd_solve(gpu_ctx):

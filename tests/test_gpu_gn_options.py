@@ -1,0 +1,89 @@
+"""Gauss-Newton options through the C ABI against the FP64 oracle: noser regularisation
+(Inverse::calcRegularizationMatrix, src/solver/inverse.cpp:83-135) and the differential
+solve (Solver::preSolve + solveDifferential, src/solver/solver.cpp:136-188) for several
+frames at once, with either regularisation."""
+import numpy as np
+import pytest
+
+import oracle.eit_oracle as eo
+from mpflow_amd.phantom import frame_seed, inclusion_phantom
+from tests.problems import eit_of, make_problem, oracle_of
+
+pytestmark = pytest.mark.gpu
+
+LAM = 1e-2
+
+
+@pytest.fixture(scope="module")
+def gnprob():
+    p = make_problem(10, 16, 0.1, C=2, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    ones = np.ones(o.E)
+    U1, phis1, _ = o.solve(ones)
+    J1 = o.jacobian(ones, phis1)
+    gs = [inclusion_phantom(p["nodes"], p["elements"], frame_seed(7, f)) for f in range(3)]
+    Um = [o.flatten_voltages(o.solve(g)[0]) for g in gs]
+    return p, o, o.flatten_voltages(U1), J1, Um
+
+
+def _opts():
+    from mpflow_amd import capi
+    from mpflow_amd.eit import SolveOptions
+    return SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=2, refine_tol=1e-3)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("reg", ["identity", "noser"])
+def test_gn_step_regularization(gpu_ctx, gnprob, reg):
+    import torch
+    from mpflow_amd.eit import Solver
+    p, o, U1, J1, Um = gnprob
+    print("case: GN step with %s regularisation" % reg, flush=True)
+    eit = eit_of(p, gpu_ctx)
+    s = Solver(eit, LAM, gn_refine=4, options=_opts())
+    s.set_regularization(reg)
+    s.measurement[0] = torch.tensor(Um[0], dtype=torch.float64)
+    s.solve_absolute()
+    g = s.material_distribution[0].cpu().numpy().astype(np.float64)
+    R = eo.noser_diag(J1) if reg == "noser" else np.ones(o.E)
+    want = 1.0 + eo.gn_step_regularized(J1, Um[0] - U1, LAM ** 2, R)
+    err = _rel(g - 1.0, want - 1.0)
+    errg = _rel(g, want)
+    print("delta rel err", err, "gamma rel err", errg)
+    # the step amplifies the FP32 Jacobian's rounding by the normal matrix's conditioning
+    # (lambda = 1e-2): the step agrees to ~1e-3, gamma (the reconstruction) far better
+    assert err < 1e-2 and errg < 1e-4
+
+
+@pytest.mark.parametrize("reg", ["identity", "noser"])
+def test_differential_solve_frames(gpu_ctx, gnprob, reg):
+    """Three frames against one preparation equal three one-step reconstructions from the
+    reference distribution (oracle) and the device's own absolute step from it."""
+    import torch
+    from mpflow_amd.eit import Solver
+    p, o, U1, J1, Um = gnprob
+    print("case: differential solve, %s" % reg, flush=True)
+    eit = eit_of(p, gpu_ctx, frames=3)
+    s = Solver(eit, LAM, gn_refine=4, options=_opts())
+    s.set_regularization(reg)
+    s.pre_solve()
+    s.measurement.copy_(torch.tensor(np.stack(Um), dtype=torch.float64))
+    s.solve_differential()
+    g = s.material_distribution.cpu().numpy().astype(np.float64)
+    R = eo.noser_diag(J1) if reg == "noser" else np.ones(o.E)
+    errs, errg = [], []
+    for f in range(3):
+        want = 1.0 + eo.gn_step_regularized(J1, Um[f] - U1, LAM ** 2, R)
+        errs.append(_rel(g[f] - 1.0, want - 1.0))
+        errg.append(_rel(g[f], want))
+    # the absolute step from the reference distribution is the same linear step
+    s.material_distribution.fill_(1.0)
+    s.solve_absolute()
+    ga = s.material_distribution.cpu().numpy().astype(np.float64)
+    dab = _rel(g - 1.0, ga - 1.0)
+    print("delta rel err per frame", errs, "gamma", errg, "vs absolute step", dab)
+    assert max(errs) < 1e-2 and max(errg) < 1e-4 and dab < 1e-5

@@ -571,3 +571,21 @@ def test_cpu_baseline_two_level_matches_oracle():
     _, sj, _ = baseline.pcg(A, F, 1, tol=1e-6)
     assert np.max(np.abs(Xb - X)) < 1e-5 * np.max(np.abs(X))
     assert abs(sb - so) <= 0.25 * so and sb < 0.7 * sj
+
+
+def test_regularized_gn_step_pins():
+    """R = I reduces to the data-space step; for a diagonal R the E-space solution equals
+    the column-scaled data-space form delta = W Js^T (Js Js^T + lam I)^-1 r, Js = J W,
+    W = R^-1/2 (the form the device uses)."""
+    rng = np.random.default_rng(9)
+    J = rng.standard_normal((12, 30))
+    r = rng.standard_normal(12)
+    lam = 0.3
+    np.testing.assert_allclose(eo.gn_step_regularized(J, r, lam, np.ones(30)), eo.gn_step_exact(J, r, lam),
+                               rtol=1e-10, atol=1e-12)
+    d = eo.noser_diag(J)
+    np.testing.assert_allclose(d, np.diag(J.T @ J), rtol=1e-12)
+    w = 1.0 / np.sqrt(d)
+    Js = J * w[None, :]
+    want = w * eo.gn_step_exact(Js, r, lam)
+    np.testing.assert_allclose(eo.gn_step_regularized(J, r, lam, d), want, rtol=1e-9, atol=1e-12)
