@@ -347,41 +347,6 @@ class BatchSolver:
         assert [s.eit for s in self.solvers] == batch.models
         self.last_stats = None
 
-    def set_regularization(self, regularization_type: str = "identity", gamma_ref=None):
-        """Inverse::setRegularizationParameter's type (src/solver/inverse.cpp:83-135):
-        "identity" or "noser" (R = diag(J0^T J0) at ``gamma_ref``, default the reference
-        distribution; computed now with one forward solve and kept)."""
-        types = {"identity": capi.MPF_REG_IDENTITY, "noser": capi.MPF_REG_NOSER}
-        if regularization_type not in types:
-            raise ValueError("regularization type must be one of %s" % sorted(types))
-        g = None
-        if gamma_ref is not None:
-            g = gamma_ref.reshape(self.eit.E).to(self.eit._dev(), self.eit.torch.float32).contiguous()
-        st = capi.SolveStats()
-        check(lib().mpf_gn_set_regularization(self.eit.h, types[regularization_type],
-                                              None if g is None else C.c_void_p(g.data_ptr()),
-                                              C.byref(self.options.s), C.byref(st)))
-
-    def pre_solve(self, gamma_ref=None):
-        """Solver::preSolve (solver.cpp:136-160): forward solve, Jacobian and factored
-        normal matrix at the reference distribution (default 1), kept for
-        ``solve_differential``."""
-        torch = self.eit.torch
-        if gamma_ref is None:
-            gamma_ref = torch.ones(self.eit.E, dtype=torch.float32, device=self.eit._dev())
-        self.reference_distribution = gamma_ref.reshape(self.eit.E).to(self.eit._dev(), torch.float32).contiguous()
-        st = capi.SolveStats()
-        check(lib().mpf_diff_prepare(self.eit.h, C.c_void_p(self.reference_distribution.data_ptr()),
-                                     self.regularization_factor, C.byref(self.options.s), C.byref(st)))
-
-    def solve_differential(self):
-        """Solver::solveDifferential (solver.cpp:163-188) for every frame's measurement:
-        material_distribution = reference + one linearised step, reusing pre_solve's
-        Jacobian and factorisation."""
-        check(lib().mpf_diff_solve(self.eit.h, C.c_void_p(self.measurement.data_ptr()), self.gn_refine,
-                                   C.c_void_p(self.material_distribution.data_ptr())))
-        return self.material_distribution
-
     def solve_absolute(self, warm_start: bool = False):
         s0 = self.solvers[0]
         st = capi.SolveStats()
