@@ -508,7 +508,8 @@ def solve_direct(A, F, dc_free):
     lu = spla.splu(sp.csc_matrix(A[:N - 1, :N - 1]))
     X = np.zeros_like(F)
     X[:N - 1] = lu.solve(Fc[:N - 1])
-    X += (mean - X.sum(axis=0, keepdims=True) / N)
+    # shift by a constant c so that 1^T x = 1^T F / N = mean:  sum + N c = mean
+    X += (mean - X.sum(axis=0, keepdims=True)) / N
     return X
 
 

@@ -87,3 +87,41 @@ def test_differential_solve_frames(gpu_ctx, gnprob, reg):
     dab = _rel(g - 1.0, ga - 1.0)
     print("delta rel err per frame", errs, "gamma", errg, "vs absolute step", dab)
     assert max(errs) < 1e-2 and max(errg) < 1e-4 and dab < 1e-5
+
+
+def test_solver_from_config_differential(gpu_ctx, tmp_path):
+    """Solver::fromConfig (solver.cpp:70-135) from a JSON file and mesh text files:
+    preSolve at gamma = 1 with the configured regularisation, then one differential solve
+    per frame equals the oracle's linear step."""
+    import json
+
+    import torch
+    from mpflow_amd import config as cfg
+    from mpflow_amd.mesh import ring_mesh
+    nodes, el = ring_mesh(8, 16, 0.1)
+    d = tmp_path / "mesh"
+    d.mkdir()
+    cfg.savetxt(str(d / "nodes.txt"), nodes)
+    cfg.savetxt(str(d / "elements.txt"), el)
+    conf = {"model": {"mesh": "mesh", "ports": {"count": 16, "width": 0.1, "height": 0.1},
+                      "source": {"type": "current", "value": 1e-3},
+                      "2.5D": {"components": 2, "height": 0.4}, "material": 1.0},
+            "solver": {"parallelImages": 2, "regularizationFactor": LAM, "regularizationType": "noser"}}
+    json.dump(conf, open(tmp_path / "config.json", "w"))
+    print("case: solver from config", flush=True)
+    s = cfg.solver_from_config(str(tmp_path / "config.json"), ctx=gpu_ctx, options=_opts())
+    spec, _ = cfg.parse_config(str(tmp_path / "config.json"))
+    o = eo.EITOracle(spec.nodes, spec.elements, spec.boundary, spec.ports, spec.drive, spec.meas, spec.current,
+                     1.0, 2, 0.4, 0.1)
+    ones = np.ones(o.E)
+    U1, phis1, _ = o.solve(ones)
+    J1 = o.jacobian(ones, phis1)
+    gs = [inclusion_phantom(spec.nodes, spec.elements, frame_seed(8, f)) for f in range(2)]
+    Um = np.stack([o.flatten_voltages(o.solve(g)[0]) for g in gs])
+    s.measurement.copy_(torch.tensor(Um, dtype=torch.float64))
+    s.solve_differential()
+    g = s.material_distribution.cpu().numpy().astype(np.float64)
+    errs = [_rel(g[f], 1.0 + eo.gn_step_regularized(J1, Um[f] - o.flatten_voltages(U1), LAM ** 2,
+                                                      eo.noser_diag(J1))) for f in range(2)]
+    print("gamma rel err", errs)
+    assert max(errs) < 1e-4

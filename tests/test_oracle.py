@@ -217,6 +217,14 @@ def test_pcg_restatement_against_direct(small):
     # brute force: dense (A + 1 1^T) solve
     Ad = A.toarray() + np.ones((o.N, o.N))
     np.testing.assert_allclose(np.linalg.solve(Ad, F), X, atol=1e-12 * np.max(np.abs(X)))
+    # right-hand sides with a nonzero sum (single-electrode drives): the dense brute force
+    # of (A + 1 1^T) fixes the constant part as 1^T x = 1^T F / N
+    Fe = o.alpha_beta(0)[1] * o.B[:, :3]
+    Xe = eo.solve_direct(A, Fe, True)
+    np.testing.assert_allclose(Xe, np.linalg.solve(Ad, Fe), atol=1e-11 * np.max(np.abs(Xe)))
+    np.testing.assert_allclose(Xe.sum(axis=0), Fe.sum(axis=0) / o.N, rtol=1e-10)
+    Xce, _ = eo.pcg_reference(A, Fe, 1.0 / A.diagonal(), True, rule="tolerance", tol=1e-13)
+    np.testing.assert_allclose(Xce, Xe, atol=1e-10 * np.max(np.abs(Xe)))
     # reference stop rule: column 0 reaches sqrt(r.z / b.b) < FLT_EPSILON
     Xr, sr = eo.pcg_reference(A, F, 1.0 / A.diagonal(), True)
     assert sr < 10 * o.N
