@@ -139,9 +139,16 @@ constexpr int kSpmvRowsPerWave = 16;
 // unrolled slot loop.  Zero-valued slots (padding) gather through an offset past the
 // buffer's range, which the buffer unit answers with 0 without a memory access.
 // epi(row, c4, acc) consumes each row's 4-column result.
-template <int LG, int W, class Epi>
+// With an LDS copy `xs` of the block's own 256 rows of X (rows tile0 ...), neighbours
+// inside the tile are read from LDS and only the others from memory: the gather for an
+// in-tile neighbour is sent past the buffer's range (answered with 0, no memory access).
+template <int LG, int W, bool XT, class Epi>
 __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l64, const float* X,
-                                                const int2 (*ell)[kSpmvRowsPerWave], Epi&& epi) {
+                                                const int2 (*ell)[kSpmvRowsPerWave], const float4* xs,
+                                                int tile0, Epi&& epi) {
+    // with the LDS tile each slot holds both a memory and an LDS result: 4 in flight keep
+    // the kernel inside 64 VGPRs (8 wavefronts per SIMD)
+    constexpr int UNR = XT ? 4 : 8;
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;                       // lanes per row
     constexpr int RPS = 64 / LPR;                     // rows per wavefront step
@@ -155,20 +162,37 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
     for (int st = 0; st < STEPS; ++st) {
         const int src = st * RPS + rsub;              // < 16
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll 8
+#pragma unroll UNR
         for (int q = 0; q < W; ++q) {
             const int2 e = ell[q][src];
             const float v = __int_as_float(e.y);
-            const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
-            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-            acc.x += v * __uint_as_float(raw[0]);
-            acc.y += v * __uint_as_float(raw[1]);
-            acc.z += v * __uint_as_float(raw[2]);
-            acc.w += v * __uint_as_float(raw[3]);
+            float4 x;
+            if constexpr (XT) {
+                const int local = e.x - tile0;
+                const bool in = (unsigned)local < (unsigned)kTileRows;
+                const uint32_t off = (v != 0.0f && !in) ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
+                const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                const float4 l = xs[(in ? local : 0) * LPR + c4];
+                x = in ? l : make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
+                                         __uint_as_float(raw[2]), __uint_as_float(raw[3]));
+            } else {
+                const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
+                const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                x = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]), __uint_as_float(raw[2]),
+                                __uint_as_float(raw[3]));
+            }
+            acc.x += v * x.x;
+            acc.y += v * x.y;
+            acc.z += v * x.z;
+            acc.w += v * x.w;
         }
         epi(row0 + src, c4, acc);
     }
 }
+
+// LDS copy of a tile's rows of X for ell_gather_wave (kTileRows x Rp floats)
+template <int LG>
+using XTile = float4[kTileRows * (1 << LG) / 4];
 
 // LDS for ell_gather_wave: one [WMAX][16] (id, value) table per wavefront of the block
 template <int WMAX>
@@ -179,9 +203,16 @@ using EllStage = int2[kEwBlock / 64][WMAX][kSpmvRowsPerWave];
 // 12-slot body.  The wavefront first stages its (id, S + alpha_s R) entries in LDS with
 // coalesced loads spread over all 64 lanes; slots between the group width and the bin are
 // zero.  Contains a block barrier: every wavefront of the block must call it.
-template <int LG, int WMAX, class Epi>
+// XT: xs is LDS room for the tile's rows of X, loaded here by the whole block.
+template <int LG, int WMAX, bool XT = false, class Epi>
 __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0, int l64, const float* X,
-                                                EllStage<WMAX>& stage, Epi&& epi) {
+                                                EllStage<WMAX>& stage, Epi&& epi, float4* xs = nullptr) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int tile0 = row0 / kTileRows * kTileRows;
+    if constexpr (XT) {
+        const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
+        for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) xs[i] = X4[i];
+    }
     constexpr int WS = WMAX > 12 ? 12 : WMAX;
     const int wave = threadIdx.x >> 6;
     const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
@@ -201,8 +232,8 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         stage[wave][q][r] = v;
     }
     __syncthreads();
-    if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS>(J, row0, l64, X, stage[wave], epi);
-    else ell_gather_rows<LG, WMAX>(J, row0, l64, X, stage[wave], epi);
+    if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    else ell_gather_rows<LG, WMAX, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
 }
 
 // ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
@@ -241,6 +272,37 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
 // coarse-correction row chunks of a job (one partial each in channels 5 and 6)
 __device__ __forceinline__ int coarse_chunks(const JobDev& J) {
     return (J.ntiles + kCoarseRows - 1) / kCoarseRows;
+}
+
+// block_partials4 with the reduction buffer supplied by the caller (it may share LDS with
+// data the wavefronts used before: the block synchronises before writing it)
+template <int LG, int NV>
+__device__ __forceinline__ void block_partials4_buf(const JobDev& J, int s, int tile, const int* ch,
+                                                    double (&v)[NV][4], double (*red)[kEwBlock / 64][64]) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
+    __syncthreads();
+    if (rsub == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[c][wave][4 * c4 + j] = v[c][j];
+    __syncthreads();
+    if ((int)threadIdx.x < (1 << LG)) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double a = 0.0;
+            for (int w = 0; w < kEwBlock / 64; ++w) a += red[c][w][threadIdx.x];
+            part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
+        }
+    }
 }
 
 // rho of iteration `par` parity: r.z of the current residual lives in one of two slots so

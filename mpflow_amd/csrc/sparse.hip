@@ -153,19 +153,25 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
     const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
     const int row0 = t.z * kTileRows + wave * kSpmvRowsPerWave;
     const size_t slice_off = (size_t)s * J.np * Rp;
-    const MPF_GLOBAL float4* P4 = (const MPF_GLOBAL float4*)(gp((const float*)J.p) + slice_off);
     MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
     double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-    __shared__ EllStage<WMAX> stage;
-    ell_gather_wave<LG, WMAX>(J, s, row0, l64, J.p + slice_off, stage, [&](int row, int c4, float4 acc) {
-        const float4 own = P4[(size_t)row * LPR + c4];
+    // LDS: the wavefronts' ELL entries (later reused for the partial sums) and the tile's
+    // own rows of p, from which the in-tile gathers and p itself are read
+    __shared__ union {
+        EllStage<WMAX> stage;
+        double red[2][kEwBlock / 64][64];
+    } sh;
+    __shared__ XTile<LG> xs;
+    const int tile0 = t.z * kTileRows;
+    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, J.p + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
+        const float4 own = xs[(row - tile0) * LPR + c4];
         Q4[(size_t)row * LPR + c4] = acc;
         v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
         v[0][2] += (double)own.z * (double)acc.z; v[0][3] += (double)own.w * (double)acc.w;
         v[1][0] += (double)own.x; v[1][1] += (double)own.y; v[1][2] += (double)own.z; v[1][3] += (double)own.w;
-    });
+    }, xs);
     const int ch[2] = {0, 1};
-    block_partials4<LG, 2>(J, s, t.z, ch, v);
+    block_partials4_buf<LG, 2>(J, s, t.z, ch, v, sh.red);
 }
 
 template <int LG, int WMAX, int U>
