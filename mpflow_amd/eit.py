@@ -138,6 +138,7 @@ class EIT:
          self.C, self.nnz, self.R_unique, self.wpad) = [int(v) for v in dims[:14]]
         self.D, self.M = D, k["meas"].shape[1]
         self.reference_value = float(reference_value)
+        self.logarithmic = bool(logarithmic)
         dev = torch.device("cuda", self.ctx.device)
         self.result = torch.zeros(self.frames, self.m, dtype=torch.float32, device=dev)
         self.result64 = torch.zeros(self.frames, self.m, dtype=torch.float64, device=dev)
@@ -229,7 +230,8 @@ class Solver:
         self.regularization_factor = float(regularization_factor)
         self.gn_refine = int(gn_refine)
         self.options = options or SolveOptions()
-        init = 1.0  # linear parametrisation starts at gamma = 1 (solver.cpp:55-62)
+        # the reference distribution: gamma = 1 (linear) or 0 (logarithmic), solver.cpp:55-62
+        init = 0.0 if eit.logarithmic else 1.0
         self.material_distribution = torch.full((eit.frames, eit.E), init, dtype=torch.float32,
                                                 device=eit._dev())
         self.measurement = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
@@ -257,7 +259,8 @@ class Solver:
         ``solve_differential``."""
         torch = self.eit.torch
         if gamma_ref is None:
-            gamma_ref = torch.ones(self.eit.E, dtype=torch.float32, device=self.eit._dev())
+            gamma_ref = torch.full((self.eit.E,), 0.0 if self.eit.logarithmic else 1.0, dtype=torch.float32,
+                                   device=self.eit._dev())
         self.reference_distribution = gamma_ref.reshape(self.eit.E).to(self.eit._dev(), torch.float32).contiguous()
         st = capi.SolveStats()
         check(lib().mpf_diff_prepare(self.eit.h, C.c_void_p(self.reference_distribution.data_ptr()),

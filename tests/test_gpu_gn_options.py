@@ -125,3 +125,33 @@ def test_solver_from_config_differential(gpu_ctx, tmp_path):
                                                       eo.noser_diag(J1))) for f in range(2)]
     print("gamma rel err", errs)
     assert max(errs) < 1e-4
+
+
+def test_logarithmic_parametrisation(gpu_ctx):
+    """Equation<..., logarithmic=true> (equation_kernel.cu:35-53): sigma = ref 10^(gamma/10);
+    forward voltages and one GN step from gamma = 0 against the oracle."""
+    import torch
+    from mpflow_amd.eit import EIT, Solver
+    p = make_problem(10, 16, 0.1, C=2, height=0.4, port_height=0.1)
+    o = eo.EITOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"],
+                     2.0, 2, 0.4, 0.1, logarithmic=True)
+    g = 10.0 * np.log10(inclusion_phantom(p["nodes"], p["elements"], frame_seed(9, 0)))
+    g32 = g.astype(np.float32).astype(np.float64)
+    print("case: logarithmic", flush=True)
+    eit = EIT(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"], 2.0, 2,
+              0.4, 0.1, logarithmic=True, ctx=gpu_ctx)
+    eit.solve(torch.tensor(g, dtype=torch.float32), _opts())
+    U, _, _ = o.solve(g32)
+    ef = float(np.max(np.abs(eit.voltages().cpu().numpy()[0] - U)) / np.max(np.abs(U)))
+    s = Solver(eit, LAM, options=_opts())
+    assert float(s.material_distribution.abs().max()) == 0.0      # log reference distribution
+    Um = o.flatten_voltages(U)
+    s.measurement[0] = torch.tensor(Um, dtype=torch.float64)
+    s.solve_absolute()
+    z = np.zeros(o.E)
+    U0, phis0, _ = o.solve(z)
+    want = eo.gn_step_exact(o.jacobian(z, phis0), Um - o.flatten_voltages(U0), (2.0 * LAM) ** 2)
+    got = s.material_distribution[0].cpu().numpy().astype(np.float64)
+    es = _rel(got, want)
+    print("forward rel err", ef, "GN step rel err", es)
+    assert ef < 1e-6 and es < 1e-2
