@@ -286,7 +286,7 @@ def run_c5(args):
     fps_plain_tl, _ = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, preconditioner=two))
     # learned start, B frames per launch
     eB = mk(B)
-    rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3)
+    rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3, fallback_eit=e1)
     noise = np.sqrt(np.mean(Uh ** 2, axis=1)) * 10.0 ** (-60.0 / 20.0) * math.sqrt(Uh.shape[1])
     rec.reconstruct(Uh[:B], noise_norm=float(noise[:B].max()))
     torch.cuda.synchronize()

@@ -348,6 +348,17 @@ int mpf_gemv_t(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_A, int32
 int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const double* d_b,
                   double* d_y);
 
+/* Inverse::solve's Tikhonov system (src/solver/inverse.cpp:60-76, 163-238) for a
+ * caller's Jacobian and residual:
+ *   delta = (J^T J + shift I)^-1 J^T r = J^T (J J^T + shift I)^-1 r
+ *   d_J   : rows x cols column-major FP32 (ldj >= rows), e.g. the EIT Jacobian m x E
+ *   shift : (referenceValue * regularizationFactor)^2 > 0
+ *   d_r   : rows (FP64),  d_delta : cols (FP64, out)
+ * J J^T on MFMA (BF16 in, FP32 accumulate), FP64 Cholesky, then gn_refine passes of FP64
+ * residual refinement with the FP32 J.  Allocates its work space; synchronises. */
+int mpf_normal_solve(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J, int32_t ldj, double shift,
+                     const double* d_r, int32_t gn_refine, double* d_delta);
+
 /* ---------------------------------------------------------------------------
  * Learned start: a multilayer perceptron (dense layers, GELU between them, linear
  * output) trained with MSE and Adam.  No counterpart in the reference.  GEMMs run on

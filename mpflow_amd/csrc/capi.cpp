@@ -441,6 +441,47 @@ int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const
     });
 }
 
+int mpf_normal_solve(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J, int32_t ldj, double shift,
+                     const double* d_r, int32_t gn_refine, double* d_delta) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_J && d_r && d_delta, "mpf_normal_solve: null argument");
+        MPF_REQUIRE(rows > 0 && cols > 0 && ldj >= rows && gn_refine >= 0 && shift > 0.0,
+                    "mpf_normal_solve: bad shape, shift or refinement count");
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const int m = rows, E = cols;
+        mpf::DBuf<float> G((size_t)m * m);
+        mpf::DBuf<double> A((size_t)m * m), y(m), dy(m), res(m), vE(E);
+        mpf::DBuf<double> work(mpf::gemv_n_workspace(m, E, 1) / sizeof(double) + 1);
+        mpf::DBuf<int2> pairs(mpf::cholesky_pairs(m));
+        mpf::DBuf<int> info(1);
+        if (mpf::gram_nt_applies(m, ldj)) {
+            mpf::DBuf<char> gw(mpf::gram_nt_workspace_bytes(m, E, 1));
+            mpf::launch_gram_nt(d_J, 0, m, E, ldj, G.p, 0, 1, gw.p, st);
+            MPF_HIP(hipStreamSynchronize(st));
+        } else {
+            mpf::DBuf<char> gw(mpf::gram_workspace_bytes(m, E));
+            mpf::launch_gram(d_J, 0, m, E, ldj, G.p, gw.p, st);
+            MPF_HIP(hipStreamSynchronize(st));
+        }
+        MPF_HIP(hipMemsetAsync(info.p, 0, sizeof(int), st));
+        mpf::launch_cholesky(m, G.p, 0, shift, A.p, 0, 1, info.p, pairs.p, st);
+        mpf::launch_potrs(m, A.p, 0, d_r, 0, y.p, 0, 1, st);
+        for (int k = 0; k < gn_refine; ++k) {
+            mpf::launch_gemv_t(m, E, d_J, ldj, 0, y.p, 0, vE.p, 0, 1, st);
+            mpf::launch_gemv_n(m, E, d_J, ldj, 0, vE.p, 0, shift, y.p, 0, d_r, 0, res.p, 0, 1, work.p, st);
+            mpf::launch_potrs(m, A.p, 0, res.p, 0, dy.p, 0, 1, st);
+            mpf::launch_add_to(m, y.p, dy.p, st);
+        }
+        mpf::launch_gemv_t(m, E, d_J, ldj, 0, y.p, 0, d_delta, 0, 1, st);
+        int h = 0;
+        MPF_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        MPF_HIP(hipStreamSynchronize(st));
+        MPF_REQUIRE(h == 0, "mpf_normal_solve: normal-equation matrix is not positive definite");
+        return MPF_OK;
+    });
+}
+
 }  // extern "C"
 
 int mpf_mlp_create(mpf_ctx* ctx, int32_t n_layers, const int32_t* dims, uint64_t seed, mpf_mlp** out) {

@@ -12,7 +12,8 @@ gamma = 1, src/solver/solver.cpp:55-62).
 * ``LearnedReconstruction``: network start + ``gn_steps`` Gauss-Newton steps; the start
   is accepted only if the final forward-model residual is no worse than the noise-level
   bound (discrepancy principle, tau * ||noise||) or than the residual plain Gauss-Newton
-  reaches from the homogeneous start; otherwise the frame is reconstructed by plain GN.
+  reaches from the homogeneous start; otherwise the frame is reconstructed by plain GN
+  (one frame at a time on ``fallback_eit`` when given).
 """
 from __future__ import annotations
 
@@ -158,11 +159,14 @@ class LearnedReconstruction:
 
     def __init__(self, eit, net: Mlp, features: VoltageFeatures, lam: float, options: SolveOptions,
                  gn_steps: int = 1, plain_steps: int = 3, tau: float = 2.0, gn_refine: int = 4,
-                 check_tol: float = 1e-4):
+                 check_tol: float = 1e-4, fallback_eit=None):
         self.eit, self.net, self.features = eit, net, features
         self.check_tol = check_tol
         self.solver = Solver(eit, lam, gn_refine=gn_refine, options=options)
         self.gn_steps, self.plain_steps, self.tau = gn_steps, plain_steps, tau
+        # frames whose network start is rejected are redone one at a time on a one-frame
+        # model of the same mesh (else the whole batch is redone)
+        self.fallback = Solver(fallback_eit, lam, gn_refine=gn_refine, options=options) if fallback_eit else None
 
     def _residual(self, Um):
         return float(np.linalg.norm(Um - self.solver.calculation.cpu().numpy(), axis=1).max())
@@ -196,12 +200,20 @@ class LearnedReconstruction:
         ok = np.ones(Um.shape[0], bool)
         if noise_norm is not None:
             ok = r_learned <= self.tau * noise_norm
-        if compare_plain or not ok.all():
+        if compare_plain or (not ok.all() and self.fallback is None):
             gp = self._gn(Um_t, self.plain_steps)
             r_plain = np.linalg.norm(Um - self.eit.result64.cpu().numpy(), axis=1)
             info["residual_plain"] = r_plain
             ok = ok & (r_learned <= r_plain) if compare_plain else ok
             for f in np.nonzero(~ok)[0]:
                 g[f] = gp[f]
+        elif not ok.all():
+            fb = self.fallback
+            for f in np.nonzero(~ok)[0]:
+                fb.material_distribution.fill_(0.0 if fb.eit.logarithmic else 1.0)
+                fb.measurement[0].copy_(Um_t[f])
+                for k in range(self.plain_steps):
+                    fb.solve_absolute(warm_start=k > 0)
+                g[f] = fb.material_distribution[0]
         info["accepted"] = ok
         return g, info

@@ -95,3 +95,26 @@ def test_gn_step_matches_oracle(gpu_ctx):
     err = np.linalg.norm(d_gpu - d_ref) / np.linalg.norm(d_ref)
     print("gn step rel err", err)
     assert err < 1e-3
+
+
+@pytest.mark.parametrize("m,E,ld", [(96, 700, 96), (130, 400, 131)])
+def test_normal_solve(gpu_ctx, m, E, ld):
+    """mpf_normal_solve: the Tikhonov step J^T (J J^T + shift I)^-1 r with FP64 refinement
+    reaches the FP64 solution for the FP32 Jacobian (the BF16 Gram only preconditions)."""
+    import torch
+    rng = np.random.default_rng(m + E)
+    J = rng.standard_normal((m, E)).astype(np.float32)
+    Jd = torch.zeros(E, ld, dtype=torch.float32, device="cuda")      # column-major m x E, ld
+    Jd[:, :m] = torch.tensor(J.T)
+    r = rng.standard_normal(m)
+    rd = torch.tensor(r, device="cuda")
+    out = torch.empty(E, dtype=torch.float64, device="cuda")
+    shift = 0.5
+    print("case: normal solve m=%d E=%d ld=%d" % (m, E, ld), flush=True)
+    check(lib().mpf_normal_solve(gpu_ctx.h, m, E, C.c_void_p(Jd.data_ptr()), ld, shift, C.c_void_p(rd.data_ptr()),
+                                 4, C.c_void_p(out.data_ptr())))
+    J64 = J.astype(np.float64)
+    want = J64.T @ np.linalg.solve(J64 @ J64.T + shift * np.eye(m), r)
+    err = np.linalg.norm(out.cpu().numpy() - want) / np.linalg.norm(want)
+    print("rel err", err)
+    assert err < 1e-9
