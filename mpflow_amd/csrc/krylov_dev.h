@@ -168,11 +168,10 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
             const float v = __int_as_float(e.y);
             float4 x;
             if constexpr (XT) {
-                const int local = e.x - tile0;
-                const bool in = (unsigned)local < (unsigned)kTileRows;
+                const bool in = e.x < 0;              // staged as -1 - (row - tile0)
                 const uint32_t off = (v != 0.0f && !in) ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
                 const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-                const float4 l = xs[(in ? local : 0) * LPR + c4];
+                const float4 l = xs[(in ? -1 - e.x : 0) * LPR + c4];
                 x = in ? l : make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
                                          __uint_as_float(raw[2]), __uint_as_float(raw[3]));
             } else {
@@ -213,10 +212,11 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
         for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) xs[i] = X4[i];
     }
+    // slot-count bins of 4 from 12 up (P2 meshes: midpoint rows 9 wide, vertex rows 19)
     constexpr int WS = WMAX > 12 ? 12 : WMAX;
     const int wave = threadIdx.x >> 6;
     const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
-    const int wb = (WS < WMAX && wl <= WS) ? WS : WMAX;
+    const int wb = wl <= WS ? WS : (wl <= 16 && WMAX > 16 ? 16 : (wl <= 20 && WMAX > 20 ? 20 : WMAX));
     const int np = J.np;
     const float al = J.alpha[s];
     const MPF_GLOBAL int32_t* ids = gp(J.ids);
@@ -227,12 +227,20 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         int2 v = make_int2(0, 0);
         if (q < wl) {
             const size_t g = (size_t)q * np + row0 + r;
-            v = make_int2(ids[g], __float_as_int(Sv[g] + al * Rv[g]));
+            int id = ids[g];
+            if constexpr (XT) {
+                // neighbours inside the tile are staged as -1 - (row - tile0)
+                const int local = id - tile0;
+                if ((unsigned)local < (unsigned)kTileRows) id = -1 - local;
+            }
+            v = make_int2(id, __float_as_int(Sv[g] + al * Rv[g]));
         }
         stage[wave][q][r] = v;
     }
     __syncthreads();
     if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
     else ell_gather_rows<LG, WMAX, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
 }
 
