@@ -162,24 +162,47 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
     for (int st = 0; st < STEPS; ++st) {
         const int src = st * RPS + rsub;              // < 16
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (XT) {
+            // groups of G slots: the G memory gathers of a group are all issued before
+            // any is used; every slot reads both its memory gather and its LDS row and
+            // weights them by v or 0, so that no branch holds a gather back (the memory
+            // gather of an in-tile or padding slot is sent past the buffer range)
+            constexpr int G = 4;
+            static_assert(W % G == 0, "slot bins are multiples of 4");
+#pragma unroll 1
+            for (int q0 = 0; q0 < W; q0 += G) {
+                v4u raw[G];
+                int2 e[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    e[u] = ell[q0 + u][src];
+                    const float v = __int_as_float(e[u].y);
+                    const uint32_t off = (v != 0.0f && e[u].x >= 0) ? (uint32_t)(e[u].x * Rp + 4 * c4) * 4u : range;
+                    raw[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const bool in = e[u].x < 0;       // staged as -1 - (row - tile0)
+                    const float v = __int_as_float(e[u].y);
+                    const float4 l = xs[(in ? -1 - e[u].x : 0) * LPR + c4];
+                    const float vi = in ? v : 0.0f, vo = in ? 0.0f : v;
+                    acc.x += vi * l.x + vo * __uint_as_float(raw[u][0]);
+                    acc.y += vi * l.y + vo * __uint_as_float(raw[u][1]);
+                    acc.z += vi * l.z + vo * __uint_as_float(raw[u][2]);
+                    acc.w += vi * l.w + vo * __uint_as_float(raw[u][3]);
+                }
+            }
+            epi(row0 + src, c4, acc);
+            continue;
+        }
 #pragma unroll UNR
         for (int q = 0; q < W; ++q) {
             const int2 e = ell[q][src];
             const float v = __int_as_float(e.y);
-            float4 x;
-            if constexpr (XT) {
-                const bool in = e.x < 0;              // staged as -1 - (row - tile0)
-                const uint32_t off = (v != 0.0f && !in) ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
-                const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-                const float4 l = xs[(in ? -1 - e.x : 0) * LPR + c4];
-                x = in ? l : make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
+            const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
+            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            const float4 x = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
                                          __uint_as_float(raw[2]), __uint_as_float(raw[3]));
-            } else {
-                const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
-                const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-                x = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]), __uint_as_float(raw[2]),
-                                __uint_as_float(raw[3]));
-            }
             acc.x += v * x.x;
             acc.y += v * x.y;
             acc.z += v * x.z;
