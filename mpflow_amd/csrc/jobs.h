@@ -17,7 +17,8 @@ namespace mpf {
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
                SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB, SC_NFIELDS };
 // per-slice integer state: [0] active, [1] steps, [2] hit cap or breakdown,
-// [3] BiCGSTAB: the next iteration is the first
+// [3] BiCGSTAB: the next iteration is the first; CG: [0] as the update kernel of the
+// current iteration saw it (the stop rule clears [0] during the pupdate launch)
 constexpr int kStInts = 4;
 // partial-sum channels per tile
 constexpr int kParts = 7;

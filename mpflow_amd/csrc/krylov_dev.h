@@ -146,9 +146,7 @@ template <int LG, int W, bool XT, class Epi>
 __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l64, const float* X,
                                                 const int2 (*ell)[kSpmvRowsPerWave], const float4* xs,
                                                 int tile0, Epi&& epi) {
-    // with the LDS tile each slot holds both a memory and an LDS result: 4 in flight keep
-    // the kernel inside 64 VGPRs (8 wavefronts per SIMD)
-    constexpr int UNR = XT ? 4 : 8;
+    constexpr int UNR = 8;
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;                       // lanes per row
     constexpr int RPS = 64 / LPR;                     // rows per wavefront step
@@ -156,8 +154,8 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
     const int rsub = l64 / LPR, c4 = l64 % LPR;
     const uint32_t range = (uint32_t)((size_t)J.np * Rp * sizeof(float));
     const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(X, (size_t)range);
-    // slots in groups of 8 so that at most 8 gathers (32 VGPRs) are in flight per lane,
-    // which keeps the kernel at 8 wavefronts per SIMD
+    // at most 4 (LDS tile) or 8 gathers (16 or 32 VGPRs) are in flight per lane, which
+    // keeps the kernels at 8 wavefronts per SIMD
 #pragma unroll 1
     for (int st = 0; st < STEPS; ++st) {
         const int src = st * RPS + rsub;              // < 16

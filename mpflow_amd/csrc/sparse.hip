@@ -256,8 +256,12 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
 // scalars, no cross-block synchronisation); tile 0's block also keeps the solver state.
 //
 // update: alpha = rho / p.q (with dcFree q = A p + 1 1^T p, so p.q += (1^T p)^2);
-// x += alpha p, r -= alpha q (a zero p.q zeroes both, updateVectorKernel
+// r -= alpha q (a zero p.q zeroes r, and x in pupdate, updateVectorKernel
 // conjugate_gradient_kernel.cu:77-89); tile partials of r.z (z = D^-1 r) and r.r.
+// x += alpha p is left to pupdate, which reads p anyway.  Tile 0's block records
+// whether the slice is live in this iteration (state [3]): pupdate's blocks test that
+// instead of [0], which its stop rule may clear while they run, so that every tile of a
+// slice that stops in this iteration still takes its last x update.
 template <int LG>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
                                                           const int4* __restrict__ tiles, int par) {
@@ -265,7 +269,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    if (!slice_active(J, s)) return;
+    const bool live = slice_active(J, s);
+    if (t.z == 0 && threadIdx.x == 0) J.st[s * kStInts + 3] = live ? 1 : 0;
+    if (!live) return;
     __shared__ double sh_al[Rp], sh_dc[Rp];
     __shared__ int sh_zero[Rp];
     {
@@ -309,29 +315,24 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
-    const MPF_GLOBAL float4* P = (const MPF_GLOBAL float4*)gp((const float*)J.p) + base;
     const MPF_GLOBAL float4* Q = (const MPF_GLOBAL float4*)gp((const float*)J.q) + base;
-    MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
     MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
     double v[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
-        const float4 p4 = P[i], q4 = Q[i];
-        float4 x4 = X[i], r4 = Rr[i];
+        const float4 q4 = Q[i];
+        const float4 r4 = Rr[i];
         const float d = dinv[row];
         const float dc = row < J.n ? 1.0f : 0.0f;
-        float pv[4] = {p4.x, p4.y, p4.z, p4.w}, qv[4] = {q4.x, q4.y, q4.z, q4.w};
-        float xv[4] = {x4.x, x4.y, x4.z, x4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
+        float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (zero[j]) { xv[j] = 0.0f; rv[j] = 0.0f; }
-            else { xv[j] = xv[j] + a[j] * pv[j]; rv[j] = rv[j] - a[j] * (qv[j] + dc * dcv[j]); }
+            rv[j] = zero[j] ? 0.0f : rv[j] - a[j] * (qv[j] + dc * dcv[j]);
             const float z = d * rv[j];
             v[0][j] += (double)rv[j] * (double)z;
             v[1][j] += (double)rv[j] * (double)rv[j];
             v[2][j] += (double)rv[j];
         }
-        X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
         Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
     }
     // channel 4: the tile's W^T r for the two-level preconditioner
@@ -487,10 +488,12 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
     }
 }
 
-// pupdate: rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p.  Tile 0's
+// pupdate: x += alpha p (alpha and p.q as the update kernel left them; a zero p.q zeroes
+// x); rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p.  Tile 0's
 // block also sums r.r, applies the stop rule (ConjugateGradient::solve,
-// src/numeric/conjugate_gradient.cpp:118-150) and counts the step.  Other blocks of the
-// same launch may see the slice leave mid-launch and skip it: p is not read again.
+// src/numeric/conjugate_gradient.cpp:118-150) and counts the step.  The blocks test the
+// slice state the update kernel recorded for this iteration, so a slice that stops here
+// still has every tile's x updated (its p is updated too and not read again).
 template <int LG>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
                                                            const int4* __restrict__ tiles, int par,
@@ -499,7 +502,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    if (!slice_active(J, s)) return;
+    if (gp(J.st)[s * kStInts + 3] == 0) return;
     const bool book = t.z == 0;
     __shared__ double sh_b[Rp];
     __shared__ int sh_bz[Rp];
@@ -565,25 +568,30 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
         }
     }
     const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
-    float b[4];
-    bool zero[4];
+    float b[4], a[4];
+    bool zero[4], xzero[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         b[j] = (float)sh_b[4 * c4 + j];
         zero[j] = sh_bz[4 * c4 + j] != 0;
+        xzero[j] = sc_ptr(J, SC_PQ, s)[4 * c4 + j] == 0.0;
+        a[j] = (float)sc_ptr(J, SC_ALPHA, s)[4 * c4 + j];
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Rr = (const MPF_GLOBAL float4*)gp((const float*)J.rv) + base;
     MPF_GLOBAL float4* P = (MPF_GLOBAL float4*)gp(J.p) + base;
+    MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
     // two-level: z = D^-1 r + y_tile on the tile's real rows
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (J.cinv) y4 = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))[((size_t)s * J.ntiles + t.z) * LPR + c4];
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
-        const float4 r4 = Rr[i], p4 = P[i];
+        const float4 r4 = Rr[i], p4 = P[i], x4 = X[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
+        X[i] = make_float4(xzero[0] ? 0.0f : x4.x + a[0] * p4.x, xzero[1] ? 0.0f : x4.y + a[1] * p4.y,
+                           xzero[2] ? 0.0f : x4.z + a[2] * p4.z, xzero[3] ? 0.0f : x4.w + a[3] * p4.w);
         P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                            zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                            zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
