@@ -146,7 +146,6 @@ template <int LG, int W, bool XT, class Epi>
 __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l64, const float* X,
                                                 const int2 (*ell)[kSpmvRowsPerWave], const float4* xs,
                                                 int tile0, Epi&& epi) {
-    constexpr int UNR = 8;
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;                       // lanes per row
     constexpr int RPS = 64 / LPR;                     // rows per wavefront step
@@ -193,18 +192,26 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
             epi(row0 + src, c4, acc);
             continue;
         }
-#pragma unroll UNR
-        for (int q = 0; q < W; ++q) {
-            const int2 e = ell[q][src];
-            const float v = __int_as_float(e.y);
-            const uint32_t off = v != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
-            const v4u raw = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-            const float4 x = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
-                                         __uint_as_float(raw[2]), __uint_as_float(raw[3]));
-            acc.x += v * x.x;
-            acc.y += v * x.y;
-            acc.z += v * x.z;
-            acc.w += v * x.w;
+        // groups of GM slots: the group's gathers are all issued before any is used
+        constexpr int GM = 4;
+#pragma unroll 1
+        for (int q0 = 0; q0 < W; q0 += GM) {
+            v4u raw[GM];
+            float v[GM];
+#pragma unroll
+            for (int u = 0; u < GM; ++u) {
+                const int2 e = ell[q0 + u][src];
+                v[u] = __int_as_float(e.y);
+                const uint32_t off = v[u] != 0.0f ? (uint32_t)(e.x * Rp + 4 * c4) * 4u : range;
+                raw[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < GM; ++u) {
+                acc.x += v[u] * __uint_as_float(raw[u][0]);
+                acc.y += v[u] * __uint_as_float(raw[u][1]);
+                acc.z += v[u] * __uint_as_float(raw[u][2]);
+                acc.w += v[u] * __uint_as_float(raw[u][3]);
+            }
         }
         epi(row0 + src, c4, acc);
     }
