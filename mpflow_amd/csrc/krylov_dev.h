@@ -43,17 +43,17 @@ __device__ __forceinline__ void block_partials(const JobDev& J, int s, int tile,
     }
 }
 
-// Sum NV partial channels over all tiles of (job, slice) in tile order with a
-// kEwBlock-thread block; each row group sums a contiguous range of tiles (loads
-// unrolled so many are in flight), then the groups are combined in order.  The result
-// is valid in threads with g == 0 (lane = column).
+// Sum NV partial channels over all tiles of (job, slice) in tile order with an
+// NT-thread block; each row group sums a contiguous range of tiles (loads unrolled so
+// many are in flight), then the groups are combined in order.  The result is valid in
+// threads with g == 0 (lane = column).
 // `count` (default: the job's tile count) partials are summed.
-template <int NV>
+template <int NV, int NT = kEwBlock>
 __device__ __forceinline__ void reduce_tiles(const JobDev& J, int s, const int* ch, double (&out)[NV],
                                              int count = -1) {
-    __shared__ double red[NV][kEwBlock];
+    __shared__ double red[NV][NT];
     const int Rp = J.Rp, lg = J.lg;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
+    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = NT >> lg;
     const int nt = count < 0 ? J.ntiles : count;
     double a[NV];
 #pragma unroll
@@ -274,9 +274,9 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
 
 // ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------
 // Combine per-lane FP64 values of 4 columns across the row sub-groups of each
-// wavefront (fixed xor tree) and across the 16 wavefronts (fixed order), then write
-// the tile partials of NV channels.
-template <int LG, int NV>
+// wavefront (fixed xor tree) and across the NT / 64 wavefronts (fixed order), then
+// write the tile partials of NV channels.
+template <int LG, int NV, int NT = kEwBlock>
 __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile, const int* ch,
                                                 double (&v)[NV][4]) {
     constexpr int LPR = (1 << LG) / 4;
@@ -288,7 +288,7 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
         for (int c = 0; c < NV; ++c)
 #pragma unroll
             for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
-    __shared__ double red[NV][kEwBlock / 64][64];
+    __shared__ double red[NV][NT / 64][64];
     if (rsub == 0)
 #pragma unroll
         for (int c = 0; c < NV; ++c)
@@ -299,7 +299,7 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
             double a = 0.0;
-            for (int w = 0; w < kEwBlock / 64; ++w) a += red[c][w][threadIdx.x];
+            for (int w = 0; w < NT / 64; ++w) a += red[c][w][threadIdx.x];
             part_ptr(J, ch[c], s, tile)[threadIdx.x] = a;
         }
     }

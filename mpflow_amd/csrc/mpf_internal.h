@@ -18,6 +18,7 @@ constexpr int kCoarseRows = 32;    // coarse rows per block of the coarse-correc
 constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level
 constexpr int kBlock = 256;        // threads per SpMV block: 4 wavefronts x 64 rows
 constexpr int kEwBlock = 1024;     // threads per streaming / reduction block
+constexpr int kUpdBlock = 256;     // threads per block of the PCG vector updates (one tile each)
 constexpr int kSlotPad = 8;        // ELL width is padded to a multiple of this
 
 struct Error : std::runtime_error {

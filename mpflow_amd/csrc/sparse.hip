@@ -263,9 +263,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
 // instead of [0], which its stop rule may clear while they run, so that every tile of a
 // slice that stops in this iteration still takes its last x update.
 template <int LG>
-__global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
-                                                          const int4* __restrict__ tiles, int par) {
-    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
+__global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
+                                                           const int4* __restrict__ tiles, int par) {
+    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
@@ -280,11 +280,11 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
         const int cnt = coarse_chunks(J);
         if (s == J.dc) {
             const int ch[2] = {5, 6};
-            reduce_tiles<2>(J, s, ch, out, cnt);
+            reduce_tiles<2, kUpdBlock>(J, s, ch, out, cnt);
         } else {
             const int ch[1] = {5};
             double o[1];
-            reduce_tiles<1>(J, s, ch, o, cnt);
+            reduce_tiles<1, kUpdBlock>(J, s, ch, o, cnt);
             out[0] = o[0];
         }
         if ((int)threadIdx.x < Rp) {
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_update4(const JobDev* __restri
     }
     // channel 4: the tile's W^T r for the two-level preconditioner
     const int ch[3] = {2, 3, 4};
-    block_partials4<LG, 3>(J, s, t.z, ch, v);
+    block_partials4<LG, 3, kUpdBlock>(J, s, t.z, ch, v);
 }
 
 template <int LG>
@@ -495,10 +495,10 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
 // slice state the update kernel recorded for this iteration, so a slice that stops here
 // still has every tile's x updated (its p is updated too and not read again).
 template <int LG>
-__global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
-                                                           const int4* __restrict__ tiles, int par,
-                                                           int* __restrict__ active) {
-    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kEwBlock / LPR;
+__global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
+                                                            const int4* __restrict__ tiles, int par,
+                                                            int* __restrict__ active) {
+    constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
@@ -514,11 +514,11 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_pupdate4(const JobDev* __restr
         const int c_rz = 5, c_rr = 6;
         if (book) {
             const int ch[2] = {c_rz, c_rr};
-            reduce_tiles<2>(J, s, ch, out, cnt);
+            reduce_tiles<2, kUpdBlock>(J, s, ch, out, cnt);
         } else {
             const int ch[1] = {c_rz};
             double o[1];
-            reduce_tiles<1>(J, s, ch, o, cnt);
+            reduce_tiles<1, kUpdBlock>(J, s, ch, o, cnt);
             out[0] = o[0];
         }
         if (threadIdx.x == 0) not_done = 0;
@@ -821,14 +821,14 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
     if (lg == 6) chunks_launch<6>(d_jobs, d_js, njs, nchunk, st);
     else if (lg == 5) chunks_launch<5>(d_jobs, d_js, njs, nchunk, st);
     else chunks_launch<4>(d_jobs, d_js, njs, nchunk, st);
-    if (lg == 6) k_pcg_update4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
-    else if (lg == 5) k_pcg_update4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
-    else k_pcg_update4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par);
+    if (lg == 6) k_pcg_update4<6><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
+    else if (lg == 5) k_pcg_update4<5><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
+    else k_pcg_update4<4><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
     // coarse correction where the job has one; chunk sums of r.z and r.r for every job
     launch_coarse_all(d_jobs, d_js, njs, nchunk, lg, 2, 3, st);
-    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
-    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
-    else k_pcg_pupdate4<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    else k_pcg_pupdate4<4><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
     check_launch();
 }
 
