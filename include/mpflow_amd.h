@@ -278,6 +278,14 @@ int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_r
  * three GEMVs per frame and pass.  Both synchronise. */
 int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, const mpf_solve_opts* opts,
                      mpf_solve_stats* stats);
+
+/* The configuration's noSensitivity mask (Solver::fromConfig, src/solver/solver.cpp:107-113:
+ * jacobian->elementwiseMultiply(jacobian, mask) after preSolve, before the regularisation
+ * is set).  d_mask: m x E FP32, column-major like the Jacobian (ld m), copied; null clears
+ * it.  The Jacobians of later mpf_gn_set_regularization (noser) and mpf_diff_prepare calls
+ * are multiplied by it elementwise; mpf_gn_step's Jacobian is not, as the reference's
+ * solveAbsolute recomputes it unmasked.  Invalidates a previous mpf_diff_prepare. */
+int mpf_set_sensitivity_mask(mpf_model* m, const float* d_mask);
 int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float* d_gamma);
 
 /* ---------------------------------------------------------------------------

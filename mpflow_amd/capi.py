@@ -23,7 +23,7 @@ EXPORTED = [
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
     "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian", "mpf_gn_set_regularization", "mpf_diff_prepare",
-    "mpf_diff_solve", "mpf_normal_solve",
+    "mpf_diff_solve", "mpf_normal_solve", "mpf_set_sensitivity_mask",
 ]
 
 
@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
     L.mpf_gn_set_regularization.argtypes = [vp, i32, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.mpf_diff_prepare.argtypes = [vp, vp, dbl, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.mpf_diff_solve.argtypes = [vp, vp, i32, vp]
+    L.mpf_set_sensitivity_mask.argtypes = [vp, vp]
     L.mpf_normal_solve.argtypes = [vp, i32, i32, vp, i32, dbl, vp, i32, vp]
     _lib = L
     return L

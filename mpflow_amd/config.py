@@ -24,8 +24,10 @@ Differences from the reference, each raising or documented:
     ``round(radius / outerEdgeLength)`` rings with nodes on the electrode corners; the
     inner edge length (distmesh's grading) is not reproduced.  The mesh is written to
     ``<path>/mesh`` as the reference does.
-  * voltage ("Fixed") sources, the laplacian regularisation and ``noSensitivity`` masks
-    raise ``NotImplementedError``.
+  * voltage ("Fixed") sources and the laplacian regularisation raise
+    ``NotImplementedError``.
+  * a ``noSensitivity`` file name (solver.cpp:107-113) is read as an (m, E) text matrix
+    and multiplied into the Jacobians of the noser weights and of preSolve.
   * port edge files index the mesh's edge list (``edges.txt``, or the unique vertex edges
     in ``mesh.unique_edges`` order when the file is absent); every port edge must lie on
     the boundary.
@@ -204,14 +206,15 @@ def parse_config(config, path: str = "."):
         gamma_ref = loadtxt(os.path.join(path, mat)).ravel()
     else:
         ref = _num(mat, 1.0)
-    if "noSensitivity" in model:
-        raise NotImplementedError("config: noSensitivity masks are not supported")
+    no_sens = None
+    if isinstance(model.get("noSensitivity"), str):
+        no_sens = loadtxt(os.path.join(path, model["noSensitivity"]))
     c25 = model.get("2.5D", {})
     spec = ModelSpec(nodes=nodes, elements=elements, boundary=boundary, ports=ports, drive=drive, meas=meas,
                      current=current, reference_value=ref, components=max(1, int(c25.get("components", 1))),
                      height=_num(c25.get("height"), 1.0),
                      port_height=_num(pcfg.get("height"), 1.0) if isinstance(pcfg, dict) else 1.0,
-                     gamma_ref=gamma_ref, edges=edges)
+                     gamma_ref=gamma_ref, edges=edges, no_sensitivity=no_sens)
     sol = {"parallel_images": max(1, int(solver.get("parallelImages", 1))),
            "regularization_factor": _num(solver.get("regularizationFactor"), 0.0),
            "regularization_type": solver.get("regularizationType", "identity")}
@@ -236,8 +239,9 @@ def eit_from_config(config, path: str = ".", ctx=None, frames: int = 1, **eit_kw
 
 def solver_from_config(config, path: str = ".", ctx=None, options=None, **eit_kw):
     """Solver::fromConfig: the forward model for ``parallelImages`` frames, the
-    regularisation, and preSolve at the reference distribution (the configured one, else
-    gamma = 1).  Returns the Solver; ``solver.eit`` is the model."""
+    noSensitivity mask, the regularisation, and preSolve at the reference distribution
+    (the configured one, else gamma = 1).  Returns the Solver; ``solver.eit`` is the
+    model."""
     import torch
 
     from .eit import EIT, Solver
@@ -246,6 +250,8 @@ def solver_from_config(config, path: str = ".", ctx=None, options=None, **eit_kw
               spec.reference_value, spec.components, spec.height, spec.port_height,
               frames=sol["parallel_images"], ctx=ctx, **eit_kw)
     s = Solver(eit, sol["regularization_factor"], options=options)
+    if spec.no_sensitivity is not None:
+        eit.set_sensitivity_mask(spec.no_sensitivity)
     g = None
     if spec.gamma_ref is not None:
         g = torch.tensor(spec.gamma_ref, dtype=torch.float32, device=eit._dev())

@@ -250,6 +250,15 @@ int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, cons
     });
 }
 
+int mpf_set_sensitivity_mask(mpf_model* m, const float* d_mask) {
+    return guarded([&] {
+        MPF_REQUIRE(m, "mpf_set_sensitivity_mask: null model");
+        set_device(m->impl->ctx);
+        m->impl->set_sensitivity_mask(d_mask);
+        return MPF_OK;
+    });
+}
+
 int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float* d_gamma) {
     return guarded([&] {
         MPF_REQUIRE(m && d_Umeas && d_gamma, "mpf_diff_solve: null argument");

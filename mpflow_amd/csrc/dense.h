@@ -64,6 +64,8 @@ void launch_col_sq(int rows, int cols, const float* A, int lda, double* d, hipSt
 // A_b[:, e] *= w[e]
 void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, int batch, const float* w,
                        hipStream_t st);
+// a[i] *= b[i], i < n
+void launch_mul(size_t n, float* a, const float* b, hipStream_t st);
 // v[b][e] *= w[e]
 void launch_scale_vec(int n, int batch, double* v, const float* w, hipStream_t st);
 // g[b][e] = g0[e] + d[b][e]

@@ -588,6 +588,12 @@ __global__ void __launch_bounds__(256) k_scale_cols(int rows, int cols, float* _
     A[blockIdx.y * strideA + r + e * lda] *= w[e];
 }
 
+// a[i] *= b[i]
+__global__ void __launch_bounds__(256) k_mul(size_t n, float* __restrict__ a, const float* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] *= b[i];
+}
+
 // v[b][e] *= w[e]
 __global__ void __launch_bounds__(256) k_scale_vec(int n, int batch, double* __restrict__ v,
                                                    const float* __restrict__ w) {
@@ -787,6 +793,12 @@ void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, in
                        hipStream_t st) {
     dim3 grid((unsigned)(((size_t)rows * cols + 255) / 256), batch);
     k_scale_cols<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, w);
+    check_launch();
+}
+
+void launch_mul(size_t n, float* a, const float* b, hipStream_t st) {
+    if (n == 0) return;
+    k_mul<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, b);
     check_launch();
 }
 

@@ -385,6 +385,7 @@ void Model::set_regularization(int type, const float* d_gamma_ref, const mpf_sol
     }
     const int rc = forward_at(d_gamma_ref, o, stats);
     MPF_REQUIRE(rc == MPF_OK, "regularization: the reference forward solve did not converge");
+    if (has_mask) launch_mul((size_t)m * E, Jm.p, sens_mask.p, st0);
     DBuf<double> d(E);
     launch_col_sq(m, E, Jm.p, m, d.p, st0);
     std::vector<double> hd(E);
@@ -419,6 +420,7 @@ void Model::diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solv
     MPF_HIP(hipMemcpyAsync(diff_J.p, Jm.p, sizeof(float) * (size_t)m * E, hipMemcpyDeviceToDevice, st0));
     MPF_HIP(hipMemcpyAsync(diff_U.p, U64.p, sizeof(double) * m, hipMemcpyDeviceToDevice, st0));
     MPF_HIP(hipMemcpyAsync(diff_gamma.p, d_gamma_ref, sizeof(float) * E, hipMemcpyDeviceToDevice, st0));
+    if (has_mask) launch_mul((size_t)m * E, diff_J.p, sens_mask.p, st0);
     if (reg_type == MPF_REG_NOSER) launch_scale_cols(m, E, diff_J.p, m, 0, 1, reg_w.p, st0);
     if (gram_nt_applies(m, m))
         launch_gram_nt(diff_J.p, 0, m, E, m, G.p, 0, 1, gram_work.p, st0);
@@ -432,6 +434,18 @@ void Model::diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solv
     MPF_HIP(hipStreamSynchronize(st0));
     if (hinfo) throw Error(MPF_EINVAL, "differential: normal-equation matrix is not positive definite");
     diff_ready = true;
+}
+
+// Solver::fromConfig's noSensitivity mask (solver.cpp:107-113)
+void Model::set_sensitivity_mask(const float* d_mask) {
+    diff_ready = false;
+    has_mask = false;
+    if (!d_mask) return;
+    const size_t n = (size_t)S.mrows * S.E;
+    if (!sens_mask.p) sens_mask.alloc(n);
+    MPF_HIP(hipMemcpyAsync(sens_mask.p, d_mask, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    MPF_HIP(hipStreamSynchronize(ctx->stream));
+    has_mask = true;
 }
 
 void Model::diff_solve(const double* d_Umeas, int gn_refine, float* d_gamma_out) {

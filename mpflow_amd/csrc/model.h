@@ -78,6 +78,8 @@ struct Model {
     // reference gamma, kept as per-element weights w = R^-1/2
     int reg_type = MPF_REG_IDENTITY;
     DBuf<float> reg_w;
+    DBuf<float> sens_mask;            // m x E, applied to the preparation Jacobians
+    bool has_mask = false;
     void set_regularization(int type, const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats);
     // differential solves (Solver::preSolve + solveDifferential, solver.cpp:136-188): U,
     // J and the factored normal matrix at a reference gamma, reused by every later call
@@ -86,6 +88,7 @@ struct Model {
     DBuf<float> diff_gamma, diff_J;
     DBuf<double> diff_U, diff_A;
     void diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solve_opts& o, mpf_solve_stats* stats);
+    void set_sensitivity_mask(const float* d_mask);
     void diff_solve(const double* d_Umeas, int gn_refine, float* d_gamma_out);
 
   private:

@@ -207,6 +207,22 @@ class EIT:
         """(frames, m, E) Jacobian (stored transposed-contiguous as E x m per frame)."""
         return self.jacobian.transpose(1, 2)
 
+    def set_sensitivity_mask(self, mask=None):
+        """The configuration's ``noSensitivity`` mask (Solver::fromConfig,
+        src/solver/solver.cpp:107-113): an (m, E) array multiplied elementwise into the
+        Jacobians of the noser regularisation and of the differential preparation; None
+        clears it."""
+        torch = self.torch
+        if mask is None:
+            check(lib().mpf_set_sensitivity_mask(self.h, None))
+            self._mask = None
+            return
+        t = torch.as_tensor(mask, dtype=torch.float32)
+        if tuple(t.shape) != (self.m, self.E):
+            raise ValueError("sensitivity mask: shape %s, expected (%d, %d)" % (tuple(t.shape), self.m, self.E))
+        self._mask = t.t().contiguous().to(self._dev())          # E x m: column-major m x E
+        check(lib().mpf_set_sensitivity_mask(self.h, C.c_void_p(self._mask.data_ptr())))
+
     def close(self):
         if getattr(self, "h", None):
             lib().mpf_model_destroy(self.h)

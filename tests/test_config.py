@@ -105,3 +105,16 @@ def test_parse_config_generated_mesh_and_rejections(tmp_path):
     bad["solver"]["regularizationType"] = "laplacian"
     with pytest.raises(NotImplementedError):
         cfg.parse_config(bad, str(tmp_path))
+
+
+def test_parse_config_no_sensitivity(tmp_path):
+    """Solver::fromConfig reads noSensitivity only when it is a file name (solver.cpp:108)."""
+    _mesh_dir(tmp_path)
+    mask = np.arange(12.0).reshape(3, 4) / 7.0
+    cfg.savetxt(str(tmp_path / "mask.txt"), mask)
+    spec, _ = cfg.parse_config(_config(noSensitivity="mask.txt"), str(tmp_path))
+    np.testing.assert_allclose(spec.no_sensitivity, mask, rtol=1e-15)
+    spec, _ = cfg.parse_config(_config(noSensitivity=1), str(tmp_path))
+    assert spec.no_sensitivity is None
+    spec, _ = cfg.parse_config(_config(), str(tmp_path))
+    assert spec.no_sensitivity is None

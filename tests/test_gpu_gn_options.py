@@ -89,6 +89,63 @@ def test_differential_solve_frames(gpu_ctx, gnprob, reg):
     assert max(errs) < 1e-2 and max(errg) < 1e-4 and dab < 1e-5
 
 
+def _masked_step(J, r, lam2, reg, mask):
+    """The differential step with the Jacobian multiplied by ``mask``: elements whose
+    masked column is zero keep delta = 0 (the reference's CG from 0 never leaves their
+    null space), the others solve the reduced regularised system."""
+    Jm = J * mask
+    keep = np.abs(Jm).sum(axis=0) > 0
+    Jk = Jm[:, keep]
+    R = eo.noser_diag(Jk) if reg == "noser" else np.ones(Jk.shape[1])
+    d = np.zeros(J.shape[1])
+    d[keep] = eo.gn_step_regularized(Jk, r, lam2, R)
+    return d
+
+
+def _mask(shape, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.uniform(0.5, 1.5, shape)
+    m[:, rng.random(shape[1]) < 0.2] = 0.0          # elements without sensitivity
+    m[rng.random(shape[0]) < 0.1, :] = 0.0          # and some measurements dropped
+    return m
+
+
+@pytest.mark.parametrize("reg", ["identity", "noser"])
+def test_differential_sensitivity_mask(gpu_ctx, gnprob, reg):
+    """noSensitivity (Solver::fromConfig, solver.cpp:107-113): the mask multiplies the
+    preSolve Jacobian (and so the noser weights); masked elements do not move."""
+    import torch
+    from mpflow_amd.eit import Solver
+    p, o, U1, J1, Um = gnprob
+    print("case: differential solve with a sensitivity mask, %s" % reg, flush=True)
+    eit = eit_of(p, gpu_ctx, frames=3)
+    mask = _mask(J1.shape, 11)
+    eit.set_sensitivity_mask(mask.astype(np.float32))
+    s = Solver(eit, LAM, gn_refine=4, options=_opts())
+    s.set_regularization(reg)
+    s.pre_solve()
+    s.measurement.copy_(torch.tensor(np.stack(Um), dtype=torch.float64))
+    s.solve_differential()
+    g = s.material_distribution.cpu().numpy().astype(np.float64)
+    m32 = mask.astype(np.float32).astype(np.float64)
+    errs = []
+    for f in range(3):
+        want = 1.0 + _masked_step(J1, Um[f] - U1, LAM ** 2, reg, m32)
+        errs.append(_rel(g[f], want))
+        dead = np.abs(J1 * m32).sum(axis=0) == 0
+        assert np.all(g[f][dead] == 1.0)
+    print("gamma rel err per frame", errs)
+    assert max(errs) < 1e-4
+    # clearing the mask restores the unmasked preparation
+    eit.set_sensitivity_mask(None)
+    s.set_regularization(reg)
+    s.pre_solve()
+    s.solve_differential()
+    g0 = s.material_distribution.cpu().numpy().astype(np.float64)
+    R = eo.noser_diag(J1) if reg == "noser" else np.ones(o.E)
+    assert _rel(g0[0], 1.0 + eo.gn_step_regularized(J1, Um[0] - U1, LAM ** 2, R)) < 1e-4
+
+
 def test_solver_from_config_differential(gpu_ctx, tmp_path):
     """Solver::fromConfig (solver.cpp:70-135) from a JSON file and mesh text files:
     preSolve at gamma = 1 with the configured regularisation, then one differential solve
