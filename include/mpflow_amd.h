@@ -56,6 +56,7 @@ extern "C" {
 
 #define MPF_REG_IDENTITY 0
 #define MPF_REG_NOSER    1
+#define MPF_REG_LAPLACIAN 2
 
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
@@ -265,7 +266,12 @@ int mpf_jacobian(mpf_model* m, const float* d_phi, int32_t ldphi, const float* d
  *                      computed here with one forward solve (opts) and kept, as the
  *                      reference keeps the matrix it computed when the type was set.
  *                      Entries below 1e-12 of the largest are raised to that floor.
- * The reference's laplacian type is not supported (MPF_EINVAL).  Synchronises. */
+ *   MPF_REG_LAPLACIAN: R = L^T L, L[edge][element] = +-|edge| over the elements' vertex
+ *                      edges (inverse.cpp:96-117); no forward solve.  The step is solved
+ *                      in E-space by FP64 CG on (J^T J + (ref lambda)^2 R) preconditioned by
+ *                      the data-space solve with R's diagonal, to a relative residual of
+ *                      1e-10 (at most 2000 iterations).
+ * Synchronises. */
 int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_ref,
                               const mpf_solve_opts* opts, mpf_solve_stats* stats);
 

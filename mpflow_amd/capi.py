@@ -12,7 +12,7 @@ MPF_STOP_REFERENCE, MPF_STOP_TOLERANCE = 0, 1
 MPF_PREC_FP32, MPF_PREC_MIXED = 0, 1
 MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB = 0, 1
 MPF_PRECOND_JACOBI, MPF_PRECOND_TWO_LEVEL = 0, 1
-MPF_REG_IDENTITY, MPF_REG_NOSER = 0, 1
+MPF_REG_IDENTITY, MPF_REG_NOSER, MPF_REG_LAPLACIAN = 0, 1, 2
 
 EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",

@@ -17,15 +17,14 @@ Schema (JSON integers are accepted wherever the reference reads doubles)::
         "material": ref | {"referenceValue": ref, "distribution": "gamma.txt"} | "gamma.txt",
         "2.5D": {"components": C, "height": h}},
      "solver": {"parallelImages": F, "regularizationFactor": lam,
-                "regularizationType": "identity" | "noser"}}
+                "regularizationType": "identity" | "noser" | "laplacian"}}
 
 Differences from the reference, each raising or documented:
   * the distmesh branch builds this package's ring mesh (``mesh.ring_mesh``) of
     ``round(radius / outerEdgeLength)`` rings with nodes on the electrode corners; the
     inner edge length (distmesh's grading) is not reproduced.  The mesh is written to
     ``<path>/mesh`` as the reference does.
-  * voltage ("Fixed") sources and the laplacian regularisation raise
-    ``NotImplementedError``.
+  * voltage ("Fixed") sources raise ``NotImplementedError``.
   * a ``noSensitivity`` file name (solver.cpp:107-113) is read as an (m, E) text matrix
     and multiplied into the Jacobians of the noser weights and of preSolve.
   * port edge files index the mesh's edge list (``edges.txt``, or the unique vertex edges
@@ -218,9 +217,7 @@ def parse_config(config, path: str = "."):
     sol = {"parallel_images": max(1, int(solver.get("parallelImages", 1))),
            "regularization_factor": _num(solver.get("regularizationFactor"), 0.0),
            "regularization_type": solver.get("regularizationType", "identity")}
-    if sol["regularization_type"] not in ("identity", "noser"):
-        if sol["regularization_type"] == "laplacian":
-            raise NotImplementedError("config: the laplacian regularisation is not supported")
+    if sol["regularization_type"] not in ("identity", "noser", "laplacian"):
         sol["regularization_type"] = "identity"   # solver.cpp:126-128: anything else is identity
     if gamma_ref is not None and gamma_ref.size != elements.shape[0]:
         raise ValueError("config: material distribution has %d entries for %d elements"

@@ -226,8 +226,8 @@ int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_r
                               mpf_solve_stats* stats) {
     return guarded([&] {
         MPF_REQUIRE(m, "mpf_gn_set_regularization: null model");
-        MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER,
-                    "mpf_gn_set_regularization: only identity and noser are supported");
+        MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER || type == MPF_REG_LAPLACIAN,
+                    "mpf_gn_set_regularization: unknown regularisation type");
         set_device(m->impl->ctx);
         mpf_solve_opts o;
         if (opts) o = *opts; else mpf_solve_opts_default(&o);

@@ -66,6 +66,19 @@ void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, in
                        hipStream_t st);
 // a[i] *= b[i], i < n
 void launch_mul(size_t n, float* a, const float* b, hipStream_t st);
+// E-space CG pieces of the laplacian-regularised step (batch columns of n entries)
+// out[b] = u[b] + c (x[b] + B x[b]), B[e] = sum over k < 3 of val[e][k] x[nb[e][k]] (nb < 0: none)
+void launch_lap_combine(int n, int batch, const double* u, double c, const double* x, const int32_t* nb,
+                        const double* val, double* out, hipStream_t st);
+// z = a (r - u)
+void launch_axmy(size_t n, double a, const double* r, const double* u, double* z, hipStream_t st);
+// out[b * ostride] = a[b] . c[b] (fixed-order FP64 sums)
+void launch_dots(int n, int batch, const double* a, const double* c, double* out, int ostride, hipStream_t st);
+// x[b] += alpha[b] p[b], r[b] -= alpha[b] q[b]
+void launch_cg_xr(int n, int batch, const double* alpha, double* x, const double* p, double* r,
+                  const double* q, hipStream_t st);
+// p[b] = z[b] + beta[b] p[b]
+void launch_cg_p(int n, int batch, const double* beta, double* p, const double* z, hipStream_t st);
 // v[b][e] *= w[e]
 void launch_scale_vec(int n, int batch, double* v, const float* w, hipStream_t st);
 // g[b][e] = g0[e] + d[b][e]

@@ -588,6 +588,63 @@ __global__ void __launch_bounds__(256) k_scale_cols(int rows, int cols, float* _
     A[blockIdx.y * strideA + r + e * lda] *= w[e];
 }
 
+// out[b][e] = u[b][e] + c (x[b][e] + sum_k val[e][k] x[b][nb[e][k]])
+__global__ void __launch_bounds__(256) k_lap_combine(int n, const double* __restrict__ u, double c,
+                                                     const double* __restrict__ x, const int32_t* __restrict__ nb,
+                                                     const double* __restrict__ val, double* __restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const size_t o = (size_t)blockIdx.y * n;
+    double bx = x[o + e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int f = nb[(size_t)e * 3 + k];
+        if (f >= 0) bx += val[(size_t)e * 3 + k] * x[o + f];
+    }
+    out[o + e] = u[o + e] + c * bx;
+}
+
+__global__ void __launch_bounds__(256) k_axmy(size_t n, double a, const double* __restrict__ r,
+                                              const double* __restrict__ u, double* __restrict__ z) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = a * (r[i] - u[i]);
+}
+
+// one block per batch column: strided FP64 partial sums, then a fixed tree
+__global__ void __launch_bounds__(256) k_dots(int n, const double* __restrict__ a, const double* __restrict__ c,
+                                              double* __restrict__ out, int ostride) {
+    __shared__ double red[256];
+    const size_t o = (size_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += a[o + i] * c[o + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)blockIdx.x * ostride] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_cg_xr(int n, const double* __restrict__ alpha, double* __restrict__ x,
+                                               const double* __restrict__ p, double* __restrict__ r,
+                                               const double* __restrict__ q) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const size_t i = (size_t)blockIdx.y * n + e;
+    const double al = alpha[blockIdx.y];
+    x[i] += al * p[i];
+    r[i] -= al * q[i];
+}
+
+__global__ void __launch_bounds__(256) k_cg_p(int n, const double* __restrict__ beta, double* __restrict__ p,
+                                              const double* __restrict__ z) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const size_t i = (size_t)blockIdx.y * n + e;
+    p[i] = z[i] + beta[blockIdx.y] * p[i];
+}
+
 // a[i] *= b[i]
 __global__ void __launch_bounds__(256) k_mul(size_t n, float* __restrict__ a, const float* __restrict__ b) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -793,6 +850,33 @@ void launch_scale_cols(int rows, int cols, float* A, int lda, size_t strideA, in
                        hipStream_t st) {
     dim3 grid((unsigned)(((size_t)rows * cols + 255) / 256), batch);
     k_scale_cols<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, w);
+    check_launch();
+}
+
+void launch_lap_combine(int n, int batch, const double* u, double c, const double* x, const int32_t* nb,
+                        const double* val, double* out, hipStream_t st) {
+    k_lap_combine<<<dim3((n + 255) / 256, batch), 256, 0, st>>>(n, u, c, x, nb, val, out);
+    check_launch();
+}
+
+void launch_axmy(size_t n, double a, const double* r, const double* u, double* z, hipStream_t st) {
+    k_axmy<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, r, u, z);
+    check_launch();
+}
+
+void launch_dots(int n, int batch, const double* a, const double* c, double* out, int ostride, hipStream_t st) {
+    k_dots<<<batch, 256, 0, st>>>(n, a, c, out, ostride);
+    check_launch();
+}
+
+void launch_cg_xr(int n, int batch, const double* alpha, double* x, const double* p, double* r,
+                  const double* q, hipStream_t st) {
+    k_cg_xr<<<dim3((n + 255) / 256, batch), 256, 0, st>>>(n, alpha, x, p, r, q);
+    check_launch();
+}
+
+void launch_cg_p(int n, int batch, const double* beta, double* p, const double* z, hipStream_t st) {
+    k_cg_p<<<dim3((n + 255) / 256, batch), 256, 0, st>>>(n, beta, p, z);
     check_launch();
 }
 

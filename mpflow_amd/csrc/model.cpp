@@ -323,7 +323,7 @@ void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int g
     // r = U_meas - U(gamma)  (Inverse::calcExcitation, inverse.cpp:163-193)
     launch_sub(m * frames, d_Umeas, U64.p, rgn.p, st0);
     // noser: (J^T J + lam^2 W^-2) delta = J^T r is solved as J W = Js, delta = W Js^T y
-    if (reg_type == MPF_REG_NOSER) launch_scale_cols(m, E, Jm.p, m, (size_t)m * E, frames, reg_w.p, st0);
+    if (reg_type != MPF_REG_IDENTITY) launch_scale_cols(m, E, Jm.p, m, (size_t)m * E, frames, reg_w.p, st0);
     // J J^T on MFMA, then (J J^T + lam I) by FP64 Cholesky
     if (gram_nt_applies(m, m))
         launch_gram_nt(Jm.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
@@ -336,7 +336,8 @@ void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int g
     normal_solve(Jm.p, (size_t)m * E, A64.p, (size_t)m * m, lam2, frames, gn_refine);
     // delta = J^T y ; gamma += delta   (solver.cpp:222)
     launch_gemv_t(m, E, Jm.p, m, (size_t)m * E, y.p, m, vE.p, E, frames, st0);
-    if (reg_type == MPF_REG_NOSER) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
+    if (reg_type == MPF_REG_LAPLACIAN) laplacian_cg(Jm.p, (size_t)m * E, A64.p, (size_t)m * m, lam2, frames);
+    if (reg_type != MPF_REG_IDENTITY) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
     launch_gamma_update(E * frames, d_gamma, vE.p, st0);
     int hinfo = 0;
     MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
@@ -358,6 +359,85 @@ void Model::normal_solve(const float* Jp, size_t strideJ, const double* A, size_
     }
 }
 
+void Model::laplacian_cg(const float* K, size_t strideK, const double* A, size_t strideA, double mu, int nb) {
+    hipStream_t st0 = ctx->stream;
+    const int m = S.mrows, E = S.E;
+    const size_t nE = (size_t)E * frames, nm = (size_t)m * frames;
+    if (!cg_r.p) {
+        cg_r.alloc(nE); cg_z.alloc(nE); cg_p.alloc(nE); cg_q.alloc(nE); cg_u.alloc(nE);
+        cg_t.alloc(nm); cg_s.alloc(nm); cg_dots.alloc((size_t)4 * frames);
+    }
+    double* x = vE.p;
+    const size_t NE = (size_t)E * nb;
+    // q = K^T K v + mu (v + B v)
+    auto apply = [&](const double* v, double* out) {
+        launch_gemv_n(m, E, K, m, strideK, v, E, 0.0, nullptr, 0, nullptr, 0, cg_t.p, m, nb, work_n.p, st0);
+        launch_gemv_t(m, E, K, m, strideK, cg_t.p, m, cg_u.p, E, nb, st0);
+        launch_lap_combine(E, nb, cg_u.p, mu, v, lap_nb.p, lap_bval.p, out, st0);
+    };
+    // z = (K^T K + mu I)^-1 v = (v - K^T (K K^T + mu I)^-1 K v) / mu
+    auto precond = [&](const double* v, double* z) {
+        launch_gemv_n(m, E, K, m, strideK, v, E, 0.0, nullptr, 0, nullptr, 0, cg_t.p, m, nb, work_n.p, st0);
+        launch_potrs(m, A, strideA, cg_t.p, m, cg_s.p, m, nb, st0);
+        launch_gemv_t(m, E, K, m, strideK, cg_s.p, m, cg_u.p, E, nb, st0);
+        launch_axmy(NE, 1.0 / mu, v, cg_u.p, z, st0);
+    };
+    std::vector<double> h((size_t)4 * nb);
+    auto fetch = [&]() {
+        MPF_HIP(hipMemcpyAsync(h.data(), cg_dots.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, st0));
+        MPF_HIP(hipStreamSynchronize(st0));
+    };
+    // |K^T r|^2 per column (the stopping reference)
+    launch_gemv_t(m, E, K, m, strideK, rgn.p, m, cg_q.p, E, nb, st0);
+    launch_dots(E, nb, cg_q.p, cg_q.p, cg_dots.p + 3, 4, st0);
+    // r = K^T (r_data - K x) - mu (x + B x)
+    launch_gemv_n(m, E, K, m, strideK, x, E, 0.0, nullptr, 0, rgn.p, m, cg_t.p, m, nb, work_n.p, st0);
+    launch_gemv_t(m, E, K, m, strideK, cg_t.p, m, cg_u.p, E, nb, st0);
+    launch_lap_combine(E, nb, cg_u.p, -mu, x, lap_nb.p, lap_bval.p, cg_r.p, st0);
+    precond(cg_r.p, cg_z.p);
+    MPF_HIP(hipMemcpyAsync(cg_p.p, cg_z.p, sizeof(double) * NE, hipMemcpyDeviceToDevice, st0));
+    launch_dots(E, nb, cg_r.p, cg_z.p, cg_dots.p + 0, 4, st0);
+    launch_dots(E, nb, cg_r.p, cg_r.p, cg_dots.p + 1, 4, st0);
+    fetch();
+    constexpr double kTol = 1e-10;
+    constexpr int kMaxIt = 2000;
+    std::vector<double> rz(nb), bb(nb), coef(nb);
+    std::vector<char> done(nb, 0);
+    DBuf<double> dcoef(nb);
+    int it = 0;
+    auto converged = [&]() {
+        bool all = true;
+        for (int b = 0; b < nb; ++b) {
+            if (!done[b] && !(h[4 * b + 1] > kTol * kTol * bb[b])) done[b] = 1;
+            all = all && done[b];
+        }
+        return all;
+    };
+    for (int b = 0; b < nb; ++b) { rz[b] = h[4 * b]; bb[b] = h[4 * b + 3]; }
+    while (!converged() && it < kMaxIt) {
+        ++it;
+        apply(cg_p.p, cg_q.p);
+        launch_dots(E, nb, cg_p.p, cg_q.p, cg_dots.p + 2, 4, st0);
+        fetch();
+        for (int b = 0; b < nb; ++b) coef[b] = (done[b] || !(h[4 * b + 2] > 0.0)) ? 0.0 : rz[b] / h[4 * b + 2];
+        dcoef.upload(coef.data(), nb, st0);
+        launch_cg_xr(E, nb, dcoef.p, x, cg_p.p, cg_r.p, cg_q.p, st0);
+        precond(cg_r.p, cg_z.p);
+        launch_dots(E, nb, cg_r.p, cg_z.p, cg_dots.p + 0, 4, st0);
+        launch_dots(E, nb, cg_r.p, cg_r.p, cg_dots.p + 1, 4, st0);
+        fetch();
+        for (int b = 0; b < nb; ++b) {
+            const double rzn = h[4 * b];
+            coef[b] = (done[b] || rz[b] == 0.0) ? 0.0 : rzn / rz[b];
+            rz[b] = rzn;
+        }
+        dcoef.upload(coef.data(), nb, st0);
+        launch_cg_p(E, nb, dcoef.p, cg_p.p, cg_z.p, st0);
+    }
+    MPF_HIP(hipStreamSynchronize(st0));
+    lap_iterations = it;
+}
+
 int Model::forward_at(const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats) {
     ensure_gn();
     for (int f = 0; f < frames; ++f)
@@ -368,12 +448,35 @@ int Model::forward_at(const float* d_gamma_ref, const mpf_solve_opts& o, mpf_sol
 
 void Model::set_regularization(int type, const float* d_gamma_ref, const mpf_solve_opts& o,
                                mpf_solve_stats* stats) {
-    MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER, "regularization: unknown type");
+    MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER || type == MPF_REG_LAPLACIAN,
+                "regularization: unknown type");
     reg_type = MPF_REG_IDENTITY;
     diff_ready = false;
     if (type == MPF_REG_IDENTITY) return;
     hipStream_t st0 = ctx->stream;
     const int m = S.mrows, E = S.E;
+    if (type == MPF_REG_LAPLACIAN) {
+        // R = L^T L from the mesh alone: w = diag(R)^-1/2 and B = W (R - diag R) W
+        std::vector<float> w(E);
+        for (int e = 0; e < E; ++e) {
+            MPF_REQUIRE(S.lap_diag[e] > 0.0, "regularization: degenerate element in the laplacian");
+            w[e] = (float)(1.0 / std::sqrt(S.lap_diag[e]));
+        }
+        std::vector<double> bv((size_t)E * 3, 0.0);
+        for (int e = 0; e < E; ++e)
+            for (int k = 0; k < 3; ++k) {
+                const int f = S.lap_nb[(size_t)e * 3 + k];
+                if (f >= 0) bv[(size_t)e * 3 + k] = S.lap_val[(size_t)e * 3 + k] * (double)w[e] * (double)w[f];
+            }
+        if (!reg_w.p) reg_w.alloc(E);
+        if (!lap_nb.p) { lap_nb.alloc((size_t)E * 3); lap_bval.alloc((size_t)E * 3); }
+        reg_w.upload(w.data(), E, st0);
+        lap_nb.upload(S.lap_nb.data(), (size_t)E * 3, st0);
+        lap_bval.upload(bv.data(), (size_t)E * 3, st0);
+        MPF_HIP(hipStreamSynchronize(st0));
+        reg_type = type;
+        return;
+    }
     std::vector<float> g1;
     DBuf<float> gref;
     if (!d_gamma_ref) {
@@ -421,7 +524,7 @@ void Model::diff_prepare(const float* d_gamma_ref, double lambda, const mpf_solv
     MPF_HIP(hipMemcpyAsync(diff_U.p, U64.p, sizeof(double) * m, hipMemcpyDeviceToDevice, st0));
     MPF_HIP(hipMemcpyAsync(diff_gamma.p, d_gamma_ref, sizeof(float) * E, hipMemcpyDeviceToDevice, st0));
     if (has_mask) launch_mul((size_t)m * E, diff_J.p, sens_mask.p, st0);
-    if (reg_type == MPF_REG_NOSER) launch_scale_cols(m, E, diff_J.p, m, 0, 1, reg_w.p, st0);
+    if (reg_type != MPF_REG_IDENTITY) launch_scale_cols(m, E, diff_J.p, m, 0, 1, reg_w.p, st0);
     if (gram_nt_applies(m, m))
         launch_gram_nt(diff_J.p, 0, m, E, m, G.p, 0, 1, gram_work.p, st0);
     else
@@ -455,7 +558,8 @@ void Model::diff_solve(const double* d_Umeas, int gn_refine, float* d_gamma_out)
     launch_sub_bcast(m, frames, d_Umeas, diff_U.p, rgn.p, st0);
     normal_solve(diff_J.p, 0, diff_A.p, 0, diff_lam2, frames, gn_refine);
     launch_gemv_t(m, E, diff_J.p, m, 0, y.p, m, vE.p, E, frames, st0);
-    if (reg_type == MPF_REG_NOSER) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
+    if (reg_type == MPF_REG_LAPLACIAN) laplacian_cg(diff_J.p, 0, diff_A.p, 0, diff_lam2, frames);
+    if (reg_type != MPF_REG_IDENTITY) launch_scale_vec(E, frames, vE.p, reg_w.p, st0);
     launch_gamma_from(E, frames, diff_gamma.p, vE.p, d_gamma_out, st0);
     MPF_HIP(hipStreamSynchronize(st0));
 }

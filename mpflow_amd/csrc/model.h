@@ -74,10 +74,15 @@ struct Model {
     void gn_begin(const float* d_gamma);
     void gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int gn_refine, double* d_Ucalc);
     // regularisation of the GN normal equations (Inverse::calcRegularizationMatrix,
-    // inverse.cpp:83-135): identity, or noser R = diag(J0^T J0) with J0 the Jacobian at a
-    // reference gamma, kept as per-element weights w = R^-1/2
+    // inverse.cpp:83-135): identity; noser R = diag(J0^T J0) with J0 the Jacobian at a
+    // reference gamma; laplacian R = L^T L.  Non-identity types keep per-element weights
+    // w = diag(R)^-1/2; the laplacian also keeps W (R - diag R) W per element (lap_nb,
+    // lap_bval) for its E-space CG
     int reg_type = MPF_REG_IDENTITY;
     DBuf<float> reg_w;
+    DBuf<int32_t> lap_nb;
+    DBuf<double> lap_bval;
+    int lap_iterations = 0;           // CG iterations of the last laplacian step
     DBuf<float> sens_mask;            // m x E, applied to the preparation Jacobians
     bool has_mask = false;
     void set_regularization(int type, const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats);
@@ -100,6 +105,11 @@ struct Model {
     // (stride 0 when shared), with gn_refine FP64-residual passes using J
     void normal_solve(const float* J, size_t strideJ, const double* A, size_t strideA, double lam2, int nb,
                       int gn_refine);
+    // laplacian: xi (in vE, started at K^T y) <- (K^T K + mu (I + B))^-1 K^T r for `nb`
+    // columns, K = J W (strideK), B = W (R - diag R) W, preconditioned by
+    // (K^T K + mu I)^-1 through the factored K K^T + mu I (A, strideA)
+    void laplacian_cg(const float* K, size_t strideK, const double* A, size_t strideA, double mu, int nb);
+    DBuf<double> cg_r, cg_z, cg_p, cg_q, cg_u, cg_t, cg_s, cg_dots;
     KrylovDriver& driver();
     std::unique_ptr<KrylovDriver> drv;
 };

@@ -108,6 +108,11 @@ struct MeshSetup {
     int coarse_nt = 0, coarse_ld = 0;   // tiles, padded row length of coarse_inv
     std::vector<float> coarse_inv;
     std::vector<double> coarse_d1;
+    // laplacian regularisation matrix: diagonal [E], neighbours across the three vertex
+    // edges [E][3] (-1: boundary edge) and their entries [E][3]
+    std::vector<double> lap_diag;
+    std::vector<int32_t> lap_nb;
+    std::vector<double> lap_val;
 };
 
 void build_setup(const mpf_model_desc& d, MeshSetup& s);

@@ -337,6 +337,40 @@ static void coarse_setup(MeshSetup& s) {
     s.coarse_ld = ld;
 }
 
+// Laplacian regularisation R = L^T L (Inverse::calcRegularizationMatrix,
+// src/solver/inverse.cpp:96-117): L[edge][element] = +|edge| for the first element that
+// uses an edge and -|edge| for the second, over the elements' vertex edges.  Hence
+// R_ee = sum of |edge|^2 over e's edges and R_ab = -|edge|^2 for elements a, b sharing
+// an edge; kept as the diagonal and up to three (neighbour, value) pairs per element.
+static void laplacian_setup(const mpf_model_desc& d, MeshSetup& s) {
+    const int E = d.n_elements, k = d.nodes_per_element;
+    s.lap_diag.assign(E, 0.0);
+    s.lap_nb.assign((size_t)E * 3, -1);
+    s.lap_val.assign((size_t)E * 3, 0.0);
+    std::map<std::pair<int, int>, std::pair<int, int>> first;   // edge -> (element, local edge)
+    for (int e = 0; e < E; ++e) {
+        const int32_t* el = d.elements + (size_t)e * k;
+        for (int i = 0; i < 3; ++i) {
+            const int a = el[i], b = el[(i + 1) % 3];
+            const double dx = d.nodes[2 * a] - d.nodes[2 * b], dy = d.nodes[2 * a + 1] - d.nodes[2 * b + 1];
+            const double l2 = dx * dx + dy * dy;
+            s.lap_diag[e] += l2;
+            const auto key = std::make_pair(std::min(a, b), std::max(a, b));
+            const auto it = first.find(key);
+            if (it == first.end()) {
+                first.emplace(key, std::make_pair(e, i));
+                continue;
+            }
+            const int f = it->second.first, j = it->second.second;
+            MPF_REQUIRE(s.lap_nb[(size_t)f * 3 + j] < 0, "laplacian: an edge is shared by more than two elements");
+            s.lap_nb[(size_t)f * 3 + j] = e;
+            s.lap_val[(size_t)f * 3 + j] = -l2;
+            s.lap_nb[(size_t)e * 3 + i] = f;
+            s.lap_val[(size_t)e * 3 + i] = -l2;
+        }
+    }
+}
+
 void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     const mpf_model_desc& c0 = caller;
     MPF_REQUIRE(c0.nodes && c0.elements && c0.boundary, "model: null mesh array");
@@ -625,6 +659,7 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         s.w_off[m + 1] = (int32_t)s.w_node.size();
     }
     coarse_setup(s);
+    laplacian_setup(d, s);
 }
 
 }  // namespace mpf

@@ -256,9 +256,11 @@ class Solver:
 
     def set_regularization(self, regularization_type: str = "identity", gamma_ref=None):
         """Inverse::setRegularizationParameter's type (src/solver/inverse.cpp:83-135):
-        "identity" or "noser" (R = diag(J0^T J0) at ``gamma_ref``, default the reference
-        distribution; computed now with one forward solve and kept)."""
-        types = {"identity": capi.MPF_REG_IDENTITY, "noser": capi.MPF_REG_NOSER}
+        "identity", "noser" (R = diag(J0^T J0) at ``gamma_ref``, default the reference
+        distribution; computed now with one forward solve and kept) or "laplacian"
+        (R = L^T L over the elements' shared edges, inverse.cpp:96-117)."""
+        types = {"identity": capi.MPF_REG_IDENTITY, "noser": capi.MPF_REG_NOSER,
+                 "laplacian": capi.MPF_REG_LAPLACIAN}
         if regularization_type not in types:
             raise ValueError("regularization type must be one of %s" % sorted(types))
         g = None

@@ -665,6 +665,39 @@ def gn_step_regularized(J, r, lam_eff2, R_diag):
     return sla.cho_solve(sla.cho_factor(A), J.T @ r)
 
 
+def laplacian_matrix(nodes, elements):
+    """RegularizationType::laplacian (Inverse::calcRegularizationMatrix, inverse.cpp:96-117):
+    L[edge, element] = sign(edge) * |edge| over each element's three vertex edges, the
+    edge's sign flipping after every use; R = L^T L (E x E, dense)."""
+    el = np.asarray(elements)[:, :3]
+    E = el.shape[0]
+    index = {}
+    elem_edges = np.empty((E, 3), np.int64)
+    for e in range(E):
+        for i in range(3):
+            a, b = int(el[e, i]), int(el[e, (i + 1) % 3])
+            key = (min(a, b), max(a, b))
+            if key not in index:
+                index[key] = len(index)
+            elem_edges[e, i] = index[key]
+    ends = np.array(sorted(index, key=index.get))
+    length = np.hypot(*(nodes[ends[:, 0]] - nodes[ends[:, 1]]).T)
+    L = np.zeros((len(index), E))
+    signs = np.ones(len(index))
+    for e in range(E):
+        for i in range(3):
+            k = elem_edges[e, i]
+            L[k, e] = signs[k] * length[k]
+            signs[k] *= -1.0
+    return L.T @ L
+
+
+def gn_step_general(J, r, lam_eff2, R):
+    """delta from (J^T J + lam R) delta = J^T r for a dense SPD R (small E only)."""
+    A = J.T @ J + lam_eff2 * R
+    return sla.cho_solve(sla.cho_factor(A), J.T @ r)
+
+
 def gn_step_cg_reference(J, r, lam_eff2, max_iter=0, eps=FLT_EPSILON):
     """Inverse::solve as written: A = J^T J + (sigma_ref lambda)^2 I, b = J^T r,
     unpreconditioned CG from 0 with the column-0 stop rule (inverse.cpp:212-238)."""

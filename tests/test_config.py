@@ -101,10 +101,12 @@ def test_parse_config_generated_mesh_and_rejections(tmp_path):
     assert os.path.exists(tmp_path / "mesh" / "elementEdges.txt")
     with pytest.raises(NotImplementedError):
         cfg.parse_config(_config(source={"type": "voltage", "value": 1.0}), str(tmp_path))
-    bad = _config()
-    bad["solver"]["regularizationType"] = "laplacian"
-    with pytest.raises(NotImplementedError):
-        cfg.parse_config(bad, str(tmp_path))
+    lap = _config()
+    lap["solver"]["regularizationType"] = "laplacian"
+    assert cfg.parse_config(lap, str(tmp_path))[1]["regularization_type"] == "laplacian"
+    other = _config()
+    other["solver"]["regularizationType"] = "tikhonov"
+    assert cfg.parse_config(other, str(tmp_path))[1]["regularization_type"] == "identity"
 
 
 def test_parse_config_no_sensitivity(tmp_path):

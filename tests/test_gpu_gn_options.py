@@ -89,6 +89,38 @@ def test_differential_solve_frames(gpu_ctx, gnprob, reg):
     assert max(errs) < 1e-2 and max(errg) < 1e-4 and dab < 1e-5
 
 
+@pytest.mark.parametrize("differential", [False, True])
+def test_laplacian_regularization(gpu_ctx, gnprob, differential):
+    """RegularizationType::laplacian (inverse.cpp:96-117): R = L^T L from the mesh; the
+    device's E-space CG (preconditioned by the data-space solve with R's diagonal) reaches
+    the oracle's dense solve of (J^T J + lam^2 R) delta = J^T r."""
+    import torch
+    from mpflow_amd.eit import Solver
+    p, o, U1, J1, Um = gnprob
+    print("case: laplacian regularisation, differential=%d" % differential, flush=True)
+    R = eo.laplacian_matrix(p["nodes"], p["elements"])
+    eit = eit_of(p, gpu_ctx, frames=3 if differential else 1)
+    s = Solver(eit, LAM, gn_refine=4, options=_opts())
+    s.set_regularization("laplacian")
+    if differential:
+        s.pre_solve()
+        s.measurement.copy_(torch.tensor(np.stack(Um), dtype=torch.float64))
+        s.solve_differential()
+        frames = range(3)
+    else:
+        s.measurement[0] = torch.tensor(Um[0], dtype=torch.float64)
+        s.solve_absolute()
+        frames = range(1)
+    g = s.material_distribution.cpu().numpy().astype(np.float64)
+    errs, errg = [], []
+    for f in frames:
+        want = 1.0 + eo.gn_step_general(J1, Um[f] - U1, LAM ** 2, R)
+        errs.append(_rel(g[f] - 1.0, want - 1.0))
+        errg.append(_rel(g[f], want))
+    print("delta rel err", errs, "gamma rel err", errg)
+    assert max(errs) < 1e-2 and max(errg) < 1e-4
+
+
 def _masked_step(J, r, lam2, reg, mask):
     """The differential step with the Jacobian multiplied by ``mask``: elements whose
     masked column is zero keep delta = 0 (the reference's CG from 0 never leaves their

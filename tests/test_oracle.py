@@ -597,3 +597,40 @@ def test_regularized_gn_step_pins():
     Js = J * w[None, :]
     want = w * eo.gn_step_exact(Js, r, lam)
     np.testing.assert_allclose(eo.gn_step_regularized(J, r, lam, d), want, rtol=1e-9, atol=1e-12)
+
+
+def test_laplacian_matrix_pins():
+    """R = L^T L of inverse.cpp:96-117 against its quadratic form written from the edge
+    list: x^T R x = sum over interior edges |e|^2 (x_a - x_b)^2 + sum over boundary edges
+    |e|^2 x_a^2, so R 1 carries the boundary-edge lengths only; also P2 meshes use their
+    vertex edges."""
+    nodes, el = ring_mesh(4, 8, 0.2)
+    R = eo.laplacian_matrix(nodes, el)
+    E = el.shape[0]
+    assert np.allclose(R, R.T)
+    users = {}
+    for e in range(E):
+        for i in range(3):
+            a, b = int(el[e, i]), int(el[e, (i + 1) % 3])
+            users.setdefault((min(a, b), max(a, b)), []).append(e)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        x = rng.standard_normal(E)
+        q = 0.0
+        for (a, b), es in users.items():
+            l2 = float(np.sum((nodes[a] - nodes[b]) ** 2))
+            q += l2 * ((x[es[0]] - x[es[1]]) ** 2 if len(es) == 2 else x[es[0]] ** 2)
+        assert math.isclose(float(x @ R @ x), q, rel_tol=1e-12)
+    bd = np.zeros(E)
+    for (a, b), es in users.items():
+        if len(es) == 1:
+            bd[es[0]] += float(np.sum((nodes[a] - nodes[b]) ** 2))
+    np.testing.assert_allclose(R @ np.ones(E), bd, atol=1e-14)
+    assert np.all(np.linalg.eigvalsh(R) > 0.0)
+    n2, e2 = promote_p2(nodes, el)
+    np.testing.assert_allclose(eo.laplacian_matrix(n2, e2), R, rtol=1e-12, atol=1e-15)
+    # the general step with R = I is the identity-regularised step
+    J = rng.standard_normal((10, E))
+    r = rng.standard_normal(10)
+    np.testing.assert_allclose(eo.gn_step_general(J, r, 0.2, np.eye(E)), eo.gn_step_exact(J, r, 0.2),
+                               rtol=1e-9, atol=1e-12)
