@@ -58,6 +58,10 @@ extern "C" {
 #define MPF_REG_NOSER    1
 #define MPF_REG_LAPLACIAN 2
 
+/* mpf_model_desc.source_type */
+#define MPF_SOURCE_CURRENT 0
+#define MPF_SOURCE_VOLTAGE 1
+
 typedef struct mpf_ctx mpf_ctx;
 typedef struct mpf_model mpf_model;
 typedef struct mpf_batch mpf_batch;
@@ -151,6 +155,14 @@ typedef struct mpf_model_desc {
                                      become extra unknowns, current enters through them and
                                      the voltages are the electrode potentials; each harmonic
                                      carries its own CEM system.  0: gap model (reference) */
+    int32_t source_type;          /* MPF_SOURCE_CURRENT (Open, the default) or MPF_SOURCE_VOLTAGE
+                                     (Fixed, Sources::Type, sources.h; eit.cpp:89-91,159-194):
+                                     the rows of every node with a clipped excitation
+                                     |B| >= 1e-9 become Dirichlet rows holding (B_clip Pi),
+                                     no beta scaling, no dcFree harmonic, and the result is
+                                     height * Mpat^T B_clip^T (A_c Phi_c) summed over the
+                                     harmonics (the electrode currents); Jacobian x height
+                                     (not -height).  Gap model only; jacobian_exact = 0 */
 } mpf_model_desc;
 
 typedef struct mpf_solve_opts {

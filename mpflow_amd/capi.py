@@ -13,6 +13,7 @@ MPF_PREC_FP32, MPF_PREC_MIXED = 0, 1
 MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB = 0, 1
 MPF_PRECOND_JACOBI, MPF_PRECOND_TWO_LEVEL = 0, 1
 MPF_REG_IDENTITY, MPF_REG_NOSER, MPF_REG_LAPLACIAN = 0, 1, 2
+MPF_SOURCE_CURRENT, MPF_SOURCE_VOLTAGE = 0, 1
 
 EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",
@@ -39,7 +40,7 @@ class ModelDesc(C.Structure):
         ("reference_value", C.c_double), ("components", C.c_int32), ("height", C.c_double),
         ("port_height", C.c_double), ("logarithmic", C.c_int32), ("quirks", C.c_int32),
         ("jacobian_exact", C.c_int32), ("ell_max_width", C.c_int32), ("frames", C.c_int32),
-        ("keep_node_order", C.c_int32), ("contact_impedance", C.c_double),
+        ("keep_node_order", C.c_int32), ("contact_impedance", C.c_double), ("source_type", C.c_int32),
     ]
 
 

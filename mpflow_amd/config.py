@@ -24,7 +24,8 @@ Differences from the reference, each raising or documented:
     ``round(radius / outerEdgeLength)`` rings with nodes on the electrode corners; the
     inner edge length (distmesh's grading) is not reproduced.  The mesh is written to
     ``<path>/mesh`` as the reference does.
-  * voltage ("Fixed") sources raise ``NotImplementedError``.
+  * ``"type": "voltage"`` gives Fixed sources (``value``: the applied voltages), any other
+    type current sources (sources.cpp:123-125).
   * a ``noSensitivity`` file name (solver.cpp:107-113) is read as an (m, E) text matrix
     and multiplied into the Jacobians of the noser weights and of preSolve.
   * port edge files index the mesh's edge list (``edges.txt``, or the unique vertex edges
@@ -182,8 +183,8 @@ def parse_config(config, path: str = "."):
     P = ports.shape[0]
 
     scfg = model.get("source", {})
-    if scfg.get("type", "current") == "voltage":
-        raise NotImplementedError("config: voltage (Fixed) sources are not supported")
+    # sources.cpp:123-125: "voltage" is Fixed, anything else Open (current)
+    source = "voltage" if scfg.get("type", "current") == "voltage" else "current"
     drive = np.asarray(scfg["drivePattern"], np.float64) if "drivePattern" in scfg else np.eye(P)
     meas = np.asarray(scfg["measurementPattern"], np.float64) if "measurementPattern" in scfg else np.eye(P)
     if drive.ndim == 1:
@@ -210,7 +211,8 @@ def parse_config(config, path: str = "."):
         no_sens = loadtxt(os.path.join(path, model["noSensitivity"]))
     c25 = model.get("2.5D", {})
     spec = ModelSpec(nodes=nodes, elements=elements, boundary=boundary, ports=ports, drive=drive, meas=meas,
-                     current=current, reference_value=ref, components=max(1, int(c25.get("components", 1))),
+                     current=current, source=source, reference_value=ref,
+                     components=max(1, int(c25.get("components", 1))),
                      height=_num(c25.get("height"), 1.0),
                      port_height=_num(pcfg.get("height"), 1.0) if isinstance(pcfg, dict) else 1.0,
                      gamma_ref=gamma_ref, edges=edges, no_sensitivity=no_sens)
@@ -231,7 +233,7 @@ def eit_from_config(config, path: str = ".", ctx=None, frames: int = 1, **eit_kw
     spec, _ = parse_config(config, path)
     return EIT(spec.nodes, spec.elements, spec.boundary, spec.ports, spec.drive, spec.meas, spec.current,
                spec.reference_value, spec.components, spec.height, spec.port_height, frames=frames, ctx=ctx,
-               **eit_kw)
+               source=spec.source, **eit_kw)
 
 
 def solver_from_config(config, path: str = ".", ctx=None, options=None, **eit_kw):
@@ -245,7 +247,7 @@ def solver_from_config(config, path: str = ".", ctx=None, options=None, **eit_kw
     spec, sol = parse_config(config, path)
     eit = EIT(spec.nodes, spec.elements, spec.boundary, spec.ports, spec.drive, spec.meas, spec.current,
               spec.reference_value, spec.components, spec.height, spec.port_height,
-              frames=sol["parallel_images"], ctx=ctx, **eit_kw)
+              frames=sol["parallel_images"], ctx=ctx, source=spec.source, **eit_kw)
     s = Solver(eit, sol["regularization_factor"], options=options)
     if spec.no_sensitivity is not None:
         eit.set_sensitivity_mask(spec.no_sensitivity)

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kEwBlock) k_bicg_init(const JobDev* __restrict
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
-        if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[(size_t)row * J.Rp + L.lane]);
+        if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[s * J.fstride + (size_t)row * J.Rp + L.lane]);
         J.x[i] = 0.0f;
         J.rv[i] = b;
         J.rh[i] = b;

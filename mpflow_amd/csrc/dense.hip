@@ -26,7 +26,9 @@ __global__ void __launch_bounds__(256) k_voltages(const JobDev* __restrict__ job
             for (int s = 0; s < J.C; ++s)
                 for (int q = md.w_off[m]; q < md.w_off[m + 1]; ++q) {
                     const size_t i = vec_index(J, s, md.w_node[q], md.colmap[d]);
-                    v += md.w_val[q] * md.colscale[d] * (use64 ? J.x64[i] : (double)J.x[i]);
+                    // voltage sources measure the electrode currents A_c Phi
+                    const double phi = J.cur ? J.cur[i] : (use64 ? J.x64[i] : (double)J.x[i]);
+                    v += md.w_val[q] * md.colscale[d] * phi;
                 }
         }
         if (U32) U32[(size_t)j * mrows + idx] = (float)v;

@@ -122,6 +122,62 @@ __global__ void __launch_bounds__(256) k_fem_jacobian(int E, int k, int D, int M
     }
 }
 
+// Voltage sources (applyMixedBoundaryCondition, eit.cpp:89-91,159-194): the device matrix
+// holds identity rows on D and no columns of D, so the excitation of slice s is b_D on D
+// and -A_s,iD b_D elsewhere, from the kept coupling contributions at this frame's sigma.
+__global__ void __launch_bounds__(256) k_fixed_rhs(const JobDev* __restrict__ jobs, FixDev fd) {
+    const JobDev& J = jobs[blockIdx.z];
+    const int s = blockIdx.y;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= fd.np * fd.Rp) return;
+    const int row = idx / fd.Rp, u = idx % fd.Rp;
+    double v = 0.0;
+    if (fd.drow_off[row + 1] > fd.drow_off[row]) {
+        v = fd.Fsh[idx];
+    } else {
+        const double al = J.alpha64[s];
+        for (int q = fd.cpl_off[row]; q < fd.cpl_off[row + 1]; ++q) {
+            const int el = fd.cpl_elem[q];
+            const double sg = el < 0 ? 1.0 : mat_param(J.logarithmic, J.ref, J.gamma[el]);
+            v -= sg * (fd.cpl_k[q] + al * fd.cpl_m[q]) * fd.Fsh[(size_t)fd.cpl_node[q] * fd.Rp + u];
+        }
+    }
+    const_cast<double*>(J.F)[s * J.fstride + idx] = v;
+}
+
+// electrode currents (eit.cpp:185-190): cur_s = A_s Phi_s on the rows of D with the full
+// (uneliminated) rows; the other rows of cur stay 0
+__global__ void __launch_bounds__(256) k_fixed_currents(const JobDev* __restrict__ jobs, FixDev fd, int use64) {
+    const JobDev& J = jobs[blockIdx.z];
+    const int s = blockIdx.y;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= fd.np * fd.Rp) return;
+    const int row = idx / fd.Rp, u = idx % fd.Rp;
+    const int q0 = fd.drow_off[row], q1 = fd.drow_off[row + 1];
+    if (q0 == q1) return;
+    const double al = J.alpha64[s];
+    double a = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        const int el = fd.drow_elem[q];
+        const double sg = el < 0 ? 1.0 : mat_param(J.logarithmic, J.ref, J.gamma[el]);
+        const size_t i = vec_index(J, s, fd.drow_node[q], u);
+        a += sg * (fd.drow_k[q] + al * fd.drow_m[q]) * (use64 ? J.x64[i] : (double)J.x[i]);
+    }
+    J.cur[vec_index(J, s, row, u)] = a;
+}
+
+void launch_fixed_rhs(const JobDev* d_jobs, int njobs, const FixDev& fd, hipStream_t st) {
+    dim3 grid((unsigned)((fd.np * fd.Rp + 255) / 256), fd.C, njobs);
+    k_fixed_rhs<<<grid, 256, 0, st>>>(d_jobs, fd);
+    MPF_HIP(hipGetLastError());
+}
+
+void launch_fixed_currents(const JobDev* d_jobs, int njobs, const FixDev& fd, int use64, hipStream_t st) {
+    dim3 grid((unsigned)((fd.np * fd.Rp + 255) / 256), fd.C, njobs);
+    k_fixed_currents<<<grid, 256, 0, st>>>(d_jobs, fd, use64);
+    MPF_HIP(hipGetLastError());
+}
+
 void fem_assemble(Model& M, const float* d_gamma, double alpha, uint32_t* d_ids, float* d_values) {
     const MeshSetup& S = M.S;
     MPF_REQUIRE(S.wpad <= kRefSlots, "fem_assemble: rows wider than the SparseMatrix's 32 slots");

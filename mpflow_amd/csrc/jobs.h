@@ -37,7 +37,9 @@ struct JobDev {
     float xscale[kMaxSlices];    // beta_c excitation scale
     double alpha64[kMaxSlices];  // the same two constants in FP64 for the refinement residual
     double xscale64[kMaxSlices];
-    const double* F;             // [np][Rp] excitation B*Pi, FP64
+    const double* F;             // [np][Rp] excitation B*Pi, FP64 (slice s at F + s * fstride)
+    size_t fstride;              // 0: one excitation for every slice (Open sources)
+    double* cur;                 // [C][np][Rp] voltage sources: A_c Phi on the Dirichlet rows
     float* rhs;                  // [C][np][Rp] FP32 right-hand side buffer (always valid)
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
     int ref_b;                   // 1: stop relative to ||beta_c F|| (SC_X2) instead of ||rhs||

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "model.h"
+#include "fem.h"
 
 std::pair<hipEvent_t, hipEvent_t> mpf_ctx::next_events() {
     if (ev_used == ev.size()) {
@@ -50,6 +51,14 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     ek.alloc(S.elem_k.size()); ek.upload(S.elem_k.data(), S.elem_k.size(), st0);
     em.alloc(S.elem_m.size()); em.upload(S.elem_m.data(), S.elem_m.size(), st0);
     F.alloc(S.F.size()); F.upload(S.F.data(), S.F.size(), st0);
+    if (S.fixed) {
+        auto up_i = [&](DBuf<int32_t>& b, const std::vector<int32_t>& v) { b.alloc(std::max<size_t>(v.size(), 1)); if (!v.empty()) b.upload(v.data(), v.size(), st0); };
+        auto up_d = [&](DBuf<double>& b, const std::vector<double>& v) { b.alloc(std::max<size_t>(v.size(), 1)); if (!v.empty()) b.upload(v.data(), v.size(), st0); };
+        up_i(cpl_off, S.cpl_off); up_i(cpl_node, S.cpl_node); up_i(cpl_elem, S.cpl_elem);
+        up_d(cpl_k, S.cpl_k); up_d(cpl_m, S.cpl_m);
+        up_i(drow_off, S.drow_off); up_i(drow_node, S.drow_node); up_i(drow_elem, S.drow_elem);
+        up_d(drow_k, S.drow_k); up_d(drow_m, S.drow_m);
+    }
     w_off.alloc(S.w_off.size()); w_off.upload(S.w_off.data(), S.w_off.size(), st0);
     w_node.alloc(std::max<size_t>(1, S.w_node.size())); if (!S.w_node.empty()) w_node.upload(S.w_node.data(), S.w_node.size(), st0);
     w_val.alloc(std::max<size_t>(1, S.w_val.size())); if (!S.w_val.empty()) w_val.upload(S.w_val.data(), S.w_val.size(), st0);
@@ -70,6 +79,11 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     x.alloc(V * frames); rv.alloc(V * frames); p.alloc(V * frames); q.alloc(V * frames);
     rhs.alloc(V * frames);
     x64.alloc(V * frames);
+    if (S.fixed) {
+        fslice.alloc((size_t)S.C * np * S.Rp * frames);
+        cur.alloc(V * frames);
+        MPF_HIP(hipMemsetAsync(cur.p, 0, sizeof(double) * V * frames, st0));
+    }
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
     gamma.alloc((size_t)S.E * frames);
@@ -103,6 +117,13 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
             J.alpha64[c] = S.alpha[c]; J.xscale64[c] = S.beta[c];
         }
         J.F = F.p;
+        if (S.fixed) {
+            // voltage sources: one excitation per slice, no beta scaling
+            J.F = fslice.p + (size_t)S.C * np * S.Rp * f;
+            J.fstride = (size_t)np * S.Rp;
+            J.cur = cur.p + V * f;
+            for (int c = 0; c < S.C; ++c) { J.xscale[c] = 1.0f; J.xscale64[c] = 1.0; }
+        }
         J.rhs = rhs.p + V * f;
         J.use_rhs = 0;
         J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f;
@@ -172,10 +193,12 @@ void Model::prepare(const mpf_solve_opts& o) {
     for (auto& J : hjobs) {
         J.dc = (o.krylov == MPF_KRYLOV_BICGSTAB && o.bicgstab_ignore_dc) ? -1 : 0;
         J.dc_rank1 = o.krylov == MPF_KRYLOV_CG ? 1 : 0;
+        if (S.fixed) { J.dc = -1; J.dc_rank1 = 0; }   // Dirichlet rows: no constant mode
     }
     upload_jobs();
     launch_assemble(jobs.p, asm_items.p, nasm, st0);
     launch_dinv(jobs.p, tiles.p, ntiles, st0);
+    if (S.fixed) launch_fixed_rhs(jobs.p, frames, fixdev(), st0);
     if (S.coarse_nt && o.preconditioner == MPF_PRECOND_TWO_LEVEL && o.krylov == MPF_KRYLOV_CG)
         launch_coarse_scale(jobs.p, tiles.p, ntiles, cd1.p, st0);
 }
@@ -203,6 +226,7 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
                 }
         stats->refine_residual = worst;
     }
+    if (S.fixed) launch_fixed_currents(jobs.p, frames, fixdev(), mixed ? 1 : 0, st0);
     MeasDev md{w_off.p, w_node.p, w_val.p, colmap.p, colscale.p, S.D, S.M, S.Dp, S.Mp};
     launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
     if (d_J) {
@@ -215,9 +239,22 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
         for (int c = 0; c < S.C; ++c) {
             jd.alpha[c] = (float)S.alpha[c];
             jd.weight[c] = S.jacobian_exact ? (float)(1.0 / S.beta[c]) : (float)S.height;
+            // voltage sources: jacobian x height without the current sources' sign
+            // (eit.cpp:202-205); the kernel applies -sigma' w
+            if (S.fixed) jd.weight[c] = -(float)S.height;
         }
         launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
     }
+}
+
+FixDev Model::fixdev() const {
+    FixDev fd;
+    fd.cpl_off = cpl_off.p; fd.cpl_node = cpl_node.p; fd.cpl_elem = cpl_elem.p; fd.cpl_k = cpl_k.p; fd.cpl_m = cpl_m.p;
+    fd.drow_off = drow_off.p; fd.drow_node = drow_node.p; fd.drow_elem = drow_elem.p;
+    fd.drow_k = drow_k.p; fd.drow_m = drow_m.p;
+    fd.Fsh = F.p;
+    fd.np = S.np; fd.Rp = S.Rp; fd.C = S.C;
+    return fd;
 }
 
 KrylovDriver& Model::driver() {

@@ -33,6 +33,10 @@ struct Model {
     int quirks = 1;
     // mesh data on the device
     DBuf<int32_t> ids, diag_slot, ctb_off, ctb_elem, elements, w_off, w_node, colmap;
+    // voltage sources (S.fixed): coupling / Dirichlet-row contributions, per-slice
+    // excitations [frames][C][np][Rp] and electrode currents [frames][C][np][Rp]
+    DBuf<int32_t> cpl_off, cpl_node, cpl_elem, drow_off, drow_node, drow_elem;
+    DBuf<double> cpl_k, cpl_m, drow_k, drow_m, fslice, cur;
     DBuf<int32_t> node_of_row;   // [np] device row -> caller's node (identity past the mesh nodes)
     DBuf<uint8_t> gw;
     DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
@@ -112,6 +116,7 @@ struct Model {
     DBuf<double> cg_r, cg_z, cg_p, cg_q, cg_u, cg_t, cg_s, cg_dots;
     KrylovDriver& driver();
     std::unique_ptr<KrylovDriver> drv;
+    struct FixDev fixdev() const;
 };
 
 int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_opts& o, float* const* d_U,

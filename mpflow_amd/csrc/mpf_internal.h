@@ -108,6 +108,16 @@ struct MeshSetup {
     int coarse_nt = 0, coarse_ld = 0;   // tiles, padded row length of coarse_inv
     std::vector<float> coarse_inv;
     std::vector<double> coarse_d1;
+    // Fixed (voltage) sources: Dirichlet rows D are identity rows of the device matrix and
+    // columns of D are eliminated; the eliminated couplings of each other row i
+    // (i, j in D) and the full rows of D are kept as element contributions (CSR over rows:
+    // column node, element, K_e and M_e entries) for the right-hand side
+    // -A_iD (B_clip Pi)_D and the electrode currents A_c Phi on D
+    int fixed = 0;
+    std::vector<int32_t> cpl_off, cpl_node, cpl_elem;
+    std::vector<double> cpl_k, cpl_m;
+    std::vector<int32_t> drow_off, drow_node, drow_elem;
+    std::vector<double> drow_k, drow_m;
     // laplacian regularisation matrix: diagonal [E], neighbours across the three vertex
     // edges [E][3] (-1: boundary edge) and their entries [E][3]
     std::vector<double> lap_diag;

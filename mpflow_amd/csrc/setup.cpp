@@ -322,7 +322,7 @@ static void coarse_setup(MeshSetup& s) {
                 if (s.ctb_off[ent] == s.ctb_off[ent + 1]) continue;   // padding slot
                 Ec[(size_t)(row / kTileRows) * nt + s.ids[ent] / kTileRows] += sk[ent] + s.alpha[c] * sm[ent];
             }
-        if (c == 0)
+        if (c == 0 && !s.fixed)
             for (int t = 0; t < nt; ++t)
                 for (int u = 0; u < nt; ++u) Ec[(size_t)t * nt + u] += cnt[t] * cnt[u];
         // tiles without real rows (none with the padded tile count) keep an identity block
@@ -369,6 +369,47 @@ static void laplacian_setup(const mpf_model_desc& d, MeshSetup& s) {
             s.lap_val[(size_t)e * 3 + i] = -l2;
         }
     }
+}
+
+// Fixed sources: rows D (a nonzero clipped excitation) become identity rows and the
+// columns of D leave the other rows, so the device solves the symmetric system
+// [I 0; 0 A_II] x = [b_D; -A_ID b_D] — the solution of the reference's row-replaced
+// system A_mod x = b.  The dropped contributions are kept per row: couplings of rows
+// outside D to columns in D (for the right-hand side) and the full rows of D (for the
+// electrode currents A_c x on D).
+static void fixed_setup(MeshSetup& s, const std::vector<std::map<int, double>>& Bn, int Nu) {
+    const int np = s.np, wp = s.wpad;
+    std::vector<char> dir(np, 0);
+    for (int n = 0; n < Nu; ++n)
+        for (auto& kv : Bn[n]) if (kv.second != 0.0) dir[n] = 1;
+    struct Ctb { int32_t node, elem; double k, m; };
+    std::vector<std::vector<Ctb>> cpl(np), drow(np);
+    std::vector<int32_t> off(1, 0), elem;
+    std::vector<double> ck, cm;
+    for (int q = 0; q < wp; ++q)
+        for (int row = 0; row < np; ++row) {
+            const size_t ent = (size_t)q * np + row;
+            const int col = s.ids[ent];
+            for (int i = s.ctb_off[ent]; i < s.ctb_off[ent + 1]; ++i) {
+                const Ctb c{col, s.ctb_elem[i], s.ctb_k[i], s.ctb_m[i]};
+                if (dir[row]) drow[row].push_back(c);
+                else if (dir[col]) cpl[row].push_back(c);
+                else { elem.push_back(c.elem); ck.push_back(c.k); cm.push_back(c.m); }
+            }
+            if (dir[row] && q == s.diag_slot[row]) { elem.push_back(-1); ck.push_back(1.0); cm.push_back(0.0); }
+            off.push_back((int32_t)elem.size());
+        }
+    s.ctb_off.swap(off); s.ctb_elem.swap(elem); s.ctb_k.swap(ck); s.ctb_m.swap(cm);
+    auto flatten = [&](const std::vector<std::vector<Ctb>>& v, std::vector<int32_t>& o, std::vector<int32_t>& nd,
+                       std::vector<int32_t>& el, std::vector<double>& kk, std::vector<double>& mm) {
+        o.assign(1, 0);
+        for (int row = 0; row < np; ++row) {
+            for (const Ctb& c : v[row]) { nd.push_back(c.node); el.push_back(c.elem); kk.push_back(c.k); mm.push_back(c.m); }
+            o.push_back((int32_t)nd.size());
+        }
+    };
+    flatten(cpl, s.cpl_off, s.cpl_node, s.cpl_elem, s.cpl_k, s.cpl_m);
+    flatten(drow, s.drow_off, s.drow_node, s.drow_elem, s.drow_k, s.drow_m);
 }
 
 void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
@@ -631,6 +672,19 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
             for (int q = 0; q < kb; ++q) Bn[ed[q]][p] += w[q] / length;
         }
     }
+    // Fixed sources (applyMixedBoundaryConditionKernel, eit_kernel.cu:34-54): the
+    // excitation is clipped to B_clip = (|B| >= 1e-9); every row with a nonzero clipped
+    // entry becomes an identity row
+    MPF_REQUIRE(d.source_type == MPF_SOURCE_CURRENT || d.source_type == MPF_SOURCE_VOLTAGE,
+                "model: unknown source type");
+    s.fixed = d.source_type == MPF_SOURCE_VOLTAGE ? 1 : 0;
+    if (s.fixed) {
+        MPF_REQUIRE(!cem, "model: voltage sources use the gap model (contact_impedance 0)");
+        MPF_REQUIRE(!d.jacobian_exact, "model: voltage sources use the reference Jacobian (jacobian_exact 0)");
+        for (int n = 0; n < Nu; ++n)
+            for (auto& kv : Bn[n]) kv.second = std::fabs(kv.second) >= 1e-9 ? 1.0 : 0.0;
+        fixed_setup(s, Bn, Nu);
+    }
     // F = B * Pi for the unique columns, Pi = [I*Dpat | Mpat]  (sources.cpp:57-71)
     s.F.assign((size_t)np * s.Rp, 0.0);
     auto Pi = [&](int p, int j) {
@@ -654,6 +708,8 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
             double v = 0.0;
             bool any = false;
             for (auto& kv : Bn[n]) { v += d.meas_pattern[(size_t)kv.first * s.M + m] * kv.second; any = true; }
+            // voltage sources measure height * (A_c Phi) through the clipped attachment
+            if (s.fixed) v *= d.height;
             if (any && v != 0.0) { s.w_node.push_back(n); s.w_val.push_back(v); }
         }
         s.w_off[m + 1] = (int32_t)s.w_node.size();

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
-        if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[(size_t)row * J.Rp + L.lane]);
+        if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[s * J.fstride + (size_t)row * J.Rp + L.lane]);
         const float z = dinv[row] * b;
         J.x[i] = 0.0f;
         J.rv[i] = b;
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
         }
         double b = 0.0, r = 0.0;
         if (row < J.n && L.lane < J.R) {
-            b = xs * J.F[(size_t)row * J.Rp + L.lane];
+            b = xs * J.F[s * J.fstride + (size_t)row * J.Rp + L.lane];
             r = b - acc - dcs;
         }
         out[vec_index(J, s, row, L.lane)] = (float)r;

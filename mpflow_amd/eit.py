@@ -91,12 +91,15 @@ class SolveOptions:
 
 
 class EIT:
-    """models::EIT with Open (current) sources on one mesh, for ``frames`` frames at once."""
+    """models::EIT on one mesh, for ``frames`` frames at once.  ``source`` is "current"
+    (Open sources, the default: ``current`` is the injected current per drive column) or
+    "voltage" (Fixed sources, eit.cpp:89-91,159-194: ``current`` is then the applied
+    voltage per drive column and the results are electrode currents)."""
 
     def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
                  quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None,
-                 keep_node_order=False, contact_impedance=0.0):
+                 keep_node_order=False, contact_impedance=0.0, source="current"):
         import torch
         self.ctx = ctx or Context()
         self.torch = torch
@@ -129,6 +132,10 @@ class EIT:
         d.ell_max_width, d.frames = int(ell_max_width), int(frames)
         d.keep_node_order = int(bool(keep_node_order))
         d.contact_impedance = float(contact_impedance)
+        if source not in ("current", "voltage"):
+            raise ValueError("source must be 'current' or 'voltage'")
+        d.source_type = capi.MPF_SOURCE_VOLTAGE if source == "voltage" else capi.MPF_SOURCE_CURRENT
+        self.source = source
         h = C.c_void_p()
         check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -518,7 +518,8 @@ def solve_direct(A, F, dc_free):
 # ---------------------------------------------------------------------------
 
 class EITOracle:
-    """FP64 restatement of ``models::EIT`` with Open (current) sources.
+    """FP64 restatement of ``models::EIT`` with Open (current) sources, or with Fixed
+    (voltage) sources when ``voltage`` (``current`` then holds the applied voltages).
 
     Parameters mirror ``EIT(mesh, sources, referenceValue, ..., components, height,
     portHeight)`` (src/models/eit.cpp:29-95).
@@ -526,7 +527,7 @@ class EITOracle:
 
     def __init__(self, nodes, elements, boundary, ports, drive, meas, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0,
-                 quirks=True, logarithmic=False, contact_impedance=0.0):
+                 quirks=True, logarithmic=False, contact_impedance=0.0, voltage=False):
         self.nodes = np.asarray(nodes, np.float64)
         self.elements = np.asarray(elements, np.int64)
         self.boundary = np.asarray(boundary, np.int64)
@@ -548,6 +549,9 @@ class EITOracle:
         self.Dp = ceil16(self.D)
         self.z = float(contact_impedance)
         self.L = self.ports.shape[0]
+        self.voltage = bool(voltage)
+        if self.voltage and self.z > 0.0:
+            raise ValueError("voltage sources use the gap model")
         if self.z > 0.0:
             # complete electrode model: unknowns [phi (N); V (L)]; current enters at the
             # electrode rows and the voltages are the electrode potentials.  Every harmonic
@@ -557,6 +561,12 @@ class EITOracle:
             self.W = np.hstack([np.zeros((self.M, self.N)), self.meas.T])
         else:
             self.B = excitation_matrix(self.nodes, self.boundary, self.ports, quirks)
+            if self.voltage:
+                # applyMixedBoundaryConditionKernel (eit_kernel.cu:34-54), called in the
+                # constructor before the attachment matrix (eit.cpp:89-94): the excitation
+                # is clipped to 0/1 and its nonzero rows become identity rows
+                self.B = (np.abs(self.B) >= 1e-9).astype(np.float64)
+                self.dirichlet = np.flatnonzero(self.B.any(axis=1))
             self.W = self.meas.T @ self.B.T          # electrodesAttachmentMatrix (eit.cpp:93-94)
 
     def alpha_beta(self, c):
@@ -577,12 +587,28 @@ class EITOracle:
                          [sp.csr_matrix(-self.Cc.T), sp.diags(self.Dc)]])
         return A.tocsr()
 
+    def mixed_system(self, A):
+        """The system matrix with the Dirichlet rows replaced by identity rows, as the
+        kernel writes them (values of the row: 1 on the diagonal, 0 elsewhere)."""
+        A = sp.lil_matrix(A)
+        for i in self.dirichlet:
+            A.rows[i] = [i]
+            A.data[i] = [1.0]
+        return A.tocsr()
+
     def fields(self, gamma, method="direct", **cg):
         """Phi_c (N x (D+M)) for every harmonic; also returns CG step counts."""
         out, steps = [], 0
         for c in range(self.C):
             A = self.system(gamma, c)
             _, beta = self.alpha_beta(c)
+            if self.voltage:
+                # eit.cpp:159-177: no beta, no dcFree; the row-replaced system solved directly
+                if method != "direct":
+                    raise ValueError("voltage sources: direct solves only")
+                X = spla.spsolve(self.mixed_system(A).tocsc(), self.B @ self.Pi)
+                out.append(np.asarray(X).reshape(self.N, -1))
+                continue
             F = beta * (self.B @ self.Pi)
             dc = (c == 0)
             if method == "direct":
@@ -594,16 +620,21 @@ class EITOracle:
             out.append(X)
         return out, steps
 
-    def voltages(self, phis):
-        """U = sum_c W Phi_c[:, :D] (M x D) (eit.cpp:192-198, 239-244)."""
+    def voltages(self, phis, gamma=None):
+        """U = sum_c W Phi_c[:, :D] (M x D) (eit.cpp:192-198, 239-244).  Voltage sources
+        (eit.cpp:181-190): U = sum_c W (height A_c Phi_c)[:, :D], the electrode currents,
+        with A_c the system matrix without the Dirichlet rows (``gamma`` needed)."""
         U = np.zeros((self.M, self.D))
-        for X in phis:
-            U += self.W @ X[:, :self.D]
+        for c, X in enumerate(phis):
+            if self.voltage:
+                U += self.W @ (self.h * (self.system(gamma, c) @ X[:, :self.D]))
+            else:
+                U += self.W @ X[:, :self.D]
         return U
 
     def solve(self, gamma, method="direct", **cg):
         phis, steps = self.fields(gamma, method, **cg)
-        return self.voltages(phis), phis, steps
+        return self.voltages(phis, gamma), phis, steps
 
     def jacobian(self, gamma, phis, quirks=None):
         """J[(d * ceil16(M) + m), e] (rows ceil16(D)*ceil16(M), padding rows 0).
@@ -613,6 +644,8 @@ class EITOracle:
         Exact: J = -sigma'_e * sum_c (1/beta_c) Phi_c,d^T (K_e + alpha_c M_e) Phi_c,m.
         """
         quirks = self.quirks if quirks is None else quirks
+        if self.voltage and not quirks:
+            raise ValueError("voltage sources: the reference Jacobian only")
         ds = mat_param_deriv(np.asarray(gamma, np.float64), self.ref, self.log)
         J = np.zeros((self.Dp, self.Mp, self.E))
         el = self.elements
@@ -624,7 +657,8 @@ class EITOracle:
             Pm = X[el][:, :, self.D:]            # [E, k, M]
             T = np.einsum("eij,ejm->eim", Kc, Pm)
             J[:self.D, :self.M, :] += w * np.einsum("eid,eim->dme", Pd, T)
-        J *= -ds[None, None, :]
+        # eit.cpp:202-205: the sign flip is for current sources only
+        J *= (1.0 if self.voltage else -1.0) * ds[None, None, :]
         return J.reshape(self.Dp * self.Mp, self.E)
 
     def flatten_voltages(self, U):

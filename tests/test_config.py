@@ -99,8 +99,8 @@ def test_parse_config_generated_mesh_and_rejections(tmp_path):
     spec, _ = cfg.parse_config(c, str(tmp_path))
     assert spec.nodes.shape[0] == 1 + 3 * 10 * 11        # 10 rings of 6j nodes
     assert os.path.exists(tmp_path / "mesh" / "elementEdges.txt")
-    with pytest.raises(NotImplementedError):
-        cfg.parse_config(_config(source={"type": "voltage", "value": 1.0}), str(tmp_path))
+    assert cfg.parse_config(_config(source={"type": "voltage", "value": 1.0}), str(tmp_path))[0].source == "voltage"
+    assert cfg.parse_config(_config(source={"type": "open", "value": 1.0}), str(tmp_path))[0].source == "current"
     lap = _config()
     lap["solver"]["regularizationType"] = "laplacian"
     assert cfg.parse_config(lap, str(tmp_path))[1]["regularization_type"] == "laplacian"

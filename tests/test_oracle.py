@@ -634,3 +634,53 @@ def test_laplacian_matrix_pins():
     r = rng.standard_normal(10)
     np.testing.assert_allclose(eo.gn_step_general(J, r, 0.2, np.eye(E)), eo.gn_step_exact(J, r, 0.2),
                                rtol=1e-9, atol=1e-12)
+
+
+def _voltage_oracle(C=1, rings=5, L=8, width=0.3, drive=None, scale=1.0, p2=False):
+    nodes, el = ring_mesh(rings, L, width)
+    if p2:
+        nodes, el = promote_p2(nodes, el)
+    bd = boundary_edges(el)
+    ports = eo.circular_boundary_ports(nodes, el, bd, L, width).astype(np.int32)
+    pat = eo.adjacent_pattern(L) if drive is None else drive
+    o = eo.EITOracle(nodes, el, bd, ports, pat, pat, 1.0, scale, C, 0.4 if C > 1 else 1.0,
+                     0.1 if C > 1 else 1.0, voltage=True)
+    return o, nodes, el
+
+
+def test_voltage_sources_pins():
+    """Fixed sources (eit.cpp:89-91,159-205; eit_kernel.cu:34-54) against what the physics
+    fixes: the row-replaced solve equals the symmetric elimination of the Dirichlet nodes;
+    the electrode-current map is symmetric (reciprocity, drive = measurement pattern);
+    equal potentials on every electrode draw no current at harmonic 0; the currents scale
+    with sigma; and the Jacobian is their finite-difference derivative (C = 1)."""
+    o, nodes, el = _voltage_oracle(C=2)
+    g = ellipse_phantom(nodes, el, frame_seed(3, 1))
+    U, phis, _ = o.solve(g)
+    D = o.dirichlet
+    I = np.setdiff1d(np.arange(o.N), D)
+    A = o.system(g, 1)
+    b = o.B @ o.Pi
+    x = np.zeros_like(b)
+    x[D] = b[D]
+    x[I] = spla.spsolve(A[I][:, I].tocsc(), -(A[I][:, D] @ b[D]))
+    np.testing.assert_allclose(phis[1], x, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(U, U.T, rtol=1e-9, atol=1e-12 * np.abs(U).max())
+    U2, _, _ = o.solve(2.0 * g)
+    np.testing.assert_allclose(U2, 2.0 * U, rtol=1e-10, atol=1e-14)
+    # one drive column puts 1 V on every electrode: no current at harmonic 0
+    dr = np.ones((8, 1))
+    oc, _, _ = _voltage_oracle(C=1, drive=dr)
+    Uc, _, _ = oc.solve(g)
+    assert np.abs(Uc).max() < 1e-10
+    # Jacobian = d U / d gamma (2D, one harmonic; linear parametrisation, ref = 1.3)
+    o1, _, _ = _voltage_oracle(C=1, scale=1.3)
+    U1, ph1, _ = o1.solve(g)
+    J = o1.jacobian(g, ph1)
+    eps = 1e-4          # central differences: truncation ~eps^2, solver round-off ~1e-14 / eps
+    for e in (0, 7, len(g) // 2, len(g) - 1):
+        gp, gm = g.copy(), g.copy()
+        gp[e] += eps
+        gm[e] -= eps
+        fd = (o1.flatten_voltages(o1.solve(gp)[0]) - o1.flatten_voltages(o1.solve(gm)[0])) / (2 * eps)
+        np.testing.assert_allclose(J[:, e], fd, rtol=1e-6, atol=1e-5 * np.abs(J[:, e]).max())
