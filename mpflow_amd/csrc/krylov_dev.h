@@ -5,6 +5,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "jobs.h"
 
@@ -164,10 +165,9 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
             // any is used; every slot reads both its memory gather and its LDS row and
             // weights them by v or 0, so that no branch holds a gather back (the memory
             // gather of an in-tile or padding slot is sent past the buffer range)
-            constexpr int G = 4;
-            static_assert(W % G == 0, "slot bins are multiples of 4");
-#pragma unroll 1
-            for (int q0 = 0; q0 < W; q0 += G) {
+            static_assert(W % 2 == 0, "slot bins are even");
+            auto group = [&](auto gsize, int q0) {
+                constexpr int G = decltype(gsize)::value;
                 v4u raw[G];
                 int2 e[G];
 #pragma unroll
@@ -188,14 +188,17 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
                     acc.z += vi * l.z + vo * __uint_as_float(raw[u][2]);
                     acc.w += vi * l.w + vo * __uint_as_float(raw[u][3]);
                 }
-            }
+            };
+#pragma unroll 1
+            for (int q0 = 0; q0 + 4 <= W; q0 += 4) group(std::integral_constant<int, 4>(), q0);
+            if constexpr (W % 4 == 2) group(std::integral_constant<int, 2>(), W - 2);
             epi(row0 + src, c4, acc);
             continue;
         }
-        // groups of GM slots: the group's gathers are all issued before any is used
-        constexpr int GM = 4;
-#pragma unroll 1
-        for (int q0 = 0; q0 < W; q0 += GM) {
+        // groups of 4 slots (a last group of 2 in the 10-slot bin): the group's gathers
+        // are all issued before any is used
+        auto group = [&](auto gsize, int q0) {
+            constexpr int GM = decltype(gsize)::value;
             v4u raw[GM];
             float v[GM];
 #pragma unroll
@@ -212,7 +215,11 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
                 acc.z += v[u] * __uint_as_float(raw[u][2]);
                 acc.w += v[u] * __uint_as_float(raw[u][3]);
             }
-        }
+        };
+        static_assert(W % 2 == 0, "slot bins are even");
+#pragma unroll 1
+        for (int q0 = 0; q0 + 4 <= W; q0 += 4) group(std::integral_constant<int, 4>(), q0);
+        if constexpr (W % 4 == 2) group(std::integral_constant<int, 2>(), W - 2);
         epi(row0 + src, c4, acc);
     }
 }
@@ -240,11 +247,13 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
         for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) xs[i] = X4[i];
     }
-    // slot-count bins of 4 from 12 up (P2 meshes: midpoint rows 9 wide, vertex rows 19)
+    // slot-count bins 10, 12, 16, 20, WMAX (P2 meshes: midpoint rows 9 wide, vertex rows 19)
     constexpr int WS = WMAX > 12 ? 12 : WMAX;
+    constexpr int W10 = WMAX > 10 ? 10 : WMAX;    // P2 edge-midpoint rows: 9 entries
     const int wave = threadIdx.x >> 6;
     const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
-    const int wb = wl <= WS ? WS : (wl <= 16 && WMAX > 16 ? 16 : (wl <= 20 && WMAX > 20 ? 20 : WMAX));
+    const int wb = (wl <= W10 && W10 < WS) ? W10
+                 : (wl <= WS ? WS : (wl <= 16 && WMAX > 16 ? 16 : (wl <= 20 && WMAX > 20 ? 20 : WMAX)));
     const int np = J.np;
     const float al = J.alpha[s];
     const MPF_GLOBAL int32_t* ids = gp(J.ids);
@@ -266,7 +275,8 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         stage[wave][q][r] = v;
     }
     __syncthreads();
-    if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
     else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
     else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
     else ell_gather_rows<LG, WMAX, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
