@@ -15,7 +15,8 @@ namespace mpf {
 
 // scalar fields per (slice, lane), FP64
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
-               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB, SC_NFIELDS };
+               SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB,
+               SC_ALPHA1, SC_PQ1, SC_NFIELDS };
 // per-slice integer state: [0] active, [1] steps, [2] hit cap or breakdown,
 // [3] BiCGSTAB: the next iteration is the first; CG: [0] as the update kernel of the
 // current iteration saw it (the stop rule clears [0] during the pupdate launch)
@@ -44,6 +45,7 @@ struct JobDev {
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
     int ref_b;                   // 1: stop relative to ||beta_c F|| (SC_X2) instead of ||rhs||
     float* x; float* rv; float* p; float* q;   // [C][np][Rp] (q = nu in BiCGSTAB)
+    float* p2;                   // [C][np][Rp] PCG: p of odd iterations (p of even ones in p)
     // BiCGSTAB only (null otherwise): r-hat, y = K^-1 p, s, z = K^-1 s, t   [C][np][Rp]
     float* rh; float* yv; float* sv; float* zv; float* tv;
     double* x64;                 // [C][np][Rp] FP64 refined solution
@@ -113,7 +115,9 @@ void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, 
 
 // the SpMV of launch_pcg_iter alone (autotuning)
 void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
-                     hipStream_t st);
+                     hipStream_t st, int par = 0);
+// PCG: the x update that a slice stopping at an even iteration still owes
+void launch_pcg_flush(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                        const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,

@@ -262,6 +262,7 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
         }
     }
     drop_graph();
+    if (!bicg) launch_pcg_flush(jobs.p, init_tiles.p, n_init_tiles, s0);
     bool capped = false;
     int64_t slice_iters = 0;
     double bytes = 0.0;

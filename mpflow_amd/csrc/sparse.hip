@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
 template <int LG, int WMAX>
 __global__ void __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_spmv4(const JobDev* __restrict__ jobs,
-                                                        const int4* __restrict__ tiles) {
+                                                        const int4* __restrict__ tiles, int par) {
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;
     const int4 t = tiles[blockIdx.x];
@@ -163,7 +163,8 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
     } sh;
     __shared__ XTile<LG> xs;
     const int tile0 = t.z * kTileRows;
-    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, J.p + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
+    // p of this iteration: even iterations in p, odd ones in p2
+    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, (par ? J.p2 : J.p) + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
         const float4 own = xs[(row - tile0) * LPR + c4];
         Q4[(size_t)row * LPR + c4] = acc;
         v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
@@ -176,7 +177,7 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
 
 template <int LG, int WMAX, int U>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
-                                                       const int4* __restrict__ tiles) {
+                                                       const int4* __restrict__ tiles, int par) {
     constexpr int Rp = 1 << LG;
     constexpr int G = 64 / Rp;                       // rows per wavefront step
     constexpr int STEPS = kSpmvRowsPerWave / G;      // steps per wavefront
@@ -205,12 +206,13 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
             vr[q] = Sv[e] + al * Rv[e];
         }
     }
-    const MPF_GLOBAL float* P = gp((const float*)J.p) + (size_t)s * np * Rp;
+    const float* Pbase = par ? J.p2 : J.p;    // p of this iteration
+    const MPF_GLOBAL float* P = gp(Pbase) + (size_t)s * np * Rp;
     MPF_GLOBAL float* Q = gp(J.q) + (size_t)s * np * Rp;
     // gathers go through a buffer resource: 32-bit offsets, reads past the slice return 0
     // without a memory access (used for the zero-valued padding slots)
     const uint32_t range = (uint32_t)((size_t)np * Rp * sizeof(float));
-    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(J.p + (size_t)s * np * Rp, (size_t)range);
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(Pbase + (size_t)s * np * Rp, (size_t)range);
     double pq = 0.0, ps = 0.0;
     for (int i = 0; i < STEPS; i += U) {
         int src[U];
@@ -296,9 +298,11 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restr
             sh_dc[lane] = ps;
             sh_zero[lane] = pq == 0.0;
             if (t.z == 0) {
-                sc_ptr(J, SC_PQ, s)[lane] = pq;
+                // alpha and p.q by iteration parity: pupdate applies two iterations' x
+                // updates at once
+                sc_ptr(J, par ? SC_PQ1 : SC_PQ, s)[lane] = pq;
                 sc_ptr(J, SC_PSUM, s)[lane] = ps;
-                sc_ptr(J, SC_ALPHA, s)[lane] = al;
+                sc_ptr(J, par ? SC_ALPHA1 : SC_ALPHA, s)[lane] = al;
             }
         }
         __syncthreads();
@@ -488,8 +492,12 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
     }
 }
 
-// pupdate: x += alpha p (alpha and p.q as the update kernel left them; a zero p.q zeroes
-// x); rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p.  Tile 0's
+// pupdate: rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p, written to
+// the other direction buffer (p of even iterations in p, of odd ones in p2).  x moves
+// every second iteration: after an odd iteration k, x += alpha_{k-1} p_{k-1} +
+// alpha_k p_k (alpha and p.q as the update kernels left them; a zero p.q zeroes x, as
+// updateVectorKernel does at every iteration); a slice that stops after an even iteration
+// gets its last update from k_pcg_flush.  Tile 0's
 // block also sums r.r, applies the stop rule (ConjugateGradient::solve,
 // src/numeric/conjugate_gradient.cpp:118-150) and counts the step.  The blocks test the
 // slice state the update kernel recorded for this iteration, so a slice that stops here
@@ -568,36 +576,70 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
         }
     }
     const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
-    float b[4], a[4];
-    bool zero[4], xzero[4];
+    float b[4], a0[4], a1[4];
+    bool zero[4], xz0[4], xz1[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         b[j] = (float)sh_b[4 * c4 + j];
         zero[j] = sh_bz[4 * c4 + j] != 0;
-        xzero[j] = sc_ptr(J, SC_PQ, s)[4 * c4 + j] == 0.0;
-        a[j] = (float)sc_ptr(J, SC_ALPHA, s)[4 * c4 + j];
+        xz0[j] = sc_ptr(J, SC_PQ, s)[4 * c4 + j] == 0.0;
+        xz1[j] = sc_ptr(J, SC_PQ1, s)[4 * c4 + j] == 0.0;
+        a0[j] = (float)sc_ptr(J, SC_ALPHA, s)[4 * c4 + j];
+        a1[j] = (float)sc_ptr(J, SC_ALPHA1, s)[4 * c4 + j];
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Rr = (const MPF_GLOBAL float4*)gp((const float*)J.rv) + base;
-    MPF_GLOBAL float4* P = (MPF_GLOBAL float4*)gp(J.p) + base;
+    // p_k is read from this iteration's buffer, p_{k+1} written to the other one
+    const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)(par ? J.p2 : J.p)) + base;
+    MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(par ? J.p : J.p2) + base;
     MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
     // two-level: z = D^-1 r + y_tile on the tile's real rows
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (J.cinv) y4 = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))[((size_t)s * J.ntiles + t.z) * LPR + c4];
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
-        const float4 r4 = Rr[i], p4 = P[i], x4 = X[i];
+        const float4 r4 = Rr[i], p4 = Pk[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
-        X[i] = make_float4(xzero[0] ? 0.0f : x4.x + a[0] * p4.x, xzero[1] ? 0.0f : x4.y + a[1] * p4.y,
-                           xzero[2] ? 0.0f : x4.z + a[2] * p4.z, xzero[3] ? 0.0f : x4.w + a[3] * p4.w);
-        P[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
-                           zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
-                           zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
-                           zero[3] ? 0.0f : d * r4.w + w * y4.w + b[3] * p4.w);
+        if (par) {
+            // p_{k-1} sits where p_{k+1} goes: read before the write below
+            const float4 q4 = Pn[i], x4 = X[i];
+            float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+            const float pp[4] = {q4.x, q4.y, q4.z, q4.w}, pc[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float x1 = xz0[j] ? 0.0f : xv[j] + a0[j] * pp[j];
+                xv[j] = xz1[j] ? 0.0f : x1 + a1[j] * pc[j];
+            }
+            X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+        }
+        Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
+                            zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
+                            zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
+                            zero[3] ? 0.0f : d * r4.w + w * y4.w + b[3] * p4.w);
     }
 }
+
+// The x update owed by slices that stopped after an even iteration k (their step count
+// is odd): x += alpha_k p_k (zero when p.q was zero), p_k in p.
+__global__ void __launch_bounds__(kUpdBlock) k_pcg_flush(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles) {
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int s = t.y;
+    if ((gp(J.st)[s * kStInts + 1] & 1) == 0) return;
+    const int Rp = J.Rp;
+    for (int i = threadIdx.x; i < kTileRows * Rp; i += kUpdBlock) {
+        const int lane = i % Rp, row = t.z * kTileRows + i / Rp;
+        const double pq = sc_ptr(J, SC_PQ, s)[lane];
+        const float a = (float)sc_ptr(J, SC_ALPHA, s)[lane];
+        const size_t v = vec_index(J, s, row, lane);
+        J.x[v] = pq == 0.0 ? 0.0f : J.x[v] + a * J.p[v];
+    }
+}
+
+
 
 // --------------------------------------------------------------------------------
 // FP64 refinement: residual r64 = beta F - (A64 + dc) x64 written as the FP32 RHS
@@ -778,35 +820,41 @@ void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, 
 // v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner).
 template <int LG>
 static void spmv_dispatch(int wpad, int variant, const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                          hipStream_t st) {
+                          hipStream_t st, int par) {
     constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
     constexpr int U2 = STEPS >= 2 ? 2 : 1;
     if (variant == 0) {
         switch (wpad) {
-            case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
-            case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
-            case 24: k_pcg_spmv4<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); return;
+            case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
+            case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
+            case 24: k_pcg_spmv4<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
             default: break;   // wider rows use the 32-bit variant (no register spills)
         }
     }
     switch (wpad) {
-        case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 24: k_pcg_spmv<LG, 24, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 32: k_pcg_spmv<LG, 32, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 40: k_pcg_spmv<LG, 40, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 48: k_pcg_spmv<LG, 48, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 56: k_pcg_spmv<LG, 56, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
-        case 64: k_pcg_spmv<LG, 64, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles); break;
+        case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 24: k_pcg_spmv<LG, 24, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 32: k_pcg_spmv<LG, 32, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 40: k_pcg_spmv<LG, 40, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 48: k_pcg_spmv<LG, 48, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 56: k_pcg_spmv<LG, 56, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 64: k_pcg_spmv<LG, 64, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
         default: throw Error(MPF_EINVAL, "spmv: unsupported padded ELL width");
     }
 }
 
+void launch_pcg_flush(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st) {
+    if (ntiles == 0) return;
+    k_pcg_flush<<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles);
+    check_launch();
+}
+
 void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
-                     hipStream_t st) {
-    if (lg == 6) spmv_dispatch<6>(wpad, variant, d_jobs, d_tiles, ntiles, st);
-    else if (lg == 5) spmv_dispatch<5>(wpad, variant, d_jobs, d_tiles, ntiles, st);
-    else if (lg == 4) spmv_dispatch<4>(wpad, variant, d_jobs, d_tiles, ntiles, st);
+                     hipStream_t st, int par) {
+    if (lg == 6) spmv_dispatch<6>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
+    else if (lg == 5) spmv_dispatch<5>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
+    else if (lg == 4) spmv_dispatch<4>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
     else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
 }
 
@@ -815,7 +863,7 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
                      const int2* d_js, int njs, int nchunk) {
     const int par = iter & 1;
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
-    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st);
+    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st, par);
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
     MPF_REQUIRE(d_js && njs > 0 && nchunk > 0, "pcg iteration: no (job, slice) list");
     if (lg == 6) chunks_launch<6>(d_jobs, d_js, njs, nchunk, st);
