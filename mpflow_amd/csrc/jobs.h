@@ -16,7 +16,10 @@ namespace mpf {
 // scalar fields per (slice, lane), FP64
 enum ScField { SC_RHO = 0, SC_PQ, SC_PSUM, SC_BB, SC_RR, SC_ALPHA, SC_BETA, SC_BZERO,
                SC_RHO0, SC_RS, SC_V, SC_W, SC_OMEGA, SC_X0, SC_X1, SC_X2, SC_RHO1, SC_MB,
-               SC_ALPHA1, SC_PQ1, SC_NFIELDS };
+               SC_PA0, SC_PA1, SC_PA2, SC_PA3, SC_PQ0, SC_PQ1, SC_PQ2, SC_PQ3, SC_NFIELDS };
+// PCG keeps the last kRing search directions: p of iteration k in buffer k % kRing, and
+// alpha / p.q of iteration k in SC_PA0 / SC_PQ0 + k % kRing; x moves every kRing iterations
+constexpr int kRing = 4;
 // per-slice integer state: [0] active, [1] steps, [2] hit cap or breakdown,
 // [3] BiCGSTAB: the next iteration is the first; CG: [0] as the update kernel of the
 // current iteration saw it (the stop rule clears [0] during the pupdate launch)
@@ -45,7 +48,7 @@ struct JobDev {
     int use_rhs;                 // 1: PCG starts from rhs; 0: from beta_c * F
     int ref_b;                   // 1: stop relative to ||beta_c F|| (SC_X2) instead of ||rhs||
     float* x; float* rv; float* p; float* q;   // [C][np][Rp] (q = nu in BiCGSTAB)
-    float* p2;                   // [C][np][Rp] PCG: p of odd iterations (p of even ones in p)
+    float* pr[kRing - 1];        // [C][np][Rp] PCG: p of iterations k % kRing = 1.. (0: in p)
     // BiCGSTAB only (null otherwise): r-hat, y = K^-1 p, s, z = K^-1 s, t   [C][np][Rp]
     float* rh; float* yv; float* sv; float* zv; float* tv;
     double* x64;                 // [C][np][Rp] FP64 refined solution
@@ -69,6 +72,9 @@ struct JobDev {
     float* csc;                  // [ntiles] Sc^-1/2 (written after each assembly)
     float* cy;                   // [C][ntiles][Rp] coarse correction Einv W^T r
 };
+
+// PCG search direction of iteration k (k4 = k % kRing)
+__host__ __device__ inline float* pbuf(const JobDev& J, int k4) { return k4 == 0 ? J.p : J.pr[k4 - 1]; }
 
 __host__ __device__ inline size_t vec_index(const JobDev& J, int s, int row, int lane) {
     return ((size_t)s * J.np + row) * J.Rp + lane;

@@ -198,7 +198,7 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
         if (!bicg) tune(g);
     }
     int every = o.check_every > 0 ? o.check_every : 16;
-    every += every & 1;                       // even: iteration parity restarts each group
+    every = (every + kRing - 1) / kRing * kRing;   // whole direction-ring cycles per poll
     const bool graphs = o.graphs != 0 && !ctx->prof;
     int maxit = 0;
     for (const auto& J : h) maxit = std::max(maxit, J.maxit);

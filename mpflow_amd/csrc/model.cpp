@@ -76,7 +76,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     const int ntile_job = np / kTileRows;
     s.alloc(ell * frames); r.alloc(ell * frames); s_lo.alloc(ell * frames); r_lo.alloc(ell * frames);
     dinv.alloc((size_t)S.C * np * frames);
-    x.alloc(V * frames); rv.alloc(V * frames); p.alloc(V * frames); q.alloc(V * frames); p2.alloc(V * frames);
+    x.alloc(V * frames); rv.alloc(V * frames); p.alloc(V * frames); q.alloc(V * frames);
+    pring.alloc(V * frames * (kRing - 1));
     rhs.alloc(V * frames);
     x64.alloc(V * frames);
     if (S.fixed) {
@@ -126,7 +127,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         }
         J.rhs = rhs.p + V * f;
         J.use_rhs = 0;
-        J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f; J.p2 = p2.p + V * f;
+        J.x = x.p + V * f; J.rv = rv.p + V * f; J.p = p.p + V * f; J.q = q.p + V * f;
+        for (int b = 0; b < kRing - 1; ++b) J.pr[b] = pring.p + V * frames * b + V * f;
         J.x64 = x64.p + V * f;
         J.part = part.p + (size_t)kParts * S.C * ntile_job * S.Rp * f;
         J.sc = sc.p + (size_t)SC_NFIELDS * S.C * S.Rp * f;

@@ -44,7 +44,7 @@ struct Model {
     DBuf<double> cd1;
     DBuf<float> ek, em;
     // per-frame state
-    DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, p2, rhs;
+    DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, pring, rhs;
     DBuf<float> rh, yv, sv, zv, tv;   // BiCGSTAB vectors (allocated on first use)
     DBuf<double> x64, part, sc;
     DBuf<float> gamma;

@@ -88,7 +88,7 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
     const size_t ell = (size_t)wpad * np, V = (size_t)np * Rp;
     DBuf<int32_t> ids(ell), diag(np), used(np);
     DBuf<uint8_t> gw(np / 16);
-    DBuf<float> s(ell), zero(ell), dinv(np), x(V), rv(V), p(V), q(V), p2(V), rhs(V);
+    DBuf<float> s(ell), zero(ell), dinv(np), x(V), rv(V), p(V), q(V), pring(V * (kRing - 1)), rhs(V);
     DBuf<double> F(V), x64(V), part((size_t)kParts * ntiles * Rp), sc((size_t)SC_NFIELDS * Rp);
     DBuf<float> rh, yv, sv, zv, tv;
     if (bicg) { rh.alloc(V); yv.alloc(V); sv.alloc(V); zv.alloc(V); tv.alloc(V); }
@@ -109,7 +109,8 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
     J.ids = ids.p; J.s = s.p; J.r = zero.p; J.s_lo = zero.p; J.r_lo = zero.p; J.dinv = dinv.p;
     J.alpha[0] = 0.0f; J.xscale[0] = 1.0f; J.alpha64[0] = 0.0; J.xscale64[0] = 1.0;
     J.F = F.p; J.rhs = rhs.p; J.use_rhs = 1;
-    J.x = x.p; J.rv = rv.p; J.p = p.p; J.q = q.p; J.p2 = p2.p;
+    J.x = x.p; J.rv = rv.p; J.p = p.p; J.q = q.p;
+    for (int b = 0; b < kRing - 1; ++b) J.pr[b] = pring.p + V * b;
     J.rh = rh.p; J.yv = yv.p; J.sv = sv.p; J.zv = zv.p; J.tv = tv.p;
     J.x64 = x64.p; J.part = part.p; J.sc = sc.p;
     J.diag_slot = diag.p; J.gw = gw.p;

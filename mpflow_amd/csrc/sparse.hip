@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __re
 template <int LG, int WMAX>
 __global__ void __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_spmv4(const JobDev* __restrict__ jobs,
-                                                        const int4* __restrict__ tiles, int par) {
+                                                        const int4* __restrict__ tiles, int k4) {
     constexpr int Rp = 1 << LG;
     constexpr int LPR = Rp / 4;
     const int4 t = tiles[blockIdx.x];
@@ -163,8 +163,8 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
     } sh;
     __shared__ XTile<LG> xs;
     const int tile0 = t.z * kTileRows;
-    // p of this iteration: even iterations in p, odd ones in p2
-    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, (par ? J.p2 : J.p) + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
+    // k4: the iteration index modulo kRing, which selects this iteration's p
+    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, pbuf(J, k4) + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
         const float4 own = xs[(row - tile0) * LPR + c4];
         Q4[(size_t)row * LPR + c4] = acc;
         v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
@@ -177,7 +177,7 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
 
 template <int LG, int WMAX, int U>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
-                                                       const int4* __restrict__ tiles, int par) {
+                                                       const int4* __restrict__ tiles, int k4) {
     constexpr int Rp = 1 << LG;
     constexpr int G = 64 / Rp;                       // rows per wavefront step
     constexpr int STEPS = kSpmvRowsPerWave / G;      // steps per wavefront
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
             vr[q] = Sv[e] + al * Rv[e];
         }
     }
-    const float* Pbase = par ? J.p2 : J.p;    // p of this iteration
+    const float* Pbase = pbuf(J, k4);         // p of this iteration (k4 = k % kRing)
     const MPF_GLOBAL float* P = gp(Pbase) + (size_t)s * np * Rp;
     MPF_GLOBAL float* Q = gp(J.q) + (size_t)s * np * Rp;
     // gathers go through a buffer resource: 32-bit offsets, reads past the slice return 0
@@ -266,8 +266,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
 // slice that stops in this iteration still takes its last x update.
 template <int LG>
 __global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
-                                                           const int4* __restrict__ tiles, int par) {
+                                                           const int4* __restrict__ tiles, int iter) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
+    const int par = iter & 1, k4 = iter & (kRing - 1);
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
@@ -298,11 +299,11 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restr
             sh_dc[lane] = ps;
             sh_zero[lane] = pq == 0.0;
             if (t.z == 0) {
-                // alpha and p.q by iteration parity: pupdate applies two iterations' x
+                // alpha and p.q per ring slot: pupdate applies kRing iterations' x
                 // updates at once
-                sc_ptr(J, par ? SC_PQ1 : SC_PQ, s)[lane] = pq;
+                sc_ptr(J, SC_PQ0 + k4, s)[lane] = pq;
                 sc_ptr(J, SC_PSUM, s)[lane] = ps;
-                sc_ptr(J, par ? SC_ALPHA1 : SC_ALPHA, s)[lane] = al;
+                sc_ptr(J, SC_PA0 + k4, s)[lane] = al;
             }
         }
         __syncthreads();
@@ -493,20 +494,22 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
 }
 
 // pupdate: rho' = r.z, beta = rho' / rho (rho == 0 zeroes p), p = z + beta p, written to
-// the other direction buffer (p of even iterations in p, of odd ones in p2).  x moves
-// every second iteration: after an odd iteration k, x += alpha_{k-1} p_{k-1} +
-// alpha_k p_k (alpha and p.q as the update kernels left them; a zero p.q zeroes x, as
-// updateVectorKernel does at every iteration); a slice that stops after an even iteration
-// gets its last update from k_pcg_flush.  Tile 0's
+// the next ring buffer (p of iteration k in buffer k % kRing).  x moves every kRing
+// iterations: after iteration k = kRing - 1 (mod kRing), x takes the updates
+// alpha_i p_i of the last kRing iterations in order (alpha and p.q as the update kernels
+// left them; a zero p.q zeroes x, as updateVectorKernel does at every iteration); a slice
+// that stops elsewhere in the cycle gets the updates it is owed from k_pcg_flush.
+// Tile 0's
 // block also sums r.r, applies the stop rule (ConjugateGradient::solve,
 // src/numeric/conjugate_gradient.cpp:118-150) and counts the step.  The blocks test the
 // slice state the update kernel recorded for this iteration, so a slice that stops here
 // still has every tile's x updated (its p is updated too and not read again).
-template <int LG>
+template <int LG, bool XUP>
 __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
-                                                            const int4* __restrict__ tiles, int par,
+                                                            const int4* __restrict__ tiles, int iter,
                                                             int* __restrict__ active) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
+    const int par = iter & 1, k4 = iter & (kRing - 1);
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
@@ -576,44 +579,67 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
         }
     }
     const int c4 = threadIdx.x % LPR, r0 = threadIdx.x / LPR;
-    float b[4], a0[4], a1[4];
-    bool zero[4], xz0[4], xz1[4];
+    float b[4];
+    bool zero[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         b[j] = (float)sh_b[4 * c4 + j];
         zero[j] = sh_bz[4 * c4 + j] != 0;
-        xz0[j] = sc_ptr(J, SC_PQ, s)[4 * c4 + j] == 0.0;
-        xz1[j] = sc_ptr(J, SC_PQ1, s)[4 * c4 + j] == 0.0;
-        a0[j] = (float)sc_ptr(J, SC_ALPHA, s)[4 * c4 + j];
-        a1[j] = (float)sc_ptr(J, SC_ALPHA1, s)[4 * c4 + j];
     }
     const size_t base = (size_t)s * J.np * LPR;
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Rr = (const MPF_GLOBAL float4*)gp((const float*)J.rv) + base;
-    // p_k is read from this iteration's buffer, p_{k+1} written to the other one
-    const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)(par ? J.p2 : J.p)) + base;
-    MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(par ? J.p : J.p2) + base;
-    MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
+    // p_k is read from this iteration's buffer, p_{k+1} written to the next one
+    const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)pbuf(J, k4)) + base;
+    MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(pbuf(J, (k4 + 1) & (kRing - 1))) + base;
     // two-level: z = D^-1 r + y_tile on the tile's real rows
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (J.cinv) y4 = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))[((size_t)s * J.ntiles + t.z) * LPR + c4];
+    if constexpr (XUP) {
+        // k4 == kRing - 1: the cycle's x updates, oldest first; p of the cycle's first iteration sits where
+        // p_{k+1} goes and is read before the write
+        float a[kRing][4];
+        bool xz[kRing][4];
+#pragma unroll
+        for (int q = 0; q < kRing; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[q][j] = (float)sc_ptr(J, SC_PA0 + q, s)[4 * c4 + j];
+                xz[q][j] = sc_ptr(J, SC_PQ0 + q, s)[4 * c4 + j] == 0.0;
+            }
+        MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
+        const MPF_GLOBAL float4* Pq[kRing - 1];
+#pragma unroll
+        for (int q = 0; q < kRing - 1; ++q) Pq[q] = (const MPF_GLOBAL float4*)gp((const float*)pbuf(J, q)) + base;
+        for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
+            const size_t i = (size_t)row * LPR + c4;
+            const float4 r4 = Rr[i], p4 = Pk[i], x4 = X[i];
+            float4 pq4[kRing];
+#pragma unroll
+            for (int q = 0; q < kRing - 1; ++q) pq4[q] = Pq[q][i];
+            pq4[kRing - 1] = p4;
+            float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+            for (int q = 0; q < kRing; ++q) {
+                const float pv[4] = {pq4[q].x, pq4[q].y, pq4[q].z, pq4[q].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[j] = xz[q][j] ? 0.0f : xv[j] + a[q][j] * pv[j];
+            }
+            X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
+            const float d = dinv[row];
+            const float w = row < J.n ? 1.0f : 0.0f;
+            Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
+                                zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
+                                zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
+                                zero[3] ? 0.0f : d * r4.w + w * y4.w + b[3] * p4.w);
+        }
+        return;
+    }
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
         const float4 r4 = Rr[i], p4 = Pk[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
-        if (par) {
-            // p_{k-1} sits where p_{k+1} goes: read before the write below
-            const float4 q4 = Pn[i], x4 = X[i];
-            float xv[4] = {x4.x, x4.y, x4.z, x4.w};
-            const float pp[4] = {q4.x, q4.y, q4.z, q4.w}, pc[4] = {p4.x, p4.y, p4.z, p4.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float x1 = xz0[j] ? 0.0f : xv[j] + a0[j] * pp[j];
-                xv[j] = xz1[j] ? 0.0f : x1 + a1[j] * pc[j];
-            }
-            X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
-        }
         Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                             zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                             zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
@@ -621,21 +647,26 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
     }
 }
 
-// The x update owed by slices that stopped after an even iteration k (their step count
-// is odd): x += alpha_k p_k (zero when p.q was zero), p_k in p.
+// The x updates owed by a slice whose step count n is not a multiple of kRing: those of
+// its last n % kRing iterations (ring slots 0 ..), in order, as pupdate would apply them.
 __global__ void __launch_bounds__(kUpdBlock) k_pcg_flush(const JobDev* __restrict__ jobs,
                                                          const int4* __restrict__ tiles) {
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    if ((gp(J.st)[s * kStInts + 1] & 1) == 0) return;
+    const int m = gp(J.st)[s * kStInts + 1] & (kRing - 1);
+    if (m == 0) return;
     const int Rp = J.Rp;
     for (int i = threadIdx.x; i < kTileRows * Rp; i += kUpdBlock) {
         const int lane = i % Rp, row = t.z * kTileRows + i / Rp;
-        const double pq = sc_ptr(J, SC_PQ, s)[lane];
-        const float a = (float)sc_ptr(J, SC_ALPHA, s)[lane];
         const size_t v = vec_index(J, s, row, lane);
-        J.x[v] = pq == 0.0 ? 0.0f : J.x[v] + a * J.p[v];
+        float xv = J.x[v];
+        for (int q = 0; q < m; ++q) {
+            const double pq = sc_ptr(J, SC_PQ0 + q, s)[lane];
+            const float a = (float)sc_ptr(J, SC_PA0 + q, s)[lane];
+            xv = pq == 0.0 ? 0.0f : xv + a * pbuf(J, q)[v];
+        }
+        J.x[v] = xv;
     }
 }
 
@@ -820,26 +851,26 @@ void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, 
 // v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner).
 template <int LG>
 static void spmv_dispatch(int wpad, int variant, const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                          hipStream_t st, int par) {
+                          hipStream_t st, int k4) {
     constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
     constexpr int U2 = STEPS >= 2 ? 2 : 1;
     if (variant == 0) {
         switch (wpad) {
-            case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
-            case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
-            case 24: k_pcg_spmv4<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); return;
+            case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
+            case 16: k_pcg_spmv4<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
+            case 24: k_pcg_spmv4<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
             default: break;   // wider rows use the 32-bit variant (no register spills)
         }
     }
     switch (wpad) {
-        case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 24: k_pcg_spmv<LG, 24, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 32: k_pcg_spmv<LG, 32, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 40: k_pcg_spmv<LG, 40, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 48: k_pcg_spmv<LG, 48, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 56: k_pcg_spmv<LG, 56, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
-        case 64: k_pcg_spmv<LG, 64, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, par); break;
+        case 8:  k_pcg_spmv<LG, 8, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 16: k_pcg_spmv<LG, 16, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 24: k_pcg_spmv<LG, 24, U2><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 32: k_pcg_spmv<LG, 32, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 40: k_pcg_spmv<LG, 40, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 48: k_pcg_spmv<LG, 48, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 56: k_pcg_spmv<LG, 56, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
+        case 64: k_pcg_spmv<LG, 64, 1><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); break;
         default: throw Error(MPF_EINVAL, "spmv: unsupported padded ELL width");
     }
 }
@@ -851,32 +882,38 @@ void launch_pcg_flush(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hip
 }
 
 void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int lg, int wpad, int variant,
-                     hipStream_t st, int par) {
-    if (lg == 6) spmv_dispatch<6>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
-    else if (lg == 5) spmv_dispatch<5>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
-    else if (lg == 4) spmv_dispatch<4>(wpad, variant, d_jobs, d_tiles, ntiles, st, par);
+                     hipStream_t st, int k4) {
+    if (lg == 6) spmv_dispatch<6>(wpad, variant, d_jobs, d_tiles, ntiles, st, k4);
+    else if (lg == 5) spmv_dispatch<5>(wpad, variant, d_jobs, d_tiles, ntiles, st, k4);
+    else if (lg == 4) spmv_dispatch<4>(wpad, variant, d_jobs, d_tiles, ntiles, st, k4);
     else throw Error(MPF_EINVAL, "spmv: unsupported lane count");
 }
 
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                      int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
                      const int2* d_js, int njs, int nchunk) {
-    const int par = iter & 1;
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
-    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st, par);
+    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st, iter & (kRing - 1));
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
     MPF_REQUIRE(d_js && njs > 0 && nchunk > 0, "pcg iteration: no (job, slice) list");
     if (lg == 6) chunks_launch<6>(d_jobs, d_js, njs, nchunk, st);
     else if (lg == 5) chunks_launch<5>(d_jobs, d_js, njs, nchunk, st);
     else chunks_launch<4>(d_jobs, d_js, njs, nchunk, st);
-    if (lg == 6) k_pcg_update4<6><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
-    else if (lg == 5) k_pcg_update4<5><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
-    else k_pcg_update4<4><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par);
+    if (lg == 6) k_pcg_update4<6><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter);
+    else if (lg == 5) k_pcg_update4<5><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter);
+    else k_pcg_update4<4><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter);
     // coarse correction where the job has one; chunk sums of r.z and r.r for every job
     launch_coarse_all(d_jobs, d_js, njs, nchunk, lg, 2, 3, st);
-    if (lg == 6) k_pcg_pupdate4<6><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
-    else if (lg == 5) k_pcg_pupdate4<5><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
-    else k_pcg_pupdate4<4><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, par, d_active);
+    // the last iteration of each direction-ring cycle also moves x
+    if ((iter & (kRing - 1)) == kRing - 1) {
+        if (lg == 6) k_pcg_pupdate4<6, true><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+        else if (lg == 5) k_pcg_pupdate4<5, true><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+        else k_pcg_pupdate4<4, true><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+    } else {
+        if (lg == 6) k_pcg_pupdate4<6, false><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+        else if (lg == 5) k_pcg_pupdate4<5, false><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+        else k_pcg_pupdate4<4, false><<<ntiles, kUpdBlock, 0, st>>>(d_jobs, d_tiles, iter, d_active);
+    }
     check_launch();
 }
 
