@@ -243,10 +243,6 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
                                                 EllStage<WMAX>& stage, Epi&& epi, float4* xs = nullptr) {
     constexpr int LPR = (1 << LG) / 4;
     const int tile0 = row0 / kTileRows * kTileRows;
-    if constexpr (XT) {
-        const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
-        for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) xs[i] = X4[i];
-    }
     // slot-count bins 10, 12, 16, 20, WMAX (P2 meshes: midpoint rows 9 wide, vertex rows 19)
     constexpr int WS = WMAX > 12 ? 12 : WMAX;
     constexpr int W10 = WMAX > 10 ? 10 : WMAX;    // P2 edge-midpoint rows: 9 entries
@@ -259,20 +255,53 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
     const MPF_GLOBAL int32_t* ids = gp(J.ids);
     const MPF_GLOBAL float* Sv = gp((const float*)J.s);
     const MPF_GLOBAL float* Rv = gp((const float*)J.r);
-    for (int e = l64; e < wb * kSpmvRowsPerWave; e += 64) {
-        const int q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
-        int2 v = make_int2(0, 0);
+    // every load of the prologue (the tile's rows of X, the wavefront's ELL entries) is
+    // issued before any of them is written to LDS
+    constexpr int NX = XT ? (kTileRows * LPR + kEwBlock - 1) / kEwBlock : 1;
+    constexpr int NE = (WMAX * kSpmvRowsPerWave + 63) / 64;
+    float4 xv[NX];
+    int idv[NE];
+    float sv[NE], rv[NE];
+    const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
+    if constexpr (XT) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int i = threadIdx.x + k * kEwBlock;
+            if (i < kTileRows * LPR) xv[k] = X4[i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
+        idv[k] = 0; sv[k] = 0.0f; rv[k] = 0.0f;
         if (q < wl) {
             const size_t g = (size_t)q * np + row0 + r;
-            int id = ids[g];
-            if constexpr (XT) {
-                // neighbours inside the tile are staged as -1 - (row - tile0)
-                const int local = id - tile0;
-                if ((unsigned)local < (unsigned)kTileRows) id = -1 - local;
-            }
-            v = make_int2(id, __float_as_int(Sv[g] + al * Rv[g]));
+            idv[k] = ids[g]; sv[k] = Sv[g]; rv[k] = Rv[g];
         }
-        stage[wave][q][r] = v;
+    }
+    if constexpr (XT) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int i = threadIdx.x + k * kEwBlock;
+            if (i < kTileRows * LPR) xs[i] = xv[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
+        if (q < wb) {
+            int2 v = make_int2(0, 0);
+            if (q < wl) {
+                int id = idv[k];
+                if constexpr (XT) {
+                    // neighbours inside the tile are staged as -1 - (row - tile0)
+                    const int local = id - tile0;
+                    if ((unsigned)local < (unsigned)kTileRows) id = -1 - local;
+                }
+                v = make_int2(id, __float_as_int(sv[k] + al * rv[k]));
+            }
+            stage[wave][q][r] = v;
+        }
     }
     __syncthreads();
     if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
