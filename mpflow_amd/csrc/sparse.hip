@@ -421,10 +421,20 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
     for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
     for (int u0 = 0; J.cinv && u0 < nt; u0 += KU) {
         const int un = min(KU, nt - u0);
+        // all staging loads issued before the LDS writes
+        constexpr int NS = KU * Rp / NT;
+        static_assert(KU * Rp % NT == 0, "staging splits evenly over the block");
+        float gv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int i = threadIdx.x + k * NT, u = i / Rp, l = i % Rp;
+            gv[k] = u < un ? (float)((double)csc[u0 + u] * part_ptr(J, 4, s, u0 + u)[l]) : 0.0f;
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < KU * Rp; i += NT) {
-            const int u = i / Rp, l = i % Rp;
-            gs[u][l] = u < un ? (float)((double)csc[u0 + u] * part_ptr(J, 4, s, u0 + u)[l]) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int i = threadIdx.x + k * NT;
+            gs[i / Rp][i % Rp] = gv[k];
         }
         __syncthreads();
         const int un4 = (un + 3) / 4;                // cinv rows are zero-padded to ld
