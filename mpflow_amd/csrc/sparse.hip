@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
         double acc = 0.0;
         // slots past the 16-row group's widest row are padding (value 0)
         const int w = J.gw ? J.gw[row >> 4] : J.width;
-#pragma unroll 4
+#pragma unroll 8
         for (int q = 0; q < w; ++q) {
             const size_t e = (size_t)q * J.np + row;
             const double v = ((double)J.s[e] + (double)J.s_lo[e]) + al * ((double)J.r[e] + (double)J.r_lo[e]);
