@@ -56,7 +56,14 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
     __shared__ float T[6][64];
     __shared__ float Kc[36];
     __shared__ int nodes[6];
+    __shared__ int cmap[64];
+    __shared__ double cscale[64];
     if (threadIdx.x < k) nodes[threadIdx.x] = jd.elements[(size_t)e * k + threadIdx.x];
+    // the column map once per block: the field loads then depend on LDS only
+    if (threadIdx.x < R) {
+        cmap[threadIdx.x] = jd.colmap[threadIdx.x];
+        cscale[threadIdx.x] = jd.colscale[threadIdx.x];
+    }
     float acc[kJacMaxRows / 256];
 #pragma unroll
     for (int q = 0; q < kJacMaxRows / 256; ++q) acc[q] = 0.0f;
@@ -69,8 +76,8 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
             Kc[i] = jd.ek[(size_t)e * k * k + i] + al * jd.em[(size_t)e * k * k + i];
         for (int i = threadIdx.x; i < k * R; i += blockDim.x) {
             const int a = i / R, c = i % R;
-            const size_t vi = vec_index(Jb, s, nodes[a], jd.colmap[c]);
-            const double sc = jd.colscale[c];
+            const size_t vi = vec_index(Jb, s, nodes[a], cmap[c]);
+            const double sc = cscale[c];
             phi[a][c] = use64 ? (float)(sc * Jb.x64[vi]) : (float)(sc * (double)Jb.x[vi]);
         }
         __syncthreads();
