@@ -25,7 +25,7 @@ constexpr int kRing = 4;
 // current iteration saw it (the stop rule clears [0] during the pupdate launch)
 constexpr int kStInts = 4;
 // partial-sum channels per tile
-constexpr int kParts = 7;
+constexpr int kParts = 8;
 
 struct JobDev {
     int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, col0;   // R = unique RHS solved
@@ -64,13 +64,15 @@ struct JobDev {
     const int32_t* diag_slot;    // [np]
     const uint8_t* gw;           // [np / 16] entry count of the widest row of each 16-row group
     // two-level preconditioner (CG only; null: Jacobi alone).  The aggregates are the
-    // 256-row tiles: M^-1 r = D^-1 r + W Einv W^T r with W the tile indicator,
-    // Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W, sigma = 1, plus
-    // n n^T on the dcFree slice; Sc: per-tile conductivity scale of this frame)
-    const float* cinv;           // [C][ntiles][cld], rows zero-padded to cld = ceil4(ntiles)
+    // two halves of each 256-row tile (aggregate 2t: the tile's first ceil(n_t / 2) real
+    // rows, 2t + 1: the rest): M^-1 r = D^-1 r + W Einv W^T r with W the aggregate
+    // indicator, Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W,
+    // sigma = 1, plus n n^T on the dcFree slice; Sc: per-aggregate conductivity scale of
+    // this frame).  W^T r of the first halves is partial channel 4, of the second 7.
+    const float* cinv;           // [C][2 ntiles][cld], rows zero-padded to cld = ceil4(2 ntiles)
     int cld;
-    float* csc;                  // [ntiles] Sc^-1/2 (written after each assembly)
-    float* cy;                   // [C][ntiles][Rp] coarse correction Einv W^T r
+    float* csc;                  // [2 ntiles] Sc^-1/2 (written after each assembly)
+    float* cy;                   // [C][2 ntiles][Rp] coarse correction Einv W^T r
 };
 
 // PCG search direction of iteration k (k4 = k % kRing)

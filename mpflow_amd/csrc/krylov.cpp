@@ -75,7 +75,7 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
                 for (int t = 0; t < h[j].ntiles; ++t) lst.push_back(make_int4(j, c, t, 0));
             }
             rows += (long long)h[j].C * h[j].np;
-            g.nchunk = std::max(g.nchunk, (h[j].ntiles + kCoarseRows - 1) / kCoarseRows);
+            g.nchunk = std::max(g.nchunk, (h[j].ntiles + kChunkTiles - 1) / kChunkTiles);
             if (h[j].cinv) init_nchunk = std::max(init_nchunk, g.nchunk);
         }
         g.njs = (int)gjs.size();

@@ -344,10 +344,58 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
     }
 }
 
-// coarse-correction row chunks of a job (one partial each in channels 5 and 6)
-__device__ __forceinline__ int coarse_chunks(const JobDev& J) {
-    return (J.ntiles + kCoarseRows - 1) / kCoarseRows;
+// block_partials4 with NF more channels carried in FP32 through the wavefront tree
+// (short sums that their consumer reads in FP32) and in FP64 across the wavefronts
+template <int LG, int NV, int NF, int NT>
+__device__ __forceinline__ void block_partials4f(const JobDev& J, int s, int tile, const int* ch,
+                                                 double (&v)[NV][4], const int* chf, float (&f)[NF][4]) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int rsub = l64 / LPR, c4 = l64 % LPR;
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[c][j] += __shfl_xor(v[c][j], off, 64);
+#pragma unroll
+        for (int c = 0; c < NF; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[c][j] += __shfl_xor(f[c][j], off, 64);
+    }
+    __shared__ double red[NV + NF][NT / 64][64];
+    if (rsub == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) red[c][wave][4 * c4 + j] = v[c][j];
+#pragma unroll
+            for (int c = 0; c < NF; ++c) red[NV + c][wave][4 * c4 + j] = (double)f[c][j];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < (1 << LG)) {
+#pragma unroll
+        for (int c = 0; c < NV + NF; ++c) {
+            double a = 0.0;
+            for (int w = 0; w < NT / 64; ++w) a += red[c][w][threadIdx.x];
+            part_ptr(J, c < NV ? ch[c] : chf[c - NV], s, tile)[threadIdx.x] = a;
+        }
+    }
 }
+
+// partial-sum chunks of a job: kChunkTiles tiles (kCoarseRows coarse rows) each, one
+// partial per chunk in channels 5 and 6
+__device__ __forceinline__ int coarse_chunks(const JobDev& J) {
+    return (J.ntiles + kChunkTiles - 1) / kChunkTiles;
+}
+
+// first local row of a tile's second coarse aggregate: the tile's real rows split
+// ceil(n / 2) + floor(n / 2)
+__device__ __forceinline__ int tile_half(const JobDev& J, int tile) {
+    return (min(kTileRows, J.n - tile * kTileRows) + 1) / 2;
+}
+static_assert(kAggPerTile == 2, "the coarse kernels index aggregates as 2 tile + half");
 
 // block_partials4 with the reduction buffer supplied by the caller (it may share LDS with
 // data the wavefronts used before: the block synchronises before writing it)

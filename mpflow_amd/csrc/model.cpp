@@ -88,7 +88,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     part.alloc((size_t)kParts * S.C * ntile_job * S.Rp * frames);
     sc.alloc((size_t)SC_NFIELDS * S.C * S.Rp * frames);
     gamma.alloc((size_t)S.E * frames);
-    const int nct = S.coarse_nt;
+    const int nct = kAggPerTile * S.coarse_nt;   // coarse rows (aggregates)
     if (nct) {
         cinv.alloc(S.coarse_inv.size()); cinv.upload(S.coarse_inv.data(), S.coarse_inv.size(), st0);
         cd1.alloc(nct); cd1.upload(S.coarse_d1.data(), nct, st0);

@@ -14,7 +14,9 @@ namespace mpf {
 
 constexpr int kMaxSlices = 8;      // harmonics per job
 constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
+constexpr int kAggPerTile = 2;     // coarse aggregates per tile: its two geometric halves
 constexpr int kCoarseRows = 32;    // coarse rows per block of the coarse-correction kernel
+constexpr int kChunkTiles = kCoarseRows / kAggPerTile;   // tiles per partial-sum chunk
 constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level
 constexpr int kBlock = 256;        // threads per SpMV block: 4 wavefronts x 64 rows
 constexpr int kEwBlock = 1024;     // threads per streaming / reduction block
@@ -103,8 +105,9 @@ struct MeshSetup {
     double alpha[kMaxSlices] = {}, beta[kMaxSlices] = {};
     double ref = 1.0, height = 1.0, port_height = 1.0;
     int logarithmic = 0, jacobian_exact = 0;
-    // two-level preconditioner: tile count (0: none), [C][nt][ld] inverses of the coarse
-    // matrices at sigma = 1, per-tile sigma = 1 stiffness-diagonal sums
+    // two-level preconditioner: tile count (0: none), [C][na][ld] inverses of the coarse
+    // matrices at sigma = 1 (na = kAggPerTile * nt aggregates), per-aggregate sigma = 1
+    // stiffness-diagonal sums
     int coarse_nt = 0, coarse_ld = 0;   // tiles, padded row length of coarse_inv
     std::vector<float> coarse_inv;
     std::vector<double> coarse_d1;

@@ -203,10 +203,11 @@ static void edge_integrals(int kb, const double* sp, double (*m)[3], double* v) 
 // median of its longer coordinate extent, the first part holding a whole number of
 // 256-node leaves, until every leaf fits one 256-row tile.  Each tile is then a compact
 // patch of the mesh: most gathers of its SpMV rows stay inside it, consecutive tiles are
-// neighbouring patches (the XCD-striped work lists keep them in one L2), and the tile
-// is one aggregate of the two-level preconditioner.  Rows are sorted by descending
-// degree inside each tile so the 16-row groups of the SpMV have near-uniform ELL widths.
-// Returns perm[node] = row.
+// neighbouring patches (the XCD-striped work lists keep them in one L2).  Each tile is
+// then bisected the same way into a first half of ceil(n / 2) rows and the rest: the
+// two aggregates of the two-level preconditioner it holds.  Rows are sorted by
+// descending degree inside each half so the 16-row groups of the SpMV have near-uniform
+// ELL widths.  Returns perm[node] = row.
 static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, const int32_t* el) {
     std::vector<int32_t> deg(N, 0);
     {
@@ -222,12 +223,9 @@ static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, 
     }
     std::vector<int32_t> order(N);
     for (int n = 0; n < N; ++n) order[n] = n;
-    std::vector<std::pair<int, int>> todo{{0, N}};
-    while (!todo.empty()) {
-        const auto [b, e] = todo.back();
-        todo.pop_back();
-        const int n = e - b;
-        if (n <= kTileRows) continue;
+    // order[b, e) split at `left` along the longer coordinate extent of the set; ties
+    // broken by node id, so the split does not depend on the library's algorithm
+    auto bisect = [&](int b, int e, int left) {
         double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
         for (int i = b; i < e; ++i)
             for (int a = 0; a < 2; ++a) {
@@ -235,19 +233,30 @@ static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, 
                 hi[a] = std::max(hi[a], xy[2 * (size_t)order[i] + a]);
             }
         const int ax = (hi[0] - lo[0]) >= (hi[1] - lo[1]) ? 0 : 1;
-        const int leaves = (n + kTileRows - 1) / kTileRows;
-        const int left = (leaves / 2) * kTileRows;
-        // ties broken by node id: the split does not depend on the library's algorithm
         std::nth_element(order.begin() + b, order.begin() + b + left, order.begin() + e, [&](int a, int c) {
             const double va = xy[2 * (size_t)a + ax], vc = xy[2 * (size_t)c + ax];
             return va < vc || (va == vc && a < c);
         });
+    };
+    std::vector<std::pair<int, int>> todo{{0, N}};
+    while (!todo.empty()) {
+        const auto [b, e] = todo.back();
+        todo.pop_back();
+        const int n = e - b;
+        if (n <= kTileRows) continue;
+        const int leaves = (n + kTileRows - 1) / kTileRows;
+        const int left = (leaves / 2) * kTileRows;
+        bisect(b, e, left);
         todo.emplace_back(b + left, e);
         todo.emplace_back(b, b + left);
     }
-    for (int w = 0; w < N; w += kTileRows)
-        std::sort(order.begin() + w, order.begin() + std::min(N, w + kTileRows),
-                  [&](int a, int c) { return deg[a] != deg[c] ? deg[a] > deg[c] : a < c; });
+    auto by_degree = [&](int a, int c) { return deg[a] != deg[c] ? deg[a] > deg[c] : a < c; };
+    for (int w = 0; w < N; w += kTileRows) {
+        const int e = std::min(N, w + kTileRows), h = (e - w + 1) / 2;
+        if (e - w > 1) bisect(w, e, h);
+        std::sort(order.begin() + w, order.begin() + w + h, by_degree);
+        std::sort(order.begin() + w + h, order.begin() + e, by_degree);
+    }
     std::vector<int32_t> perm(N);
     for (int r = 0; r < N; ++r) perm[order[r]] = r;
     return perm;
@@ -292,46 +301,52 @@ static bool spd_inverse(int n, double* A) {
     return true;
 }
 
-// Coarse level of the two-level preconditioner (aggregates = 256-row tiles, W = the
-// indicator of each tile's real rows): per harmonic, the inverse of W^T A_c(1) W at
-// sigma = 1 (plus n n^T, n = real rows per tile, on the dcFree harmonic 0, matching the
-// A + 1 1^T operator), and per tile the sum of the sigma = 1 stiffness diagonal that
-// scales it to a frame's conductivity.  Skipped (coarse_nt = 0) above kCoarseMaxTiles
-// tiles.
+// Coarse level of the two-level preconditioner (aggregates = the two halves of each
+// 256-row tile, aggregate 2t + 1 holding the tile's real rows from ceil(n_t / 2) on; W =
+// the aggregate indicator): per harmonic, the inverse of W^T A_c(1) W at sigma = 1 (plus
+// n n^T, n = real rows per aggregate, on the dcFree harmonic 0, matching the A + 1 1^T
+// operator), and per aggregate the sum of the sigma = 1 stiffness diagonal that scales
+// it to a frame's conductivity.  Skipped (coarse_nt = 0) above kCoarseMaxTiles tiles.
 static void coarse_setup(MeshSetup& s) {
-    const int np = s.np, nt = np / kTileRows;
+    const int np = s.np, nt = np / kTileRows, na = kAggPerTile * nt;
     s.coarse_nt = 0;
     if (nt > kCoarseMaxTiles) return;
     const size_t nent = (size_t)s.wpad * np;
     std::vector<double> sk(nent, 0.0), sm(nent, 0.0);
     for (size_t ent = 0; ent < nent; ++ent)
         for (int i = s.ctb_off[ent]; i < s.ctb_off[ent + 1]; ++i) { sk[ent] += s.ctb_k[i]; sm[ent] += s.ctb_m[i]; }
-    std::vector<double> cnt(nt, 0.0);
-    for (int row = 0; row < s.N; ++row) cnt[row / kTileRows] += 1.0;
-    s.coarse_d1.assign(nt, 0.0);
-    for (int row = 0; row < s.N; ++row)
-        s.coarse_d1[row / kTileRows] += sk[(size_t)s.diag_slot[row] * np + row];
-    const int ld = (nt + 3) / 4 * 4;
-    s.coarse_inv.assign((size_t)s.C * nt * ld, 0.0f);
-    std::vector<double> Ec((size_t)nt * nt);
+    // aggregate of each real row
+    std::vector<int> agg(s.N);
+    for (int row = 0; row < s.N; ++row) {
+        const int t = row / kTileRows, rows = std::min(kTileRows, s.N - t * kTileRows);
+        agg[row] = kAggPerTile * t + (row - t * kTileRows >= (rows + 1) / 2 ? 1 : 0);
+    }
+    std::vector<double> cnt(na, 0.0);
+    for (int row = 0; row < s.N; ++row) cnt[agg[row]] += 1.0;
+    s.coarse_d1.assign(na, 0.0);
+    for (int row = 0; row < s.N; ++row) s.coarse_d1[agg[row]] += sk[(size_t)s.diag_slot[row] * np + row];
+    const int ld = (na + 3) / 4 * 4;
+    s.coarse_inv.assign((size_t)s.C * na * ld, 0.0f);
+    std::vector<double> Ec((size_t)na * na);
     for (int c = 0; c < s.C; ++c) {
         std::fill(Ec.begin(), Ec.end(), 0.0);
         for (int q = 0; q < s.wpad; ++q)
             for (int row = 0; row < s.N; ++row) {
                 const size_t ent = (size_t)q * np + row;
-                if (s.ctb_off[ent] == s.ctb_off[ent + 1]) continue;   // padding slot
-                Ec[(size_t)(row / kTileRows) * nt + s.ids[ent] / kTileRows] += sk[ent] + s.alpha[c] * sm[ent];
+                if (s.ctb_off[ent] == s.ctb_off[ent + 1] || s.ids[ent] >= s.N) continue;   // padding slot
+                Ec[(size_t)agg[row] * na + agg[s.ids[ent]]] += sk[ent] + s.alpha[c] * sm[ent];
             }
         if (c == 0 && !s.fixed)
-            for (int t = 0; t < nt; ++t)
-                for (int u = 0; u < nt; ++u) Ec[(size_t)t * nt + u] += cnt[t] * cnt[u];
-        // tiles without real rows (none with the padded tile count) keep an identity block
-        for (int t = 0; t < nt; ++t)
-            if (cnt[t] == 0.0) Ec[(size_t)t * nt + t] = 1.0;
-        if (!spd_inverse(nt, Ec.data())) return;
-        for (int t = 0; t < nt; ++t)
-            for (int u = 0; u < nt; ++u)
-                s.coarse_inv[((size_t)c * nt + t) * ld + u] = (float)Ec[(size_t)t * nt + u];
+            for (int t = 0; t < na; ++t)
+                for (int u = 0; u < na; ++u) Ec[(size_t)t * na + u] += cnt[t] * cnt[u];
+        // aggregates without real rows (the second half of a one-row tile) keep an
+        // identity block
+        for (int t = 0; t < na; ++t)
+            if (cnt[t] == 0.0) Ec[(size_t)t * na + t] = 1.0;
+        if (!spd_inverse(na, Ec.data())) return;
+        for (int t = 0; t < na; ++t)
+            for (int u = 0; u < na; ++u)
+                s.coarse_inv[((size_t)c * na + t) * ld + u] = (float)Ec[(size_t)t * na + u];
     }
     s.coarse_nt = nt;
     s.coarse_ld = ld;

@@ -75,7 +75,8 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
     const Ew L = ew_of(J);
     const float* dinv = J.dinv + (size_t)s * J.np;
     const float xs = J.xscale[s];
-    double rz = 0.0, bb = 0.0, bs = 0.0;
+    const int h = t.z * kTileRows + tile_half(J, t.z);
+    double rz = 0.0, bb = 0.0, bs0 = 0.0, bs1 = 0.0;
     for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
@@ -86,12 +87,13 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
         J.p[i] = z;
         rz += (double)b * (double)z;
         bb += (double)b * (double)b;
-        bs += (double)b;
+        if (row < h) bs0 += (double)b;
+        else bs1 += (double)b;
     }
-    // channel 4: the tile's W^T r for the two-level preconditioner
-    const int ch[3] = {0, 1, 4};
-    const double v[3] = {rz, bb, bs};
-    block_partials<3, kEwBlock>(J, s, t.z, ch, v);
+    // channels 4 and 7: W^T r of the tile's two aggregates (two-level preconditioner)
+    const int ch[4] = {0, 1, 4, 7};
+    const double v[4] = {rz, bb, bs0, bs1};
+    block_partials<4, kEwBlock>(J, s, t.z, ch, v);
 }
 
 // two-level start: p = z = D^-1 b + W y (y from the coarse kernel, real rows only)
@@ -102,9 +104,11 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_p(const JobDev* __restric
     if (!J.cinv) return;
     const int s = t.y;
     const Ew L = ew_of(J);
-    const float y = J.cy[((size_t)s * J.ntiles + t.z) * J.Rp + L.lane];
+    const size_t cb = ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * J.Rp + L.lane;
+    const float y0 = J.cy[cb], y1 = J.cy[cb + J.Rp];
+    const int h = t.z * kTileRows + tile_half(J, t.z);
     for (int row = t.z * kTileRows + L.g; row < min((t.z + 1) * kTileRows, J.n); row += L.G)
-        J.p[vec_index(J, s, row, L.lane)] += y;
+        J.p[vec_index(J, s, row, L.lane)] += row < h ? y0 : y1;
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
@@ -265,7 +269,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
 // instead of [0], which its stop rule may clear while they run, so that every tile of a
 // slice that stops in this iteration still takes its last x update.
 template <int LG>
-__global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __restrict__ jobs,
                                                            const int4* __restrict__ tiles, int iter) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
     const int par = iter & 1, k4 = iter & (kRing - 1);
@@ -322,13 +326,18 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restr
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Q = (const MPF_GLOBAL float4*)gp((const float*)J.q) + base;
     MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
-    double v[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    const int h = t.z * kTileRows + tile_half(J, t.z);
+    double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    // W^T r per aggregate (read in FP32 by the coarse kernel): summed in FP32 within a
+    // wavefront, in FP64 across the block
+    float g[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
     for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
         const size_t i = (size_t)row * LPR + c4;
         const float4 q4 = Q[i];
         const float4 r4 = Rr[i];
         const float d = dinv[row];
         const float dc = row < J.n ? 1.0f : 0.0f;
+        const bool first = row < h;
         float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -336,40 +345,39 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_update4(const JobDev* __restr
             const float z = d * rv[j];
             v[0][j] += (double)rv[j] * (double)z;
             v[1][j] += (double)rv[j] * (double)rv[j];
-            v[2][j] += (double)rv[j];
+            g[0][j] += first ? rv[j] : 0.0f;
+            g[1][j] += first ? 0.0f : rv[j];
         }
         Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
     }
-    // channel 4: the tile's W^T r for the two-level preconditioner
-    const int ch[3] = {2, 3, 4};
-    block_partials4<LG, 3, kUpdBlock>(J, s, t.z, ch, v);
+    // channels 4 and 7: W^T r of the tile's two aggregates (two-level preconditioner)
+    const int ch[2] = {2, 3}, chf[2] = {4, 7};
+    block_partials4f<LG, 2, 2, kUpdBlock>(J, s, t.z, ch, v, chf, g);
 }
 
 template <int LG>
 constexpr int coarse_threads() { return (1 << LG) * kCoarseRows > 1024 ? 1024 : (1 << LG) * kCoarseRows; }
 
 // After the SpMV: the tile partials p.q (channel 0) and 1^T p (channel 1) summed per
-// 32-tile chunk in tile order into channels 5 and 6, so every update block reduces a
-// handful of chunk partials instead of every tile's.  One block per (chunk, job, slice).
+// kChunkTiles-tile chunk in a fixed order into channels 5 and 6, so every update block
+// reduces a handful of chunk partials instead of every tile's.  One block per (chunk,
+// job, slice).
 template <int LG>
 __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_chunks(const JobDev* __restrict__ jobs,
                                                                      const int2* __restrict__ js) {
-    constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp, PER = kCoarseRows / G;
+    constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp;
     const int2 pr = js[blockIdx.y];
     const JobDev& J = jobs[pr.x];
     const int s = pr.y;
     if (J.Rp != Rp || !slice_active(J, s)) return;
-    const int nt = J.ntiles, t0 = blockIdx.x * kCoarseRows;
+    const int nt = J.ntiles, t0 = blockIdx.x * kChunkTiles;
     if (t0 >= nt) return;
+    const int te = min(t0 + kChunkTiles, nt);
     const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
     double a = 0.0, b = 0.0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int t = t0 + g * PER + k;
-        if (t < nt) {
-            a += part_ptr(J, 0, s, t)[lane];
-            b += part_ptr(J, 1, s, t)[lane];
-        }
+    for (int t = t0 + g; t < te; t += G) {
+        a += part_ptr(J, 0, s, t)[lane];
+        b += part_ptr(J, 1, s, t)[lane];
     }
     __shared__ double red[2][NT];
     red[0][threadIdx.x] = a;
@@ -383,15 +391,16 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_chunks(const JobDe
     }
 }
 
-// Coarse correction of the two-level preconditioner, one block per (row chunk, job,
-// slice): y_t = c_t sum_u cinv_s[t][u] c_u g_u with g = W^T r (channel 4 tile sums) and
-// c = Sc^-1/2, written to cy; then per chunk the partial sums of
-//   rho = sum_t (r.D^-1 r)_t + g_t y_t   (from channel ch_rz)  -> channel 5
-//   r.r = sum_t (r.r)_t                  (from channel ch_rr)  -> channel 6
+// Coarse correction of the two-level preconditioner, one block per (chunk of kCoarseRows
+// coarse rows = kChunkTiles tiles, job, slice): y_a = c_a sum_u cinv_s[a][u] c_u g_u with
+// g = W^T r (aggregate 2t: channel 4 of tile t, 2t + 1: channel 7) and c = Sc^-1/2,
+// written to cy; then per chunk the partial sums of
+//   rho = sum_t (r.D^-1 r)_t + sum_a g_a y_a   (from channel ch_rz)  -> channel 5
+//   r.r = sum_t (r.r)_t                        (from channel ch_rr)  -> channel 6
 // so the next kernel reduces one partial per chunk instead of one per tile.  A thread
-// owns (coarse row, lane): the scaled g of KU tiles at a time sits in LDS and each thread
-// streams its cinv row with 128-bit loads (rows padded to a multiple of 4 with zeros; the
-// lanes of a row read the same addresses).
+// owns (coarse row, lane): the scaled g of KU aggregates at a time sits in LDS and each
+// thread streams its cinv row with 128-bit loads (rows padded to a multiple of 4 with
+// zeros; the lanes of a row read the same addresses).
 template <int LG>
 __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDev* __restrict__ jobs,
                                                                      const int2* __restrict__ js, int ch_rz,
@@ -404,23 +413,23 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
     // (at the start the slice's state is not set yet: every listed slice runs; jobs
     // without a coarse level only form the chunk sums)
     if (J.Rp != Rp || (in_iteration && !slice_active(J, s)) || (!in_iteration && !J.cinv)) return;
-    const int nt = J.ntiles, ld = J.cld;
-    const int t0 = blockIdx.x * kCoarseRows;
-    if (t0 >= nt) return;
+    const int nt = J.ntiles, na = kAggPerTile * nt, ld = J.cld;
+    const int t0 = blockIdx.x * kCoarseRows;      // first coarse row (aggregate) of the chunk
+    if (t0 >= na) return;
     const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
     __shared__ float gs[KU][Rp];
     const MPF_GLOBAL float* csc = gp((const float*)J.csc);
     const MPF_GLOBAL float4* crow[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int t = min(t0 + g + k * G, nt - 1);   // rows past nt compute a copy, discarded
-        crow[k] = J.cinv ? (const MPF_GLOBAL float4*)(gp(J.cinv) + ((size_t)s * nt + t) * ld) : nullptr;
+        const int t = min(t0 + g + k * G, na - 1);   // rows past na compute a copy, discarded
+        crow[k] = J.cinv ? (const MPF_GLOBAL float4*)(gp(J.cinv) + ((size_t)s * na + t) * ld) : nullptr;
     }
     float acc[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
-    for (int u0 = 0; J.cinv && u0 < nt; u0 += KU) {
-        const int un = min(KU, nt - u0);
+    for (int u0 = 0; J.cinv && u0 < na; u0 += KU) {
+        const int un = min(KU, na - u0);
         // all staging loads issued before the LDS writes
         constexpr int NS = KU * Rp / NT;
         static_assert(KU * Rp % NT == 0, "staging splits evenly over the block");
@@ -428,7 +437,8 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const int i = threadIdx.x + k * NT, u = i / Rp, l = i % Rp;
-            gv[k] = u < un ? (float)((double)csc[u0 + u] * part_ptr(J, 4, s, u0 + u)[l]) : 0.0f;
+            const int a = u0 + u;
+            gv[k] = u < un ? (float)((double)csc[a] * part_ptr(J, a & 1 ? 7 : 4, s, a >> 1)[l]) : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -453,14 +463,16 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
     double rz = 0.0, rr = 0.0;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int t = t0 + g + k * G;
-        if (t < nt) {
-            rz += part_ptr(J, ch_rz, s, t)[lane];
-            rr += part_ptr(J, ch_rr, s, t)[lane];
+        const int a = t0 + g + k * G, t = a >> 1;
+        if (a < na) {
+            if (!(a & 1)) {
+                rz += part_ptr(J, ch_rz, s, t)[lane];
+                rr += part_ptr(J, ch_rr, s, t)[lane];
+            }
             if (J.cinv) {
-                const float y = csc[t] * acc[k];
-                gp(J.cy)[((size_t)s * nt + t) * Rp + lane] = y;
-                rz += part_ptr(J, 4, s, t)[lane] * (double)y;
+                const float y = csc[a] * acc[k];
+                gp(J.cy)[((size_t)s * na + a) * Rp + lane] = y;
+                rz += part_ptr(J, a & 1 ? 7 : 4, s, t)[lane] * (double)y;
             }
         }
     }
@@ -476,8 +488,9 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
     }
 }
 
-// Sc^-1/2 per tile: Sc_t = sum of S_ii(gamma) over the tile's real rows / the same sum
-// at sigma = 1 (harmonic independent; one block per (job, tile) of slice 0)
+// Sc^-1/2 per aggregate: Sc_a = sum of S_ii(gamma) over the aggregate's real rows / the
+// same sum at sigma = 1 (harmonic independent; one block per (job, tile) of slice 0, the
+// tile's two aggregates)
 __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __restrict__ jobs,
                                                            const int4* __restrict__ tiles,
                                                            const double* __restrict__ d1) {
@@ -485,21 +498,27 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
     const JobDev& J = jobs[t.x];
     if (t.y != 0 || !J.csc) return;
     const int row = t.z * kTileRows + threadIdx.x;
+    const bool first = (int)threadIdx.x < tile_half(J, t.z);
     double v = 0.0;
     if (row < J.n) {
         const size_t e = (size_t)J.diag_slot[row] * J.np + row;
         v = (double)J.s[e] + (double)J.s_lo[e];
     }
-    __shared__ double red[kTileRows];
-    red[threadIdx.x] = v;
+    __shared__ double red[2][kTileRows];
+    red[0][threadIdx.x] = first ? v : 0.0;
+    red[1][threadIdx.x] = first ? 0.0 : v;
     __syncthreads();
     for (int h = kTileRows / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        if ((int)threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double sc = d1[t.z] > 0.0 ? red[0] / d1[t.z] : 0.0;
-        J.csc[t.z] = sc > 0.0 ? (float)(1.0 / sqrt(sc)) : 0.0f;
+    if (threadIdx.x < 2) {
+        const int a = kAggPerTile * t.z + threadIdx.x;
+        const double sc = d1[a] > 0.0 ? red[threadIdx.x][0] / d1[a] : 0.0;
+        J.csc[a] = sc > 0.0 ? (float)(1.0 / sqrt(sc)) : 0.0f;
     }
 }
 
@@ -515,7 +534,7 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
 // slice state the update kernel recorded for this iteration, so a slice that stops here
 // still has every tile's x updated (its p is updated too and not read again).
 template <int LG, bool XUP>
-__global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __restrict__ jobs,
                                                             const int4* __restrict__ tiles, int iter,
                                                             int* __restrict__ active) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
@@ -528,6 +547,15 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
     __shared__ double sh_b[Rp];
     __shared__ int sh_bz[Rp];
     __shared__ int not_done;
+    // two-level: the coarse corrections y_a of the tile's two aggregates (zero without)
+    __shared__ float4 sh_y[kAggPerTile][LPR];
+    if ((int)threadIdx.x < kAggPerTile * LPR) {
+        float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (J.cinv)
+            y = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))
+                [((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * LPR + threadIdx.x];
+        sh_y[threadIdx.x / LPR][threadIdx.x % LPR] = y;
+    }
     {
         double out[2] = {0.0, 0.0};
         // rho and r.r come as one partial per 32-tile chunk (k_pcg_coarse, channels 5, 6)
@@ -602,9 +630,9 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
     // p_k is read from this iteration's buffer, p_{k+1} written to the next one
     const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)pbuf(J, k4)) + base;
     MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(pbuf(J, (k4 + 1) & (kRing - 1))) + base;
-    // two-level: z = D^-1 r + y_tile on the tile's real rows
-    float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (J.cinv) y4 = ((const MPF_GLOBAL float4*)gp((const float*)J.cy))[((size_t)s * J.ntiles + t.z) * LPR + c4];
+    // two-level: z = D^-1 r + y_a on the real rows of aggregate a (the tile's first
+    // tile_half rows: 2t, the rest: 2t + 1)
+    const int h = t.z * kTileRows + tile_half(J, t.z);
     if constexpr (XUP) {
         // k4 == kRing - 1: the cycle's x updates, oldest first; p of the cycle's first iteration sits where
         // p_{k+1} goes and is read before the write
@@ -638,6 +666,7 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
             X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
             const float d = dinv[row];
             const float w = row < J.n ? 1.0f : 0.0f;
+            const float4 y4 = sh_y[row < h ? 0 : 1][c4];
             Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                                 zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                                 zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
@@ -650,6 +679,7 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_pupdate4(const JobDev* __rest
         const float4 r4 = Rr[i], p4 = Pk[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
+        const float4 y4 = sh_y[row < h ? 0 : 1][c4];
         Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                             zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                             zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,

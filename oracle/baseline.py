@@ -22,7 +22,7 @@ def _lib():
     L = C.CDLL(build_baseline.LIB)
     L.cgh_pcg.restype = C.c_int
     L.cgh_pcg.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                          C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int]
+                          C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
     L.cgh_threads.restype = C.c_int
     L.cgh_set_threads.argtypes = [C.c_int]
     return L
@@ -43,10 +43,13 @@ def csr_to_ell(A):
     return np.ascontiguousarray(ids), np.ascontiguousarray(vals)
 
 
-def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0, einv=None, tile=256):
+def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0, einv=None, tile=256, agg=None):
     """Solve (A + dc 1 1^T) X = B with the C++ baseline; returns (X, steps, seconds).
-    ``einv`` (nt x nt, nt = ceil(N / tile)): the two-level preconditioner's scaled coarse
-    inverse (``eit_oracle.two_level_preconditioner``); None: Jacobi alone."""
+    ``einv`` (na x na): the two-level preconditioner's scaled coarse inverse
+    (``eit_oracle.two_level_preconditioner``) over the aggregates ``agg`` of the rows
+    (default: the halves of ``tile``-row tiles, ``eit_oracle.tile_aggregates``); None:
+    Jacobi alone."""
+    from .eit_oracle import tile_aggregates
     L = _lib()
     if threads:
         L.cgh_set_threads(int(threads))
@@ -56,9 +59,16 @@ def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0, einv=None, tile=256):
     R = B.shape[1]
     X = np.zeros_like(B)
     E = None if einv is None else np.ascontiguousarray(einv, np.float32)
+    na, starts = 0, None
+    if E is not None:
+        agg = tile_aggregates(n, tile) if agg is None else np.asarray(agg)
+        assert np.all(np.diff(agg) >= 0), "aggregates are ranges of consecutive rows"
+        na = E.shape[0]
+        starts = np.ascontiguousarray(np.r_[0, np.cumsum(np.bincount(agg, minlength=na))], np.int32)
     t0 = time.perf_counter()
     steps = L.cgh_pcg(n, width, ids.ctypes.data, vals.ctypes.data, R, B.ctypes.data, X.ctypes.data,
-                      int(dc), float(tol), int(max_iter), None if E is None else E.ctypes.data, int(tile))
+                      int(dc), float(tol), int(max_iter), None if E is None else E.ctypes.data, int(na),
+                      None if starts is None else starts.ctypes.data)
     return X, steps, time.perf_counter() - t0
 
 

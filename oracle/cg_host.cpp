@@ -20,14 +20,15 @@ int cgh_threads(void) { return omp_get_max_threads(); }
 
 void cgh_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
-// Two-level coarse term: g = W^T r over `tile`-row aggregates, y = einv g (nt x nt,
-// row-major, already scaled), returns sum_t g_t . y_t per column into gy.
-static void coarse(int n, int R, int tile, int nt, const float* einv, const std::vector<float>& r,
+// Two-level coarse term: g = W^T r over the aggregates (aggregate t: rows astart[t] ..
+// astart[t + 1] - 1), y = einv g (nt x nt, row-major, already scaled), returns
+// sum_t g_t . y_t per column into gy.
+static void coarse(int R, const int32_t* astart, int nt, const float* einv, const std::vector<float>& r,
                    std::vector<double>& g, std::vector<float>& y, std::vector<double>& gy) {
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < nt; ++t) {
         for (int c = 0; c < R; ++c) g[(size_t)t * R + c] = 0.0;
-        for (int i = t * tile; i < std::min(n, (t + 1) * tile); ++i)
+        for (int i = astart[t]; i < astart[t + 1]; ++i)
             for (int c = 0; c < R; ++c) g[(size_t)t * R + c] += r[(size_t)i * R + c];
     }
 #pragma omp parallel for schedule(static)
@@ -46,11 +47,15 @@ static void coarse(int n, int R, int tile, int nt, const float* einv, const std:
 
 // Solves (A + dc 1 1^T) X = B for R columns.  Stops when every column has
 // ||r|| <= tol ||b|| (tol > 0), or after max_iter steps.  Returns steps taken.
-// einv (nullable): the two-level preconditioner's scaled coarse inverse over
-// aggregates of `tile` consecutive rows, M^-1 r = D^-1 r + W einv W^T r.
+// einv (nullable): the two-level preconditioner's scaled coarse inverse over na
+// aggregates of consecutive rows (aggregate t: rows astart[t] .. astart[t + 1] - 1,
+// astart[na] = n), M^-1 r = D^-1 r + W einv W^T r.
 int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, const float* B,
-            float* X, int dc, double tol, int max_iter, const float* einv, int tile) {
-    const int nt = einv ? (n + tile - 1) / tile : 0;
+            float* X, int dc, double tol, int max_iter, const float* einv, int na, const int32_t* astart) {
+    const int nt = einv ? na : 0;
+    std::vector<int32_t> aof(nt ? n : 0);
+    for (int t = 0; t < nt; ++t)
+        for (int i = astart[t]; i < astart[t + 1]; ++i) aof[i] = t;
     std::vector<double> cg((size_t)nt * R), gy(R, 0.0);
     std::vector<float> cy((size_t)nt * R, 0.0f);
     std::vector<float> r((size_t)n * R), p((size_t)n * R), q((size_t)n * R), dinv(n);
@@ -81,11 +86,11 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
         for (int c = 0; c < R; ++c) { rho[c] += lrho[c]; bb[c] += lbb[c]; }
     }
     if (nt) {
-        coarse(n, R, tile, nt, einv, r, cg, cy, gy);
+        coarse(R, astart, nt, einv, r, cg, cy, gy);
         for (int c = 0; c < R; ++c) rho[c] += gy[c];
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i)
-            for (int c = 0; c < R; ++c) p[(size_t)i * R + c] += cy[(size_t)(i / tile) * R + c];
+            for (int c = 0; c < R; ++c) p[(size_t)i * R + c] += cy[(size_t)aof[i] * R + c];
     }
     int step = 0;
     for (; step < max_iter; ++step) {
@@ -136,7 +141,7 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
             for (int c = 0; c < R; ++c) { nrho[c] += lrz[c]; rr[c] += lrr[c]; }
         }
         if (nt) {
-            coarse(n, R, tile, nt, einv, r, cg, cy, gy);
+            coarse(R, astart, nt, einv, r, cg, cy, gy);
             for (int c = 0; c < R; ++c) nrho[c] += gy[c];
         }
         bool done = true;
@@ -150,7 +155,7 @@ int cgh_pcg(int n, int width, const int32_t* ids, const float* vals, int R, cons
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < R; ++c) {
                 const size_t o = (size_t)i * R + c;
-                p[o] = dinv[i] * r[o] + (nt ? cy[(size_t)(i / tile) * R + c] : 0.0f) + (float)beta[c] * p[o];
+                p[o] = dinv[i] * r[o] + (nt ? cy[(size_t)aof[i] * R + c] : 0.0f) + (float)beta[c] * p[o];
             }
     }
     return step;

@@ -368,37 +368,55 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
 def rcb_order(xy, leaf=256):
     """Node order of recursive coordinate bisection: a set is split at the median of its
     longer coordinate extent (ties by node id), the first part holding a whole number of
-    ``leaf``-node leaves, until every leaf fits; consecutive ``leaf`` nodes of the order
-    are then compact aggregates (the product's device tiles are built the same way)."""
+    ``leaf``-node leaves, until every leaf fits; each leaf is then ordered along its own
+    longer extent, so that its first ceil(n / 2) nodes and the rest are its two halves.
+    Consecutive ``leaf`` nodes of the order are compact tiles and their halves compact
+    aggregates (``tile_aggregates``; the product's device tiles are built the same way)."""
     xy = np.asarray(xy, np.float64)
+
+    def along(idx):
+        pts = xy[idx]
+        ax = 0 if np.ptp(pts[:, 0]) >= np.ptp(pts[:, 1]) else 1
+        return idx[np.lexsort((idx, pts[:, ax]))]
 
     def split(idx):
         if len(idx) <= leaf:
-            return [idx]
-        pts = xy[idx]
-        ax = 0 if np.ptp(pts[:, 0]) >= np.ptp(pts[:, 1]) else 1
-        o = idx[np.lexsort((idx, pts[:, ax]))]
+            return [along(idx)] if len(idx) > 1 else [idx]
+        o = along(idx)
         left = (-(-len(idx) // leaf) // 2) * leaf
         return split(o[:left]) + split(o[left:])
 
     return np.concatenate(split(np.arange(xy.shape[0])))
 
 
-def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256):
+def tile_aggregates(N, tile=256):
+    """Coarse aggregate of each of N rows: the two halves of every ``tile``-row tile, the
+    first holding ceil(n_t / 2) of the tile's n_t rows (aggregate 2t) and the second the
+    rest (2t + 1) -- the product's layout.  A one-row last tile has no second half."""
+    i = np.arange(N)
+    t = i // tile
+    n_t = np.minimum(tile, N - t * tile)
+    return 2 * t + (i - t * tile >= (n_t + 1) // 2)
+
+
+def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256, halves=True):
     """The product's two-level preconditioner (an extension; the reference has only the
-    diagonal one, preconditioner.cpp:59-76), on aggregates of ``tile`` consecutive rows:
+    diagonal one, preconditioner.cpp:59-76), on aggregates of consecutive rows: the two
+    halves of every ``tile``-row tile (``tile_aggregates``; ``halves=False``: whole tiles):
 
         M^-1 r = D^-1 r + W C E1^-1 C W^T r
-        D  = diag(A),  W = aggregate indicator (N x nt)
+        D  = diag(A),  W = aggregate indicator (N x na)
         E1 = W^T A1 W  (+ n n^T with n = rows per aggregate when dc_free: the A + 1 1^T
              operator of CG's dcFree solve), A1 = the system matrix at sigma = 1
         C  = diag(c),  c_t = (sum_{i in t} S_ii / sum_{i in t} S1_ii)^(-1/2)
 
     (S: the stiffness part, harmonic 0).  Returns (prec, Minv) with prec(r) -> z and Minv
-    the dense N x N M^-1 when N <= 2000 (else None); prec.einv holds C E1^-1 C."""
+    the dense N x N M^-1 when N <= 2000 (else None); prec.einv holds C E1^-1 C and
+    prec.agg the aggregate of each row."""
     N = A.shape[0]
-    nt = -(-N // tile)
-    W = sp.csr_matrix((np.ones(N), (np.arange(N), np.arange(N) // tile)), shape=(N, nt))
+    agg = tile_aggregates(N, tile) if halves else np.arange(N) // tile
+    nt = int(agg.max()) + 1
+    W = sp.csr_matrix((np.ones(N), (np.arange(N), agg)), shape=(N, nt))
     E1 = (W.T @ A1 @ W).toarray()
     n = np.asarray(W.sum(axis=0)).ravel()
     if dc_free:
@@ -411,6 +429,7 @@ def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256):
         return dinv[:, None] * r + W @ (Einv @ (W.T @ r))
 
     prec.einv = Einv
+    prec.agg = agg
 
     Minv = None
     if N <= 2000:
