@@ -15,7 +15,7 @@ namespace mpf {
 constexpr int kMaxSlices = 8;      // harmonics per job
 constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
 constexpr int kAggPerTile = 2;     // coarse aggregates per tile: its two geometric halves
-constexpr int kCoarseRows = 32;    // coarse rows per block of the coarse-correction kernel
+constexpr int kCoarseRows = 64;    // coarse rows per block of the coarse-correction kernel
 constexpr int kChunkTiles = kCoarseRows / kAggPerTile;   // tiles per partial-sum chunk
 constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level
 constexpr int kBlock = 256;        // threads per SpMV block: 4 wavefronts x 64 rows
