@@ -448,17 +448,22 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDe
         }
         __syncthreads();
         const int un4 = (un + 3) / 4;                // cinv rows are zero-padded to ld
+        // the thread's PER rows share each staged g (one LDS read for all of them)
+        float a[PER];
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            float a = 0.0f;
+        for (int k = 0; k < PER; ++k) a[k] = 0.0f;
 #pragma unroll 8
-            for (int u = 0; u < un4; ++u) {
+        for (int u = 0; u < un4; ++u) {
+            const float g0 = gs[4 * u][lane], g1 = gs[4 * u + 1][lane], g2 = gs[4 * u + 2][lane],
+                        g3 = gs[4 * u + 3][lane];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
                 const float4 c = crow[k][u0 / 4 + u];
-                a += c.x * gs[4 * u][lane] + c.y * gs[4 * u + 1][lane] + c.z * gs[4 * u + 2][lane] +
-                     c.w * gs[4 * u + 3][lane];
+                a[k] += c.x * g0 + c.y * g1 + c.z * g2 + c.w * g3;
             }
-            acc[k] += a;
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) acc[k] += a[k];
     }
     double rz = 0.0, rr = 0.0;
 #pragma unroll
