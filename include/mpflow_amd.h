@@ -189,11 +189,12 @@ typedef struct mpf_solve_opts {
                                      preconditioner (preconditioner.cpp:59-76).  TWO_LEVEL
                                      (CG on model solves; no counterpart in the reference):
                                      Jacobi plus an additive coarse correction over compact
-                                     256-node aggregates, M^-1 = D^-1 + W Ec^-1 W^T with Ec the
-                                     aggregates' sigma = 1 coarse matrix scaled per aggregate
-                                     to the frame's conductivity; several times fewer
-                                     iterations on large meshes.  Models above 512 aggregates
-                                     (131072 unknowns) fall back to Jacobi. */
+                                     aggregates (the two halves of each 256-node tile),
+                                     M^-1 = D^-1 + W Ec^-1 W^T with Ec the aggregates'
+                                     sigma = 1 coarse matrix scaled per aggregate to the
+                                     frame's conductivity; several times fewer iterations on
+                                     large meshes.  Models above 512 tiles (131072 unknowns)
+                                     fall back to Jacobi. */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {
