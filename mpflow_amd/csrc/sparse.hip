@@ -357,6 +357,10 @@ __global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __re
 
 template <int LG>
 constexpr int coarse_threads() { return (1 << LG) * kCoarseRows > 1024 ? 1024 : (1 << LG) * kCoarseRows; }
+// waves per SIMD the coarse kernel is compiled for: two 1024-thread blocks per CU up to
+// 32 lanes per row (64-lane rows need more registers than that leaves)
+template <int LG>
+constexpr int coarse_waves() { return LG >= 6 ? 4 : 8; }
 
 // After the SpMV: the tile partials p.q (channel 0) and 1^T p (channel 1) summed per
 // kChunkTiles-tile chunk in a fixed order into channels 5 and 6, so every update block
@@ -402,7 +406,7 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_chunks(const JobDe
 // thread streams its cinv row with 128-bit loads (rows padded to a multiple of 4 with
 // zeros; the lanes of a row read the same addresses).
 template <int LG>
-__global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_coarse(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(coarse_threads<LG>(), coarse_waves<LG>()) k_pcg_coarse(const JobDev* __restrict__ jobs,
                                                                      const int2* __restrict__ js, int ch_rz,
                                                                      int ch_rr, int in_iteration) {
     constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp, PER = kCoarseRows / G, KU = 128;
