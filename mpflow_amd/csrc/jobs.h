@@ -73,6 +73,7 @@ struct JobDev {
     int cld;
     float* csc;                  // [2 ntiles] Sc^-1/2 (written after each assembly)
     float* cy;                   // [C][2 ntiles][Rp] coarse correction Einv W^T r
+    float* cg;                   // [C][2 ntiles][Rp] Sc^-1/2 W^T r of the current iteration
 };
 
 // PCG search direction of iteration k (k4 = k % kRing)

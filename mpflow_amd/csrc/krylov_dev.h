@@ -345,10 +345,12 @@ __device__ __forceinline__ void block_partials4(const JobDev& J, int s, int tile
 }
 
 // block_partials4 with NF more channels carried in FP32 through the wavefront tree
-// (short sums that their consumer reads in FP32) and in FP64 across the wavefronts
+// (short sums that their consumer reads in FP32) and in FP64 across the wavefronts; with
+// fo set, channel c's sums are also written scaled by fs[c] in FP32 to fo[c * Rp + lane]
 template <int LG, int NV, int NF, int NT>
 __device__ __forceinline__ void block_partials4f(const JobDev& J, int s, int tile, const int* ch,
-                                                 double (&v)[NV][4], const int* chf, float (&f)[NF][4]) {
+                                                 double (&v)[NV][4], const int* chf, float (&f)[NF][4],
+                                                 MPF_GLOBAL float* fo = nullptr, const float* fs = nullptr) {
     constexpr int LPR = (1 << LG) / 4;
     const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
     const int rsub = l64 / LPR, c4 = l64 % LPR;
@@ -380,6 +382,7 @@ __device__ __forceinline__ void block_partials4f(const JobDev& J, int s, int til
             double a = 0.0;
             for (int w = 0; w < NT / 64; ++w) a += red[c][w][threadIdx.x];
             part_ptr(J, c < NV ? ch[c] : chf[c - NV], s, tile)[threadIdx.x] = a;
+            if (c >= NV && fo) fo[(c - NV) * (1 << LG) + threadIdx.x] = (float)((double)fs[c - NV] * a);
         }
     }
 }

@@ -94,6 +94,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         cd1.alloc(nct); cd1.upload(S.coarse_d1.data(), nct, st0);
         csc.alloc((size_t)nct * frames);
         cy.alloc((size_t)S.C * nct * S.Rp * frames);
+        cg.alloc((size_t)S.C * nct * S.Rp * frames);
         MPF_HIP(hipMemsetAsync(cy.p, 0, cy.bytes(), st0));
     }
     MPF_HIP(hipMemsetAsync(x64.p, 0, x64.bytes(), st0));
@@ -145,6 +146,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
             J.cld = S.coarse_ld;
             J.csc = csc.p + (size_t)nct * f;
             J.cy = cy.p + (size_t)S.C * nct * S.Rp * f;
+            J.cg = cg.p + (size_t)S.C * nct * S.Rp * f;
         }
     }
     jobs.alloc(frames);

@@ -40,7 +40,7 @@ struct Model {
     DBuf<int32_t> node_of_row;   // [np] device row -> caller's node (identity past the mesh nodes)
     DBuf<uint8_t> gw;
     DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
-    DBuf<float> cinv, csc, cy;          // two-level preconditioner (allocated if the mesh has one)
+    DBuf<float> cinv, csc, cy, cg;          // two-level preconditioner (allocated if the mesh has one)
     DBuf<double> cd1;
     DBuf<float> ek, em;
     // per-frame state

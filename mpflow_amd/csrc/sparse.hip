@@ -351,8 +351,16 @@ __global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __re
         Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
     }
     // channels 4 and 7: W^T r of the tile's two aggregates (two-level preconditioner)
+    // with a coarse level also c_a (W^T r)_a in FP32 for the coarse kernel's staging
     const int ch[2] = {2, 3}, chf[2] = {4, 7};
-    block_partials4f<LG, 2, 2, kUpdBlock>(J, s, t.z, ch, v, chf, g);
+    MPF_GLOBAL float* fo = nullptr;
+    float fs[2] = {0.0f, 0.0f};
+    if (J.cinv) {
+        fo = gp(J.cg) + ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * Rp;
+        fs[0] = J.csc[kAggPerTile * t.z];
+        fs[1] = J.csc[kAggPerTile * t.z + 1];
+    }
+    block_partials4f<LG, 2, 2, kUpdBlock>(J, s, t.z, ch, v, chf, g, fo, fs);
 }
 
 template <int LG>
@@ -442,7 +450,11 @@ __global__ void __launch_bounds__(coarse_threads<LG>(), coarse_waves<LG>()) k_pc
         for (int k = 0; k < NS; ++k) {
             const int i = threadIdx.x + k * NT, u = i / Rp, l = i % Rp;
             const int a = u0 + u;
-            gv[k] = u < un ? (float)((double)csc[a] * part_ptr(J, a & 1 ? 7 : 4, s, a >> 1)[l]) : 0.0f;
+            // in the iteration the update kernel left c_a g_a in FP32; at the start only
+            // the FP64 partials exist
+            gv[k] = u >= un ? 0.0f
+                    : in_iteration ? gp((const float*)J.cg)[((size_t)s * na + a) * Rp + l]
+                                   : (float)((double)csc[a] * part_ptr(J, a & 1 ? 7 : 4, s, a >> 1)[l]);
         }
         __syncthreads();
 #pragma unroll
