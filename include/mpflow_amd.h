@@ -332,14 +332,22 @@ int mpf_batch_gn_step(mpf_batch* b, float* const* d_gamma, const double* const* 
 /* ConjugateGradient::solve / BiCGSTAB::solve with a SparseMatrix operator and the
  * diagonal preconditioner (src/numeric/conjugate_gradient.cpp:61-161,
  * src/numeric/bicgstab.cpp:83-193, src/numeric/preconditioner.cpp:59-76), all R <= 64
- * right-hand sides at once, starting from X = 0.
+ * right-hand sides at once, starting from the caller's X as the reference does
+ * (r = B - A X, conjugate_gradient.cpp:80-82, bicgstab.cpp:105-108).
  *   d_column_ids, d_values : SparseMatrix device arrays (sparse_matrix.cpp:48-114): 32
  *                            slots per row, entry (row, slot) at row*32 + slot, empty
- *                            slots with column id 0xFFFFFFFF; `density` used slots
- *   d_B, d_X               : Matrix column-major n x R with leading dimensions ldb, ldx
- *   dc_free                : solve (A + 1 1^T) X = B (CG; BiCGSTAB as opts.bicgstab_ignore_dc)
- * opts: stop rule, tol, max_iter, precision (MPF_PREC_MIXED adds FP64 refinement),
- * check_every, graphs; `krylov` is set by the entry point.  Synchronises. */
+ *                            slots with column id 0xFFFFFFFF; `density` used slots.  A
+ *                            column id that is neither 0xFFFFFFFF nor below n is refused
+ *                            (MPF_EINVAL); a row without a diagonal entry gets a Jacobi
+ *                            inverse of 0 (preconditioner_kernel.cu:63-83)
+ *   d_B                    : Matrix column-major n x R with leading dimension ldb
+ *   d_X                    : in: start X0, out: solution; column-major, leading dimension ldx
+ *   dc_free                : solve (A + 1 1^T) X = B (CG; BiCGSTAB as opts.bicgstab_ignore_dc).
+ *                            The start residual is B - A X0 + 1 (1^T X0), as the reference
+ *                            forms it (conjugate_gradient.cpp:84-88); X0 = 0 is unaffected
+ * Both stop rules compare with B's column norms.  opts: stop rule, tol, max_iter,
+ * precision (MPF_PREC_MIXED adds FP64 refinement), check_every, graphs; `krylov` is set
+ * by the entry point.  Synchronises. */
 int mpf_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
                   const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
                   int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
@@ -347,12 +355,31 @@ int mpf_bicgstab_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t*
                        const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
                        int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
 
-/* Multi-harmonic ELL SpMV, the S/R form of SparseMatrix::multiply
- * (src/numeric/sparse_matrix_kernel.cu:157-196) fused with Equation::update's
- * A_c = S + alpha_c R (src/fem/equation_kernel.cu:137-157):
+/* SparseMatrix::multiply (src/numeric/sparse_matrix.cpp:262-288; the kernel wrapper
+ * sparseMatrixKernel::multiply, include/mpflow/numeric/sparse_matrix_kernel.h:57-60,
+ * src/numeric/sparse_matrix_kernel.cu:157-196) with the same arguments minus the launch
+ * geometry and stream:
+ *   result[row + col * result_rows] = sum over slot < density of
+ *       values[row*32 + slot] * matrix[column_ids[row*32 + slot] + col * matrix_rows]
+ * skipping slots whose column id is 0xFFFFFFFF.  d_values / d_column_ids: the
+ * SparseMatrix arrays (result_rows x 32); d_matrix (matrix_rows x columns) and d_result
+ * (result_rows x columns) column-major with the row count as leading dimension
+ * (Matrix::dataRows).  A column id that is neither 0xFFFFFFFF nor below matrix_rows is
+ * refused (MPF_EINVAL).  Runs on the Krylov SpMV's gather core (X copied once into a
+ * row-interleaved layout).  Synchronises. */
+int mpf_sparse_multiply(mpf_ctx* ctx, const float* d_values, const uint32_t* d_column_ids, const float* d_matrix,
+                        uint32_t result_rows, uint32_t matrix_rows, uint32_t columns, uint32_t density,
+                        float* d_result);
+
+/* Multi-harmonic ELL SpMV on the product's own layout, the S/R form of
+ * SparseMatrix::multiply (src/numeric/sparse_matrix_kernel.cu:157-196) fused with
+ * Equation::update's A_c = S + alpha_c R (src/fem/equation_kernel.cu:137-157):
  *   Y[c][row][j] = sum_slot (S[slot][row] + alpha[c] R[slot][row]) * X[c][ids[slot][row]][j]
- * ELL is column-major (slot-major): entry (row, slot) at slot*np + row; padding slots
- * hold id = row and value 0.  X/Y: C x np x ld, j < R <= ld, ld in {16,32,64}. */
+ * ELL is column-major (slot-major): entry (row, slot) at slot*np + row, width a multiple
+ * of 8 (at most 64); padding slots hold id = row and value 0; an id outside [0, np) is
+ * refused (MPF_EINVAL).  alpha is a HOST array of C floats; every other pointer is device
+ * memory.  X/Y: C x np x ld, all ld lanes are computed, j < R <= ld, ld in {16,32,64}.
+ * Synchronises. */
 int mpf_ell_spmv(mpf_ctx* ctx, int32_t np, int32_t width, const int32_t* d_ids,
                  const float* d_S, const float* d_R, int32_t C, const float* alpha,
                  const float* d_X, float* d_Y, int32_t R, int32_t ld);

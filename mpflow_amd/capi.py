@@ -24,7 +24,7 @@ EXPORTED = [
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
     "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian", "mpf_gn_set_regularization", "mpf_diff_prepare",
-    "mpf_diff_solve", "mpf_normal_solve", "mpf_set_sensitivity_mask",
+    "mpf_diff_solve", "mpf_normal_solve", "mpf_set_sensitivity_mask", "mpf_sparse_multiply",
 ]
 
 
@@ -90,6 +90,8 @@ def lib() -> C.CDLL:
     L.mpf_eit_solve.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.mpf_gn_step.argtypes = [vp, vp, vp, dbl, i32, C.POINTER(SolveOpts), vp, C.POINTER(SolveStats)]
     L.mpf_ell_spmv.argtypes = [vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, i32, i32]
+    u32 = C.c_uint32
+    L.mpf_sparse_multiply.argtypes = [vp, vp, vp, vp, u32, u32, u32, u32, vp]
     L.mpf_gram_bf16.argtypes = [vp, i32, i32, i32, vp, i32, vp]
     L.mpf_gemv_t.argtypes = [vp, i32, i32, vp, i32, vp, vp]
     L.mpf_spd_solve.argtypes = [vp, i32, vp, dbl, vp, vp]

@@ -388,13 +388,26 @@ int mpf_ell_spmv(mpf_ctx* ctx, int32_t np, int32_t width, const int32_t* d_ids, 
         MPF_REQUIRE(np > 0 && np % mpf::kTileRows == 0, "mpf_ell_spmv: np must be a positive multiple of 256");
         MPF_REQUIRE(ld == 16 || ld == 32 || ld == 64, "mpf_ell_spmv: ld must be 16, 32 or 64");
         MPF_REQUIRE(R >= 1 && R <= ld, "mpf_ell_spmv: R out of range");
-        MPF_REQUIRE(C >= 1 && C <= mpf::kMaxSlices && width >= 1 && width <= 64, "mpf_ell_spmv: bad C or width");
+        MPF_REQUIRE(C >= 1 && C <= mpf::kMaxSlices, "mpf_ell_spmv: C out of range");
+        MPF_REQUIRE(width >= 8 && width <= 64 && width % 8 == 0, "mpf_ell_spmv: width must be 8, 16, ..., 64");
         set_device(ctx);
-        // alpha is a host array; stage it on the device
-        mpf::DBuf<float> da(C);
-        da.upload(alpha, C, ctx->stream);
-        mpf::launch_spmv(np, width, d_ids, d_S, d_R, C, da.p, d_X, d_Y, ld, ctx->stream);
-        MPF_HIP(hipStreamSynchronize(ctx->stream));
+        mpf::ell_spmv(ctx, np, width, d_ids, d_S, d_R, C, alpha, d_X, d_Y, ld);
+        return MPF_OK;
+    });
+}
+
+int mpf_sparse_multiply(mpf_ctx* ctx, const float* d_values, const uint32_t* d_column_ids, const float* d_matrix,
+                        uint32_t result_rows, uint32_t matrix_rows, uint32_t columns, uint32_t density,
+                        float* d_result) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_values && d_column_ids && d_matrix && d_result, "mpf_sparse_multiply: null argument");
+        MPF_REQUIRE(result_rows >= 1 && matrix_rows >= 1 && columns >= 1 && result_rows < (1u << 24) &&
+                        matrix_rows < (1u << 24) && columns < (1u << 16),
+                    "mpf_sparse_multiply: bad shape");
+        MPF_REQUIRE(density <= 32, "mpf_sparse_multiply: density above the 32 slots of a row");
+        set_device(ctx);
+        mpf::sparse_multiply(ctx, d_values, d_column_ids, d_matrix, (int)result_rows, (int)matrix_rows, (int)columns,
+                             (int)density, d_result);
         return MPF_OK;
     });
 }

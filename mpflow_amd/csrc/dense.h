@@ -32,9 +32,10 @@ void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use6
 // G_b = A_b A_b^T for `batch` column-major A_b (n x nk, lda; n, lda multiples of 4) on
 // MFMA with 128 x 128 tiles; G_b full (both triangles), column-major n x n
 bool gram_nt_applies(int n, int lda);
-size_t gram_nt_workspace_bytes(int n, int nk, int batch);
+// max_slabs > 0 caps the contraction slabs (and so the workspace) at that many
+size_t gram_nt_workspace_bytes(int n, int nk, int batch, int max_slabs = 0);
 void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG, int batch,
-                    void* work, hipStream_t st);
+                    void* work, hipStream_t st, int max_slabs = 0);
 int gram_kslab(int n, int nk);
 size_t gram_workspace_bytes(int n, int nk);
 void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
@@ -48,6 +49,7 @@ void launch_gemv_n(int rows, int cols, const float* A, int lda, size_t strideA, 
                    size_t striderhs, double* out, size_t strideout, int batch, double* work,
                    hipStream_t st);
 
+// device int2 entries launch_cholesky needs for its update list (filled on the device)
 size_t cholesky_pairs(int n);
 void launch_cholesky(int n, const float* G, size_t strideG, double shift, double* A, size_t strideA,
                      int batch, int* d_info, int2* d_pairs, hipStream_t st);

@@ -716,29 +716,31 @@ int gram_kslab(int n, int nk) {
     return kslab;
 }
 
-// slabs of the 128-tile kernel: about 512 blocks per launch over all batch entries
-static int gram_nt_slabs(int n, int nk, int batch) {
+// slabs of the 128-tile kernel: about 512 blocks per launch over all batch entries, at
+// most max_slabs (> 0) of them
+static int gram_nt_slabs(int n, int nk, int batch, int max_slabs) {
     const int nt = (n + kGT2 - 1) / kGT2;
     const int ntiles = nt * (nt + 1) / 2;
     int nslab = std::max(1, (512 + ntiles * batch - 1) / (ntiles * batch));
     nslab = std::min(nslab, std::max(1, (nk + 255) / 256));
+    if (max_slabs > 0) nslab = std::min(nslab, max_slabs);
     return nslab;
 }
 
 bool gram_nt_applies(int n, int lda) { return n % 4 == 0 && lda % 4 == 0; }
 
-size_t gram_nt_workspace_bytes(int n, int nk, int batch) {
+size_t gram_nt_workspace_bytes(int n, int nk, int batch, int max_slabs) {
     const int nt = (n + kGT2 - 1) / kGT2;
     const int ntiles = nt * (nt + 1) / 2;
-    return (size_t)batch * gram_nt_slabs(n, nk, batch) * ntiles * kGT2 * kGT2 * sizeof(float);
+    return (size_t)batch * gram_nt_slabs(n, nk, batch, max_slabs) * ntiles * kGT2 * kGT2 * sizeof(float);
 }
 
 void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG, int batch,
-                    void* work, hipStream_t st) {
+                    void* work, hipStream_t st, int max_slabs) {
     MPF_REQUIRE(gram_nt_applies(n, lda), "gram: rows and leading dimension must be multiples of 4");
     const int nt = (n + kGT2 - 1) / kGT2;
     const int ntiles = nt * (nt + 1) / 2;
-    const int nslab = gram_nt_slabs(n, nk, batch);
+    const int nslab = gram_nt_slabs(n, nk, batch, max_slabs);
     int kslab = (nk + nslab - 1) / nslab;
     kslab = (kslab + kGK2 - 1) / kGK2 * kGK2;
     const int used = (nk + kslab - 1) / kslab;
@@ -790,30 +792,38 @@ void launch_gemv_n(int rows, int cols, const float* A, int lda, size_t strideA, 
     check_launch();
 }
 
+// the (ib, jb) pairs of the trailing updates, elimination steps in order: step kb lists
+// jb = kb + 1 .. nblk - 1 and, for each, ib = jb .. nblk - 1, and starts at
+// chol_pair_start(nblk, kb) = sum over k < kb of T(nblk - k - 1), T(m) = m (m + 1) / 2
+__host__ __device__ inline int chol_pair_start(int nblk, int kb) {
+    int s = 0;
+    for (int k = 0; k < kb; ++k) {
+        const int m = nblk - k - 1;
+        s += m * (m + 1) / 2;
+    }
+    return s;
+}
+
+__global__ void __launch_bounds__(256) k_chol_pairs(int nblk, int2* __restrict__ pairs) {
+    const int kb = blockIdx.x * 256 + threadIdx.x;
+    if (kb >= nblk) return;
+    int i = chol_pair_start(nblk, kb);
+    for (int jb = kb + 1; jb < nblk; ++jb)
+        for (int ib = jb; ib < nblk; ++ib) pairs[i++] = make_int2(ib, jb);
+}
+
 void launch_cholesky(int n, const float* G, size_t strideG, double shift, double* A, size_t strideA,
                      int batch, int* d_info, int2* d_pairs, hipStream_t st) {
     dim3 g0((unsigned)(((size_t)n * n + 255) / 256), batch);
     k_to64_shift<<<g0, 256, 0, st>>>(n, G, strideG, shift, A, strideA);
     const int nblk = (n + kCB - 1) / kCB;
-    // all (ib, jb) pairs with jb <= ib, ordered so that pairs for step kb are a suffix
-    std::vector<int2> pairs;
-    std::vector<int> start(nblk + 1, 0);
-    for (int kb = 0; kb < nblk; ++kb) {
-        start[kb] = (int)pairs.size();
-        for (int jb = kb + 1; jb < nblk; ++jb)
-            for (int ib = jb; ib < nblk; ++ib) pairs.push_back(make_int2(ib, jb));
-    }
-    start[nblk] = (int)pairs.size();
-    if (!pairs.empty()) {
-        MPF_HIP(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-        MPF_HIP(hipStreamSynchronize(st));
-    }
+    k_chol_pairs<<<(nblk + 255) / 256, 256, 0, st>>>(nblk, d_pairs);
     for (int kb = 0; kb < nblk; ++kb) {
         k_potrf_diag<<<dim3(1, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_info);
         if (kb + 1 < nblk) {
             k_potrf_panel<<<dim3(nblk - kb - 1, 1, batch), 256, 0, st>>>(n, A, strideA, kb);
-            const int np = start[kb + 1] - start[kb];
-            k_potrf_update<<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + start[kb]);
+            const int p0 = chol_pair_start(nblk, kb), np = chol_pair_start(nblk, kb + 1) - p0;
+            k_potrf_update<<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + p0);
         }
     }
     check_launch();

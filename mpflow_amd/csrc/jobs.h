@@ -131,7 +131,5 @@ void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                        const int2* d_js, int njs, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,
                        hipStream_t st);
-void launch_spmv(int np, int width, const int32_t* ids, const float* S, const float* R,
-                 int C, const float* alpha, const float* X, float* Y, int Rp, hipStream_t st);
 
 }  // namespace mpf

@@ -144,7 +144,7 @@ constexpr int kSpmvRowsPerWave = 16;
 // inside the tile are read from LDS and only the others from memory: the gather for an
 // in-tile neighbour is sent past the buffer's range (answered with 0, no memory access).
 template <int LG, int W, bool XT, class Epi>
-__device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l64, const float* X,
+__device__ __forceinline__ void ell_gather_rows(int np, int row0, int l64, const float* X,
                                                 const int2 (*ell)[kSpmvRowsPerWave], const float4* xs,
                                                 int tile0, Epi&& epi) {
     constexpr int Rp = 1 << LG;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void ell_gather_rows(const JobDev& J, int row0, int l
     constexpr int RPS = 64 / LPR;                     // rows per wavefront step
     constexpr int STEPS = kSpmvRowsPerWave / RPS;
     const int rsub = l64 / LPR, c4 = l64 % LPR;
-    const uint32_t range = (uint32_t)((size_t)J.np * Rp * sizeof(float));
+    const uint32_t range = (uint32_t)((size_t)np * Rp * sizeof(float));
     const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(X, (size_t)range);
     // at most 4 (LDS tile) or 8 gathers (16 or 32 VGPRs) are in flight per lane, which
     // keeps the kernels at 8 wavefronts per SIMD
@@ -304,11 +304,11 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         }
     }
     __syncthreads();
-    if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
-    else ell_gather_rows<LG, WMAX, XT>(J, row0, l64, X, stage[wave], xs, tile0, epi);
+    if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
+    else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
+    else ell_gather_rows<LG, WMAX, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
 }
 
 // ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------

@@ -11,4 +11,12 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
                  const float* d_B, int ldb, float* d_X, int ldx, int dc_free, const mpf_solve_opts& o,
                  mpf_solve_stats* stats);
 
+// SparseMatrix::multiply on the reference layout (synchronises)
+void sparse_multiply(mpf_ctx* ctx, const float* d_values, const uint32_t* d_cid, const float* d_X,
+                     int result_rows, int matrix_rows, int columns, int density, float* d_Y);
+// multi-harmonic S/R SpMV on the product layout; alpha is a host array of C values
+// (synchronises)
+void ell_spmv(mpf_ctx* ctx, int np, int width, const int32_t* d_ids, const float* d_S, const float* d_R, int C,
+              const float* alpha, const float* d_X, float* d_Y, int ld);
+
 }  // namespace mpf

@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(kBlock) k_dinv(const JobDev* __restrict__ jobs
     const int row = t.z * kTileRows + threadIdx.x;
     const int s = t.y;
     float v = 0.0f;
-    if (row < J.n) {
+    // a row without a diagonal entry (diag_slot -1, caller matrices only) gets 0
+    if (row < J.n && J.diag_slot[row] >= 0) {
         const size_t e = (size_t)J.diag_slot[row] * J.np + row;
         const float a = J.s[e] + J.alpha[s] * J.r[e];
         v = a != 0.0f ? 1.0f / a : 0.0f;
@@ -828,30 +829,6 @@ __global__ void __launch_bounds__(kEwBlock) k_x64_update(const JobDev* __restric
 }
 
 // --------------------------------------------------------------------------------
-// stand-alone multi-harmonic ELL SpMV (no job table)
-// --------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kEwBlock) k_spmv_plain(int np, int width, const int32_t* __restrict__ ids,
-                                                         const float* __restrict__ S, const float* __restrict__ Rv,
-                                                         const float* __restrict__ alpha,
-                                                         const float* __restrict__ X, float* __restrict__ Y,
-                                                         int lg) {
-    const int Rp = 1 << lg;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> lg, G = kEwBlock >> lg;
-    const int c = blockIdx.y;
-    const float al = alpha[c];
-    const float* V = X + (size_t)c * np * Rp;
-    float* Out = Y + (size_t)c * np * Rp;
-    for (int row = blockIdx.x * kTileRows + g; row < (blockIdx.x + 1) * kTileRows; row += G) {
-        float acc = 0.0f;
-        for (int q = 0; q < width; ++q) {
-            const size_t e = (size_t)q * np + row;
-            acc += (S[e] + al * Rv[e]) * V[(size_t)ids[e] * Rp + lane];
-        }
-        Out[(size_t)row * Rp + lane] = acc;
-    }
-}
-
-// --------------------------------------------------------------------------------
 // launch wrappers
 // --------------------------------------------------------------------------------
 static inline void check_launch() {
@@ -993,12 +970,5 @@ void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, in
     check_launch();
 }
 
-void launch_spmv(int np, int width, const int32_t* ids, const float* S, const float* R, int C,
-                 const float* alpha, const float* X, float* Y, int Rp, hipStream_t st) {
-    const int lg = Rp == 16 ? 4 : (Rp == 32 ? 5 : 6);
-    dim3 grid(np / kTileRows, C);
-    k_spmv_plain<<<grid, kEwBlock, 0, st>>>(np, width, ids, S, R, alpha, X, Y, lg);
-    check_launch();
-}
 
 }  // namespace mpf

@@ -174,7 +174,8 @@ class LearnedReconstruction:
     def _gn(self, Um_t, steps, start=None):
         s = self.solver
         if start is None:
-            s.material_distribution.fill_(1.0)
+            # Solver's homogeneous start (solver.cpp:55-62): 1, or 0 when logarithmic
+            s.material_distribution.fill_(0.0 if self.eit.logarithmic else 1.0)
         else:
             s.material_distribution.copy_(start)
         s.measurement.copy_(Um_t)
@@ -193,7 +194,13 @@ class LearnedReconstruction:
         dev = self.eit._dev()
         Um_t = torch.tensor(Um, dtype=torch.float64, device=dev)
         X = torch.tensor(self.features(Um), dtype=torch.float32, device=dev)
-        start = torch.exp(self.net.forward(X)).to(torch.float32)
+        ln_sigma = self.net.forward(X)
+        # the network predicts ln(sigma / reference); gamma = sigma / reference, or
+        # 10 log10(sigma / reference) for a logarithmic model (equation_kernel.cu:35-43)
+        if self.eit.logarithmic:
+            start = (ln_sigma * (10.0 / math.log(10.0))).to(torch.float32)
+        else:
+            start = torch.exp(ln_sigma).to(torch.float32)
         g = self._gn(Um_t, self.gn_steps, start)
         r_learned = np.linalg.norm(Um - self.eit.result64.cpu().numpy(), axis=1)
         info = {"residual_learned": r_learned}

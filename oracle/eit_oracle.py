@@ -310,10 +310,14 @@ def ceil16(n):
 # ---------------------------------------------------------------------------
 
 def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
-                  rule="reference", tol=1e-6, prec=None):
+                  rule="reference", tol=1e-6, prec=None, x0=None, fp32=False):
     """Multi-RHS Jacobi-PCG as ``ConjugateGradient::solve`` runs it, in float64.
 
-    * x starts at 0;  dcFree adds ``1 1^T`` to the operator.
+    * x starts at ``x0`` (default 0) with r = b - A x0 (conjugate_gradient.cpp:80-82);
+      dcFree adds ``1 1^T`` to the operator, and the start residual then gets
+      ``+ 1 (1^T x0)`` -- addScalar adds the column sums of x0 (:84-88), the opposite
+      sign of the operator's term, so a nonzero start shifts the solution by
+      ``2 (1^T x0) / N`` when ``A 1 = 0``.
     * alpha = rho_old / (p.q) and beta = rho / rho_old per column; a zero denominator
       makes the updated vector 0 (``updateVectorKernel``, conjugate_gradient_kernel.cu:77-89).
     * ``rule="reference"``: stop after step s when sqrt(|rho[0]| / (b.b)[0]) < eps
@@ -321,20 +325,38 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
       ``rule="tolerance"``: stop when every column has ||r||/||b|| < tol.
     * ``prec`` (optional callable r -> z) replaces the diagonal preconditioner (the
       product's two-level preconditioner, ``two_level_preconditioner``).
+    * ``fp32``: the reference's own arithmetic -- matrix, right-hand side, vectors and the
+      Jacobi inverse rounded to float32 after every vector operation, alpha and beta
+      rounded to float32 before they scale a vector; dot products stay float64 sums of
+      the float32 values (the FP64 scalars of the device solver).  Step counts of the
+      FLT_EPSILON stop rule depend on this rounding.
     Returns (x, steps).
     """
     N, R = b.shape
     iters = 10 * N if max_iter == 0 else max_iter
     if dinv is None:
         dinv = np.ones(N)
+    if fp32:
+        rd = lambda v: np.asarray(v, np.float32).astype(np.float64)
+        A = sp.csr_matrix(A).astype(np.float32).astype(np.float64)
+        b = rd(b)
+        dinv = rd(dinv)
+    else:
+        rd = lambda v: v
     def op(v):
         y = A @ v
         if dc_free:
             y = y + v.sum(axis=0, keepdims=True)
-        return y
-    precond = prec if prec is not None else (lambda v: dinv[:, None] * v)
-    x = np.zeros((N, R))
-    r = b.copy()
+        return rd(y)
+    precond = prec if prec is not None else (lambda v: rd(dinv[:, None] * v))
+    if x0 is None:
+        x = np.zeros((N, R))
+        r = b.copy()
+    else:
+        x = np.array(x0, np.float64).reshape(N, R)
+        r = b - A @ x
+        if dc_free:
+            r = r + x.sum(axis=0, keepdims=True)
     z = precond(r)
     p = z.copy()
     rho_old = np.sum(r * z, axis=0)
@@ -344,8 +366,9 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         pq = np.sum(p * q, axis=0)
         with np.errstate(divide="ignore", invalid="ignore"):
             a = np.where(pq != 0.0, rho_old / np.where(pq != 0, pq, 1.0), 0.0)
-        r = np.where(pq != 0.0, r - a * q, 0.0)
-        x = np.where(pq != 0.0, x + a * p, 0.0)
+        a = rd(a)
+        r = rd(np.where(pq != 0.0, r - a * q, 0.0))
+        x = rd(np.where(pq != 0.0, x + a * p, 0.0))
         z = precond(r)
         rho = np.sum(r * z, axis=0)
         if rule == "reference":
@@ -359,8 +382,8 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
             if np.all(ok):
                 return x, step + 1
         with np.errstate(divide="ignore", invalid="ignore"):
-            beta = np.where(rho_old != 0.0, rho / np.where(rho_old != 0, rho_old, 1.0), 0.0)
-        p = np.where(rho_old != 0.0, z + beta * p, 0.0)
+            beta = rd(np.where(rho_old != 0.0, rho / np.where(rho_old != 0, rho_old, 1.0), 0.0))
+        p = rd(np.where(rho_old != 0.0, z + beta * p, 0.0))
         rho_old = rho
     return x, iters
 

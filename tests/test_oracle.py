@@ -147,6 +147,35 @@ def test_ports_product_matches_oracle():
         assert np.all((a >= 0).sum(axis=1) >= 1)
 
 
+@pytest.mark.parametrize("rings,L,w,p2", [(16, 16, 0.1, False), (16, 16, 0.1, True), (40, 32, 0.05, False),
+                                           (12, 8, 0.3, True)])
+def test_circular_boundary_tiles_each_arc(rings, L, w, p2):
+    """Ports::circularBoundary (ports.cpp:32-77) pinned by geometry: on a ring mesh whose
+    boundary nodes include every electrode corner, the edges selected for port l chain
+    end to end from the corner at angle 2 pi l / L to the corner at 2 pi l / L + w (they
+    tile the arc exactly), and no other boundary edge lies inside the arc."""
+    n, e = ring_mesh(rings, L, w)
+    if p2:
+        n, e = promote_p2(n, e)
+    bd = boundary_edges(e)
+    ports = eo.circular_boundary_ports(n, e, bd, L, w)
+    ang = lambda i: math.atan2(n[i, 1], n[i, 0])
+    for l in range(L):
+        t0 = 2 * math.pi * l / L
+        # angles measured counter-clockwise from the port's first corner, in [0, 2 pi)
+        rel = lambda i: (ang(i) - t0) % (2 * math.pi)
+        sel = [int(k) for k in ports[l] if k >= 0]
+        inside = [k for k in range(bd.shape[0])
+                  if rel(bd[k, 0]) <= w + 1e-12 and 0 < rel(bd[k, -1]) <= w + 1e-12
+                  and rel(bd[k, 0]) < rel(bd[k, -1])]
+        assert sorted(sel) == sorted(inside)
+        chain = sorted(sel, key=lambda k: rel(bd[k, 0]))
+        assert abs(rel(bd[chain[0], 0])) < 1e-12 or abs(rel(bd[chain[0], 0]) - 2 * math.pi) < 1e-12
+        assert abs(rel(bd[chain[-1], -1]) - w) < 1e-12
+        for a, b in zip(chain, chain[1:]):
+            assert bd[a, -1] == bd[b, 0]
+
+
 def test_splitmix64_known_answer():
     # reference outputs of SplitMix64 seeded with 1234567
     g = SplitMix64(1234567)
@@ -274,6 +303,49 @@ def test_bicgstab_tiny_spd_exact_in_few_steps():
     np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
 
 
+def test_pcg_start_vector(small):
+    """ConjugateGradient::solve starts from the caller's x (conjugate_gradient.cpp:80-88).
+    SPD harmonic: any start reaches the direct solution.  dcFree harmonic: the reference
+    forms the start residual b - A x0 + 1 (1^T x0) while the operator is A + 1 1^T, so the
+    solution moves by exactly 2 (1^T x0) / N per column (A 1 = 0); a start of 0 is the
+    default."""
+    p, o, g = small
+    rng = np.random.default_rng(3)
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    x0 = rng.standard_normal(F.shape)
+    A1 = o.system(g, 1)
+    X1 = eo.solve_direct(A1, F, False)
+    Xs, _ = eo.pcg_reference(A1, F, 1.0 / A1.diagonal(), False, rule="tolerance", tol=1e-13, x0=x0)
+    np.testing.assert_allclose(Xs, X1, atol=1e-10 * np.max(np.abs(X1)))
+    A0 = o.system(g, 0)
+    X0 = eo.solve_direct(A0, F, True)
+    Xd, _ = eo.pcg_reference(A0, F, 1.0 / A0.diagonal(), True, rule="tolerance", tol=1e-13, x0=x0)
+    shift = 2.0 * x0.sum(axis=0, keepdims=True) / o.N
+    np.testing.assert_allclose(Xd, X0 + shift, atol=1e-9 * np.max(np.abs(X0)))
+    Xz, sz = eo.pcg_reference(A0, F, 1.0 / A0.diagonal(), True, x0=np.zeros_like(F))
+    Xn, sn = eo.pcg_reference(A0, F, 1.0 / A0.diagonal(), True)
+    assert sz == sn and np.array_equal(Xz, Xn)
+
+
+def test_pcg_fp32_arithmetic_mode(small):
+    """The float32 mode of the restatement (the reference's arithmetic) reaches the FP64
+    solution to float32 accuracy, takes the FP64 step count where rounding does not bind
+    (the mass-dominated harmonic 1) and more steps than FP64 on the dcFree harmonic under
+    the FLT_EPSILON rule (float32 rounding slows the last digits)."""
+    p, o, g = small
+    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
+    A1 = o.system(g, 1)
+    X1 = eo.solve_direct(A1, F, False)
+    d1 = 1.0 / A1.diagonal()
+    x32, s32 = eo.pcg_reference(A1, F, d1, False, rule="tolerance", tol=1e-6, fp32=True)
+    assert np.max(np.abs(x32 - X1)) < 1e-5 * np.max(np.abs(X1))
+    assert x32.astype(np.float32).astype(np.float64).tolist() == x32.tolist()
+    assert eo.pcg_reference(A1, F, d1, False, fp32=True)[1] == eo.pcg_reference(A1, F, d1, False)[1]
+    A0 = o.system(g, 0)
+    d0 = 1.0 / A0.diagonal()
+    assert eo.pcg_reference(A0, F, d0, True, fp32=True)[1] > eo.pcg_reference(A0, F, d0, True)[1]
+
+
 def test_pcg_zero_rhs_stops_after_one_step(small):
     p, o, g = small
     A = o.system(g, 0)
@@ -370,6 +442,41 @@ def test_jacobian_2p5d_exact_vs_reference_formula():
     np.testing.assert_allclose(Jx[:, elems], Jfd, atol=1e-6 * np.max(np.abs(Jfd)))
     # the reference formula (K_e only, factor height) is not the derivative for C >= 2
     assert np.max(np.abs(Jr[:, elems] - Jfd)) > 1e-2 * np.max(np.abs(Jfd))
+
+
+def test_jacobian_reference_formula_is_frozen_mass_derivative():
+    """The reference Jacobian for C >= 2 (calcJacobianKernel with K_e only, summed over
+    harmonics with the factor height; equation_kernel.cu:197-245, eit.cpp:180-182,
+    202-205) pinned by finite differences: per harmonic c, U_c solved with
+    A_c = S(gamma) + alpha_c R(gamma_0) -- the mass matrix frozen at the expansion point so
+    only K_e carries sigma -- differentiated centrally and weighted by height * beta_c
+    (the reciprocity factor 1/beta_c of the derivative replaced by height), summed over c.
+    Height 2 and port height 0.5 keep alpha_c small enough that harmonics 1 and 2 carry a
+    large share of every column (weights 1, h or 1/beta_c miss by 5e-2 or more)."""
+    p = make_problem(6, 8, 0.3, C=3, height=2.0, port_height=0.5)
+    o = oracle_of(p)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 4))
+    U, phis, _ = o.solve(g)
+    Jr = o.jacobian(g, phis, quirks=True)
+    sig0 = eo.mat_param(g, o.ref, o.log)
+    R0 = eo.assemble(o.elements, o.Me, o.N, sig0)
+
+    def u_harmonic(gam, c):
+        S = eo.assemble(o.elements, o.Ke, o.N, eo.mat_param(gam, o.ref, o.log))
+        alpha, beta = o.alpha_beta(c)
+        X = eo.solve_direct((S + alpha * R0).tocsr(), beta * (o.B @ o.Pi), c == 0)
+        return o.flatten_voltages(o.W @ X[:, :o.D])
+
+    elems, eps = [3, 40, 158], 1e-5
+    for e in elems:
+        col = np.zeros(Jr.shape[0])
+        for c in range(o.C):
+            gp, gm = g.copy(), g.copy()
+            gp[e] += eps
+            gm[e] -= eps
+            beta = o.alpha_beta(c)[1]
+            col += o.h * beta * (u_harmonic(gp, c) - u_harmonic(gm, c)) / (2 * eps)
+        np.testing.assert_allclose(Jr[:, e], col, atol=1e-6 * np.max(np.abs(col)))
 
 
 def test_gn_data_space_identity():
