@@ -326,9 +326,11 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     * ``prec`` (optional callable r -> z) replaces the diagonal preconditioner (the
       product's two-level preconditioner, ``two_level_preconditioner``).
     * ``fp32``: the reference's own arithmetic -- matrix, right-hand side, vectors and the
-      Jacobi inverse rounded to float32 after every vector operation, alpha and beta
-      rounded to float32 before they scale a vector; dot products stay float64 sums of
-      the float32 values (the FP64 scalars of the device solver).  Step counts of the
+      Jacobi inverse rounded to float32; A v summed row by row in float32 over the row's
+      entries in stored order, each product rounded before it is added (the product's
+      SpMV); the other vector operations rounded to float32 once, alpha and beta rounded
+      to float32 before they scale a vector; dot products stay float64 sums of the
+      float32 values (the FP64 scalars of the device solver).  Step counts of the
       FLT_EPSILON stop rule depend on this rounding.
     Returns (x, steps).
     """
@@ -341,10 +343,26 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         A = sp.csr_matrix(A).astype(np.float32).astype(np.float64)
         b = rd(b)
         dinv = rd(dinv)
+        # the rows as (column, value) slots in stored order, padded with (row, 0)
+        nnz_row = np.diff(A.indptr)
+        width = int(nnz_row.max()) if N else 0
+        ell_id = np.repeat(np.arange(N)[:, None], width, axis=1)
+        ell_v = np.zeros((N, width))
+        for i in range(N):
+            s0, s1 = A.indptr[i], A.indptr[i + 1]
+            ell_id[i, :s1 - s0] = A.indices[s0:s1]
+            ell_v[i, :s1 - s0] = A.data[s0:s1]
+
+        def matvec(v):
+            acc = np.zeros((N, v.shape[1]), np.float32)
+            for q in range(width):
+                acc = acc + (ell_v[:, q:q + 1] * v[ell_id[:, q]]).astype(np.float32)
+            return acc.astype(np.float64)
     else:
         rd = lambda v: v
+        matvec = lambda v: A @ v
     def op(v):
-        y = A @ v
+        y = matvec(v)
         if dc_free:
             y = y + v.sum(axis=0, keepdims=True)
         return rd(y)

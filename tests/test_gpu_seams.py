@@ -198,8 +198,9 @@ def test_pcg_solve_from_caller_start(gpu_ctx, sysm, precision):
 def test_reference_stop_rule_step_count(gpu_ctx, sysm, contrast):
     """Reference stop rule (sqrt(|r.z| / b.b) of column 0 below FLT_EPSILON) on the dcFree
     harmonic: the device's step count equals the float32 restatement's within 4 steps.
-    Variants of the rule move the count by more: all columns +13/+15, r.r instead of r.z
-    +3/+11, no square root -90/-55 (float32 restatement, contrast 1 / 100)."""
+    Variants of the rule move the restatement's count by more: every column instead of
+    column 0 +14/+16, r.r instead of r.z +5/+11, no square root -89/-54 (contrast 1 / 100);
+    the same rule on the last column instead of column 0 (+1/+4) is not told apart."""
     from mpflow_amd import capi
     o, g = sysm
     gg = np.where(g > 1.5, contrast, 1.0) if contrast != 1.0 else g
