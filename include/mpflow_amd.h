@@ -413,6 +413,18 @@ int mpf_spd_solve(mpf_ctx* ctx, int32_t n, const float* d_G, double shift, const
 int mpf_normal_solve(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J, int32_t ldj, double shift,
                      const double* d_r, int32_t gn_refine, double* d_delta);
 
+/* mpf_normal_solve without any allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and result as
+ * mpf_normal_solve (the Gram kernel sums its contraction in at most 4 slabs, so results
+ * agree to rounding), computed in a fixed 24 MiB work space that each GPU holds from the
+ * library's first context on it until the process ends.  Limits: rows <= 1024,
+ * cols <= 65536, rows and ldj multiples of 4; larger problems are refused (MPF_EINVAL).
+ * Every call on one GPU shares that GPU's work space: calls that overlap on one GPU (two
+ * streams, two host threads) race.  Synchronises. */
+int mpf_normal_solve_noalloc(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J, int32_t ldj,
+                             double shift, const double* d_r, int32_t gn_refine, double* d_delta);
+
 /* ---------------------------------------------------------------------------
  * Learned start: a multilayer perceptron (dense layers, GELU between them, linear
  * output) trained with MSE and Adam.  No counterpart in the reference.  GEMMs run on

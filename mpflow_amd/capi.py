@@ -25,6 +25,7 @@ EXPORTED = [
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
     "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian", "mpf_gn_set_regularization", "mpf_diff_prepare",
     "mpf_diff_solve", "mpf_normal_solve", "mpf_set_sensitivity_mask", "mpf_sparse_multiply",
+    "mpf_normal_solve_noalloc",
 ]
 
 
@@ -120,6 +121,7 @@ def lib() -> C.CDLL:
     L.mpf_diff_solve.argtypes = [vp, vp, i32, vp]
     L.mpf_set_sensitivity_mask.argtypes = [vp, vp]
     L.mpf_normal_solve.argtypes = [vp, i32, i32, vp, i32, dbl, vp, i32, vp]
+    L.mpf_normal_solve_noalloc.argtypes = [vp, i32, i32, vp, i32, dbl, vp, i32, vp]
     _lib = L
     return L
 

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "mlp.h"
+#include "noalloc.h"
 #include "fem.h"
 #include "numeric.h"
 #include "model.h"
@@ -62,6 +63,8 @@ int mpf_ctx_create(int device, void* hip_stream, mpf_ctx** out) {
         auto* c = new mpf_ctx;
         c->device = device;
         MPF_HIP(hipSetDevice(device));
+        // the device address of this GPU's fixed work space (mpf_normal_solve_noalloc)
+        mpf::noalloc_scratch(device);
         if (hip_stream) {
             c->stream = (hipStream_t)hip_stream;
             c->own_stream = false;
@@ -500,6 +503,18 @@ int mpf_normal_solve(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J,
         MPF_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
         MPF_HIP(hipStreamSynchronize(st));
         MPF_REQUIRE(h == 0, "mpf_normal_solve: normal-equation matrix is not positive definite");
+        return MPF_OK;
+    });
+}
+
+int mpf_normal_solve_noalloc(mpf_ctx* ctx, int32_t rows, int32_t cols, const float* d_J, int32_t ldj,
+                             double shift, const double* d_r, int32_t gn_refine, double* d_delta) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_J && d_r && d_delta, "mpf_normal_solve_noalloc: null argument");
+        MPF_REQUIRE(rows > 0 && cols > 0 && ldj >= rows && gn_refine >= 0 && shift > 0.0,
+                    "mpf_normal_solve_noalloc: bad shape, shift or refinement count");
+        set_device(ctx);
+        mpf::normal_solve_noalloc(ctx, rows, cols, d_J, ldj, shift, d_r, gn_refine, d_delta);
         return MPF_OK;
     });
 }
