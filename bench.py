@@ -14,9 +14,10 @@ Prints ONE JSON line (rank 0):
   value        frames/s of the whole job (1 GPU)
   roofline     dominant kernel (ELL SpMV of the PCG): algorithmic bytes / event-timed
                launch time vs 8 TB/s HBM3E
-  cpu_baseline the same ELL PCG (same preconditioner; FP32 vectors, FP64 scalars, OpenMP)
-               on the host cores, on a bounded sample of harmonic-0 iterations, scaled
-               to frames/s with the device run's iteration counts
+  cpu_baseline whole frames of the same algorithm on the host cores (oracle/gn_host.cpp:
+               two-level FP32 PCG with FP64 scalars, warm starts and FP64 refinement over
+               all harmonics, exact Jacobian, data-space Tikhonov step), one frame per
+               thread; value = frames / wall seconds of that sample
 """
 from __future__ import annotations
 
@@ -79,46 +80,50 @@ def build_problem(w):
     return nodes, el, bd, ports, pat
 
 
-def cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters_per_frame, budget_s=12.0, two_level=False):
-    """Time the oracle's C++ ELL-PCG on harmonic 0 of the workload mesh at gamma = 1, with
-    the preconditioner the device run used (two-level: nodes in RCB order, aggregates of
-    256 consecutive rows, as on the device)."""
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def cpu_baseline(nodes, el, bd, ports, pat, w, Um, gn_steps, lam, tol, two_level=True, contact_impedance=0.0,
+                 cem="coupled"):
+    """Time whole frames of the same reconstruction on the host: oracle/gn_host.cpp runs
+    `gn_steps` Gauss-Newton steps per frame (FP32 PCG with FP64 scalars to `tol`, warm
+    started from the previous step's FP64 fields, one FP64 refinement pass to 1e-2 of
+    that; exact 2.5D Jacobian; FP32 Gram J J^T, FP64 Cholesky, 4 FP64-residual passes),
+    one frame per OpenMP thread, on the first `threads` measured frames of the run."""
     import numpy as np
-    from oracle import baseline
-    from oracle.eit_oracle import EITOracle, rcb_order, two_level_preconditioner
-    o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
-    A = o.system(np.ones(el.shape[0]), 0)
-    F = o.alpha_beta(0)[1] * (o.B @ o.Pi)
-    einv = None
-    if two_level:
-        order = rcb_order(nodes, 256)
-        A = A[order][:, order].tocsr()
-        F = F[order]
-        einv = two_level_preconditioner(A, A, A.diagonal(), A.diagonal(), True, tile=256)[0].einv
-    # the right-hand sides the device solves: columns that are scalar multiples of an
-    # earlier column share one solve (here the measurement patterns repeat the drives)
-    nrm = F / np.where(np.abs(F).max(axis=0) > 0, F[np.abs(F).argmax(axis=0), np.arange(F.shape[1])], 1.0)
-    keep = []
-    for j in range(F.shape[1]):
-        if not any(np.max(np.abs(nrm[:, j] - nrm[:, i])) <= 1e-12 for i in keep):
-            keep.append(j)
-    F = np.ascontiguousarray(F[:, keep])
-    baseline.pcg(A, F, 1, tol=0.0, max_iter=5, einv=einv)                 # first touch
-    _, s0, t0 = baseline.pcg(A, F, 1, tol=0.0, max_iter=40, einv=einv)
-    per = t0 / max(s0, 1)
-    n_it = int(max(10, min(20000, budget_s / max(per, 1e-6))))
-    _, steps, t = baseline.pcg(A, F, 1, tol=0.0, max_iter=n_it, einv=einv)
-    per_iter = t / steps
+    from oracle.gn_host import GnHost, threads
+    nthr = threads()
+    h = GnHost(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+               contact_impedance=contact_impedance, two_level=two_level, tol=tol, refine=1, refine_tol=1e-2,
+               gn_refine=4, lam=lam, cem=cem)
+    Um = np.asarray(Um)[:nthr]
+    g, secs = h.reconstruct(Um, steps=gn_steps, threads=nthr)
+    name, ncpu = cpu_model()
     return {
-        "value": 1.0 / (per_iter * slice_iters_per_frame),
+        "value": round(Um.shape[0] / secs, 4),
         "unit": "frames/s",
-        "cores": baseline.threads(),
+        "cores": nthr,
         "kind": "port",
-        "sample": ("%d %s-PCG iterations of harmonic 0 (N=%d, %d RHS, FP32 vectors, FP64 scalars) "
-                   "on the workload mesh, %.3f s; frames/s = 1 / (s per iteration x %d slice-iterations "
-                   "per frame measured on the device); Jacobian and dense GN work not included"
-                   % (steps, "two-level" if two_level else "Jacobi", A.shape[0], F.shape[1], t,
-                      slice_iters_per_frame)),
+        "sample": ("%d whole frames (%d Gauss-Newton steps each, all %d harmonics) of the workload, one per "
+                   "OpenMP thread, %.1f s wall: oracle/gn_host.cpp, the device path's algorithm on the host "
+                   "(%s-PCG, FP32 vectors, FP64 scalars, tol %g + 1 FP64 refinement, exact Jacobian, FP32 "
+                   "Gram + FP64 Cholesky, g++ -O3 -march=x86-64-v3)"
+                   % (Um.shape[0], gn_steps, w["C"], secs, "two-level" if two_level else "Jacobi", tol)),
+        "cpu_model": name,
+        "nproc": ncpu,
+        "threads": nthr,
+        "gamma": g,
     }
 
 
@@ -400,6 +405,12 @@ def main():
                     help="skip the FP64 oracle reconstruction of one frame (about a minute on config 3)")
     ap.add_argument("--probe", action="store_true", help="short counter probe for rocprofv3 --pmc (no JSON)")
     ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
+    ap.add_argument("--contact-impedance", type=float, default=0.01,
+                    help="> 0: complete electrode model with this contact impedance (0: the reference's "
+                         "gap model)")
+    ap.add_argument("--cem", default="coupled", choices=["coupled", "harmonic"],
+                    help="complete electrode model: one potential per electrode for all harmonics "
+                         "(coupled) or a potential per harmonic (harmonic)")
     ap.add_argument("--precond", default="two-level", choices=["two-level", "jacobi"],
                     help="forward PCG preconditioner (jacobi: the reference's)")
     args = ap.parse_args()
@@ -430,8 +441,9 @@ def main():
     cfg = CONFIG_INDEX[args.workload]
     nodes, el, bd, ports, pat = build_problem(w)
     B = max(1, args.batch)
+    z = args.contact_impedance
     eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
-              jacobian_exact=True, frames=B)
+              jacobian_exact=True, frames=B, contact_impedance=z, cem=args.cem)
     dev = torch.device("cuda", 0)
     two_level = args.precond == "two-level"
     opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=args.tol, precision=capi.MPF_PREC_MIXED,
@@ -451,6 +463,8 @@ def main():
         meas.append(torch.tensor(U, dtype=torch.float64, device=dev))
     solver = Solver(eit, args.lam, gn_refine=4, options=opts)
 
+    not_converged = [0]
+
     def batch(bi):
         solver.material_distribution.fill_(1.0)
         solver.measurement.copy_(meas[bi])
@@ -460,6 +474,7 @@ def main():
             # step's fields of the same frame; every frame starts cold
             solver.solve_absolute(warm_start=(k > 0 and not args.cold))
             steps += solver.last_stats.steps
+            not_converged[0] += solver.last_stats.not_converged
         return steps
 
     for bi in range(args.warmup):
@@ -479,15 +494,22 @@ def main():
     spmv_ms, spmv_launches, spmv_bytes = prof[0], int(prof[1]), prof[2]
     # the last timed batch's first frame against the FP64 oracle's 3-step reconstruction
     # from the same measured voltages (untimed)
-    err64 = None
+    err64 = err64_d = g64 = None
     if not args.no_fp64_check:
-        from oracle.eit_oracle import EITOracle, reconstruct
+        from oracle.eit_oracle import CoupledCEMOracle, EITOracle, reconstruct
         print("bench: FP64 oracle reconstruction of one frame", file=sys.stderr, flush=True)
         g_dev = solver.material_distribution[0].cpu().numpy().astype(np.float64)
-        o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
+        if z > 0.0 and args.cem == "coupled":
+            o = CoupledCEMOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                                 contact_impedance=z)
+        else:
+            o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                          contact_impedance=z)
         Um = meas[nbatches - 1][0].cpu().numpy().reshape(eit.Dp, eit.Mp)[:eit.D, :eit.M].T
         g64 = reconstruct(o, Um, np.ones(eit.E), args.lam, steps=args.gn_steps, quirks_jacobian=False)
         err64 = float(np.linalg.norm(g_dev - g64) / np.linalg.norm(g64))
+        # the update gamma - 1 alone (gamma is 1 + a small update)
+        err64_d = float(np.linalg.norm(g_dev - g64) / np.linalg.norm(g64 - 1.0))
 
     frames_timed = args.steps * B
     fps = frames_timed / elapsed
@@ -541,16 +563,32 @@ def main():
             "rhs": eit.R, "rhs_solved": eit.R_unique, "jacobian_rows": eit.m,
             "frames_per_launch": B,
             "slice_iterations_per_frame": slice_iters // frames_timed,
+            "electrode_model": ("complete electrode model, z_c = %g, %s" % (
+                z, "one potential per electrode for all harmonics (Schur complement on the electrode "
+                   "potentials)" if args.cem == "coupled" else "a potential per electrode and harmonic")
+                if z > 0.0 else "gap model (reference)"),
+            "not_converged": not_converged[0],
             "reconstruction_rel_l2_vs_fp64": None if err64 is None else float("%.3g" % err64),
+            "update_rel_l2_vs_fp64": None if err64_d is None else float("%.3g" % err64_d),
             "fp64_reference": "the oracle's FP64 3-step Gauss-Newton (sparse LU forward solves, exact "
-                              "2.5D Jacobian, data-space Cholesky) on the last timed batch's frame 0",
+                              "2.5D Jacobian, data-space Cholesky) on the last timed batch's frame 0; "
+                              "update = the error relative to ||gamma_64 - 1||",
         },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(nodes, el, bd, ports, pat, w, slice_iters / frames_timed,
-                                           two_level=two_level)
-    print(json.dumps(out))
+        print("bench: host baseline frames", file=sys.stderr, flush=True)
+        cb = cpu_baseline(nodes, el, bd, ports, pat, w, meas[nbatches - 1].cpu().numpy(), args.gn_steps, args.lam,
+                          args.tol, two_level=two_level, contact_impedance=z, cem=args.cem)
+        gh = cb.pop("gamma")
+        if g64 is not None:
+            cb["rel_l2_vs_fp64"] = float("%.3g" % (np.linalg.norm(gh[0] - g64) / np.linalg.norm(g64)))
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    if not_converged[0]:
+        print("bench: %d (frame, harmonic) forward solves hit the iteration cap or broke down"
+              % not_converged[0], file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

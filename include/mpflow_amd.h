@@ -163,6 +163,18 @@ typedef struct mpf_model_desc {
                                      height * Mpat^T B_clip^T (A_c Phi_c) summed over the
                                      harmonics (the electrode currents); Jacobian x height
                                      (not -height).  Gap model only; jacobian_exact = 0 */
+    int32_t cem_coupled;          /* with contact_impedance > 0.  0: harmonic-wise CEM (above).
+                                     1: each electrode holds ONE potential V_l for all
+                                     harmonics (the patch e_l x height port_height): mode c
+                                     solves (A_c + Zb / z') u_c = (h beta_c / z') Bc V with
+                                     z' = z height / port_height (the patch average of the
+                                     mode's contact weight), and the electrodes close the system
+                                     sum_c (n_c / z') [(h beta_c)^2 |e| V - h beta_c Bc^T u_c] = I
+                                     (n_c = height, then height / 2); solved by one Krylov solve
+                                     per harmonic with the L unit electrode sources Bc and a
+                                     Schur complement on V.  Voltages: Mpat^T V.  Needs
+                                     jacobian_exact = 1 (weights n_c); mpf_eit_measure and
+                                     mpf_jacobian (per-harmonic seams) are refused. */
 } mpf_model_desc;
 
 typedef struct mpf_solve_opts {

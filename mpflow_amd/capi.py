@@ -42,6 +42,7 @@ class ModelDesc(C.Structure):
         ("port_height", C.c_double), ("logarithmic", C.c_int32), ("quirks", C.c_int32),
         ("jacobian_exact", C.c_int32), ("ell_max_width", C.c_int32), ("frames", C.c_int32),
         ("keep_node_order", C.c_int32), ("contact_impedance", C.c_double), ("source_type", C.c_int32),
+        ("cem_coupled", C.c_int32),
     ]
 
 

@@ -190,6 +190,7 @@ void fem_assemble(Model& M, const float* d_gamma, double alpha, uint32_t* d_ids,
 
 void fem_measure(Model& M, const float* d_phi, int ldphi, int additive, float* d_U) {
     const MeshSetup& S = M.S;
+    MPF_REQUIRE(!S.cem_coupled, "mpf_eit_measure: the coupled electrode model has no per-harmonic measurement");
     MPF_REQUIRE(ldphi >= S.N, "fem_measure: ldphi < rows of the fields");
     const int n = S.Mp * S.D;
     k_fem_measure<<<(n + 255) / 256, 256, 0, M.ctx->stream>>>(S.D, S.M, S.Mp, M.w_off.p, M.w_node.p, M.w_val.p,
@@ -198,6 +199,7 @@ void fem_measure(Model& M, const float* d_phi, int ldphi, int additive, float* d
 }
 
 void fem_jacobian(Model& M, const float* d_phi, int ldphi, const float* d_gamma, int additive, float* d_J) {
+    MPF_REQUIRE(!M.S.cem_coupled, "mpf_jacobian: the coupled electrode model has no per-harmonic Jacobian seam");
     const MeshSetup& S = M.S;
     MPF_REQUIRE(ldphi >= S.N, "fem_jacobian: ldphi < rows of the fields");
     MPF_REQUIRE(S.k <= 6 && S.D <= 64 && S.M <= 64, "fem_jacobian: unsupported shape");

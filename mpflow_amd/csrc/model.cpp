@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "cem.h"
 #include "model.h"
 #include "fem.h"
 
@@ -104,7 +105,8 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     for (int f = 0; f < frames; ++f) {
         JobDev& J = hjobs[f];
         std::memset(&J, 0, sizeof(J));
-        J.n = S.N; J.np = np; J.width = W; J.C = S.C; J.R = S.Ru; J.Rp = S.Rp; J.col0 = S.colmap[0];
+        J.n = S.N; J.np = np; J.width = W; J.C = S.C; J.R = S.Ru; J.Rp = S.Rp;
+        J.col0 = S.cem_coupled ? 0 : S.colmap[0];
         J.lg = S.Rp == 16 ? 4 : (S.Rp == 32 ? 5 : 6);
         J.ntiles = ntile_job;
         J.dc = 0;  // Open sources: harmonic 0 is solved dcFree (eit.cpp:175-177)
@@ -119,6 +121,9 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
             J.alpha64[c] = S.alpha[c]; J.xscale64[c] = S.beta[c];
         }
         J.F = F.p;
+        if (S.cem_coupled)
+            // the unit electrode sources Bc, the same for every harmonic
+            for (int c = 0; c < S.C; ++c) { J.xscale[c] = 1.0f; J.xscale64[c] = 1.0; }
         if (S.fixed) {
             // voltage sources: one excitation per slice, no beta scaling
             J.F = fslice.p + (size_t)S.C * np * S.Rp * f;
@@ -151,6 +156,27 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     }
     jobs.alloc(frames);
     upload_jobs();
+    if (S.cem_coupled) {
+        const int L = S.P;
+        bc_off.alloc(S.bc_off.size()); bc_off.upload(S.bc_off.data(), S.bc_off.size(), st0);
+        bc_row.alloc(std::max<size_t>(1, S.bc_row.size())); bc_row.upload(S.bc_row.data(), S.bc_row.size(), st0);
+        bc_val.alloc(std::max<size_t>(1, S.bc_val.size())); bc_val.upload(S.bc_val.data(), S.bc_val.size(), st0);
+        elen.alloc(L); elen.upload(S.elen.data(), L, st0);
+        pi.alloc(S.pi.size()); pi.upload(S.pi.data(), S.pi.size(), st0);
+        prep.alloc(S.pRu); prep.upload(S.prep.data(), S.pRu, st0);
+        cem_T.alloc((size_t)frames * S.C * L * S.pRp);
+        cem_phi.alloc((size_t)frames * S.C * np * S.pRp);
+        hjobs_phi = hjobs;
+        for (int f = 0; f < frames; ++f) {
+            JobDev& J = hjobs_phi[f];
+            J.Rp = S.pRp;
+            J.R = S.pRu;
+            J.lg = S.pRp == 16 ? 4 : (S.pRp == 32 ? 5 : 6);
+            J.x64 = cem_phi.p + (size_t)S.C * np * S.pRp * f;
+        }
+        jobs_phi.alloc(frames);
+        jobs_phi.upload(hjobs_phi.data(), frames, st0);
+    }
 
     // work tables
     std::vector<int4> ht;
@@ -197,7 +223,8 @@ void Model::prepare(const mpf_solve_opts& o) {
     for (auto& J : hjobs) {
         J.dc = (o.krylov == MPF_KRYLOV_BICGSTAB && o.bicgstab_ignore_dc) ? -1 : 0;
         J.dc_rank1 = o.krylov == MPF_KRYLOV_CG ? 1 : 0;
-        if (S.fixed) { J.dc = -1; J.dc_rank1 = 0; }   // Dirichlet rows: no constant mode
+        // Dirichlet rows, or the contact term of every mode: no constant mode
+        if (S.fixed || S.cem_coupled) { J.dc = -1; J.dc_rank1 = 0; }
     }
     upload_jobs();
     launch_assemble(jobs.p, asm_items.p, nasm, st0);
@@ -231,8 +258,23 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
         stats->refine_residual = worst;
     }
     if (S.fixed) launch_fixed_currents(jobs.p, frames, fixdev(), mixed ? 1 : 0, st0);
-    MeasDev md{w_off.p, w_node.p, w_val.p, colmap.p, colscale.p, S.D, S.M, S.Dp, S.Mp};
-    launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
+    if (S.cem_coupled) {
+        // Schur complement on the electrode potentials: voltages, then the pattern fields
+        CemDev cd;
+        std::memset(&cd, 0, sizeof(cd));
+        cd.bc_off = bc_off.p; cd.bc_row = bc_row.p; cd.bc_val = bc_val.p; cd.elen = elen.p;
+        cd.pi = pi.p; cd.prep = prep.p; cd.zp = S.zp;
+        cd.L = S.P; cd.R = S.R; cd.D = S.D; cd.M = S.M; cd.Dp = S.Dp; cd.Mp = S.Mp; cd.pRu = S.pRu; cd.pRp = S.pRp;
+        for (int c = 0; c < S.C; ++c) {
+            const double hb = S.height * S.beta[c];
+            cd.coef[c] = S.n_c[c] * hb * hb / S.zp;
+            cd.tsc[c] = hb / S.zp;
+        }
+        launch_cem(jobs.p, frames, S.C, S.np, cd, mixed ? 1 : 0, d_U, d_U64, cem_T.p, cem_phi.p, st0);
+    } else {
+        MeasDev md{w_off.p, w_node.p, w_val.p, colmap.p, colscale.p, S.D, S.M, S.Dp, S.Mp};
+        launch_voltages(jobs.p, frames, md, mixed ? 1 : 0, d_U, d_U64, st0);
+    }
     if (d_J) {
         JacDev jd;
         std::memset(&jd, 0, sizeof(jd));
@@ -246,8 +288,11 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
             // voltage sources: jacobian x height without the current sources' sign
             // (eit.cpp:202-205); the kernel applies -sigma' w
             if (S.fixed) jd.weight[c] = -(float)S.height;
+            // coupled electrodes: the mode weights n_c of the symmetric block system
+            if (S.cem_coupled) jd.weight[c] = (float)S.n_c[c];
         }
-        launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
+        if (S.cem_coupled) launch_jacobian(jobs_phi.p, frames, jd, 1, d_J, st0);
+        else launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
     }
 }
 

@@ -54,6 +54,13 @@ struct Model {
     DBuf<int4> tiles, asm_items;
     DBuf<int2> js;
     int ntiles = 0, nasm = 0, njs = 0;
+    // coupled complete electrode model (S.cem_coupled): sources, electrode lengths, Pi,
+    // representative pattern columns, T [frames][C][L][pRp], the pattern fields
+    // phi [frames][C][np][pRp] and a job table pointing the Jacobian at them
+    DBuf<int32_t> bc_off, bc_row, prep;
+    DBuf<double> bc_val, elen, pi, cem_T, cem_phi;
+    std::vector<JobDev> hjobs_phi;
+    DBuf<JobDev> jobs_phi;
     // Gauss-Newton buffers (allocated on first use)
     DBuf<float> Jm, G;
     DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;

@@ -121,6 +121,19 @@ struct MeshSetup {
     std::vector<double> cpl_k, cpl_m;
     std::vector<int32_t> drow_off, drow_node, drow_elem;
     std::vector<double> drow_k, drow_m;
+    // coupled complete electrode model (cem_coupled): contact impedance z' of every mode,
+    // electrode lengths |e_l|, the unit electrode sources Bc as CSR over ports (device
+    // rows), n_c = int cos^2, Pi (P x R row-major), the pattern's unique columns (pRu, lane
+    // count pRp) and the representative column of each
+    int cem_coupled = 0;
+    double zp = 0.0;
+    std::vector<double> elen;
+    std::vector<int32_t> bc_off, bc_row;
+    std::vector<double> bc_val;
+    double n_c[kMaxSlices] = {};
+    std::vector<double> pi;
+    int pRu = 0, pRp = 0;
+    std::vector<int32_t> prep;
     // laplacian regularisation matrix: diagonal [E], neighbours across the three vertex
     // edges [E][3] (-1: boundary edge) and their entries [E][3]
     std::vector<double> lap_diag;

@@ -336,7 +336,7 @@ static void coarse_setup(MeshSetup& s) {
                 if (s.ctb_off[ent] == s.ctb_off[ent + 1] || s.ids[ent] >= s.N) continue;   // padding slot
                 Ec[(size_t)agg[row] * na + agg[s.ids[ent]]] += sk[ent] + s.alpha[c] * sm[ent];
             }
-        if (c == 0 && !s.fixed)
+        if (c == 0 && !s.fixed && !s.cem_coupled)
             for (int t = 0; t < na; ++t)
                 for (int u = 0; u < na; ++u) Ec[(size_t)t * na + u] += cnt[t] * cnt[u];
         // aggregates without real rows (the second half of a one-row tile) keep an
@@ -468,8 +468,15 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     const int N = d.n_nodes, E = d.n_elements, k = d.nodes_per_element;
     MPF_REQUIRE(d.contact_impedance >= 0.0, "model: contact_impedance < 0");
     const bool cem = d.contact_impedance > 0.0;
-    // unknowns: the mesh nodes, then (complete electrode model) one potential per port
-    const int Nu = N + (cem ? d.n_ports : 0);
+    // the coupled electrode model eliminates the electrode potentials (Schur complement on
+    // V after the solves): its unknowns are the mesh nodes alone
+    const bool coupled = cem && d.cem_coupled != 0;
+    MPF_REQUIRE(d.cem_coupled == 0 || cem, "model: cem_coupled needs contact_impedance > 0");
+    MPF_REQUIRE(!coupled || d.jacobian_exact, "model: the coupled electrode model needs jacobian_exact = 1");
+    s.cem_coupled = coupled ? 1 : 0;
+    // unknowns: the mesh nodes, then (harmonic-wise complete electrode model) one potential
+    // per port
+    const int Nu = N + (cem && !coupled ? d.n_ports : 0);
     s.N = Nu; s.E = E; s.k = k;
     s.np = round_up(Nu, kTileRows);
     s.P = d.n_ports; s.D = d.n_drive; s.M = d.n_meas; s.R = s.D + s.M;
@@ -509,6 +516,15 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         s.Ru = (int)uniq.size();
     }
     s.Rp = s.Ru <= 16 ? 16 : (s.Ru <= 32 ? 32 : 64);
+    if (coupled) {
+        // the pattern columns are combinations of the L unit electrode fields: the solves
+        // run one lane per port; the pattern's unique columns index the combined fields
+        MPF_REQUIRE(d.n_ports <= 64, "model: the coupled electrode model supports at most 64 ports");
+        s.pRu = s.Ru;
+        s.pRp = s.Rp;
+        s.Ru = d.n_ports;
+        s.Rp = s.Ru <= 16 ? 16 : (s.Ru <= 32 ? 32 : 64);
+    }
     s.Mp = ceil16(s.M); s.Dp = ceil16(s.D); s.mrows = s.Mp * s.Dp;
     s.C = d.components;
     s.ref = d.reference_value; s.height = d.height; s.port_height = d.port_height;
@@ -569,7 +585,42 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     // int phi_i and |e|/z on the port edges, kept as (row, slot, value) contributions
     struct Const { int row, sl; double v; };
     std::vector<Const> consts;
-    if (cem) {
+    if (coupled) {
+        // Zb / z' on the port edges; the sources Bc and the electrode block live in the
+        // Schur complement (bc_*, elen)
+        s.zp = d.contact_impedance * d.height / d.port_height;
+        const double iz = 1.0 / s.zp;
+        const int kb = d.nodes_per_edge;
+        s.elen.assign(s.P, 0.0);
+        std::vector<std::map<int, double>> bc(s.P);
+        for (int p = 0; p < s.P; ++p)
+            for (int q = 0; q < d.port_max_edges; ++q) {
+                const int eidx = d.port_edges[(size_t)p * d.port_max_edges + q];
+                if (eidx < 0) continue;
+                MPF_REQUIRE(eidx < d.n_boundary, "model: port edge index out of range");
+                const int32_t* ed = d.boundary + (size_t)eidx * kb;
+                double sp[3], m[3][3], v[3];
+                for (int a = 0; a < kb; ++a) {
+                    const double dx = d.nodes[2 * ed[a]] - d.nodes[2 * ed[0]];
+                    const double dy = d.nodes[2 * ed[a] + 1] - d.nodes[2 * ed[0] + 1];
+                    sp[a] = std::sqrt(dx * dx + dy * dy);
+                }
+                edge_integrals(kb, sp, m, v);
+                for (int a = 0; a < kb; ++a) {
+                    for (int b = 0; b < kb; ++b) consts.push_back({ed[a], slot(ed[a], ed[b]), iz * m[a][b]});
+                    bc[p][ed[a]] += v[a];
+                }
+                s.elen[p] += sp[kb - 1];
+            }
+        s.bc_off.assign(s.P + 1, 0);
+        s.bc_row.clear();
+        s.bc_val.clear();
+        for (int p = 0; p < s.P; ++p) {
+            for (auto& kv : bc[p]) { s.bc_row.push_back(kv.first); s.bc_val.push_back(kv.second); }
+            s.bc_off[p + 1] = (int32_t)s.bc_row.size();
+        }
+        for (int c = 0; c < s.C; ++c) s.n_c[c] = c == 0 ? d.height : 0.5 * d.height;
+    } else if (cem) {
         const double iz = 1.0 / d.contact_impedance;
         const int kb = d.nodes_per_edge;
         for (int p = 0; p < s.P; ++p) {
@@ -647,7 +698,7 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     // complete electrode model the current enters at the electrode rows instead: B = [0; I].
     const int kb = d.nodes_per_edge;
     std::vector<std::map<int, double>> Bn(Nu);
-    if (cem)
+    if (cem && !coupled)
         for (int p = 0; p < s.P; ++p) Bn[N + p][p] = 1.0;
     for (int p = 0; p < (cem ? 0 : s.P); ++p) {
         std::vector<int> edges;
@@ -706,6 +757,24 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
         return j < s.D ? d.drive_current[j] * d.drive_pattern[(size_t)p * s.D + j]
                        : d.meas_pattern[(size_t)p * s.M + (j - s.D)];
     };
+    if (coupled) {
+        // the solves' right-hand sides: the L unit electrode sources Bc[:, l]; Pi and the
+        // representative of each unique pattern column for the Schur complement
+        for (int p = 0; p < s.P; ++p)
+            for (int t = s.bc_off[p]; t < s.bc_off[p + 1]; ++t) s.F[(size_t)s.bc_row[t] * s.Rp + p] = s.bc_val[t];
+        s.pi.assign((size_t)s.P * s.R, 0.0);
+        for (int p = 0; p < s.P; ++p)
+            for (int j = 0; j < s.R; ++j) s.pi[(size_t)p * s.R + j] = Pi(p, j);
+        s.prep.assign(s.pRu, -1);
+        for (int j = 0; j < s.R; ++j)
+            if (s.prep[s.colmap[j]] < 0) s.prep[s.colmap[j]] = j;
+        for (int u = 0; u < s.pRu; ++u) s.colscale[s.prep[u]] = 1.0;
+        s.w_off.assign(s.M + 1, 0);
+        s.w_node.clear(); s.w_val.clear();
+        coarse_setup(s);
+        laplacian_setup(d, s);
+        return;
+    }
     std::vector<int> rep(s.Ru, -1);   // first original column of each unique column
     for (int j = 0; j < s.R; ++j)
         if (rep[s.colmap[j]] < 0) rep[s.colmap[j]] = j;

@@ -99,7 +99,7 @@ class EIT:
     def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
                  quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None,
-                 keep_node_order=False, contact_impedance=0.0, source="current"):
+                 keep_node_order=False, contact_impedance=0.0, source="current", cem="harmonic"):
         import torch
         self.ctx = ctx or Context()
         self.torch = torch
@@ -132,6 +132,10 @@ class EIT:
         d.ell_max_width, d.frames = int(ell_max_width), int(frames)
         d.keep_node_order = int(bool(keep_node_order))
         d.contact_impedance = float(contact_impedance)
+        if cem not in ("harmonic", "coupled"):
+            raise ValueError("cem must be 'harmonic' or 'coupled'")
+        d.cem_coupled = int(cem == "coupled" and contact_impedance > 0.0)
+        self.cem = cem if contact_impedance > 0.0 else None
         if source not in ("current", "voltage"):
             raise ValueError("source must be 'current' or 'voltage'")
         d.source_type = capi.MPF_SOURCE_VOLTAGE if source == "voltage" else capi.MPF_SOURCE_CURRENT

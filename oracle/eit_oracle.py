@@ -728,6 +728,110 @@ class EITOracle:
         return out.reshape(-1)
 
 
+class CoupledCEMOracle(EITOracle):
+    """2.5D complete electrode model in which each electrode carries ONE potential V_l for
+    all harmonics (an extension; the reference has the gap model only).  The cylinder of
+    height h carries u(x, y, z) = sum_c u_c(x, y) cos(k_c z), k_c = 2 pi c / h; electrode l
+    is the patch e_l x [-h_p/2, h_p/2] with contact impedance z.  Each mode c has its own
+    contact term against its Fourier share h beta_c V of the electrode potential, with the
+    patch average of cos^2 (int_patch cos^2(k_c z) dz ~ n_c h_p / h, exact for c = 0), so
+    all modes see z' = z h / h_p.  The energy
+
+        sum_c n_c [ u_c^T A_c u_c + (1/z') int_e (u_c - h beta_c V)^2 ] - 2 I^T V
+
+    (n_c = int_0^h cos^2(k_c z) dz: h, then h/2; A_c = S + alpha_c R, the reference's
+    harmonic matrices, eit.cpp:154-157; beta_c its excitation scale) gives
+
+        n_c (A_c + Zb / z') u_c - (n_c h beta_c / z') Bc V = 0            c = 0 .. C-1
+        sum_c (n_c / z') [ (h beta_c)^2 |e| V - h beta_c Bc^T u_c ] = I
+
+    Zb[i, j] = int_e phi_i phi_j, Bc[i, l] = int_{e_l} phi_i, |e|_l = |e_l|: symmetric and
+    positive definite for C >= 2 (positive semidefinite with the constant null mode
+    u_0 = 1, V = 1 for C = 1, where it is the harmonic-wise model with current I / h).
+    The whole block system is solved at once here; the product solves it by a Schur
+    complement on V.  Voltages: U = Mpat^T V.  Jacobian (exact): dU_md / dgamma_e =
+    -sigma'_e sum_c n_c u_c^(m)T (K_e + alpha_c M_e) u_c^(d), u_c^(x) the mode-c field of
+    electrode currents x (reciprocity of the symmetric block system)."""
+
+    def __init__(self, nodes, elements, boundary, ports, drive, meas, current, reference_value=1.0,
+                 components=1, height=1.0, port_height=1.0, logarithmic=False, contact_impedance=0.01):
+        super().__init__(nodes, elements, boundary, ports, drive, meas, current, reference_value, components,
+                         height, port_height, quirks=False, logarithmic=logarithmic, contact_impedance=0.0)
+        self.z = float(contact_impedance)
+        if not self.z > 0.0:
+            raise ValueError("the coupled complete electrode model needs contact_impedance > 0")
+        self.zp = self.z * self.h / self.hp
+        # the blocks for unit contact impedance: Zb, Bc, |e|
+        Zb, Bc, el = cem_blocks(self.nodes, self.boundary, self.ports, 1.0)
+        self.Zb, self.Bc, self.elen = Zb.tocsr(), Bc, el
+        self.n_c = np.array([self.h if c == 0 else 0.5 * self.h for c in range(self.C)])
+
+    def system(self, gamma, c):
+        """The mode-c block n_c (A_c + Zb / z') (not the coupled system)."""
+        sig = mat_param(np.asarray(gamma, np.float64), self.ref, self.log)
+        alpha, _ = self.alpha_beta(c)
+        A = assemble(self.elements, self.Ke, self.N, sig) + alpha * assemble(self.elements, self.Me, self.N, sig)
+        return (self.n_c[c] * (A + self.Zb / self.zp)).tocsr()
+
+    def coupled_system(self, gamma):
+        N, L, C = self.N, self.L, self.C
+        blocks = [[None] * (C + 1) for _ in range(C + 1)]
+        for c in range(C):
+            _, beta = self.alpha_beta(c)
+            blocks[c][c] = self.system(gamma, c)
+            cpl = sp.csr_matrix(-(self.n_c[c] * self.h * beta / self.zp) * self.Bc)
+            blocks[c][C] = cpl
+            blocks[C][c] = cpl.T
+        vv = sum(self.n_c[c] * (self.h * self.alpha_beta(c)[1]) ** 2 for c in range(C))
+        blocks[C][C] = sp.diags((vv / self.zp) * self.elen)
+        return sp.bmat(blocks).tocsr()
+
+    def fields(self, gamma, method="direct", **cg):
+        """Mode fields u_c for the currents Pi (N x (D+M) each) and the electrode
+        potentials V (L x (D+M))."""
+        if method != "direct":
+            raise ValueError("coupled CEM: direct solves only")
+        N, L, C = self.N, self.L, self.C
+        H = self.coupled_system(gamma)
+        rhs = np.zeros((C * N + L, self.Pi.shape[1]))
+        rhs[C * N:] = self.Pi
+        if C == 1:
+            # the constant mode: currents summing to zero are consistent; the solution
+            # orthogonal to the null vector is taken
+            nv = np.zeros(C * N + L)
+            nv[:N] = 1.0
+            nv[C * N:] = 1.0
+            nv /= np.linalg.norm(nv)
+            rhs = rhs - np.outer(nv, nv @ rhs)
+            H = H + sp.csr_matrix(np.outer(nv, nv))
+        X = spla.spsolve(H.tocsc(), rhs)
+        X = np.asarray(X).reshape(C * N + L, -1)
+        phis = [X[c * N:(c + 1) * N] for c in range(C)]
+        self.V = X[C * N:]
+        return phis, 0
+
+    def voltages(self, phis, gamma=None):
+        return self.meas.T @ self.V[:, :self.D]
+
+    def solve(self, gamma, method="direct", **cg):
+        phis, steps = self.fields(gamma, method, **cg)
+        return self.voltages(phis), phis, steps
+
+    def jacobian(self, gamma, phis, quirks=None):
+        ds = mat_param_deriv(np.asarray(gamma, np.float64), self.ref, self.log)
+        J = np.zeros((self.Dp, self.Mp, self.E))
+        el = self.elements
+        for c, X in enumerate(phis):
+            alpha, _ = self.alpha_beta(c)
+            Kc = self.Ke + alpha * self.Me
+            Pd = X[el][:, :, :self.D]
+            Pm = X[el][:, :, self.D:]
+            T = np.einsum("eij,ejm->eim", Kc, Pm)
+            J[:self.D, :self.M, :] += self.n_c[c] * np.einsum("eid,eim->dme", Pd, T)
+        J *= -ds[None, None, :]
+        return J.reshape(self.Dp * self.Mp, self.E)
+
+
 # ---------------------------------------------------------------------------
 # Tikhonov Gauss-Newton (src/solver/inverse.cpp:60-238, src/solver/solver.cpp:142-225)
 # ---------------------------------------------------------------------------

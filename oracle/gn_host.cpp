@@ -56,6 +56,21 @@ struct GnhProblem {
     int refine;                       // FP64 refinement passes per forward solve
     int gn_refine;                    // FP64-residual passes of the normal-equation solve
     double lam2;                      // (reference lambda)^2
+    // coupled complete electrode model (cem_coupled): the solves' right-hand sides F are
+    // the L unit electrode sources Bc (no scaling), the electrode potentials are
+    // eliminated per frame by a Schur complement, and the fields of the pattern's unique
+    // columns are combinations of the unit fields (colmap / colscale then index those)
+    int cem_coupled;
+    int L, pRu;
+    const int32_t* bc_off;            // [L+1] Bc as CSR over ports
+    const int32_t* bc_row;
+    const double* bc_val;
+    const double* elen;               // [L] electrode lengths
+    const double* pi;                 // [L][D+M] Pi = [I Dpat | Mpat]
+    const int32_t* prep;              // [pRu] representative column of each unique pattern column
+    const double* coef;               // [C] n_c (h beta_c)^2 / z'
+    const double* tsc;                // [C] h beta_c / z'
+    double zp;
 };
 
 int gnh_threads(void) { return omp_get_max_threads(); }
@@ -216,7 +231,7 @@ void chol_solve(int n, const std::vector<double>& L, const double* b, double* y)
 }
 
 // one frame: `steps` Gauss-Newton steps from gamma (in/out)
-void frame(const GnhProblem& P, const double* Um, int steps, float* gamma) {
+void frame(const GnhProblem& P, const double* Um, int steps, float* gamma, double* Uout, float* Jout) {
     const int N = P.N, E = P.E, k = P.k, R = P.Ru, C = P.C, kk = k * k;
     const int m = P.Dp * P.Mp;
     const size_t V = (size_t)N * R;
@@ -255,7 +270,7 @@ void frame(const GnhProblem& P, const double* Um, int steps, float* gamma) {
         std::fill(U.begin(), U.end(), 0.0);
         std::fill(J.begin(), J.end(), 0.0f);
         for (int c = 0; c < C; ++c) {
-            const bool dc = c == 0;
+            const bool dc = c == 0 && !P.cem_coupled;
             for (int q = 0; q < P.nnz; ++q) {
                 A64[q] = S[q] + P.alpha[c] * Rm[q];
                 A32[q] = (float)A64[q];
@@ -302,43 +317,117 @@ void frame(const GnhProblem& P, const double* Um, int steps, float* gamma) {
                     dx.data(), w, 10 * N);
                 for (size_t o = 0; o < V; ++o) x[o] += dx[o];
             }
-            // measured voltages U[d * Mp + mm] += sum_node W[mm][node] Phi[node][d]
-            for (int d = 0; d < P.D; ++d) {
-                const int ld = P.colmap[d];
-                const double sd = P.colscale[d];
-                for (int mm = 0; mm < P.M; ++mm) {
-                    double s = 0.0;
-                    for (int t = P.w_off[mm]; t < P.w_off[mm + 1]; ++t) s += P.w_val[t] * x[(size_t)P.w_node[t] * R + ld];
-                    U[(size_t)d * P.Mp + mm] += sd * s;
+        }
+        // the fields the Jacobian contracts, one lane per (unique) pattern column: the
+        // solved fields themselves, or (coupled electrodes) their combinations Y_c T_c
+        std::vector<double> Tm;
+        if (P.cem_coupled) {
+            const int L = P.L;
+            std::vector<double> A((size_t)L * 2 * L, 0.0);
+            for (int l = 0; l < L; ++l)
+                for (int l2 = 0; l2 < L; ++l2) {
+                    double v = 0.0;
+                    for (int c = 0; c < C; ++c) {
+                        double g = 0.0;
+                        for (int t = P.bc_off[l]; t < P.bc_off[l + 1]; ++t)
+                            g += P.bc_val[t] * x64[(size_t)c * V + (size_t)P.bc_row[t] * R + l2];
+                        v += P.coef[c] * ((l == l2 ? P.elen[l] : 0.0) - g / P.zp);
+                    }
+                    A[(size_t)l * 2 * L + l2] = v;
+                    A[(size_t)l * 2 * L + L + l2] = l == l2 ? 1.0 : 0.0;
+                }
+            if (C == 1) {
+                double tr = 0.0;
+                for (int l = 0; l < L; ++l) tr += A[(size_t)l * 2 * L + l];
+                for (int l = 0; l < L; ++l)
+                    for (int l2 = 0; l2 < L; ++l2) A[(size_t)l * 2 * L + l2] += tr / L;
+            }
+            for (int kk2 = 0; kk2 < L; ++kk2) {
+                const double piv = A[(size_t)kk2 * 2 * L + kk2];
+                for (int j = 0; j < 2 * L; ++j) A[(size_t)kk2 * 2 * L + j] /= piv;
+                for (int i = 0; i < L; ++i) {
+                    if (i == kk2) continue;
+                    const double f = A[(size_t)i * 2 * L + kk2];
+                    for (int j = 0; j < 2 * L; ++j) A[(size_t)i * 2 * L + j] -= f * A[(size_t)kk2 * 2 * L + j];
                 }
             }
-            // exact Jacobian: J[(d, m), e] -= sigma'_e (1/beta_c) Phi_d^T (K_e + alpha_c M_e) Phi_m
-            std::vector<double> phi((size_t)k * R), T((size_t)k * R), Gl((size_t)R * R);
+            const int Rt = P.D + P.M;
+            auto winv_pi = [&](int l, int col) {
+                double v = 0.0;
+                for (int l2 = 0; l2 < L; ++l2) v += A[(size_t)l * 2 * L + L + l2] * P.pi[(size_t)l2 * Rt + col];
+                return v;
+            };
+            for (int d = 0; d < P.D; ++d) {
+                std::vector<double> Vd(L);
+                for (int l = 0; l < L; ++l) Vd[l] = winv_pi(l, d);
+                for (int mm = 0; mm < P.M; ++mm) {
+                    double v = 0.0;
+                    for (int l = 0; l < L; ++l) v += P.pi[(size_t)l * Rt + P.D + mm] * Vd[l];
+                    U[(size_t)d * P.Mp + mm] = v;
+                }
+            }
+            Tm.assign((size_t)L * P.pRu, 0.0);
+            for (int l = 0; l < L; ++l)
+                for (int u = 0; u < P.pRu; ++u) Tm[(size_t)l * P.pRu + u] = winv_pi(l, P.prep[u]);
+        }
+        const int RJ = P.cem_coupled ? P.pRu : R;
+        std::vector<double> phiall;
+        for (int c = 0; c < C; ++c) {
+            const double* xc = &x64[(size_t)c * V];
+            const double* fields = xc;
+            if (P.cem_coupled) {
+                phiall.assign((size_t)N * RJ, 0.0);
+                for (int i = 0; i < N; ++i)
+                    for (int u = 0; u < RJ; ++u) {
+                        double v = 0.0;
+                        for (int l = 0; l < P.L; ++l) v += xc[(size_t)i * R + l] * Tm[(size_t)l * RJ + u];
+                        phiall[(size_t)i * RJ + u] = P.tsc[c] * v;
+                    }
+                fields = phiall.data();
+            } else {
+                // measured voltages U[d * Mp + mm] += sum_node W[mm][node] Phi[node][d]
+                for (int d = 0; d < P.D; ++d) {
+                    const int ld = P.colmap[d];
+                    const double sd = P.colscale[d];
+                    for (int mm = 0; mm < P.M; ++mm) {
+                        double s2 = 0.0;
+                        for (int t = P.w_off[mm]; t < P.w_off[mm + 1]; ++t) s2 += P.w_val[t] * xc[(size_t)P.w_node[t] * R + ld];
+                        U[(size_t)d * P.Mp + mm] += sd * s2;
+                    }
+                }
+            }
+            // exact Jacobian: J[(d, m), e] -= sigma'_e w_c Phi_d^T (K_e + alpha_c M_e) Phi_m
+            std::vector<double> phi((size_t)k * RJ), T((size_t)k * RJ), Gl((size_t)RJ * RJ);
             for (int e = 0; e < E; ++e) {
                 const int32_t* el = &P.elements[(size_t)e * k];
                 for (int i = 0; i < k; ++i)
-                    for (int col = 0; col < R; ++col) phi[(size_t)i * R + col] = x[(size_t)el[i] * R + col];
+                    for (int col = 0; col < RJ; ++col) phi[(size_t)i * RJ + col] = fields[(size_t)el[i] * RJ + col];
                 for (int i = 0; i < k; ++i)
-                    for (int col = 0; col < R; ++col) {
-                        double s = 0.0;
+                    for (int col = 0; col < RJ; ++col) {
+                        double s2 = 0.0;
                         for (int j = 0; j < k; ++j)
-                            s += (P.Ke[(size_t)e * kk + i * k + j] + P.alpha[c] * P.Me[(size_t)e * kk + i * k + j]) *
-                                 phi[(size_t)j * R + col];
-                        T[(size_t)i * R + col] = s;
+                            s2 += (P.Ke[(size_t)e * kk + i * k + j] + P.alpha[c] * P.Me[(size_t)e * kk + i * k + j]) *
+                                  phi[(size_t)j * RJ + col];
+                        T[(size_t)i * RJ + col] = s2;
                     }
-                for (int a = 0; a < R; ++a)
-                    for (int b = 0; b < R; ++b) {
-                        double s = 0.0;
-                        for (int i = 0; i < k; ++i) s += phi[(size_t)i * R + a] * T[(size_t)i * R + b];
-                        Gl[(size_t)a * R + b] = s;
+                for (int a = 0; a < RJ; ++a)
+                    for (int b = 0; b < RJ; ++b) {
+                        double s2 = 0.0;
+                        for (int i = 0; i < k; ++i) s2 += phi[(size_t)i * RJ + a] * T[(size_t)i * RJ + b];
+                        Gl[(size_t)a * RJ + b] = s2;
                     }
                 const double wgt = -dsig[e] * P.jw[c];
                 for (int d = 0; d < P.D; ++d)
                     for (int mm = 0; mm < P.M; ++mm)
                         J[(size_t)e * m + d * P.Mp + mm] +=
                             (float)(wgt * P.colscale[d] * P.colscale[P.D + mm] *
-                                    Gl[(size_t)P.colmap[d] * R + P.colmap[P.D + mm]]);
+                                    Gl[(size_t)P.colmap[d] * RJ + P.colmap[P.D + mm]]);
             }
+        }
+        // the voltages and Jacobian of the last step, when asked for
+        if (step == steps - 1) {
+            if (Uout) std::copy(U.begin(), U.end(), Uout);
+            if (Jout) std::copy(J.begin(), J.end(), Jout);
         }
         // data-space Tikhonov step: (J J^T + lam2 I) y = Um - U, delta = J^T y
         for (int i = 0; i < m; ++i) r[i] = Um[i] - U[i];
@@ -373,13 +462,18 @@ void frame(const GnhProblem& P, const double* Um, int steps, float* gamma) {
 }  // namespace
 
 // `frames` reconstructions of `steps` Gauss-Newton steps each, one frame per thread;
-// gamma [frames][E] in/out, Um [frames][Dp*Mp].  Returns the wall time in seconds.
-double gnh_reconstruct(const GnhProblem* P, int frames, const double* Um, int steps, float* gamma, int threads) {
+// gamma [frames][E] in/out, Um [frames][Dp*Mp].  Uout [frames][m] and Jout [frames][E][m]
+// (nullable) receive the voltages and the Jacobian of each frame's last step.  Returns
+// the wall time in seconds.
+double gnh_reconstruct(const GnhProblem* P, int frames, const double* Um, int steps, float* gamma, int threads,
+                       double* Uout, float* Jout) {
     if (threads > 0) omp_set_num_threads(threads);
     const int m = P->Dp * P->Mp;
     const auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int f = 0; f < frames; ++f) frame(*P, Um + (size_t)f * m, steps, gamma + (size_t)f * P->E);
+    for (int f = 0; f < frames; ++f)
+        frame(*P, Um + (size_t)f * m, steps, gamma + (size_t)f * P->E, Uout ? Uout + (size_t)f * m : nullptr,
+              Jout ? Jout + (size_t)f * P->E * m : nullptr);
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -17,7 +17,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import build_baseline
-from .eit_oracle import EITOracle, all_element_integrals, rcb_order, tile_aggregates
+from .eit_oracle import CoupledCEMOracle, EITOracle, all_element_integrals, rcb_order, tile_aggregates
 
 _I, _D, _F = C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_float)
 
@@ -34,6 +34,9 @@ class _Problem(C.Structure):
         ("na", C.c_int), ("astart", _I), ("cinv", _F), ("d1", _D),
         ("tol", C.c_double), ("refine_tol", C.c_double), ("refine", C.c_int), ("gn_refine", C.c_int),
         ("lam2", C.c_double),
+        ("cem_coupled", C.c_int), ("L", C.c_int), ("pRu", C.c_int),
+        ("bc_off", _I), ("bc_row", _I), ("bc_val", _D), ("elen", _D), ("pi", _D), ("prep", _I),
+        ("coef", _D), ("tsc", _D), ("zp", C.c_double),
     ]
 
 
@@ -43,7 +46,7 @@ def _lib():
         build_baseline.build()
     L = C.CDLL(path)
     L.gnh_reconstruct.restype = C.c_double
-    L.gnh_reconstruct.argtypes = [C.POINTER(_Problem), C.c_int, _D, C.c_int, _F, C.c_int]
+    L.gnh_reconstruct.argtypes = [C.POINTER(_Problem), C.c_int, _D, C.c_int, _F, C.c_int, _D, _F]
     L.gnh_threads.restype = C.c_int
     return L
 
@@ -81,7 +84,7 @@ class GnHost:
 
     def __init__(self, nodes, elements, boundary, ports, drive, meas, current, reference_value=1.0,
                  components=1, height=1.0, port_height=1.0, contact_impedance=0.0, two_level=True,
-                 tol=1e-4, refine=1, refine_tol=1e-2, gn_refine=4, lam=1e-3):
+                 tol=1e-4, refine=1, refine_tol=1e-2, gn_refine=4, lam=1e-3, cem="harmonic"):
         nodes = np.asarray(nodes, np.float64)
         order = rcb_order(nodes, 256)
         perm = np.empty_like(order)
@@ -89,12 +92,21 @@ class GnHost:
         nodes = nodes[order]
         elements = perm[np.asarray(elements)]
         boundary = perm[np.asarray(boundary)]
-        o = EITOracle(nodes, elements, boundary, ports, drive, meas, current, reference_value, components, height,
-                      port_height, quirks=False, contact_impedance=contact_impedance)
+        coupled = contact_impedance > 0.0 and cem == "coupled"
+        if coupled:
+            o = CoupledCEMOracle(nodes, elements, boundary, ports, drive, meas, current, reference_value, components,
+                                 height, port_height, contact_impedance=contact_impedance)
+        else:
+            o = EITOracle(nodes, elements, boundary, ports, drive, meas, current, reference_value, components,
+                          height, port_height, quirks=False, contact_impedance=contact_impedance)
         self.o = o
-        N = o.N + (o.L if o.z > 0.0 else 0)       # CEM: the electrode potentials follow the nodes
+        self.coupled = coupled
+        # harmonic-wise CEM: the electrode potentials follow the nodes
+        N = o.N + (o.L if o.z > 0.0 and not coupled else 0)
         k = o.k
-        A1 = o.system(np.ones(o.E), 0).tocsr()
+        # coupled: the mode blocks without their factor n_c, A_c + Zb / z'
+        sys1 = (lambda c: o.system(np.ones(o.E), c) / o.n_c[c]) if coupled else (lambda c: o.system(np.ones(o.E), c))
+        A1 = sys1(0).tocsr()
         A1.sort_indices()
         self.N, self.E, self.k = N, o.E, k
         Ke, Me = o.Ke, o.Me
@@ -112,6 +124,7 @@ class GnHost:
         self.S0 = None
         if o.z > 0.0:
             S1 = (A1 - self._mesh_part(o, np.ones(o.E), N)).tocsr()
+            S1.eliminate_zeros()
             S1.sort_indices()
             S0 = np.zeros(indices.size)
             for r in range(N):
@@ -130,10 +143,21 @@ class GnHost:
         self.alpha = np.ascontiguousarray([a for a, _ in ab], np.float64)
         self.beta = np.ascontiguousarray([b for _, b in ab], np.float64)
         self.jw = np.ascontiguousarray(1.0 / self.beta)
-        F = o.B @ o.Pi
-        keep, colmap, colscale = _unique_columns(F)
-        self.F = np.ascontiguousarray(F[:, keep])
+        if coupled:
+            # unit electrode sources, unscaled; Jacobian weights n_c; the pattern's unique
+            # columns index the combined fields
+            F = o.Bc
+            self.F = np.ascontiguousarray(F)
+            self.jw = np.ascontiguousarray(o.n_c, np.float64)
+            keep, colmap, colscale = _unique_columns(o.Pi)
+            self.beta_rhs = np.ones(C_)
+        else:
+            F = o.B @ o.Pi
+            keep, colmap, colscale = _unique_columns(F)
+            self.F = np.ascontiguousarray(F[:, keep])
+            self.beta_rhs = self.beta
         self.colmap, self.colscale = colmap, colscale
+        self.keep = keep
         Wd = np.asarray(o.W)
         Wc = sp.csr_matrix(Wd)
         self.w_off = np.ascontiguousarray(Wc.indptr, np.int32)
@@ -151,11 +175,11 @@ class GnHost:
             n_a = np.asarray(W.sum(axis=0)).ravel()
             cinv = []
             for c in range(C_):
-                E1 = (W.T @ o.system(np.ones(o.E), c) @ W).toarray()
-                if c == 0:
+                E1 = (W.T @ sys1(c) @ W).toarray()
+                if c == 0 and not coupled:
                     E1 = E1 + np.outer(n_a, n_a)
                 cinv.append(np.linalg.inv(E1))
-            S1 = o.system(np.ones(o.E), 0).diagonal()
+            S1 = sys1(0).diagonal()
             self.na = na
             self.astart = np.ascontiguousarray(np.r_[0, np.cumsum(np.bincount(agg, minlength=na))], np.int32)
             self.cinv = np.ascontiguousarray(np.stack(cinv), np.float32)
@@ -169,7 +193,8 @@ class GnHost:
         p.rowptr, p.col, p.emap = (a.ctypes.data_as(_I) for a in (self.rowptr, self.col, self.emap))
         p.S0 = self.S0.ctypes.data_as(_D) if self.S0 is not None else None
         p.C = C_
-        p.alpha, p.beta, p.jw = (a.ctypes.data_as(_D) for a in (self.alpha, self.beta, self.jw))
+        self.beta_rhs = np.ascontiguousarray(self.beta_rhs, np.float64)
+        p.alpha, p.beta, p.jw = (a.ctypes.data_as(_D) for a in (self.alpha, self.beta_rhs, self.jw))
         p.Ru = self.F.shape[1]
         p.F = self.F.ctypes.data_as(_D)
         p.D, p.M, p.Dp, p.Mp = o.D, o.M, o.Dp, o.Mp
@@ -182,6 +207,25 @@ class GnHost:
         p.tol, p.refine_tol, p.refine, p.gn_refine = tol, refine_tol, refine, gn_refine
         p.lam2 = (o.ref * lam) ** 2
         self.m = o.Dp * o.Mp
+        p.cem_coupled = int(coupled)
+        if coupled:
+            L = o.L
+            Bc = sp.csc_matrix(o.Bc)
+            self.bc_off = np.ascontiguousarray(Bc.indptr, np.int32)
+            self.bc_row = np.ascontiguousarray(Bc.indices, np.int32)
+            self.bc_val = np.ascontiguousarray(Bc.data, np.float64)
+            self.elen = np.ascontiguousarray(o.elen, np.float64)
+            self.pi = np.ascontiguousarray(o.Pi, np.float64)
+            self.prep = np.ascontiguousarray(keep, np.int32)
+            hb = np.array([o.h * b for b in self.beta])
+            self.coef = np.ascontiguousarray(o.n_c * hb * hb / o.zp)
+            self.tsc = np.ascontiguousarray(hb / o.zp)
+            p.L, p.pRu = L, len(keep)
+            p.bc_off, p.bc_row = self.bc_off.ctypes.data_as(_I), self.bc_row.ctypes.data_as(_I)
+            p.bc_val, p.elen = self.bc_val.ctypes.data_as(_D), self.elen.ctypes.data_as(_D)
+            p.pi, p.prep = self.pi.ctypes.data_as(_D), self.prep.ctypes.data_as(_I)
+            p.coef, p.tsc = self.coef.ctypes.data_as(_D), self.tsc.ctypes.data_as(_D)
+            p.zp = o.zp
 
     @staticmethod
     def _mesh_part(o, gamma, N):
@@ -191,14 +235,21 @@ class GnHost:
         S.resize((N, N))
         return S
 
-    def reconstruct(self, Umeas, steps=3, threads=0):
+    def reconstruct(self, Umeas, steps=3, threads=0, outputs=False):
         """Umeas: frames x m flattened measured voltages (EIT::result layout).  Returns
-        (gamma frames x E, wall seconds)."""
+        (gamma frames x E, wall seconds), and with ``outputs`` also the voltages
+        (frames x m) and Jacobian (frames x m x E) of each frame's last step."""
         Um = np.ascontiguousarray(np.atleast_2d(Umeas), np.float64)
         frames = Um.shape[0]
         g = np.ones((frames, self.E), np.float32)
+        Uo = np.zeros((frames, self.m)) if outputs else None
+        Jo = np.zeros((frames, self.E, self.m), np.float32) if outputs else None
         t = _lib().gnh_reconstruct(C.byref(self.p), frames, Um.ctypes.data_as(_D), int(steps),
-                                   g.ctypes.data_as(_F), int(threads))
+                                   g.ctypes.data_as(_F), int(threads),
+                                   Uo.ctypes.data_as(_D) if outputs else None,
+                                   Jo.ctypes.data_as(_F) if outputs else None)
+        if outputs:
+            return g.astype(np.float64), t, Uo, np.transpose(Jo, (0, 2, 1)).astype(np.float64)
         return g.astype(np.float64), t
 
 

@@ -93,3 +93,80 @@ def test_cem_gn_step(gpu_ctx):
     # J J^T + lambda^2 I; the reconstruction (gamma) is the quantity held to 1e-3
     assert err < 3e-3
     assert err_g < 1e-5
+
+
+def _coupled_pair(p, gpu_ctx, frames=1):
+    import oracle.eit_oracle as eo
+    from mpflow_amd.eit import EIT
+    o = eo.CoupledCEMOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
+                            p["current"], 1.0, p["C"], p["height"], p["port_height"], contact_impedance=Z)
+    eit = EIT(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"], 1.0,
+              p["C"], p["height"], p["port_height"], jacobian_exact=True, frames=frames, contact_impedance=Z,
+              cem="coupled", ctx=gpu_ctx)
+    return o, eit
+
+
+@pytest.mark.parametrize("C,h,hp,mode", [(1, 1.0, 1.0, "fp32"), (3, 0.4, 0.1, "fp32"), (5, 0.4, 0.1, "mixed"),
+                                         (3, 2.0, 0.5, "mixed")])
+def test_cem_coupled_forward_and_jacobian(gpu_ctx, C, h, hp, mode):
+    """Electrodes at one potential for all harmonics (Schur complement on V): voltages and
+    Jacobian of the device against the oracle's direct solve of the whole block system."""
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import SolveOptions
+    p = make_problem(16, 16, 0.1, C=C, height=h, port_height=hp)
+    o, eit = _coupled_pair(p, gpu_ctx)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 2))
+    U, phis, _ = o.solve(g)
+    Jo = o.jacobian(g, phis)
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    if mode == "mixed":
+        opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                            refine_steps=2, refine_tol=1e-3)
+    print("case: coupled cem C=%d h=%g hp=%g %s" % (C, h, hp, mode), flush=True)
+    eit.solve(torch.tensor(g, dtype=torch.float32), opts, jacobian=True)
+    Ug = eit.voltages().cpu().numpy()[0]
+    Jg = eit.jacobian_matrix().cpu().numpy()[0]
+    eu = _rel(Ug, U)
+    ej = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
+    print("voltages rel err %.2e, jacobian %.2e, steps %d, not converged %d"
+          % (eu, ej, eit.last_stats.steps, eit.last_stats.not_converged), flush=True)
+    assert eit.last_stats.not_converged == 0
+    assert eu < (1e-5 if mode == "fp32" else 1e-7)
+    assert ej < 1e-4
+    assert _rel(Ug, Ug.T) < 1e-5
+
+
+def test_cem_coupled_c2_reconstruction(gpu_ctx):
+    """Config 2 with the coupled electrode model (40-ring mesh, 32 electrodes, 5 harmonics,
+    z_c = 0.01): 3 Gauss-Newton steps, two frames per launch, against the oracle's FP64
+    reconstruction."""
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import SolveOptions, Solver
+    from oracle.eit_oracle import reconstruct
+    p = make_problem(40, 32, 0.05, C=5, height=0.4, port_height=0.1)
+    o, eit = _coupled_pair(p, gpu_ctx, frames=2)
+    Ums, refs = [], []
+    for f in range(2):
+        g = ellipse_phantom(p["nodes"], p["elements"], frame_seed(2, 10 + f))
+        U, _, _ = o.solve(g)
+        Um = add_noise(U, 60.0, frame_seed(2, 10 + f))
+        Ums.append(o.flatten_voltages(Um))
+        print("case: coupled cem c2 oracle reconstruction %d" % f, flush=True)
+        refs.append(reconstruct(o, Um, np.ones(o.E), 1e-3, steps=3))
+    opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=capi.MPF_PREC_MIXED,
+                        refine_steps=1, refine_tol=1e-2, preconditioner=capi.MPF_PRECOND_TWO_LEVEL)
+    s = Solver(eit, 1e-3, gn_refine=4, options=opts)
+    s.measurement.copy_(torch.tensor(np.stack(Ums)))
+    print("case: coupled cem c2 device reconstruction", flush=True)
+    for k in range(3):
+        s.solve_absolute(warm_start=k > 0)
+        assert s.last_stats.not_converged == 0
+    gd = s.material_distribution.cpu().numpy().astype(np.float64)
+    for f in range(2):
+        err = np.linalg.norm(gd[f] - refs[f]) / np.linalg.norm(refs[f])
+        err_d = np.linalg.norm(gd[f] - refs[f]) / np.linalg.norm(refs[f] - 1)
+        print("frame %d: gamma %.2e, update %.2e" % (f, err, err_d), flush=True)
+        assert err < 1e-3
+        assert err_d < 1e-2

@@ -10,10 +10,17 @@ from oracle.gn_host import GnHost
 from tests.problems import make_problem, oracle_of
 
 
-@pytest.mark.parametrize("z,two_level", [(0.0, True), (0.0, False), (0.01, True)])
-def test_host_reconstruction_matches_oracle(z, two_level):
+@pytest.mark.parametrize("z,two_level,cem", [(0.0, True, "harmonic"), (0.0, False, "harmonic"),
+                                            (0.01, True, "harmonic"), (0.01, True, "coupled"),
+                                            (0.01, False, "coupled")])
+def test_host_reconstruction_matches_oracle(z, two_level, cem):
+    from oracle.eit_oracle import CoupledCEMOracle
     p = make_problem(10, 16, 0.1, C=3, height=0.4, port_height=0.1)
-    o = oracle_of(p, quirks=False, contact_impedance=z)
+    if cem == "coupled":
+        o = CoupledCEMOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
+                             p["current"], 1.0, p["C"], p["height"], p["port_height"], contact_impedance=z)
+    else:
+        o = oracle_of(p, quirks=False, contact_impedance=z)
     frames = []
     for f in range(2):
         g = ellipse_phantom(p["nodes"], p["elements"], frame_seed(2, f))
@@ -21,7 +28,7 @@ def test_host_reconstruction_matches_oracle(z, two_level):
         frames.append(add_noise(U, 60.0, frame_seed(2, f)))
     h = GnHost(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"], 1.0,
                p["C"], p["height"], p["port_height"], contact_impedance=z, two_level=two_level, tol=1e-6,
-               refine=1, refine_tol=1e-2)
+               refine=1, refine_tol=1e-2, cem=cem)
     Um = np.stack([o.flatten_voltages(U) for U in frames])
     G, secs = h.reconstruct(Um, steps=3, threads=2)
     for f, U in enumerate(frames):

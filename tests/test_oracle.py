@@ -479,6 +479,61 @@ def test_jacobian_reference_formula_is_frozen_mass_derivative():
         np.testing.assert_allclose(Jr[:, e], col, atol=1e-6 * np.max(np.abs(col)))
 
 
+def _coupled(C, h, hp, z=0.01, rings=6, L=8, w=0.3):
+    p = make_problem(rings, L, w, C=C, height=h, port_height=hp)
+    o = eo.CoupledCEMOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
+                            p["current"], 1.0, C, h, hp, contact_impedance=z)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 4))
+    return p, o, g
+
+
+def test_coupled_cem_one_harmonic_is_harmonic_wise():
+    """One harmonic with the electrode over the full height: the coupled model is the
+    harmonic-wise complete electrode model with current I / h (beta_0 = 1 / h)."""
+    p, o, g = _coupled(1, 1.0, 1.0)
+    U, _, _ = o.solve(g)
+    oh = eo.EITOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"],
+                      1.0, 1, 1.0, 1.0, quirks=False, contact_impedance=0.01)
+    Uh, _, _ = oh.solve(g)
+    np.testing.assert_allclose(U, Uh, atol=1e-12 * np.max(np.abs(Uh)))
+
+
+@pytest.mark.parametrize("C,h,hp", [(3, 0.4, 0.1), (5, 0.4, 0.1), (3, 2.0, 0.5)])
+def test_coupled_cem_energy_reciprocity_definiteness(C, h, hp):
+    """The block system is symmetric positive definite for C >= 2; U = U^T; and for
+    every drive column the input power I^T V equals the energy
+    sum_c n_c [u_c^T A_c u_c + (1/z') int_e (u_c - h beta_c V)^2], evaluated from the
+    fields with the edge integrals of the contact term."""
+    p, o, g = _coupled(C, h, hp)
+    H = o.coupled_system(g).toarray()
+    assert np.linalg.eigvalsh(H)[0] > 0.0
+    U, phis, _ = o.solve(g)
+    assert np.max(np.abs(U - U.T)) < 1e-12 * np.max(np.abs(U))
+    sig = eo.mat_param(g, 1.0)
+    for d in range(o.D):
+        V = o.V[:, d]
+        power = o.Pi[:, d] @ V
+        energy = 0.0
+        for c, X in enumerate(phis):
+            alpha, beta = o.alpha_beta(c)
+            A = eo.assemble(o.elements, o.Ke, o.N, sig) + alpha * eo.assemble(o.elements, o.Me, o.N, sig)
+            u = X[:, d]
+            hb = o.h * beta
+            # int_e (u - hb V_l)^2 over the electrode edges: u^T Zb u - 2 hb u^T Bc V + hb^2 |e| V^2
+            contact = u @ (o.Zb @ u) - 2.0 * hb * (u @ (o.Bc @ V)) + hb * hb * (o.elen @ (V * V))
+            energy += o.n_c[c] * (u @ (A @ u) + contact / o.zp)
+        assert abs(power - energy) < 1e-10 * abs(power)
+
+
+def test_coupled_cem_jacobian_matches_finite_differences():
+    p, o, g = _coupled(3, 0.4, 0.1)
+    U, phis, _ = o.solve(g)
+    J = o.jacobian(g, phis)
+    elems = [3, 40, 158]
+    Jfd = _fd_jacobian(o, g, elems)
+    np.testing.assert_allclose(J[:, elems], Jfd, atol=1e-6 * np.max(np.abs(Jfd)))
+
+
 def test_gn_data_space_identity():
     rng = np.random.default_rng(5)
     J = rng.standard_normal((40, 120)) * 1e-3
