@@ -11,7 +11,8 @@ Schema (JSON integers are accepted wherever the reference reads doubles)::
         "mesh": "meshdir" | {"path": "meshdir"}                  nodes/elements(/edges) text files
               | {"radius": r, "outerEdgeLength": h, "innerEdgeLength": h_i},
         "ports": "ports.txt" | {"edges": "ports.txt"}            rows of edge indices, -1 unused
-               | {"count": P, "width": w, "offset": o, "clockwise": false, "height": hp},
+               | {"count": P, "width": w, "offset": o, "clockwise": false, "height": hp,
+                  "invertDirection": false},       the mesher's corner points read this key
         "source": {"type": "current", "value": I | [I_1..I_D],
                    "drivePattern": [[..]], "measurementPattern": [[..]]},   default eye(P)
         "material": ref | {"referenceValue": ref, "distribution": "gamma.txt"} | "gamma.txt",
@@ -20,10 +21,11 @@ Schema (JSON integers are accepted wherever the reference reads doubles)::
                 "regularizationType": "identity" | "noser" | "laplacian"}}
 
 Differences from the reference, each raising or documented:
-  * the distmesh branch builds this package's ring mesh (``mesh.ring_mesh``) of
-    ``round(radius / outerEdgeLength)`` rings with nodes on the electrode corners; the
-    inner edge length (distmesh's grading) is not reproduced.  The mesh is written to
-    ``<path>/mesh`` as the reference does.
+  * the distmesh branch runs this package's force-equilibrium mesher
+    (``mesh.distmesh_disc``, Persson & Strang's algorithm with the reference's distance
+    and edge-length functions and the electrode corners as fixed points); libdistmesh
+    itself is unavailable, so node positions are not the library's.  The mesh is written
+    to ``<path>/mesh`` as the reference does.
   * ``"type": "voltage"`` gives Fixed sources (``value``: the applied voltages), any other
     type current sources (sources.cpp:123-125).
   * a ``noSensitivity`` file name (solver.cpp:107-113) is read as an (m, E) text matrix
@@ -142,13 +144,19 @@ def parse_config(config, path: str = "."):
     elif isinstance(mcfg, dict) and all(k in mcfg for k in ("radius", "outerEdgeLength", "innerEdgeLength")):
         mesh_path = "mesh"
         radius = float(mcfg["radius"])
-        rings = max(2, int(round(radius / float(mcfg["outerEdgeLength"]))))
         count = int(pcfg.get("count", 0)) if isinstance(pcfg, dict) else 0
         width = float(pcfg.get("width", 0.0)) if isinstance(pcfg, dict) else 0.0
         offset = float(pcfg.get("offset", 0.0)) if isinstance(pcfg, dict) else 0.0
-        # port width and offset are arc lengths on the circle (ports.cpp:32-77)
-        nodes, elements = meshmod.ring_mesh(rings, count or None, width / radius if count else None,
-                                            offset / radius, radius=radius)
+        invert = bool(pcfg.get("invertDirection", False)) if isinstance(pcfg, dict) else False
+        # the electrode corners are fixed points of the mesh (irregular_mesh.cpp:92-103):
+        # circularPoints at the port offsets and offsets + width (arc lengths)
+        fixed = None
+        if count and width:
+            dist = 2.0 * math.pi * radius / count
+            fixed = np.concatenate([meshmod.circular_points(radius, dist, offset, invert),
+                                    meshmod.circular_points(radius, dist, offset + width, invert)])
+        nodes, elements = meshmod.distmesh_disc(radius, float(mcfg["outerEdgeLength"]), float(mcfg["innerEdgeLength"]),
+                                                fixed)
         edges, elem_edges = meshmod.unique_edges(elements)
         d = os.path.join(path, "mesh")
         os.makedirs(d, exist_ok=True)

@@ -1,9 +1,13 @@
-"""Mesh inputs for the EIT path: concentric-ring disc meshes, P1->P2 promotion,
-boundary edges and the text formats of ``IrregularMesh``.
+"""Mesh inputs for the EIT path: concentric-ring disc meshes, force-equilibrium disc
+meshes with fixed electrode corners, P1->P2 promotion, boundary edges and the text
+formats of ``IrregularMesh``.
 
-The reference meshes with libdistmesh (``src/numeric/irregular_mesh.cpp:87-122``),
-which is not available; every mesh used here comes from the deterministic ring
-generator below so that the GPU path, the CPU baseline and the FP64 oracle all see
+The reference meshes with libdistmesh (``src/numeric/irregular_mesh.cpp:87-122``), which
+is not available (an unpinned third-party library).  ``distmesh_disc`` restates the
+published algorithm it implements (Persson & Strang, "A Simple Mesh Generator in
+MATLAB", SIAM Review 46(2), 2004) for the reference's call: a disc, graded edge length,
+electrode corners as fixed points.  The benchmark configurations use the deterministic
+ring generator below so that the GPU path, the CPU baseline and the FP64 oracle all see
 byte-identical ``nodes``/``elements`` arrays.
 
 Ring generator: ring ``j`` (1..n_r) of radius ``j/n_r`` holds ``6j`` equally spaced
@@ -21,7 +25,8 @@ import numpy as np
 
 __all__ = [
     "ring_mesh", "promote_p2", "boundary_edges", "unique_edges",
-    "port_corner_angles", "save_mesh", "load_mesh", "rings_for_elements",
+    "port_corner_angles", "save_mesh", "load_mesh", "rings_for_elements", "distmesh_disc",
+    "circular_points",
 ]
 
 
@@ -115,6 +120,103 @@ def ring_mesh(n_rings: int, port_count: int | None = None, port_width: float | N
                 tris.append((sa + i % na, sb + k % nb, sa + (i + 1) % na))
                 i += 1
     elements = np.asarray(tris, dtype=np.int32)
+    _check_ccw(nodes, elements)
+    return nodes, elements
+
+
+def circular_points(radius: float, distance: float, offset: float, invert_direction: bool):
+    """``math::circularPoints`` (src/mathematics.cpp:79-99): points on the circle of
+    ``radius`` spaced by the arc length ``distance``, starting at arc length ``offset``
+    from angle 0, counter-clockwise (clockwise: y mirrored)."""
+    n = int(math.ceil(2.0 * math.pi / (distance / radius)))
+    phi = offset / radius + np.arange(n) * (distance / radius)
+    sgn = -1.0 if invert_direction else 1.0
+    return np.stack([radius * np.cos(phi), sgn * radius * np.sin(phi)], axis=1)
+
+
+def distmesh_disc(radius: float, outer_edge_length: float, inner_edge_length: float,
+                  fixed_points=None, seed: int = 0, max_iter: int = 1000):
+    """Disc mesh by force equilibrium (Persson & Strang 2004), as the reference's
+    ``IrregularMesh::fromConfig`` asks libdistmesh for it (irregular_mesh.cpp:87-122):
+
+      * signed distance d(p) = |p| - radius;
+      * relative edge length h(p) = 1 + (1 - inner / outer) d(p) / radius (outer edge
+        length on the boundary, inner at the centre), initial spacing outer_edge_length;
+      * bounding box [-1.1 radius, 1.1 radius]^2, ``fixed_points`` kept in place (the
+        electrode corners, ``circular_points`` of the port configuration).
+
+    Steps: an equilateral grid in the box, points outside removed, the rest kept with
+    probability (min h / h(p))^2 (seeded generator); then repeated Delaunay
+    triangulation, bar forces max(L0 - L, 0) with L0 = h(mid) * 1.2 * sqrt(sum L^2 /
+    sum h(mid)^2), explicit steps of 0.2, points leaving the disc projected back onto the
+    circle, until interior points move less than 1e-3 of the spacing.  Returns
+    ``(nodes[V, 2], elements[E, 3] int32)``, triangles counter-clockwise, the fixed points
+    first."""
+    from scipy.spatial import Delaunay
+    h0 = float(outer_edge_length)
+    ratio = float(inner_edge_length) / h0
+    fd = lambda p: np.hypot(p[:, 0], p[:, 1]) - radius
+    fh = lambda p: 1.0 + (1.0 - ratio) * fd(p) / radius
+    geps = 1e-3 * h0
+    pfix = np.zeros((0, 2)) if fixed_points is None else np.asarray(fixed_points, np.float64).reshape(-1, 2)
+    # 1. equilateral grid over the bounding box
+    b = 1.1 * radius
+    xs = np.arange(-b, b + 1e-12, h0)
+    ys = np.arange(-b, b + 1e-12, h0 * math.sqrt(3.0) / 2.0)
+    X, Y = np.meshgrid(xs, ys)
+    X[1::2, :] += h0 / 2.0
+    p = np.stack([X.ravel(), Y.ravel()], axis=1)
+    # 2. inside points, thinned to the size function
+    p = p[fd(p) < -geps]
+    r0 = 1.0 / fh(p) ** 2
+    rng = np.random.default_rng(seed)
+    p = p[rng.random(p.shape[0]) < r0 / r0.max()]
+    # drop grid points on top of fixed points
+    if pfix.shape[0]:
+        dmin = np.min(np.hypot(p[:, None, 0] - pfix[None, :, 0], p[:, None, 1] - pfix[None, :, 1]), axis=1)
+        p = p[dmin > 0.5 * h0 * ratio]
+    p = np.concatenate([pfix, p], axis=0)
+    nfix = pfix.shape[0]
+    pold = np.full_like(p, np.inf)
+    bars = tri = None
+    for _ in range(max_iter):
+        if np.max(np.hypot(*(p - pold).T)) / h0 > 0.1:
+            pold = p.copy()
+            tri = Delaunay(p).simplices
+            tri = tri[fd(p[tri].mean(axis=1)) < -geps]
+            e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], axis=0)
+            bars = np.unique(np.sort(e, axis=1), axis=0)
+        vec = p[bars[:, 0]] - p[bars[:, 1]]
+        L = np.hypot(vec[:, 0], vec[:, 1])
+        hb = fh(0.5 * (p[bars[:, 0]] + p[bars[:, 1]]))
+        L0 = hb * 1.2 * math.sqrt(np.sum(L ** 2) / np.sum(hb ** 2))
+        F = np.maximum(L0 - L, 0.0)
+        Fv = (F / L)[:, None] * vec
+        Ft = np.zeros_like(p)
+        np.add.at(Ft, bars[:, 0], Fv)
+        np.add.at(Ft, bars[:, 1], -Fv)
+        Ft[:nfix] = 0.0
+        p = p + 0.2 * Ft
+        # back onto the circle (the exact projection for d = |p| - radius)
+        r = np.hypot(p[:, 0], p[:, 1])
+        out = r > radius
+        p[out] *= (radius / r[out])[:, None]
+        inside = fd(p) < -geps
+        if np.max(np.hypot(*(0.2 * Ft[inside]).T), initial=0.0) / h0 < 1e-3:
+            break
+    tri = Delaunay(p).simplices
+    tri = tri[fd(p[tri].mean(axis=1)) < -geps]
+    # counter-clockwise, unused points removed (fixed points stay first)
+    a, bb, c = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    cross = (bb[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (bb[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    tri[cross < 0] = tri[cross < 0][:, [0, 2, 1]]
+    used = np.zeros(p.shape[0], bool)
+    used[tri.ravel()] = True
+    used[:nfix] = True
+    new = -np.ones(p.shape[0], np.int64)
+    new[used] = np.arange(int(used.sum()))
+    nodes = p[used]
+    elements = new[tri].astype(np.int32)
     _check_ccw(nodes, elements)
     return nodes, elements
 

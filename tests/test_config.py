@@ -97,7 +97,13 @@ def test_parse_config_patterns_values_material_file(tmp_path):
 def test_parse_config_generated_mesh_and_rejections(tmp_path):
     c = _config(mesh={"radius": 1.0, "outerEdgeLength": 0.1, "innerEdgeLength": 0.1})
     spec, _ = cfg.parse_config(c, str(tmp_path))
-    assert spec.nodes.shape[0] == 1 + 3 * 10 * 11        # 10 rings of 6j nodes
+    # the mesher keeps the 16 electrodes' corners (arc lengths 0 and 0.2 past 2 pi l / 16)
+    # as its first nodes, and every port tiles its arc
+    t = 2 * np.pi * np.arange(16) / 16
+    corners = np.concatenate([np.stack([np.cos(t), np.sin(t)], 1), np.stack([np.cos(t + 0.2), np.sin(t + 0.2)], 1)])
+    np.testing.assert_allclose(spec.nodes[:32], corners, atol=1e-15)
+    assert 300 < spec.nodes.shape[0] < 700
+    assert (spec.ports >= 0).sum(axis=1).min() >= 1
     assert os.path.exists(tmp_path / "mesh" / "elementEdges.txt")
     assert cfg.parse_config(_config(source={"type": "voltage", "value": 1.0}), str(tmp_path))[0].source == "voltage"
     assert cfg.parse_config(_config(source={"type": "open", "value": 1.0}), str(tmp_path))[0].source == "current"
