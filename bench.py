@@ -211,19 +211,22 @@ def run_c4(args):
 
 
 def run_c5(args):
-    """Config 5: learned start on the config-3 mesh (--mesh c3, default: 40344 P2
-    elements, 32 electrodes, 5 harmonics) or the config-2 mesh (9600 P1 elements).  Trains
-    the 1024 -> 2048 -> 2048 -> 2048 -> E network on device-generated (voltage, ln gamma)
-    pairs (1-4 ellipses, 60 dB), then times on held-out frames:
-      plain    FP32 Gauss-Newton from gamma = 1, 3 steps, one frame at a time: the
-               reference's algorithm (Jacobi-PCG, reference stop rule), and the same with
-               this build's two-level PCG
-      learned  network start + 1 mixed-precision Gauss-Newton step (two-level PCG), B
-               frames per launch, with the residual check and the plain fallback included
-               in the time
-    and reports both frame rates, the reconstruction errors against the true phantoms and
-    the learned path's gamma against the FP64 oracle's Gauss-Newton step from the same
-    start.  Not the headline metric: its own JSON line."""
+    """Config 5: learned start on the config-2 mesh (--mesh c2, default: 9600 P1 elements,
+    32 electrodes, 5 harmonics) or the config-3 mesh.  Trains the 1024 -> 2048 -> 2048 ->
+    2048 -> E network on --train device-generated (voltage, ln gamma) pairs (1-4
+    ellipses, 60 dB; default 200k on the config-2 mesh), then times on --holdout held-out
+    frames, B = --batch frames per launch:
+      learned  network start + 1 mixed-precision Gauss-Newton step (two-level PCG), with
+               the acceptance check (the final voltage residual within tau = 2 times the
+               noise norm) and the plain fallback of rejected frames in the time
+      plain    FP32 Gauss-Newton from gamma = 1, 3 steps, the same batched two-level PCG
+               (FP32 to 1e-6, no refinement): this build's plain reconstruction
+      plain_1  the reference's algorithm one frame at a time (FP32 Jacobi-PCG, reference
+               stop rule), on a few frames
+    and reports the frame rates, the errors against the true phantoms, the learned
+    gamma against the FP64 oracle's Gauss-Newton step from the same start, and (untimed)
+    how often the acceptance check agrees with the rule "final residual no worse than
+    plain GN's".  Not the headline metric: its own JSON line."""
     import numpy as np
     import torch
     from mpflow_amd import capi
@@ -239,96 +242,118 @@ def run_c5(args):
                             jacobian_exact=True, frames=frames, ctx=ctx)
     phantom = lambda sd: ellipse_phantom(nodes, el, sd, n_max=4)
     log = lambda msg: print("c5: " + msg, file=sys.stderr, flush=True)
+    two = capi.MPF_PRECOND_TWO_LEVEL
     t0 = time.perf_counter()
     gen = mk(64 if not w["p2"] else 32)
-    two = capi.MPF_PRECOND_TWO_LEVEL
-    ntrain = args.train if args.train else (4096 if w["p2"] else 16384)
+    ntrain = args.train if args.train else (200000 if not w["p2"] else 4096)
     U, G = generate(gen, ntrain, frame_seed(5, 0), phantom, log=log,
                     options=SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-5, preconditioner=two))
     t_data = time.perf_counter() - t0
+    log("generated %d pairs in %.1f s" % (ntrain, t_data))
     feats = VoltageFeatures().fit(U)
+    X = feats(U)
+    del U
     net = Mlp([gen.m, 2048, 2048, 2048, gen.E], seed=5, ctx=ctx)
     t0 = time.perf_counter()
-    hist = train(net, feats(U), G, epochs=args.epochs, batch=256, lr=1e-3, lr_decay=0.85, log=log)
+    hist = train(net, X, G, epochs=args.epochs, batch=256, lr=1e-3, lr_decay=0.85, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    nh = args.steps * B
+    del X, G
+    nh = (args.holdout + B - 1) // B * B
     Uh, Gh = generate(gen, nh, frame_seed(5, 1000000), phantom)
-    log("timing plain and learned reconstructions")
     gtrue = np.exp(Gh)
     del gen
-    # the learned path's forward solves: FP32 two-level PCG to 1e-4 plus one FP64 residual
-    # refinement to 1e-2 of that (about 1e-6 overall)
-    mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-4, precision=capi.MPF_PREC_MIXED,
-                         refine_steps=1, refine_tol=1e-2, preconditioner=two)
-    # plain FP32 Gauss-Newton as the reference runs it: 3 steps from gamma = 1, one frame
-    # at a time, FP32 PCG with the reference stop rule (column 0, sqrt(rho / b.b) below
-    # FLT_EPSILON); also timed with the ||r|| <= tol ||b|| rule for comparison
-    e1 = mk(1)
-    n_plain = min(nh, 6)
-
-    def plain_fps(opts):
-        sp = Solver(e1, args.lam, gn_refine=4, options=opts)
-        out = []
-
-        def one(i):
-            sp.material_distribution.fill_(1.0)
-            sp.measurement[0] = torch.tensor(Uh[i], dtype=torch.float64)
-            for _ in range(3):
-                sp.solve_absolute()
-            return sp.material_distribution[0].cpu().numpy().copy()
-
-        one(0)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(n_plain):
-            out.append(one(i))
-        torch.cuda.synchronize()
-        return n_plain / (time.perf_counter() - t0), out
-
-    fps_plain, gplain = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_REFERENCE))
-    # plain FP32 GN with this build's solver: two-level PCG to ||r|| <= tol ||b||
-    fps_plain_tl, _ = plain_fps(SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, preconditioner=two))
-    # learned start, B frames per launch
-    eB = mk(B)
-    rec = LearnedReconstruction(eB, net, feats, args.lam, mixed, gn_steps=1, plain_steps=3, fallback_eit=e1)
     noise = np.sqrt(np.mean(Uh ** 2, axis=1)) * 10.0 ** (-60.0 / 20.0) * math.sqrt(Uh.shape[1])
-    rec.reconstruct(Uh[:B], noise_norm=float(noise[:B].max()))
+    eB = mk(B)
+    lam = args.lam
+    # plain FP32 Gauss-Newton, batched: 3 steps from gamma = 1, two-level FP32 PCG to 1e-6
+    plain_opts = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, preconditioner=two)
+    sp = Solver(eB, lam, gn_refine=4, options=plain_opts)
+
+    def plain_batch(k):
+        sp.material_distribution.fill_(1.0)
+        sp.measurement.copy_(torch.tensor(Uh[k * B:(k + 1) * B], dtype=torch.float64))
+        for _ in range(3):
+            sp.solve_absolute()
+        return sp.material_distribution.clone()
+
+    log("timing plain batched GN on %d frames" % nh)
+    plain_batch(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    glearn, acc = [], 0
-    for k in range(args.steps):
+    gplain = []
+    for k in range(nh // B):
+        gplain.append(plain_batch(k))
+    torch.cuda.synchronize()
+    fps_plain = nh / (time.perf_counter() - t0)
+    # untimed: the plain results' final voltage residuals (one more forward solve each)
+    r_plain = []
+    for k in range(nh // B):
+        eB.solve(gplain[k], plain_opts)
+        r_plain.append(np.linalg.norm(Uh[k * B:(k + 1) * B] - eB.result64.cpu().numpy(), axis=1))
+    r_plain = np.concatenate(r_plain)
+    gplain = torch.cat(gplain).cpu().numpy()
+    # learned start + 1 mixed-precision GN step, B frames per launch
+    mixed = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-4, precision=capi.MPF_PREC_MIXED,
+                         refine_steps=1, refine_tol=1e-2, preconditioner=two)
+    e1 = mk(1)
+    rec = LearnedReconstruction(eB, net, feats, lam, mixed, gn_steps=1, plain_steps=3, fallback_eit=e1)
+    rec.reconstruct(Uh[:B], noise_norm=float(noise[:B].max()))
+    log("timing learned reconstruction on %d frames" % nh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    glearn, acc, r_learn = [], [], []
+    for k in range(nh // B):
         sl = slice(k * B, (k + 1) * B)
         g, info = rec.reconstruct(Uh[sl], noise_norm=float(noise[sl].max()))
-        glearn.append(g.cpu().numpy())
-        acc += int(info["accepted"].sum())
+        glearn.append(g)
+        acc.append(info["accepted"])
+        r_learn.append(info["residual_learned"])
     torch.cuda.synchronize()
     fps_learn = nh / (time.perf_counter() - t0)
-    glearn = np.concatenate(glearn)
+    glearn = torch.cat(glearn).cpu().numpy()
+    acc, r_learn = np.concatenate(acc), np.concatenate(r_learn)
     err_l = np.linalg.norm(glearn - gtrue, axis=1) / np.linalg.norm(gtrue, axis=1)
-    err_p = np.linalg.norm(np.stack(gplain) - gtrue[:n_plain], axis=1) / np.linalg.norm(gtrue[:n_plain], axis=1)
+    err_p = np.linalg.norm(gplain - gtrue, axis=1) / np.linalg.norm(gtrue, axis=1)
+    # the reference's algorithm, one frame at a time, on a few frames
+    n_ref = min(nh, 4)
+    sr = Solver(e1, lam, gn_refine=4, options=SolveOptions(stop_rule=capi.MPF_STOP_REFERENCE))
+
+    def ref_one(i):
+        sr.material_distribution.fill_(1.0)
+        sr.measurement[0] = torch.tensor(Uh[i], dtype=torch.float64)
+        for _ in range(3):
+            sr.solve_absolute()
+
+    ref_one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_ref):
+        ref_one(i)
+    torch.cuda.synchronize()
+    fps_ref = n_ref / (time.perf_counter() - t0)
     # FP64 oracle: one Gauss-Newton step from the same network start, first frames
     o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
-    X = torch.tensor(feats(Uh[:2]), dtype=torch.float32, device="cuda")
-    start = torch.exp(net.forward(X)).cpu().numpy().astype(np.float64)
-    starts32 = torch.exp(net.forward(X)).to(torch.float32)
+    Xh = torch.tensor(feats(Uh[:2]), dtype=torch.float32, device="cuda")
+    start = torch.exp(net.forward(Xh)).cpu().numpy().astype(np.float64)
     e2 = mk(2)
-    s2 = Solver(e2, args.lam, gn_refine=4, options=mixed)
-    s2.material_distribution.copy_(starts32)
+    s2 = Solver(e2, lam, gn_refine=4, options=mixed)
+    s2.material_distribution.copy_(torch.tensor(start, dtype=torch.float32))
     s2.measurement.copy_(torch.tensor(Uh[:2], dtype=torch.float64))
     s2.solve_absolute()
     gdev = s2.material_distribution.cpu().numpy().astype(np.float64)
     err64 = []
     for f in range(2):
         Um = Uh[f].reshape(e2.Dp, e2.Mp)[:e2.D, :e2.M].T
-        g64 = reconstruct(o, Um, start[f], args.lam, steps=1, quirks_jacobian=False)
+        g64 = reconstruct(o, Um, start[f], lam, steps=1, quirks_jacobian=False)
         err64.append(float(np.linalg.norm(gdev[f] - g64) / np.linalg.norm(g64)))
+    worse = acc & (r_learn > r_plain)
     out = {
         "metric": "frames/s, config 5: learned-start mixed-precision batched GN vs plain FP32 GN",
         "value": round(fps_learn, 3),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": nh // B,
         "warmup": 1,
         "ms_per_step": round(1e3 * B / fps_learn, 2),
         "higher_is_better": True,
@@ -338,26 +363,32 @@ def run_c5(args):
         "data": "synthetic",
         "config": {
             "workload": "config 5 (c5) on the %s mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
-                        "network %s; %d training pairs, %d epochs; held-out frames 1-4 ellipses, 60 dB"
-                        % (args.mesh, e1.E, e1.N, net.dims, ntrain, args.epochs),
+                        "network %s; %d training pairs, %d epochs; %d held-out frames, 1-4 ellipses, 60 dB"
+                        % (args.mesh, e1.E, e1.N, net.dims, ntrain, args.epochs, nh),
             "frames_per_launch": B,
-            "plain_fp32_gn_fps": round(fps_plain, 3),
-            "plain_fp32_gn": "3 GN steps from gamma = 1, one frame at a time, FP32 Jacobi-PCG with the "
-                             "reference stop rule (the reference's algorithm)",
-            "plain_fp32_gn_two_level_fps": round(fps_plain_tl, 3),
-            "speedup_vs_plain": round(fps_learn / fps_plain, 2),
-            "speedup_vs_plain_two_level": round(fps_learn / fps_plain_tl, 2),
+            "plain_fp32_gn_batched_fps": round(fps_plain, 3),
+            "plain_fp32_gn_batched": "3 GN steps from gamma = 1, B frames per launch, FP32 two-level PCG to "
+                                     "1e-6 (no refinement), BF16-MFMA Gram + FP64 Cholesky",
+            "speedup_vs_plain_batched": round(fps_learn / fps_plain, 2),
+            "plain_fp32_gn_reference_fps": round(fps_ref, 3),
+            "plain_fp32_gn_reference": "the reference's algorithm: 3 GN steps, one frame at a time, FP32 "
+                                       "Jacobi-PCG with the reference stop rule (%d frames)" % n_ref,
+            "speedup_vs_reference_algorithm": round(fps_learn / fps_ref, 2),
             "learned_forward": "FP32 two-level PCG to 1e-4 + 1 FP64 refinement to 1e-2; residual check "
                                "warm-started to 1e-4",
+            "acceptance": "network start kept when the final voltage residual <= 2 x the noise norm "
+                          "(else plain 3-step GN, one frame at a time, inside the timed region)",
+            "accepted": "%d/%d" % (int(acc.sum()), nh),
+            "accepted_but_worse_than_plain": int(worse.sum()),
+            "rule_no_worse_than_plain_holds": "%d/%d" % (int((r_learn <= r_plain).sum()), nh),
             "learned_error_vs_phantom": round(float(err_l.mean()), 4),
             "plain_error_vs_phantom": round(float(err_p.mean()), 4),
             "learned_vs_fp64_oracle_same_start": [float("%.3g" % v) for v in err64],
-            "accepted": "%d/%d" % (acc, nh),
             "train_loss": [round(hist[0], 5), round(hist[-1], 5)],
             "data_s": round(t_data, 1), "train_s": round(t_train, 1),
         },
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 def run_probe():
@@ -390,9 +421,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2, help="timed batches of --batch frames")
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches")
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
-    ap.add_argument("--train", type=int, default=0, help="config 5: training pairs (0: 4096 on c3, 16384 on c2)")
+    ap.add_argument("--train", type=int, default=0, help="config 5: training pairs (0: 200000 on c2, 4096 on c3)")
+    ap.add_argument("--holdout", type=int, default=4096, help="config 5: held-out frames")
     ap.add_argument("--epochs", type=int, default=12, help="config 5: training epochs")
-    ap.add_argument("--mesh", default="c3", choices=["c2", "c3"], help="config 5: mesh of the learned start")
+    ap.add_argument("--mesh", default="c2", choices=["c2", "c3"], help="config 5: mesh of the learned start")
     ap.add_argument("--jobs", type=int, default=256, help="config 4: number of jobs")
     ap.add_argument("--gn-steps", type=int, default=3)
     ap.add_argument("--lam", type=float, default=1e-3)

@@ -422,7 +422,14 @@ int mpf_gram_bf16(mpf_ctx* ctx, int32_t trans, int32_t rows, int32_t k, const fl
         MPF_REQUIRE(rows > 0 && k > 0 && lda >= rows, "mpf_gram_bf16: bad shape");
         set_device(ctx);
         const int n = trans ? k : rows, nk = trans ? rows : k;
-        if (trans == 0 && mpf::gram_nt_applies(n, lda)) {
+        if (trans == 0 && mpf::gram_bf16_applies(n, lda)) {
+            // one rounding pass to BF16, then the 256-tile kernel
+            mpf::DBuf<uint16_t> ab((size_t)lda * nk);
+            mpf::launch_to_bf16((size_t)lda * nk, d_A, ab.p, ctx->stream);
+            mpf::DBuf<char> work(mpf::gram_bf16_workspace_bytes(n, nk, 1));
+            mpf::launch_gram_bf16(ab.p, 0, n, nk, lda, d_G, 0, 1, work.p, ctx->stream);
+            MPF_HIP(hipStreamSynchronize(ctx->stream));
+        } else if (trans == 0 && mpf::gram_nt_applies(n, lda)) {
             mpf::DBuf<char> work(mpf::gram_nt_workspace_bytes(n, nk, 1));
             mpf::launch_gram_nt(d_A, 0, n, nk, lda, d_G, 0, 1, work.p, ctx->stream);
             MPF_HIP(hipStreamSynchronize(ctx->stream));

@@ -1,6 +1,7 @@
 // Launch interface of the dense kernels (dense.hip).
 #pragma once
 
+#include "cplx.h"
 #include "jobs.h"
 
 namespace mpf {
@@ -26,8 +27,9 @@ struct JacDev {
 
 void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use64, float* U32,
                      double* U64, hipStream_t st);
+// Jb (nullable): a BF16 copy of J for launch_gram_bf16
 void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use64, float* J,
-                     hipStream_t st);
+                     hipStream_t st, uint16_t* Jb = nullptr);
 
 // G_b = A_b A_b^T for `batch` column-major A_b (n x nk, lda; n, lda multiples of 4) on
 // MFMA with 128 x 128 tiles; G_b full (both triangles), column-major n x n
@@ -36,6 +38,14 @@ bool gram_nt_applies(int n, int lda);
 size_t gram_nt_workspace_bytes(int n, int nk, int batch, int max_slabs = 0);
 void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG, int batch,
                     void* work, hipStream_t st, int max_slabs = 0);
+// the same Gram from a BF16 copy of A (raw bfloat16 bits) with 256 x 256 tiles; n and
+// lda multiples of 8
+bool gram_bf16_applies(int n, int lda);
+size_t gram_bf16_workspace_bytes(int n, int nk, int batch);
+void launch_gram_bf16(const uint16_t* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG,
+                      int batch, void* work, hipStream_t st);
+// b[i] = bf16(a[i]) (round to nearest even)
+void launch_to_bf16(size_t n, const float* a, uint16_t* b, hipStream_t st);
 int gram_kslab(int n, int nk);
 size_t gram_workspace_bytes(int n, int nk);
 void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, void* work,
@@ -55,6 +65,11 @@ void launch_cholesky(int n, const float* G, size_t strideG, double shift, double
                      int batch, int* d_info, int2* d_pairs, hipStream_t st);
 void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_t strideb, double* y,
                   size_t stridey, int batch, hipStream_t st);
+// complex symmetric (not Hermitian) A = L L^T in place and its solves, FP64; A holds
+// both triangles of the already shifted matrices; d_pairs as for launch_cholesky
+void launch_cholesky_z(int n, zd* A, size_t strideA, int batch, int* d_info, int2* d_pairs, hipStream_t st);
+void launch_potrs_z(int n, const zd* A, size_t strideA, const zd* b, size_t strideb, zd* y, size_t stridey,
+                    int batch, hipStream_t st);
 
 void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t st);
 void launch_add_to(int n, double* y, const double* dy, hipStream_t st);

@@ -46,7 +46,8 @@ __global__ void __launch_bounds__(256) k_voltages(const JobDev* __restrict__ job
 constexpr int kJacMaxRows = 4096;   // Dp * Mp <= 64 * 64
 
 __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ jobs, JacDev jd,
-                                                  int use64, float* __restrict__ Jout) {
+                                                  int use64, float* __restrict__ Jout,
+                                                  __bf16* __restrict__ Jbf) {
     const int e = blockIdx.x;
     const int j = blockIdx.y;
     const JobDev& Jb = jobs[j];
@@ -105,10 +106,14 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
                                      : jd.ref;
     const float scale = (float)(-sd);
     float* col = Jout + ((size_t)j * jd.E + e) * mrows;
+    __bf16* colb = Jbf ? Jbf + ((size_t)j * jd.E + e) * mrows : nullptr;
     for (int q = 0; q < per; ++q) {
         const int idx = threadIdx.x + 256 * q;
         if (idx >= mrows) break;
-        col[idx] = scale * acc[q];
+        const float v = scale * acc[q];
+        col[idx] = v;
+        // the Gram's BF16 operand, rounded once here instead of at every panel read
+        if (colb) colb[idx] = (__bf16)v;
     }
 }
 
@@ -288,6 +293,123 @@ __global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, si
             }
 }
 
+// G_b = A_b A_b^T from a BF16 copy of A (column-major n x nk, lda; the Jacobian's layout
+// [element][row]), 256 x 256 tiles of the upper triangle per block of 8 wavefronts (each
+// a 128 x 64 sub-tile: 4 x 2 v_mfma_f32_32x32x16_bf16 accumulators), contraction in
+// stages of 32 staged through two LDS buffers: every tile reads its two 256-row panels
+// once per slab, a quarter of the 64-row tiles' panel traffic and half of the FP32 source's
+// bytes.  Partial tiles go to part[batch][slab][tile]; k_gram_reduce256 sums them in
+// slab order.  Diagonal tiles stage one panel.
+constexpr int kGT3 = 256;
+constexpr int kGK3 = 32;
+constexpr int kGL3 = kGK3 + 8;   // LDS row stride (BF16): 16-byte aligned rows, staggered banks
+
+__global__ void __launch_bounds__(512) k_gram_bf16(const __bf16* __restrict__ A, size_t strideA, int nrow, int nk,
+                                                   int lda, int nt, int kslab, float* __restrict__ part) {
+    const int ntiles = nt * (nt + 1) / 2;
+    const int2 tile = upper_tile(blockIdx.x, nt);
+    const int I0 = tile.x * kGT3, J0 = tile.y * kGT3;
+    const bool diag = tile.x == tile.y;
+    const int k0 = blockIdx.z * kslab, k1 = min(nk, k0 + kslab);
+    const __bf16* Ab = A + blockIdx.y * strideA;
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][kGT3][kGL3];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][kGT3][kGL3];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = (wave >> 2) * 128, wj = (wave & 3) * 64;
+    f32x16_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    // staging: threads 0..255 stage panel I, 256..511 panel J; each thread owns 8
+    // consecutive rows x 4 consecutive k: four 128-bit loads (one per k), then per row the
+    // 4 k values as one 64-bit LDS write
+    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+    const int half = tid >> 8, t8 = tid & 255;
+    const int ro = (t8 & 31) * 8, kq = (t8 >> 5) * 4;
+    const int P0 = half ? J0 : I0;
+    const bool active = !(half && diag);
+    bf16x8_t r[4];
+    auto load = [&](int kb) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kg = kb + kq + u;
+            bf16x8_t z;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) z[q] = (__bf16)0.0f;
+            r[u] = (active && kg < k1 && P0 + ro < nrow)
+                       ? *reinterpret_cast<const bf16x8_t*>(Ab + (size_t)kg * lda + P0 + ro) : z;
+        }
+    };
+    auto store = [&](int buf) {
+        if (!active) return;
+        __bf16 (*dst)[kGL3] = half ? sb[buf] : sa[buf];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            bf16x4_t v = {r[0][q], r[1][q], r[2][q], r[3][q]};
+            *reinterpret_cast<bf16x4_t*>(&dst[ro + q][kq]) = v;
+        }
+    };
+    const int nst = (k1 - k0 + kGK3 - 1) / kGK3;
+    if (nst > 0) {
+        load(k0);
+        store(0);
+    }
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) load(k0 + (st + 1) * kGK3);
+        const __bf16 (*pb)[kGL3] = diag ? sa[buf] : sb[buf];
+#pragma unroll
+        for (int ks = 0; ks < kGK3; ks += 16) {
+            const int kk = ks + (lane >> 5) * 8;
+            bf16x8_t av[4], bv[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][wi + 32 * a + (lane & 31)][kk]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const bf16x8_t*>(&pb[wj + 32 * b + (lane & 31)][kk]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        if (st + 1 < nst) store(buf ^ 1);
+        __syncthreads();
+    }
+    // C layout: lane -> column (lane & 31); register v -> row 8*(v/4) + 4*(lane/32) + v%4
+    float* out = part + (((size_t)blockIdx.y * gridDim.z + blockIdx.z) * ntiles + blockIdx.x) * kGT3 * kGT3;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = wi + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int col = wj + 32 * b + (lane & 31);
+                out[row * kGT3 + col] = acc[a][b][v];
+            }
+}
+
+__global__ void __launch_bounds__(256) k_gram_reduce256(const float* __restrict__ part, int nt, int nslab, int n,
+                                                        float* __restrict__ G, size_t strideG) {
+    const int ntiles = nt * (nt + 1) / 2;
+    const int2 tile = upper_tile(blockIdx.x, nt);
+    const float* pb = part + (size_t)blockIdx.z * nslab * ntiles * kGT3 * kGT3;
+    float* Gb = G + blockIdx.z * strideG;
+    for (int i = blockIdx.y * 256 + threadIdx.x; i < kGT3 * kGT3; i += 256 * gridDim.y) {
+        float s = 0.0f;
+        for (int z = 0; z < nslab; ++z) s += pb[((size_t)z * ntiles + blockIdx.x) * kGT3 * kGT3 + i];
+        const int r = tile.x * kGT3 + i / kGT3, c = tile.y * kGT3 + i % kGT3;
+        if (r < n && c < n && (tile.x != tile.y || r <= c)) {
+            Gb[r + (size_t)c * n] = s;
+            Gb[c + (size_t)r * n] = s;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gram_reduce128(const float* __restrict__ part, int nt, int nslab, int n,
                                                         float* __restrict__ G, size_t strideG) {
     const int ntiles = nt * (nt + 1) / 2;
@@ -388,25 +510,30 @@ __global__ void __launch_bounds__(256) k_to64_shift(int n, const float* __restri
     A[b * strideA + i] = (double)G[b * strideG + i] + (r == c ? shift : 0.0);
 }
 
+// The blocked L L^T kernels below are templates over the scalar T: double for the real
+// symmetric positive definite normal equations, zd for the complex symmetric ones of the
+// complex-conductivity model (the same formulas without conjugation).
+//
 // factor the diagonal block kb in place
-__global__ void __launch_bounds__(256) k_potrf_diag(int n, double* __restrict__ A, size_t strideA, int kb,
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_diag(int n, T* __restrict__ A, size_t strideA, int kb,
                                                     int* __restrict__ info) {
-    double* a = A + blockIdx.z * strideA;
-    __shared__ double L[kCB][kCB + 1];
+    T* a = A + blockIdx.z * strideA;
+    __shared__ T L[kCB][kCB + 1];
     const int o = kb * kCB, nb = min(kCB, n - o);
     for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
         const int r = i % kCB, c = i / kCB;
-        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : 0.0;
+        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : Scal<T>::zero();
     }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
         if (threadIdx.x == 0) {
-            double d = L[j][j];
-            if (!(d > 0.0)) { atomicExch(info, 1); d = 1e-300; }
-            L[j][j] = sqrt(d);
+            T d = L[j][j];
+            if (!Scal<T>::pivot_ok(d)) { atomicExch(info, 1); d = Scal<T>::tiny(); }
+            L[j][j] = Scal<T>::sqrt_(d);
         }
         __syncthreads();
-        const double djj = L[j][j];
+        const T djj = L[j][j];
         for (int r = j + 1 + threadIdx.x; r < nb; r += 256) L[r][j] /= djj;
         __syncthreads();
         // trailing update of the block: L[r][c] -= L[r][j] L[c][j], c in (j, r]
@@ -423,26 +550,27 @@ __global__ void __launch_bounds__(256) k_potrf_diag(int n, double* __restrict__ 
 }
 
 // panel: A[ib, kb] <- A[ib, kb] L_kk^-T for every row block ib > kb (blockIdx.x)
-__global__ void __launch_bounds__(256) k_potrf_panel(int n, double* __restrict__ A, size_t strideA, int kb) {
-    double* a = A + blockIdx.z * strideA;
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_panel(int n, T* __restrict__ A, size_t strideA, int kb) {
+    T* a = A + blockIdx.z * strideA;
     const int o = kb * kCB;
     const int ib = kb + 1 + blockIdx.x;
     const int ro = ib * kCB;
     if (ro >= n) return;
     const int nr = min(kCB, n - ro), nb = min(kCB, n - o);
-    __shared__ double L[kCB][kCB + 1];
-    __shared__ double X[kCB][kCB + 1];
+    __shared__ T L[kCB][kCB + 1];
+    __shared__ T X[kCB][kCB + 1];
     for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
         const int r = i % kCB, c = i / kCB;
-        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : 0.0;
-        X[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : 0.0;
+        L[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : Scal<T>::zero();
+        X[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : Scal<T>::zero();
     }
     __syncthreads();
     // each row r of X solves x L^T = a  (forward substitution over columns)
     if (threadIdx.x < nr) {
         const int r = threadIdx.x;
         for (int c = 0; c < nb; ++c) {
-            double s = X[r][c];
+            T s = X[r][c];
             for (int t = 0; t < c; ++t) s -= X[r][t] * L[c][t];
             X[r][c] = s / L[c][c];
         }
@@ -455,25 +583,26 @@ __global__ void __launch_bounds__(256) k_potrf_panel(int n, double* __restrict__
 }
 
 // trailing update: A[ib, jb] -= L[ib, kb] L[jb, kb]^T for kb < jb <= ib
-__global__ void __launch_bounds__(256) k_potrf_update(int n, double* __restrict__ A, size_t strideA, int kb,
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_update(int n, T* __restrict__ A, size_t strideA, int kb,
                                                       const int2* __restrict__ pairs) {
-    double* a = A + blockIdx.z * strideA;
+    T* a = A + blockIdx.z * strideA;
     const int2 pr = pairs[blockIdx.x];
     const int ib = pr.x, jb = pr.y;
     const int o = kb * kCB, ro = ib * kCB, co = jb * kCB;
     const int nb = min(kCB, n - o), nr = min(kCB, n - ro), nc = min(kCB, n - co);
-    __shared__ double Li[kCB][kCB + 1];
-    __shared__ double Lj[kCB][kCB + 1];
+    __shared__ T Li[kCB][kCB + 1];
+    __shared__ T Lj[kCB][kCB + 1];
     for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
         const int r = i % kCB, c = i / kCB;
-        Li[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : 0.0;
-        Lj[r][c] = (r < nc && c < nb) ? a[(co + r) + (size_t)(o + c) * n] : 0.0;
+        Li[r][c] = (r < nr && c < nb) ? a[(ro + r) + (size_t)(o + c) * n] : Scal<T>::zero();
+        Lj[r][c] = (r < nc && c < nb) ? a[(co + r) + (size_t)(o + c) * n] : Scal<T>::zero();
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kCB * kCB; i += 256) {
         const int r = i % kCB, c = i / kCB;
         if (r < nr && c < nc && (ib != jb || r >= c)) {
-            double s = 0.0;
+            T s = Scal<T>::zero();
             for (int t = 0; t < nb; ++t) s += Li[r][t] * Lj[c][t];
             a[(ro + r) + (size_t)(co + c) * n] -= s;
         }
@@ -486,15 +615,16 @@ __global__ void __launch_bounds__(256) k_potrf_update(int n, double* __restrict_
 // wavefront read contiguous column segments of L and reduce them with shuffles.
 constexpr int kTrsvThreads = 1024;
 
-__global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __restrict__ A, size_t strideA,
-                                                        const double* __restrict__ b, size_t strideb,
-                                                        double* __restrict__ y, size_t stridey) {
-    const double* a = A + blockIdx.x * strideA;
-    double* yv = y + blockIdx.x * stridey;
-    const double* bv = b + blockIdx.x * strideb;
+template <class T>
+__global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const T* __restrict__ A, size_t strideA,
+                                                        const T* __restrict__ b, size_t strideb,
+                                                        T* __restrict__ y, size_t stridey) {
+    const T* a = A + blockIdx.x * strideA;
+    T* yv = y + blockIdx.x * stridey;
+    const T* bv = b + blockIdx.x * strideb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = kTrsvThreads / 64;
-    __shared__ double Lb[kCB][kCB + 1];
-    __shared__ double z[kCB];
+    __shared__ T Lb[kCB][kCB + 1];
+    __shared__ T z[kCB];
     for (int i = tid; i < n; i += kTrsvThreads) yv[i] = bv[i];
     __syncthreads();
     const int nblk = (n + kCB - 1) / kCB;
@@ -503,13 +633,13 @@ __global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __r
         const int o = kb * kCB, nb = min(kCB, n - o);
         for (int i = tid; i < kCB * kCB; i += kTrsvThreads) {
             const int r = i % kCB, c = i / kCB;
-            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? 1.0 : 0.0);
+            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? Scal<T>::one() : Scal<T>::zero());
         }
         __syncthreads();
         if (wave == 0) {
-            double v = lane < nb ? yv[o + lane] : 0.0;
+            T v = lane < nb ? yv[o + lane] : Scal<T>::zero();
             for (int r = 0; r < nb; ++r) {
-                const double zr = __shfl(v, r, 64) / Lb[r][r];
+                const T zr = Scal<T>::shfl(v, r) / Lb[r][r];
                 if (lane == r) v = zr;
                 else if (lane > r && lane < nb) v -= Lb[lane][r] * zr;
             }
@@ -517,7 +647,7 @@ __global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __r
         }
         __syncthreads();
         for (int r = o + nb + tid; r < n; r += kTrsvThreads) {
-            double s = 0.0;
+            T s = Scal<T>::zero();
             for (int t = 0; t < nb; ++t) s += a[r + (size_t)(o + t) * n] * z[t];
             yv[r] -= s;
         }
@@ -528,13 +658,13 @@ __global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __r
         const int o = kb * kCB, nb = min(kCB, n - o);
         for (int i = tid; i < kCB * kCB; i += kTrsvThreads) {
             const int r = i % kCB, c = i / kCB;
-            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? 1.0 : 0.0);
+            Lb[r][c] = (r < nb && c < nb) ? a[(o + r) + (size_t)(o + c) * n] : (r == c ? Scal<T>::one() : Scal<T>::zero());
         }
         __syncthreads();
         if (wave == 0) {
-            double v = lane < nb ? yv[o + lane] : 0.0;
+            T v = lane < nb ? yv[o + lane] : Scal<T>::zero();
             for (int r = nb - 1; r >= 0; --r) {
-                const double zr = __shfl(v, r, 64) / Lb[r][r];
+                const T zr = Scal<T>::shfl(v, r) / Lb[r][r];
                 if (lane == r) v = zr;
                 else if (lane < r) v -= Lb[r][lane] * zr;
             }
@@ -543,8 +673,8 @@ __global__ void __launch_bounds__(kTrsvThreads) k_potrs(int n, const double* __r
         __syncthreads();
         // y[c] -= sum_t L[o+t][c] z[t] for c < o: column segment a[o..o+nb, c] is contiguous
         for (int c = wave; c < o; c += nwave) {
-            double p = lane < nb ? a[(o + lane) + (size_t)c * n] * z[lane] : 0.0;
-            for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+            T p = lane < nb ? a[(o + lane) + (size_t)c * n] * z[lane] : Scal<T>::zero();
+            for (int off = 32; off > 0; off >>= 1) p += Scal<T>::shfl_xor(p, off);
             if (lane == 0) yv[c] -= p;
         }
         __syncthreads();
@@ -689,11 +819,11 @@ void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use
 }
 
 void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use64, float* J,
-                     hipStream_t st) {
+                     hipStream_t st, uint16_t* Jb) {
     MPF_REQUIRE(jd.Mp * jd.Dp <= kJacMaxRows, "jacobian: too many rows");
     MPF_REQUIRE(jd.D + jd.M <= 64 && jd.k <= 6, "jacobian: unsupported shape");
     dim3 grid(jd.E, njobs);
-    k_jacobian<<<grid, 256, 0, st>>>(d_jobs, jd, use64, J);
+    k_jacobian<<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, reinterpret_cast<__bf16*>(Jb));
     check_launch();
 }
 
@@ -742,11 +872,55 @@ void launch_gram_nt(const float* A, size_t strideA, int n, int nk, int lda, floa
     const int ntiles = nt * (nt + 1) / 2;
     const int nslab = gram_nt_slabs(n, nk, batch, max_slabs);
     int kslab = (nk + nslab - 1) / nslab;
-    kslab = (kslab + kGK2 - 1) / kGK2 * kGK2;
-    const int used = (nk + kslab - 1) / kslab;
+    kslab = std::max(kGK2, (kslab + kGK2 - 1) / kGK2 * kGK2);
+    const int used = std::max(1, (nk + kslab - 1) / kslab);
     float* part = (float*)work;
     k_gram_nt<<<dim3(ntiles, batch, used), 256, 0, st>>>(A, strideA, n, nk, lda, nt, kslab, part);
     k_gram_reduce128<<<dim3(ntiles, 16, batch), 256, 0, st>>>(part, nt, used, n, G, strideG);
+    check_launch();
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16(size_t n, const float* __restrict__ a, __bf16* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = (__bf16)a[i];
+}
+
+void launch_to_bf16(size_t n, const float* a, uint16_t* b, hipStream_t st) {
+    if (!n) return;
+    k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, reinterpret_cast<__bf16*>(b));
+    check_launch();
+}
+
+// slabs of the 256-tile kernel: about 512 blocks over all batch entries, slabs of at
+// least 1024 contraction steps
+static int gram_bf16_slabs(int n, int nk, int batch) {
+    const int nt = (n + kGT3 - 1) / kGT3;
+    const int ntiles = nt * (nt + 1) / 2;
+    int nslab = std::max(1, (512 + ntiles * batch - 1) / (ntiles * batch));
+    return std::min(nslab, std::max(1, nk / 1024));
+}
+
+bool gram_bf16_applies(int n, int lda) { return n % 8 == 0 && lda % 8 == 0; }
+
+size_t gram_bf16_workspace_bytes(int n, int nk, int batch) {
+    const int nt = (n + kGT3 - 1) / kGT3;
+    const int ntiles = nt * (nt + 1) / 2;
+    return (size_t)batch * gram_bf16_slabs(n, nk, batch) * ntiles * kGT3 * kGT3 * sizeof(float);
+}
+
+void launch_gram_bf16(const uint16_t* A, size_t strideA, int n, int nk, int lda, float* G, size_t strideG,
+                      int batch, void* work, hipStream_t st) {
+    MPF_REQUIRE(gram_bf16_applies(n, lda), "gram: rows and leading dimension must be multiples of 8");
+    const int nt = (n + kGT3 - 1) / kGT3;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int nslab = gram_bf16_slabs(n, nk, batch);
+    int kslab = (nk + nslab - 1) / nslab;
+    kslab = std::max(kGK3, (kslab + kGK3 - 1) / kGK3 * kGK3);
+    const int used = std::max(1, (nk + kslab - 1) / kslab);
+    float* part = (float*)work;
+    k_gram_bf16<<<dim3(ntiles, batch, used), 512, 0, st>>>(reinterpret_cast<const __bf16*>(A), strideA, n, nk, lda,
+                                                            nt, kslab, part);
+    k_gram_reduce256<<<dim3(ntiles, 16, batch), 256, 0, st>>>(part, nt, used, n, G, strideG);
     check_launch();
 }
 
@@ -819,11 +993,11 @@ void launch_cholesky(int n, const float* G, size_t strideG, double shift, double
     const int nblk = (n + kCB - 1) / kCB;
     k_chol_pairs<<<(nblk + 255) / 256, 256, 0, st>>>(nblk, d_pairs);
     for (int kb = 0; kb < nblk; ++kb) {
-        k_potrf_diag<<<dim3(1, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_info);
+        k_potrf_diag<double><<<dim3(1, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_info);
         if (kb + 1 < nblk) {
-            k_potrf_panel<<<dim3(nblk - kb - 1, 1, batch), 256, 0, st>>>(n, A, strideA, kb);
+            k_potrf_panel<double><<<dim3(nblk - kb - 1, 1, batch), 256, 0, st>>>(n, A, strideA, kb);
             const int p0 = chol_pair_start(nblk, kb), np = chol_pair_start(nblk, kb + 1) - p0;
-            k_potrf_update<<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + p0);
+            k_potrf_update<double><<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + p0);
         }
     }
     check_launch();
@@ -836,7 +1010,28 @@ size_t cholesky_pairs(int n) {
 
 void launch_potrs(int n, const double* A, size_t strideA, const double* b, size_t strideb, double* y,
                   size_t stridey, int batch, hipStream_t st) {
-    k_potrs<<<batch, kTrsvThreads, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
+    k_potrs<double><<<batch, kTrsvThreads, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
+    check_launch();
+}
+
+// complex symmetric A = L L^T in place (A: shifted FP64 matrices, both triangles)
+void launch_cholesky_z(int n, zd* A, size_t strideA, int batch, int* d_info, int2* d_pairs, hipStream_t st) {
+    const int nblk = (n + kCB - 1) / kCB;
+    k_chol_pairs<<<(nblk + 255) / 256, 256, 0, st>>>(nblk, d_pairs);
+    for (int kb = 0; kb < nblk; ++kb) {
+        k_potrf_diag<zd><<<dim3(1, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_info);
+        if (kb + 1 < nblk) {
+            k_potrf_panel<zd><<<dim3(nblk - kb - 1, 1, batch), 256, 0, st>>>(n, A, strideA, kb);
+            const int p0 = chol_pair_start(nblk, kb), np = chol_pair_start(nblk, kb + 1) - p0;
+            k_potrf_update<zd><<<dim3(np, 1, batch), 256, 0, st>>>(n, A, strideA, kb, d_pairs + p0);
+        }
+    }
+    check_launch();
+}
+
+void launch_potrs_z(int n, const zd* A, size_t strideA, const zd* b, size_t strideb, zd* y, size_t stridey,
+                    int batch, hipStream_t st) {
+    k_potrs<zd><<<batch, kTrsvThreads, 0, st>>>(n, A, strideA, b, strideb, y, stridey);
     check_launch();
 }
 

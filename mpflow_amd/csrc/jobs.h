@@ -106,17 +106,19 @@ void launch_bicg_init(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
 void launch_bicg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                       int lg, int wpad, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // ev0/ev1 (nullable) are recorded right before and after the SpMV launch
-// one PCG iteration (SpMV, update, pupdate); only the parity of `iter` (iterations since
-// the solve's init) is used: it selects the rho slot.  `variant` picks the SpMV kernel
-// (0: 128-bit gathers, width-binned, for padded widths <= 24; 1: 32-bit gathers).
-constexpr int kSpmvVariants = 2;
+// one PCG iteration (SpMV, update, pupdate); `iter` (iterations since the solve's init)
+// selects the rho slot and the direction-ring entry.  `variant` picks the SpMV kernel
+// (0: 128-bit gathers, width-binned, for padded widths <= 24; 1: 32-bit gathers; 2: as 0
+// with all harmonics of a (job, tile) in one block, over the (job, -, tile) list d_ftiles).
+constexpr int kSpmvVariants = 3;
 // d_js lists the group's (job, slice) pairs and `nchunk` the largest job's 32-tile
 // chunks: after the SpMV a small launch sums p.q and 1^T p per chunk, and between update
 // and pupdate one launch applies the two-level coarse correction (jobs that have one) and
 // sums r.z and r.r per chunk, so update and pupdate reduce chunk partials only.
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                      int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0 = nullptr,
-                     hipEvent_t ev1 = nullptr, const int2* d_js = nullptr, int njs = 0, int nchunk = 0);
+                     hipEvent_t ev1 = nullptr, const int2* d_js = nullptr, int njs = 0, int nchunk = 0,
+                     const int4* d_ftiles = nullptr, int nftiles = 0);
 // per-tile conductivity scale of the two-level preconditioner (after assembly); d1: the
 // model's per-tile sigma = 1 diagonal sums; work items with slice != 0 are skipped
 void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const double* d1,

@@ -66,13 +66,17 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
     }
     size_t total = 0;
     for (auto& g : groups) {
-        std::vector<int4> lst;
+        std::vector<int4> lst, lst_t, flst;
         std::vector<int2> gjs;
         long long rows = 0;
         for (int j : g.jobs) {
             for (int c = 0; c < h[j].C; ++c) {
                 gjs.push_back(make_int2(j, c));
                 for (int t = 0; t < h[j].ntiles; ++t) lst.push_back(make_int4(j, c, t, 0));
+            }
+            for (int t = 0; t < h[j].ntiles; ++t) {
+                for (int c = 0; c < h[j].C; ++c) lst_t.push_back(make_int4(j, c, t, 0));
+                flst.push_back(make_int4(j, 0, t, 0));
             }
             rows += (long long)h[j].C * h[j].np;
             g.nchunk = std::max(g.nchunk, (h[j].ntiles + kChunkTiles - 1) / kChunkTiles);
@@ -82,13 +86,24 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
         g.js.alloc(g.njs);
         g.js.upload(gjs.data(), gjs.size(), s0);
         g.nall = (int)lst.size();
-        g.all.resize(lst.size());
-        xcd_order(lst.data(), g.nall, g.all.data());
-        g.tiles.alloc(g.nall);
-        g.tiles.upload(g.all.data(), g.all.size(), s0);
+        const std::vector<int4>* src_o[2] = {&lst, &lst_t};
+        for (int o = 0; o < 2; ++o) {
+            g.all_o[o].resize(lst.size());
+            xcd_order(src_o[o]->data(), g.nall, g.all_o[o].data());
+            g.tiles_o[o].alloc(g.nall);
+            g.tiles_o[o].upload(g.all_o[o].data(), g.all_o[o].size(), s0);
+        }
         g.act.alloc(g.nall);
         g.hoff = total;
         total += g.nall;
+        g.nfall = (int)flst.size();
+        g.fall.resize(flst.size());
+        xcd_order(flst.data(), g.nfall, g.fall.data());
+        g.ftiles.alloc(g.nfall);
+        g.ftiles.upload(g.fall.data(), g.fall.size(), s0);
+        g.fact.alloc(g.nfall);
+        g.fhoff = total;
+        total += g.nfall;
         // autotune key: lane count, width, and the power-of-two bucket of the rows per
         // launch (small launches favour different kernels than large ones)
         int bucket = 0;
@@ -131,40 +146,51 @@ void KrylovDriver::launch_group_iter(Group& g, int iter, bool bicg, hipStream_t 
         launch_bicg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, s, e0, e1);
     else
         launch_pcg_iter(jobs.p, g.cur, g.ncur, active.p, iter, g.lg, g.wpad, g.variant, s, e0, e1, g.js.p, g.njs,
-                        g.nchunk);
+                        g.nchunk, g.fcur, g.nfcur);
 }
 
-// Time each SpMV variant that supports the group's width on the live job data (the SpMV
-// only writes q and the SpMV partials, which the next iteration recomputes) and keep the
-// fastest in the context's cache.
+// Time each SpMV candidate that supports the group's width on the live job data (the
+// SpMV only writes q and the SpMV partials, which the next iteration recomputes) and keep
+// the fastest in the context's cache.  Candidates: the 128-bit gather kernel over
+// (job, slice, tile) and over (job, tile, slice) work lists (in the second, the harmonics
+// of one tile run back to back on one XCD and share its L2 copy of the tile's matrix),
+// the 32-bit gather kernel, and the harmonic-fused kernel over (job, tile).
 void KrylovDriver::tune(Group& g) {
+    int choice = -1;
     {
         std::lock_guard<std::mutex> lk(ctx->tune.mu);
         auto it = ctx->tune.spmv.find(g.key);
-        if (it != ctx->tune.spmv.end()) { g.variant = it->second; return; }
+        if (it != ctx->tune.spmv.end()) choice = it->second;
     }
-    hipStream_t s0 = ctx->stream;
-    hipEvent_t a, b;
-    MPF_HIP(hipEventCreate(&a));
-    MPF_HIP(hipEventCreate(&b));
-    float best = 1e30f;
-    int pick = g.wpad <= 24 ? 0 : 1;
-    for (int v = 0; v < kSpmvVariants; ++v) {
-        if (v == 0 && g.wpad > 24) continue;
-        launch_pcg_spmv(jobs.p, g.tiles.p, g.nall, g.lg, g.wpad, v, s0);   // warm-up
-        MPF_HIP(hipEventRecord(a, s0));
-        for (int r = 0; r < 8; ++r) launch_pcg_spmv(jobs.p, g.tiles.p, g.nall, g.lg, g.wpad, v, s0);
-        MPF_HIP(hipEventRecord(b, s0));
-        MPF_HIP(hipEventSynchronize(b));
-        float ms = 0.0f;
-        MPF_HIP(hipEventElapsedTime(&ms, a, b));
-        if (ms < best) { best = ms; pick = v; }
+    if (choice < 0) {
+        hipStream_t s0 = ctx->stream;
+        hipEvent_t a, b;
+        MPF_HIP(hipEventCreate(&a));
+        MPF_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        choice = g.wpad <= 24 ? 0 : 1;
+        const int cand[4][2] = {{0, 0}, {0, 1}, {1, 0}, {2, 0}};
+        for (const auto& c : cand) {
+            const int v = c[0], o = c[1];
+            if (v != 1 && g.wpad > 24) continue;
+            const int4* lst = v == 2 ? g.ftiles.p : g.tiles_o[o].p;
+            const int n = v == 2 ? g.nfall : g.nall;
+            launch_pcg_spmv(jobs.p, lst, n, g.lg, g.wpad, v, s0);   // warm-up
+            MPF_HIP(hipEventRecord(a, s0));
+            for (int r = 0; r < 8; ++r) launch_pcg_spmv(jobs.p, lst, n, g.lg, g.wpad, v, s0);
+            MPF_HIP(hipEventRecord(b, s0));
+            MPF_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            MPF_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) { best = ms; choice = v | (o << 4); }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        std::lock_guard<std::mutex> lk(ctx->tune.mu);
+        ctx->tune.spmv[g.key] = choice;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    g.variant = pick;
-    std::lock_guard<std::mutex> lk(ctx->tune.mu);
-    ctx->tune.spmv[g.key] = pick;
+    g.variant = choice & 15;
+    g.order = choice >> 4;
 }
 
 bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_rhs, bool ref_b,
@@ -193,9 +219,12 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
     else launch_pcg_init(jobs.p, init_tiles.p, n_init_tiles, init_js.p, n_init_js, active.p, s0,
                          coarse ? init_nchunk : 0);
     for (auto& g : groups) {
-        g.cur = g.tiles.p;
-        g.ncur = g.nall;
         if (!bicg) tune(g);
+        else g.order = 0;
+        g.cur = g.tiles_o[g.order].p;
+        g.ncur = g.nall;
+        g.fcur = g.ftiles.p;
+        g.nfcur = g.nfall;
     }
     int every = o.check_every > 0 ? o.check_every : 16;
     every = (every + kRing - 1) / kRing * kRing;   // whole direction-ring cycles per poll
@@ -239,18 +268,21 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
         // listed tiles belong to finished pairs (finished pairs' blocks exit at once)
         for (auto& g : groups) {
             std::vector<int4> un;
-            for (int b = 0; b < g.nall; ++b) {
-                const int4 t = g.all[b];
+            for (const int4& t : g.all_o[g.order])
                 if (h_st[((size_t)t.x * kMaxSlices + t.y) * kStInts]) un.push_back(t);
-            }
             const int nt = (int)un.size();
             if (nt == g.ncur) continue;
             if (graphs && nt * 4 > g.ncur * 3 && nt > 0) continue;
-            // back to (job, slice, tile) order, then striped over the XCDs again so that
-            // XCD x gets a contiguous run of the survivors
-            std::sort(un.begin(), un.end(), [](const int4& a, const int4& b) {
-                return a.x != b.x ? a.x < b.x : (a.y != b.y ? a.y < b.y : a.z < b.z);
-            });
+            // back to (job, slice, tile) or (job, tile, slice) order, then striped over
+            // the XCDs again so that XCD x gets a contiguous run of the survivors
+            if (g.order == 0)
+                std::sort(un.begin(), un.end(), [](const int4& a, const int4& b) {
+                    return a.x != b.x ? a.x < b.x : (a.y != b.y ? a.y < b.y : a.z < b.z);
+                });
+            else
+                std::sort(un.begin(), un.end(), [](const int4& a, const int4& b) {
+                    return a.x != b.x ? a.x < b.x : (a.z != b.z ? a.z < b.z : a.y < b.y);
+                });
             int4* ordered = h_tiles + g.hoff;
             xcd_order(un.data(), nt, ordered);
             if (nt) MPF_HIP(hipMemcpyAsync(g.act.p, ordered, sizeof(int4) * nt, hipMemcpyHostToDevice, s0));
@@ -258,6 +290,23 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
             // synchronisation
             g.cur = g.act.p;
             g.ncur = nt;
+            // the fused SpMV's (job, tile) items: jobs with a live harmonic
+            std::vector<int4> fun;
+            for (int b = 0; b < g.nfall; ++b) {
+                const int4 t = g.fall[b];
+                bool live = false;
+                for (int c = 0; c < h[t.x].C; ++c) live = live || h_st[((size_t)t.x * kMaxSlices + c) * kStInts];
+                if (live) fun.push_back(t);
+            }
+            std::sort(fun.begin(), fun.end(), [](const int4& a, const int4& b) {
+                return a.x != b.x ? a.x < b.x : a.z < b.z;
+            });
+            const int nf = (int)fun.size();
+            int4* fordered = h_tiles + g.fhoff;
+            xcd_order(fun.data(), nf, fordered);
+            if (nf) MPF_HIP(hipMemcpyAsync(g.fact.p, fordered, sizeof(int4) * nf, hipMemcpyHostToDevice, s0));
+            g.fcur = g.fact.p;
+            g.nfcur = nf;
             drop_graph();
         }
     }

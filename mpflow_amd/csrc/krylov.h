@@ -17,11 +17,14 @@ namespace mpf {
 // XCD x receives a contiguous run of consecutive tiles (shared neighbours in its L2)
 void xcd_order(const int4* in, int n, int4* out);
 
-// SpMV variant chosen per (lane count, padded width, row-count bucket) by timing the
-// candidates once on live data; shared by all models of a context
+// Launch choices made by timing the candidates once on live data, shared by all models
+// of a context: the SpMV kernel and work-list order per (lane count, padded width,
+// row-count bucket) (value: variant | order << 4), and the Gram kernel's tile per
+// (rows, contraction length, batch) of the Gauss-Newton step (value: 128 or 256).
 struct AutotuneCache {
     std::mutex mu;
     std::map<long long, int> spmv;
+    std::map<long long, int> gram;
 };
 
 class KrylovDriver {
@@ -46,15 +49,23 @@ class KrylovDriver {
   private:
     struct Group {
         int lg = 0, wpad = 0, variant = 0;
+        int order = 0;                // work-list order: 0 (job, slice, tile), 1 (job, tile, slice)
         long long key = 0;
         std::vector<int> jobs;        // indices into h
-        std::vector<int4> all;        // every tile of the group's jobs, XCD-ordered
-        DBuf<int4> tiles, act;        // all tiles / the active ones
+        std::vector<int4> all_o[2];   // every (job, slice, tile) item in either order, XCD-ordered
+        DBuf<int4> tiles_o[2];
+        DBuf<int4> act;               // the active items
         DBuf<int2> js;                // the group's (job, slice) pairs (coarse correction)
         int njs = 0, nchunk = 0;      // coarse row chunks of the group's largest job
         int nall = 0, ncur = 0;
         const int4* cur = nullptr;
         size_t hoff = 0;              // the group's region of the pinned staging buffer
+        // (job, 0, tile) items of the harmonic-fused SpMV (variant 2)
+        std::vector<int4> fall;
+        DBuf<int4> ftiles, fact;
+        int nfall = 0, nfcur = 0;
+        const int4* fcur = nullptr;
+        size_t fhoff = 0;
     };
     void launch_group_iter(Group& g, int iter, bool bicg, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
     bool coarse = false;              // this run uses the two-level preconditioner

@@ -232,24 +232,44 @@ using XTile = float4[kTileRows * (1 << LG) / 4];
 template <int WMAX>
 using EllStage = int2[kEwBlock / 64][WMAX][kSpmvRowsPerWave];
 
+// Slot-count bin of a wavefront whose widest row has wl entries: bins 10, 12, 16, 20 and
+// WMAX (P2 meshes: edge-midpoint rows 9 wide, vertex rows 19).
+template <int WMAX>
+__device__ __forceinline__ int ell_bin(int wl) {
+    constexpr int WS = WMAX > 12 ? 12 : WMAX;
+    constexpr int W10 = WMAX > 10 ? 10 : WMAX;
+    return (wl <= W10 && W10 < WS) ? W10
+         : (wl <= WS ? WS : (wl <= 16 && WMAX > 16 ? 16 : (wl <= 20 && WMAX > 20 ? 20 : WMAX)));
+}
+
+// ell_gather_rows with the slot loop unrolled for bin wb (= ell_bin<WMAX>)
+template <int LG, int WMAX, bool XT, class Epi>
+__device__ __forceinline__ void ell_gather_bin(int wb, int np, int row0, int l64, const float* X,
+                                               const int2 (*ell)[kSpmvRowsPerWave], const float4* xs, int tile0,
+                                               Epi&& epi) {
+    constexpr int WS = WMAX > 12 ? 12 : WMAX;
+    constexpr int W10 = WMAX > 10 ? 10 : WMAX;
+    if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(np, row0, l64, X, ell, xs, tile0, epi);
+    else if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(np, row0, l64, X, ell, xs, tile0, epi);
+    else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(np, row0, l64, X, ell, xs, tile0, epi);
+    else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(np, row0, l64, X, ell, xs, tile0, epi);
+    else ell_gather_rows<LG, WMAX, XT>(np, row0, l64, X, ell, xs, tile0, epi);
+}
+
 // One wavefront's 16 rows of y = A_s X.  The slot count is binned by the wavefront's group
-// width: a group whose rows are all at most 12 entries wide (P2 midpoint rows: 9) runs the
-// 12-slot body.  The wavefront first stages its (id, S + alpha_s R) entries in LDS with
-// coalesced loads spread over all 64 lanes; slots between the group width and the bin are
-// zero.  Contains a block barrier: every wavefront of the block must call it.
+// width (ell_bin): a group whose rows are all at most 12 entries wide (P2 midpoint rows: 9)
+// runs the 12-slot body.  The wavefront first stages its (id, S + alpha_s R) entries in LDS
+// with coalesced loads spread over all 64 lanes; slots between the group width and the bin
+// are zero.  Contains a block barrier: every wavefront of the block must call it.
 // XT: xs is LDS room for the tile's rows of X, loaded here by the whole block.
 template <int LG, int WMAX, bool XT = false, class Epi>
 __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0, int l64, const float* X,
                                                 EllStage<WMAX>& stage, Epi&& epi, float4* xs = nullptr) {
     constexpr int LPR = (1 << LG) / 4;
     const int tile0 = row0 / kTileRows * kTileRows;
-    // slot-count bins 10, 12, 16, 20, WMAX (P2 meshes: midpoint rows 9 wide, vertex rows 19)
-    constexpr int WS = WMAX > 12 ? 12 : WMAX;
-    constexpr int W10 = WMAX > 10 ? 10 : WMAX;    // P2 edge-midpoint rows: 9 entries
     const int wave = threadIdx.x >> 6;
     const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
-    const int wb = (wl <= W10 && W10 < WS) ? W10
-                 : (wl <= WS ? WS : (wl <= 16 && WMAX > 16 ? 16 : (wl <= 20 && WMAX > 20 ? 20 : WMAX)));
+    const int wb = ell_bin<WMAX>(wl);
     const int np = J.np;
     const float al = J.alpha[s];
     const MPF_GLOBAL int32_t* ids = gp(J.ids);
@@ -304,11 +324,7 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         }
     }
     __syncthreads();
-    if (wb == W10 && W10 < WS) ell_gather_rows<LG, W10, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == WS && WS < WMAX) ell_gather_rows<LG, WS, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == 16 && WMAX > 16) ell_gather_rows<LG, (WMAX > 16 ? 16 : WMAX), XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
-    else if (wb == 20 && WMAX > 20) ell_gather_rows<LG, (WMAX > 20 ? 20 : WMAX), XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
-    else ell_gather_rows<LG, WMAX, XT>(J.np, row0, l64, X, stage[wave], xs, tile0, epi);
+    ell_gather_bin<LG, WMAX, XT>(wb, np, row0, l64, X, stage[wave], xs, tile0, epi);
 }
 
 // ---- 128-bit streaming variants: a lane owns 4 consecutive RHS columns -------------

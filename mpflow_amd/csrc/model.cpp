@@ -291,8 +291,10 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
             // coupled electrodes: the mode weights n_c of the symmetric block system
             if (S.cem_coupled) jd.weight[c] = (float)S.n_c[c];
         }
-        if (S.cem_coupled) launch_jacobian(jobs_phi.p, frames, jd, 1, d_J, st0);
-        else launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0);
+        // the Gauss-Newton step's own Jacobian also gets its BF16 copy for the Gram
+        uint16_t* jb = (d_J == Jm.p && Jb.p && reg_type == MPF_REG_IDENTITY) ? Jb.p : nullptr;
+        if (S.cem_coupled) launch_jacobian(jobs_phi.p, frames, jd, 1, d_J, st0, jb);
+        else launch_jacobian(jobs.p, frames, jd, mixed ? 1 : 0, d_J, st0, jb);
     }
 }
 
@@ -373,6 +375,7 @@ void Model::ensure_gn() {
     const int m = S.mrows;
     if (Jm.p) return;
     Jm.alloc((size_t)frames * m * S.E);
+    if (gram_bf16_applies(m, m)) Jb.alloc((size_t)frames * m * S.E);
     G.alloc((size_t)frames * m * m);
     A64.alloc((size_t)frames * m * m);
     U64.alloc((size_t)frames * m);
@@ -380,7 +383,8 @@ void Model::ensure_gn() {
     rgn.alloc((size_t)frames * m);
     vE.alloc((size_t)frames * S.E);
     work_n.alloc(gemv_n_workspace(m, S.E, frames) / sizeof(double) + 1);
-    gram_work.alloc(std::max(gram_workspace_bytes(m, S.E), gram_nt_workspace_bytes(m, S.E, frames)));
+    gram_work.alloc(std::max({gram_workspace_bytes(m, S.E), gram_nt_workspace_bytes(m, S.E, frames),
+                              Jb.p ? gram_bf16_workspace_bytes(m, S.E, frames) : (size_t)0}));
     pairs.alloc(cholesky_pairs(m));
     info.alloc(1);
 }
@@ -411,7 +415,9 @@ void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int g
     // noser: (J^T J + lam^2 W^-2) delta = J^T r is solved as J W = Js, delta = W Js^T y
     if (reg_type != MPF_REG_IDENTITY) launch_scale_cols(m, E, Jm.p, m, (size_t)m * E, frames, reg_w.p, st0);
     // J J^T on MFMA, then (J J^T + lam I) by FP64 Cholesky
-    if (gram_nt_applies(m, m))
+    if (gram_tile() == 256)
+        launch_gram_bf16(Jb.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
+    else if (gram_nt_applies(m, m))
         launch_gram_nt(Jm.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
     else
         for (int f = 0; f < frames; ++f)
@@ -429,6 +435,46 @@ void Model::gn_dense(float* d_gamma, const double* d_Umeas, double lambda, int g
     MPF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st0));
     MPF_HIP(hipStreamSynchronize(st0));
     if (hinfo) throw Error(MPF_EINVAL, "gn_step: normal-equation matrix is not positive definite");
+}
+
+// Tile of the Gauss-Newton step's Gram: 256 (k_gram_bf16 on the BF16 copy of J that the
+// Jacobian kernel writes with identity regularisation) or 128 (k_gram_nt, converting the
+// FP32 J while staging).  Both round J to BF16 the same way; they differ in the order of
+// the FP32 sums.  Where both apply, each is timed once per (rows, elements, frames) on the
+// current J and the faster is kept in the context's cache.
+int Model::gram_tile() {
+    const int m = S.mrows, E = S.E;
+    if (!(Jb.p && reg_type == MPF_REG_IDENTITY)) return 128;
+    if (!gram_nt_applies(m, m)) return 256;
+    const long long key = ((long long)m << 44) | ((long long)E << 16) | frames;
+    {
+        std::lock_guard<std::mutex> lk(ctx->tune.mu);
+        auto it = ctx->tune.gram.find(key);
+        if (it != ctx->tune.gram.end()) return it->second;
+    }
+    hipStream_t st0 = ctx->stream;
+    hipEvent_t a, b;
+    MPF_HIP(hipEventCreate(&a));
+    MPF_HIP(hipEventCreate(&b));
+    float ms[2] = {0.0f, 0.0f};
+    for (int c = 0; c < 2; ++c) {
+        auto run = [&]() {
+            if (c == 0) launch_gram_nt(Jm.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
+            else launch_gram_bf16(Jb.p, (size_t)m * E, m, E, m, G.p, (size_t)m * m, frames, gram_work.p, st0);
+        };
+        run();   // warm-up
+        MPF_HIP(hipEventRecord(a, st0));
+        for (int r = 0; r < 3; ++r) run();
+        MPF_HIP(hipEventRecord(b, st0));
+        MPF_HIP(hipEventSynchronize(b));
+        MPF_HIP(hipEventElapsedTime(&ms[c], a, b));
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const int tile = ms[1] < ms[0] ? 256 : 128;
+    std::lock_guard<std::mutex> lk(ctx->tune.mu);
+    ctx->tune.gram[key] = tile;
+    return tile;
 }
 
 void Model::normal_solve(const float* Jp, size_t strideJ, const double* A, size_t strideA, double lam2, int nb,

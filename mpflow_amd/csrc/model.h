@@ -63,6 +63,7 @@ struct Model {
     DBuf<JobDev> jobs_phi;
     // Gauss-Newton buffers (allocated on first use)
     DBuf<float> Jm, G;
+    DBuf<uint16_t> Jb;            // BF16 copy of Jm for the Gram (identity regularisation)
     DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;
     DBuf<char> gram_work;
     DBuf<int2> pairs;
@@ -110,6 +111,7 @@ struct Model {
   private:
     void ensure_bicg();
     void ensure_gn();
+    int gram_tile();
     // frames' gamma <- one reference gamma; forward solve with Jacobian into U64 / Jm
     int forward_at(const float* d_gamma_ref, const mpf_solve_opts& o, mpf_solve_stats* stats);
     // (J J^T + lam2 I) y = r for `nb` right-hand sides against one factored matrix A

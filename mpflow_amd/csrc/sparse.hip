@@ -180,6 +180,94 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
     block_partials4_buf<LG, 2>(J, s, t.z, ch, v, sh.red);
 }
 
+// k_pcg_spmv4 for every live harmonic of one (job, tile) work item (tiles[b].y unused):
+// the wavefront's ELL entries (id, S, R) are loaded once, kept in registers and restaged
+// in LDS with each harmonic's alpha_s (the same S + alpha_s R value as k_pcg_spmv4), so
+// the tile's matrix is read once for all harmonics instead of once per harmonic.  Per
+// harmonic the tile's rows of p go to LDS, then q, p.q and 1^T p follow k_pcg_spmv4.
+template <int LG, int WMAX>
+__global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4h(const JobDev* __restrict__ jobs,
+                                                         const int4* __restrict__ tiles, int k4) {
+    constexpr int Rp = 1 << LG;
+    constexpr int LPR = Rp / 4;
+    constexpr int NX = (kTileRows * LPR + kEwBlock - 1) / kEwBlock;
+    constexpr int NE = (WMAX * kSpmvRowsPerWave + 63) / 64;
+    const int4 t = tiles[blockIdx.x];
+    const JobDev& J = jobs[t.x];
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int tile0 = t.z * kTileRows;
+    const int row0 = tile0 + wave * kSpmvRowsPerWave;
+    const int np = J.np;
+    __shared__ union {
+        EllStage<WMAX> stage;
+        double red[2][kEwBlock / 64][64];
+    } sh;
+    __shared__ XTile<LG> xs;
+    const int wl = __builtin_amdgcn_readfirstlane((int)J.gw[row0 / kSpmvRowsPerWave]);
+    const int wb = ell_bin<WMAX>(wl);
+    int idv[NE];
+    float sv[NE], rv[NE];
+    {
+        const MPF_GLOBAL int32_t* ids = gp(J.ids);
+        const MPF_GLOBAL float* Sv = gp((const float*)J.s);
+        const MPF_GLOBAL float* Rv = gp((const float*)J.r);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
+            idv[k] = 0; sv[k] = 0.0f; rv[k] = 0.0f;
+            if (q < wl) {
+                const size_t g = (size_t)q * np + row0 + r;
+                idv[k] = ids[g]; sv[k] = Sv[g]; rv[k] = Rv[g];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            // neighbours inside the tile are staged as -1 - (row - tile0)
+            const int local = idv[k] - tile0;
+            if ((unsigned)local < (unsigned)kTileRows) idv[k] = -1 - local;
+        }
+    }
+    const int ch[2] = {0, 1};
+    for (int s = 0; s < J.C; ++s) {
+        if (!slice_active(J, s)) continue;
+        const size_t slice_off = (size_t)s * np * Rp;
+        const float* X = pbuf(J, k4) + slice_off;
+        const MPF_GLOBAL float4* X4 = (const MPF_GLOBAL float4*)gp(X) + (size_t)tile0 * LPR;
+        float4 xv[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int i = threadIdx.x + k * kEwBlock;
+            if (i < kTileRows * LPR) xv[k] = X4[i];
+        }
+        // the previous harmonic's readers of xs and of the stage / partial-sum room are done
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int i = threadIdx.x + k * kEwBlock;
+            if (i < kTileRows * LPR) xs[i] = xv[k];
+        }
+        const float al = J.alpha[s];
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
+            if (q < wb) sh.stage[wave][q][r] = q < wl ? make_int2(idv[k], __float_as_int(sv[k] + al * rv[k]))
+                                                      : make_int2(0, 0);
+        }
+        __syncthreads();
+        MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
+        double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        ell_gather_bin<LG, WMAX, true>(wb, np, row0, l64, X, sh.stage[wave], xs, tile0,
+                                       [&](int row, int c4, float4 acc) {
+            const float4 own = xs[(row - tile0) * LPR + c4];
+            Q4[(size_t)row * LPR + c4] = acc;
+            v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
+            v[0][2] += (double)own.z * (double)acc.z; v[0][3] += (double)own.w * (double)acc.w;
+            v[1][0] += (double)own.x; v[1][1] += (double)own.y; v[1][2] += (double)own.z; v[1][3] += (double)own.w;
+        });
+        block_partials4_buf<LG, 2>(J, s, t.z, ch, v, sh.red);
+    }
+}
+
 template <int LG, int WMAX, int U>
 __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict__ jobs,
                                                        const int4* __restrict__ tiles, int k4) {
@@ -886,12 +974,22 @@ void launch_coarse_scale(const JobDev* d_jobs, const int4* d_tiles, int ntiles, 
 }
 
 // SpMV variant 0: 128-bit gathers (k_pcg_spmv4); variant 1: 32-bit gathers with
-// v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner).
+// v_readlane broadcast (k_pcg_spmv, kept for comparison by the tuner); variant 2: 128-bit
+// gathers, all harmonics of a (job, tile) per block (k_pcg_spmv4h; d_tiles lists
+// (job, -, tile) items).
 template <int LG>
 static void spmv_dispatch(int wpad, int variant, const JobDev* d_jobs, const int4* d_tiles, int ntiles,
                           hipStream_t st, int k4) {
     constexpr int STEPS = kSpmvRowsPerWave / (64 >> LG);
     constexpr int U2 = STEPS >= 2 ? 2 : 1;
+    if (variant == 2) {
+        switch (wpad) {
+            case 8:  k_pcg_spmv4h<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
+            case 16: k_pcg_spmv4h<LG, 16><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
+            case 24: k_pcg_spmv4h<LG, 24><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
+            default: throw Error(MPF_EINVAL, "spmv: the harmonic-fused variant takes padded widths up to 24");
+        }
+    }
     if (variant == 0) {
         switch (wpad) {
             case 8:  k_pcg_spmv4<LG, 8><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles, k4); return;
@@ -929,9 +1027,14 @@ void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int 
 
 void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int* d_active, int iter,
                      int lg, int wpad, int variant, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
-                     const int2* d_js, int njs, int nchunk) {
+                     const int2* d_js, int njs, int nchunk, const int4* d_ftiles, int nftiles) {
     if (ev0) MPF_HIP(hipEventRecord(ev0, st));
-    launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st, iter & (kRing - 1));
+    if (variant == 2) {
+        MPF_REQUIRE(d_ftiles, "pcg iteration: the harmonic-fused SpMV needs a (job, tile) list");
+        if (nftiles) launch_pcg_spmv(d_jobs, d_ftiles, nftiles, lg, wpad, variant, st, iter & (kRing - 1));
+    } else {
+        launch_pcg_spmv(d_jobs, d_tiles, ntiles, lg, wpad, variant, st, iter & (kRing - 1));
+    }
     if (ev1) MPF_HIP(hipEventRecord(ev1, st));
     MPF_REQUIRE(d_js && njs > 0 && nchunk > 0, "pcg iteration: no (job, slice) list");
     if (lg == 6) chunks_launch<6>(d_jobs, d_js, njs, nchunk, st);

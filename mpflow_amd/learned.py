@@ -124,7 +124,7 @@ def generate(eit, n: int, seed0: int, phantom, noise_db: float = 60.0, options: 
         U = eit.result64.cpu().numpy()[:k]
         s = np.sqrt(np.mean(U ** 2, axis=1, keepdims=True)) * 10.0 ** (-noise_db / 20.0)
         Us.append(U + s * rng.standard_normal(U.shape))
-        Gs.append(np.log(g[:k]))
+        Gs.append(np.log(g[:k]).astype(np.float32))
         if log and (b0 // F) % 8 == 7:
             log("generated %d / %d" % (b0 + k, n))
     return np.concatenate(Us), np.concatenate(Gs)

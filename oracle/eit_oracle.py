@@ -325,6 +325,10 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
       ``rule="tolerance"``: stop when every column has ||r||/||b|| < tol.
     * ``prec`` (optional callable r -> z) replaces the diagonal preconditioner (the
       product's two-level preconditioner, ``two_level_preconditioner``).
+    * complex A or b (``ConjugateGradient<thrust::complex<float>>``,
+      conjugate_gradient.cpp:231-232): every dot product conjugates its second operand
+      (vectorDotProductKernel, matrix_kernel.cu:405-413): rho = sum r conj(z),
+      p.q = sum p conj(q), b.b = sum b conj(b); the stop rule takes |sqrt(rho / b.b)|.
     * ``fp32``: the reference's own arithmetic -- matrix, right-hand side, vectors and the
       Jacobi inverse rounded to float32; A v summed row by row in float32 over the row's
       entries in stored order, each product rounded before it is added (the product's
@@ -338,6 +342,10 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     iters = 10 * N if max_iter == 0 else max_iter
     if dinv is None:
         dinv = np.ones(N)
+    cplx = np.iscomplexobj(b) or np.iscomplexobj(dinv) or sp.csr_matrix(A).dtype.kind == "c"
+    dt = np.complex128 if cplx else np.float64
+    if fp32 and cplx:
+        raise ValueError("fp32 emulation: real systems only")
     if fp32:
         rd = lambda v: np.asarray(v, np.float32).astype(np.float64)
         A = sp.csr_matrix(A).astype(np.float32).astype(np.float64)
@@ -368,34 +376,34 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         return rd(y)
     precond = prec if prec is not None else (lambda v: rd(dinv[:, None] * v))
     if x0 is None:
-        x = np.zeros((N, R))
-        r = b.copy()
+        x = np.zeros((N, R), dt)
+        r = np.array(b, dt)
     else:
-        x = np.array(x0, np.float64).reshape(N, R)
+        x = np.array(x0, dt).reshape(N, R)
         r = b - A @ x
         if dc_free:
             r = r + x.sum(axis=0, keepdims=True)
     z = precond(r)
     p = z.copy()
-    rho_old = np.sum(r * z, axis=0)
-    ref = np.sum(b * b, axis=0)
+    rho_old = np.sum(r * np.conj(z), axis=0)
+    ref = np.real(np.sum(b * np.conj(b), axis=0))
     for step in range(iters):
         q = op(p)
-        pq = np.sum(p * q, axis=0)
+        pq = np.sum(p * np.conj(q), axis=0)
         with np.errstate(divide="ignore", invalid="ignore"):
             a = np.where(pq != 0.0, rho_old / np.where(pq != 0, pq, 1.0), 0.0)
         a = rd(a)
         r = rd(np.where(pq != 0.0, r - a * q, 0.0))
         x = rd(np.where(pq != 0.0, x + a * p, 0.0))
         z = precond(r)
-        rho = np.sum(r * z, axis=0)
+        rho = np.sum(r * np.conj(z), axis=0)
         if rule == "reference":
             with np.errstate(divide="ignore", invalid="ignore"):
                 crit = abs(math.sqrt(abs(rho[0]) / ref[0])) if ref[0] != 0 else float("nan")
             if not (crit >= eps):
                 return x, step + 1
         else:
-            rr = np.sum(r * r, axis=0)
+            rr = np.real(np.sum(r * np.conj(r), axis=0))
             ok = np.where(ref > 0, rr <= (tol * tol) * ref, True)
             if np.all(ok):
                 return x, step + 1
@@ -558,7 +566,8 @@ def solve_direct(A, F, dc_free):
     grounded (row and column removed) and the result shifted to the required nodal sum.
     """
     N = A.shape[0]
-    F = np.asarray(F, dtype=np.float64)
+    cplx = np.iscomplexobj(F) or sp.csr_matrix(A).dtype.kind == "c"
+    F = np.asarray(F, dtype=np.complex128 if cplx else np.float64)
     if not dc_free:
         lu = spla.splu(sp.csc_matrix(A))
         return lu.solve(F)
@@ -836,18 +845,95 @@ class CoupledCEMOracle(EITOracle):
 # Tikhonov Gauss-Newton (src/solver/inverse.cpp:60-238, src/solver/solver.cpp:142-225)
 # ---------------------------------------------------------------------------
 
+class ComplexEITOracle(EITOracle):
+    """``models::EIT`` with complex conductivity (``Equation<thrust::complex<float>, ...>``,
+    src/models/eit.cpp:289-304): gamma, the system matrices, fields, voltages and the
+    Jacobian are complex; the excitation and the 2.5D constants stay real.  The system
+    A_c = sum_e sigma_e (K_e + alpha_c M_e) is complex symmetric (no conjugation anywhere
+    but in the Krylov solvers' dot products, ``pcg_reference``), and the Jacobian keeps the
+    real formula with complex fields (calcJacobianKernel, equation_kernel.cu:197-245).
+    Open (current) sources and the gap model only."""
+
+    def __init__(self, *args, **kw):
+        if kw.get("contact_impedance", 0.0) or kw.get("voltage", False):
+            raise ValueError("complex conductivity: current sources and the gap model only")
+        super().__init__(*args, **kw)
+
+    def _sigma(self, gamma):
+        g = np.asarray(gamma, np.complex128)
+        return self.ref * np.exp(np.log(10.0) * g / 10.0) if self.log else self.ref * g
+
+    def _dsigma(self, gamma):
+        g = np.asarray(gamma, np.complex128)
+        if self.log:
+            return (np.log(10.0) / 10.0) * self.ref * np.exp(np.log(10.0) * g / 10.0)
+        return self.ref * np.ones_like(g)
+
+    def system(self, gamma, c):
+        sig = self._sigma(gamma)
+        alpha, _ = self.alpha_beta(c)
+        return (assemble(self.elements, self.Ke, self.N, sig) + alpha * assemble(self.elements, self.Me, self.N, sig)).tocsr()
+
+    def fields(self, gamma, method="direct", **cg):
+        out, steps = [], 0
+        for c in range(self.C):
+            A = self.system(gamma, c)
+            _, beta = self.alpha_beta(c)
+            F = (beta * (self.B @ self.Pi)).astype(np.complex128)
+            if method == "direct":
+                out.append(solve_direct(A, F, c == 0))
+            else:
+                X, st = pcg_reference(A, F, 1.0 / A.diagonal(), c == 0, **cg)
+                out.append(X)
+                steps += st
+        return out, steps
+
+    def voltages(self, phis, gamma=None):
+        U = np.zeros((self.M, self.D), np.complex128)
+        for X in phis:
+            U += self.W @ X[:, :self.D]
+        return U
+
+    def jacobian(self, gamma, phis, quirks=None):
+        quirks = self.quirks if quirks is None else quirks
+        ds = self._dsigma(gamma)
+        J = np.zeros((self.Dp, self.Mp, self.E), np.complex128)
+        el = self.elements
+        for c, X in enumerate(phis):
+            alpha, beta = self.alpha_beta(c)
+            Kc = self.Ke if quirks else self.Ke + alpha * self.Me
+            w = self.h if quirks else 1.0 / beta
+            Pd = X[el][:, :, :self.D]
+            T = np.einsum("eij,ejm->eim", Kc, X[el][:, :, self.D:])
+            J[:self.D, :self.M, :] += w * np.einsum("eid,eim->dme", Pd, T)
+        J *= -ds[None, None, :]
+        return J.reshape(self.Dp * self.Mp, self.E)
+
+    def flatten_voltages(self, U):
+        out = np.zeros((self.Dp, self.Mp), np.complex128)
+        out[:self.D, :self.M] = U.T
+        return out.reshape(-1)
+
+
 def gn_step_exact(J, r, lam_eff2):
     """delta = (J^T J + lam I)^-1 J^T r, computed as J^T (J J^T + lam I)^-1 r
     (Cholesky of the m x m data-space matrix, FP64).  Equal to the E x E normal
-    equations of Inverse::solve by the push-through identity."""
+    equations of Inverse::solve by the push-through identity.  Complex J: the reference
+    forms J^T J and J^T r with plain transposes (CUBLAS_OP_T, inverse.cpp:148-152,
+    196-204), so the matrix is complex symmetric, not Hermitian: solved by LU."""
     G = J @ J.T + lam_eff2 * np.eye(J.shape[0])
+    if np.iscomplexobj(G):
+        return J.T @ np.linalg.solve(G, r)
     cf = sla.cho_factor(G)
     return J.T @ sla.cho_solve(cf, r)
 
 
 def gn_step_normal(J, r, lam_eff2):
-    """delta from the E x E normal equations by dense Cholesky (small E only)."""
+    """delta from the E x E normal equations (plain transposes) by dense Cholesky, or LU
+    when J is complex (small E only)."""
     A = J.T @ J + lam_eff2 * np.eye(J.shape[1])
+    if np.iscomplexobj(A):
+        return np.linalg.solve(A, J.T @ r)
     return sla.cho_solve(sla.cho_factor(A), J.T @ r)
 
 
@@ -908,7 +994,7 @@ def gn_step_cg_reference(J, r, lam_eff2, max_iter=0, eps=FLT_EPSILON):
 def reconstruct(oracle: EITOracle, U_meas, gamma0, lam, steps=3, quirks_jacobian=None):
     """``steps`` x Solver::solveAbsolute from gamma0; lam is regularizationFactor and the
     diagonal term is (referenceValue * lam)^2 (inverse.cpp:73-75)."""
-    g = np.asarray(gamma0, np.float64).copy()
+    g = np.asarray(gamma0, np.complex128 if isinstance(oracle, ComplexEITOracle) else np.float64).copy()
     lam2 = (oracle.ref * lam) ** 2
     um = oracle.flatten_voltages(U_meas)
     for _ in range(steps):

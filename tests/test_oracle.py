@@ -6,6 +6,7 @@ import math
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from mpflow_amd.mesh import boundary_edges, promote_p2, ring_mesh
@@ -857,3 +858,104 @@ def test_voltage_sources_pins():
         gm[e] -= eps
         fd = (o1.flatten_voltages(o1.solve(gp)[0]) - o1.flatten_voltages(o1.solve(gm)[0])) / (2 * eps)
         np.testing.assert_allclose(J[:, e], fd, rtol=1e-6, atol=1e-5 * np.abs(J[:, e]).max())
+
+
+# ---------------------------------------------------------------------------
+# complex conductivity (Equation<thrust::complex<float>>, src/models/eit.cpp:289-304)
+# ---------------------------------------------------------------------------
+
+def _complex_oracle(C=1, rings=6, L=8, w=0.3, p2=False, h=1.0, hp=1.0):
+    from oracle.eit_oracle import ComplexEITOracle
+    p = make_problem(rings, L, w, C=C, height=h, port_height=hp, p2=p2)
+    o = ComplexEITOracle(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"],
+                         p["current"], 1.0, C, h, hp)
+    return p, o
+
+
+def _complex_gamma(E, seed=5, imag=0.3):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.5, 2.0, E) + 1j * imag * rng.uniform(-1.0, 1.0, E)
+
+
+@pytest.mark.parametrize("C,h,hp", [(1, 1.0, 1.0), (3, 0.4, 0.1)])
+def test_complex_real_conductivity_equals_real_model(C, h, hp):
+    """A real gamma gives the real model's voltages and Jacobian (imaginary parts 0)."""
+    p, o = _complex_oracle(C, h=h, hp=hp)
+    r = oracle_of(p)
+    g = np.random.default_rng(2).uniform(0.5, 2.0, o.E)
+    U, ph, _ = o.solve(g)
+    U0, ph0, _ = r.solve(g)
+    np.testing.assert_allclose(U.real, U0, rtol=1e-12, atol=1e-15 * np.abs(U0).max())
+    assert np.abs(U.imag).max() < 1e-14 * np.abs(U0).max()
+    for q in (True, False):
+        J, J0 = o.jacobian(g, ph, q), r.jacobian(g, ph0, q)
+        np.testing.assert_allclose(J.real, J0, rtol=1e-11, atol=1e-14 * np.abs(J0).max())
+
+
+def test_complex_holomorphic_jacobian():
+    """U(gamma) is holomorphic: central differences along a real and along an imaginary
+    step of gamma_e both give J[:, e] (Cauchy-Riemann).  A Jacobian with conjugated fields
+    would match the real step only on real data and fail the imaginary one."""
+    p, o = _complex_oracle(C=2, h=0.5, hp=0.2)
+    g = _complex_gamma(o.E)
+    U, ph, _ = o.solve(g)
+    J = o.jacobian(g, ph, quirks=False)
+    eps = 1e-6
+    for e in (0, o.E // 3, o.E - 1):
+        for step in (eps, 1j * eps):
+            gp_, gm_ = g.copy(), g.copy()
+            gp_[e] += step
+            gm_[e] -= step
+            d = (o.flatten_voltages(o.solve(gp_)[0]) - o.flatten_voltages(o.solve(gm_)[0])) / (2 * step)
+            assert np.abs(d - J[:, e]).max() < 1e-6 * np.abs(J[:, e]).max()
+
+
+def test_complex_reciprocity():
+    """Adjacent drive = measurement patterns: U = U^T, not U^H (the system is complex
+    symmetric, not Hermitian)."""
+    p, o = _complex_oracle(C=1)
+    g = _complex_gamma(o.E, seed=7, imag=0.8)
+    U, ph, _ = o.solve(g)
+    np.testing.assert_allclose(U, U.T, rtol=0, atol=1e-10 * np.abs(U).max())
+    assert np.abs(U - U.conj().T).max() > 1e-3 * np.abs(U).max()
+
+
+def test_complex_pcg_hermitian_dots():
+    """The reference's complex CG conjugates the second dot operand.  On a Hermitian
+    positive definite matrix that is textbook CG: the exact solution within n steps (dots
+    without the conjugate do not converge there); on the EIT system with a small imaginary
+    part it reaches the direct solve."""
+    from oracle.eit_oracle import pcg_reference, solve_direct
+    rng = np.random.default_rng(3)
+    n = 12
+    Z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = Z @ Z.conj().T + n * np.eye(n)
+    b = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    x, s = pcg_reference(sp.csr_matrix(H), b, rule="tolerance", tol=1e-12, max_iter=3 * n)
+    np.testing.assert_allclose(x, np.linalg.solve(H, b), rtol=1e-9, atol=1e-11)
+    assert s <= n + 2
+    p, o = _complex_oracle(C=1)
+    g = _complex_gamma(o.E, imag=0.05)
+    A = o.system(g, 0)
+    F = (o.B @ o.Pi).astype(complex)
+    xc, _ = pcg_reference(A, F, 1.0 / A.diagonal(), True, rule="tolerance", tol=1e-10, max_iter=2000)
+    xd = solve_direct(A, F, True)
+    assert np.abs(xc - xd).max() < 1e-7 * np.abs(xd).max()
+
+
+def test_complex_gn_step_plain_transpose():
+    """Inverse::solve forms J^T J and J^T r with plain transposes (CUBLAS_OP_T): the data-
+    space step equals the E x E one, equals the real step for real data, and differs
+    from the Hermitian (J^H) least-squares step for complex J."""
+    from oracle.eit_oracle import gn_step_exact, gn_step_normal
+    rng = np.random.default_rng(4)
+    m, E = 20, 60
+    J = rng.standard_normal((m, E)) + 0.3j * rng.standard_normal((m, E))
+    r = rng.standard_normal(m) + 0.3j * rng.standard_normal(m)
+    d = gn_step_exact(J, r, 0.5)
+    np.testing.assert_allclose(d, gn_step_normal(J, r, 0.5), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(J.T @ J @ d + 0.5 * d, J.T @ r, rtol=1e-10, atol=1e-12)
+    dh = np.linalg.solve(J.conj().T @ J + 0.5 * np.eye(E), J.conj().T @ r)
+    assert np.abs(d - dh).max() > 1e-3 * np.abs(d).max()
+    Jr, rr = J.real, r.real
+    np.testing.assert_allclose(gn_step_exact(Jr + 0j, rr + 0j, 0.5).real, gn_step_exact(Jr, rr, 0.5), rtol=1e-10)
