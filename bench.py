@@ -80,6 +80,33 @@ def build_problem(w):
     return nodes, el, bd, ports, pat
 
 
+def autotune_report(ctx):
+    """The context's autotuner choices (mpf_ctx_autotune_read): SpMV kernel and work-list
+    order per (lane count, padded ELL width, row-count bucket), Gram tile per shape."""
+    import ctypes as C
+    from mpflow_amd import capi
+    L = capi.lib()
+    kernels = {0: "k_pcg_spmv4", 1: "k_pcg_spmv (32-bit gathers)", 2: "k_pcg_spmv4h (harmonics fused)"}
+    orders = {0: "(job, harmonic, tile)", 1: "(job, tile, harmonic)"}
+    out = {}
+    for which, name in ((0, "spmv"), (1, "gram")):
+        n = C.c_int32()
+        capi.check(L.mpf_ctx_autotune_read(ctx.h, which, None, None, 0, C.byref(n)))
+        keys = (C.c_int64 * max(n.value, 1))()
+        vals = (C.c_int32 * max(n.value, 1))()
+        capi.check(L.mpf_ctx_autotune_read(ctx.h, which, keys, vals, n.value, C.byref(n)))
+        rows = []
+        for k, v in zip(keys[:n.value], vals[:n.value]):
+            if which == 0:
+                rows.append({"lanes": 1 << (k >> 40), "ell_width": (k >> 32) & 0xFF, "rows_log2": k & 0xFFFFFFFF,
+                             "kernel": kernels.get(v & 15, str(v & 15)), "order": orders.get(v >> 4, str(v >> 4))})
+            else:
+                rows.append({"rows": k >> 44, "elements": (k >> 16) & 0xFFFFFFF, "frames": k & 0xFFFF,
+                             "tile": v})
+        out[name] = rows
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (from /proc/cpuinfo) and its logical CPU count."""
     name = "unknown"
@@ -233,13 +260,14 @@ def run_c5(args):
     from mpflow_amd.eit import EIT, Context, SolveOptions, Solver
     from mpflow_amd.learned import LearnedReconstruction, Mlp, VoltageFeatures, generate, train
     from mpflow_amd.phantom import ellipse_phantom, frame_seed
-    from oracle.eit_oracle import EITOracle, reconstruct
+    from oracle.eit_oracle import CoupledCEMOracle, EITOracle, reconstruct
     w = WORKLOADS[args.mesh]
     nodes, el, bd, ports, pat = build_problem(w)
     ctx = Context()
     B = max(1, args.batch)
+    z = args.contact_impedance
     mk = lambda frames: EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
-                            jacobian_exact=True, frames=frames, ctx=ctx)
+                            jacobian_exact=True, frames=frames, ctx=ctx, contact_impedance=z, cem=args.cem)
     phantom = lambda sd: ellipse_phantom(nodes, el, sd, n_max=4)
     log = lambda msg: print("c5: " + msg, file=sys.stderr, flush=True)
     two = capi.MPF_PRECOND_TWO_LEVEL
@@ -333,7 +361,12 @@ def run_c5(args):
     torch.cuda.synchronize()
     fps_ref = n_ref / (time.perf_counter() - t0)
     # FP64 oracle: one Gauss-Newton step from the same network start, first frames
-    o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
+    if z > 0.0 and args.cem == "coupled":
+        o = CoupledCEMOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                             contact_impedance=z)
+    else:
+        o = EITOracle(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
+                      contact_impedance=z)
     Xh = torch.tensor(feats(Uh[:2]), dtype=torch.float32, device="cuda")
     start = torch.exp(net.forward(Xh)).cpu().numpy().astype(np.float64)
     e2 = mk(2)
@@ -365,6 +398,9 @@ def run_c5(args):
             "workload": "config 5 (c5) on the %s mesh (%d elements, %d nodes), 32 electrodes, 5 harmonics; "
                         "network %s; %d training pairs, %d epochs; %d held-out frames, 1-4 ellipses, 60 dB"
                         % (args.mesh, e1.E, e1.N, net.dims, ntrain, args.epochs, nh),
+            "electrode_model": ("complete electrode model, z_c = %g, %s" % (z, "one potential per electrode for "
+                                "all harmonics" if args.cem == "coupled" else "harmonic-wise"))
+                               if z > 0.0 else "gap model",
             "frames_per_launch": B,
             "plain_fp32_gn_batched_fps": round(fps_plain, 3),
             "plain_fp32_gn_batched": "3 GN steps from gamma = 1, B frames per launch, FP32 two-level PCG to "
@@ -391,21 +427,45 @@ def run_c5(args):
     print(json.dumps(out), flush=True)
 
 
-def run_probe():
-    """Counter probe for rocprofv3 --pmc passes: 32 two-level PCG iterations of config 3 at
-    frame 0's phantom (all 5 harmonics active, so every launch covers 5 harmonic-SpMVs),
-    then a plain 256 MiB device-to-device copy that calibrates FETCH_SIZE / WRITE_SIZE."""
+def run_probe(args):
+    """Counter probe for rocprofv3 --pmc passes: 32 two-level PCG iterations of config 3
+    (the headline's electrode model) at frame 0's phantom (all 5 harmonics active, so every
+    launch covers 5 harmonic-SpMVs); the Gauss-Newton Gram of the config-3 shape for 32
+    frames (m = 1024, E = 40344) through the 256-tile BF16 kernel; one training step of the
+    config-5 network (batch 256, 1024 -> 3 x 2048 -> 9600); then a plain 256 MiB
+    device-to-device copy that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    import numpy as np
     import torch
     from mpflow_amd import capi
     from mpflow_amd.eit import EIT, SolveOptions
+    from mpflow_amd.learned import Mlp
     from mpflow_amd.phantom import ellipse_phantom, frame_seed
     w = WORKLOADS["c3"]
     nodes, el, bd, ports, pat = build_problem(w)
-    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"], jacobian_exact=True)
+    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"], jacobian_exact=True,
+              contact_impedance=args.contact_impedance, cem=args.cem)
     g = torch.tensor(ellipse_phantom(nodes, el, frame_seed(3, 0)), dtype=torch.float32)
     eit.solve(g, SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, max_iter=32, check_every=32,
                               preconditioner=capi.MPF_PRECOND_TWO_LEVEL))
     torch.cuda.synchronize()
+    print("probe: pcg done", flush=True)
+    L = capi.lib()
+    m, E = 1024, 40344
+    A = torch.randn(E, m, device="cuda", dtype=torch.float32) * 1e-3   # column-major m x E
+    G = torch.empty(m, m, device="cuda", dtype=torch.float32)
+    capi.check(L.mpf_gram_bf16(eit.ctx.h, 0, m, E, C.c_void_p(A.data_ptr()), m, C.c_void_p(G.data_ptr())))
+    torch.cuda.synchronize()
+    del A, G
+    print("probe: gram done", flush=True)
+    net = Mlp([1024, 2048, 2048, 2048, 9600], seed=1, ctx=eit.ctx)
+    X = torch.randn(256, 1024, device="cuda", dtype=torch.float32)
+    T = torch.randn(256, 9600, device="cuda", dtype=torch.float32)
+    net.forward(X)
+    net.backward(T)
+    net.adam(1e-3)
+    torch.cuda.synchronize()
+    print("probe: mlp done", flush=True)
     a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 256 MiB
     b = torch.empty_like(a)
     a.fill_(1.0)
@@ -461,7 +521,7 @@ def main():
     from mpflow_amd.phantom import add_noise, ellipse_phantom, frame_seed
 
     if args.probe:
-        run_probe()
+        run_probe(args)
         return
     if args.workload == "c4":
         run_c4(args)
@@ -608,6 +668,7 @@ def main():
         },
         "roofline": roofline,
     }
+    out["config"]["autotune"] = autotune_report(eit.ctx)
     if not args.no_cpu_baseline:
         print("bench: host baseline frames", file=sys.stderr, flush=True)
         cb = cpu_baseline(nodes, el, bd, ports, pat, w, meas[nbatches - 1].cpu().numpy(), args.gn_steps, args.lam,

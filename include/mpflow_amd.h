@@ -89,6 +89,15 @@ void* mpf_ctx_stream(mpf_ctx* ctx);
  * (8 nnz + 8 N R per harmonic per launch). */
 int mpf_ctx_profile(mpf_ctx* ctx, int32_t enable);
 int mpf_ctx_profile_read(mpf_ctx* ctx, double out[3]);
+/* The context's autotuner choices so far (no counterpart: the reference's launch
+ * geometry is fixed).  which = 0: SpMV, keys (lane-count log2 << 40 | padded ELL width
+ * << 32 | row-count bucket) and values (kernel variant | work-list order << 4: variant 0
+ * 128-bit gathers, 1 32-bit gathers, 2 harmonic-fused; order 0 (job, harmonic, tile),
+ * 1 (job, tile, harmonic)); which = 1: Gauss-Newton Gram, keys (rows << 44 | elements
+ * << 16 | frames) and values (tile edge 128 or 256).  Writes up to `cap` pairs and the
+ * number of entries to *count. */
+int mpf_ctx_autotune_read(mpf_ctx* ctx, int32_t which, int64_t* keys, int32_t* values, int32_t cap,
+                          int32_t* count);
 
 /* ---------------------------------------------------------------------------
  * Host helpers (no device work)
@@ -175,6 +184,18 @@ typedef struct mpf_model_desc {
                                      Schur complement on V.  Voltages: Mpat^T V.  Needs
                                      jacobian_exact = 1 (weights n_c); mpf_eit_measure and
                                      mpf_jacobian (per-harmonic seams) are refused. */
+    int32_t complex_gamma;        /* 1: complex conductivity (models::EIT over
+                                     Equation<thrust::complex<float>, ...>, eit.cpp:289-304;
+                                     ConjugateGradient<thrust::complex<float>>,
+                                     conjugate_gradient.cpp:231-232).  gamma is [frames][E]
+                                     complex (re, im float pairs); mpf_eit_solve writes U, U64
+                                     and J as (re, im) pairs of float / double; mpf_gn_step
+                                     takes and returns them the same way and solves the complex
+                                     symmetric J^T J of Inverse::solve (plain transposes,
+                                     inverse.cpp:148-152, 196-204).  CG with the diagonal
+                                     preconditioner, FP32 (opts.precision MPF_PREC_FP32),
+                                     current sources, gap model; the other model entry points
+                                     and batches refuse such a model. */
 } mpf_model_desc;
 
 typedef struct mpf_solve_opts {
@@ -207,6 +228,13 @@ typedef struct mpf_solve_opts {
                                      frame's conductivity; several times fewer iterations on
                                      large meshes.  Models above 512 tiles (131072 unknowns)
                                      fall back to Jacobi. */
+    int32_t complex_hermitian;    /* complex-conductivity models only.  0 (default): conjugate
+                                     orthogonal CG (dot products r^T z, p^T q without
+                                     conjugation), the Krylov method for complex symmetric
+                                     systems.  1: the reference's iteration, whose dot products
+                                     conjugate their second operand (vectorDotProductKernel,
+                                     matrix_kernel.cu:405-413); it stalls or diverges when the
+                                     conductivity's imaginary part is not small. */
 } mpf_solve_opts;
 
 typedef struct mpf_solve_stats {

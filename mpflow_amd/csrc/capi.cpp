@@ -9,6 +9,7 @@
 #include "fem.h"
 #include "numeric.h"
 #include "model.h"
+#include "complex.h"
 
 using mpf::Error;
 
@@ -36,7 +37,9 @@ void set_device(mpf_ctx* ctx) { MPF_HIP(hipSetDevice(ctx->device)); }
 }  // namespace
 
 struct mpf_model {
-    mpf::Model* impl;
+    mpf::Model* impl = nullptr;             // real conductivity
+    mpf::ComplexModel* cimpl = nullptr;     // complex conductivity (desc.complex_gamma)
+    mpf_ctx* ctx = nullptr;
 };
 
 struct mpf_batch {
@@ -136,6 +139,23 @@ void mpf_solve_opts_default(mpf_solve_opts* o) {
     o->check_every = 16;
 }
 
+int mpf_ctx_autotune_read(mpf_ctx* ctx, int32_t which, int64_t* keys, int32_t* values, int32_t cap,
+                          int32_t* count) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && count, "mpf_ctx_autotune_read: null argument");
+        MPF_REQUIRE(which == 0 || which == 1, "mpf_ctx_autotune_read: which must be 0 or 1");
+        std::lock_guard<std::mutex> lk(ctx->tune.mu);
+        const auto& mp = which == 0 ? ctx->tune.spmv : ctx->tune.gram;
+        int n = 0;
+        for (const auto& kv : mp) {
+            if (n < cap && keys && values) { keys[n] = kv.first; values[n] = kv.second; }
+            ++n;
+        }
+        *count = n;
+        return MPF_OK;
+    });
+}
+
 int mpf_ports_circular(int32_t n_nodes, const double* nodes, int32_t n_elements,
                        int32_t nodes_per_element, const int32_t* elements, int32_t n_boundary,
                        int32_t nodes_per_edge, const int32_t* boundary, int32_t count, double width,
@@ -171,8 +191,10 @@ int mpf_model_create(mpf_ctx* ctx, const mpf_model_desc* desc, mpf_model** out) 
         MPF_REQUIRE(ctx && desc && out, "mpf_model_create: null argument");
         set_device(ctx);
         auto* m = new mpf_model;
+        m->ctx = ctx;
         try {
-            m->impl = new mpf::Model(ctx, *desc);
+            if (desc->complex_gamma) m->cimpl = new mpf::ComplexModel(ctx, *desc);
+            else m->impl = new mpf::Model(ctx, *desc);
         } catch (...) {
             delete m;
             throw;
@@ -185,10 +207,9 @@ int mpf_model_create(mpf_ctx* ctx, const mpf_model_desc* desc, mpf_model** out) 
 int mpf_model_destroy(mpf_model* m) {
     return guarded([&] {
         if (!m) return MPF_OK;
-        if (m->impl) {
-            (void)hipStreamSynchronize(m->impl->ctx->stream);
-            delete m->impl;
-        }
+        if (m->impl || m->cimpl) (void)hipStreamSynchronize(m->ctx->stream);
+        delete m->impl;
+        delete m->cimpl;
         delete m;
         return MPF_OK;
     });
@@ -197,9 +218,10 @@ int mpf_model_destroy(mpf_model* m) {
 int mpf_model_dims(const mpf_model* m, int64_t dims[16]) {
     return guarded([&] {
         MPF_REQUIRE(m && dims, "mpf_model_dims: null argument");
-        const auto& S = m->impl->S;
+        const auto& S = m->impl ? m->impl->S : m->cimpl->setup();
         dims[0] = S.N; dims[1] = S.E; dims[2] = S.np; dims[3] = S.width; dims[4] = S.mrows;
-        dims[5] = S.Mp; dims[6] = S.Dp; dims[7] = S.R; dims[8] = S.Rp; dims[9] = m->impl->frames;
+        dims[5] = S.Mp; dims[6] = S.Dp; dims[7] = S.R; dims[8] = S.Rp;
+        dims[9] = m->impl ? m->impl->frames : m->cimpl->frames();
         dims[10] = S.C; dims[11] = S.nnz; dims[12] = S.Ru; dims[13] = S.wpad;
         dims[14] = 0; dims[15] = 0;
         return MPF_OK;
@@ -211,10 +233,18 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
     return guarded([&] {
         MPF_REQUIRE(m && d_gamma, "mpf_eit_solve: null argument");
         auto t0 = std::chrono::steady_clock::now();
-        set_device(m->impl->ctx);
+        set_device(m->ctx);
         mpf_solve_opts o;
         if (opts) o = *opts; else mpf_solve_opts_default(&o);
         if (stats) std::memset(stats, 0, sizeof(*stats));
+        if (m->cimpl) {
+            // complex conductivity: gamma, U, U64 and J hold (re, im) pairs
+            int rc = m->cimpl->forward(o, reinterpret_cast<const float2*>(d_gamma), reinterpret_cast<float2*>(d_U),
+                                       reinterpret_cast<mpf::zd*>(d_U64), reinterpret_cast<float2*>(d_J), stats);
+            if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (rc == MPF_ENOCONV) g_last_error = "mpf_eit_solve: a Krylov solve hit its iteration cap";
+            return rc;
+        }
         auto& M = *m->impl;
         MPF_HIP(hipMemcpyAsync(M.gamma.p, d_gamma, sizeof(float) * (size_t)M.S.E * M.frames,
                                hipMemcpyDeviceToDevice, M.ctx->stream));
@@ -231,6 +261,7 @@ int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_r
         MPF_REQUIRE(m, "mpf_gn_set_regularization: null model");
         MPF_REQUIRE(type == MPF_REG_IDENTITY || type == MPF_REG_NOSER || type == MPF_REG_LAPLACIAN,
                     "mpf_gn_set_regularization: unknown regularisation type");
+        MPF_REQUIRE(m->impl, "mpf_gn_set_regularization: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         mpf_solve_opts o;
         if (opts) o = *opts; else mpf_solve_opts_default(&o);
@@ -244,6 +275,7 @@ int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, cons
                      mpf_solve_stats* stats) {
     return guarded([&] {
         MPF_REQUIRE(m && d_gamma_ref, "mpf_diff_prepare: null argument");
+        MPF_REQUIRE(m->impl, "mpf_diff_prepare: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         mpf_solve_opts o;
         if (opts) o = *opts; else mpf_solve_opts_default(&o);
@@ -256,6 +288,7 @@ int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, cons
 int mpf_set_sensitivity_mask(mpf_model* m, const float* d_mask) {
     return guarded([&] {
         MPF_REQUIRE(m, "mpf_set_sensitivity_mask: null model");
+        MPF_REQUIRE(m->impl, "mpf_set_sensitivity_mask: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         m->impl->set_sensitivity_mask(d_mask);
         return MPF_OK;
@@ -266,6 +299,7 @@ int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float
     return guarded([&] {
         MPF_REQUIRE(m && d_Umeas && d_gamma, "mpf_diff_solve: null argument");
         MPF_REQUIRE(gn_refine >= 0, "mpf_diff_solve: gn_refine < 0");
+        MPF_REQUIRE(m->impl, "mpf_diff_solve: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         m->impl->diff_solve(d_Umeas, gn_refine, d_gamma);
         return MPF_OK;
@@ -275,6 +309,7 @@ int mpf_diff_solve(mpf_model* m, const double* d_Umeas, int32_t gn_refine, float
 int mpf_fem_assemble(mpf_model* m, const float* d_gamma, double alpha, uint32_t* d_column_ids, float* d_values) {
     return guarded([&] {
         MPF_REQUIRE(m && d_gamma && d_values, "mpf_fem_assemble: null argument");
+        MPF_REQUIRE(m->impl, "mpf_fem_assemble: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         mpf::fem_assemble(*m->impl, d_gamma, alpha, d_column_ids, d_values);
         MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
@@ -285,6 +320,7 @@ int mpf_fem_assemble(mpf_model* m, const float* d_gamma, double alpha, uint32_t*
 int mpf_eit_measure(mpf_model* m, const float* d_phi, int32_t ldphi, int32_t additive, float* d_U) {
     return guarded([&] {
         MPF_REQUIRE(m && d_phi && d_U, "mpf_eit_measure: null argument");
+        MPF_REQUIRE(m->impl, "mpf_eit_measure: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         mpf::fem_measure(*m->impl, d_phi, ldphi, additive, d_U);
         MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
@@ -296,6 +332,7 @@ int mpf_jacobian(mpf_model* m, const float* d_phi, int32_t ldphi, const float* d
                  float* d_J) {
     return guarded([&] {
         MPF_REQUIRE(m && d_phi && d_gamma && d_J, "mpf_jacobian: null argument");
+        MPF_REQUIRE(m->impl, "mpf_jacobian: not available for a complex-conductivity model");
         set_device(m->impl->ctx);
         mpf::fem_jacobian(*m->impl, d_phi, ldphi, d_gamma, additive, d_J);
         MPF_HIP(hipStreamSynchronize(m->impl->ctx->stream));
@@ -308,11 +345,18 @@ int mpf_gn_step(mpf_model* m, float* d_gamma, const double* d_Umeas, double lamb
     return guarded([&] {
         MPF_REQUIRE(m && d_gamma && d_Umeas, "mpf_gn_step: null argument");
         auto t0 = std::chrono::steady_clock::now();
-        set_device(m->impl->ctx);
+        set_device(m->ctx);
         mpf_solve_opts o;
         if (opts) o = *opts; else mpf_solve_opts_default(&o);
         if (stats) std::memset(stats, 0, sizeof(*stats));
-        int rc = m->impl->gn_step(d_gamma, d_Umeas, lambda, gn_refine, o, d_Ucalc, stats);
+        int rc;
+        if (m->cimpl)
+            // complex conductivity: gamma, U_meas and U_calc hold (re, im) pairs; the
+            // normal equations are formed in FP64, so gn_refine has nothing to refine
+            rc = m->cimpl->gn_step(reinterpret_cast<float2*>(d_gamma), reinterpret_cast<const mpf::zd*>(d_Umeas),
+                                   lambda, o, reinterpret_cast<mpf::zd*>(d_Ucalc), stats);
+        else
+            rc = m->impl->gn_step(d_gamma, d_Umeas, lambda, gn_refine, o, d_Ucalc, stats);
         if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return rc;
     });
@@ -324,7 +368,7 @@ int mpf_batch_create(mpf_ctx* ctx, mpf_model* const* models, int32_t n_models, m
         set_device(ctx);
         std::vector<mpf::Model*> ms;
         for (int i = 0; i < n_models; ++i) {
-            MPF_REQUIRE(models[i] && models[i]->impl, "mpf_batch_create: null model");
+            MPF_REQUIRE(models[i] && models[i]->impl, "mpf_batch_create: null or complex-conductivity model");
             ms.push_back(models[i]->impl);
         }
         auto* b = new mpf_batch;

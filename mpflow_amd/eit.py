@@ -71,7 +71,7 @@ def circular_ports(nodes, elements, boundary, count, width, offset=0.0, clockwis
 class SolveOptions:
     def __init__(self, stop_rule=capi.MPF_STOP_REFERENCE, tol=1e-6, max_iter=0, precision=capi.MPF_PREC_FP32,
                  refine_steps=2, refine_tol=1e-3, krylov=capi.MPF_KRYLOV_CG, check_every=16, warm_start=0,
-                 bicgstab_ignore_dc=0, graphs=0, preconditioner=capi.MPF_PRECOND_JACOBI):
+                 bicgstab_ignore_dc=0, graphs=0, preconditioner=capi.MPF_PRECOND_JACOBI, complex_hermitian=0):
         self.s = capi.SolveOpts()
         lib().mpf_solve_opts_default(C.byref(self.s))
         self.s.stop_rule, self.s.tol, self.s.max_iter = stop_rule, tol, max_iter
@@ -81,6 +81,7 @@ class SolveOptions:
         self.s.bicgstab_ignore_dc = int(bicgstab_ignore_dc)
         self.s.graphs = int(graphs)
         self.s.preconditioner = int(preconditioner)
+        self.s.complex_hermitian = int(complex_hermitian)
 
     def copy(self, **kw):
         o = SolveOptions()
@@ -99,7 +100,8 @@ class EIT:
     def __init__(self, nodes, elements, boundary, ports, drive_pattern, meas_pattern, current,
                  reference_value=1.0, components=1, height=1.0, port_height=1.0, logarithmic=False,
                  quirks=True, jacobian_exact=False, frames=1, ell_max_width=0, ctx: Context | None = None,
-                 keep_node_order=False, contact_impedance=0.0, source="current", cem="harmonic"):
+                 keep_node_order=False, contact_impedance=0.0, source="current", cem="harmonic",
+                 complex_gamma=False):
         import torch
         self.ctx = ctx or Context()
         self.torch = torch
@@ -140,6 +142,10 @@ class EIT:
             raise ValueError("source must be 'current' or 'voltage'")
         d.source_type = capi.MPF_SOURCE_VOLTAGE if source == "voltage" else capi.MPF_SOURCE_CURRENT
         self.source = source
+        # complex conductivity (Equation<thrust::complex<float>>): complex gamma, voltages
+        # and Jacobian (torch complex64 / complex128 tensors)
+        d.complex_gamma = int(bool(complex_gamma))
+        self.complex = bool(complex_gamma)
         h = C.c_void_p()
         check(lib().mpf_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h
@@ -151,8 +157,10 @@ class EIT:
         self.reference_value = float(reference_value)
         self.logarithmic = bool(logarithmic)
         dev = torch.device("cuda", self.ctx.device)
-        self.result = torch.zeros(self.frames, self.m, dtype=torch.float32, device=dev)
-        self.result64 = torch.zeros(self.frames, self.m, dtype=torch.float64, device=dev)
+        self.dtype = torch.complex64 if self.complex else torch.float32
+        self.dtype64 = torch.complex128 if self.complex else torch.float64
+        self.result = torch.zeros(self.frames, self.m, dtype=self.dtype, device=dev)
+        self.result64 = torch.zeros(self.frames, self.m, dtype=self.dtype64, device=dev)
         self.jacobian = None
         self.last_stats = None
 
@@ -161,13 +169,13 @@ class EIT:
 
     def solve(self, gamma, options: SolveOptions | None = None, jacobian=False):
         """EIT::solve: voltages (frames x m, column-major M x D with ld ceil16(M)).
-        ``gamma``: (frames, E) or (E,) float32 cuda tensor."""
+        ``gamma``: (frames, E) or (E,) float32 cuda tensor (complex64 for a complex model)."""
         torch = self.torch
-        g = gamma.reshape(self.frames, self.E).to(self._dev(), torch.float32).contiguous()
+        g = gamma.reshape(self.frames, self.E).to(self._dev(), self.dtype).contiguous()
         Jp = None
         if jacobian:
             if self.jacobian is None:
-                self.jacobian = torch.empty(self.frames, self.E, self.m, dtype=torch.float32, device=self._dev())
+                self.jacobian = torch.empty(self.frames, self.E, self.m, dtype=self.dtype, device=self._dev())
             Jp = C.c_void_p(self.jacobian.data_ptr())
         st = capi.SolveStats()
         opts = (options or SolveOptions()).s
@@ -259,10 +267,9 @@ class Solver:
         self.options = options or SolveOptions()
         # the reference distribution: gamma = 1 (linear) or 0 (logarithmic), solver.cpp:55-62
         init = 0.0 if eit.logarithmic else 1.0
-        self.material_distribution = torch.full((eit.frames, eit.E), init, dtype=torch.float32,
-                                                device=eit._dev())
-        self.measurement = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
-        self.calculation = torch.zeros(eit.frames, eit.m, dtype=torch.float64, device=eit._dev())
+        self.material_distribution = torch.full((eit.frames, eit.E), init, dtype=eit.dtype, device=eit._dev())
+        self.measurement = torch.zeros(eit.frames, eit.m, dtype=eit.dtype64, device=eit._dev())
+        self.calculation = torch.zeros(eit.frames, eit.m, dtype=eit.dtype64, device=eit._dev())
         self.last_stats = None
 
     def set_regularization(self, regularization_type: str = "identity", gamma_ref=None):

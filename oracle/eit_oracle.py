@@ -310,7 +310,7 @@ def ceil16(n):
 # ---------------------------------------------------------------------------
 
 def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
-                  rule="reference", tol=1e-6, prec=None, x0=None, fp32=False):
+                  rule="reference", tol=1e-6, prec=None, x0=None, fp32=False, bilinear=False):
     """Multi-RHS Jacobi-PCG as ``ConjugateGradient::solve`` runs it, in float64.
 
     * x starts at ``x0`` (default 0) with r = b - A x0 (conjugate_gradient.cpp:80-82);
@@ -329,6 +329,10 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
       conjugate_gradient.cpp:231-232): every dot product conjugates its second operand
       (vectorDotProductKernel, matrix_kernel.cu:405-413): rho = sum r conj(z),
       p.q = sum p conj(q), b.b = sum b conj(b); the stop rule takes |sqrt(rho / b.b)|.
+      ``bilinear``: the product's alternative for complex symmetric systems, conjugate
+      orthogonal CG (COCG): rho = r^T z and p.q = p^T q without conjugation (b.b and the
+      tolerance rule keep the Hermitian norms).  Identical to the reference iteration
+      for real data.
     * ``fp32``: the reference's own arithmetic -- matrix, right-hand side, vectors and the
       Jacobi inverse rounded to float32; A v summed row by row in float32 over the row's
       entries in stored order, each product rounded before it is added (the product's
@@ -383,20 +387,21 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
         r = b - A @ x
         if dc_free:
             r = r + x.sum(axis=0, keepdims=True)
+    cj = (lambda v: v) if bilinear else np.conj
     z = precond(r)
     p = z.copy()
-    rho_old = np.sum(r * np.conj(z), axis=0)
+    rho_old = np.sum(r * cj(z), axis=0)
     ref = np.real(np.sum(b * np.conj(b), axis=0))
     for step in range(iters):
         q = op(p)
-        pq = np.sum(p * np.conj(q), axis=0)
+        pq = np.sum(p * cj(q), axis=0)
         with np.errstate(divide="ignore", invalid="ignore"):
             a = np.where(pq != 0.0, rho_old / np.where(pq != 0, pq, 1.0), 0.0)
         a = rd(a)
         r = rd(np.where(pq != 0.0, r - a * q, 0.0))
         x = rd(np.where(pq != 0.0, x + a * p, 0.0))
         z = precond(r)
-        rho = np.sum(r * np.conj(z), axis=0)
+        rho = np.sum(r * cj(z), axis=0)
         if rule == "reference":
             with np.errstate(divide="ignore", invalid="ignore"):
                 crit = abs(math.sqrt(abs(rho[0]) / ref[0])) if ref[0] != 0 else float("nan")

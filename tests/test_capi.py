@@ -71,3 +71,35 @@ def test_context_without_device_fails_cleanly():
     rc = L.mpf_ctx_create(0, None, C.byref(h))
     assert rc in (capi.MPF_EHIP, capi.MPF_EINVAL)
     assert L.mpf_last_error().decode() != ""
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes structures of mpflow_amd/capi.py have the header's field offsets and
+    sizes: a C program compiled against include/mpflow_amd.h prints offsetof / sizeof."""
+    import ctypes
+    import os
+    import shutil
+    import subprocess
+    from mpflow_amd import capi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    structs = {"mpf_model_desc": capi.ModelDesc, "mpf_solve_opts": capi.SolveOpts,
+               "mpf_solve_stats": capi.SolveStats}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "mpflow_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)

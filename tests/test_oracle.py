@@ -943,6 +943,36 @@ def test_complex_pcg_hermitian_dots():
     assert np.abs(xc - xd).max() < 1e-7 * np.abs(xd).max()
 
 
+def test_complex_cocg_bilinear_dots():
+    """Conjugate orthogonal CG (``bilinear``): on a complex symmetric, non-Hermitian
+    matrix the exact solution within n steps, where the conjugated iteration does not
+    converge; on real data both are the reference's CG; on the EIT system with a large
+    imaginary part it reaches the direct solve (the dcFree harmonic included)."""
+    from oracle.eit_oracle import pcg_reference, solve_direct
+    rng = np.random.default_rng(8)
+    n = 12
+    Bm = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = Bm @ Bm.T + n * np.eye(n)
+    assert np.abs(A - A.T).max() == 0 and np.abs(A - A.conj().T).max() > 1.0
+    b = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    x, s = pcg_reference(sp.csr_matrix(A), b, rule="tolerance", tol=1e-12, max_iter=3 * n, bilinear=True)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-10)
+    assert s <= n + 4
+    xh, _ = pcg_reference(sp.csr_matrix(A), b, rule="tolerance", tol=1e-12, max_iter=3 * n)
+    assert np.abs(xh - np.linalg.solve(A, b)).max() > 1e-4
+    Ar = sp.csr_matrix(np.real(Bm @ Bm.T) + n * np.eye(n))
+    xr0, s0 = pcg_reference(Ar, np.real(b), rule="tolerance", tol=1e-10)
+    xr1, s1 = pcg_reference(Ar, np.real(b), rule="tolerance", tol=1e-10, bilinear=True)
+    assert s0 == s1 and np.array_equal(xr0, xr1)
+    p, o = _complex_oracle(C=1)
+    g = _complex_gamma(o.E, imag=1.0)
+    A = o.system(g, 0)
+    F = (o.B @ o.Pi).astype(complex)
+    xc, _ = pcg_reference(A, F, 1.0 / A.diagonal(), True, rule="tolerance", tol=1e-10, max_iter=2000, bilinear=True)
+    xd = solve_direct(A, F, True)
+    assert np.abs(xc - xd).max() < 1e-7 * np.abs(xd).max()
+
+
 def test_complex_gn_step_plain_transpose():
     """Inverse::solve forms J^T J and J^T r with plain transposes (CUBLAS_OP_T): the data-
     space step equals the E x E one, equals the real step for real data, and differs
