@@ -491,7 +491,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4,
                     help="FP32 Krylov tolerance; one FP64 refinement to 1e-2 of it follows (final "
                          "relative residual ~1e-2 tol)")
-    ap.add_argument("--batch", type=int, default=32, help="frames solved together per launch")
+    ap.add_argument("--batch", type=int, default=64, help="frames solved together per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 oracle reconstruction of one frame (about a minute on config 3)")

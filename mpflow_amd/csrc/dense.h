@@ -23,6 +23,9 @@ struct JacDev {
     double ref;
     float alpha[kMaxSlices];
     float weight[kMaxSlices];
+    // nullable: elements in a locality order (neighbouring elements share nodes); XCD x
+    // processes the x-th contiguous eighth of it, so node fields are re-read from its L2
+    const int32_t* order;
 };
 
 void launch_voltages(const JobDev* d_jobs, int njobs, const MeasDev& md, int use64, float* U32,

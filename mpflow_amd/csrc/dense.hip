@@ -41,65 +41,111 @@ __global__ void __launch_bounds__(256) k_voltages(const JobDev* __restrict__ job
 // src/models/eit.cpp:180-182, 202-205):
 //   J[d*Mp + m, e] = -sigma'_e sum_c w_c Phi_c[el, d]^T Kc_e Phi_c[el, D+m]
 //   reference: Kc = K_e, w_c = height;   exact: Kc = K_e + alpha_c M_e, w_c = 1/beta_c
-// One block per (element, job); the column J[:, e] is written once, contiguously.
+// One wavefront per (element, job), four per block, each independent of the others: the
+// work of an element is a chain of dependent steps (node ids, gathered fields, K_e Phi,
+// the column), so the kernel is latency-bound and wants as many chains in flight per CU
+// as possible; the wavefronts synchronise only among their own lanes.  The column J[:, e]
+// is written once, contiguously (and its BF16 copy for the Gram when asked).
 // --------------------------------------------------------------------------------
 constexpr int kJacMaxRows = 4096;   // Dp * Mp <= 64 * 64
+constexpr int kJacWaves = 4;        // elements per 256-thread block
 
+// LDS writes of a wavefront made visible to its own lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// PER: column rows per lane (mrows <= 64 PER); FIXEDM: Mp divides 64, so a lane's
+// measurement column is the same for all of its rows
+template <int PER, bool FIXEDM>
 __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ jobs, JacDev jd,
                                                   int use64, float* __restrict__ Jout,
                                                   __bf16* __restrict__ Jbf) {
-    const int e = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int E = jd.E;
+    int pos = blockIdx.x * kJacWaves + wave;
+    if (jd.order) {
+        // block b runs on XCD b % 8: give XCD x the x-th eighth of the locality order
+        const int nb = (E + kJacWaves - 1) / kJacWaves;
+        const int base = nb / 8, extra = nb % 8, x = blockIdx.x % 8;
+        pos = (x * base + min(x, extra) + blockIdx.x / 8) * kJacWaves + wave;
+    }
+    if (pos >= E) return;   // the whole wavefront leaves; no block barrier follows
+    const int e = jd.order ? jd.order[pos] : pos;
     const int j = blockIdx.y;
     const JobDev& Jb = jobs[j];
     const int k = jd.k, D = jd.D, M = jd.M, Mp = jd.Mp, Dp = jd.Dp, R = D + M;
     const int mrows = Mp * Dp;
-    __shared__ float phi[6][64];
-    __shared__ float T[6][64];
-    __shared__ float Kc[36];
-    __shared__ int nodes[6];
-    __shared__ int cmap[64];
-    __shared__ double cscale[64];
-    if (threadIdx.x < k) nodes[threadIdx.x] = jd.elements[(size_t)e * k + threadIdx.x];
-    // the column map once per block: the field loads then depend on LDS only
-    if (threadIdx.x < R) {
-        cmap[threadIdx.x] = jd.colmap[threadIdx.x];
-        cscale[threadIdx.x] = jd.colscale[threadIdx.x];
+    __shared__ float phi_s[kJacWaves][6][64];
+    __shared__ float T_s[kJacWaves][6][64];
+    __shared__ float Kc_s[kJacWaves][36];
+    __shared__ int nodes_s[kJacWaves][6];
+    __shared__ int cmap_s[kJacWaves][64];
+    __shared__ double cscale_s[kJacWaves][64];
+    float (*phi)[64] = phi_s[wave];
+    float (*T)[64] = T_s[wave];
+    float* Kc = Kc_s[wave];
+    int* nodes = nodes_s[wave];
+    if (lane < k) nodes[lane] = jd.elements[(size_t)e * k + lane];
+    // the column map in LDS: the field loads then depend on LDS only
+    if (lane < R) {
+        cmap_s[wave][lane] = jd.colmap[lane];
+        cscale_s[wave][lane] = jd.colscale[lane];
     }
-    float acc[kJacMaxRows / 256];
+    float acc[PER];
 #pragma unroll
-    for (int q = 0; q < kJacMaxRows / 256; ++q) acc[q] = 0.0f;
-    const int per = (mrows + 255) / 256;
-    __syncthreads();
+    for (int q = 0; q < PER; ++q) acc[q] = 0.0f;
+    const int m0 = lane % Mp, d0 = lane / Mp, dstep = 64 / Mp;
+    wave_sync();
     for (int s = 0; s < Jb.C; ++s) {
         const float al = jd.exact ? jd.alpha[s] : 0.0f;
         const float w = jd.weight[s];
-        for (int i = threadIdx.x; i < k * k; i += blockDim.x)
+        for (int i = lane; i < k * k; i += 64)
             Kc[i] = jd.ek[(size_t)e * k * k + i] + al * jd.em[(size_t)e * k * k + i];
-        for (int i = threadIdx.x; i < k * R; i += blockDim.x) {
+        for (int i = lane; i < k * R; i += 64) {
             const int a = i / R, c = i % R;
-            const size_t vi = vec_index(Jb, s, nodes[a], cmap[c]);
-            const double sc = cscale[c];
+            const size_t vi = vec_index(Jb, s, nodes[a], cmap_s[wave][c]);
+            const double sc = cscale_s[wave][c];
             phi[a][c] = use64 ? (float)(sc * Jb.x64[vi]) : (float)(sc * (double)Jb.x[vi]);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < k * M; i += blockDim.x) {
+        wave_sync();
+        for (int i = lane; i < k * M; i += 64) {
             const int a = i / M, m = i % M;
             float t = 0.0f;
             for (int b = 0; b < k; ++b) t += Kc[a * k + b] * phi[b][D + m];
             T[a][m] = w * t;
         }
-        __syncthreads();
-        for (int q = 0; q < per; ++q) {
-            const int idx = threadIdx.x + 256 * q;
-            if (idx >= mrows) break;
-            const int d = idx / Mp, m = idx % Mp;
-            if (d < D && m < M) {
-                float v = 0.0f;
-                for (int a = 0; a < k; ++a) v += phi[a][d] * T[a][m];
-                acc[q] += v;
+        wave_sync();
+        if constexpr (FIXEDM) {
+            // Mp divides 64: the lane's measurement column m is the same for every q, so
+            // its k entries of T stay in registers and d = d0 + q (64 / Mp)
+            float tm[6];
+            for (int a = 0; a < 6; ++a) tm[a] = (a < k && m0 < M) ? T[a][m0] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int d = d0 + q * dstep;
+                if (d < D && m0 < M && lane + 64 * q < mrows) {
+                    float v = 0.0f;
+                    for (int a = 0; a < k; ++a) v += phi[a][d] * tm[a];
+                    acc[q] += v;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int idx = lane + 64 * q;
+                const int d = idx / Mp, m = idx % Mp;
+                if (idx < mrows && d < D && m < M) {
+                    float v = 0.0f;
+                    for (int a = 0; a < k; ++a) v += phi[a][d] * T[a][m];
+                    acc[q] += v;
+                }
             }
         }
-        __syncthreads();
+        // every lane is done with this harmonic's phi and T before they are overwritten
+        wave_sync();
     }
     const float g = jd.gamma[(size_t)j * jd.E + e];
     const double sd = jd.logarithmic ? (2.302585092994046 / 10.0) * jd.ref * exp(2.302585092994046 * (double)g / 10.0)
@@ -107,13 +153,15 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
     const float scale = (float)(-sd);
     float* col = Jout + ((size_t)j * jd.E + e) * mrows;
     __bf16* colb = Jbf ? Jbf + ((size_t)j * jd.E + e) * mrows : nullptr;
-    for (int q = 0; q < per; ++q) {
-        const int idx = threadIdx.x + 256 * q;
-        if (idx >= mrows) break;
-        const float v = scale * acc[q];
-        col[idx] = v;
-        // the Gram's BF16 operand, rounded once here instead of at every panel read
-        if (colb) colb[idx] = (__bf16)v;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < mrows) {
+            const float v = scale * acc[q];
+            col[idx] = v;
+            // the Gram's BF16 operand, rounded once here instead of at every panel read
+            if (colb) colb[idx] = (__bf16)v;
+        }
     }
 }
 
@@ -199,6 +247,15 @@ __device__ __forceinline__ int2 upper_tile(int b, int nt) {
     return make_int2(i, i + b);
 }
 
+// LDS panel layout of the MFMA Gram kernels: rows of `ld` BF16 (16-byte aligned) with 16
+// more bytes after every 8 rows.  The staging writes of a wavefront go to rows 4 or 8
+// apart, which a plain row stride puts on the same LDS banks (every 16-byte-aligned
+// stride is a multiple of 4 dwords, so rows 8 apart are 32 dwords apart: one bank); the
+// extra 16 bytes per 8 rows spread them over 8 bank groups.  The MFMA operand reads (8
+// consecutive rows per pass) stay conflict-free.
+__device__ __forceinline__ int panel_off(int r, int ld) { return r * ld + (r >> 3) * 8; }
+constexpr int panel_elems(int rows, int ld) { return rows * ld + (rows / 8) * 8; }
+
 __global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, size_t strideA, int nrow, int nk,
                                                  int lda, int nt, int kslab, float* __restrict__ part) {
     const int ntiles = nt * (nt + 1) / 2;
@@ -206,8 +263,8 @@ __global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, si
     const int I0 = tile.x * kGT2, J0 = tile.y * kGT2;
     const int k0 = blockIdx.z * kslab, k1 = min(nk, k0 + kslab);
     const float* Ab = A + blockIdx.y * strideA;
-    __shared__ __attribute__((aligned(16))) __bf16 sa[2][kGT2][kGL2];
-    __shared__ __attribute__((aligned(16))) __bf16 sb[2][kGT2][kGL2];
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][panel_elems(kGT2, kGL2)];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][panel_elems(kGT2, kGL2)];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
     f32x16_t acc[2][2];
@@ -243,14 +300,14 @@ __global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, si
         bf16x4_t b1 = {(__bf16)rb[0].y, (__bf16)rb[1].y, (__bf16)rb[2].y, (__bf16)rb[3].y};
         bf16x4_t b2 = {(__bf16)rb[0].z, (__bf16)rb[1].z, (__bf16)rb[2].z, (__bf16)rb[3].z};
         bf16x4_t b3 = {(__bf16)rb[0].w, (__bf16)rb[1].w, (__bf16)rb[2].w, (__bf16)rb[3].w};
-        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq][kq4]) = a0;
-        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 1][kq4]) = a1;
-        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 2][kq4]) = a2;
-        *reinterpret_cast<bf16x4_t*>(&sa[buf][rq + 3][kq4]) = a3;
-        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq][kq4]) = b0;
-        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 1][kq4]) = b1;
-        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 2][kq4]) = b2;
-        *reinterpret_cast<bf16x4_t*>(&sb[buf][rq + 3][kq4]) = b3;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][panel_off(rq, kGL2) + kq4]) = a0;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][panel_off(rq + 1, kGL2) + kq4]) = a1;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][panel_off(rq + 2, kGL2) + kq4]) = a2;
+        *reinterpret_cast<bf16x4_t*>(&sa[buf][panel_off(rq + 3, kGL2) + kq4]) = a3;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][panel_off(rq, kGL2) + kq4]) = b0;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][panel_off(rq + 1, kGL2) + kq4]) = b1;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][panel_off(rq + 2, kGL2) + kq4]) = b2;
+        *reinterpret_cast<bf16x4_t*>(&sb[buf][panel_off(rq + 3, kGL2) + kq4]) = b3;
     };
     const int nst = (k1 - k0 + kGK2 - 1) / kGK2;
     if (nst > 0) {
@@ -267,8 +324,8 @@ __global__ void __launch_bounds__(256) k_gram_nt(const float* __restrict__ A, si
             bf16x8_t av[2], bv[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][wi + 32 * a + (lane & 31)][kq]);
-                bv[a] = *reinterpret_cast<const bf16x8_t*>(&sb[buf][wj + 32 * a + (lane & 31)][kq]);
+                av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][panel_off(wi + 32 * a + (lane & 31), kGL2) + kq]);
+                bv[a] = *reinterpret_cast<const bf16x8_t*>(&sb[buf][panel_off(wj + 32 * a + (lane & 31), kGL2) + kq]);
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
@@ -312,8 +369,8 @@ __global__ void __launch_bounds__(512) k_gram_bf16(const __bf16* __restrict__ A,
     const bool diag = tile.x == tile.y;
     const int k0 = blockIdx.z * kslab, k1 = min(nk, k0 + kslab);
     const __bf16* Ab = A + blockIdx.y * strideA;
-    __shared__ __attribute__((aligned(16))) __bf16 sa[2][kGT3][kGL3];
-    __shared__ __attribute__((aligned(16))) __bf16 sb[2][kGT3][kGL3];
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][panel_elems(kGT3, kGL3)];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][panel_elems(kGT3, kGL3)];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = (wave >> 2) * 128, wj = (wave & 3) * 64;
     f32x16_t acc[4][2];
@@ -345,11 +402,11 @@ __global__ void __launch_bounds__(512) k_gram_bf16(const __bf16* __restrict__ A,
     };
     auto store = [&](int buf) {
         if (!active) return;
-        __bf16 (*dst)[kGL3] = half ? sb[buf] : sa[buf];
+        __bf16* dst = half ? sb[buf] : sa[buf];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             bf16x4_t v = {r[0][q], r[1][q], r[2][q], r[3][q]};
-            *reinterpret_cast<bf16x4_t*>(&dst[ro + q][kq]) = v;
+            *reinterpret_cast<bf16x4_t*>(&dst[panel_off(ro + q, kGL3) + kq]) = v;
         }
     };
     const int nst = (k1 - k0 + kGK3 - 1) / kGK3;
@@ -361,15 +418,17 @@ __global__ void __launch_bounds__(512) k_gram_bf16(const __bf16* __restrict__ A,
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
         if (st + 1 < nst) load(k0 + (st + 1) * kGK3);
-        const __bf16 (*pb)[kGL3] = diag ? sa[buf] : sb[buf];
+        const __bf16* pb = diag ? sa[buf] : sb[buf];
 #pragma unroll
         for (int ks = 0; ks < kGK3; ks += 16) {
             const int kk = ks + (lane >> 5) * 8;
             bf16x8_t av[4], bv[2];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][wi + 32 * a + (lane & 31)][kk]);
+            for (int a = 0; a < 4; ++a)
+                av[a] = *reinterpret_cast<const bf16x8_t*>(&sa[buf][panel_off(wi + 32 * a + (lane & 31), kGL3) + kk]);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const bf16x8_t*>(&pb[wj + 32 * b + (lane & 31)][kk]);
+            for (int b = 0; b < 2; ++b)
+                bv[b] = *reinterpret_cast<const bf16x8_t*>(&pb[panel_off(wj + 32 * b + (lane & 31), kGL3) + kk]);
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -822,8 +881,20 @@ void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use6
                      hipStream_t st, uint16_t* Jb) {
     MPF_REQUIRE(jd.Mp * jd.Dp <= kJacMaxRows, "jacobian: too many rows");
     MPF_REQUIRE(jd.D + jd.M <= 64 && jd.k <= 6, "jacobian: unsupported shape");
-    dim3 grid(jd.E, njobs);
-    k_jacobian<<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, reinterpret_cast<__bf16*>(Jb));
+    dim3 grid((jd.E + kJacWaves - 1) / kJacWaves, njobs);
+    const int mrows = jd.Mp * jd.Dp;
+    __bf16* jb = reinterpret_cast<__bf16*>(Jb);
+    const bool fixedm = 64 % jd.Mp == 0;
+    if (mrows <= 256) {
+        if (fixedm) k_jacobian<4, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+        else k_jacobian<4, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+    } else if (mrows <= 1024) {
+        if (fixedm) k_jacobian<16, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+        else k_jacobian<16, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+    } else {
+        if (fixedm) k_jacobian<64, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+        else k_jacobian<64, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+    }
     check_launch();
 }
 

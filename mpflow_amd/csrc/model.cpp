@@ -51,6 +51,21 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     elements.alloc(S.elements.size()); elements.upload(S.elements.data(), S.elements.size(), st0);
     ek.alloc(S.elem_k.size()); ek.upload(S.elem_k.data(), S.elem_k.size(), st0);
     em.alloc(S.elem_m.size()); em.upload(S.elem_m.data(), S.elem_m.size(), st0);
+    {
+        // the Jacobian visits elements in the order of their lowest device node: the
+        // device numbering is a recursive bisection, so consecutive elements share nodes
+        std::vector<int64_t> key(S.E);
+        for (int e = 0; e < S.E; ++e) {
+            int lo = S.elements[(size_t)e * S.k];
+            for (int a = 1; a < S.k; ++a) lo = std::min(lo, S.elements[(size_t)e * S.k + a]);
+            key[e] = ((int64_t)lo << 32) | e;
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int32_t> ord(S.E);
+        for (int e = 0; e < S.E; ++e) ord[e] = (int32_t)(key[e] & 0xFFFFFFFF);
+        jac_order.alloc(S.E);
+        jac_order.upload(ord.data(), S.E, st0);
+    }
     F.alloc(S.F.size()); F.upload(S.F.data(), S.F.size(), st0);
     if (S.fixed) {
         auto up_i = [&](DBuf<int32_t>& b, const std::vector<int32_t>& v) { b.alloc(std::max<size_t>(v.size(), 1)); if (!v.empty()) b.upload(v.data(), v.size(), st0); };
@@ -279,6 +294,7 @@ void Model::finish(const mpf_solve_opts& o, float* d_U, double* d_U64, float* d_
         JacDev jd;
         std::memset(&jd, 0, sizeof(jd));
         jd.elements = elements.p; jd.ek = ek.p; jd.em = em.p; jd.gamma = gamma.p;
+        jd.order = jac_order.p;
         jd.colmap = colmap.p; jd.colscale = colscale.p;
         jd.E = S.E; jd.k = S.k; jd.D = S.D; jd.M = S.M; jd.Dp = S.Dp; jd.Mp = S.Mp;
         jd.exact = S.jacobian_exact; jd.logarithmic = S.logarithmic; jd.ref = S.ref;

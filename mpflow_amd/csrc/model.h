@@ -43,6 +43,7 @@ struct Model {
     DBuf<float> cinv, csc, cy, cg;          // two-level preconditioner (allocated if the mesh has one)
     DBuf<double> cd1;
     DBuf<float> ek, em;
+    DBuf<int32_t> jac_order;     // elements sorted by their first device node (locality)
     // per-frame state
     DBuf<float> s, r, s_lo, r_lo, dinv, x, rv, p, q, pring, rhs;
     DBuf<float> rh, yv, sv, zv, tv;   // BiCGSTAB vectors (allocated on first use)
