@@ -65,12 +65,14 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
                                                   __bf16* __restrict__ Jbf) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int E = jd.E;
+    // the grid's x extent is a multiple of 8, so block (x, frame) runs on XCD x % 8 for
+    // every frame (blocks go to the XCDs round-robin in linear order)
     int pos = blockIdx.x * kJacWaves + wave;
     if (jd.order) {
-        // block b runs on XCD b % 8: give XCD x the x-th eighth of the locality order
-        const int nb = (E + kJacWaves - 1) / kJacWaves;
-        const int base = nb / 8, extra = nb % 8, x = blockIdx.x % 8;
-        pos = (x * base + min(x, extra) + blockIdx.x / 8) * kJacWaves + wave;
+        // give XCD x the x-th eighth of the locality order
+        const int nb = gridDim.x;
+        const int base = nb / 8, x = blockIdx.x % 8;
+        pos = (x * base + blockIdx.x / 8) * kJacWaves + wave;
     }
     if (pos >= E) return;   // the whole wavefront leaves; no block barrier follows
     const int e = jd.order ? jd.order[pos] : pos;
@@ -881,7 +883,7 @@ void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use6
                      hipStream_t st, uint16_t* Jb) {
     MPF_REQUIRE(jd.Mp * jd.Dp <= kJacMaxRows, "jacobian: too many rows");
     MPF_REQUIRE(jd.D + jd.M <= 64 && jd.k <= 6, "jacobian: unsupported shape");
-    dim3 grid((jd.E + kJacWaves - 1) / kJacWaves, njobs);
+    dim3 grid(((jd.E + kJacWaves - 1) / kJacWaves + 7) / 8 * 8, njobs);
     const int mrows = jd.Mp * jd.Dp;
     __bf16* jb = reinterpret_cast<__bf16*>(Jb);
     const bool fixedm = 64 % jd.Mp == 0;
