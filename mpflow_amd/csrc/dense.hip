@@ -58,8 +58,10 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // PER: column rows per lane (mrows <= 64 PER); FIXEDM: Mp divides 64, so a lane's
-// measurement column is the same for all of its rows
-template <int PER, bool FIXEDM>
+// measurement column is the same for all of its rows; K: nodes per element (3 or 6), a
+// compile-time constant so that the per-node loops unroll and the lane's column of T
+// stays in registers
+template <int PER, bool FIXEDM, int K>
 __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ jobs, JacDev jd,
                                                   int use64, float* __restrict__ Jout,
                                                   __bf16* __restrict__ Jbf) {
@@ -78,7 +80,8 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
     const int e = jd.order ? jd.order[pos] : pos;
     const int j = blockIdx.y;
     const JobDev& Jb = jobs[j];
-    const int k = jd.k, D = jd.D, M = jd.M, Mp = jd.Mp, Dp = jd.Dp, R = D + M;
+    constexpr int k = K;
+    const int D = jd.D, M = jd.M, Mp = jd.Mp, Dp = jd.Dp, R = D + M;
     const int mrows = Mp * Dp;
     __shared__ float phi_s[kJacWaves][6][64];
     __shared__ float T_s[kJacWaves][6][64];
@@ -123,14 +126,16 @@ __global__ void __launch_bounds__(256) k_jacobian(const JobDev* __restrict__ job
         if constexpr (FIXEDM) {
             // Mp divides 64: the lane's measurement column m is the same for every q, so
             // its k entries of T stay in registers and d = d0 + q (64 / Mp)
-            float tm[6];
-            for (int a = 0; a < 6; ++a) tm[a] = (a < k && m0 < M) ? T[a][m0] : 0.0f;
+            float tm[K];
+#pragma unroll
+            for (int a = 0; a < K; ++a) tm[a] = m0 < M ? T[a][m0] : 0.0f;
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int d = d0 + q * dstep;
                 if (d < D && m0 < M && lane + 64 * q < mrows) {
                     float v = 0.0f;
-                    for (int a = 0; a < k; ++a) v += phi[a][d] * tm[a];
+#pragma unroll
+                    for (int a = 0; a < K; ++a) v += phi[a][d] * tm[a];
                     acc[q] += v;
                 }
             }
@@ -887,16 +892,20 @@ void launch_jacobian(const JobDev* d_jobs, int njobs, const JacDev& jd, int use6
     const int mrows = jd.Mp * jd.Dp;
     __bf16* jb = reinterpret_cast<__bf16*>(Jb);
     const bool fixedm = 64 % jd.Mp == 0;
-    if (mrows <= 256) {
-        if (fixedm) k_jacobian<4, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-        else k_jacobian<4, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-    } else if (mrows <= 1024) {
-        if (fixedm) k_jacobian<16, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-        else k_jacobian<16, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-    } else {
-        if (fixedm) k_jacobian<64, true><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-        else k_jacobian<64, false><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
-    }
+    auto go = [&](auto per) {
+        constexpr int P = decltype(per)::value;
+        if (jd.k == 3) {
+            if (fixedm) k_jacobian<P, true, 3><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+            else k_jacobian<P, false, 3><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+        } else {
+            if (fixedm) k_jacobian<P, true, 6><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+            else k_jacobian<P, false, 6><<<grid, 256, 0, st>>>(d_jobs, jd, use64, J, jb);
+        }
+    };
+    MPF_REQUIRE(jd.k == 3 || jd.k == 6, "jacobian: 3 or 6 nodes per element");
+    if (mrows <= 256) go(std::integral_constant<int, 4>());
+    else if (mrows <= 1024) go(std::integral_constant<int, 16>());
+    else go(std::integral_constant<int, 64>());
     check_launch();
 }
 

@@ -224,3 +224,51 @@ class LearnedReconstruction:
                 g[f] = fb.material_distribution[0]
         info["accepted"] = ok
         return g, info
+
+    def reconstruct_many(self, Um: np.ndarray, noise_norm: np.ndarray):
+        """All frames of ``Um`` (a multiple of the model's frame count), ``noise_norm`` per
+        frame: network start + Gauss-Newton per batch of the model's frames; the frames
+        whose start is rejected are collected and redone together afterwards, plain
+        Gauss-Newton from the homogeneous start in batches of the same model (the last
+        batch padded with copies of its first frame).  Returns (gamma, accepted, residual
+        of the learned result) for every frame."""
+        import torch
+        F = self.eit.frames
+        n = Um.shape[0]
+        if n % F:
+            raise ValueError("reconstruct_many: the frame count must be a multiple of the model's frames")
+        gam, acc, res = [], [], []
+        for b0 in range(0, n, F):
+            sl = slice(b0, b0 + F)
+            nn = np.asarray(noise_norm[sl], np.float64)
+            g, info = self._learned_batch(Um[sl], nn)
+            gam.append(g)
+            acc.append(info["accepted"])
+            res.append(info["residual_learned"])
+        gam = torch.cat(gam)
+        acc = np.concatenate(acc)
+        res = np.concatenate(res)
+        rej = np.nonzero(~acc)[0]
+        dev = self.eit._dev()
+        for b0 in range(0, rej.size, F):
+            idx = rej[b0:b0 + F]
+            rows = np.concatenate([idx, np.full(F - idx.size, idx[0])])
+            gp = self._gn(torch.tensor(Um[rows], dtype=torch.float64, device=dev), self.plain_steps)
+            gam[torch.tensor(idx, device=gam.device)] = gp[:idx.size]
+        return gam, acc, res
+
+    def _learned_batch(self, Um, noise_norm):
+        """One batch of the model's frames: network start, gn_steps Gauss-Newton steps and
+        the per-frame acceptance test r_learned <= tau * noise_norm (no fallback)."""
+        import torch
+        dev = self.eit._dev()
+        Um_t = torch.tensor(Um, dtype=torch.float64, device=dev)
+        X = torch.tensor(self.features(Um), dtype=torch.float32, device=dev)
+        ln_sigma = self.net.forward(X)
+        if self.eit.logarithmic:
+            start = (ln_sigma * (10.0 / math.log(10.0))).to(torch.float32)
+        else:
+            start = torch.exp(ln_sigma).to(torch.float32)
+        g = self._gn(Um_t, self.gn_steps, start)
+        r = np.linalg.norm(Um - self.eit.result64.cpu().numpy(), axis=1)
+        return g, {"residual_learned": r, "accepted": r <= self.tau * noise_norm}

@@ -113,3 +113,15 @@ def test_learned_start_gains_over_plain_gn(gpu_ctx):
           % (el_.mean(), ep_.mean(), info["accepted"].sum(), F))
     assert el_.mean() < ep_.mean()
     assert info["accepted"].mean() >= 0.5
+    # several batches through reconstruct_many: accepted frames keep the learned result,
+    # rejected ones get the plain result, computed together afterwards (a noise norm of
+    # zero for the odd frames forces their rejection)
+    U2 = np.concatenate([Uh, Uh[::-1].copy()])
+    nn = np.full(U2.shape[0], 1e30)
+    nn[1::2] = 0.0
+    g2, acc2, r2 = rec.reconstruct_many(U2, nn)
+    assert acc2[0::2].all() and not acc2[1::2].any()
+    gl0, _ = rec._learned_batch(Uh, np.full(F, 1e30))
+    np.testing.assert_allclose(g2[:F][0::2].cpu().numpy(), gl0[0::2].cpu().numpy(), rtol=1e-6, atol=1e-7)
+    gp3 = rec._gn(torch.tensor(Uh, device="cuda"), rec.plain_steps)
+    np.testing.assert_allclose(g2[:F][1::2].cpu().numpy(), gp3[1::2].cpu().numpy(), rtol=1e-5, atol=1e-6)
