@@ -96,26 +96,42 @@ __global__ void __launch_bounds__(256) k_cem_couple(const JobDev* __restrict__ j
     }
 }
 
-// Phi_c[row][u] = sum_l Y_c[row][l] T_c[l][u] (FP64) for every (frame, harmonic, tile)
+// Phi_c[row][u] = sum_l Y_c[row][l] T_c[l][u] (FP64) for every (frame, harmonic, tile):
+// the tile's rows of Y_c are staged in LDS in chunks of 32 rows (one coalesced read of
+// each row's L unit fields), T_c once per block; every output then reads its row of Y
+// and its column of T from LDS.
+constexpr int kCemRows = 32;   // rows per LDS chunk (dynamic LDS stays below 64 KiB for L, pRp <= 64)
+
 __global__ void __launch_bounds__(256) k_cem_fields(const JobDev* __restrict__ jobs, CemDev cd, int use64,
                                                     const double* __restrict__ T, double* __restrict__ phi) {
     const int tile = blockIdx.x, c = blockIdx.y, f = blockIdx.z;
     const JobDev& J = jobs[f];
-    const int L = cd.L, pRp = cd.pRp;
-    __shared__ double Ts[kCemMaxPorts][kCemMaxPorts];
-    for (int i = threadIdx.x; i < L * pRp; i += blockDim.x)
-        Ts[i / pRp][i % pRp] = T[(((size_t)f * J.C + c) * L) * pRp + i];
-    __syncthreads();
+    const int L = cd.L, pRp = cd.pRp, ly = L + 1;
+    // dynamic LDS: T [L][pRp], then the chunk of Y [kCemRows][L + 1]
+    extern __shared__ double sm[];
+    double* Ts = sm;
+    double* Ys = sm + L * pRp;
+    for (int i = threadIdx.x; i < L * pRp; i += blockDim.x) Ts[i] = T[(((size_t)f * J.C + c) * L) * pRp + i];
     double* out = phi + ((size_t)f * J.C + c) * (size_t)J.np * pRp;
-    for (int i = threadIdx.x; i < kTileRows * pRp; i += blockDim.x) {
-        const int row = tile * kTileRows + i / pRp, u = i % pRp;
-        double v = 0.0;
-        if (row < J.n && u < cd.pRu)
-            for (int l = 0; l < L; ++l) {
+    for (int r0 = tile * kTileRows; r0 < (tile + 1) * kTileRows; r0 += kCemRows) {
+        __syncthreads();   // Ts written / the previous chunk's readers done
+        for (int i = threadIdx.x; i < kCemRows * L; i += blockDim.x) {
+            const int rr = i / L, l = i % L, row = r0 + rr;
+            double y = 0.0;
+            if (row < J.n) {
                 const size_t vi = vec_index(J, c, row, l);
-                v += (use64 ? J.x64[vi] : (double)J.x[vi]) * Ts[l][u];
+                y = use64 ? J.x64[vi] : (double)J.x[vi];
             }
-        out[(size_t)row * pRp + u] = v;
+            Ys[rr * ly + l] = y;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kCemRows * pRp; i += blockDim.x) {
+            const int rr = i / pRp, u = i % pRp, row = r0 + rr;
+            double v = 0.0;
+            if (row < J.n && u < cd.pRu)
+                for (int l = 0; l < L; ++l) v += Ys[rr * ly + l] * Ts[l * pRp + u];
+            out[(size_t)row * pRp + u] = v;
+        }
     }
 }
 
@@ -124,7 +140,8 @@ void launch_cem(const JobDev* d_jobs, int frames, int C, int np, const CemDev& c
     MPF_REQUIRE(cd.L <= kCemMaxPorts && cd.pRp <= kCemMaxPorts && cd.D <= kCemMaxPorts,
                 "coupled electrode model: at most 64 ports and pattern columns");
     k_cem_couple<<<frames, 256, 0, st>>>(d_jobs, cd, use64, U32, U64, T);
-    k_cem_fields<<<dim3(np / kTileRows, C, frames), 256, 0, st>>>(d_jobs, cd, use64, T, phi);
+    const size_t lds = sizeof(double) * ((size_t)cd.L * cd.pRp + (size_t)kCemRows * (cd.L + 1));
+    k_cem_fields<<<dim3(np / kTileRows, C, frames), 256, lds, st>>>(d_jobs, cd, use64, T, phi);
     MPF_HIP(hipGetLastError());
 }
 

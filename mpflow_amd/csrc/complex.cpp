@@ -140,11 +140,11 @@ ComplexModel::ComplexModel(mpf_ctx* c, const mpf_model_desc& d) : impl(new Impl)
     std::vector<int4> ht, ha;
     std::vector<int2> hj;
     const int nasm_job = (int)((ell + 255) / 256);
+    // (frame, tile, harmonic) order: a tile's harmonics share its matrix rows in L2
     for (int f = 0; f < M.frames; ++f) {
-        for (int k = 0; k < S.C; ++k) {
-            hj.push_back(make_int2(f, k));
-            for (int t = 0; t < M.ntile_job; ++t) ht.push_back(make_int4(f, k, t, 0));
-        }
+        for (int k = 0; k < S.C; ++k) hj.push_back(make_int2(f, k));
+        for (int t = 0; t < M.ntile_job; ++t)
+            for (int k = 0; k < S.C; ++k) ht.push_back(make_int4(f, k, t, 0));
         for (int b = 0; b < nasm_job; ++b) ha.push_back(make_int4(f, b, 0, 0));
     }
     M.ntiles = (int)ht.size(); M.njs = (int)hj.size(); M.nasm = (int)ha.size();

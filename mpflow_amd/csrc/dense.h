@@ -57,6 +57,14 @@ void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, vo
 void launch_gemv_t(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
                    size_t stridex, double* y, size_t stridey, int batch, hipStream_t st);
 size_t gemv_n_workspace(int rows, int cols, int batch);
+// out = rhs - (A A^T x + shift x) in one pass over A (work: gemv_n_workspace bytes);
+// applies for rows <= 1024 with rows, lda, strideA multiples of 4 and A 16-byte aligned
+bool gemv_tn_applies(int rows, int lda, size_t strideA, const float* A);
+void launch_gemv_tn(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                    size_t stridex, double shift, const double* rhs, size_t striderhs, double* out,
+                    size_t strideout, int batch, double* work, hipStream_t st);
+// y_b += dy_b; ratio[b] = max|dy_b| / max|y_b| for the batch's n-vectors
+void launch_add_to_ratio(int n, int batch, double* y, const double* dy, double* ratio, hipStream_t st);
 void launch_gemv_n(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
                    size_t stridex, double shift, const double* z, size_t stridez, const double* rhs,
                    size_t striderhs, double* out, size_t strideout, int batch, double* work,

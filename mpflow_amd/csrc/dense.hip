@@ -531,8 +531,76 @@ __global__ void __launch_bounds__(256) k_gemv_t(int rows, int cols, const float*
     }
 }
 
+// k_gemv_t for rows % 4 == 0 (and lda, strideA multiples of 4): 128-bit loads, a
+// wavefront reads 1 KiB of its column per instruction
+__global__ void __launch_bounds__(256) k_gemv_t4(int rows, int cols, const float* __restrict__ A, int lda,
+                                                 size_t strideA, const double* __restrict__ x, size_t stridex,
+                                                 double* __restrict__ y, size_t stridey) {
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.y;
+    if (col >= cols) return;
+    const float* a = A + b * strideA + (size_t)col * lda;
+    const double* xv = x + b * stridex;
+    double s = 0.0;
+    for (int r = 4 * lane; r < rows; r += 256) {
+        const float4 v = *(const float4*)(a + r);
+        s += (double)v.x * xv[r] + (double)v.y * xv[r + 1] + (double)v.z * xv[r + 2] + (double)v.w * xv[r + 3];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[b * stridey + col] = s;
+}
+
 // partial[slab][r] = sum_{c in slab} A[r + c*lda] x[c]
 constexpr int kNSlab = 256;
+
+// partial[slab][r] = sum_{c in slab} A[r + c*lda] (sum_r' A[r' + c*lda] x[r']): A (A^T x)
+// in one pass over A (rows <= 1024, rows, lda and strideA multiples of 4).  A block owns
+// the kNSlab columns of its slab, kTnCols at a time: thread t loads rows 4t..4t+3 of each
+// column (128-bit), the block reduces the column dots in FP64, and the same registers
+// then give the rows' partial sums, so every element of A is read once.
+constexpr int kTnCols = 8;
+__global__ void __launch_bounds__(256) k_gemv_tn_part(int rows, int cols, const float* __restrict__ A, int lda,
+                                                      size_t strideA, const double* __restrict__ x,
+                                                      size_t stridex, double* __restrict__ part) {
+    const int slab = blockIdx.x, b = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = 4 * threadIdx.x;
+    const bool act = r0 < rows;
+    const float* a = A + b * strideA;
+    double xv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (act)
+        for (int i = 0; i < 4; ++i) xv[i] = x[b * stridex + r0 + i];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    __shared__ double red[4][kTnCols];
+    const int c0 = slab * kNSlab, c1 = min(cols, c0 + kNSlab);
+    for (int cb = c0; cb < c1; cb += kTnCols) {
+        float4 p[kTnCols];
+#pragma unroll
+        for (int j = 0; j < kTnCols; ++j)
+            p[j] = (act && cb + j < c1) ? *(const float4*)(a + (size_t)(cb + j) * lda + r0)
+                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+        double d[kTnCols];
+#pragma unroll
+        for (int j = 0; j < kTnCols; ++j) {
+            d[j] = (double)p[j].x * xv[0] + (double)p[j].y * xv[1] + (double)p[j].z * xv[2] + (double)p[j].w * xv[3];
+            for (int off = 32; off > 0; off >>= 1) d[j] += __shfl_xor(d[j], off, 64);
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < kTnCols; ++j) red[wave][j] = d[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kTnCols; ++j) {
+            const double v = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+            acc[0] += (double)p[j].x * v; acc[1] += (double)p[j].y * v;
+            acc[2] += (double)p[j].z * v; acc[3] += (double)p[j].w * v;
+        }
+        __syncthreads();   // red is rewritten by the next group of columns
+    }
+    if (act)
+        for (int i = 0; i < 4; ++i) part[((size_t)b * gridDim.x + slab) * rows + r0 + i] = acc[i];
+}
 __global__ void __launch_bounds__(256) k_gemv_n_part(int rows, int cols, const float* __restrict__ A, int lda,
                                                      size_t strideA, const double* __restrict__ x, size_t stridex,
                                                      double* __restrict__ part) {
@@ -757,6 +825,33 @@ __global__ void __launch_bounds__(256) k_sub(int n, const double* __restrict__ a
 __global__ void __launch_bounds__(256) k_add_to(int n, double* __restrict__ y, const double* __restrict__ dy) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < (size_t)n) y[i] += dy[i];
+}
+
+// y_b += dy_b and ratio[b] = max|dy_b| / max|y_b| (after the update): one block per vector
+__global__ void __launch_bounds__(256) k_add_to_ratio(int n, double* __restrict__ y, const double* __restrict__ dy,
+                                                      double* __restrict__ ratio) {
+    const int b = blockIdx.x;
+    double* yb = y + (size_t)b * n;
+    const double* db = dy + (size_t)b * n;
+    double md = 0.0, my = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double v = yb[i] + db[i];
+        yb[i] = v;
+        md = fmax(md, fabs(db[i]));
+        my = fmax(my, fabs(v));
+    }
+    __shared__ double sd[256], sy[256];
+    sd[threadIdx.x] = md;
+    sy[threadIdx.x] = my;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
+            sy[threadIdx.x] = fmax(sy[threadIdx.x], sy[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ratio[b] = sy[0] > 0.0 ? sd[0] / sy[0] : (sd[0] > 0.0 ? 1.0 : 0.0);
 }
 
 __global__ void __launch_bounds__(256) k_gamma_update(int n, float* __restrict__ g, const double* __restrict__ d) {
@@ -1028,7 +1123,25 @@ void launch_gram(const float* A, int trans, int n, int nk, int lda, float* G, vo
 void launch_gemv_t(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
                    size_t stridex, double* y, size_t stridey, int batch, hipStream_t st) {
     dim3 grid((cols + 3) / 4, batch);
-    k_gemv_t<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, y, stridey, 0.0);
+    if (rows % 4 == 0 && lda % 4 == 0 && strideA % 4 == 0 && ((uintptr_t)A & 15) == 0)
+        k_gemv_t4<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, y, stridey);
+    else
+        k_gemv_t<<<grid, 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, y, stridey, 0.0);
+    check_launch();
+}
+
+bool gemv_tn_applies(int rows, int lda, size_t strideA, const float* A) {
+    return rows > 0 && rows <= 1024 && rows % 4 == 0 && lda % 4 == 0 && strideA % 4 == 0 && ((uintptr_t)A & 15) == 0;
+}
+
+void launch_gemv_tn(int rows, int cols, const float* A, int lda, size_t strideA, const double* x,
+                    size_t stridex, double shift, const double* rhs, size_t striderhs, double* out,
+                    size_t strideout, int batch, double* work, hipStream_t st) {
+    MPF_REQUIRE(gemv_tn_applies(rows, lda, strideA, A), "gemv_tn: rows <= 1024, multiples of 4, aligned A");
+    const int nslab = (cols + kNSlab - 1) / kNSlab;
+    k_gemv_tn_part<<<dim3(nslab, batch), 256, 0, st>>>(rows, cols, A, lda, strideA, x, stridex, work);
+    dim3 g2((rows + 255) / 256, batch);
+    k_gemv_n_reduce<<<g2, 256, 0, st>>>(rows, nslab, work, shift, x, stridex, rhs, striderhs, out, strideout);
     check_launch();
 }
 
@@ -1124,6 +1237,11 @@ void launch_sub(int n, const double* a, const double* b, double* out, hipStream_
 
 void launch_add_to(int n, double* y, const double* dy, hipStream_t st) {
     k_add_to<<<(n + 255) / 256, 256, 0, st>>>(n, y, dy);
+    check_launch();
+}
+
+void launch_add_to_ratio(int n, int batch, double* y, const double* dy, double* ratio, hipStream_t st) {
+    k_add_to_ratio<<<batch, 256, 0, st>>>(n, y, dy, ratio);
     check_launch();
 }
 

@@ -198,11 +198,12 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     std::vector<int2> hj;
     std::vector<int4> ha;
     const int nasm_job = (int)((ell + kBlock - 1) / kBlock);
+    // (frame, tile, harmonic) order: the harmonics of one tile run back to back on one XCD
+    // and read the tile's matrix rows (ids, S, R) from its L2 after the first
     for (int f = 0; f < frames; ++f) {
-        for (int c = 0; c < S.C; ++c) {
-            hj.push_back(make_int2(f, c));
-            for (int t = 0; t < ntile_job; ++t) ht.push_back(make_int4(f, c, t, 0));
-        }
+        for (int c = 0; c < S.C; ++c) hj.push_back(make_int2(f, c));
+        for (int t = 0; t < ntile_job; ++t)
+            for (int c = 0; c < S.C; ++c) ht.push_back(make_int4(f, c, t, 0));
         for (int b = 0; b < nasm_job; ++b) ha.push_back(make_int4(f, b, 0, 0));
     }
     ntiles = (int)ht.size(); njs = (int)hj.size(); nasm = (int)ha.size();
@@ -351,7 +352,7 @@ int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_o
     if (warm) {
         // start from the previous FP64 fields: solve for the correction of the FP64
         // residual at the new sigma, to the same absolute accuracy tol * ||beta F||
-        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
         capped |= drv.run(o, MPF_STOP_TOLERANCE, o.tol, true, true, stats);
         for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
     } else {
@@ -362,12 +363,12 @@ int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_o
     for (Model* m : ms) m->x64_valid = mixed;
     if (mixed) {
         for (int k = 0; k < o.refine_steps; ++k) {
-            for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+            for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
             capped |= drv.run(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats);
             for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
         }
         // final FP64 residual for the report
-        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, st0);
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
     }
     for (size_t i = 0; i < ms.size(); ++i)
         ms[i]->finish(o, d_U ? d_U[i] : nullptr, d_U64 ? d_U64[i] : nullptr, d_J ? d_J[i] : nullptr, stats);
@@ -396,6 +397,7 @@ void Model::ensure_gn() {
     A64.alloc((size_t)frames * m * m);
     U64.alloc((size_t)frames * m);
     y.alloc((size_t)frames * m); dy.alloc((size_t)frames * m); res.alloc((size_t)frames * m);
+    ratio.alloc(frames);
     rgn.alloc((size_t)frames * m);
     vE.alloc((size_t)frames * S.E);
     work_n.alloc(gemv_n_workspace(m, S.E, frames) / sizeof(double) + 1);
@@ -498,12 +500,33 @@ void Model::normal_solve(const float* Jp, size_t strideJ, const double* A, size_
     hipStream_t st0 = ctx->stream;
     const int m = S.mrows, E = S.E;
     launch_potrs(m, A, strideA, rgn.p, m, y.p, m, nb, st0);
-    // FP64 refinement of (J J^T + lam I) y = r with residuals from the FP32 Jacobian
+    // FP64 refinement of (J J^T + lam I) y = r with residuals from the FP32 Jacobian: at
+    // most gn_refine passes, fewer once every frame's remaining error, predicted from the
+    // last two corrections (|dy_k|^2 / |dy_k-1|, relative to |y|, the refinement converging
+    // linearly), is below kRefineStop — far below what the FP32 Jacobian resolves
+    constexpr double kRefineStop = 1e-10;
+    const bool one_pass = gemv_tn_applies(m, m, strideJ, Jp);
+    if ((int)ratio_h.size() < nb) ratio_h.assign(nb, 0.0);
+    std::vector<double> prev(nb, 1.0);
     for (int k = 0; k < gn_refine; ++k) {
-        launch_gemv_t(m, E, Jp, m, strideJ, y.p, m, vE.p, E, nb, st0);
-        launch_gemv_n(m, E, Jp, m, strideJ, vE.p, E, lam2, y.p, m, rgn.p, m, res.p, m, nb, work_n.p, st0);
+        if (one_pass) {
+            launch_gemv_tn(m, E, Jp, m, strideJ, y.p, m, lam2, rgn.p, m, res.p, m, nb, work_n.p, st0);
+        } else {
+            launch_gemv_t(m, E, Jp, m, strideJ, y.p, m, vE.p, E, nb, st0);
+            launch_gemv_n(m, E, Jp, m, strideJ, vE.p, E, lam2, y.p, m, rgn.p, m, res.p, m, nb, work_n.p, st0);
+        }
         launch_potrs(m, A, strideA, res.p, m, dy.p, m, nb, st0);
-        launch_add_to(m * nb, y.p, dy.p, st0);
+        launch_add_to_ratio(m, nb, y.p, dy.p, ratio.p, st0);
+        if (k + 1 == gn_refine) break;
+        MPF_HIP(hipMemcpyAsync(ratio_h.data(), ratio.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st0));
+        MPF_HIP(hipStreamSynchronize(st0));
+        bool done = true;
+        for (int b = 0; b < nb; ++b) {
+            const double r = ratio_h[b];
+            done = done && std::isfinite(r) && r * r <= kRefineStop * prev[b];
+            prev[b] = r > 0.0 ? r : 1e-300;
+        }
+        if (done) break;
     }
 }
 

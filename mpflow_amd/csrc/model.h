@@ -66,6 +66,8 @@ struct Model {
     DBuf<float> Jm, G;
     DBuf<uint16_t> Jb;            // BF16 copy of Jm for the Gram (identity regularisation)
     DBuf<double> A64, U64, y, dy, res, rgn, vE, work_n;
+    DBuf<double> ratio;                 // per frame: |dy| / |y| of the last refinement pass
+    std::vector<double> ratio_h;
     DBuf<char> gram_work;
     DBuf<int2> pairs;
     DBuf<int> info;

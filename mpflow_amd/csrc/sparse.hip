@@ -852,40 +852,93 @@ __global__ void __launch_bounds__(kEwBlock) k_colsum64_reduce(const JobDev* __re
         sc_ptr(J, SC_X0, s)[threadIdx.x] = (s == J.dc && J.dc_rank1) ? out[0] : 0.0;
 }
 
+// r = beta_c F - A_c x64 - dcs in FP64 for every lane of one (job, slice, tile) item:
+// a thread owns two lanes of one row (128-bit gathers of x64), the block takes PR rows
+// at a time, and the ELL entries of those rows are staged through LDS kResSlots slots at
+// a time with coalesced loads (id and the FP64 value S + alpha R, each formed once per
+// row and slot instead of once per lane).  The sums run over the slots in ELL order.
+constexpr int kResSlots = 8;
+template <int LG>
 __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restrict__ jobs,
                                                          const int4* __restrict__ tiles) {
-    const int4 t = tiles[blockIdx.x];
-    const JobDev& J = jobs[t.x];
-    const int s = t.y;
-    const Ew L = ew_of(J);
+    constexpr int Rp = 1 << LG, T = Rp / 2, PR = kEwBlock / T;
+    const int4 t4 = tiles[blockIdx.x];
+    const JobDev& J = jobs[t4.x];
+    if (J.Rp != Rp) return;   // the launcher checks that every job has 2^LG lanes
+    const int s = t4.y;
     const double al = J.alpha64[s];
     const double xs = J.xscale64[s];
-    const double dcs = sc_ptr(J, SC_X0, s)[L.lane];
-    const double* X = J.x64 + (size_t)s * J.np * J.Rp;
+    const int t = threadIdx.x & (T - 1), rl = threadIdx.x / T;
+    const int l0 = 2 * t;
+    const double dcs0 = sc_ptr(J, SC_X0, s)[l0], dcs1 = sc_ptr(J, SC_X0, s)[l0 + 1];
+    const MPF_GLOBAL double2* X2 = (const MPF_GLOBAL double2*)(gp(J.x64) + (size_t)s * J.np * Rp);
     float* out = J.rhs;
-    double rr = 0.0, bb = 0.0;
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
-        double acc = 0.0;
-        // slots past the 16-row group's widest row are padding (value 0)
-        const int w = J.gw ? J.gw[row >> 4] : J.width;
-#pragma unroll 8
-        for (int q = 0; q < w; ++q) {
-            const size_t e = (size_t)q * J.np + row;
-            const double v = ((double)J.s[e] + (double)J.s_lo[e]) + al * ((double)J.r[e] + (double)J.r_lo[e]);
-            acc += v * X[(size_t)J.ids[e] * J.Rp + L.lane];
+    __shared__ union {
+        struct {
+            int id[kResSlots][PR];
+            double v[kResSlots][PR];
+        } e;
+        double red[2][PR][Rp];
+    } sh;
+    double rr0 = 0.0, rr1 = 0.0, bb0 = 0.0, bb1 = 0.0;
+    const int tile0 = t4.z * kTileRows;
+    for (int base = tile0; base < tile0 + kTileRows; base += PR) {
+        const int row = base + rl;
+        // slots past a 16-row group's widest row are padding (value 0)
+        int wp = J.width;
+        if (J.gw) {
+            wp = 0;
+            for (int g = base >> 4; g < (base + PR) >> 4; ++g) wp = max(wp, (int)J.gw[g]);
         }
-        double b = 0.0, r = 0.0;
-        if (row < J.n && L.lane < J.R) {
-            b = xs * J.F[s * J.fstride + (size_t)row * J.Rp + L.lane];
-            r = b - acc - dcs;
+        const int wr = J.gw ? J.gw[row >> 4] : J.width;
+        double a0 = 0.0, a1 = 0.0;
+        for (int q0 = 0; q0 < wp; q0 += kResSlots) {
+            __syncthreads();   // the previous step's entries have been read
+            for (int i = threadIdx.x; i < kResSlots * PR; i += kEwBlock) {
+                const int qq = i / PR, r2 = i % PR, q = q0 + qq;
+                int id = 0;
+                double v = 0.0;
+                if (q < wp) {
+                    const size_t e = (size_t)q * J.np + base + r2;
+                    id = J.ids[e];
+                    v = ((double)J.s[e] + (double)J.s_lo[e]) + al * ((double)J.r[e] + (double)J.r_lo[e]);
+                }
+                sh.e.id[qq][r2] = id;
+                sh.e.v[qq][r2] = v;
+            }
+            __syncthreads();
+            const int qn = wr - q0;
+#pragma unroll
+            for (int qq = 0; qq < kResSlots; ++qq) {
+                if (qq < qn) {
+                    const double2 xv = X2[(size_t)sh.e.id[qq][rl] * T + t];
+                    const double v = sh.e.v[qq][rl];
+                    a0 += v * xv.x;
+                    a1 += v * xv.y;
+                }
+            }
         }
-        out[vec_index(J, s, row, L.lane)] = (float)r;
-        rr += r * r;
-        bb += b * b;
+        double b0 = 0.0, b1 = 0.0, r0 = 0.0, r1 = 0.0;
+        if (row < J.n) {
+            const double* F = J.F + s * J.fstride + (size_t)row * Rp;
+            if (l0 < J.R) { b0 = xs * F[l0]; r0 = b0 - a0 - dcs0; }
+            if (l0 + 1 < J.R) { b1 = xs * F[l0 + 1]; r1 = b1 - a1 - dcs1; }
+        }
+        *(float2*)(out + vec_index(J, s, row, l0)) = make_float2((float)r0, (float)r1);
+        rr0 += r0 * r0; rr1 += r1 * r1;
+        bb0 += b0 * b0; bb1 += b1 * b1;
     }
-    const int ch[2] = {0, 1};
-    const double v[2] = {rr, bb};
-    block_partials<2, kEwBlock>(J, s, t.z, ch, v);
+    // per-lane tile partials of r.r and b.b, summed over the rows in a fixed order
+    __syncthreads();
+    sh.red[0][rl][l0] = rr0; sh.red[0][rl][l0 + 1] = rr1;
+    sh.red[1][rl][l0] = bb0; sh.red[1][rl][l0 + 1] = bb1;
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * Rp) {
+        const int c = threadIdx.x / Rp, lane = threadIdx.x % Rp;
+        double a = 0.0;
+        for (int g = 0; g < PR; ++g) a += sh.red[c][g][lane];
+        part_ptr(J, c, s, t4.z)[lane] = a;
+    }
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_residual64_reduce(const JobDev* __restrict__ jobs,
@@ -1059,10 +1112,13 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
 }
 
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                       int njs, hipStream_t st) {
+                       int njs, int lg, hipStream_t st) {
     k_colsum64<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     k_colsum64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
-    k_residual64<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    if (lg == 4) k_residual64<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 5) k_residual64<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 6) k_residual64<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else throw Error(MPF_EINVAL, "residual: unsupported lane count");
     k_residual64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
     check_launch();
 }

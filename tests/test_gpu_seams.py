@@ -277,3 +277,30 @@ def test_model_sparse_format_full(gpu_ctx):
     assert e.value.code == capi.MPF_EFULL
     m = EIT(nodes, el, bd, ports, pat, pat, 1e-3, ctx=gpu_ctx, ell_max_width=64)
     assert m.N == k + 1
+
+
+def test_autotune_read_reports_choices(gpu_ctx):
+    """mpf_ctx_autotune_read: after a model solve the context holds one SpMV choice per
+    (lane count, width, row bucket) with a valid kernel and order; the query with no
+    room reports the count."""
+    import ctypes as C
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.eit import Context, EIT, SolveOptions
+    from tests.problems import make_problem
+    p = make_problem(8, 16, 0.1)
+    ctx = Context()
+    eit = EIT(p["nodes"], p["elements"], p["boundary"], p["ports"], p["drive"], p["meas"], p["current"], ctx=ctx)
+    eit.solve(torch.ones(eit.E), SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6))
+    L = capi.lib()
+    n = C.c_int32()
+    capi.check(L.mpf_ctx_autotune_read(ctx.h, 0, None, None, 0, C.byref(n)))
+    assert n.value >= 1
+    keys = (C.c_int64 * n.value)()
+    vals = (C.c_int32 * n.value)()
+    capi.check(L.mpf_ctx_autotune_read(ctx.h, 0, keys, vals, n.value, C.byref(n)))
+    for k, v in zip(keys, vals):
+        assert (k >> 40) in (4, 5, 6) and (v & 15) in (0, 1, 2) and (v >> 4) in (0, 1)
+    assert L.mpf_ctx_autotune_read(ctx.h, 2, None, None, 0, C.byref(n)) == capi.MPF_EINVAL
+    eit.close()
+    ctx.close()
