@@ -25,7 +25,7 @@ constexpr int kRing = 4;
 // current iteration saw it (the stop rule clears [0] during the pupdate launch)
 constexpr int kStInts = 4;
 // partial-sum channels per tile
-constexpr int kParts = 8;
+constexpr int kParts = 10;
 
 struct JobDev {
     int n, np, width, C, R, Rp, lg, ntiles, dc, maxit, rule, col0;   // R = unique RHS solved

@@ -409,12 +409,12 @@ __device__ __forceinline__ int coarse_chunks(const JobDev& J) {
     return (J.ntiles + kChunkTiles - 1) / kChunkTiles;
 }
 
-// first local row of a tile's second coarse aggregate: the tile's real rows split
-// ceil(n / 2) + floor(n / 2)
-__device__ __forceinline__ int tile_half(const JobDev& J, int tile) {
-    return (min(kTileRows, J.n - tile * kTileRows) + 1) / 2;
-}
-static_assert(kAggPerTile == 2, "the coarse kernels index aggregates as 2 tile + half");
+// Coarse aggregates: local rows [kAggRows q, kAggRows (q + 1)) of tile t form aggregate
+// kAggPerTile t + q (the device numbering bisects every tile into kAggRows-row leaves).
+// The FP64 tile partials of W^T r for quarter q sit in channel agg_channel(q).
+__device__ __forceinline__ constexpr int agg_channel(int q) { return q == 0 ? 4 : 6 + q; }
+static_assert(kAggPerTile == 4 && kAggPerTile * kAggRows == kTileRows,
+              "aggregate channels 4, 7, 8, 9: four quarters per tile");
 
 // block_partials4 with the reduction buffer supplied by the caller (it may share LDS with
 // data the wavefronts used before: the block synchronises before writing it)

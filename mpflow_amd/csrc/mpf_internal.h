@@ -14,7 +14,8 @@ namespace mpf {
 
 constexpr int kMaxSlices = 8;      // harmonics per job
 constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
-constexpr int kAggPerTile = 2;     // coarse aggregates per tile: its two geometric halves
+constexpr int kAggPerTile = 4;     // coarse aggregates per tile: its four 64-row geometric quarters
+constexpr int kAggRows = 64;       // rows per coarse aggregate (local rows [64 q, 64 q + 64) of a tile)
 constexpr int kCoarseRows = 64;    // coarse rows per block of the coarse-correction kernel
 constexpr int kChunkTiles = kCoarseRows / kAggPerTile;   // tiles per partial-sum chunk
 constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level

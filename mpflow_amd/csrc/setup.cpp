@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <thread>
 
 #include "mpf_internal.h"
 
@@ -204,10 +205,11 @@ static void edge_integrals(int kb, const double* sp, double (*m)[3], double* v) 
 // 256-node leaves, until every leaf fits one 256-row tile.  Each tile is then a compact
 // patch of the mesh: most gathers of its SpMV rows stay inside it, consecutive tiles are
 // neighbouring patches (the XCD-striped work lists keep them in one L2).  Each tile is
-// then bisected the same way into a first half of ceil(n / 2) rows and the rest: the
-// two aggregates of the two-level preconditioner it holds.  Rows are sorted by
-// descending degree inside each half so the 16-row groups of the SpMV have near-uniform
-// ELL widths.  Returns perm[node] = row.
+// then bisected the same way into leaves of kAggRows rows (a part holding a whole number
+// of them first): the aggregates of the two-level preconditioner, local rows
+// [kAggRows q, kAggRows (q + 1)).  Rows are sorted by descending degree inside each
+// aggregate so the 16-row groups of the SpMV have near-uniform ELL widths.  Returns
+// perm[node] = row.
 static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, const int32_t* el) {
     std::vector<int32_t> deg(N, 0);
     {
@@ -238,24 +240,28 @@ static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, 
             return va < vc || (va == vc && a < c);
         });
     };
-    std::vector<std::pair<int, int>> todo{{0, N}};
-    while (!todo.empty()) {
-        const auto [b, e] = todo.back();
-        todo.pop_back();
-        const int n = e - b;
-        if (n <= kTileRows) continue;
-        const int leaves = (n + kTileRows - 1) / kTileRows;
-        const int left = (leaves / 2) * kTileRows;
-        bisect(b, e, left);
-        todo.emplace_back(b + left, e);
-        todo.emplace_back(b, b + left);
-    }
+    // [b, e) split until every part fits one leaf of `leaf` rows
+    auto split = [&](int b0, int e0, int leaf) {
+        std::vector<std::pair<int, int>> todo{{b0, e0}};
+        while (!todo.empty()) {
+            const auto [b, e] = todo.back();
+            todo.pop_back();
+            const int n = e - b;
+            if (n <= leaf) continue;
+            const int leaves = (n + leaf - 1) / leaf;
+            const int left = (leaves / 2) * leaf;
+            bisect(b, e, left);
+            todo.emplace_back(b + left, e);
+            todo.emplace_back(b, b + left);
+        }
+    };
+    split(0, N, kTileRows);
     auto by_degree = [&](int a, int c) { return deg[a] != deg[c] ? deg[a] > deg[c] : a < c; };
     for (int w = 0; w < N; w += kTileRows) {
-        const int e = std::min(N, w + kTileRows), h = (e - w + 1) / 2;
-        if (e - w > 1) bisect(w, e, h);
-        std::sort(order.begin() + w, order.begin() + w + h, by_degree);
-        std::sort(order.begin() + w + h, order.begin() + e, by_degree);
+        const int e = std::min(N, w + kTileRows);
+        split(w, e, kAggRows);
+        for (int b = w; b < e; b += kAggRows)
+            std::sort(order.begin() + b, order.begin() + std::min(e, b + kAggRows), by_degree);
     }
     std::vector<int32_t> perm(N);
     for (int r = 0; r < N; ++r) perm[order[r]] = r;
@@ -265,45 +271,56 @@ static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, 
 // In-place inverse of an SPD matrix (n x n, row-major, FP64) by Cholesky: A = L L^T,
 // A^-1 = L^-T L^-1.  Returns false if A is not numerically positive definite.
 static bool spd_inverse(int n, double* A) {
+    // Cholesky A = L L^T, dot-product form over contiguous row prefixes
     std::vector<double> L((size_t)n * n, 0.0);
     for (int j = 0; j < n; ++j) {
+        const double* lj = &L[(size_t)j * n];
         double d = A[(size_t)j * n + j];
-        for (int q = 0; q < j; ++q) d -= L[(size_t)j * n + q] * L[(size_t)j * n + q];
+        for (int q = 0; q < j; ++q) d -= lj[q] * lj[q];
         if (!(d > 0.0)) return false;
         const double ljj = std::sqrt(d);
         L[(size_t)j * n + j] = ljj;
         for (int i = j + 1; i < n; ++i) {
-            double v = A[(size_t)i * n + j];
             const double* li = &L[(size_t)i * n];
-            const double* lj = &L[(size_t)j * n];
+            double v = A[(size_t)i * n + j];
             for (int q = 0; q < j; ++q) v -= li[q] * lj[q];
             L[(size_t)i * n + j] = v / ljj;
         }
     }
-    // M = L^-1 (lower triangular), row by row of forward substitution
+    // M = L^-1, one row at a time: M_i = (e_i - sum_{q < i} L_iq M_q) / L_ii, the sum as
+    // row updates over M_q's nonzero prefix [0, q]
     std::vector<double> M((size_t)n * n, 0.0);
-    for (int c = 0; c < n; ++c) {
-        M[(size_t)c * n + c] = 1.0 / L[(size_t)c * n + c];
-        for (int i = c + 1; i < n; ++i) {
-            double v = 0.0;
-            for (int q = c; q < i; ++q) v -= L[(size_t)i * n + q] * M[(size_t)q * n + c];
-            M[(size_t)i * n + c] = v / L[(size_t)i * n + i];
+    for (int i = 0; i < n; ++i) {
+        double* mi = &M[(size_t)i * n];
+        const double* li = &L[(size_t)i * n];
+        for (int q = 0; q < i; ++q) {
+            const double c = li[q];
+            const double* mq = &M[(size_t)q * n];
+            for (int u = 0; u <= q; ++u) mi[u] -= c * mq[u];
+        }
+        mi[i] += 1.0;
+        const double inv = 1.0 / li[i];
+        for (int u = 0; u <= i; ++u) mi[u] *= inv;
+    }
+    // A^-1 = M^T M, accumulated as rank-one row updates: row i (j <= i) += M_qi M_q[0, i]
+    std::fill(A, A + (size_t)n * n, 0.0);
+    for (int q = 0; q < n; ++q) {
+        const double* mq = &M[(size_t)q * n];
+        for (int i = 0; i <= q; ++i) {
+            const double c = mq[i];
+            double* ai = &A[(size_t)i * n];
+            for (int j = 0; j <= i; ++j) ai[j] += c * mq[j];
         }
     }
-    // A^-1 = M^T M: (i, j) = sum_{q >= max(i, j)} M[q][i] M[q][j]
     for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double v = 0.0;
-            for (int q = i; q < n; ++q) v += M[(size_t)q * n + i] * M[(size_t)q * n + j];
-            A[(size_t)i * n + j] = v;
-            A[(size_t)j * n + i] = v;
-        }
+        for (int j = 0; j < i; ++j) A[(size_t)j * n + i] = A[(size_t)i * n + j];
     return true;
 }
 
-// Coarse level of the two-level preconditioner (aggregates = the two halves of each
-// 256-row tile, aggregate 2t + 1 holding the tile's real rows from ceil(n_t / 2) on; W =
-// the aggregate indicator): per harmonic, the inverse of W^T A_c(1) W at sigma = 1 (plus
+// Coarse level of the two-level preconditioner (aggregates = the kAggRows-row leaves of
+// each 256-row tile, aggregate kAggPerTile t + q holding its local rows
+// [kAggRows q, kAggRows (q + 1)); W = the aggregate indicator): per harmonic, the inverse
+// of W^T A_c(1) W at sigma = 1 (plus
 // n n^T, n = real rows per aggregate, on the dcFree harmonic 0, matching the A + 1 1^T
 // operator), and per aggregate the sum of the sigma = 1 stiffness diagonal that scales
 // it to a frame's conductivity.  Skipped (coarse_nt = 0) above kCoarseMaxTiles tiles.
@@ -317,19 +334,17 @@ static void coarse_setup(MeshSetup& s) {
         for (int i = s.ctb_off[ent]; i < s.ctb_off[ent + 1]; ++i) { sk[ent] += s.ctb_k[i]; sm[ent] += s.ctb_m[i]; }
     // aggregate of each real row
     std::vector<int> agg(s.N);
-    for (int row = 0; row < s.N; ++row) {
-        const int t = row / kTileRows, rows = std::min(kTileRows, s.N - t * kTileRows);
-        agg[row] = kAggPerTile * t + (row - t * kTileRows >= (rows + 1) / 2 ? 1 : 0);
-    }
+    for (int row = 0; row < s.N; ++row) agg[row] = row / kAggRows;
     std::vector<double> cnt(na, 0.0);
     for (int row = 0; row < s.N; ++row) cnt[agg[row]] += 1.0;
     s.coarse_d1.assign(na, 0.0);
     for (int row = 0; row < s.N; ++row) s.coarse_d1[agg[row]] += sk[(size_t)s.diag_slot[row] * np + row];
     const int ld = (na + 3) / 4 * 4;
     s.coarse_inv.assign((size_t)s.C * na * ld, 0.0f);
-    std::vector<double> Ec((size_t)na * na);
-    for (int c = 0; c < s.C; ++c) {
-        std::fill(Ec.begin(), Ec.end(), 0.0);
+    // one host thread per harmonic (each forms and inverts its own coarse matrix)
+    std::vector<char> ok(s.C, 0);
+    auto harmonic = [&](int c) {
+        std::vector<double> Ec((size_t)na * na, 0.0);
         for (int q = 0; q < s.wpad; ++q)
             for (int row = 0; row < s.N; ++row) {
                 const size_t ent = (size_t)q * np + row;
@@ -339,7 +354,7 @@ static void coarse_setup(MeshSetup& s) {
         if (c == 0 && !s.fixed && !s.cem_coupled)
             for (int t = 0; t < na; ++t)
                 for (int u = 0; u < na; ++u) Ec[(size_t)t * na + u] += cnt[t] * cnt[u];
-        // aggregates without real rows (the second half of a one-row tile) keep an
+        // aggregates without real rows (past the last real row of the last tile) keep an
         // identity block
         for (int t = 0; t < na; ++t)
             if (cnt[t] == 0.0) Ec[(size_t)t * na + t] = 1.0;
@@ -347,7 +362,14 @@ static void coarse_setup(MeshSetup& s) {
         for (int t = 0; t < na; ++t)
             for (int u = 0; u < na; ++u)
                 s.coarse_inv[((size_t)c * na + t) * ld + u] = (float)Ec[(size_t)t * na + u];
-    }
+        ok[c] = 1;
+    };
+    std::vector<std::thread> pool;
+    for (int c = 1; c < s.C; ++c) pool.emplace_back(harmonic, c);
+    harmonic(0);
+    for (auto& th : pool) th.join();
+    for (int c = 0; c < s.C; ++c)
+        if (!ok[c]) return;
     s.coarse_nt = nt;
     s.coarse_ld = ld;
 }

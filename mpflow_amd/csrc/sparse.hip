@@ -76,9 +76,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
     const Ew L = ew_of(J);
     const float* dinv = J.dinv + (size_t)s * J.np;
     const float xs = J.xscale[s];
-    const int h = t.z * kTileRows + tile_half(J, t.z);
-    double rz = 0.0, bb = 0.0, bs0 = 0.0, bs1 = 0.0;
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
+    const int tile0 = t.z * kTileRows;
+    double rz = 0.0, bb = 0.0, bs[kAggPerTile] = {0.0, 0.0, 0.0, 0.0};
+    for (int row = tile0 + L.g; row < tile0 + kTileRows; row += L.G) {
         const size_t i = vec_index(J, s, row, L.lane);
         float b = 0.0f;
         if (L.lane < J.R) b = J.use_rhs ? J.rhs[i] : (float)(xs * J.F[s * J.fstride + (size_t)row * J.Rp + L.lane]);
@@ -88,13 +88,14 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
         J.p[i] = z;
         rz += (double)b * (double)z;
         bb += (double)b * (double)b;
-        if (row < h) bs0 += (double)b;
-        else bs1 += (double)b;
+        const int q = (row - tile0) / kAggRows;
+#pragma unroll
+        for (int a = 0; a < kAggPerTile; ++a) bs[a] += a == q ? (double)b : 0.0;
     }
-    // channels 4 and 7: W^T r of the tile's two aggregates (two-level preconditioner)
-    const int ch[4] = {0, 1, 4, 7};
-    const double v[4] = {rz, bb, bs0, bs1};
-    block_partials<4, kEwBlock>(J, s, t.z, ch, v);
+    // channels agg_channel(q): W^T r of the tile's aggregates (two-level preconditioner)
+    const int ch[6] = {0, 1, agg_channel(0), agg_channel(1), agg_channel(2), agg_channel(3)};
+    const double v[6] = {rz, bb, bs[0], bs[1], bs[2], bs[3]};
+    block_partials<6, kEwBlock>(J, s, t.z, ch, v);
 }
 
 // two-level start: p = z = D^-1 b + W y (y from the coarse kernel, real rows only)
@@ -106,10 +107,9 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_p(const JobDev* __restric
     const int s = t.y;
     const Ew L = ew_of(J);
     const size_t cb = ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * J.Rp + L.lane;
-    const float y0 = J.cy[cb], y1 = J.cy[cb + J.Rp];
-    const int h = t.z * kTileRows + tile_half(J, t.z);
-    for (int row = t.z * kTileRows + L.g; row < min((t.z + 1) * kTileRows, J.n); row += L.G)
-        J.p[vec_index(J, s, row, L.lane)] += row < h ? y0 : y1;
+    const int tile0 = t.z * kTileRows;
+    for (int row = tile0 + L.g; row < min(tile0 + kTileRows, J.n); row += L.G)
+        J.p[vec_index(J, s, row, L.lane)] += J.cy[cb + (size_t)((row - tile0) / kAggRows) * J.Rp];
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
@@ -415,49 +415,59 @@ __global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __re
     const MPF_GLOBAL float* dinv = gp((const float*)J.dinv) + (size_t)s * J.np;
     const MPF_GLOBAL float4* Q = (const MPF_GLOBAL float4*)gp((const float*)J.q) + base;
     MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
-    const int h = t.z * kTileRows + tile_half(J, t.z);
     double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     // W^T r per aggregate (read in FP32 by the coarse kernel): summed in FP32 within a
-    // wavefront, in FP64 across the block
-    float g[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
-    for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
-        const size_t i = (size_t)row * LPR + c4;
-        const float4 q4 = Q[i];
-        const float4 r4 = Rr[i];
-        const float d = dinv[row];
-        const float dc = row < J.n ? 1.0f : 0.0f;
-        const bool first = row < h;
-        float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
+    // wavefront, in FP64 across the block.  ROWS divides kAggRows: the thread's rows of
+    // aggregate q are passes kk of the inner loop.
+    static_assert(kAggRows % ROWS == 0, "a pass of the row loop stays inside one aggregate");
+    float g[kAggPerTile][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            rv[j] = zero[j] ? 0.0f : rv[j] - a[j] * (qv[j] + dc * dcv[j]);
-            const float z = d * rv[j];
-            v[0][j] += (double)rv[j] * (double)z;
-            v[1][j] += (double)rv[j] * (double)rv[j];
-            g[0][j] += first ? rv[j] : 0.0f;
-            g[1][j] += first ? 0.0f : rv[j];
+    for (int q = 0; q < kAggPerTile; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[q][j] = 0.0f;
+#pragma unroll 1
+    for (int q = 0; q < kAggPerTile; ++q) {
+        float gq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < kAggRows / ROWS; ++kk) {
+            const int row = t.z * kTileRows + q * kAggRows + r0 + kk * ROWS;
+            const size_t i = (size_t)row * LPR + c4;
+            const float4 q4 = Q[i];
+            const float4 r4 = Rr[i];
+            const float d = dinv[row];
+            const float dc = row < J.n ? 1.0f : 0.0f;
+            float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                rv[j] = zero[j] ? 0.0f : rv[j] - a[j] * (qv[j] + dc * dcv[j]);
+                const float z = d * rv[j];
+                v[0][j] += (double)rv[j] * (double)z;
+                v[1][j] += (double)rv[j] * (double)rv[j];
+                gq[j] += rv[j];
+            }
+            Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
         }
-        Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
+#pragma unroll
+        for (int qq = 0; qq < kAggPerTile; ++qq)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) g[qq][j] = qq == q ? gq[j] : g[qq][j];
     }
-    // channels 4 and 7: W^T r of the tile's two aggregates (two-level preconditioner)
+    // channels agg_channel(q): W^T r of the tile's aggregates (two-level preconditioner),
     // with a coarse level also c_a (W^T r)_a in FP32 for the coarse kernel's staging
-    const int ch[2] = {2, 3}, chf[2] = {4, 7};
+    const int ch[2] = {2, 3};
+    const int chf[kAggPerTile] = {agg_channel(0), agg_channel(1), agg_channel(2), agg_channel(3)};
     MPF_GLOBAL float* fo = nullptr;
-    float fs[2] = {0.0f, 0.0f};
+    float fs[kAggPerTile] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (J.cinv) {
         fo = gp(J.cg) + ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * Rp;
-        fs[0] = J.csc[kAggPerTile * t.z];
-        fs[1] = J.csc[kAggPerTile * t.z + 1];
+#pragma unroll
+        for (int q = 0; q < kAggPerTile; ++q) fs[q] = J.csc[kAggPerTile * t.z + q];
     }
-    block_partials4f<LG, 2, 2, kUpdBlock>(J, s, t.z, ch, v, chf, g, fo, fs);
+    block_partials4f<LG, 2, kAggPerTile, kUpdBlock>(J, s, t.z, ch, v, chf, g, fo, fs);
 }
 
 template <int LG>
 constexpr int coarse_threads() { return (1 << LG) * kCoarseRows > 1024 ? 1024 : (1 << LG) * kCoarseRows; }
-// waves per SIMD the coarse kernel is compiled for: two 1024-thread blocks per CU up to
-// 32 lanes per row (64-lane rows need more registers than that leaves)
-template <int LG>
-constexpr int coarse_waves() { return LG >= 6 ? 4 : 8; }
 
 // After the SpMV: the tile partials p.q (channel 0) and 1^T p (channel 1) summed per
 // kChunkTiles-tile chunk in a fixed order into channels 5 and 6, so every update block
@@ -494,107 +504,102 @@ __global__ void __launch_bounds__(coarse_threads<LG>()) k_pcg_chunks(const JobDe
 
 // Coarse correction of the two-level preconditioner, one block per (chunk of kCoarseRows
 // coarse rows = kChunkTiles tiles, job, slice): y_a = c_a sum_u cinv_s[a][u] c_u g_u with
-// g = W^T r (aggregate 2t: channel 4 of tile t, 2t + 1: channel 7) and c = Sc^-1/2,
-// written to cy; then per chunk the partial sums of
+// g = W^T r (aggregate kAggPerTile t + q: channel agg_channel(q) of tile t), written to
+// cy; then per chunk the partial sums of
 //   rho = sum_t (r.D^-1 r)_t + sum_a g_a y_a   (from channel ch_rz)  -> channel 5
 //   r.r = sum_t (r.r)_t                        (from channel ch_rr)  -> channel 6
-// so the next kernel reduces one partial per chunk instead of one per tile.  A thread
-// owns (coarse row, lane): the scaled g of KU aggregates at a time sits in LDS and each
-// thread streams its cinv row with 128-bit loads (rows padded to a multiple of 4 with
-// zeros; the lanes of a row read the same addresses).
+// so the next kernel reduces one partial per chunk instead of one per tile.  The product
+// Y[64 x Rp] = cinv_s[chunk rows][:] G[: x Rp] runs on the matrix cores
+// (v_mfma_f32_16x16x4_f32: FP32 operands, FP32 accumulation).  Wavefront w
+// owns coarse rows 16w..16w+15 of the chunk and all Rp columns (Rp / 16 accumulator
+// tiles).  G is staged through LDS kCoarseK aggregates at a time; each lane reads four
+// consecutive cinv entries of its row with one 128-bit load and feeds them to four MFMAs
+// (step j takes k = 4 (lane / 16) + j), with the matching G rows read from LDS.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kCoarseK = 64;
 template <int LG>
-__global__ void __launch_bounds__(coarse_threads<LG>(), coarse_waves<LG>()) k_pcg_coarse(const JobDev* __restrict__ jobs,
-                                                                     const int2* __restrict__ js, int ch_rz,
-                                                                     int ch_rr, int in_iteration) {
-    constexpr int Rp = 1 << LG, NT = coarse_threads<LG>(), G = NT / Rp, PER = kCoarseRows / G, KU = 128;
-    static_assert(kCoarseRows % G == 0, "coarse rows per block must split over the row groups");
+__global__ void __launch_bounds__(256) k_pcg_coarse_mfma(const JobDev* __restrict__ jobs,
+                                                         const int2* __restrict__ js, int ch_rz, int ch_rr,
+                                                         int in_iteration) {
+    constexpr int Rp = 1 << LG, CT = Rp / 16, GLD = Rp + 4;
+    static_assert(kCoarseRows == 64, "four wavefronts of 16 coarse rows");
     const int2 pr = js[blockIdx.y];
     const JobDev& J = jobs[pr.x];
     const int s = pr.y;
-    // (at the start the slice's state is not set yet: every listed slice runs; jobs
-    // without a coarse level only form the chunk sums)
     if (J.Rp != Rp || (in_iteration && !slice_active(J, s)) || (!in_iteration && !J.cinv)) return;
     const int nt = J.ntiles, na = kAggPerTile * nt, ld = J.cld;
-    const int t0 = blockIdx.x * kCoarseRows;      // first coarse row (aggregate) of the chunk
-    if (t0 >= na) return;
-    const int lane = threadIdx.x & (Rp - 1), g = threadIdx.x >> LG;
-    __shared__ float gs[KU][Rp];
+    const int a0 = blockIdx.x * kCoarseRows;
+    if (a0 >= na) return;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r16 = l & 15, kb = l >> 4;
+    __shared__ float gs[kCoarseK][GLD];
+    __shared__ double red[2][16][Rp];
     const MPF_GLOBAL float* csc = gp((const float*)J.csc);
-    const MPF_GLOBAL float4* crow[PER];
+    v4f acc[CT];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int t = min(t0 + g + k * G, na - 1);   // rows past na compute a copy, discarded
-        crow[k] = J.cinv ? (const MPF_GLOBAL float4*)(gp(J.cinv) + ((size_t)s * na + t) * ld) : nullptr;
-    }
-    float acc[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
-    for (int u0 = 0; J.cinv && u0 < na; u0 += KU) {
-        const int un = min(KU, na - u0);
-        // all staging loads issued before the LDS writes
-        constexpr int NS = KU * Rp / NT;
-        static_assert(KU * Rp % NT == 0, "staging splits evenly over the block");
-        float gv[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int i = threadIdx.x + k * NT, u = i / Rp, l = i % Rp;
-            const int a = u0 + u;
+    for (int ct = 0; ct < CT; ++ct) acc[ct] = v4f{0.f, 0.f, 0.f, 0.f};
+    // rows past na compute a copy of the last row, discarded
+    const int arow = min(a0 + 16 * wave + r16, na - 1);
+    const MPF_GLOBAL float* crow = J.cinv ? gp(J.cinv) + ((size_t)s * na + arow) * ld : nullptr;
+    for (int u0 = 0; J.cinv && u0 < na; u0 += kCoarseK) {
+        const int un = min(kCoarseK, na - u0);
+        __syncthreads();   // the previous step's G has been read
+        for (int i = threadIdx.x; i < kCoarseK * Rp; i += 256) {
+            const int u = i / Rp, c = i % Rp, a = u0 + u;
             // in the iteration the update kernel left c_a g_a in FP32; at the start only
             // the FP64 partials exist
-            gv[k] = u >= un ? 0.0f
-                    : in_iteration ? gp((const float*)J.cg)[((size_t)s * na + a) * Rp + l]
-                                   : (float)((double)csc[a] * part_ptr(J, a & 1 ? 7 : 4, s, a >> 1)[l]);
+            gs[u][c] = u >= un ? 0.0f
+                       : in_iteration ? gp((const float*)J.cg)[((size_t)s * na + a) * Rp + c]
+                                      : (float)((double)csc[a] * part_ptr(J, agg_channel(a % kAggPerTile), s, a / kAggPerTile)[c]);
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int i = threadIdx.x + k * NT;
-            gs[i / Rp][i % Rp] = gv[k];
-        }
-        __syncthreads();
-        const int un4 = (un + 3) / 4;                // cinv rows are zero-padded to ld
-        // the thread's PER rows share each staged g (one LDS read for all of them)
-        float a[PER];
+        for (int k0 = 0; k0 < kCoarseK; k0 += 16) {
+            const int u = u0 + k0 + 4 * kb;   // cinv rows are zero-padded to ld, a multiple of 4
+            float4 c4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (u < ld) c4 = *(const MPF_GLOBAL float4*)(crow + u);
+            const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-        for (int k = 0; k < PER; ++k) a[k] = 0.0f;
-#pragma unroll 8
-        for (int u = 0; u < un4; ++u) {
-            const float g0 = gs[4 * u][lane], g1 = gs[4 * u + 1][lane], g2 = gs[4 * u + 2][lane],
-                        g3 = gs[4 * u + 3][lane];
+            for (int j = 0; j < 4; ++j) {
+                const float* grow = gs[k0 + 4 * kb + j];
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const float4 c = crow[k][u0 / 4 + u];
-                a[k] += c.x * g0 + c.y * g1 + c.z * g2 + c.w * g3;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k) acc[k] += a[k];
-    }
-    double rz = 0.0, rr = 0.0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int a = t0 + g + k * G, t = a >> 1;
-        if (a < na) {
-            if (!(a & 1)) {
-                rz += part_ptr(J, ch_rz, s, t)[lane];
-                rr += part_ptr(J, ch_rr, s, t)[lane];
-            }
-            if (J.cinv) {
-                const float y = csc[a] * acc[k];
-                gp(J.cy)[((size_t)s * na + a) * Rp + lane] = y;
-                rz += part_ptr(J, a & 1 ? 7 : 4, s, t)[lane] * (double)y;
+                for (int ct = 0; ct < CT; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv[j], grow[16 * ct + r16], acc[ct], 0, 0, 0);
             }
         }
     }
-    __shared__ double red[2][NT];
-    red[0][threadIdx.x] = rz;
-    red[1][threadIdx.x] = rr;
+    // lane (r16, kb) holds Y[16 w + 4 kb + v][16 ct + r16], v = 0..3
+    double rz[CT], rr[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        rz[ct] = 0.0;
+        rr[ct] = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int a = a0 + 16 * wave + 4 * kb + v, t = a / kAggPerTile, col = 16 * ct + r16;
+            if (a < na) {
+                if (a % kAggPerTile == 0) {
+                    rz[ct] += part_ptr(J, ch_rz, s, t)[col];
+                    rr[ct] += part_ptr(J, ch_rr, s, t)[col];
+                }
+                if (J.cinv) {
+                    const float y = csc[a] * acc[ct][v];
+                    gp(J.cy)[((size_t)s * na + a) * Rp + col] = y;
+                    rz[ct] += part_ptr(J, agg_channel(a % kAggPerTile), s, t)[col] * (double)y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        red[0][4 * wave + kb][16 * ct + r16] = rz[ct];
+        red[1][4 * wave + kb][16 * ct + r16] = rr[ct];
+    }
     __syncthreads();
-    if (g == 0) {
-        double a = 0.0, b = 0.0;
-        for (int gg = 0; gg < G; ++gg) { a += red[0][gg * Rp + lane]; b += red[1][gg * Rp + lane]; }
-        part_ptr(J, 5, s, blockIdx.x)[lane] = a;
-        part_ptr(J, 6, s, blockIdx.x)[lane] = b;
+    if ((int)threadIdx.x < 2 * Rp) {
+        const int c = threadIdx.x / Rp, col = threadIdx.x % Rp;
+        double v = 0.0;
+        for (int g = 0; g < 16; ++g) v += red[c][g][col];
+        part_ptr(J, 5 + c, s, blockIdx.x)[col] = v;
     }
 }
 
@@ -608,26 +613,23 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
     const JobDev& J = jobs[t.x];
     if (t.y != 0 || !J.csc) return;
     const int row = t.z * kTileRows + threadIdx.x;
-    const bool first = (int)threadIdx.x < tile_half(J, t.z);
     double v = 0.0;
     if (row < J.n) {
         const size_t e = (size_t)J.diag_slot[row] * J.np + row;
         v = (double)J.s[e] + (double)J.s_lo[e];
     }
-    __shared__ double red[2][kTileRows];
-    red[0][threadIdx.x] = first ? v : 0.0;
-    red[1][threadIdx.x] = first ? 0.0 : v;
+    // aggregate q of the tile = local rows [kAggRows q, kAggRows (q + 1)): a tree sum
+    // inside each kAggRows-row range
+    __shared__ double red[kTileRows];
+    red[threadIdx.x] = v;
     __syncthreads();
-    for (int h = kTileRows / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + h];
-            red[1][threadIdx.x] += red[1][threadIdx.x + h];
-        }
+    for (int h = kAggRows / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x % kAggRows < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x < 2) {
+    if ((int)threadIdx.x < kAggPerTile) {
         const int a = kAggPerTile * t.z + threadIdx.x;
-        const double sc = d1[a] > 0.0 ? red[threadIdx.x][0] / d1[a] : 0.0;
+        const double sc = d1[a] > 0.0 ? red[threadIdx.x * kAggRows] / d1[a] : 0.0;
         J.csc[a] = sc > 0.0 ? (float)(1.0 / sqrt(sc)) : 0.0f;
     }
 }
@@ -668,7 +670,8 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
     }
     {
         double out[2] = {0.0, 0.0};
-        // rho and r.r come as one partial per 32-tile chunk (k_pcg_coarse, channels 5, 6)
+        // rho and r.r come as one partial per kChunkTiles-tile chunk (k_pcg_coarse_mfma,
+        // channels 5, 6)
         const int cnt = coarse_chunks(J);
         const int c_rz = 5, c_rr = 6;
         if (book) {
@@ -740,9 +743,9 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
     // p_k is read from this iteration's buffer, p_{k+1} written to the next one
     const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)pbuf(J, k4)) + base;
     MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(pbuf(J, (k4 + 1) & (kRing - 1))) + base;
-    // two-level: z = D^-1 r + y_a on the real rows of aggregate a (the tile's first
-    // tile_half rows: 2t, the rest: 2t + 1)
-    const int h = t.z * kTileRows + tile_half(J, t.z);
+    // two-level: z = D^-1 r + y_a on the real rows of aggregate a (local rows
+    // [kAggRows q, kAggRows (q + 1)) of tile t: a = kAggPerTile t + q)
+    const int tile0 = t.z * kTileRows;
     if constexpr (XUP) {
         // k4 == kRing - 1: the cycle's x updates, oldest first; p of the cycle's first iteration sits where
         // p_{k+1} goes and is read before the write
@@ -776,7 +779,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
             X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
             const float d = dinv[row];
             const float w = row < J.n ? 1.0f : 0.0f;
-            const float4 y4 = sh_y[row < h ? 0 : 1][c4];
+            const float4 y4 = sh_y[(row - tile0) / kAggRows][c4];
             Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                                 zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                                 zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
@@ -789,7 +792,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
         const float4 r4 = Rr[i], p4 = Pk[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
-        const float4 y4 = sh_y[row < h ? 0 : 1][c4];
+        const float4 y4 = sh_y[(row - tile0) / kAggRows][c4];
         Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                             zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                             zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
@@ -991,7 +994,7 @@ void launch_dinv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStrea
 template <int LG>
 static void coarse_launch(const JobDev* d_jobs, const int2* d_js, int njs, int nchunk, int ch_rz, int ch_rr,
                           hipStream_t st) {
-    k_pcg_coarse<LG><<<dim3(nchunk, njs), coarse_threads<LG>(), 0, st>>>(d_jobs, d_js, ch_rz, ch_rr, ch_rz != 0);
+    k_pcg_coarse_mfma<LG><<<dim3(nchunk, njs), 256, 0, st>>>(d_jobs, d_js, ch_rz, ch_rr, ch_rz != 0);
 }
 
 // every lane count present in the list runs its own instantiation; blocks whose job has

@@ -47,7 +47,8 @@ def pcg(A, B, dc, tol=1e-5, max_iter=100000, threads=0, einv=None, tile=256, agg
     """Solve (A + dc 1 1^T) X = B with the C++ baseline; returns (X, steps, seconds).
     ``einv`` (na x na): the two-level preconditioner's scaled coarse inverse
     (``eit_oracle.two_level_preconditioner``) over the aggregates ``agg`` of the rows
-    (default: the halves of ``tile``-row tiles, ``eit_oracle.tile_aggregates``); None:
+    (default: the 4 runs of tile / 4 rows of each ``tile``-row tile,
+    ``eit_oracle.tile_aggregates``); None:
     Jacobi alone."""
     from .eit_oracle import tile_aggregates
     L = _lib()

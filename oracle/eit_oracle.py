@@ -419,13 +419,14 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     return x, iters
 
 
-def rcb_order(xy, leaf=256):
+def rcb_order(xy, leaf=256, agg=64):
     """Node order of recursive coordinate bisection: a set is split at the median of its
     longer coordinate extent (ties by node id), the first part holding a whole number of
-    ``leaf``-node leaves, until every leaf fits; each leaf is then ordered along its own
-    longer extent, so that its first ceil(n / 2) nodes and the rest are its two halves.
-    Consecutive ``leaf`` nodes of the order are compact tiles and their halves compact
-    aggregates (``tile_aggregates``; the product's device tiles are built the same way)."""
+    ``leaf``-node leaves, until every leaf fits; each leaf is then split the same way into
+    leaves of ``agg`` nodes, each ordered along its own longer extent.  Consecutive
+    ``leaf`` nodes of the order are compact tiles and their consecutive ``agg``-node runs
+    compact aggregates (``tile_aggregates``; the product's device tiles and aggregates are
+    built the same way)."""
     xy = np.asarray(xy, np.float64)
 
     def along(idx):
@@ -433,30 +434,30 @@ def rcb_order(xy, leaf=256):
         ax = 0 if np.ptp(pts[:, 0]) >= np.ptp(pts[:, 1]) else 1
         return idx[np.lexsort((idx, pts[:, ax]))]
 
-    def split(idx):
-        if len(idx) <= leaf:
+    def split(idx, lf):
+        if len(idx) <= lf:
             return [along(idx)] if len(idx) > 1 else [idx]
         o = along(idx)
-        left = (-(-len(idx) // leaf) // 2) * leaf
-        return split(o[:left]) + split(o[left:])
+        left = (-(-len(idx) // lf) // 2) * lf
+        return split(o[:left], lf) + split(o[left:], lf)
 
-    return np.concatenate(split(np.arange(xy.shape[0])))
-
-
-def tile_aggregates(N, tile=256):
-    """Coarse aggregate of each of N rows: the two halves of every ``tile``-row tile, the
-    first holding ceil(n_t / 2) of the tile's n_t rows (aggregate 2t) and the second the
-    rest (2t + 1) -- the product's layout.  A one-row last tile has no second half."""
-    i = np.arange(N)
-    t = i // tile
-    n_t = np.minimum(tile, N - t * tile)
-    return 2 * t + (i - t * tile >= (n_t + 1) // 2)
+    return np.concatenate([a for t in split(np.arange(xy.shape[0]), leaf) for a in split(t, agg)])
 
 
-def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256, halves=True):
+def tile_aggregates(N, tile=256, parts=4):
+    """Coarse aggregate of each of N rows: every ``tile``-row tile split into ``parts``
+    runs of tile / parts consecutive rows, run q of tile t being aggregate parts t + q --
+    the product's layout (64-row quarters of 256-row tiles).  Runs past the last row are
+    empty and get no number."""
+    assert tile % parts == 0
+    return np.arange(N) // (tile // parts)
+
+
+def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256, parts=4):
     """The product's two-level preconditioner (an extension; the reference has only the
-    diagonal one, preconditioner.cpp:59-76), on aggregates of consecutive rows: the two
-    halves of every ``tile``-row tile (``tile_aggregates``; ``halves=False``: whole tiles):
+    diagonal one, preconditioner.cpp:59-76), on aggregates of consecutive rows: the
+    ``parts`` runs of tile / parts rows of every ``tile``-row tile (``tile_aggregates``;
+    ``parts=1``: whole tiles):
 
         M^-1 r = D^-1 r + W C E1^-1 C W^T r
         D  = diag(A),  W = aggregate indicator (N x na)
@@ -468,7 +469,7 @@ def two_level_preconditioner(A, A1, S_diag, S1_diag, dc_free, tile=256, halves=T
     the dense N x N M^-1 when N <= 2000 (else None); prec.einv holds C E1^-1 C and
     prec.agg the aggregate of each row."""
     N = A.shape[0]
-    agg = tile_aggregates(N, tile) if halves else np.arange(N) // tile
+    agg = tile_aggregates(N, tile, parts)
     nt = int(agg.max()) + 1
     W = sp.csr_matrix((np.ones(N), (np.arange(N), agg)), shape=(N, nt))
     E1 = (W.T @ A1 @ W).toarray()

@@ -80,7 +80,7 @@ def _unique_columns(F):
 class GnHost:
     """Host reconstruction of a gap-model or complete-electrode-model EIT problem (the
     mesh renumbered by recursive coordinate bisection so that the two-level
-    preconditioner's aggregates are ranges of rows, as on the device)."""
+    preconditioner's aggregates are 64-row ranges, as on the device)."""
 
     def __init__(self, nodes, elements, boundary, ports, drive, meas, current, reference_value=1.0,
                  components=1, height=1.0, port_height=1.0, contact_impedance=0.0, two_level=True,
@@ -163,7 +163,7 @@ class GnHost:
         self.w_off = np.ascontiguousarray(Wc.indptr, np.int32)
         self.w_node = np.ascontiguousarray(Wc.indices, np.int32)
         self.w_val = np.ascontiguousarray(Wc.data, np.float64)
-        # two-level: aggregates = halves of 256-row tiles, coarse inverses at sigma = 1
+        # two-level: aggregates = 64-row quarters of 256-row tiles, coarse inverses at sigma = 1
         self.na = 0
         self.astart = np.zeros(1, np.int32)
         self.cinv = np.zeros(1, np.float32)

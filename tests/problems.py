@@ -35,10 +35,11 @@ def eit_of(p, ctx=None, frames=1, quirks=True, jacobian_exact=False, keep_node_o
 
 def rcb_renumbered(p, leaf=256):
     """The problem with its nodes renumbered by recursive coordinate bisection into
-    leaves of ``leaf`` consecutive nodes (compact aggregates for the two-level
-    preconditioner when the device keeps the caller's node order)."""
+    leaves of ``leaf`` consecutive nodes, each bisected into four runs of leaf / 4 (compact
+    aggregates for the two-level preconditioner when the device keeps the caller's node
+    order)."""
     from oracle.eit_oracle import rcb_order
-    order = rcb_order(p["nodes"], leaf)
+    order = rcb_order(p["nodes"], leaf, leaf // 4)
     perm = np.empty_like(order)
     perm[order] = np.arange(order.size)
     q = dict(p)

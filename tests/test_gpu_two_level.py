@@ -1,4 +1,4 @@
-"""The two-level preconditioner (Jacobi + coarse correction over the halves of the 256-row tiles; an
+"""The two-level preconditioner (Jacobi + coarse correction over the 64-row quarters of the 256-row tiles; an
 extension, the reference has only the diagonal preconditioner of
 src/numeric/preconditioner.cpp:59-76) on the device, through the C ABI: step counts
 against the oracle's restatement, voltages against the FP64 direct solve, and the same

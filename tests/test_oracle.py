@@ -655,14 +655,16 @@ def test_cem_approaches_shunt_limit():
 # ----------------------------------------------------------------------------------
 def _two_level_brute(A, A1, Sd, S1d, dc, tile):
     """Dense, loop-by-loop construction of D^-1 + W C E1^-1 C W^T over the aggregates
-    'first ceil(n / 2) rows of each tile' and 'the rest of the tile'."""
+    'rows [q tile / 4, (q + 1) tile / 4) of each tile', q = 0 .. 3."""
     N = A.shape[0]
     Ad, A1d = A.toarray(), A1.toarray()
     groups = []
     for start in range(0, N, tile):
         rows = list(range(start, min(N, start + tile)))
-        h = (len(rows) + 1) // 2
-        groups += [rows[:h]] + ([rows[h:]] if rows[h:] else [])
+        for q in range(4):
+            part = rows[q * tile // 4:(q + 1) * tile // 4]
+            if part:
+                groups.append(part)
     na = len(groups)
     a_of = [0] * N
     for a, rows in enumerate(groups):
@@ -706,7 +708,7 @@ def test_two_level_single_aggregate_closed_form(small):
     p, o, g = small
     A, A1 = o.system(g, 0), o.system(np.ones(o.E), 0)
     Sd, S1d = A.diagonal(), A1.diagonal()
-    prec, _ = eo.two_level_preconditioner(A, A1, Sd, S1d, True, tile=o.N, halves=False)
+    prec, _ = eo.two_level_preconditioner(A, A1, Sd, S1d, True, tile=o.N, parts=1)
     r = np.random.default_rng(5).standard_normal((o.N, 2))
     c2 = S1d.sum() / Sd.sum()
     want = r / A.diagonal()[:, None] + c2 * np.ones((o.N, 1)) * r.sum(axis=0, keepdims=True) / o.N ** 2
