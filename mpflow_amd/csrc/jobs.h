@@ -63,17 +63,18 @@ struct JobDev {
     double ref; int logarithmic; int E;
     const int32_t* diag_slot;    // [np]
     const uint8_t* gw;           // [np / 16] entry count of the widest row of each 16-row group
-    // two-level preconditioner (CG only; null: Jacobi alone).  The aggregates are the
-    // two halves of each 256-row tile (aggregate 2t: the tile's first ceil(n_t / 2) real
-    // rows, 2t + 1: the rest): M^-1 r = D^-1 r + W Einv W^T r with W the aggregate
-    // indicator, Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W,
-    // sigma = 1, plus n n^T on the dcFree slice; Sc: per-aggregate conductivity scale of
-    // this frame).  W^T r of the first halves is partial channel 4, of the second 7.
-    const float* cinv;           // [C][2 ntiles][cld], rows zero-padded to cld = ceil4(2 ntiles)
+    // two-level preconditioner (CG only; null: Jacobi alone).  Aggregate 4t + q is the set
+    // of tile t's real rows with aggq = q (its four 64-node bisection leaves):
+    // M^-1 r = D^-1 r + W Einv W^T r with W the aggregate indicator,
+    // Einv = Sc^-1/2 cinv_s Sc^-1/2 (cinv_s: inverse of W^T A_s(1) W, sigma = 1, plus n n^T
+    // on the dcFree slice; Sc: per-aggregate conductivity scale of this frame).  W^T r of
+    // aggregate q of a tile is partial channel agg_channel(q).
+    const uint8_t* aggq;         // [np] aggregate of each row within its tile
+    const float* cinv;           // [C][4 ntiles][cld], rows zero-padded to cld = ceil4(4 ntiles)
     int cld;
-    float* csc;                  // [2 ntiles] Sc^-1/2 (written after each assembly)
-    float* cy;                   // [C][2 ntiles][Rp] coarse correction Einv W^T r
-    float* cg;                   // [C][2 ntiles][Rp] Sc^-1/2 W^T r of the current iteration
+    float* csc;                  // [4 ntiles] Sc^-1/2 (written after each assembly)
+    float* cy;                   // [C][4 ntiles][Rp] coarse correction Einv W^T r
+    float* cg;                   // [C][4 ntiles][Rp] Sc^-1/2 W^T r of the current iteration
 };
 
 // PCG search direction of iteration k (k4 = k % kRing)

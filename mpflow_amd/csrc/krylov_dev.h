@@ -409,10 +409,14 @@ __device__ __forceinline__ int coarse_chunks(const JobDev& J) {
     return (J.ntiles + kChunkTiles - 1) / kChunkTiles;
 }
 
-// Coarse aggregates: local rows [kAggRows q, kAggRows (q + 1)) of tile t form aggregate
-// kAggPerTile t + q (the device numbering bisects every tile into kAggRows-row leaves).
-// The FP64 tile partials of W^T r for quarter q sit in channel agg_channel(q).
+// Coarse aggregates: the rows of tile t with aggq = q form aggregate kAggPerTile t + q
+// (the device numbering bisects every tile into kAggRows-node leaves; a job without the
+// table takes local rows [kAggRows q, kAggRows (q + 1))).  The FP64 tile partials of
+// W^T r for quarter q sit in channel agg_channel(q).
 __device__ __forceinline__ constexpr int agg_channel(int q) { return q == 0 ? 4 : 6 + q; }
+__device__ __forceinline__ int row_agg(const JobDev& J, int row) {
+    return J.aggq ? (int)J.aggq[row] : (row % kTileRows) / kAggRows;
+}
 static_assert(kAggPerTile == 4 && kAggPerTile * kAggRows == kTileRows,
               "aggregate channels 4, 7, 8, 9: four quarters per tile");
 

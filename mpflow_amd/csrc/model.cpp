@@ -44,6 +44,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
     ids.alloc((size_t)W * np); ids.upload(S.ids.data(), S.ids.size(), st0);
     diag_slot.alloc(np); diag_slot.upload(S.diag_slot.data(), np, st0);
     gw.alloc(np / 16); gw.upload(S.group_width.data(), np / 16, st0);
+    aggq.alloc(np); aggq.upload(S.agg_q.data(), np, st0);
     ctb_off.alloc(S.ctb_off.size()); ctb_off.upload(S.ctb_off.data(), S.ctb_off.size(), st0);
     ctb_elem.alloc(S.ctb_elem.size()); ctb_elem.upload(S.ctb_elem.data(), S.ctb_elem.size(), st0);
     ctb_k.alloc(S.ctb_k.size()); ctb_k.upload(S.ctb_k.data(), S.ctb_k.size(), st0);
@@ -161,6 +162,7 @@ Model::Model(mpf_ctx* c, const mpf_model_desc& d) : ctx(c) {
         J.ref = S.ref; J.logarithmic = S.logarithmic; J.E = S.E;
         J.diag_slot = diag_slot.p;
         J.gw = gw.p;
+        J.aggq = aggq.p;
         if (nct) {
             J.cinv = cinv.p;
             J.cld = S.coarse_ld;

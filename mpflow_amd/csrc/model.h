@@ -38,7 +38,7 @@ struct Model {
     DBuf<int32_t> cpl_off, cpl_node, cpl_elem, drow_off, drow_node, drow_elem;
     DBuf<double> cpl_k, cpl_m, drow_k, drow_m, fslice, cur;
     DBuf<int32_t> node_of_row;   // [np] device row -> caller's node (identity past the mesh nodes)
-    DBuf<uint8_t> gw;
+    DBuf<uint8_t> gw, aggq;
     DBuf<double> ctb_k, ctb_m, F, w_val, colscale;
     DBuf<float> cinv, csc, cy, cg;          // two-level preconditioner (allocated if the mesh has one)
     DBuf<double> cd1;

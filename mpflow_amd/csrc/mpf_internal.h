@@ -15,7 +15,7 @@ namespace mpf {
 constexpr int kMaxSlices = 8;      // harmonics per job
 constexpr int kTileRows = 256;     // rows per work tile of the sparse kernels
 constexpr int kAggPerTile = 4;     // coarse aggregates per tile: its four 64-row geometric quarters
-constexpr int kAggRows = 64;       // rows per coarse aggregate (local rows [64 q, 64 q + 64) of a tile)
+constexpr int kAggRows = 64;       // rows per coarse aggregate (a tile's 64-node bisection leaves)
 constexpr int kCoarseRows = 64;    // coarse rows per block of the coarse-correction kernel
 constexpr int kChunkTiles = kCoarseRows / kAggPerTile;   // tiles per partial-sum chunk
 constexpr int kCoarseMaxTiles = 512;   // largest tile count per job with a coarse level
@@ -89,6 +89,8 @@ struct MeshSetup {
     std::vector<uint8_t> group_width;             // [np / 16]
     // device numbering: perm[caller node] = device row
     std::vector<int32_t> perm;
+    // [np] coarse aggregate of each row within its tile (0 .. kAggPerTile - 1)
+    std::vector<uint8_t> agg_q;
     // gather plan: contributions of (element, local ij) to ELL entry slot*np+row
     std::vector<int32_t> ctb_off;                 // [wpad*np + 1]
     std::vector<int32_t> ctb_elem;                // [E*k*k]

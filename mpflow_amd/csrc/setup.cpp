@@ -205,12 +205,13 @@ static void edge_integrals(int kb, const double* sp, double (*m)[3], double* v) 
 // 256-node leaves, until every leaf fits one 256-row tile.  Each tile is then a compact
 // patch of the mesh: most gathers of its SpMV rows stay inside it, consecutive tiles are
 // neighbouring patches (the XCD-striped work lists keep them in one L2).  Each tile is
-// then bisected the same way into leaves of kAggRows rows (a part holding a whole number
-// of them first): the aggregates of the two-level preconditioner, local rows
-// [kAggRows q, kAggRows (q + 1)).  Rows are sorted by descending degree inside each
-// aggregate so the 16-row groups of the SpMV have near-uniform ELL widths.  Returns
-// perm[node] = row.
-static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, const int32_t* el) {
+// then bisected the same way into leaves of kAggRows nodes (a part holding a whole number
+// of them first): the kAggPerTile aggregates of the two-level preconditioner, quarter[node]
+// = the leaf's index in its tile.  The tile's rows are then sorted by descending degree
+// (the aggregates are sets, not row ranges) so the 16-row groups of the SpMV have
+// near-uniform ELL widths.  Returns perm[node] = row.
+static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, const int32_t* el,
+                                         std::vector<uint8_t>& quarter) {
     std::vector<int32_t> deg(N, 0);
     {
         std::vector<std::vector<int32_t>> nb(N);
@@ -257,11 +258,12 @@ static std::vector<int32_t> device_order(int N, const double* xy, int E, int k, 
     };
     split(0, N, kTileRows);
     auto by_degree = [&](int a, int c) { return deg[a] != deg[c] ? deg[a] > deg[c] : a < c; };
+    quarter.assign(N, 0);
     for (int w = 0; w < N; w += kTileRows) {
         const int e = std::min(N, w + kTileRows);
         split(w, e, kAggRows);
-        for (int b = w; b < e; b += kAggRows)
-            std::sort(order.begin() + b, order.begin() + std::min(e, b + kAggRows), by_degree);
+        for (int i = w; i < e; ++i) quarter[order[i]] = (uint8_t)((i - w) / kAggRows);
+        std::sort(order.begin() + w, order.begin() + e, by_degree);
     }
     std::vector<int32_t> perm(N);
     for (int r = 0; r < N; ++r) perm[order[r]] = r;
@@ -317,9 +319,9 @@ static bool spd_inverse(int n, double* A) {
     return true;
 }
 
-// Coarse level of the two-level preconditioner (aggregates = the kAggRows-row leaves of
-// each 256-row tile, aggregate kAggPerTile t + q holding its local rows
-// [kAggRows q, kAggRows (q + 1)); W = the aggregate indicator): per harmonic, the inverse
+// Coarse level of the two-level preconditioner (aggregates = the kAggRows-node leaves of
+// each 256-row tile, aggregate kAggPerTile t + q holding the rows with agg_q = q; W = the
+// aggregate indicator): per harmonic, the inverse
 // of W^T A_c(1) W at sigma = 1 (plus
 // n n^T, n = real rows per aggregate, on the dcFree harmonic 0, matching the A + 1 1^T
 // operator), and per aggregate the sum of the sigma = 1 stiffness diagonal that scales
@@ -334,7 +336,7 @@ static void coarse_setup(MeshSetup& s) {
         for (int i = s.ctb_off[ent]; i < s.ctb_off[ent + 1]; ++i) { sk[ent] += s.ctb_k[i]; sm[ent] += s.ctb_m[i]; }
     // aggregate of each real row
     std::vector<int> agg(s.N);
-    for (int row = 0; row < s.N; ++row) agg[row] = row / kAggRows;
+    for (int row = 0; row < s.N; ++row) agg[row] = kAggPerTile * (row / kTileRows) + s.agg_q[row];
     std::vector<double> cnt(na, 0.0);
     for (int row = 0; row < s.N; ++row) cnt[agg[row]] += 1.0;
     s.coarse_d1.assign(na, 0.0);
@@ -463,12 +465,17 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     mpf_model_desc d = caller;
     std::vector<double> nodes_r;
     std::vector<int32_t> el_r, bd_r;
+    // aggregate of each caller node within its tile (kept order: consecutive kAggRows runs)
+    std::vector<uint8_t> quarter(caller.n_nodes);
     if (caller.keep_node_order) {
         s.perm.resize(caller.n_nodes);
-        for (int n = 0; n < caller.n_nodes; ++n) s.perm[n] = n;
+        for (int n = 0; n < caller.n_nodes; ++n) {
+            s.perm[n] = n;
+            quarter[n] = (uint8_t)((n % kTileRows) / kAggRows);
+        }
     } else {
         s.perm = device_order(caller.n_nodes, caller.nodes, caller.n_elements, caller.nodes_per_element,
-                              caller.elements);
+                              caller.elements, quarter);
         nodes_r.resize((size_t)2 * caller.n_nodes);
         for (int n = 0; n < caller.n_nodes; ++n) {
             nodes_r[2 * (size_t)s.perm[n]] = caller.nodes[2 * (size_t)n];
@@ -501,6 +508,11 @@ void build_setup(const mpf_model_desc& caller, MeshSetup& s) {
     const int Nu = N + (cem && !coupled ? d.n_ports : 0);
     s.N = Nu; s.E = E; s.k = k;
     s.np = round_up(Nu, kTileRows);
+    // aggregate within the tile of every row: the mesh nodes' from the ordering, the
+    // electrode rows (harmonic-wise electrode model) and padding rows by position
+    s.agg_q.resize(s.np);
+    for (int r = 0; r < s.np; ++r) s.agg_q[r] = (uint8_t)((r % kTileRows) / kAggRows);
+    for (int n = 0; n < N; ++n) s.agg_q[s.perm[n]] = quarter[n];
     s.P = d.n_ports; s.D = d.n_drive; s.M = d.n_meas; s.R = s.D + s.M;
     MPF_REQUIRE(s.R <= 64, "model: D + M > 64 is not supported");
     // Pi = [I*Dpat | Mpat] (sources.cpp:57-71); columns that are scalar multiples of an

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init(const JobDev* __restrict_
         J.p[i] = z;
         rz += (double)b * (double)z;
         bb += (double)b * (double)b;
-        const int q = (row - tile0) / kAggRows;
+        const int q = row_agg(J, row);
 #pragma unroll
         for (int a = 0; a < kAggPerTile; ++a) bs[a] += a == q ? (double)b : 0.0;
     }
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_p(const JobDev* __restric
     const size_t cb = ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * J.Rp + L.lane;
     const int tile0 = t.z * kTileRows;
     for (int row = tile0 + L.g; row < min(tile0 + kTileRows, J.n); row += L.G)
-        J.p[vec_index(J, s, row, L.lane)] += J.cy[cb + (size_t)((row - tile0) / kAggRows) * J.Rp];
+        J.p[vec_index(J, s, row, L.lane)] += J.cy[cb + (size_t)row_agg(J, row) * J.Rp];
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
@@ -417,40 +417,30 @@ __global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __re
     MPF_GLOBAL float4* Rr = (MPF_GLOBAL float4*)gp(J.rv) + base;
     double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     // W^T r per aggregate (read in FP32 by the coarse kernel): summed in FP32 within a
-    // wavefront, in FP64 across the block.  ROWS divides kAggRows: the thread's rows of
-    // aggregate q are passes kk of the inner loop.
-    static_assert(kAggRows % ROWS == 0, "a pass of the row loop stays inside one aggregate");
+    // wavefront, in FP64 across the block
     float g[kAggPerTile][4];
 #pragma unroll
     for (int q = 0; q < kAggPerTile; ++q)
 #pragma unroll
         for (int j = 0; j < 4; ++j) g[q][j] = 0.0f;
-#pragma unroll 1
-    for (int q = 0; q < kAggPerTile; ++q) {
-        float gq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int row = t.z * kTileRows + r0; row < (t.z + 1) * kTileRows; row += ROWS) {
+        const size_t i = (size_t)row * LPR + c4;
+        const float4 q4 = Q[i];
+        const float4 r4 = Rr[i];
+        const float d = dinv[row];
+        const float dc = row < J.n ? 1.0f : 0.0f;
+        const int qa = row_agg(J, row);
+        float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
 #pragma unroll
-        for (int kk = 0; kk < kAggRows / ROWS; ++kk) {
-            const int row = t.z * kTileRows + q * kAggRows + r0 + kk * ROWS;
-            const size_t i = (size_t)row * LPR + c4;
-            const float4 q4 = Q[i];
-            const float4 r4 = Rr[i];
-            const float d = dinv[row];
-            const float dc = row < J.n ? 1.0f : 0.0f;
-            float qv[4] = {q4.x, q4.y, q4.z, q4.w}, rv[4] = {r4.x, r4.y, r4.z, r4.w};
+        for (int j = 0; j < 4; ++j) {
+            rv[j] = zero[j] ? 0.0f : rv[j] - a[j] * (qv[j] + dc * dcv[j]);
+            const float z = d * rv[j];
+            v[0][j] += (double)rv[j] * (double)z;
+            v[1][j] += (double)rv[j] * (double)rv[j];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                rv[j] = zero[j] ? 0.0f : rv[j] - a[j] * (qv[j] + dc * dcv[j]);
-                const float z = d * rv[j];
-                v[0][j] += (double)rv[j] * (double)z;
-                v[1][j] += (double)rv[j] * (double)rv[j];
-                gq[j] += rv[j];
-            }
-            Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
+            for (int q = 0; q < kAggPerTile; ++q) g[q][j] += q == qa ? rv[j] : 0.0f;
         }
-#pragma unroll
-        for (int qq = 0; qq < kAggPerTile; ++qq)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) g[qq][j] = qq == q ? gq[j] : g[qq][j];
+        Rr[i] = make_float4(rv[0], rv[1], rv[2], rv[3]);
     }
     // channels agg_channel(q): W^T r of the tile's aggregates (two-level preconditioner),
     // with a coarse level also c_a (W^T r)_a in FP32 for the coarse kernel's staging
@@ -618,18 +608,21 @@ __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __rest
         const size_t e = (size_t)J.diag_slot[row] * J.np + row;
         v = (double)J.s[e] + (double)J.s_lo[e];
     }
-    // aggregate q of the tile = local rows [kAggRows q, kAggRows (q + 1)): a tree sum
-    // inside each kAggRows-row range
-    __shared__ double red[kTileRows];
-    red[threadIdx.x] = v;
+    // per aggregate of the tile (the rows with aggq = q): tree sums over the tile
+    const int qa = row_agg(J, row);
+    __shared__ double red[kAggPerTile][kTileRows];
+#pragma unroll
+    for (int q = 0; q < kAggPerTile; ++q) red[q][threadIdx.x] = q == qa ? v : 0.0;
     __syncthreads();
-    for (int h = kAggRows / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x % kAggRows < h) red[threadIdx.x] += red[threadIdx.x + h];
+    for (int h = kTileRows / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < kAggPerTile; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
         __syncthreads();
     }
     if ((int)threadIdx.x < kAggPerTile) {
         const int a = kAggPerTile * t.z + threadIdx.x;
-        const double sc = d1[a] > 0.0 ? red[threadIdx.x * kAggRows] / d1[a] : 0.0;
+        const double sc = d1[a] > 0.0 ? red[threadIdx.x][0] / d1[a] : 0.0;
         J.csc[a] = sc > 0.0 ? (float)(1.0 / sqrt(sc)) : 0.0f;
     }
 }
@@ -743,9 +736,8 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
     // p_k is read from this iteration's buffer, p_{k+1} written to the next one
     const MPF_GLOBAL float4* Pk = (const MPF_GLOBAL float4*)gp((const float*)pbuf(J, k4)) + base;
     MPF_GLOBAL float4* Pn = (MPF_GLOBAL float4*)gp(pbuf(J, (k4 + 1) & (kRing - 1))) + base;
-    // two-level: z = D^-1 r + y_a on the real rows of aggregate a (local rows
-    // [kAggRows q, kAggRows (q + 1)) of tile t: a = kAggPerTile t + q)
-    const int tile0 = t.z * kTileRows;
+    // two-level: z = D^-1 r + y_a on the real rows of aggregate a (a = kAggPerTile t + q
+    // for the rows of tile t with aggq = q)
     if constexpr (XUP) {
         // k4 == kRing - 1: the cycle's x updates, oldest first; p of the cycle's first iteration sits where
         // p_{k+1} goes and is read before the write
@@ -779,7 +771,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
             X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
             const float d = dinv[row];
             const float w = row < J.n ? 1.0f : 0.0f;
-            const float4 y4 = sh_y[(row - tile0) / kAggRows][c4];
+            const float4 y4 = sh_y[row_agg(J, row)][c4];
             Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                                 zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                                 zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
@@ -792,7 +784,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
         const float4 r4 = Rr[i], p4 = Pk[i];
         const float d = dinv[row];
         const float w = row < J.n ? 1.0f : 0.0f;
-        const float4 y4 = sh_y[(row - tile0) / kAggRows][c4];
+        const float4 y4 = sh_y[row_agg(J, row)][c4];
         Pn[i] = make_float4(zero[0] ? 0.0f : d * r4.x + w * y4.x + b[0] * p4.x,
                             zero[1] ? 0.0f : d * r4.y + w * y4.y + b[1] * p4.y,
                             zero[2] ? 0.0f : d * r4.z + w * y4.z + b[2] * p4.z,
