@@ -77,19 +77,145 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(int M, int N, int K, const flo
     }
 }
 
+// The same product for operands whose unit-stride dimensions, sizes and pitches are
+// multiples of 4 floats: each 64 x 64 tile pair is staged 64 K at a time with 128-bit
+// loads along each operand's unit-stride dimension (AK: A(m, k) contiguous in k, else in
+// m; BK: B(k, n) contiguous in k, else in n), converted to BF16 and written to one of two
+// LDS buffers while the MFMAs consume the other.  The MFMA sequence per output (k chunks
+// of 16 in order) is the one k_mlp_gemm runs, so both give the same sums.
+constexpr int kGK = 64;                // K step of the staged kernel
+constexpr int kGL = kGK + 8;           // LDS row: 72 BF16 (144 bytes, 16-byte aligned)
+template <bool AK, bool BK>
+__global__ void __launch_bounds__(256) k_mlp_gemm4(int M, int N, int K, const float* __restrict__ A, long sam,
+                                                   long sak, const float* __restrict__ B, long sbk, long sbn,
+                                                   float* __restrict__ C, int ldc, int epi,
+                                                   const float* __restrict__ bias, float* __restrict__ H) {
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][kMT][kGL];   // [m][k]
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][kMT][kGL];   // [n][k]
+    const int m0 = blockIdx.y * kMT, n0 = blockIdx.x * kMT;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // a tile is 64 x 64 floats = 1024 float4: four per thread.  Contiguous in k: float4 u
+    // of thread t covers row (t + 256 u) / 16 and k from 4 ((t + 256 u) % 16).  Contiguous
+    // in m (or n): thread t loads rows k = 4 (t / 16) + u, u = 0..3, of the four columns
+    // from 4 (t % 16), a 4 x 4 block it writes to LDS transposed, four k per 8-byte store.
+    float4 ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (AK) {
+                const int f = tid + 256 * u, r = f >> 4, c = 4 * (f & 15);   // r = m, c = k
+                if (m0 + r < M && k0 + c < K) v = *(const float4*)(A + (size_t)(m0 + r) * sam + (k0 + c));
+            } else {
+                const int k = 4 * (tid >> 4) + u, c = 4 * (tid & 15);        // c = m
+                if (k0 + k < K && m0 + c < M) v = *(const float4*)(A + (size_t)(k0 + k) * sak + (m0 + c));
+            }
+            ra[u] = v;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (BK) {
+                const int f = tid + 256 * u, r = f >> 4, c = 4 * (f & 15);   // r = n, c = k
+                if (n0 + r < N && k0 + c < K) w = *(const float4*)(B + (size_t)(n0 + r) * sbn + (k0 + c));
+            } else {
+                const int k = 4 * (tid >> 4) + u, c = 4 * (tid & 15);        // c = n
+                if (k0 + k < K && n0 + c < N) w = *(const float4*)(B + (size_t)(k0 + k) * sbk + (n0 + c));
+            }
+            rb[u] = w;
+        }
+    };
+    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+    auto store = [&](int buf) {
+        if (AK) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int f = tid + 256 * u, r = f >> 4, c = 4 * (f & 15);
+                *reinterpret_cast<bf16x4_t*>(&sa[buf][r][c]) =
+                    bf16x4_t{(__bf16)ra[u].x, (__bf16)ra[u].y, (__bf16)ra[u].z, (__bf16)ra[u].w};
+            }
+        } else {
+            const int kq = 4 * (tid >> 4), c = 4 * (tid & 15);
+            *reinterpret_cast<bf16x4_t*>(&sa[buf][c][kq]) =
+                bf16x4_t{(__bf16)ra[0].x, (__bf16)ra[1].x, (__bf16)ra[2].x, (__bf16)ra[3].x};
+            *reinterpret_cast<bf16x4_t*>(&sa[buf][c + 1][kq]) =
+                bf16x4_t{(__bf16)ra[0].y, (__bf16)ra[1].y, (__bf16)ra[2].y, (__bf16)ra[3].y};
+            *reinterpret_cast<bf16x4_t*>(&sa[buf][c + 2][kq]) =
+                bf16x4_t{(__bf16)ra[0].z, (__bf16)ra[1].z, (__bf16)ra[2].z, (__bf16)ra[3].z};
+            *reinterpret_cast<bf16x4_t*>(&sa[buf][c + 3][kq]) =
+                bf16x4_t{(__bf16)ra[0].w, (__bf16)ra[1].w, (__bf16)ra[2].w, (__bf16)ra[3].w};
+        }
+        if (BK) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int f = tid + 256 * u, r = f >> 4, c = 4 * (f & 15);
+                *reinterpret_cast<bf16x4_t*>(&sb[buf][r][c]) =
+                    bf16x4_t{(__bf16)rb[u].x, (__bf16)rb[u].y, (__bf16)rb[u].z, (__bf16)rb[u].w};
+            }
+        } else {
+            const int kq = 4 * (tid >> 4), c = 4 * (tid & 15);
+            *reinterpret_cast<bf16x4_t*>(&sb[buf][c][kq]) =
+                bf16x4_t{(__bf16)rb[0].x, (__bf16)rb[1].x, (__bf16)rb[2].x, (__bf16)rb[3].x};
+            *reinterpret_cast<bf16x4_t*>(&sb[buf][c + 1][kq]) =
+                bf16x4_t{(__bf16)rb[0].y, (__bf16)rb[1].y, (__bf16)rb[2].y, (__bf16)rb[3].y};
+            *reinterpret_cast<bf16x4_t*>(&sb[buf][c + 2][kq]) =
+                bf16x4_t{(__bf16)rb[0].z, (__bf16)rb[1].z, (__bf16)rb[2].z, (__bf16)rb[3].z};
+            *reinterpret_cast<bf16x4_t*>(&sb[buf][c + 3][kq]) =
+                bf16x4_t{(__bf16)rb[0].w, (__bf16)rb[1].w, (__bf16)rb[2].w, (__bf16)rb[3].w};
+        }
+    };
+    f32x16_t acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    const int nk = (K + kGK - 1) / kGK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * kGK);
+#pragma unroll
+        for (int ks = 0; ks < kGK; ks += 16) {
+            const int kq = ks + (lane >> 5) * 8;
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&sa[buf][wm + (lane & 31)][kq]);
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&sb[buf][wn + (lane & 31)][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        if (kt + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = m0 + wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int col = n0 + wn + (lane & 31);
+        if (row < M && col < N) {
+            float z = acc[v];
+            if (epi >= 1) z += bias[col];
+            C[(size_t)row * ldc + col] = z;
+            if (epi == 1) H[(size_t)row * ldc + col] = gelu(z);
+        }
+    }
+}
+
 // dZ = dH * GELU'(Z) elementwise (in place on dH)
 __global__ void k_mlp_gelu_back(size_t n, const float* __restrict__ Z, float* __restrict__ dH) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dH[i] *= gelu_grad(Z[i]);
 }
 
-// bias gradient: db[n] = sum over rows of dZ (fixed row order)
-__global__ void k_mlp_colsum(int M, int N, const float* __restrict__ dZ, float* __restrict__ db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// bias gradient: db[n] = sum over rows of dZ.  A 256-thread block owns 64 columns: row
+// group g (0..3) sums rows g, g + 4, ... (loads coalesced along the columns), then the
+// four group sums are added in order (a fixed summation order).
+__global__ void __launch_bounds__(256) k_mlp_colsum(int M, int N, const float* __restrict__ dZ,
+                                                    float* __restrict__ db) {
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+#pragma unroll 8
+        for (int m = g; m < M; m += 4) s += dZ[(size_t)m * N + n];
+    __shared__ float red[4][64];
+    red[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) db[n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
 }
 
 // MSE: dY = 2 (Y - T) / (M N); per-block partial sums of (Y - T)^2 in FP64
@@ -111,12 +237,19 @@ __global__ void __launch_bounds__(256) k_mlp_mse(size_t n, const float* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_mlp_sum(int n, const double* __restrict__ part, double scale, double* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += part[i];
-        *out = s * scale;
+// out = scale * sum of the n partials: thread t sums t, t + 256, ..., then a tree sum
+__global__ void __launch_bounds__(256) k_mlp_sum(int n, const double* __restrict__ part, double scale,
+                                                 double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = red[0] * scale;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction folded into the step size
@@ -138,7 +271,22 @@ __global__ void k_mlp_adam(size_t n, float* __restrict__ w, const float* __restr
 static void gemm(int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk, long sbn,
                  float* C, int ldc, int epi, const float* bias, float* H, hipStream_t st) {
     dim3 grid((N + kMT - 1) / kMT, (M + kMT - 1) / kMT);
-    k_mlp_gemm<<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
+    // the staged kernel needs each operand contiguous along k or along its other index,
+    // with sizes and the other stride multiples of 4 floats and 16-byte aligned bases
+    const bool ak = sak == 1, bk = sbk == 1;
+    const long pa = ak ? sam : sak, pb = bk ? sbn : sbk;
+    const bool vec = (ak || sam == 1) && (bk || sbn == 1) && M % 4 == 0 && N % 4 == 0 && K % 4 == 0 &&
+                     pa % 4 == 0 && pb % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+    if (!vec)
+        k_mlp_gemm<<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
+    else if (ak && bk)
+        k_mlp_gemm4<true, true><<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
+    else if (ak)
+        k_mlp_gemm4<true, false><<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
+    else if (bk)
+        k_mlp_gemm4<false, true><<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
+    else
+        k_mlp_gemm4<false, false><<<grid, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, epi, bias, H);
     MPF_HIP(hipGetLastError());
 }
 
@@ -213,7 +361,7 @@ double Mlp::backward(const float* T) {
     // dZ_L = 2 (Y - T) / (B * out) into dA
     const int nb = (int)std::min<size_t>(1024, (ny + 255) / 256);
     k_mlp_mse<<<nb, 256, 0, st>>>(ny, Z[L - 1].p, T, 1.0f / (float)ny, dA.p, part.p);
-    k_mlp_sum<<<1, 64, 0, st>>>(nb, part.p, 1.0 / (double)ny, loss.p);
+    k_mlp_sum<<<1, 256, 0, st>>>(nb, part.p, 1.0 / (double)ny, loss.p);
     float* dz = dA.p;
     float* dh = dB.p;
     for (int l = L - 1; l >= 0; --l) {
@@ -221,7 +369,7 @@ double Mlp::backward(const float* T) {
         const float* hin = l == 0 ? x_in : H[l - 1].p;
         // dW = H_in^T dZ ; db = column sums of dZ
         gemm(ni, no, B, hin, 1, ni, dz, no, 1, gW[l].p, no, 0, nullptr, nullptr, st);
-        k_mlp_colsum<<<(no + 255) / 256, 256, 0, st>>>(B, no, dz, gb[l].p);
+        k_mlp_colsum<<<(no + 63) / 64, 256, 0, st>>>(B, no, dz, gb[l].p);
         if (l > 0) {
             // dH_in = dZ W^T, then dZ_in = dH_in * GELU'(Z_in)
             gemm(B, ni, no, dz, no, 1, W[l].p, 1, no, dh, ni, 0, nullptr, nullptr, st);
