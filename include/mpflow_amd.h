@@ -222,7 +222,8 @@ typedef struct mpf_solve_opts {
                                      preconditioner (preconditioner.cpp:59-76).  TWO_LEVEL
                                      (CG on model solves; no counterpart in the reference):
                                      Jacobi plus an additive coarse correction over compact
-                                     aggregates (the two halves of each 256-node tile),
+                                     aggregates (the four 64-node bisection leaves of each
+                                     256-node tile),
                                      M^-1 = D^-1 + W Ec^-1 W^T with Ec the aggregates'
                                      sigma = 1 coarse matrix scaled per aggregate to the
                                      frame's conductivity; several times fewer iterations on
@@ -269,8 +270,9 @@ int mpf_eit_solve(mpf_model* m, const float* d_gamma, float* d_U, double* d_U64,
 /* One Solver::solveAbsolute per frame (src/solver/solver.cpp:195-225):
  * forward solve + Jacobian at gamma, r = U_meas - U(gamma), then
  * gamma += (J^T J + (ref*lambda)^2 I)^-1 J^T r, solved in the m x m data space
- * (J J^T on MFMA, BF16 in / FP32 accumulate, FP64 Cholesky, FP64-residual
- * refinement `gn_refine` times).
+ * (J J^T on MFMA, BF16 in / FP32 accumulate, FP64 Cholesky, at most `gn_refine`
+ * FP64-residual refinement passes, fewer once every frame's predicted remaining error
+ * is below 1e-10 of its solution).
  *   d_gamma  : frames x E (device float, in/out)
  *   d_Umeas  : frames x m (device double)
  *   d_Ucalc  : frames x m (device double, out, nullable) voltages at the input gamma */
@@ -333,8 +335,9 @@ int mpf_gn_set_regularization(mpf_model* m, int32_t type, const float* d_gamma_r
  * d_gamma_ref (E floats), once.  mpf_diff_solve: for every frame's measurement
  *   d_Umeas : frames x m (FP64, EIT::result layout)
  *   d_gamma : frames x E (out) = gamma_ref + (J^T J + (ref lambda)^2 R)^-1 J^T (U_meas - U(gamma_ref))
- * with gn_refine FP64-residual passes: no forward solve, one triangular solve pair and
- * three GEMVs per frame and pass.  Both synchronise. */
+ * with at most gn_refine FP64-residual passes (fewer once converged, as in mpf_gn_step):
+ * no forward solve; per frame and pass one triangular solve pair and one pass over J
+ * (J J^T y in one sweep, J rows <= 1024).  Both synchronise. */
 int mpf_diff_prepare(mpf_model* m, const float* d_gamma_ref, double lambda, const mpf_solve_opts* opts,
                      mpf_solve_stats* stats);
 
