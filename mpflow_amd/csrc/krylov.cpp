@@ -59,6 +59,8 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
             g->wpad = J.width;
         }
         g->jobs.push_back(j);
+        // the 128-bit gather kernels keep byte offsets of a slice vector in 31 bits
+        if ((long long)J.np * J.Rp * 4 >= (1LL << 31) - (1LL << 16)) g->wide = true;
         for (int c = 0; c < J.C; ++c) {
             all_js.push_back(make_int2(j, c));
             for (int t = 0; t < J.ntiles; ++t) all_tiles.push_back(make_int4(j, c, t, 0));
@@ -168,11 +170,11 @@ void KrylovDriver::tune(Group& g) {
         MPF_HIP(hipEventCreate(&a));
         MPF_HIP(hipEventCreate(&b));
         float best = 1e30f;
-        choice = g.wpad <= 24 ? 0 : 1;
+        choice = g.wpad <= 24 && !g.wide ? 0 : 1;
         const int cand[4][2] = {{0, 0}, {0, 1}, {1, 0}, {2, 0}};
         for (const auto& c : cand) {
             const int v = c[0], o = c[1];
-            if (v != 1 && g.wpad > 24) continue;
+            if (v != 1 && (g.wpad > 24 || g.wide)) continue;
             const int4* lst = v == 2 ? g.ftiles.p : g.tiles_o[o].p;
             const int n = v == 2 ? g.nfall : g.nall;
             launch_pcg_spmv(jobs.p, lst, n, g.lg, g.wpad, v, s0);   // warm-up

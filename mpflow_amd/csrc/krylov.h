@@ -50,6 +50,7 @@ class KrylovDriver {
     struct Group {
         int lg = 0, wpad = 0, variant = 0;
         int order = 0;                // work-list order: 0 (job, slice, tile), 1 (job, tile, slice)
+        bool wide = false;            // a slice vector of >= 2^31 - 2^16 bytes: 32-bit-gather SpMV only
         long long key = 0;
         std::vector<int> jobs;        // indices into h
         std::vector<int4> all_o[2];   // every (job, slice, tile) item in either order, XCD-ordered

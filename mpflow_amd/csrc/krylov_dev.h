@@ -140,9 +140,24 @@ constexpr int kSpmvRowsPerWave = 16;
 // unrolled slot loop.  Zero-valued slots (padding) gather through an offset past the
 // buffer's range, which the buffer unit answers with 0 without a memory access.
 // epi(row, c4, acc) consumes each row's 4-column result.
-// With an LDS copy `xs` of the block's own 256 rows of X (rows tile0 ...), neighbours
-// inside the tile are read from LDS and only the others from memory: the gather for an
-// in-tile neighbour is sent past the buffer's range (answered with 0, no memory access).
+// With an LDS copy `xs` of the block's first kXtRows = 255 rows of X (rows tile0 ...)
+// followed by one row of zeros (the tile's last row is read from memory like a neighbour
+// outside the tile), neighbours inside the tile are read from LDS and only the others
+// from memory.  The stage then holds a byte key per entry (xt_key): every slot issues one
+// memory gather at offset key + 16 c4 (past the buffer's range, so answered with 0 and no
+// memory access, when the key is negative) and one LDS read of row
+// min(-key / (4 Rp) - 1, kXtRows) (the zero row for a neighbour read from memory), and
+// adds the two: exactly one of them is the neighbour's value.
+constexpr int kXtRows = kTileRows - 1;
+template <int LG>
+constexpr int xt_zero_offset() { return kXtRows * (1 << LG) * 4; }
+template <int LG>
+__device__ __forceinline__ int xt_key(int id, int tile0, bool real) {
+    constexpr int RB = (1 << LG) * 4;
+    if (!real) return -(kXtRows + 1) * RB;            // padding slot: the zero row, no gather
+    const int local = id - tile0;
+    return (unsigned)local < (unsigned)kXtRows ? -(local + 1) * RB : id * RB;
+}
 template <int LG, int W, bool XT, class Epi>
 __device__ __forceinline__ void ell_gather_rows(int np, int row0, int l64, const float* X,
                                                 const int2 (*ell)[kSpmvRowsPerWave], const float4* xs,
@@ -162,10 +177,11 @@ __device__ __forceinline__ void ell_gather_rows(int np, int row0, int l64, const
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (XT) {
             // groups of G slots: the G memory gathers of a group are all issued before
-            // any is used; every slot reads both its memory gather and its LDS row and
-            // weights them by v or 0, so that no branch holds a gather back (the memory
-            // gather of an in-tile or padding slot is sent past the buffer range)
+            // any is used; every slot adds its memory gather and its LDS row (one of
+            // them zero), so that no branch holds a gather back
             static_assert(W % 2 == 0, "slot bins are even");
+            constexpr uint32_t RB = Rp * 4, ZERO = xt_zero_offset<LG>();
+            const char* xsb = reinterpret_cast<const char*>(xs) + 16 * c4;
             auto group = [&](auto gsize, int q0) {
                 constexpr int G = decltype(gsize)::value;
                 v4u raw[G];
@@ -173,20 +189,17 @@ __device__ __forceinline__ void ell_gather_rows(int np, int row0, int l64, const
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
                     e[u] = ell[q0 + u][src];
-                    const float v = __int_as_float(e[u].y);
-                    const uint32_t off = (v != 0.0f && e[u].x >= 0) ? (uint32_t)(e[u].x * Rp + 4 * c4) * 4u : range;
-                    raw[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                    raw[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)(e[u].x + 16 * c4), 0, 0);
                 }
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
-                    const bool in = e[u].x < 0;       // staged as -1 - (row - tile0)
                     const float v = __int_as_float(e[u].y);
-                    const float4 l = xs[(in ? -1 - e[u].x : 0) * LPR + c4];
-                    const float vi = in ? v : 0.0f, vo = in ? 0.0f : v;
-                    acc.x += vi * l.x + vo * __uint_as_float(raw[u][0]);
-                    acc.y += vi * l.y + vo * __uint_as_float(raw[u][1]);
-                    acc.z += vi * l.z + vo * __uint_as_float(raw[u][2]);
-                    acc.w += vi * l.w + vo * __uint_as_float(raw[u][3]);
+                    const uint32_t lo = min((uint32_t)(-e[u].x) - RB, ZERO);
+                    const float4 l = *reinterpret_cast<const float4*>(xsb + lo);
+                    acc.x = fmaf(v, l.x + __uint_as_float(raw[u][0]), acc.x);
+                    acc.y = fmaf(v, l.y + __uint_as_float(raw[u][1]), acc.y);
+                    acc.z = fmaf(v, l.z + __uint_as_float(raw[u][2]), acc.z);
+                    acc.w = fmaf(v, l.w + __uint_as_float(raw[u][3]), acc.w);
                 }
             };
 #pragma unroll 1
@@ -224,9 +237,27 @@ __device__ __forceinline__ void ell_gather_rows(int np, int row0, int l64, const
     }
 }
 
-// LDS copy of a tile's rows of X for ell_gather_wave (kTileRows x Rp floats)
+// LDS copy of a tile's first kXtRows rows of X for ell_gather_wave, then a row of zeros
+// (kTileRows x Rp floats in all): element i of the tile's rows is staged as xt_value(i, x)
 template <int LG>
 using XTile = float4[kTileRows * (1 << LG) / 4];
+template <int LG>
+__device__ __forceinline__ float4 xt_value(int i, float4 x) {
+    return i < kXtRows * ((1 << LG) / 4) ? x : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+// the tile's own row `row` of X (buffer resource rs over the slice's X): the LDS row
+// (zeros for the tile's last row) plus a buffer load that is out of range (0, no memory
+// access) except for the last row -- no branch, so the LDS read stays an LDS instruction
+template <int LG>
+__device__ __forceinline__ float4 xt_own(const float4* xs, __amdgpu_buffer_rsrc_t rs, int row, int tile0, int c4) {
+    constexpr int LPR = (1 << LG) / 4;
+    const int local = row - tile0;
+    const float4 l = xs[min(local, kXtRows) * LPR + c4];
+    const uint32_t off = local == kXtRows ? (uint32_t)(row * LPR + c4) * 16u : 0xFFFFFFF0u;
+    const v4u g = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    return make_float4(l.x + __uint_as_float(g[0]), l.y + __uint_as_float(g[1]), l.z + __uint_as_float(g[2]),
+                       l.w + __uint_as_float(g[3]));
+}
 
 // LDS for ell_gather_wave: one [WMAX][16] (id, value) table per wavefront of the block
 template <int WMAX>
@@ -303,7 +334,7 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int i = threadIdx.x + k * kEwBlock;
-            if (i < kTileRows * LPR) xs[i] = xv[k];
+            if (i < kTileRows * LPR) xs[i] = xt_value<LG>(i, xv[k]);
         }
     }
 #pragma unroll
@@ -311,14 +342,10 @@ __device__ __forceinline__ void ell_gather_wave(const JobDev& J, int s, int row0
         const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
         if (q < wb) {
             int2 v = make_int2(0, 0);
+            if constexpr (XT) v.x = xt_key<LG>(idv[k], tile0, q < wl);
             if (q < wl) {
-                int id = idv[k];
-                if constexpr (XT) {
-                    // neighbours inside the tile are staged as -1 - (row - tile0)
-                    const int local = id - tile0;
-                    if ((unsigned)local < (unsigned)kTileRows) id = -1 - local;
-                }
-                v = make_int2(id, __float_as_int(sv[k] + al * rv[k]));
+                if constexpr (!XT) v.x = idv[k];
+                v.y = __float_as_int(sv[k] + al * rv[k]);
             }
             stage[wave][q][r] = v;
         }

@@ -169,8 +169,10 @@ k_pcg_spmv4(const JobDev* __restrict__ jobs,
     __shared__ XTile<LG> xs;
     const int tile0 = t.z * kTileRows;
     // k4: the iteration index modulo kRing, which selects this iteration's p
-    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, pbuf(J, k4) + slice_off, sh.stage, [&](int row, int c4, float4 acc) {
-        const float4 own = xs[(row - tile0) * LPR + c4];
+    const float* P = pbuf(J, k4) + slice_off;
+    const __amdgpu_buffer_rsrc_t prs = uniform_rsrc(P, (size_t)J.np * Rp * sizeof(float));
+    ell_gather_wave<LG, WMAX, true>(J, s, row0, l64, P, sh.stage, [&](int row, int c4, float4 acc) {
+        const float4 own = xt_own<LG>(xs, prs, row, tile0, c4);
         Q4[(size_t)row * LPR + c4] = acc;
         v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
         v[0][2] += (double)own.z * (double)acc.z; v[0][3] += (double)own.w * (double)acc.w;
@@ -222,9 +224,8 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4h(const JobDev* __restric
         }
 #pragma unroll
         for (int k = 0; k < NE; ++k) {
-            // neighbours inside the tile are staged as -1 - (row - tile0)
-            const int local = idv[k] - tile0;
-            if ((unsigned)local < (unsigned)kTileRows) idv[k] = -1 - local;
+            const int q = (l64 + 64 * k) / kSpmvRowsPerWave;
+            idv[k] = xt_key<LG>(idv[k], tile0, q < wl);
         }
     }
     const int ch[2] = {0, 1};
@@ -244,21 +245,21 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv4h(const JobDev* __restric
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
             const int i = threadIdx.x + k * kEwBlock;
-            if (i < kTileRows * LPR) xs[i] = xv[k];
+            if (i < kTileRows * LPR) xs[i] = xt_value<LG>(i, xv[k]);
         }
         const float al = J.alpha[s];
 #pragma unroll
         for (int k = 0; k < NE; ++k) {
             const int e = l64 + 64 * k, q = e / kSpmvRowsPerWave, r = e % kSpmvRowsPerWave;
-            if (q < wb) sh.stage[wave][q][r] = q < wl ? make_int2(idv[k], __float_as_int(sv[k] + al * rv[k]))
-                                                      : make_int2(0, 0);
+            if (q < wb) sh.stage[wave][q][r] = make_int2(idv[k], q < wl ? __float_as_int(sv[k] + al * rv[k]) : 0);
         }
         __syncthreads();
         MPF_GLOBAL float4* Q4 = (MPF_GLOBAL float4*)(gp(J.q) + slice_off);
         double v[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        const __amdgpu_buffer_rsrc_t xrs = uniform_rsrc(X, (size_t)np * Rp * sizeof(float));
         ell_gather_bin<LG, WMAX, true>(wb, np, row0, l64, X, sh.stage[wave], xs, tile0,
                                        [&](int row, int c4, float4 acc) {
-            const float4 own = xs[(row - tile0) * LPR + c4];
+            const float4 own = xt_own<LG>(xs, xrs, row, tile0, c4);
             Q4[(size_t)row * LPR + c4] = acc;
             v[0][0] += (double)own.x * (double)acc.x; v[0][1] += (double)own.y * (double)acc.y;
             v[0][2] += (double)own.z * (double)acc.z; v[0][3] += (double)own.w * (double)acc.w;
