@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_spmv(const JobDev* __restrict_
 // instead of [0], which its stop rule may clear while they run, so that every tile of a
 // slice that stops in this iteration still takes its last x update.
 template <int LG>
-__global__ void __launch_bounds__(kUpdBlock, 8) k_pcg_update4(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_update4(const JobDev* __restrict__ jobs,
                                                            const int4* __restrict__ tiles, int iter) {
     constexpr int Rp = 1 << LG, LPR = Rp / 4, ROWS = kUpdBlock / LPR;
     const int par = iter & 1, k4 = iter & (kRing - 1);
