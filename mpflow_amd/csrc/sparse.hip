@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_update4(const JobDev* __re
     __shared__ double sh_al[Rp], sh_dc[Rp];
     __shared__ int sh_zero[Rp];
     {
-        // p.q and 1^T p arrive as one partial per 32-tile chunk (k_pcg_chunks)
+        // p.q and 1^T p arrive as one partial per kChunkTiles-tile chunk (k_pcg_chunks)
         double out[2] = {0.0, 0.0};
         const int cnt = coarse_chunks(J);
         if (s == J.dc) {
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(256) k_pcg_coarse_mfma(const JobDev* __restric
 
 // Sc^-1/2 per aggregate: Sc_a = sum of S_ii(gamma) over the aggregate's real rows / the
 // same sum at sigma = 1 (harmonic independent; one block per (job, tile) of slice 0, the
-// tile's two aggregates)
+// tile's kAggPerTile aggregates)
 __global__ void __launch_bounds__(kTileRows) k_coarse_scale(const JobDev* __restrict__ jobs,
                                                            const int4* __restrict__ tiles,
                                                            const double* __restrict__ d1) {
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(kUpdBlock, 6) k_pcg_pupdate4(const JobDev* __r
     __shared__ double sh_b[Rp];
     __shared__ int sh_bz[Rp];
     __shared__ int not_done;
-    // two-level: the coarse corrections y_a of the tile's two aggregates (zero without)
+    // two-level: the coarse corrections y_a of the tile's kAggPerTile aggregates (zero without)
     __shared__ float4 sh_y[kAggPerTile][LPR];
     if ((int)threadIdx.x < kAggPerTile * LPR) {
         float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
