@@ -264,7 +264,7 @@ def run_c5(args):
     w = WORKLOADS[args.mesh]
     nodes, el, bd, ports, pat = build_problem(w)
     ctx = Context()
-    B = max(1, args.batch)
+    B = args.batch if args.batch > 0 else 64
     z = args.contact_impedance
     mk = lambda frames: EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
                             jacobian_exact=True, frames=frames, ctx=ctx, contact_impedance=z, cem=args.cem)
@@ -485,7 +485,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4,
                     help="FP32 Krylov tolerance; one FP64 refinement to 1e-2 of it follows (final "
                          "relative residual ~1e-2 tol)")
-    ap.add_argument("--batch", type=int, default=64, help="frames solved together per launch")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="frames solved together per launch (0: 128 for the headline workloads, 64 for config 5)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 oracle reconstruction of one frame (about a minute on config 3)")
@@ -526,7 +527,7 @@ def main():
     w = WORKLOADS[args.workload]
     cfg = CONFIG_INDEX[args.workload]
     nodes, el, bd, ports, pat = build_problem(w)
-    B = max(1, args.batch)
+    B = args.batch if args.batch > 0 else 128
     z = args.contact_impedance
     eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"],
               jacobian_exact=True, frames=B, contact_impedance=z, cem=args.cem)
@@ -601,12 +602,15 @@ def main():
     fps = frames_timed / elapsed
     avg_launch_us = 1e3 * spmv_ms / max(spmv_launches, 1)
     # HBM bytes per launch from the committed rocprofv3 --pmc summary (FETCH_SIZE and
-    # WRITE_SIZE per harmonic-SpMV, calibrated against a copy kernel), scaled by the live
-    # run's harmonic-SpMVs per launch
+    # WRITE_SIZE per harmonic-SpMV, calibrated against a copy kernel) for the work-list
+    # order the autotuner chose, scaled by the live run's harmonic-SpMVs per launch
     traffic = None
+    tune = autotune_report(eit.ctx)
     tfile = os.path.join(ROOT, "profiles", "spmv_traffic_%s.json" % args.workload)
-    if os.path.exists(tfile) and spmv_launches:
-        per_h = json.load(open(tfile))["traffic_bytes_per_harmonic_launch"]
+    if os.path.exists(tfile) and spmv_launches and len(tune["spmv"]) == 1:
+        tj = json.load(open(tfile))
+        tile_major = tune["spmv"][0]["order"] == "(job, tile, harmonic)"
+        per_h = tj["traffic_bytes_per_harmonic_launch" if tile_major else "traffic_bytes_per_harmonic_launch_slice_major"]
         per_launch_h = spmv_bytes / (8.0 * eit.nnz + 8.0 * eit.N * eit.R_unique) / spmv_launches
         traffic = round(per_h * per_launch_h)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
@@ -662,7 +666,7 @@ def main():
         },
         "roofline": roofline,
     }
-    out["config"]["autotune"] = autotune_report(eit.ctx)
+    out["config"]["autotune"] = tune
     if not args.no_cpu_baseline:
         print("bench: host baseline frames", file=sys.stderr, flush=True)
         cb = cpu_baseline(nodes, el, bd, ports, pat, w, meas[nbatches - 1].cpu().numpy(), args.gn_steps, args.lam,

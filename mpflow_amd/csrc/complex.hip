@@ -352,7 +352,6 @@ __global__ void __launch_bounds__(256) k_c_jacobian(const CJob* __restrict__ job
     float2 acc[kCJacMaxRows / 256];
 #pragma unroll
     for (int q = 0; q < kCJacMaxRows / 256; ++q) acc[q] = make_float2(0.0f, 0.0f);
-    const int per = (mrows + 255) / 256;
     __syncthreads();
     for (int s = 0; s < Jb.C; ++s) {
         const float al = jd.exact ? jd.alpha[s] : 0.0f;
@@ -376,11 +375,12 @@ __global__ void __launch_bounds__(256) k_c_jacobian(const CJob* __restrict__ job
             T[a][m] = make_float2(w * tr, w * ti);
         }
         __syncthreads();
-        for (int q = 0; q < per; ++q) {
+        // the loop is unrolled to its compile-time bound so that acc stays in registers
+#pragma unroll
+        for (int q = 0; q < kCJacMaxRows / 256; ++q) {
             const int idx = threadIdx.x + 256 * q;
-            if (idx >= mrows) break;
             const int d = idx / Mp, m = idx % Mp;
-            if (d < D && m < M)
+            if (idx < mrows && d < D && m < M)
                 for (int a = 0; a < k; ++a) {
                     const float2 u = phi[a][d], t = T[a][m];
                     acc[q].x += u.x * t.x - u.y * t.y;
@@ -393,10 +393,10 @@ __global__ void __launch_bounds__(256) k_c_jacobian(const CJob* __restrict__ job
     const zd ds = Jb.logarithmic ? (kLn10 / 10.0) * (Jb.ref * zexp((kLn10 / 10.0) * zf(g))) : zmake(Jb.ref);
     const zd sc = -ds;
     float2* col = Jout + ((size_t)j * jd.E + e) * mrows;
-    for (int q = 0; q < per; ++q) {
+#pragma unroll
+    for (int q = 0; q < kCJacMaxRows / 256; ++q) {
         const int idx = threadIdx.x + 256 * q;
-        if (idx >= mrows) break;
-        col[idx] = fz(sc * zf(acc[q]));
+        if (idx < mrows) col[idx] = fz(sc * zf(acc[q]));
     }
 }
 
