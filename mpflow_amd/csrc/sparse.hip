@@ -531,23 +531,47 @@ __global__ void __launch_bounds__(256) k_pcg_coarse_mfma(const JobDev* __restric
     // rows past na compute a copy of the last row, discarded
     const int arow = min(a0 + 16 * wave + r16, na - 1);
     const MPF_GLOBAL float* crow = J.cinv ? gp(J.cinv) + ((size_t)s * na + arow) * ld : nullptr;
-    for (int u0 = 0; J.cinv && u0 < na; u0 += kCoarseK) {
+    // software pipeline: the next step's cinv fragments and G values are loaded while the
+    // current step's are staged and multiplied, so that each wavefront waits for the
+    // inverse (read from L2 or the MALL) once per step of kCoarseK aggregates, not four times
+    constexpr int NK = kCoarseK / 16, NS = kCoarseK * Rp / 256;
+    static_assert(kCoarseK * Rp % 256 == 0, "G staging splits evenly over the block");
+    float4 cn[NK];
+    float gn[NS];
+    auto load_step = [&](int u0) {
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const int u = u0 + 16 * q + 4 * kb;   // cinv rows are zero-padded to ld, a multiple of 4
+            cn[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (u < ld) cn[q] = *(const MPF_GLOBAL float4*)(crow + u);
+        }
         const int un = min(kCoarseK, na - u0);
-        __syncthreads();   // the previous step's G has been read
-        for (int i = threadIdx.x; i < kCoarseK * Rp; i += 256) {
-            const int u = i / Rp, c = i % Rp, a = u0 + u;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const int i = threadIdx.x + 256 * q, u = i / Rp, c = i % Rp, a = u0 + u;
             // in the iteration the update kernel left c_a g_a in FP32; at the start only
             // the FP64 partials exist
-            gs[u][c] = u >= un ? 0.0f
-                       : in_iteration ? gp((const float*)J.cg)[((size_t)s * na + a) * Rp + c]
-                                      : (float)((double)csc[a] * part_ptr(J, agg_channel(a % kAggPerTile), s, a / kAggPerTile)[c]);
+            gn[q] = u >= un ? 0.0f
+                    : in_iteration ? gp((const float*)J.cg)[((size_t)s * na + a) * Rp + c]
+                                   : (float)((double)csc[a] * part_ptr(J, agg_channel(a % kAggPerTile), s, a / kAggPerTile)[c]);
         }
+    };
+    if (J.cinv) load_step(0);
+    for (int u0 = 0; J.cinv && u0 < na; u0 += kCoarseK) {
+        float4 cc[NK];
+#pragma unroll
+        for (int q = 0; q < NK; ++q) cc[q] = cn[q];
+        __syncthreads();   // the previous step's G has been read
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const int i = threadIdx.x + 256 * q;
+            gs[i / Rp][i % Rp] = gn[q];
+        }
+        if (u0 + kCoarseK < na) load_step(u0 + kCoarseK);
         __syncthreads();
 #pragma unroll
         for (int k0 = 0; k0 < kCoarseK; k0 += 16) {
-            const int u = u0 + k0 + 4 * kb;   // cinv rows are zero-padded to ld, a multiple of 4
-            float4 c4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (u < ld) c4 = *(const MPF_GLOBAL float4*)(crow + u);
+            const float4 c4 = cc[k0 / 16];
             const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
