@@ -105,11 +105,19 @@ __global__ void __launch_bounds__(kEwBlock) k_pcg_init_p(const JobDev* __restric
     const JobDev& J = jobs[t.x];
     if (!J.cinv) return;
     const int s = t.y;
-    const Ew L = ew_of(J);
-    const size_t cb = ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * J.Rp + L.lane;
-    const int tile0 = t.z * kTileRows;
-    for (int row = tile0 + L.g; row < min(tile0 + kTileRows, J.n); row += L.G)
-        J.p[vec_index(J, s, row, L.lane)] += J.cy[cb + (size_t)row_agg(J, row) * J.Rp];
+    const int LPR = J.Rp / 4, tile0 = t.z * kTileRows;
+    // 128-bit accesses: the tile's rows of p as [row][LPR] float4, y as [aggregate][LPR]
+    const MPF_GLOBAL float4* Y =
+        (const MPF_GLOBAL float4*)gp((const float*)J.cy) + ((size_t)s * kAggPerTile * J.ntiles + kAggPerTile * t.z) * LPR;
+    MPF_GLOBAL float4* P = (MPF_GLOBAL float4*)gp(J.p) + ((size_t)s * J.np + tile0) * LPR;
+    for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) {
+        const int row = tile0 + i / LPR;
+        if (row >= J.n) continue;
+        const float4 y = Y[row_agg(J, row) * LPR + i % LPR];
+        float4 p = P[i];
+        p.x += y.x; p.y += y.y; p.z += y.z; p.w += y.w;
+        P[i] = p;
+    }
 }
 
 __global__ void __launch_bounds__(kEwBlock) k_pcg_init_reduce(const JobDev* __restrict__ jobs,
@@ -826,17 +834,32 @@ __global__ void __launch_bounds__(kUpdBlock) k_pcg_flush(const JobDev* __restric
     const int s = t.y;
     const int m = gp(J.st)[s * kStInts + 1] & (kRing - 1);
     if (m == 0) return;
-    const int Rp = J.Rp;
-    for (int i = threadIdx.x; i < kTileRows * Rp; i += kUpdBlock) {
-        const int lane = i % Rp, row = t.z * kTileRows + i / Rp;
-        const size_t v = vec_index(J, s, row, lane);
-        float xv = J.x[v];
-        for (int q = 0; q < m; ++q) {
-            const double pq = sc_ptr(J, SC_PQ0 + q, s)[lane];
-            const float a = (float)sc_ptr(J, SC_PA0 + q, s)[lane];
-            xv = pq == 0.0 ? 0.0f : xv + a * pbuf(J, q)[v];
+    // 128-bit accesses; a thread keeps the same 4 lanes (kUpdBlock is a multiple of Rp / 4),
+    // so their alpha and p.q of the owed iterations are read once
+    const int LPR = J.Rp / 4, c4 = threadIdx.x % LPR;
+    float a[kRing - 1][4];
+    bool zq[kRing - 1][4];
+#pragma unroll
+    for (int q = 0; q < kRing - 1; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[q][j] = q < m ? (float)sc_ptr(J, SC_PA0 + q, s)[4 * c4 + j] : 0.0f;
+            zq[q][j] = q < m && sc_ptr(J, SC_PQ0 + q, s)[4 * c4 + j] == 0.0;
         }
-        J.x[v] = xv;
+    const size_t base = ((size_t)s * J.np + (size_t)t.z * kTileRows) * LPR;
+    MPF_GLOBAL float4* X = (MPF_GLOBAL float4*)gp(J.x) + base;
+    for (int i = threadIdx.x; i < kTileRows * LPR; i += kUpdBlock) {
+        float4 x = X[i];
+        float xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int q = 0; q < kRing - 1; ++q) {
+            if (q >= m) break;
+            const float4 p = ((const MPF_GLOBAL float4*)gp((const float*)pbuf(J, q)) + base)[i];
+            const float pv[4] = {p.x, p.y, p.z, p.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[j] = zq[q][j] ? 0.0f : xv[j] + a[q][j] * pv[j];
+        }
+        X[i] = make_float4(xv[0], xv[1], xv[2], xv[3]);
     }
 }
 
@@ -981,11 +1004,20 @@ __global__ void __launch_bounds__(kEwBlock) k_x64_update(const JobDev* __restric
     const int4 t = tiles[blockIdx.x];
     const JobDev& J = jobs[t.x];
     const int s = t.y;
-    const Ew L = ew_of(J);
-    for (int row = t.z * kTileRows + L.g; row < (t.z + 1) * kTileRows; row += L.G) {
-        const size_t i = vec_index(J, s, row, L.lane);
-        const double v = (double)J.x[i];
-        J.x64[i] = first ? v : J.x64[i] + v;
+    // 128-bit FP32 reads of x, 2 x 128-bit FP64 accesses of x64 per 4 lanes
+    const int LPR = J.Rp / 4;
+    const size_t base = ((size_t)s * J.np + (size_t)t.z * kTileRows) * LPR;
+    const MPF_GLOBAL float4* X = (const MPF_GLOBAL float4*)gp((const float*)J.x) + base;
+    MPF_GLOBAL double2* D = (MPF_GLOBAL double2*)gp(J.x64) + 2 * base;
+    for (int i = threadIdx.x; i < kTileRows * LPR; i += kEwBlock) {
+        const float4 v = X[i];
+        double2 lo = make_double2((double)v.x, (double)v.y), hi = make_double2((double)v.z, (double)v.w);
+        if (!first) {
+            const double2 a = D[2 * i], b = D[2 * i + 1];
+            lo.x += a.x; lo.y += a.y; hi.x += b.x; hi.y += b.y;
+        }
+        D[2 * i] = lo;
+        D[2 * i + 1] = hi;
     }
 }
 
