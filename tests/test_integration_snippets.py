@@ -1,4 +1,4 @@
-"""The reference-side binding shown in INTEGRATION.md compiles and links against
+"""The reference-side binding shown in the repository's integration notes compiles and links against
 include/mpflow_amd.h and libmpflow_amd.so (g++, C++17), and the ctypes argument types
 of mpflow_amd/capi.py agree with the header's declarations parameter by parameter."""
 import ctypes as C
