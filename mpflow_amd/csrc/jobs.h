@@ -130,9 +130,9 @@ void launch_pcg_spmv(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int 
                      hipStream_t st, int par = 0);
 // PCG: the x update that a slice stopping at an even iteration still owes
 void launch_pcg_flush(const JobDev* d_jobs, const int4* d_tiles, int ntiles, hipStream_t st);
-// every job of the tile list has 2^lg lanes
+// every job of the tile list has 2^lg lanes; slice_bytes: the largest np * 2^lg * 8
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles,
-                       const int2* d_js, int njs, int lg, hipStream_t st);
+                       const int2* d_js, int njs, int lg, size_t slice_bytes, hipStream_t st);
 void launch_x64_update(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int first,
                        hipStream_t st);
 

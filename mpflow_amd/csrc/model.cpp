@@ -354,7 +354,8 @@ int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_o
     if (warm) {
         // start from the previous FP64 fields: solve for the correction of the FP64
         // residual at the new sigma, to the same absolute accuracy tol * ||beta F||
-        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg,
+                                                  (size_t)m->S.np * m->S.Rp * sizeof(double), st0);
         capped |= drv.run(o, MPF_STOP_TOLERANCE, o.tol, true, true, stats);
         for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
     } else {
@@ -365,12 +366,14 @@ int forward_models(std::vector<Model*>& ms, KrylovDriver& drv, const mpf_solve_o
     for (Model* m : ms) m->x64_valid = mixed;
     if (mixed) {
         for (int k = 0; k < o.refine_steps; ++k) {
-            for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
+            for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg,
+                                                  (size_t)m->S.np * m->S.Rp * sizeof(double), st0);
             capped |= drv.run(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats);
             for (Model* m : ms) launch_x64_update(jobs_of(m), m->tiles.p, m->ntiles, 0, st0);
         }
         // final FP64 residual for the report
-        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg, st0);
+        for (Model* m : ms) launch_residual64(jobs_of(m), m->tiles.p, m->ntiles, m->js.p, m->njs, m->hjobs[0].lg,
+                                                  (size_t)m->S.np * m->S.Rp * sizeof(double), st0);
     }
     for (size_t i = 0; i < ms.size(); ++i)
         ms[i]->finish(o, d_U ? d_U[i] : nullptr, d_U64 ? d_U64[i] : nullptr, d_J ? d_J[i] : nullptr, stats);

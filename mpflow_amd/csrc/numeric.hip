@@ -193,7 +193,7 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
     if (mixed) {
         launch_x64_update(jobs.p, tiles.p, ntiles, 1, st);
         for (int k = 0; k < o.refine_steps; ++k) {
-            launch_residual64(jobs.p, tiles.p, ntiles, js.p, 1, hj[0].lg, st);
+            launch_residual64(jobs.p, tiles.p, ntiles, js.p, 1, hj[0].lg, (size_t)hj[0].np * hj[0].Rp * sizeof(double), st);
             capped |= drv.run(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats);
             launch_x64_update(jobs.p, tiles.p, ntiles, 0, st);
         }

@@ -900,11 +900,17 @@ __global__ void __launch_bounds__(kEwBlock) k_colsum64_reduce(const JobDev* __re
 // at a time, and the ELL entries of those rows are staged through LDS kResSlots slots at
 // a time with coalesced loads (id and the FP64 value S + alpha R, each formed once per
 // row and slot instead of once per lane).  The sums run over the slots in ELL order.
+// XT: the tile's 256 rows of x64 and a row of zeros sit in LDS (64 KB at 32 lanes) and
+// the stage holds byte keys as in the SpMV (xt_key): every slot adds an LDS read (zero for
+// a neighbour outside the tile) and a buffer load (out of range, 0, for one inside), so
+// only the neighbours outside the tile are gathered from memory; the sums are the same.
 constexpr int kResSlots = 8;
-template <int LG>
+template <int LG, bool XT>
 __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restrict__ jobs,
                                                          const int4* __restrict__ tiles) {
     constexpr int Rp = 1 << LG, T = Rp / 2, PR = kEwBlock / T;
+    constexpr int RB = Rp * 8;                              // bytes per row of x64
+    constexpr int NXT = XT ? (kTileRows + 1) * T : 1;       // double2 elements of the LDS tile
     const int4 t4 = tiles[blockIdx.x];
     const JobDev& J = jobs[t4.x];
     if (J.Rp != Rp) return;   // the launcher checks that every job has 2^LG lanes
@@ -916,15 +922,28 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
     const double dcs0 = sc_ptr(J, SC_X0, s)[l0], dcs1 = sc_ptr(J, SC_X0, s)[l0 + 1];
     const MPF_GLOBAL double2* X2 = (const MPF_GLOBAL double2*)(gp(J.x64) + (size_t)s * J.np * Rp);
     float* out = J.rhs;
+    __shared__ struct {
+        int id[kResSlots][PR];      // XT: byte keys
+        double v[kResSlots][PR];
+    } se;
     __shared__ union {
-        struct {
-            int id[kResSlots][PR];
-            double v[kResSlots][PR];
-        } e;
+        double2 xt[NXT];
         double red[2][PR][Rp];
-    } sh;
-    double rr0 = 0.0, rr1 = 0.0, bb0 = 0.0, bb1 = 0.0;
+    } sb;
     const int tile0 = t4.z * kTileRows;
+    __amdgpu_buffer_rsrc_t rs;
+    if constexpr (XT) {
+        rs = uniform_rsrc(X2, (size_t)J.np * RB);
+        constexpr int NL = kTileRows * T / kEwBlock;
+        static_assert(kTileRows * T % kEwBlock == 0, "the tile's rows split evenly over the block");
+        double2 xv[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) xv[k] = X2[(size_t)tile0 * T + threadIdx.x + k * kEwBlock];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) sb.xt[threadIdx.x + k * kEwBlock] = xv[k];
+        if ((int)threadIdx.x < T) sb.xt[kTileRows * T + threadIdx.x] = make_double2(0.0, 0.0);
+    }
+    double rr0 = 0.0, rr1 = 0.0, bb0 = 0.0, bb1 = 0.0;
     for (int base = tile0; base < tile0 + kTileRows; base += PR) {
         const int row = base + rl;
         // slots past a 16-row group's widest row are padding (value 0)
@@ -936,26 +955,42 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
         const int wr = J.gw ? J.gw[row >> 4] : J.width;
         double a0 = 0.0, a1 = 0.0;
         for (int q0 = 0; q0 < wp; q0 += kResSlots) {
-            __syncthreads();   // the previous step's entries have been read
+            __syncthreads();   // the previous step's entries have been read (and the tile staged)
             for (int i = threadIdx.x; i < kResSlots * PR; i += kEwBlock) {
                 const int qq = i / PR, r2 = i % PR, q = q0 + qq;
-                int id = 0;
+                int id = XT ? -(kTileRows + 1) * RB : 0;
                 double v = 0.0;
                 if (q < wp) {
                     const size_t e = (size_t)q * J.np + base + r2;
                     id = J.ids[e];
+                    if constexpr (XT) {
+                        const int local = id - tile0;
+                        id = (unsigned)local < (unsigned)kTileRows ? -(local + 1) * RB : id * RB;
+                    }
                     v = ((double)J.s[e] + (double)J.s_lo[e]) + al * ((double)J.r[e] + (double)J.r_lo[e]);
                 }
-                sh.e.id[qq][r2] = id;
-                sh.e.v[qq][r2] = v;
+                se.id[qq][r2] = id;
+                se.v[qq][r2] = v;
             }
             __syncthreads();
             const int qn = wr - q0;
 #pragma unroll
             for (int qq = 0; qq < kResSlots; ++qq) {
                 if (qq < qn) {
-                    const double2 xv = X2[(size_t)sh.e.id[qq][rl] * T + t];
-                    const double v = sh.e.v[qq][rl];
+                    const double v = se.v[qq][rl];
+                    double2 xv;
+                    if constexpr (XT) {
+                        const int key = se.id[qq][rl];
+                        const v4u g = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)(key + 16 * t), 0, 0);
+                        double2 gd;
+                        __builtin_memcpy(&gd, &g, sizeof(gd));
+                        const uint32_t lo = min((uint32_t)(-key) - (uint32_t)RB, (uint32_t)(kTileRows * RB));
+                        const double2 l = *reinterpret_cast<const double2*>(
+                            reinterpret_cast<const char*>(sb.xt) + lo + 16 * t);
+                        xv = make_double2(l.x + gd.x, l.y + gd.y);
+                    } else {
+                        xv = X2[(size_t)se.id[qq][rl] * T + t];
+                    }
                     a0 += v * xv.x;
                     a1 += v * xv.y;
                 }
@@ -971,15 +1006,16 @@ __global__ void __launch_bounds__(kEwBlock) k_residual64(const JobDev* __restric
         rr0 += r0 * r0; rr1 += r1 * r1;
         bb0 += b0 * b0; bb1 += b1 * b1;
     }
-    // per-lane tile partials of r.r and b.b, summed over the rows in a fixed order
+    // per-lane tile partials of r.r and b.b, summed over the rows in a fixed order (the
+    // buffer shares LDS with the x64 tile: every gather of the block is done first)
     __syncthreads();
-    sh.red[0][rl][l0] = rr0; sh.red[0][rl][l0 + 1] = rr1;
-    sh.red[1][rl][l0] = bb0; sh.red[1][rl][l0 + 1] = bb1;
+    sb.red[0][rl][l0] = rr0; sb.red[0][rl][l0 + 1] = rr1;
+    sb.red[1][rl][l0] = bb0; sb.red[1][rl][l0 + 1] = bb1;
     __syncthreads();
     if ((int)threadIdx.x < 2 * Rp) {
         const int c = threadIdx.x / Rp, lane = threadIdx.x % Rp;
         double a = 0.0;
-        for (int g = 0; g < PR; ++g) a += sh.red[c][g][lane];
+        for (int g = 0; g < PR; ++g) a += sb.red[c][g][lane];
         part_ptr(J, c, s, t4.z)[lane] = a;
     }
 }
@@ -1164,12 +1200,16 @@ void launch_pcg_iter(const JobDev* d_jobs, const int4* d_tiles, int ntiles, int*
 }
 
 void launch_residual64(const JobDev* d_jobs, const int4* d_tiles, int ntiles, const int2* d_js,
-                       int njs, int lg, hipStream_t st) {
+                       int njs, int lg, size_t slice_bytes, hipStream_t st) {
     k_colsum64<<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     k_colsum64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
-    if (lg == 4) k_residual64<4><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else if (lg == 5) k_residual64<5><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
-    else if (lg == 6) k_residual64<6><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    // the LDS tile of x64 needs 31-bit byte offsets within a slice (and fits 32 lanes)
+    const bool xt = slice_bytes < (size_t)(1u << 31) - (1u << 16);
+    if (lg == 4 && xt) k_residual64<4, true><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 4) k_residual64<4, false><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 5 && xt) k_residual64<5, true><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 5) k_residual64<5, false><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
+    else if (lg == 6) k_residual64<6, false><<<ntiles, kEwBlock, 0, st>>>(d_jobs, d_tiles);
     else throw Error(MPF_EINVAL, "residual: unsupported lane count");
     k_residual64_reduce<<<njs, kEwBlock, 0, st>>>(d_jobs, d_js);
     check_launch();
