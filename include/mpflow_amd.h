@@ -50,6 +50,7 @@ extern "C" {
 /* Krylov method */
 #define MPF_KRYLOV_CG       0
 #define MPF_KRYLOV_BICGSTAB 1
+#define MPF_KRYLOV_BLOCK_CG 2     /* mpf_block_pcg_solve only */
 
 #define MPF_PRECOND_JACOBI    0
 #define MPF_PRECOND_TWO_LEVEL 1
@@ -397,6 +398,21 @@ int mpf_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_co
 int mpf_bicgstab_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
                        const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
                        int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
+
+/* mpf_pcg_solve's problem (same arguments and meaning) solved by block PCG: the R <= 64
+ * columns share one Krylov space (O'Leary's block CG with the diagonal preconditioner;
+ * the block of search directions is orthonormalised every iteration and directions that
+ * have become linearly dependent are dropped, so right-hand sides that are linear
+ * combinations of each other -- drive patterns that sum to zero -- are handled).  No
+ * counterpart in the reference, whose ConjugateGradient::solve
+ * (src/numeric/conjugate_gradient.cpp:61-161) iterates the columns independently; a
+ * caller replaces that call with this one when its columns share the matrix.
+ * stats->steps counts block iterations (one SpMV of all columns each); not_converged is 1
+ * when the iteration cap was hit or no independent direction was left (MPF_ENOCONV).
+ * opts.check_every: iterations between host convergence polls; opts.graphs is ignored. */
+int mpf_block_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
+                        const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X,
+                        int32_t ldx, int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats);
 
 /* SparseMatrix::multiply (src/numeric/sparse_matrix.cpp:262-288; the kernel wrapper
  * sparseMatrixKernel::multiply, include/mpflow/numeric/sparse_matrix_kernel.h:57-60,

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "libmpflow_amd.so")
 MPF_OK, MPF_EINVAL, MPF_ENOMEM, MPF_EHIP, MPF_ENOCONV, MPF_EFULL = range(6)
 MPF_STOP_REFERENCE, MPF_STOP_TOLERANCE = 0, 1
 MPF_PREC_FP32, MPF_PREC_MIXED = 0, 1
-MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB = 0, 1
+MPF_KRYLOV_CG, MPF_KRYLOV_BICGSTAB, MPF_KRYLOV_BLOCK_CG = 0, 1, 2
 MPF_PRECOND_JACOBI, MPF_PRECOND_TWO_LEVEL = 0, 1
 MPF_REG_IDENTITY, MPF_REG_NOSER, MPF_REG_LAPLACIAN = 0, 1, 2
 MPF_SOURCE_CURRENT, MPF_SOURCE_VOLTAGE = 0, 1
@@ -22,7 +22,7 @@ EXPORTED = [
     "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read", "mpf_ctx_autotune_read",
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
-    "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve",
+    "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve", "mpf_block_pcg_solve",
     "mpf_fem_assemble", "mpf_eit_measure", "mpf_jacobian", "mpf_gn_set_regularization", "mpf_diff_prepare",
     "mpf_diff_solve", "mpf_normal_solve", "mpf_set_sensitivity_mask", "mpf_sparse_multiply",
     "mpf_normal_solve_noalloc",
@@ -115,7 +115,7 @@ def lib() -> C.CDLL:
     L.mpf_mlp_param_count.argtypes = [vp]
     L.mpf_mlp_param_count.restype = i64
     L.mpf_mlp_params.argtypes = [vp, i32, i32, vp]
-    for f in (L.mpf_pcg_solve, L.mpf_bicgstab_solve):
+    for f in (L.mpf_pcg_solve, L.mpf_bicgstab_solve, L.mpf_block_pcg_solve):
         f.argtypes = [vp, i32, i32, vp, vp, i32, vp, i32, vp, i32, i32, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.mpf_fem_assemble.argtypes = [vp, vp, dbl, vp, vp]
     L.mpf_eit_measure.argtypes = [vp, vp, i32, i32, vp]

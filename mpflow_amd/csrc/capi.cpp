@@ -676,3 +676,10 @@ int mpf_bicgstab_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t*
     return krylov_entry("mpf_bicgstab_solve", MPF_KRYLOV_BICGSTAB, ctx, n, density, d_column_ids, d_values, R,
                         d_B, ldb, d_X, ldx, dc_free, opts, stats);
 }
+
+int mpf_block_pcg_solve(mpf_ctx* ctx, int32_t n, int32_t density, const uint32_t* d_column_ids,
+                        const float* d_values, int32_t R, const float* d_B, int32_t ldb, float* d_X, int32_t ldx,
+                        int32_t dc_free, const mpf_solve_opts* opts, mpf_solve_stats* stats) {
+    return krylov_entry("mpf_block_pcg_solve", MPF_KRYLOV_BLOCK_CG, ctx, n, density, d_column_ids, d_values, R,
+                        d_B, ldb, d_X, ldx, dc_free, opts, stats);
+}

@@ -14,6 +14,10 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
 // SparseMatrix::multiply on the reference layout (synchronises)
 void sparse_multiply(mpf_ctx* ctx, const float* d_values, const uint32_t* d_cid, const float* d_X,
                      int result_rows, int matrix_rows, int columns, int density, float* d_Y);
+// Y = A X on the product layout for one matrix with values S and a zero R term
+// (`zero`), Rp lanes; enqueued on st (no synchronisation); an id outside [0, np) sets *bad
+void ell_spmv_product(const int32_t* ids, const float* S, const float* zero, int np, int width, int Rp,
+                      const float* X, float* Y, int* bad, hipStream_t st);
 // multi-harmonic S/R SpMV on the product layout; alpha is a host array of C values
 // (synchronises)
 void ell_spmv(mpf_ctx* ctx, int np, int width, const int32_t* d_ids, const float* d_S, const float* d_R, int C,

@@ -9,7 +9,9 @@
 // the values, R = 0) and solved by the same Krylov driver as the EIT models.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 
+#include "blockcg.h"
 #include "krylov_dev.h"
 #include "model.h"
 #include "numeric.h"
@@ -125,7 +127,8 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
     MPF_REQUIRE(R >= 1 && R <= 64, "krylov solve: 1 <= R <= 64 right-hand sides");
     MPF_REQUIRE(ldb >= n && ldx >= n, "krylov solve: leading dimension < rows");
     const bool bicg = o.krylov == MPF_KRYLOV_BICGSTAB;
-    MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG || bicg, "krylov solve: unknown Krylov method");
+    const bool block = o.krylov == MPF_KRYLOV_BLOCK_CG;
+    MPF_REQUIRE(o.krylov == MPF_KRYLOV_CG || bicg || block, "krylov solve: unknown Krylov method");
     hipStream_t st = ctx->stream;
     const int np = (n + kTileRows - 1) / kTileRows * kTileRows;
     const int wpad = (density + kSlotPad - 1) / kSlotPad * kSlotPad;
@@ -184,17 +187,38 @@ int krylov_solve(mpf_ctx* ctx, int n, int density, const uint32_t* d_cid, const 
     DBuf<int2> js(1);
     js.upload(&hjs, 1, st);
     launch_dinv(jobs.p, tiles.p, ntiles, st);
-    KrylovDriver drv(ctx, {&hj});
     bool capped = false;
     const bool mixed = o.precision == MPF_PREC_MIXED;
     // the correction from the caller's X0 starts at 0 (EIT::solve's fields start at 0,
     // eit.cpp:165; then the correction is the solution itself) and stops relative to b.b
-    capped |= drv.run(o, o.stop_rule, o.tol, true, true, stats);
+    std::unique_ptr<KrylovDriver> drv;
+    BcgArgs ba;
+    if (block) {
+        ba.n = n; ba.np = np; ba.R = R; ba.Rp = Rp; ba.wpad = wpad; ba.dc = dc_free ? 1 : 0;
+        ba.ids = ids.p; ba.s = s.p; ba.zero = zero.p; ba.dinv = dinv.p; ba.rhs = rhs.p;
+        ba.x = x.p; ba.p = p.p; ba.q = q.p; ba.bad = bad.p;
+    } else {
+        drv.reset(new KrylovDriver(ctx, {&hj}));
+    }
+    const int max_iter = o.max_iter > 0 ? o.max_iter : 10 * n;
+    // one solve of the current rhs: from the caller's b.b (ref_b) or relative to the rhs
+    auto solve = [&](int rule, double tol, bool ref_b) {
+        if (!block) return drv->run(o, rule, tol, true, ref_b, stats);
+        ba.bb_ref = ref_b ? sc.p + (size_t)SC_X2 * Rp : nullptr;
+        const BcgResult res = block_pcg(ba, rule, tol, max_iter, o.check_every, st);
+        if (stats) {
+            stats->steps += res.steps;
+            stats->max_steps = std::max<int64_t>(stats->max_steps, res.steps);
+            stats->not_converged += (res.capped || res.broke) ? 1 : 0;
+        }
+        return res.capped || res.broke;
+    };
+    capped |= solve(o.stop_rule, o.tol, true);
     if (mixed) {
         launch_x64_update(jobs.p, tiles.p, ntiles, 1, st);
         for (int k = 0; k < o.refine_steps; ++k) {
             launch_residual64(jobs.p, tiles.p, ntiles, js.p, 1, hj[0].lg, (size_t)hj[0].np * hj[0].Rp * sizeof(double), st);
-            capped |= drv.run(o, MPF_STOP_TOLERANCE, o.refine_tol, true, false, stats);
+            capped |= solve(MPF_STOP_TOLERANCE, o.refine_tol, false);
             launch_x64_update(jobs.p, tiles.p, ntiles, 0, st);
         }
     }
@@ -357,6 +381,18 @@ void sparse_multiply(mpf_ctx* ctx, const float* d_values, const uint32_t* d_cid,
     MPF_HIP(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
     MPF_HIP(hipStreamSynchronize(st));
     MPF_REQUIRE(h == 0, "sparse multiply: a column id is neither 0xFFFFFFFF nor below matrix_rows");
+}
+
+void ell_spmv_product(const int32_t* ids, const float* S, const float* zero, int np, int width, int Rp,
+                      const float* X, float* Y, int* bad, hipStream_t st) {
+    const SeamAlpha al{};
+    const dim3 grid(np / kTileRows, 1);
+    if (Rp == 16)
+        seam_launch<4, false>(width, grid, np, np, np, Rp, width, nullptr, nullptr, ids, S, zero, al, X, Y, Rp, bad, st);
+    else if (Rp == 32)
+        seam_launch<5, false>(width, grid, np, np, np, Rp, width, nullptr, nullptr, ids, S, zero, al, X, Y, Rp, bad, st);
+    else
+        seam_launch<6, false>(width, grid, np, np, np, Rp, width, nullptr, nullptr, ids, S, zero, al, X, Y, Rp, bad, st);
 }
 
 void ell_spmv(mpf_ctx* ctx, int np, int width, const int32_t* d_ids, const float* d_S, const float* d_R, int C,

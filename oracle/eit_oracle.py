@@ -419,6 +419,112 @@ def pcg_reference(A, b, dinv=None, dc_free=False, max_iter=0, eps=FLT_EPSILON,
     return x, iters
 
 
+def _pivoted_cholesky_orth(W, drop):
+    """T with W T orthonormal on the columns of W that keep more than ``drop`` of their
+    squared norm after projection on the columns chosen before them (greedy pivoting on
+    the largest kept fraction); the kept directions fill columns 0..k-1 of T.  Returns
+    (T [Rp][Rp], k)."""
+    G = W.T @ W
+    Rp = G.shape[0]
+    orig = np.diag(G).copy()
+    S = G.copy()
+    used = np.zeros(Rp, bool)
+    piv, cols = [], []
+    for _ in range(Rp):
+        frac = np.where(~used & (orig > 0), np.diag(S) / np.where(orig > 0, orig, 1.0), -np.inf)
+        p = int(np.argmax(frac))
+        if not frac[p] > drop:
+            break
+        lpp = math.sqrt(S[p, p])
+        used[p] = True
+        l = np.where(used & (np.arange(Rp) != p), 0.0, S[:, p] / lpp)
+        rem = ~used
+        S[np.ix_(rem, rem)] -= np.outer(l[rem], l[rem])
+        piv.append(p)
+        cols.append(l)
+    k = len(piv)
+    T = np.zeros((Rp, Rp))
+    if k:
+        L = np.array([[cols[m][piv[a]] if m <= a else 0.0 for m in range(k)] for a in range(k)])
+        Tk = sla.solve_triangular(L.T, np.eye(k), lower=False)
+        T[piv, :k] = Tk
+    return T, k
+
+
+def block_pcg(A, B, dinv=None, dc_free=False, tol=1e-6, rule="tolerance", max_iter=0, drop=1e-10,
+              fp32=False):
+    """Block Jacobi-PCG over the columns of B (the product's ``mpf_block_pcg_solve``; no
+    counterpart in the reference, whose ConjugateGradient::solve,
+    conjugate_gradient.cpp:61-161, iterates the columns independently).  O'Leary's block
+    CG with the directions orthonormalised every iteration:
+
+        R = B, Z = M R, P = orth(Z)
+        repeat:  Q = A P; alpha = (P^T Q)^-1 P^T R; X += P alpha; R -= Q alpha
+                 stop (rule); Z = M R; beta = -(P^T Q)^-1 Q^T Z; P = orth(Z + P beta)
+
+    orth: ``_pivoted_cholesky_orth`` (directions keeping less than ``drop`` of their norm
+    are dropped; the kept ones occupy the first k lanes, the others are zero and get a
+    unit diagonal in P^T Q).  dc_free: the operator is A + 1 1^T.  Rules as
+    ``pcg_reference``: "tolerance" (every column ||r|| <= tol ||b||) or "reference"
+    (sqrt(|r.z| / b.b) < FLT_EPSILON on column 0).  ``fp32``: the device's precision
+    split -- A, B, the Jacobi inverse, P and Q = A P rounded to float32, T rounded to
+    float32 before it forms P; X, R, Z, W, alpha and beta float64.  The lane count is that
+    of the device (16, 32 or 64 >= the columns).  Returns (X, steps)."""
+    A = sp.csr_matrix(A)
+    N, R = B.shape
+    Rp = 16 if R <= 16 else (32 if R <= 32 else 64)
+    iters = 10 * N if max_iter == 0 else max_iter
+    if dinv is None:
+        dinv = 1.0 / A.diagonal()
+    rnd = (lambda v: v.astype(np.float32).astype(np.float64)) if fp32 else (lambda v: v)
+    A_op = A.astype(np.float32) if fp32 else A
+    dinv = rnd(np.asarray(dinv, np.float64))
+    Bp = np.zeros((N, Rp))
+    Bp[:, :R] = rnd(np.asarray(B, np.float64))
+    bb = (Bp * Bp).sum(0)
+    X = np.zeros((N, Rp))
+    Rr = Bp.copy()
+    Z = dinv[:, None] * Rr
+
+    def done(Rr, Z):
+        if rule == "reference":
+            return bb[0] <= 0 or math.sqrt(abs(float((Rr[:, 0] * Z[:, 0]).sum())) / bb[0]) < FLT_EPSILON
+        return bool(np.all((Rr[:, :R] ** 2).sum(0) <= tol * tol * bb[:R]))
+
+    if done(Rr, Z):
+        return X[:, :R], 0
+    T, k = _pivoted_cholesky_orth(Z, drop)
+    P = rnd(Z @ rnd(T))
+    for it in range(1, iters + 1):
+        Q = rnd(np.asarray(A_op @ P.astype(np.float32 if fp32 else np.float64), np.float64))
+        cP = P.sum(0)
+        PQ = P.T @ Q
+        PQ = 0.5 * (PQ + PQ.T)
+        if dc_free:
+            PQ += np.outer(cP, cP)
+        PQ[k:, :] = 0.0
+        PQ[:, k:] = 0.0
+        PQ[np.arange(k, Rp), np.arange(k, Rp)] = 1.0
+        Lc = np.linalg.cholesky(PQ)
+        solve = lambda M: sla.solve_triangular(Lc.T, sla.solve_triangular(Lc, M, lower=True), lower=False)
+        alpha = solve(P.T @ Rr)
+        X += P @ alpha
+        Rr = Rr - Q @ alpha - (np.outer(np.ones(N), cP @ alpha) if dc_free else 0.0)
+        Z = dinv[:, None] * Rr
+        if done(Rr, Z):
+            return X[:, :R], it
+        QZ = Q.T @ Z
+        if dc_free:
+            QZ += np.outer(cP, Z.sum(0))
+        beta = -solve(QZ)
+        W = Z + P @ beta
+        T, k = _pivoted_cholesky_orth(W, drop)
+        if k == 0:
+            raise np.linalg.LinAlgError("block PCG: no independent direction left")
+        P = rnd(W @ rnd(T))
+    return X[:, :R], iters
+
+
 def rcb_order(xy, leaf=256, agg=64):
     """Node order of recursive coordinate bisection: a set is split at the median of its
     longer coordinate extent (ties by node id), the first part holding a whole number of

@@ -1,0 +1,136 @@
+"""mpf_block_pcg_solve (block PCG over the columns of one matrix handed over in the
+reference's SparseMatrix layout) against the FP64 direct solve, the oracle's block PCG
+restatement run in the device's precision split (step counts) and the single-column
+solver mpf_pcg_solve on the same system (step counts)."""
+import numpy as np
+import pytest
+
+import oracle.eit_oracle as eo
+from mpflow_amd.phantom import inclusion_phantom, frame_seed
+from tests.problems import make_problem, oracle_of
+from tests.test_gpu_numeric import _rel, _solve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sysm():
+    p = make_problem(16, 16, 0.1, C=2, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(1, 8))
+    return o, g
+
+
+def _close_steps(dev, ref):
+    """the device's step count within rounding of the restatement's"""
+    return abs(dev - ref) <= max(3, 0.15 * ref)
+
+
+@pytest.mark.parametrize("c,dc", [(1, False), (0, True)])
+def test_block_pcg_solution_and_steps(gpu_ctx, sysm, c, dc):
+    from mpflow_amd import capi
+    o, g = sysm
+    A = o.system(g, c)
+    B = o.B @ o.Pi
+    Xd = eo.solve_direct(A, B, dc)
+    print("case: mpf_block_pcg_solve harmonic %d dcFree %d" % (c, dc), flush=True)
+    Xb, st = _solve(gpu_ctx, "mpf_block_pcg_solve", A, B, dc, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    Xs, ss = _solve(gpu_ctx, "mpf_pcg_solve", A, B, dc, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    _, so = eo.block_pcg(A, B, None, dc, tol=1e-6, fp32=True)
+    err = _rel(Xb, Xd)
+    print("block steps", st.steps, "oracle", so, "single-column steps", ss.steps, "rel err", err,
+          "not converged", st.not_converged)
+    assert st.not_converged == 0 and err < 1e-5
+    assert _close_steps(st.steps, so)
+    assert st.steps <= ss.steps
+
+
+def test_block_pcg_reference_rule(gpu_ctx, sysm):
+    """The reference stop rule (column 0, FLT_EPSILON) on the singular harmonic."""
+    from mpflow_amd import capi
+    o, g = sysm
+    A = o.system(g, 0)
+    B = o.B @ o.Pi
+    print("case: mpf_block_pcg_solve reference rule", flush=True)
+    Xb, st = _solve(gpu_ctx, "mpf_block_pcg_solve", A, B, True, stop_rule=capi.MPF_STOP_REFERENCE)
+    _, so = eo.block_pcg(A, B, None, True, rule="reference", fp32=True)
+    _, sc = eo.pcg_reference(A, B, 1.0 / A.diagonal(), True, fp32=True)
+    print("block steps", st.steps, "oracle", so, "single-column restatement", sc)
+    assert _close_steps(st.steps, so) and st.steps < sc
+    assert _rel(Xb, eo.solve_direct(A, B, True)) < 1e-5
+
+
+def test_block_pcg_mixed_precision(gpu_ctx, sysm):
+    """FP64 refinement: each correction solve is a block solve of the FP64 residual."""
+    from mpflow_amd import capi
+    o, g = sysm
+    A = o.system(g, 0)
+    B = o.B @ o.Pi
+    print("case: mpf_block_pcg_solve mixed precision", flush=True)
+    Xb, st = _solve(gpu_ctx, "mpf_block_pcg_solve", A, B, True, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6,
+                    precision=1, refine_steps=2, refine_tol=1e-3)
+    err = _rel(Xb, eo.solve_direct(A, B, True))
+    print("rel err", err, "steps", st.steps)
+    assert st.not_converged == 0 and err < 3e-7
+
+
+@pytest.mark.parametrize("cols", [12, 64])
+def test_block_pcg_lane_counts(gpu_ctx, cols):
+    """16 and 64 lanes: 12 columns of a 16-electrode pattern, the 64 columns of a
+    32-electrode one (drive and measurement patterns, rank-deficient)."""
+    from mpflow_amd import capi
+    p = make_problem(12, 16 if cols < 32 else 32, 0.05, C=2, height=0.4, port_height=0.1)
+    o = oracle_of(p)
+    g = inclusion_phantom(p["nodes"], p["elements"], frame_seed(2, 3))
+    A = o.system(g, 1)
+    B = (o.B @ o.Pi)[:, :cols]
+    assert B.shape[1] == cols
+    print("case: mpf_block_pcg_solve %d columns" % cols, flush=True)
+    Xb, st = _solve(gpu_ctx, "mpf_block_pcg_solve", A, B, False, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    _, so = eo.block_pcg(A, B, None, False, tol=1e-6, fp32=True)
+    err = _rel(Xb, eo.solve_direct(A, B, False))
+    print("steps", st.steps, "oracle", so, "rel err", err)
+    assert st.not_converged == 0 and err < 1e-5 and _close_steps(st.steps, so)
+
+
+def test_block_pcg_start_vector_and_zero_rhs(gpu_ctx, sysm):
+    """The caller's X0 is the start (the correction B - A X0 is solved); B = A X0 returns
+    X0 without a step."""
+    import ctypes as C
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.capi import lib
+    from mpflow_amd.eit import SolveOptions
+    from tests.test_gpu_numeric import _sparse_matrix_layout
+    o, g = sysm
+    A = o.system(g, 1)
+    B = o.B @ o.Pi
+    n, R = B.shape
+    X0 = np.random.default_rng(4).standard_normal((n, R))
+    cid, val, density = _sparse_matrix_layout(A)
+    ld = (n + 15) // 16 * 16
+    dc = torch.tensor(cid.view(np.int32), device="cuda")
+    dv = torch.tensor(val, device="cuda")
+    for rhs, name in ((B, "B"), (None, "A X0")):
+        X0f = X0.astype(np.float32)
+        Bh = rhs if rhs is not None else (A.astype(np.float32) @ X0f).astype(np.float64)
+        Bd = torch.zeros(R, ld, dtype=torch.float32, device="cuda")
+        Bd[:, :n] = torch.tensor(Bh.T, dtype=torch.float32)
+        Xd = torch.zeros(R, ld, dtype=torch.float32, device="cuda")
+        Xd[:, :n] = torch.tensor(X0f.T)
+        st = capi.SolveStats()
+        o_ = SolveOptions(stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6, precision=1, refine_steps=2,
+                          refine_tol=1e-3)
+        print("case: block PCG start vector, rhs", name, flush=True)
+        rc = lib().mpf_block_pcg_solve(gpu_ctx.h, n, density, C.c_void_p(dc.data_ptr()), C.c_void_p(dv.data_ptr()),
+                                       R, C.c_void_p(Bd.data_ptr()), ld, C.c_void_p(Xd.data_ptr()), ld, 0,
+                                       C.byref(o_.s), C.byref(st))
+        capi.check(rc)
+        X = Xd[:, :n].cpu().numpy().T.astype(np.float64)
+        want = eo.solve_direct(A, B, False) if rhs is not None else X0f.astype(np.float64)
+        err = _rel(X, want)
+        print("rhs", name, "steps", st.steps, "rel err", err)
+        assert err < 1e-5
+        if rhs is None:
+            # the FP64 residual of X0 is the rounding of A X0 to FP32: tiny, refined in few steps
+            assert st.not_converged == 0

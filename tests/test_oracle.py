@@ -355,6 +355,69 @@ def test_pcg_zero_rhs_stops_after_one_step(small):
     assert steps == 1 and np.all(x == 0.0)
 
 
+def _grid_laplacian(m, shift=0.0):
+    """5-point Laplacian on an m x m grid with Neumann edges (A 1 = 0 when shift = 0)."""
+    n = m * m
+    rows, cols, vals = [], [], []
+    for i in range(m):
+        for j in range(m):
+            k = i * m + j
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                a, b = i + di, j + dj
+                if 0 <= a < m and 0 <= b < m:
+                    rows += [k, k]
+                    cols += [k, a * m + b]
+                    vals += [1.0, -1.0]
+    return (sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) + shift * sp.eye(n)).tocsr()
+
+
+def test_block_pcg_exact_when_the_block_spans_the_space():
+    """With as many independent columns as unknowns, the first block of directions spans
+    the whole space: one block step solves every column exactly."""
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((24, 24))
+    A = sp.csr_matrix(M @ M.T + 24 * np.eye(24))
+    B = rng.standard_normal((24, 32))
+    X, steps = eo.block_pcg(A, B, tol=1e-12)
+    assert steps == 1
+    np.testing.assert_allclose(X, np.linalg.solve(A.toarray(), B), rtol=0, atol=1e-10)
+
+
+def test_block_pcg_no_more_steps_than_single_columns():
+    """Each column's iterate minimises the A-norm error over the sum of every column's
+    Krylov space, which contains its own: the block method needs no more steps than
+    single-column PCG for the same stop rule (FP64)."""
+    A = _grid_laplacian(16, 1e-3)
+    B = np.random.default_rng(2).standard_normal((A.shape[0], 8))
+    Xs, ss = eo.pcg_reference(A, B, 1.0 / A.diagonal(), False, rule="tolerance", tol=1e-8)
+    Xb, sb = eo.block_pcg(A, B, tol=1e-8)
+    assert sb <= ss and sb < 0.8 * ss
+    Xd = spla.spsolve(A.tocsc(), B)
+    assert np.max(np.abs(Xb - Xd)) / np.max(np.abs(Xd)) < 1e-6
+
+
+def test_block_pcg_rank_deficient_and_dcfree(small):
+    """Drive patterns that sum to zero give a rank-deficient block (the dependent
+    directions are dropped); the dcFree operator A + 1 1^T of the singular harmonic;
+    a zero right-hand side stops before the first step."""
+    p, o, g = small
+    B = o.B @ o.Pi
+    assert np.linalg.matrix_rank(B) < B.shape[1]
+    T, k = eo._pivoted_cholesky_orth(B, 1e-10)
+    assert k == np.linalg.matrix_rank(B)
+    Q = B @ T
+    np.testing.assert_allclose(Q[:, :k].T @ Q[:, :k], np.eye(k), atol=1e-8)
+    for c, dc in ((1, False), (0, True)):
+        A = o.system(g, c)
+        Xb, sb = eo.block_pcg(A, B, None, dc, tol=1e-10)
+        Xd = eo.solve_direct(A, B, dc)
+        assert np.max(np.abs(Xb - Xd)) / np.max(np.abs(Xd)) < 1e-8
+        _, ss = eo.pcg_reference(A, B, 1.0 / A.diagonal(), dc, rule="tolerance", tol=1e-10)
+        assert sb <= ss
+    X0, s0 = eo.block_pcg(o.system(g, 1), np.zeros_like(B))
+    assert s0 == 0 and np.all(X0 == 0.0)
+
+
 def _disc_voltages(L, width, current, nmax=4000):
     """Gap-model voltages of the unit disc (sigma = 1): Fourier series of the Neumann
     problem with current density +-I/w on the driving electrodes, potentials averaged
