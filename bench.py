@@ -421,6 +421,93 @@ def run_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def _port_excitation(nodes, bd, ports, pat, current):
+    """B Pi on the host: B[node, port] = integral over the port's edges of the node's 1D
+    Lagrange basis (exact: trapezoid for 2-node and Simpson for 3-node edges) divided by
+    the port's length, Pi = [drive pattern x current, measurement pattern]; the
+    right-hand sides of one harmonic's solve up to its factor beta_c."""
+    import numpy as np
+    N = nodes.shape[0]
+    L = ports.shape[0]
+    Bp = np.zeros((N, L))
+    for q in range(L):
+        edges = [bd[ei] for ei in ports[q] if ei >= 0]
+        length = sum(float(np.linalg.norm(nodes[e[-1]] - nodes[e[0]])) for e in edges)
+        for e in edges:
+            s = np.linalg.norm(nodes[e] - nodes[e[0]], axis=1)
+            V = np.vander(s, len(e), increasing=True)
+            a, b = s.min(), s.max()
+            mom = np.array([(b ** (p + 1) - a ** (p + 1)) / (p + 1) for p in range(len(e))])
+            Bp[e, q] += np.linalg.solve(V.T, mom) / length
+    return Bp @ np.hstack([pat * current, pat])
+
+
+def run_block_cg(args):
+    """Block PCG study (not the headline metric): mpf_block_pcg_solve against
+    mpf_pcg_solve on one frame's harmonic-0 (singular, dcFree) and harmonic-1 systems of
+    the workload's mesh (gap model, Jacobi preconditioner, the matrices from
+    mpf_fem_assemble in the reference SparseMatrix layout), every drive and measurement
+    pattern as a right-hand side (64 columns for 32 electrodes).  Prints one JSON line."""
+    import ctypes as C
+    import json
+    import math as m_
+    import time
+    import numpy as np
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.capi import lib
+    from mpflow_amd.eit import EIT, SolveOptions
+    from mpflow_amd.phantom import ellipse_phantom, frame_seed
+    w = WORKLOADS[args.workload]
+    nodes, el, bd, ports, pat = build_problem(w)
+    eit = EIT(nodes, el, bd, ports, pat, pat, 1e-3, 1.0, w["C"], w["height"], w["port_height"])
+    g = torch.tensor(ellipse_phantom(nodes, el, frame_seed(3, 0)), dtype=torch.float32)
+    F = _port_excitation(nodes, bd, ports, pat, 1e-3)
+    n, R = F.shape
+    ld = (n + 15) // 16 * 16
+    Bd = torch.zeros(R, ld, dtype=torch.float32, device="cuda")
+    Bd[:, :n] = torch.tensor(F.T, dtype=torch.float32)
+    cases = []
+    for c in (0, 1):
+        alpha = (2.0 * m_.pi * c / w["height"]) ** 2
+        cid, val = eit.update(g, alpha)
+        density = 32
+        for rule, tol in ((capi.MPF_STOP_REFERENCE, 0.0), (capi.MPF_STOP_TOLERANCE, args.tol)):
+            row = dict(harmonic=c, dc_free=int(c == 0), rule="reference" if rule == capi.MPF_STOP_REFERENCE else
+                       "tolerance %g" % tol)
+            sols = {}
+            for name in ("mpf_pcg_solve", "mpf_block_pcg_solve"):
+                f = getattr(lib(), name)
+                best, st = None, None
+                for rep in range(3):   # the first call also loads the kernels
+                    Xd = torch.zeros(R, ld, dtype=torch.float32, device="cuda")
+                    st = capi.SolveStats()
+                    o = SolveOptions(stop_rule=rule, tol=tol if tol > 0 else 1e-6)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    rc = f(eit.ctx.h, n, density, C.c_void_p(cid.data_ptr()), C.c_void_p(val.data_ptr()), R,
+                           C.c_void_p(Bd.data_ptr()), ld, C.c_void_p(Xd.data_ptr()), ld, int(c == 0), C.byref(o.s),
+                           C.byref(st))
+                    torch.cuda.synchronize()
+                    dt = time.perf_counter() - t0
+                    capi.check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
+                    best = dt if best is None else min(best, dt)
+                key = "pcg" if name == "mpf_pcg_solve" else "block"
+                row[key + "_steps"] = int(st.steps)
+                row[key + "_ms"] = round(best * 1e3, 2)
+                row[key + "_not_converged"] = int(st.not_converged)
+                sols[key] = Xd[:, :n].double()
+                print("block-cg study: harmonic %d %s %s steps %d %.2f ms" % (c, row["rule"], name, st.steps,
+                                                                             best * 1e3), flush=True)
+            D = sols["block"] - sols["pcg"]
+            if c == 0:   # dcFree solutions agree up to a constant per column
+                D = D - D.mean(dim=1, keepdim=True)
+            row["max_rel_diff"] = float(D.abs().max() / sols["pcg"].abs().max())
+            cases.append(row)
+    print(json.dumps({"study": "block_pcg_vs_single_column_pcg", "workload": args.workload, "rows": n,
+                      "columns": R, "preconditioner": "jacobi", "precision": "fp32", "cases": cases}))
+
+
 def run_probe(args):
     """Counter probe for rocprofv3 --pmc passes: 32 two-level PCG iterations of config 3
     (the headline's electrode model) at frame 0's phantom (all 5 harmonics active, so every
@@ -491,6 +578,8 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 oracle reconstruction of one frame (about a minute on config 3)")
     ap.add_argument("--probe", action="store_true", help="short counter probe for rocprofv3 --pmc (no JSON)")
+    ap.add_argument("--block-cg-study", action="store_true",
+                    help="block PCG against single-column PCG on one frame's systems (own JSON line)")
     ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
     ap.add_argument("--contact-impedance", type=float, default=0.01,
                     help="> 0: complete electrode model with this contact impedance (0: the reference's "
@@ -517,6 +606,9 @@ def main():
 
     if args.probe:
         run_probe(args)
+        return
+    if args.block_cg_study:
+        run_block_cg(args)
         return
     if args.workload == "c4":
         run_c4(args)

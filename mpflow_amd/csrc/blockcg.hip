@@ -259,35 +259,64 @@ __device__ inline bool bcg_converged(const BcgDev& D, const double* rr, const do
     return true;
 }
 
-// X (Rp x Rp, row-major, LDS) <- L^-T L^-1 X for the lower-triangular L (LDS); one
-// thread per column
-template <int Rp>
+// sum over the four consecutive lanes of a column group (every lane gets the same value)
+__device__ inline double bcg_sum4(double s) {
+    s += __shfl_xor(s, 1);
+    return s + __shfl_xor(s, 2);
+}
+
+// sum over k = k0, k0 + 4, ... < k1 of U[k] * V[k] with four independent partial sums
+// (the LDS loads of four terms in flight), combined in a fixed order
+__device__ inline double bcg_dot4(const double* U, int us, const double* V, int vs, int k0, int k1) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = k0;
+    for (; k + 12 < k1; k += 16) {
+        s0 = fma(U[k * us], V[k * vs], s0);
+        s1 = fma(U[(k + 4) * us], V[(k + 4) * vs], s1);
+        s2 = fma(U[(k + 8) * us], V[(k + 8) * vs], s2);
+        s3 = fma(U[(k + 12) * us], V[(k + 12) * vs], s3);
+    }
+    for (; k < k1; k += 4) s0 = fma(U[k * us], V[k * vs], s0);
+    return (s0 + s1) + (s2 + s3);
+}
+
+// X (Rp x Rp, LDS, row stride LS) <- L^-T L^-1 X for the lower-triangular L (LDS, row
+// stride LS).  Four lanes per column (tid / 4) split each dot product; all four write the
+// same value, so each lane only reads entries it wrote itself.
+template <int Rp, int LS>
 __device__ inline void bcg_chol_solve(const double* L, double* X) {
-    const int c = threadIdx.x;
+    static_assert(4 * Rp <= kBcgThreads, "four lanes per column");
+    const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
     if (c < Rp) {
         for (int i = 0; i < Rp; ++i) {
-            double s = X[i * Rp + c];
-            for (int k = 0; k < i; ++k) s -= L[i * Rp + k] * X[k * Rp + c];
-            X[i * Rp + c] = s / L[i * Rp + i];
+            const double s = bcg_sum4(bcg_dot4(L + i * LS, 1, X + c, LS, part, i));
+            X[i * LS + c] = (X[i * LS + c] - s) / L[i * LS + i];
         }
         for (int i = Rp - 1; i >= 0; --i) {
-            double s = X[i * Rp + c];
-            for (int k = i + 1; k < Rp; ++k) s -= L[k * Rp + i] * X[k * Rp + c];
-            X[i * Rp + c] = s / L[i * Rp + i];
+            const double s = bcg_sum4(bcg_dot4(L + i, LS, X + c, LS, i + 1 + part, Rp));
+            X[i * LS + c] = (X[i * LS + c] - s) / L[i * LS + i];
         }
     }
     __syncthreads();
 }
 
+// The dense step of the last reduction block.  LDS matrices use the row stride Rp + 1
+// (rows start on different banks).
 template <int Rp, int MODE>
 __device__ void bcg_dense(const BcgDev& D) {
-    __shared__ double A[Rp * Rp], B[Rp * Rp], F[Rp * Rp];
+    constexpr int LS = Rp + 1;
+    __shared__ double A[Rp * LS], B[Rp * LS], F[Rp * LS];
     __shared__ double vec[kBcgNV][Rp], cp[Rp], lcol[Rp], orig[Rp];
     __shared__ int piv[Rp], used[Rp], sk, sstop;
     const int tid = threadIdx.x;
-    for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
-        A[e] = D.gsum[e];
-        if (MODE == DM_ALPHA) B[e] = D.gsum[Rp * Rp + e];
+    constexpr int NPT = Rp * Rp / kBcgThreads;   // matrix entries per thread
+    static_assert(Rp * Rp % kBcgThreads == 0, "whole matrix rows per block");
+    // global loads issued together (one latency, not NPT)
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int e = tid + r * kBcgThreads, i = e / Rp, c = e % Rp;
+        A[i * LS + c] = D.gsum[e];
+        if (MODE == DM_ALPHA) B[i * LS + c] = D.gsum[Rp * Rp + e];
     }
     for (int e = tid; e < kBcgNV * Rp; e += kBcgThreads) vec[e / Rp][e % Rp] = D.vsum[e];
     if (tid < Rp) cp[tid] = D.cP[tid];
@@ -300,56 +329,50 @@ __device__ void bcg_dense(const BcgDev& D) {
             D.cP[tid] = vec[2][tid];
         }
         __syncthreads();
-    }
-
-    if constexpr (MODE == DM_ALPHA) {
-        // A = P^T Q symmetrised (+ cP cP^T), unit diagonal on the empty lanes; Cholesky in
-        // place (the symmetrisation reads a copy in lcol-free scratch: the factor array F)
-        for (int e = tid; e < Rp * Rp; e += kBcgThreads) F[e] = A[e];
-        __syncthreads();
+        // F = P^T Q symmetrised (+ cP cP^T), unit diagonal on the empty lanes
         for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
             const int i = e / Rp, c = e % Rp;
-            double v = 0.5 * (F[e] + F[c * Rp + i]);
+            double v = 0.5 * (A[i * LS + c] + A[c * LS + i]);
             if (D.dc) v += cp[i] * cp[c];
             if (i >= k || c >= k) v = (i == c) ? 1.0 : 0.0;
-            A[e] = v;
+            F[i * LS + c] = v;
         }
         __syncthreads();
+        // Cholesky F = L L^T with one barrier per step: the trailing update uses the
+        // unscaled column (F[i][c] -= F[i][m] F[c][m] / F[m][m]); the columns are scaled at
+        // the end
+        bool ok = true;
         for (int m = 0; m < Rp; ++m) {
-            if (tid == 0) {
-                const double dmm = A[m * Rp + m];
-                sstop = !(dmm > 0.0);
-                A[m * Rp + m] = sstop ? 1.0 : sqrt(dmm);
-            }
-            __syncthreads();
-            if (sstop) break;
-            const double lmm = A[m * Rp + m];
-            for (int i = m + 1 + tid; i < Rp; i += kBcgThreads) A[i * Rp + m] /= lmm;
-            __syncthreads();
-            for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
-                const int i = e / Rp, c = e % Rp;
-                if (i > m && c > m && c <= i) A[e] -= A[i * Rp + m] * A[c * Rp + m];
+            const double dmm = F[m * LS + m];
+            if (!(dmm > 0.0)) { ok = false; break; }   // uniform: every thread reads the same value
+            const double inv = 1.0 / dmm;
+            const int nt = Rp - 1 - m;
+#pragma unroll 4
+            for (int e = tid; e < nt * nt; e += kBcgThreads) {
+                const int i = m + 1 + e / nt, c = m + 1 + e % nt;
+                if (c <= i) F[i * LS + c] -= F[i * LS + m] * F[c * LS + m] * inv;
             }
             __syncthreads();
         }
-        if (sstop) {   // P^T A' P not positive definite: breakdown
+        if (!ok) {   // P^T A' P not positive definite: breakdown
             if (tid == 0) { D.flags[BF_BROKE] = 1; D.flags[BF_DONE] = 1; }
             return;
         }
         for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
             const int i = e / Rp, c = e % Rp;
-            if (c > i) A[e] = 0.0;
+            A[i * LS + c] = c > i ? 0.0 : F[i * LS + c] / sqrt(F[c * LS + c]);
         }
         __syncthreads();
-        bcg_chol_solve<Rp>(A, B);   // alpha = (P^T A' P)^-1 P^T R
+        bcg_chol_solve<Rp, LS>(A, B);   // alpha = (P^T A' P)^-1 P^T R
         for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
-            D.L[e] = A[e];
-            D.mat[e] = B[e];
+            const int i = e / Rp, c = e % Rp;
+            D.L[e] = A[i * LS + c];
+            D.mat[e] = B[i * LS + c];
         }
         if (tid < Rp) {
             double s = 0.0;
             if (D.dc)
-                for (int i = 0; i < Rp; ++i) s += cp[i] * B[i * Rp + tid];
+                for (int i = 0; i < Rp; ++i) s += cp[i] * B[i * LS + tid];
             D.d[tid] = s;
         }
     } else if constexpr (MODE == DM_BETA) {
@@ -368,14 +391,15 @@ __device__ void bcg_dense(const BcgDev& D) {
         }
         __syncthreads();
         if (sstop) return;
-        for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
-            const int i = e / Rp, c = e % Rp;
-            B[e] = -(A[e] + (D.dc ? cp[i] * vec[2][c] : 0.0));
-            A[e] = D.L[e];
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) {
+            const int e = tid + r * kBcgThreads, i = e / Rp, c = e % Rp;
+            B[i * LS + c] = -(A[i * LS + c] + (D.dc ? cp[i] * vec[2][c] : 0.0));
+            A[i * LS + c] = D.L[e];
         }
         __syncthreads();
-        bcg_chol_solve<Rp>(A, B);
-        for (int e = tid; e < Rp * Rp; e += kBcgThreads) D.mat[e] = B[e];
+        bcg_chol_solve<Rp, LS>(A, B);
+        for (int e = tid; e < Rp * Rp; e += kBcgThreads) D.mat[e] = B[(e / Rp) * LS + e % Rp];
     } else {
         // DM_ORTH0 / DM_ORTH: A = W^T W
         if constexpr (MODE == DM_ORTH0) {
@@ -392,39 +416,46 @@ __device__ void bcg_dense(const BcgDev& D) {
         // Schur complement of the remaining columns, row m of F is factor column m indexed
         // by the original column
         if (tid < Rp) {
-            orig[tid] = A[tid * Rp + tid];
+            orig[tid] = A[tid * LS + tid];
             used[tid] = 0;
         }
         if (tid == 0) sk = 0;
         __syncthreads();
         for (int m = 0; m < Rp; ++m) {
-            if (tid == 0) {
-                int best = -1;
-                double bf = kBcgDrop;
-                for (int i = 0; i < Rp; ++i) {
-                    if (used[i] || !(orig[i] > 0.0)) continue;
-                    const double f = A[i * Rp + i] / orig[i];
-                    if (f > bf) { bf = f; best = i; }
+            // the remaining column keeping the largest fraction of its norm (first of equal
+            // ones), by a butterfly over the first wavefront
+            if (tid < 64) {
+                double f = -1.0;
+                int idx = tid;
+                if (tid < Rp && !used[tid] && orig[tid] > 0.0) f = A[tid * LS + tid] / orig[tid];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double of = __shfl_xor(f, off);
+                    const int oi = __shfl_xor(idx, off);
+                    if (of > f || (of == f && oi < idx)) { f = of; idx = oi; }
                 }
-                sstop = best < 0;
-                if (!sstop) {
-                    piv[m] = best;
-                    used[best] = 1;
-                    sk = m + 1;
+                if (tid == 0) {
+                    sstop = !(f > kBcgDrop);
+                    if (!sstop) {
+                        piv[m] = idx;
+                        used[idx] = 1;
+                        sk = m + 1;
+                    }
                 }
             }
             __syncthreads();
             if (sstop) break;
             const int p = piv[m];
-            const double lpp = sqrt(A[p * Rp + p]);
-            if (tid < Rp) lcol[tid] = used[tid] && tid != p ? 0.0 : A[tid * Rp + p] / lpp;
+            const double lpp = sqrt(A[p * LS + p]);
+            if (tid < Rp) lcol[tid] = used[tid] && tid != p ? 0.0 : A[tid * LS + p] / lpp;
             __syncthreads();
-            // Schur complement of the remaining columns
+            // Schur complement of the remaining columns; factor column m
+#pragma unroll 4
             for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
                 const int i = e / Rp, c = e % Rp;
-                if (!used[i] && !used[c]) A[e] -= lcol[i] * lcol[c];
+                if (!used[i] && !used[c]) A[i * LS + c] -= lcol[i] * lcol[c];
             }
-            if (tid < Rp) F[m * Rp + tid] = lcol[tid];
+            if (tid < Rp) F[m * LS + tid] = lcol[tid];
             __syncthreads();
         }
         const int kk = sk;
@@ -432,16 +463,16 @@ __device__ void bcg_dense(const BcgDev& D) {
         __syncthreads();
         for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
             const int a = e / Rp, m = e % Rp;
-            A[e] = (a < kk && m < kk && m <= a) ? F[m * Rp + piv[a]] : 0.0;
+            A[a * LS + m] = (a < kk && m < kk && m <= a) ? F[m * LS + piv[a]] : 0.0;
         }
         __syncthreads();
-        // T = L^-T (upper triangular) in B: L^T T = I, one thread per column
-        if (tid < kk) {
-            const int m = tid;
+        // T = L^-T (upper triangular) in B: L^T T = I, four lanes per column as in
+        // bcg_chol_solve
+        if ((tid >> 2) < kk) {
+            const int m = tid >> 2, part = tid & 3;
             for (int a = kk - 1; a >= 0; --a) {
-                double s = (a == m) ? 1.0 : 0.0;
-                for (int b = a + 1; b < kk; ++b) s -= A[b * Rp + a] * B[b * Rp + m];
-                B[a * Rp + m] = s / A[a * Rp + a];
+                const double s = bcg_sum4(bcg_dot4(A + a, LS, B + m, LS, a + 1 + part, kk));
+                B[a * LS + m] = ((a == m ? 1.0 : 0.0) - s) / A[a * LS + a];
             }
         }
         __syncthreads();
@@ -450,7 +481,7 @@ __device__ void bcg_dense(const BcgDev& D) {
         __syncthreads();
         for (int e = tid; e < Rp * Rp; e += kBcgThreads) {
             const int a = e / Rp, m = e % Rp;
-            if (a < kk && m < kk) D.mat[piv[a] * Rp + m] = B[a * Rp + m];
+            if (a < kk && m < kk) D.mat[piv[a] * Rp + m] = B[a * LS + m];
         }
         if (tid == 0) {
             D.flags[BF_K] = kk;
@@ -469,15 +500,22 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_reduce(BcgDev D) {
     if (D.flags[BF_DONE]) return;
     const int e = blockIdx.x * kBcgThreads + threadIdx.x;
     if (e < NE) {
+        // the partials in block order, 8 loads in flight
+        const bool gram = e < NG * Rp * Rp;
+        const double* src = gram ? D.gpart + e : D.vpart + (e - NG * Rp * Rp);
+        const size_t stride = gram ? (size_t)kBcgNG * Rp * Rp : (size_t)kBcgNV * Rp;
         double s = 0.0;
-        if (e < NG * Rp * Rp) {
-            for (int b = 0; b < D.nparts; ++b) s += D.gpart[(size_t)b * kBcgNG * Rp * Rp + e];
-            D.gsum[e] = s;
-        } else {
-            const int v = e - NG * Rp * Rp;
-            for (int b = 0; b < D.nparts; ++b) s += D.vpart[(size_t)b * kBcgNV * Rp + v];
-            D.vsum[v] = s;
+        int b = 0;
+        for (; b + 8 <= D.nparts; b += 8) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = src[(size_t)(b + u) * stride];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += t[u];
         }
+        for (; b < D.nparts; ++b) s += src[(size_t)b * stride];
+        if (gram) D.gsum[e] = s;
+        else D.vsum[e - NG * Rp * Rp] = s;
     }
     __threadfence();
     __syncthreads();

