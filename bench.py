@@ -442,6 +442,47 @@ def _port_excitation(nodes, bd, ports, pat, current):
     return Bp @ np.hstack([pat * current, pat])
 
 
+def run_stream_probe(args):
+    """HBM streaming reference (not the headline metric): mpf_ctx_stream_probe over 1 GiB
+    buffers, copy (y = x) and the update kernels' two-reads-one-write pattern, 1/2/4 float4
+    in flight per thread, 2048-65536 blocks; torch's copy_ beside it.  Prints one JSON line
+    with every rate and the best of each pattern."""
+    import ctypes as C
+    import json
+    import torch
+    from mpflow_amd import capi
+    from mpflow_amd.capi import lib
+    from mpflow_amd.eit import Context
+    ctx = Context()
+    n = (1 << 30) // 4
+    x = torch.rand(n, device="cuda")
+    y = torch.zeros(n, device="cuda")
+    rates = []
+    for pattern in (0, 1):
+        for loads in (1, 2, 4):
+            for blocks in (2048, 8192, 32768, 65536):
+                g = C.c_double()
+                capi.check(lib().mpf_ctx_stream_probe(ctx.h, pattern, loads, C.c_void_p(x.data_ptr()),
+                                                      C.c_void_p(y.data_ptr()), n, blocks, 5, C.byref(g)))
+                rates.append(dict(pattern="copy" if pattern == 0 else "two reads one write", loads=loads,
+                                  blocks=blocks, gbps=round(g.value, 1)))
+                print("stream probe:", rates[-1], flush=True)
+    best = {}
+    for r in rates:
+        best[r["pattern"]] = max(best.get(r["pattern"], 0.0), r["gbps"])
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tc = []
+    for _ in range(6):
+        ev0.record()
+        y.copy_(x)
+        ev1.record()
+        torch.cuda.synchronize()
+        tc.append(ev0.elapsed_time(ev1))
+    torch_copy = 2.0 * n * 4 / (min(tc[1:]) * 1e-3) / 1e9
+    print(json.dumps({"study": "hbm_streaming_ceiling", "bytes_per_buffer": n * 4, "best_gbps": best,
+                      "torch_copy_gbps": round(torch_copy, 1), "peak_gbps": 8000.0, "rates": rates}))
+
+
 def run_block_cg(args):
     """Block PCG study (not the headline metric): mpf_block_pcg_solve against
     mpf_pcg_solve on one frame's harmonic-0 (singular, dcFree) and harmonic-1 systems of
@@ -578,6 +619,8 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 oracle reconstruction of one frame (about a minute on config 3)")
     ap.add_argument("--probe", action="store_true", help="short counter probe for rocprofv3 --pmc (no JSON)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="HBM streaming reference rates of the library's probe kernel (own JSON line)")
     ap.add_argument("--block-cg-study", action="store_true",
                     help="block PCG against single-column PCG on one frame's systems (own JSON line)")
     ap.add_argument("--cold", action="store_true", help="every GN step's forward solve starts from 0")
@@ -609,6 +652,9 @@ def main():
         return
     if args.block_cg_study:
         run_block_cg(args)
+        return
+    if args.stream_probe:
+        run_stream_probe(args)
         return
     if args.workload == "c4":
         run_c4(args)

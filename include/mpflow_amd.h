@@ -90,6 +90,15 @@ void* mpf_ctx_stream(mpf_ctx* ctx);
  * (8 nnz + 8 N R per harmonic per launch). */
 int mpf_ctx_profile(mpf_ctx* ctx, int32_t enable);
 int mpf_ctx_profile_read(mpf_ctx* ctx, double out[3]);
+
+/* HBM streaming reference for the rooflines (no counterpart in the reference): the best
+ * rate in GB/s over `reps` timed launches (after one warm-up) of a grid-stride 128-bit
+ * kernel of `blocks` 256-thread blocks with `loads` (1, 2, 4) float4 in flight per thread.
+ * pattern 0: y = x (8 bytes moved per float); 1: y = x + 0.5 y (12 bytes per float, the
+ * Krylov update kernels' two-reads-one-write pattern).  d_x, d_y: n floats (n a multiple
+ * of 4, 16-byte aligned), on the context's device.  Synchronises. */
+int mpf_ctx_stream_probe(mpf_ctx* ctx, int32_t pattern, int32_t loads, const float* d_x, float* d_y, int64_t n,
+                         int32_t blocks, int32_t reps, double* gbps);
 /* The context's autotuner choices so far (no counterpart: the reference's launch
  * geometry is fixed).  which = 0: SpMV, keys (lane-count log2 << 40 | padded ELL width
  * << 32 | row-count bucket) and values (kernel variant | work-list order << 4: variant 0

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpflow_amd.so")
-SOURCES = ["setup.cpp", "sparse.hip", "bicgstab.hip", "dense.hip", "krylov.cpp", "model.cpp", "mlp.hip", "numeric.hip", "blockcg.hip", "fem.hip", "noalloc.hip", "cem.hip", "complex.hip", "complex.cpp", "capi.cpp"]
+SOURCES = ["setup.cpp", "sparse.hip", "bicgstab.hip", "dense.hip", "krylov.cpp", "model.cpp", "mlp.hip", "numeric.hip", "blockcg.hip", "stream.hip", "fem.hip", "noalloc.hip", "cem.hip", "complex.hip", "complex.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPF_OFFLOAD_ARCH", "gfx950")
 

@@ -19,7 +19,7 @@ EXPORTED = [
     "mpf_version", "mpf_last_error", "mpf_ctx_create", "mpf_ctx_destroy", "mpf_ctx_synchronize",
     "mpf_ctx_stream", "mpf_solve_opts_default", "mpf_ports_circular", "mpf_model_create",
     "mpf_model_destroy", "mpf_model_dims", "mpf_eit_solve", "mpf_gn_step", "mpf_ell_spmv",
-    "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read", "mpf_ctx_autotune_read",
+    "mpf_gram_bf16", "mpf_gemv_t", "mpf_spd_solve", "mpf_ctx_profile", "mpf_ctx_profile_read", "mpf_ctx_autotune_read", "mpf_ctx_stream_probe",
     "mpf_element_integrals", "mpf_batch_create", "mpf_batch_destroy", "mpf_batch_solve",
     "mpf_batch_gn_step", "mpf_mlp_create", "mpf_mlp_destroy", "mpf_mlp_fwd", "mpf_mlp_bwd",
     "mpf_mlp_adam", "mpf_mlp_param_count", "mpf_mlp_params", "mpf_pcg_solve", "mpf_bicgstab_solve", "mpf_block_pcg_solve",
@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
     L.mpf_spd_solve.argtypes = [vp, i32, vp, dbl, vp, vp]
     L.mpf_ctx_profile.argtypes = [vp, i32]
     L.mpf_ctx_profile_read.argtypes = [vp, C.POINTER(dbl)]
+    L.mpf_ctx_stream_probe.argtypes = [vp, i32, i32, vp, vp, C.c_int64, i32, i32, C.POINTER(dbl)]
     L.mpf_ctx_autotune_read.argtypes = [vp, i32, C.POINTER(C.c_int64), C.POINTER(i32), i32, C.POINTER(i32)]
     L.mpf_element_integrals.argtypes = [i32, vp, vp, vp]
     L.mpf_batch_create.argtypes = [vp, vp, i32, C.POINTER(vp)]

@@ -126,6 +126,17 @@ int mpf_ctx_profile_read(mpf_ctx* ctx, double out[3]) {
     });
 }
 
+int mpf_ctx_stream_probe(mpf_ctx* ctx, int32_t pattern, int32_t loads, const float* d_x, float* d_y, int64_t n,
+                         int32_t blocks, int32_t reps, double* gbps) {
+    return guarded([&] {
+        MPF_REQUIRE(ctx && d_x && d_y && gbps, "mpf_ctx_stream_probe: null argument");
+        MPF_REQUIRE(n > 0, "mpf_ctx_stream_probe: n <= 0");
+        set_device(ctx);
+        *gbps = mpf::stream_probe(pattern, loads, d_x, d_y, (size_t)n, blocks, reps, ctx->stream);
+        return MPF_OK;
+    });
+}
+
 void mpf_solve_opts_default(mpf_solve_opts* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
