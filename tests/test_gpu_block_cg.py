@@ -137,3 +137,24 @@ def test_block_pcg_start_vector_and_zero_rhs(gpu_ctx, sysm):
         if rhs is None:
             # the FP64 residual of X0 is the rounding of A X0 to FP32: tiny, refined in few steps
             assert st.not_converged == 0
+
+
+def test_block_pcg_iteration_cap_and_single_column(gpu_ctx, sysm):
+    """max_iter ends the solve with not_converged (MPF_ENOCONV) after exactly that many
+    block steps; one column is plain PCG (the same step count as mpf_pcg_solve within
+    rounding)."""
+    from mpflow_amd import capi
+    o, g = sysm
+    A = o.system(g, 1)
+    B = o.B @ o.Pi
+    print("case: block PCG iteration cap", flush=True)
+    _, st = _solve(gpu_ctx, "mpf_block_pcg_solve", A, B, False, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6,
+                   max_iter=3)
+    print("capped steps", st.steps, "not converged", st.not_converged)
+    assert st.steps == 3 and st.not_converged == 1
+    print("case: block PCG one column", flush=True)
+    b = B[:, 1:2]
+    Xb, sb = _solve(gpu_ctx, "mpf_block_pcg_solve", A, b, False, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    _, ss = _solve(gpu_ctx, "mpf_pcg_solve", A, b, False, stop_rule=capi.MPF_STOP_TOLERANCE, tol=1e-6)
+    print("one column: block steps", sb.steps, "pcg steps", ss.steps)
+    assert _rel(Xb, eo.solve_direct(A, b, False)) < 1e-5 and abs(sb.steps - ss.steps) <= 2
