@@ -519,7 +519,7 @@ def run_block_cg(args):
             sols = {}
             for name in ("mpf_pcg_solve", "mpf_block_pcg_solve"):
                 f = getattr(lib(), name)
-                best, st = None, None
+                best, st, reps = None, None, []
                 for rep in range(3):   # the first call also loads the kernels
                     Xd = torch.zeros(R, ld, dtype=torch.float32, device="cuda")
                     st = capi.SolveStats()
@@ -533,8 +533,10 @@ def run_block_cg(args):
                     dt = time.perf_counter() - t0
                     capi.check(rc, allow=(capi.MPF_OK, capi.MPF_ENOCONV))
                     best = dt if best is None else min(best, dt)
+                    reps.append(int(st.steps))
                 key = "pcg" if name == "mpf_pcg_solve" else "block"
                 row[key + "_steps"] = int(st.steps)
+                row[key + "_steps_each_call"] = reps
                 row[key + "_ms"] = round(best * 1e3, 2)
                 row[key + "_not_converged"] = int(st.not_converged)
                 sols[key] = Xd[:, :n].double()

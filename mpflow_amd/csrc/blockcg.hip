@@ -116,20 +116,14 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
     __shared__ float ta[kBcgChunk * RS], tb[kBcgChunk * RS];
     __shared__ double tc[kBcgChunk * RS];
     __shared__ double red[kBcgNV][RG][Rp];
-    // alpha / beta in FP64 (LDS): block CG's orthogonality relations need the exact step;
-    // T in FP32 registers (any combination of the W columns is a valid direction block)
-    constexpr bool MS = MODE == RM_UPD1 || MODE == RM_UPD2;
-    __shared__ double ms[MS ? Rp * Rp : 1];
+    // the row products (P alpha and Q alpha, P beta, W T) for all of a thread's rows of a
+    // chunk, the matrix index outermost: each entry of column j is read once per chunk.
+    // alpha and beta in FP64 (block CG's orthogonality relations need the exact step), T
+    // rounded to FP32 (any combination of the W columns is a valid direction block)
+    constexpr bool PROD = MODE == RM_UPD1 || MODE == RM_UPD2 || MODE == RM_UPD3;
     const int j = threadIdx.x % Rp, rg = threadIdx.x / Rp;
     const int gi = threadIdx.x / 16, gj = threadIdx.x % 16;
-    float mc[MODE == RM_UPD3 ? Rp : 1];
-    if constexpr (MODE == RM_UPD3) {
-#pragma unroll
-        for (int i = 0; i < Rp; ++i) mc[i] = (float)D.mat[i * Rp + j];
-    }
-    if constexpr (MS) {
-        for (int e = threadIdx.x; e < Rp * Rp; e += kBcgThreads) ms[e] = D.mat[e];
-    }
+    double acc1[PROD ? RPT : 1], acc2[MODE == RM_UPD1 ? RPT : 1];
     const double dj = (MODE == RM_UPD1) ? D.d[j] : 0.0;
     double g0[BI][BI], g1[BI][BI];
 #pragma unroll
@@ -155,6 +149,28 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
             bcg_stage<Rp>(tc, D.z, row0);
         }
         __syncthreads();
+        if constexpr (PROD) {
+#pragma unroll
+            for (int t = 0; t < RPT; ++t) {
+                acc1[t] = 0.0;
+                if constexpr (MODE == RM_UPD1) acc2[t] = 0.0;
+            }
+#pragma unroll 4
+            for (int i = 0; i < Rp; ++i) {
+                const double m = MODE == RM_UPD3 ? (double)(float)D.mat[i * Rp + j] : D.mat[i * Rp + j];
+#pragma unroll
+                for (int t = 0; t < RPT; ++t) {
+                    const int lr = rg + RG * t;
+                    if constexpr (MODE == RM_UPD3) {
+                        acc1[t] = fma(tc[lr * RS + i], m, acc1[t]);
+                    } else {
+                        acc1[t] = fma((double)ta[lr * RS + i], m, acc1[t]);
+                        if constexpr (MODE == RM_UPD1) acc2[t] = fma((double)tb[lr * RS + i], m, acc2[t]);
+                    }
+                }
+            }
+        }
+#pragma unroll
         for (int t = 0; t < RPT; ++t) {
             const int lr = rg + RG * t, row = row0 + lr;
             const size_t e = (size_t)row * Rp + j;
@@ -169,13 +185,7 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
             } else if constexpr (MODE == RM_GRAM) {
                 v2 += (double)ta[lr * RS + j];
             } else if constexpr (MODE == RM_UPD1) {
-                double px = 0.0, qa = 0.0;
-#pragma unroll
-                for (int i = 0; i < Rp; ++i) {
-                    const double m = ms[i * Rp + j];
-                    px = fma((double)ta[lr * RS + i], m, px);
-                    qa = fma((double)tb[lr * RS + i], m, qa);
-                }
+                const double px = acc1[t], qa = acc2[t];
                 const double r = D.r[e] - qa - (row < D.n ? dj : 0.0);
                 const double z = (double)D.dinv[row] * r;
                 D.x[e] += px;
@@ -184,18 +194,12 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
                 tc[lr * RS + j] = z;
                 v0 += r * r; v1 += r * z; v2 += z;
             } else if constexpr (MODE == RM_UPD2) {
-                double pb = 0.0;
-#pragma unroll
-                for (int i = 0; i < Rp; ++i) pb = fma((double)ta[lr * RS + i], ms[i * Rp + j], pb);
-                const double w = D.z[e] + pb;
+                const double w = D.z[e] + acc1[t];
                 D.z[e] = w;
                 tc[lr * RS + j] = w;
                 v2 += w;
             } else if constexpr (MODE == RM_UPD3) {
-                double wt = 0.0;
-#pragma unroll
-                for (int i = 0; i < Rp; ++i) wt = fma(tc[lr * RS + i], (double)mc[i], wt);
-                D.p[e] = (float)wt;
+                D.p[e] = (float)acc1[t];
             }
         }
         if constexpr (GRAM) {
