@@ -116,14 +116,20 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
     __shared__ float ta[kBcgChunk * RS], tb[kBcgChunk * RS];
     __shared__ double tc[kBcgChunk * RS];
     __shared__ double red[kBcgNV][RG][Rp];
-    // the row products (P alpha and Q alpha, P beta, W T) for all of a thread's rows of a
-    // chunk, the matrix index outermost: each entry of column j is read once per chunk.
-    // alpha and beta in FP64 (block CG's orthogonality relations need the exact step), T
-    // rounded to FP32 (any combination of the W columns is a valid direction block)
-    constexpr bool PROD = MODE == RM_UPD1 || MODE == RM_UPD2 || MODE == RM_UPD3;
+    // The row products P beta and W T for all of a thread's rows of a chunk, the matrix
+    // index outermost (each entry of column j read once per chunk); P alpha and Q alpha
+    // row by row from alpha in LDS (the x and r loads of a row then overlap the next row's
+    // products).  alpha and beta in FP64 (block CG's orthogonality relations need the
+    // exact step), T rounded to FP32 (any combination of the W columns is a valid
+    // direction block).
+    constexpr bool PROD = MODE == RM_UPD2 || MODE == RM_UPD3;
+    __shared__ double ms[MODE == RM_UPD1 ? Rp * Rp : 1];
     const int j = threadIdx.x % Rp, rg = threadIdx.x / Rp;
     const int gi = threadIdx.x / 16, gj = threadIdx.x % 16;
-    double acc1[PROD ? RPT : 1], acc2[MODE == RM_UPD1 ? RPT : 1];
+    double acc1[PROD ? RPT : 1];
+    if constexpr (MODE == RM_UPD1) {
+        for (int e = threadIdx.x; e < Rp * Rp; e += kBcgThreads) ms[e] = D.mat[e];
+    }
     const double dj = (MODE == RM_UPD1) ? D.d[j] : 0.0;
     double g0[BI][BI], g1[BI][BI];
 #pragma unroll
@@ -151,26 +157,19 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
         __syncthreads();
         if constexpr (PROD) {
 #pragma unroll
-            for (int t = 0; t < RPT; ++t) {
-                acc1[t] = 0.0;
-                if constexpr (MODE == RM_UPD1) acc2[t] = 0.0;
-            }
+            for (int t = 0; t < RPT; ++t) acc1[t] = 0.0;
 #pragma unroll 4
             for (int i = 0; i < Rp; ++i) {
                 const double m = MODE == RM_UPD3 ? (double)(float)D.mat[i * Rp + j] : D.mat[i * Rp + j];
 #pragma unroll
                 for (int t = 0; t < RPT; ++t) {
                     const int lr = rg + RG * t;
-                    if constexpr (MODE == RM_UPD3) {
-                        acc1[t] = fma(tc[lr * RS + i], m, acc1[t]);
-                    } else {
-                        acc1[t] = fma((double)ta[lr * RS + i], m, acc1[t]);
-                        if constexpr (MODE == RM_UPD1) acc2[t] = fma((double)tb[lr * RS + i], m, acc2[t]);
-                    }
+                    if constexpr (MODE == RM_UPD3) acc1[t] = fma(tc[lr * RS + i], m, acc1[t]);
+                    else acc1[t] = fma((double)ta[lr * RS + i], m, acc1[t]);
                 }
             }
         }
-#pragma unroll
+#pragma unroll(MODE == RM_UPD1 ? 2 : RPT)
         for (int t = 0; t < RPT; ++t) {
             const int lr = rg + RG * t, row = row0 + lr;
             const size_t e = (size_t)row * Rp + j;
@@ -185,7 +184,13 @@ __global__ void __launch_bounds__(kBcgThreads) k_bcg_rows(BcgDev D) {
             } else if constexpr (MODE == RM_GRAM) {
                 v2 += (double)ta[lr * RS + j];
             } else if constexpr (MODE == RM_UPD1) {
-                const double px = acc1[t], qa = acc2[t];
+                double px = 0.0, qa = 0.0;
+#pragma unroll
+                for (int i = 0; i < Rp; ++i) {
+                    const double m = ms[i * Rp + j];
+                    px = fma((double)ta[lr * RS + i], m, px);
+                    qa = fma((double)tb[lr * RS + i], m, qa);
+                }
                 const double r = D.r[e] - qa - (row < D.n ? dj : 0.0);
                 const double z = (double)D.dinv[row] * r;
                 D.x[e] += px;
