@@ -90,6 +90,7 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
         g.nall = (int)lst.size();
         const std::vector<int4>* src_o[2] = {&lst, &lst_t};
         for (int o = 0; o < 2; ++o) {
+            g.base_o[o] = *src_o[o];
             g.all_o[o].resize(lst.size());
             xcd_order(src_o[o]->data(), g.nall, g.all_o[o].data());
             g.tiles_o[o].alloc(g.nall);
@@ -99,6 +100,7 @@ KrylovDriver::KrylovDriver(mpf_ctx* c, std::vector<std::vector<JobDev>*> sources
         g.hoff = total;
         total += g.nall;
         g.nfall = (int)flst.size();
+        g.fbase = flst;
         g.fall.resize(flst.size());
         xcd_order(flst.data(), g.nfall, g.fall.data());
         g.ftiles.alloc(g.nfall);
@@ -269,22 +271,16 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
         // compact the work lists: always without graphs, otherwise once a quarter of the
         // listed tiles belong to finished pairs (finished pairs' blocks exit at once)
         for (auto& g : groups) {
+            // the survivors in (job, slice, tile) or (job, tile, slice) order (a filter of
+            // the unstriped list keeps its order), then striped over the XCDs again so that
+            // XCD x gets a contiguous run of them
             std::vector<int4> un;
-            for (const int4& t : g.all_o[g.order])
+            un.reserve(g.ncur);
+            for (const int4& t : g.base_o[g.order])
                 if (h_st[((size_t)t.x * kMaxSlices + t.y) * kStInts]) un.push_back(t);
             const int nt = (int)un.size();
             if (nt == g.ncur) continue;
             if (graphs && nt * 4 > g.ncur * 3 && nt > 0) continue;
-            // back to (job, slice, tile) or (job, tile, slice) order, then striped over
-            // the XCDs again so that XCD x gets a contiguous run of the survivors
-            if (g.order == 0)
-                std::sort(un.begin(), un.end(), [](const int4& a, const int4& b) {
-                    return a.x != b.x ? a.x < b.x : (a.y != b.y ? a.y < b.y : a.z < b.z);
-                });
-            else
-                std::sort(un.begin(), un.end(), [](const int4& a, const int4& b) {
-                    return a.x != b.x ? a.x < b.x : (a.z != b.z ? a.z < b.z : a.y < b.y);
-                });
             int4* ordered = h_tiles + g.hoff;
             xcd_order(un.data(), nt, ordered);
             if (nt) MPF_HIP(hipMemcpyAsync(g.act.p, ordered, sizeof(int4) * nt, hipMemcpyHostToDevice, s0));
@@ -294,15 +290,12 @@ bool KrylovDriver::run(const mpf_solve_opts& o, int rule, double tol, bool use_r
             g.ncur = nt;
             // the fused SpMV's (job, tile) items: jobs with a live harmonic
             std::vector<int4> fun;
-            for (int b = 0; b < g.nfall; ++b) {
-                const int4 t = g.fall[b];
+            fun.reserve(g.nfall);
+            for (const int4& t : g.fbase) {
                 bool live = false;
                 for (int c = 0; c < h[t.x].C; ++c) live = live || h_st[((size_t)t.x * kMaxSlices + c) * kStInts];
                 if (live) fun.push_back(t);
             }
-            std::sort(fun.begin(), fun.end(), [](const int4& a, const int4& b) {
-                return a.x != b.x ? a.x < b.x : a.z < b.z;
-            });
             const int nf = (int)fun.size();
             int4* fordered = h_tiles + g.fhoff;
             xcd_order(fun.data(), nf, fordered);

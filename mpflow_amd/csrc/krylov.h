@@ -53,7 +53,9 @@ class KrylovDriver {
         bool wide = false;            // a slice vector of >= 2^31 - 2^16 bytes: 32-bit-gather SpMV only
         long long key = 0;
         std::vector<int> jobs;        // indices into h
-        std::vector<int4> all_o[2];   // every (job, slice, tile) item in either order, XCD-ordered
+        std::vector<int4> base_o[2];  // every (job, slice, tile) item in (job, slice, tile) /
+                                      // (job, tile, slice) order
+        std::vector<int4> all_o[2];   // the same, XCD-ordered
         DBuf<int4> tiles_o[2];
         DBuf<int4> act;               // the active items
         DBuf<int2> js;                // the group's (job, slice) pairs (coarse correction)
@@ -62,7 +64,7 @@ class KrylovDriver {
         const int4* cur = nullptr;
         size_t hoff = 0;              // the group's region of the pinned staging buffer
         // (job, 0, tile) items of the harmonic-fused SpMV (variant 2)
-        std::vector<int4> fall;
+        std::vector<int4> fbase, fall;   // in (job, tile) order; XCD-ordered
         DBuf<int4> ftiles, fact;
         int nfall = 0, nfcur = 0;
         const int4* fcur = nullptr;
